@@ -1,0 +1,230 @@
+// capi.cpp -- extern "C" entry points of libgvc (declared in include/gvc.h).
+// Thin: argument checks, status mapping, then the stream-ordered launchers.
+#include <stdio.h>
+#include <string.h>
+
+#include "gvc_internal.h"
+
+namespace gvc {
+
+static thread_local char g_last_error[512] = "";
+
+int set_error(int status, const char* what)
+{
+    snprintf(g_last_error, sizeof(g_last_error), "%s", what ? what : "");
+    return status;
+}
+
+int hip_status(hipError_t e, const char* what)
+{
+    snprintf(g_last_error, sizeof(g_last_error), "%s: %s", what ? what : "hip", hipGetErrorString(e));
+    return GVC_ERR_HIP;
+}
+
+// mixture.hip
+int preprocess_launch(int P, const float* means, const float* scales, const float* rots, const float* weights,
+                      float k, unsigned flags, float* conics, float* boxes, hipStream_t st);
+int morton_launch(int n, const float* xyz, int clamp, uint64_t* codes, hipStream_t st);
+size_t sample_set_bytes(int S);
+int sample_set_build(int S, const float* samples, void* set, size_t bytes, hipStream_t st);
+size_t forward_state_bytes(int P, int S, const gvc_params* prm);
+int mixture_forward(int P, const float* means, const float* scales, const float* rots, const float* values,
+                    const float* weights, float k, const void* sample_set, int S, const gvc_params* prm,
+                    float* out_cells, float* out_weights, float* conics, void* state, size_t state_bytes,
+                    hipStream_t st);
+int forward_stats(const void* state, int P, int S, const gvc_params* prm, uint64_t* n_records, uint64_t* n_pairs,
+                  uint32_t* error_flags, hipStream_t st);
+size_t backward_bytes(int P, uint64_t cap);
+int mixture_backward(int P, const float* means, const float* scales, const float* rots, const float* values,
+                     const float* weights, float k, const void* sample_set, int S, const gvc_params* prm,
+                     const float* out_cells, const float* out_weights, const float* gcells, const float* gweights,
+                     void* state, size_t state_bytes, uint64_t capacity, void* ws, size_t ws_bytes, float* dmeans,
+                     float* dscales, float* drots, float* dvalues, float* dweights, hipStream_t st);
+int sort_pairs_u64(int n, const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, int key_bits,
+                   void* ws, size_t ws_bytes, hipStream_t st);
+size_t sort_pairs_u64_bytes(int n);
+
+// mesh.hip, knn.hip
+size_t tet_locator_bytes(int n_points, int C, unsigned long long cap);
+int tet_locator_build(int n_points, const float* pts, int C, const int* tets, unsigned long long cap, void* loc,
+                      size_t bytes, unsigned long long* needed, hipStream_t st);
+int tet_locate_interp(int n_points, const float* pts, int C, const void* loc, size_t bytes, unsigned long long cap,
+                      const double* values, int S, const float* samples, int* cell_ids, double* out_values,
+                      float* bary, hipStream_t st);
+int grid_sample(const long long dims[3], const float origin[3], const float spacing[3], int dtype, const void* values,
+                int S, const float* samples, void* out, hipStream_t st);
+size_t knn3_bytes(int n);
+int knn3(int n, const float* pts, float* out, void* ws, size_t bytes, hipStream_t st);
+
+}  // namespace gvc
+
+using namespace gvc;
+
+#define GVC_REQUIRE(cond, msg)                                        \
+    do {                                                              \
+        if (!(cond)) return set_error(GVC_ERR_INVALID_ARGUMENT, msg); \
+    } while (0)
+
+extern "C" {
+
+int gvc_version(void) { return 1; }
+
+const char* gvc_status_string(int status)
+{
+    switch (status) {
+    case GVC_OK: return "ok";
+    case GVC_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case GVC_ERR_WORKSPACE_TOO_SMALL: return "workspace too small";
+    case GVC_ERR_HIP: return "HIP runtime error";
+    case GVC_ERR_UNSUPPORTED: return "unsupported";
+    case GVC_ERR_CAPACITY: return "capacity exceeded";
+    default: return "unknown status";
+    }
+}
+
+const char* gvc_last_error(void) { return g_last_error; }
+
+int gvc_preprocess(int P, const float* means3D, const float* scales, const float* rotations, const float* weights,
+                   float scale_modifier, unsigned flags, float* conics, float* boxes, void* stream)
+{
+    GVC_REQUIRE(P >= 0, "P < 0");
+    if (P == 0) return GVC_OK;
+    GVC_REQUIRE(means3D && scales && rotations && weights, "NULL input");
+    return preprocess_launch(P, means3D, scales, rotations, weights, scale_modifier, flags, conics, boxes,
+                             (hipStream_t)stream);
+}
+
+int gvc_morton3d_unit(int n, const float* xyz, int clamp, uint64_t* codes, void* stream)
+{
+    GVC_REQUIRE(n >= 0, "n < 0");
+    if (n == 0) return GVC_OK;
+    GVC_REQUIRE(xyz && codes, "NULL pointer");
+    return morton_launch(n, xyz, clamp, codes, (hipStream_t)stream);
+}
+
+size_t gvc_sort_pairs_u64_bytes(int n) { return n > 0 ? sort_pairs_u64_bytes(n) : 256; }
+
+int gvc_sort_pairs_u64(int n, const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
+                       uint32_t* vals_out, int key_bits, void* workspace, size_t workspace_bytes, void* stream)
+{
+    GVC_REQUIRE(n >= 0, "n < 0");
+    GVC_REQUIRE(key_bits >= 0 && key_bits <= 64, "key_bits must be in [0, 64]");
+    if (n == 0) return GVC_OK;
+    GVC_REQUIRE(keys_in && vals_in && keys_out && vals_out && workspace, "NULL pointer");
+    return sort_pairs_u64(n, keys_in, vals_in, keys_out, vals_out, key_bits, workspace, workspace_bytes,
+                          (hipStream_t)stream);
+}
+
+size_t gvc_sample_set_bytes(int S) { return S >= 0 ? sample_set_bytes(S) : 0; }
+
+int gvc_sample_set_build(int S, const float* samples, void* sample_set, size_t sample_set_bytes_, void* stream)
+{
+    GVC_REQUIRE(S >= 0, "S < 0");
+    GVC_REQUIRE(sample_set, "NULL sample set");
+    GVC_REQUIRE(S == 0 || samples, "NULL samples");
+    return sample_set_build(S, samples, sample_set, sample_set_bytes_, (hipStream_t)stream);
+}
+
+size_t gvc_forward_state_bytes(int P, int S, const gvc_params* params)
+{
+    if (P < 0 || S < 0 || !params) return 0;
+    return forward_state_bytes(P, S, params);
+}
+
+int gvc_mixture_forward(int P, const float* means3D, const float* scales, const float* rotations,
+                        const float* values, const float* weights, float scale_modifier, const void* sample_set, int S,
+                        const gvc_params* params, float* out_cells, float* out_weights, float* conics, void* state,
+                        size_t state_bytes, void* stream)
+{
+    GVC_REQUIRE(P >= 0 && S >= 0, "negative size");
+    GVC_REQUIRE(sample_set && state, "NULL sample set or state");
+    GVC_REQUIRE(S == 0 || (out_cells && out_weights), "NULL outputs");
+    GVC_REQUIRE(P == 0 || (means3D && scales && rotations && values && weights), "NULL Gaussian input");
+    return mixture_forward(P, means3D, scales, rotations, values, weights, scale_modifier, sample_set, S, params,
+                           out_cells, out_weights, conics, state, state_bytes, (hipStream_t)stream);
+}
+
+int gvc_forward_stats(const void* state, int P, int S, const gvc_params* params, uint64_t* n_records,
+                      uint64_t* n_pairs, uint32_t* error_flags, void* stream)
+{
+    GVC_REQUIRE(state, "NULL state");
+    return forward_stats(state, P, S, params, n_records, n_pairs, error_flags, (hipStream_t)stream);
+}
+
+size_t gvc_backward_bytes(int P, uint64_t record_capacity)
+{
+    return P >= 0 ? backward_bytes(P, record_capacity) : 0;
+}
+
+int gvc_mixture_backward(int P, const float* means3D, const float* scales, const float* rotations,
+                         const float* values, const float* weights, float scale_modifier, const void* sample_set, int S,
+                         const gvc_params* params, const float* out_cells, const float* out_weights,
+                         const float* dL_dout_cells, const float* dL_dout_cell_weights, void* state, size_t state_bytes,
+                         uint64_t record_capacity, void* workspace, size_t workspace_bytes, float* dL_dmeans3D,
+                         float* dL_dscales, float* dL_drotations, float* dL_dvalues, float* dL_dweights, void* stream)
+{
+    GVC_REQUIRE(P >= 0 && S >= 0, "negative size");
+    if (P == 0) return GVC_OK;
+    GVC_REQUIRE(sample_set && state && workspace, "NULL sample set, state or workspace");
+    GVC_REQUIRE(scales && rotations, "NULL scales or rotations");
+    GVC_REQUIRE(S == 0 || (out_cells && out_weights && dL_dout_cells && dL_dout_cell_weights), "NULL per-sample input");
+    GVC_REQUIRE(dL_dmeans3D && dL_dscales && dL_drotations && dL_dvalues && dL_dweights, "NULL gradient output");
+    return mixture_backward(P, means3D, scales, rotations, values, weights, scale_modifier, sample_set, S, params,
+                            out_cells, out_weights, dL_dout_cells, dL_dout_cell_weights, state, state_bytes, record_capacity,
+                            workspace, workspace_bytes, dL_dmeans3D, dL_dscales, dL_drotations, dL_dvalues,
+                            dL_dweights, (hipStream_t)stream);
+}
+
+size_t gvc_tet_locator_bytes(int n_points, int n_tets, uint64_t entry_capacity)
+{
+    if (n_points < 0 || n_tets < 0) return 0;
+    return tet_locator_bytes(n_points, n_tets, entry_capacity);
+}
+
+int gvc_tet_locator_build(int n_points, const float* points, int n_tets, const int32_t* tets, uint64_t entry_capacity,
+                          void* locator, size_t locator_bytes, uint64_t* entries_needed, void* stream)
+{
+    GVC_REQUIRE(n_points > 0 && n_tets > 0, "empty mesh");
+    GVC_REQUIRE(points && tets && locator, "NULL pointer");
+    GVC_REQUIRE(entry_capacity <= 0xffffffffull, "entry capacity above 2^32");
+    unsigned long long need = 0;
+    int st = tet_locator_build(n_points, points, n_tets, tets, entry_capacity, locator, locator_bytes, &need,
+                               (hipStream_t)stream);
+    if (entries_needed) *entries_needed = need;
+    return st;
+}
+
+int gvc_tet_locate_interp(int n_points, const float* points, int n_tets, const void* locator, size_t locator_bytes,
+                          uint64_t entry_capacity, const double* point_values, int S, const float* samples,
+                          int32_t* cell_ids, double* values_out, float* bary_out, void* stream)
+{
+    GVC_REQUIRE(S >= 0, "S < 0");
+    if (S == 0) return GVC_OK;
+    GVC_REQUIRE(points && locator && samples && cell_ids, "NULL pointer");
+    GVC_REQUIRE(!values_out || point_values, "values_out needs point_values");
+    return tet_locate_interp(n_points, points, n_tets, locator, locator_bytes, entry_capacity, point_values, S,
+                             samples, cell_ids, values_out, bary_out, (hipStream_t)stream);
+}
+
+int gvc_grid_sample(const int64_t* dims, const float* origin, const float* spacing, int value_dtype,
+                    const void* values, int S, const float* samples, void* out, void* stream)
+{
+    GVC_REQUIRE(dims && origin && spacing, "NULL grid description");
+    GVC_REQUIRE(S >= 0, "S < 0");
+    if (S == 0) return GVC_OK;
+    GVC_REQUIRE(values && samples && out, "NULL pointer");
+    const long long d[3] = { (long long)dims[0], (long long)dims[1], (long long)dims[2] };
+    return grid_sample(d, origin, spacing, value_dtype, values, S, samples, out, (hipStream_t)stream);
+}
+
+size_t gvc_knn3_bytes(int n) { return n >= 0 ? knn3_bytes(n) : 0; }
+
+int gvc_knn3(int n, const float* points, float* mean_sq_dist, void* workspace, size_t workspace_bytes, void* stream)
+{
+    GVC_REQUIRE(n >= 0, "n < 0");
+    if (n == 0) return GVC_OK;
+    GVC_REQUIRE(points && mean_sq_dist && workspace, "NULL pointer");
+    return knn3(n, points, mean_sq_dist, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+}  // extern "C"

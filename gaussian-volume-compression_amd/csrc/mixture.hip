@@ -1,0 +1,1029 @@
+// mixture.hip -- the differentiable Gaussian-mixture evaluator on gfx950.
+//
+// Pipeline (one stream, no allocation, no host synchronisation):
+//   sample set   : 30-bit Morton keys of the samples -> stable radix sort -> float4
+//                  points (x, y, z, original index) in Morton order; every 64
+//                  consecutive points form one group = one wavefront.
+//   forward      : 63-bit Morton codes of the means (the reference's morton3d_unit)
+//                  -> stable radix sort -> canonical preprocess written as 80-byte
+//                  records in BVH order -> Karras radix tree over leaves of
+//                  `leaf_size` consecutive records -> bottom-up box refit with
+//                  atomic arrival counters -> one wavefront per group walks the
+//                  tree with the group's box (wave-uniform stack, scalar node and
+//                  record loads), each lane owns one point and accumulates
+//                  w exp(power) and w exp(power) v with Kahan compensation.
+//                  The walk also counts, per group, the Gaussians with at least
+//                  one accepted pair: those are the backward's records.
+//   backward     : the same walk per group; candidate Gaussians are queued in a
+//                  VGPR (one per lane) and each lane loops over the 64 points of
+//                  the group (read from LDS) accumulating its Gaussian's 11
+//                  partial sums -> one record per (Gaussian, group) at a position
+//                  fixed by the forward's counts -> stable radix sort by Gaussian
+//                  -> per-Gaussian FP64 sum in group order -> covariance chain in
+//                  FP64 -> gradients in the caller's Gaussian order.
+// Reference path: CR/forward.cu:10-274, CR/backward.cu:9-660,
+// CR/rasterizer_impl.cu:27-452, DGR/ext.cu:13-147 (CR/ = diff-gaussian-
+// rasterization/cuda_rasterizer/, DGR/ = diff-gaussian-rasterization/).
+#include <math.h>
+
+#include "gvc_device.h"
+#include "gvc_internal.h"
+
+namespace gvc {
+
+namespace {
+
+constexpr int kStackDepth = 128;
+constexpr float kLog2e = 1.4426950408889634f;
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float uniformf(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ int uniformi(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ float wave_minf(float v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_maxf(float v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_sumu(uint32_t v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// Blocks are dealt round-robin to the 8 XCDs; remap so each XCD (own L2) works on
+// one contiguous, spatially coherent range of groups.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
+{
+    const unsigned x = b & 7u, q = nb >> 3, r = nb & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// generic kernels
+// ---------------------------------------------------------------------------
+__global__ void iota_kernel(uint32_t* p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (uint32_t)i;
+}
+
+__global__ void morton63_kernel(const float* __restrict__ xyz, int n, int clamp, uint64_t* __restrict__ codes)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) codes[i] = morton3d_unit(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], clamp != 0);
+}
+
+__global__ void preprocess_kernel(int P, const float* __restrict__ means, const float* __restrict__ scales,
+                                  const float* __restrict__ rots, const float* __restrict__ weights, float k,
+                                  int fp64_cond, float* __restrict__ conics, float* __restrict__ boxes)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    float mu[3] = { means[3 * i], means[3 * i + 1], means[3 * i + 2] };
+    float sc[3] = { scales[3 * i], scales[3 * i + 1], scales[3 * i + 2] };
+    float q[4] = { rots[4 * i], rots[4 * i + 1], rots[4 * i + 2], rots[4 * i + 3] };
+    float cn[6], bx[6];
+    preprocess_one(mu, sc, q, weights[i], k, fp64_cond != 0, cn, bx);
+    if (conics)
+        for (int c = 0; c < 6; ++c) conics[6 * i + c] = cn[c];
+    if (boxes)
+        for (int c = 0; c < 6; ++c) boxes[6 * i + c] = bx[c];
+}
+
+// ---------------------------------------------------------------------------
+// sample set
+// ---------------------------------------------------------------------------
+struct SampleSetHeader {
+    int S;
+    int NG;
+    int magic;
+    int pad;
+};
+constexpr int kSampleMagic = 0x47564353;
+
+__global__ void sample_codes_kernel(const float* __restrict__ s, int S, uint32_t* __restrict__ keys,
+                                    uint32_t* __restrict__ idx)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S) return;
+    uint32_t c = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float t = s[3 * i + a];
+        t = (t == t) ? fminf(fmaxf(t, 0.0f), 1.0f) : 0.0f;
+        c |= spread10((uint32_t)(t * 1023.0f)) << a;
+    }
+    keys[i] = c;
+    idx[i] = (uint32_t)i;
+}
+
+__global__ void sample_gather_kernel(const float* __restrict__ s, int S, const uint32_t* __restrict__ perm,
+                                     float4* __restrict__ pts)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S) return;
+    uint32_t j = perm[i];
+    pts[i] = make_float4(s[3 * j], s[3 * j + 1], s[3 * j + 2], __builtin_bit_cast(float, (int)j));
+}
+
+__global__ void sample_header_kernel(SampleSetHeader* h, int S, int NG)
+{
+    h->S = S;
+    h->NG = NG;
+    h->magic = kSampleMagic;
+    h->pad = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Gaussian BVH
+// ---------------------------------------------------------------------------
+__global__ void gaussian_codes_kernel(const float* __restrict__ means, int P, uint64_t* __restrict__ codes,
+                                      uint32_t* __restrict__ idx)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    codes[i] = morton3d_unit(means[3 * i], means[3 * i + 1], means[3 * i + 2], false);
+    idx[i] = (uint32_t)i;
+}
+
+// Canonical preprocess, written as records in BVH (sorted) order.
+__global__ void record_kernel(int P, const uint32_t* __restrict__ perm, const float* __restrict__ means,
+                              const float* __restrict__ scales, const float* __restrict__ rots,
+                              const float* __restrict__ values, const float* __restrict__ weights, float k,
+                              int fp64_cond, GRecord* __restrict__ rec, float* __restrict__ conics_out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const int j = (int)perm[i];
+    float mu[3] = { means[3 * j], means[3 * j + 1], means[3 * j + 2] };
+    float sc[3] = { scales[3 * j], scales[3 * j + 1], scales[3 * j + 2] };
+    float q[4] = { rots[4 * j], rots[4 * j + 1], rots[4 * j + 2], rots[4 * j + 3] };
+    const float w = weights[j];
+    float cn[6], bx[6];
+    preprocess_one(mu, sc, q, w, k, fp64_cond != 0, cn, bx);
+    GRecord r;
+    r.mx = mu[0]; r.my = mu[1]; r.mz = mu[2]; r.w = w;
+    r.c0 = cn[0]; r.c1 = cn[1]; r.c2 = cn[2]; r.c3 = cn[3]; r.c4 = cn[4]; r.c5 = cn[5];
+    r.v = values[j];
+    r.id = j;
+    r.lox = bx[0]; r.loy = bx[1]; r.loz = bx[2];
+    r.hix = bx[3]; r.hiy = bx[4]; r.hiz = bx[5];
+    r.pad0 = 0.f; r.pad1 = 0.f;
+    rec[i] = r;
+    if (conics_out)
+        for (int c = 0; c < 6; ++c) conics_out[6 * j + c] = cn[c];
+}
+
+__global__ void leaf_codes_kernel(const uint64_t* __restrict__ sorted_codes, int NL, int L,
+                                  uint64_t* __restrict__ leaf_codes)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < NL) leaf_codes[j] = sorted_codes[(size_t)j * L];
+}
+
+__device__ __forceinline__ int karras_delta(const uint64_t* k, int n, int i, int j)
+{
+    if (j < 0 || j >= n) return -1;
+    const uint64_t a = k[i], b = k[j];
+    if (a == b) return 64 + __clz((uint32_t)i ^ (uint32_t)j);
+    return __clzll(a ^ b);
+}
+
+// Karras (2012): one thread per internal node.  parent codes: (p << 1) | side.
+__global__ void karras_kernel(const uint64_t* __restrict__ keys, int n, BNode* __restrict__ nodes,
+                              int* __restrict__ parent_int, int* __restrict__ parent_leaf)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (karras_delta(keys, n, i, i + 1) - karras_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = karras_delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (karras_delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (karras_delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = karras_delta(keys, n, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) / 2;
+        if (karras_delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * d + (d < 0 ? d : 0);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    const int left = (lo == gamma) ? ~gamma : gamma;
+    const int right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+    nodes[i].left = left;
+    nodes[i].right = right;
+    nodes[i].pad0 = lo;
+    nodes[i].pad1 = hi;
+    if (left < 0) parent_leaf[~left] = (i << 1) | 0; else parent_int[left] = (i << 1) | 0;
+    if (right < 0) parent_leaf[~right] = (i << 1) | 1; else parent_int[right] = (i << 1) | 1;
+}
+
+__device__ __forceinline__ void store_child_box(BNode* nodes, int pcode, const float lo[3], const float hi[3])
+{
+    BNode& n = nodes[pcode >> 1];
+    if ((pcode & 1) == 0) {
+        n.llo[0] = lo[0]; n.llo[1] = lo[1]; n.llo[2] = lo[2];
+        n.lhi[0] = hi[0]; n.lhi[1] = hi[1]; n.lhi[2] = hi[2];
+    } else {
+        n.rlo[0] = lo[0]; n.rlo[1] = lo[1]; n.rlo[2] = lo[2];
+        n.rhi[0] = hi[0]; n.rhi[1] = hi[1]; n.rhi[2] = hi[2];
+    }
+}
+
+// Bottom-up refit: one thread per leaf; the second thread to arrive at a node
+// (atomic counter, acquire/release at agent scope) merges both child boxes and
+// continues upwards.  The root's box goes to root_box.
+__global__ void refit_kernel(const GRecord* __restrict__ rec, int P, int NL, int L, BNode* nodes,
+                             const int* __restrict__ parent_int, const int* __restrict__ parent_leaf,
+                             uint32_t* counters, float* root_box)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= NL) return;
+    float lo[3] = { __builtin_inff(), __builtin_inff(), __builtin_inff() };
+    float hi[3] = { -__builtin_inff(), -__builtin_inff(), -__builtin_inff() };
+    bool any = false;
+    const int g0 = j * L, g1 = min(g0 + L, P);
+    for (int g = g0; g < g1; ++g) {
+        const GRecord& r = rec[g];
+        // NaN boxes are skipped by fminf/fmaxf; a leaf of only NaN boxes stays NaN.
+        lo[0] = fminf(lo[0], r.lox); lo[1] = fminf(lo[1], r.loy); lo[2] = fminf(lo[2], r.loz);
+        hi[0] = fmaxf(hi[0], r.hix); hi[1] = fmaxf(hi[1], r.hiy); hi[2] = fmaxf(hi[2], r.hiz);
+        any |= (r.lox == r.lox);
+    }
+    if (!any) {
+        lo[0] = lo[1] = lo[2] = __builtin_nanf("");
+        hi[0] = hi[1] = hi[2] = __builtin_nanf("");
+    }
+    if (NL == 1) {
+        for (int c = 0; c < 3; ++c) { root_box[c] = lo[c]; root_box[3 + c] = hi[c]; }
+        return;
+    }
+    int pcode = parent_leaf[j];
+    store_child_box(nodes, pcode, lo, hi);
+    while (true) {
+        const int p = pcode >> 1;
+        const uint32_t old = __hip_atomic_fetch_add(&counters[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) return;
+        const BNode& n = nodes[p];
+        float nlo[3], nhi[3];
+        for (int c = 0; c < 3; ++c) {
+            nlo[c] = fminf(n.llo[c], n.rlo[c]);
+            nhi[c] = fmaxf(n.lhi[c], n.rhi[c]);
+        }
+        if (p == 0) {
+            for (int c = 0; c < 3; ++c) { root_box[c] = nlo[c]; root_box[3 + c] = nhi[c]; }
+            return;
+        }
+        pcode = parent_int[p];
+        store_child_box(nodes, pcode, nlo, nhi);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// forward walk
+// ---------------------------------------------------------------------------
+struct GroupBox {
+    float lx, ly, lz, hx, hy, hz;
+};
+
+__device__ __forceinline__ GroupBox group_box(float px, float py, float pz)
+{
+    GroupBox b;
+    b.lx = uniformf(wave_minf(px)); b.ly = uniformf(wave_minf(py)); b.lz = uniformf(wave_minf(pz));
+    b.hx = uniformf(wave_maxf(px)); b.hy = uniformf(wave_maxf(py)); b.hz = uniformf(wave_maxf(pz));
+    return b;
+}
+
+__device__ __forceinline__ bool node_overlaps(const float lo[3], const float hi[3], const GroupBox& b)
+{
+    return boxes_overlap(lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], b.lx, b.ly, b.lz, b.hx, b.hy, b.hz);
+}
+
+// Visits, in a fixed depth-first order, every leaf whose box overlaps the group
+// box and calls leaf_fn(leaf).  Returns false on stack overflow.
+template <class LeafFn>
+__device__ __forceinline__ bool walk_tree(const BNode* __restrict__ nodes, const float* __restrict__ root_box,
+                                          int NL, const GroupBox& gb, int* stack, LeafFn&& leaf_fn)
+{
+    if (NL <= 0) return true;
+    if (NL == 1) {
+        float lo[3] = { root_box[0], root_box[1], root_box[2] };
+        float hi[3] = { root_box[3], root_box[4], root_box[5] };
+        if (node_overlaps(lo, hi, gb)) leaf_fn(0);
+        return true;
+    }
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        const BNode nd = nodes[node];
+        const bool ovl = node_overlaps(nd.llo, nd.lhi, gb);
+        const bool ovr = node_overlaps(nd.rlo, nd.rhi, gb);
+        const int cl = nd.left, cr = nd.right;
+        if (ovl && cl < 0) leaf_fn(~cl);
+        if (ovr && cr < 0) leaf_fn(~cr);
+        const bool dl = ovl && cl >= 0, dr = ovr && cr >= 0;
+        if (dl) {
+            if (dr) {
+                if (sp >= kStackDepth) return false;
+                stack[sp++] = cr;
+            }
+            node = cl;
+        } else if (dr) {
+            node = cr;
+        } else {
+            if (sp == 0) break;
+            node = uniformi(stack[--sp]);
+        }
+    }
+    return true;
+}
+
+template <bool KAHAN>
+__global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict__ rec, const BNode* __restrict__ nodes,
+                                                      const float* __restrict__ root_box, int P, int NL, int L,
+                                                      const float4* __restrict__ pts, int S, int NG,
+                                                      float* __restrict__ out_cells, float* __restrict__ out_weights,
+                                                      uint32_t* __restrict__ group_records,
+                                                      uint32_t* __restrict__ group_pairs, uint32_t* __restrict__ err)
+{
+    __shared__ int stacks[16][kStackDepth];
+    const int wpb = blockDim.x >> 6;
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = (int)xcd_block(blockIdx.x, gridDim.x) * wpb + wid;
+    if (g >= NG) return;
+    const int pi = g * 64 + lane;
+    float px = __builtin_nanf(""), py = px, pz = px;
+    int orig = -1;
+    if (pi < S) {
+        const float4 p = pts[pi];
+        px = p.x; py = p.y; pz = p.z;
+        orig = __builtin_bit_cast(int, p.w);
+    }
+    const GroupBox gb = group_box(px, py, pz);
+
+    float W = 0.f, cW = 0.f, U = 0.f, cU = 0.f;
+    uint32_t npairs = 0;
+    uint32_t nrec = 0;
+    auto leaf = [&](int j) {
+        const int g0 = j * L;
+        const int g1 = min(g0 + L, P);
+        for (int gi = g0; gi < g1; ++gi) {
+            const GRecord& r = rec[gi];
+            if (!boxes_overlap(r.lox, r.loy, r.loz, r.hix, r.hiy, r.hiz, gb.lx, gb.ly, gb.lz, gb.hx, gb.hy, gb.hz))
+                continue;
+            const bool inb = r.lox <= px && px <= r.hix && r.loy <= py && py <= r.hiy && r.loz <= pz && pz <= r.hiz;
+            const float dx = px - r.mx, dy = py - r.my, dz = pz - r.mz;
+            float r0, r1, r2;
+            const float power = pair_power(r.c0, r.c1, r.c2, r.c3, r.c4, r.c5, dx, dy, dz, r0, r1, r2);
+            const bool acc = inb && in_window(power);
+            if (__ballot(acc) == 0ull) continue;
+            ++nrec;
+            if (acc) {
+                const float e = __builtin_amdgcn_exp2f(power * kLog2e);
+                const float wgt = r.w * e;
+                const float wv = wgt * r.v;
+                if (KAHAN) {
+                    float y = wgt - cW;
+                    float t = W + y;
+                    cW = (t - W) - y;
+                    W = t;
+                    y = wv - cU;
+                    t = U + y;
+                    cU = (t - U) - y;
+                    U = t;
+                } else {
+                    W += wgt;
+                    U += wv;
+                }
+                ++npairs;
+            }
+        }
+    };
+    const bool ok = walk_tree(nodes, root_box, NL, gb, stacks[wid], leaf);
+    if (!ok && lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_STACK_OVERFLOW);
+    if (orig >= 0) {
+        float V, Wo;
+        if ((double)W <= kWeightCutoff) {
+            V = -1.0f;
+            Wo = 0.0f;
+        } else {
+            V = U / W;
+            Wo = W;
+        }
+        out_cells[orig] = V;
+        out_weights[orig] = Wo;
+    }
+    const uint32_t tp = wave_sumu(npairs);
+    if (lane == 0) {
+        group_records[g] = nrec;
+        group_pairs[g] = tp;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// backward walk
+// ---------------------------------------------------------------------------
+struct alignas(16) PartialRecord {
+    int gid;           // Gaussian index in BVH order
+    float dv, dw;      // d/dvalue, d/dweight
+    float dmx, dmy, dmz;
+    float dc[6];       // d/dconic, packed, off-diagonals counted once each
+};
+static_assert(sizeof(PartialRecord) == 48, "PartialRecord layout");
+
+__global__ __launch_bounds__(256) void backward_kernel(
+    const GRecord* __restrict__ rec, const BNode* __restrict__ nodes, const float* __restrict__ root_box, int P,
+    int NL, int L, const float4* __restrict__ pts, int S, int NG, const float* __restrict__ out_cells,
+    const float* __restrict__ out_weights, const float* __restrict__ gcells, const float* __restrict__ gweights,
+    const uint32_t* __restrict__ group_offsets, uint64_t capacity, PartialRecord* __restrict__ out_rec,
+    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ err)
+{
+    __shared__ int stacks[16][kStackDepth];
+    __shared__ float4 spA[16][64];  // x, y, z, V
+    __shared__ float2 spB[16][64];  // gV / W, gW   (zero where W <= cutoff)
+    const int wpb = blockDim.x >> 6;
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = (int)xcd_block(blockIdx.x, gridDim.x) * wpb + wid;
+    if (g >= NG) return;
+    const int pi = g * 64 + lane;
+    float px = __builtin_nanf(""), py = px, pz = px;
+    float V = 0.f, gVi = 0.f, gWe = 0.f;
+    if (pi < S) {
+        const float4 p = pts[pi];
+        px = p.x; py = p.y; pz = p.z;
+        const int orig = __builtin_bit_cast(int, p.w);
+        const float Wp = out_weights[orig];
+        if ((double)Wp > kWeightCutoff) {
+            V = out_cells[orig];
+            gVi = gcells[orig] / Wp;
+            gWe = gweights[orig];
+        }
+    }
+    spA[wid][lane] = make_float4(px, py, pz, V);
+    spB[wid][lane] = make_float2(gVi, gWe);
+    const int npts = min(64, S - g * 64);
+    const GroupBox gb = group_box(px, py, pz);
+    __builtin_amdgcn_wave_barrier();
+
+    uint32_t base = group_offsets[g];
+    const uint32_t limit = group_offsets[g + 1];
+    int qn = 0;
+    int qv = 0;
+
+    auto flush = [&]() {
+        const bool active = lane < qn;
+        const int gi = active ? qv : 0;
+        const GRecord r = rec[gi];
+        float dv = 0.f, dw = 0.f, dmx = 0.f, dmy = 0.f, dmz = 0.f;
+        float dc0 = 0.f, dc1 = 0.f, dc2 = 0.f, dc3 = 0.f, dc4 = 0.f, dc5 = 0.f;
+        bool any = false;
+        for (int p = 0; p < npts; ++p) {
+            const float4 a = spA[wid][p];
+            const float2 b = spB[wid][p];
+            const bool inb = active && r.lox <= a.x && a.x <= r.hix && r.loy <= a.y && a.y <= r.hiy &&
+                             r.loz <= a.z && a.z <= r.hiz;
+            const float dx = a.x - r.mx, dy = a.y - r.my, dz = a.z - r.mz;
+            float r0, r1, r2;
+            const float power = pair_power(r.c0, r.c1, r.c2, r.c3, r.c4, r.c5, dx, dy, dz, r0, r1, r2);
+            const bool acc = inb && in_window(power);
+            if (__ballot(acc) == 0ull) continue;
+            any |= acc;
+            if (acc) {
+                const float e = __builtin_amdgcn_exp2f(power * kLog2e);
+                const float wgt = r.w * e;
+                dv = __builtin_fmaf(b.x, wgt, dv);
+                const float t = __builtin_fmaf(b.x, r.v - a.w, b.y);
+                dw = __builtin_fmaf(t, e, dw);
+                const float st = wgt * t;
+                dmx = __builtin_fmaf(st, r0, dmx);
+                dmy = __builtin_fmaf(st, r1, dmy);
+                dmz = __builtin_fmaf(st, r2, dmz);
+                const float dq = -0.5f * st;
+                const float h0 = dq * dx, h1 = dq * dy, h2 = dq * dz;
+                const float h0x2 = h0 + h0, h1x2 = h1 + h1;
+                dc0 = __builtin_fmaf(h0, dx, dc0);
+                dc1 = __builtin_fmaf(h0x2, dy, dc1);
+                dc2 = __builtin_fmaf(h0x2, dz, dc2);
+                dc3 = __builtin_fmaf(h1, dy, dc3);
+                dc4 = __builtin_fmaf(h1x2, dz, dc4);
+                dc5 = __builtin_fmaf(h2, dz, dc5);
+            }
+        }
+        const uint64_t bal = __ballot(any);
+        const uint32_t rank = (uint32_t)__popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+        if (any) {
+            const uint64_t pos = (uint64_t)base + rank;
+            if (pos < limit && pos < capacity) {
+                PartialRecord o;
+                o.gid = gi;
+                o.dv = dv; o.dw = dw;
+                o.dmx = dmx; o.dmy = dmy; o.dmz = dmz;
+                o.dc[0] = dc0; o.dc[1] = dc1; o.dc[2] = dc2; o.dc[3] = dc3; o.dc[4] = dc4; o.dc[5] = dc5;
+                out_rec[pos] = o;
+                keys[pos] = (uint32_t)gi;
+                vals[pos] = (uint32_t)pos;
+            } else {
+                atomicOr(err, (uint32_t)GVC_ERRFLAG_RECORD_OVERFLOW);
+            }
+        }
+        base += (uint32_t)__popcll(bal);
+        qn = 0;
+    };
+    auto leaf = [&](int j) {
+        const int g0 = j * L;
+        const int g1 = min(g0 + L, P);
+        for (int gi = g0; gi < g1; ++gi) {
+            const GRecord& r = rec[gi];
+            if (!boxes_overlap(r.lox, r.loy, r.loz, r.hix, r.hiy, r.hiz, gb.lx, gb.ly, gb.lz, gb.hx, gb.hy, gb.hz))
+                continue;
+            if (lane == qn) qv = gi;
+            ++qn;
+            if (qn == 64) flush();
+        }
+    };
+    const bool ok = walk_tree(nodes, root_box, NL, gb, stacks[wid], leaf);
+    if (!ok && lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_STACK_OVERFLOW);
+    if (qn > 0) flush();
+}
+
+// Pads the sort input beyond the emitted records with key P (sorts last).
+__global__ void pad_keys_kernel(uint32_t* keys, uint32_t* vals, const uint32_t* total, uint64_t capacity, uint32_t pad)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= capacity) return;
+    if (k >= (uint64_t)*total) {
+        keys[k] = pad;
+        vals[k] = (uint32_t)k;
+    }
+}
+
+__global__ void segment_kernel(const uint32_t* __restrict__ keys, uint64_t n, uint32_t P, uint32_t* __restrict__ begin,
+                               uint32_t* __restrict__ end)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t key = keys[k];
+    if (key >= P) return;
+    if (k == 0 || keys[k - 1] != key) begin[key] = (uint32_t)k;
+    if (k + 1 == n || keys[k + 1] != key) end[key] = (uint32_t)(k + 1);
+}
+
+// Per Gaussian: FP64 sum of its partial records in group order, then the
+// covariance chain (conic gradient -> Sigma -> M -> scales, rotations), all in
+// double from the float inputs.
+__global__ void reduce_chain_kernel(int P, const GRecord* __restrict__ rec, const uint32_t* __restrict__ begin,
+                                    const uint32_t* __restrict__ end, const uint32_t* __restrict__ sorted_vals,
+                                    const PartialRecord* __restrict__ parts, const float* __restrict__ scales,
+                                    const float* __restrict__ rots, float kmod, float* __restrict__ dmeans,
+                                    float* __restrict__ dscales, float* __restrict__ drots,
+                                    float* __restrict__ dvalues, float* __restrict__ dweights)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    double acc[11] = { 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+    const uint32_t b = begin[i], e = end[i];
+    for (uint32_t k = b; k < e; ++k) {
+        const PartialRecord& pr = parts[sorted_vals[k]];
+        acc[0] += pr.dv; acc[1] += pr.dw;
+        acc[2] += pr.dmx; acc[3] += pr.dmy; acc[4] += pr.dmz;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[5 + c] += pr.dc[c];
+    }
+    const int id = rec[i].id;
+    dvalues[id] = (float)acc[0];
+    dweights[id] = (float)acc[1];
+    dmeans[3 * id + 0] = (float)acc[2];
+    dmeans[3 * id + 1] = (float)acc[3];
+    dmeans[3 * id + 2] = (float)acc[4];
+
+    // covariance chain
+    const float q4[4] = { rots[4 * id], rots[4 * id + 1], rots[4 * id + 2], rots[4 * id + 3] };
+    double R[3][3];
+    rotation_d(q4[0], q4[1], q4[2], q4[3], R);
+    const double km = kmod;
+    const double s[3] = { km * scales[3 * id], km * scales[3 * id + 1], km * scales[3 * id + 2] };
+    double M[3][3];
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) M[a][c] = s[a] * R[c][a];
+    double Sg[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) Sg[r][c] = M[0][r] * M[0][c] + M[1][r] * M[1][c] + M[2][r] * M[2][c];
+    int jx = 0;
+    double best = fabs(Sg[0][0]);
+    if (fabs(Sg[1][1]) > best) { best = fabs(Sg[1][1]); jx = 1; }
+    if (fabs(Sg[2][2]) > best) { best = fabs(Sg[2][2]); jx = 2; }
+    const double eps = best * 1e-5;
+    const double a_ = Sg[0][0] + eps, b_ = Sg[0][1], c_ = Sg[0][2], d_ = Sg[1][1] + eps, e_ = Sg[1][2],
+                 f_ = Sg[2][2] + eps;
+    const double A00 = d_ * f_ - e_ * e_, A01 = c_ * e_ - b_ * f_, A02 = b_ * e_ - c_ * d_;
+    const double A11 = a_ * f_ - c_ * c_, A12 = b_ * c_ - a_ * e_, A22 = a_ * d_ - b_ * b_;
+    const double det = a_ * A00 + b_ * A01 + c_ * A02;
+    const double C[3][3] = { { A00 / det, A01 / det, A02 / det },
+                             { A01 / det, A11 / det, A12 / det },
+                             { A02 / det, A12 / det, A22 / det } };
+    const double G[3][3] = { { acc[5], 0.5 * acc[6], 0.5 * acc[7] },
+                             { 0.5 * acc[6], acc[8], 0.5 * acc[9] },
+                             { 0.5 * acc[7], 0.5 * acc[9], acc[10] } };
+    double CG[3][3], K[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) CG[r][c] = C[r][0] * G[0][c] + C[r][1] * G[1][c] + C[r][2] * G[2][c];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) K[r][c] = -(CG[r][0] * C[0][c] + CG[r][1] * C[1][c] + CG[r][2] * C[2][c]);
+    const double trH = K[0][0] + K[1][1] + K[2][2];
+    const double sg = Sg[jx][jx] > 0 ? 1.0 : (Sg[jx][jx] < 0 ? -1.0 : 0.0);
+    K[jx][jx] += trH * 1e-5 * sg;
+    double dM[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) dM[r][c] = 2.0 * (M[r][0] * K[0][c] + M[r][1] * K[1][c] + M[r][2] * K[2][c]);
+    for (int a = 0; a < 3; ++a) {
+        const double v = dM[a][0] * R[0][a] + dM[a][1] * R[1][a] + dM[a][2] * R[2][a];
+        dscales[3 * id + a] = (float)(km * v);
+    }
+    double Gq[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int a = 0; a < 3; ++a) Gq[r][a] = dM[a][r] * s[a];
+    const double qr = q4[0], qx = q4[1], qy = q4[2], qz = q4[3];
+    drots[4 * id + 0] = (float)(2.0 * (qz * (Gq[1][0] - Gq[0][1]) + qy * (Gq[0][2] - Gq[2][0]) +
+                                       qx * (Gq[2][1] - Gq[1][2])));
+    drots[4 * id + 1] = (float)(2.0 * (qy * (Gq[0][1] + Gq[1][0]) + qz * (Gq[0][2] + Gq[2][0]) +
+                                       qr * (Gq[2][1] - Gq[1][2])) - 4.0 * qx * (Gq[1][1] + Gq[2][2]));
+    drots[4 * id + 2] = (float)(2.0 * (qx * (Gq[0][1] + Gq[1][0]) + qr * (Gq[0][2] - Gq[2][0]) +
+                                       qz * (Gq[1][2] + Gq[2][1])) - 4.0 * qy * (Gq[0][0] + Gq[2][2]));
+    drots[4 * id + 3] = (float)(2.0 * (qr * (Gq[1][0] - Gq[0][1]) + qx * (Gq[0][2] + Gq[2][0]) +
+                                       qy * (Gq[1][2] + Gq[2][1])) - 4.0 * qz * (Gq[0][0] + Gq[1][1]));
+}
+
+__global__ void zero_floats_kernel(float* p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+__global__ void stats_kernel(const uint32_t* __restrict__ group_pairs, int NG, const uint32_t* __restrict__ offsets,
+                             unsigned long long* __restrict__ out)
+{
+    // out[0] = records, out[1] = pairs
+    __shared__ unsigned long long part[256];
+    unsigned long long s = 0;
+    for (int i = threadIdx.x; i < NG; i += blockDim.x) s += group_pairs[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = 128; d >= 1; d >>= 1) {
+        if ((int)threadIdx.x < d) part[threadIdx.x] += part[threadIdx.x + d];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = offsets[NG];
+        out[1] = part[0];
+    }
+}
+
+inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+inline int key_bits_for(uint64_t maxv)
+{
+    int b = 0;
+    while (b < 64 && (maxv >> b) != 0) ++b;
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// workspace layouts
+// ---------------------------------------------------------------------------
+struct SampleSetLayout {
+    SampleSetHeader* header;
+    float4* pts;
+    uint32_t *keys, *idx, *keys_alt, *idx_alt, *scratch;
+    SampleSetLayout(Arena& a, int S)
+    {
+        header = a.take<SampleSetHeader>(1);
+        pts = a.take<float4>((size_t)S);
+        keys = a.take<uint32_t>((size_t)S);
+        idx = a.take<uint32_t>((size_t)S);
+        keys_alt = a.take<uint32_t>((size_t)S);
+        idx_alt = a.take<uint32_t>((size_t)S);
+        scratch = a.take<uint32_t>(radix_scratch_elems((size_t)S));
+    }
+};
+
+struct ForwardLayout {
+    GRecord* rec;
+    uint64_t *codes, *codes_alt, *leaf_codes;
+    uint32_t *idx, *idx_alt, *sort_scratch;
+    BNode* nodes;
+    int *parent_int, *parent_leaf;
+    uint32_t* counters;
+    float* root_box;
+    uint32_t *group_records, *group_offsets, *group_pairs, *scan_scratch, *err;
+    unsigned long long* stats;
+    int NL, NG;
+    ForwardLayout(Arena& a, int P, int S, int L)
+    {
+        NL = P > 0 ? (P + L - 1) / L : 0;
+        NG = (S + 63) / 64;
+        rec = a.take<GRecord>((size_t)P);
+        codes = a.take<uint64_t>((size_t)P);
+        codes_alt = a.take<uint64_t>((size_t)P);
+        idx = a.take<uint32_t>((size_t)P);
+        idx_alt = a.take<uint32_t>((size_t)P);
+        sort_scratch = a.take<uint32_t>(radix_scratch_elems((size_t)P));
+        leaf_codes = a.take<uint64_t>((size_t)NL);
+        nodes = a.take<BNode>((size_t)(NL > 1 ? NL - 1 : 1));
+        parent_int = a.take<int>((size_t)(NL > 1 ? NL - 1 : 1));
+        parent_leaf = a.take<int>((size_t)(NL > 0 ? NL : 1));
+        counters = a.take<uint32_t>((size_t)(NL > 1 ? NL - 1 : 1));
+        root_box = a.take<float>(8);
+        group_records = a.take<uint32_t>((size_t)NG + 1);
+        group_offsets = a.take<uint32_t>((size_t)NG + 1);
+        group_pairs = a.take<uint32_t>((size_t)NG + 1);
+        scan_scratch = a.take<uint32_t>(scan_scratch_elems((size_t)NG) + 1);
+        err = a.take<uint32_t>(4);
+        stats = a.take<unsigned long long>(2);
+    }
+};
+
+struct BackwardLayout {
+    PartialRecord* parts;
+    uint32_t *keys, *vals, *keys_alt, *vals_alt, *scratch, *begin, *end;
+    BackwardLayout(Arena& a, int P, uint64_t cap)
+    {
+        const size_t c = (size_t)(cap > 0 ? cap : 1);
+        parts = a.take<PartialRecord>(c);
+        keys = a.take<uint32_t>(c);
+        vals = a.take<uint32_t>(c);
+        keys_alt = a.take<uint32_t>(c);
+        vals_alt = a.take<uint32_t>(c);
+        scratch = a.take<uint32_t>(radix_scratch_elems(c));
+        begin = a.take<uint32_t>((size_t)(P > 0 ? P : 1));
+        end = a.take<uint32_t>((size_t)(P > 0 ? P : 1));
+    }
+};
+
+int check_params(const gvc_params* prm)
+{
+    if (!prm) return set_error(GVC_ERR_INVALID_ARGUMENT, "params is NULL");
+    if (prm->leaf_size < 1 || prm->leaf_size > 64)
+        return set_error(GVC_ERR_INVALID_ARGUMENT, "leaf_size must be in [1, 64]");
+    if (prm->block_threads < 64 || prm->block_threads > 1024 || (prm->block_threads % 64) != 0)
+        return set_error(GVC_ERR_INVALID_ARGUMENT, "block_threads must be a multiple of 64 in [64, 1024]");
+    return GVC_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host entry points (called from capi.cpp)
+// ---------------------------------------------------------------------------
+int preprocess_launch(int P, const float* means, const float* scales, const float* rots, const float* weights,
+                      float k, unsigned flags, float* conics, float* boxes, hipStream_t st)
+{
+    if (P <= 0) return GVC_OK;
+    preprocess_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, means, scales, rots, weights, k,
+                                                          (flags & GVC_FLAG_FP64_COND) ? 1 : 0, conics, boxes);
+    GVC_LAUNCH_CHECK("preprocess");
+    return GVC_OK;
+}
+
+int morton_launch(int n, const float* xyz, int clamp, uint64_t* codes, hipStream_t st)
+{
+    if (n <= 0) return GVC_OK;
+    morton63_kernel<<<blocks_for(n, 256), 256, 0, st>>>(xyz, n, clamp, codes);
+    GVC_LAUNCH_CHECK("morton");
+    return GVC_OK;
+}
+
+size_t sample_set_bytes(int S)
+{
+    Arena a(nullptr, 0);
+    SampleSetLayout l(a, S);
+    return a.off + 256;
+}
+
+int sample_set_build(int S, const float* samples, void* set, size_t bytes, hipStream_t st)
+{
+    Arena a(set, bytes);
+    SampleSetLayout l(a, S);
+    if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "sample set buffer too small");
+    const int NG = (S + 63) / 64;
+    if (S > 0) {
+        sample_codes_kernel<<<blocks_for(S, 256), 256, 0, st>>>(samples, S, l.keys, l.idx);
+        GVC_LAUNCH_CHECK("sample_codes");
+        bool alt = false;
+        GVC_TRY(radix_sort_u32(l.keys, l.idx, l.keys_alt, l.idx_alt, (size_t)S, 30, l.scratch, &alt, st));
+        sample_gather_kernel<<<blocks_for(S, 256), 256, 0, st>>>(samples, S, alt ? l.idx_alt : l.idx, l.pts);
+        GVC_LAUNCH_CHECK("sample_gather");
+    }
+    sample_header_kernel<<<1, 1, 0, st>>>(l.header, S, NG);
+    GVC_LAUNCH_CHECK("sample_header");
+    return GVC_OK;
+}
+
+size_t forward_state_bytes(int P, int S, const gvc_params* prm)
+{
+    if (check_params(prm) != GVC_OK) return 0;
+    Arena a(nullptr, 0);
+    ForwardLayout l(a, P, S, prm->leaf_size);
+    return a.off + 256;
+}
+
+int mixture_forward(int P, const float* means, const float* scales, const float* rots, const float* values,
+                    const float* weights, float k, const void* sample_set, int S, const gvc_params* prm,
+                    float* out_cells, float* out_weights, float* conics, void* state, size_t state_bytes,
+                    hipStream_t st)
+{
+    GVC_TRY(check_params(prm));
+    if (P < 0 || S < 0) return set_error(GVC_ERR_INVALID_ARGUMENT, "negative size");
+    const int L = prm->leaf_size;
+    Arena a(state, state_bytes);
+    ForwardLayout l(a, P, S, L);
+    if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "forward state buffer too small");
+    Arena sa(const_cast<void*>(sample_set), sample_set_bytes(S));
+    SampleSetLayout sl(sa, S);
+    const int NG = l.NG, NL = l.NL;
+
+    GVC_TRY(fill_u32(l.err, 0u, 4, st));
+    if (P == 0) {
+        // DGR/rasterize_points.cu:51: the reference skips the kernels, leaving zeros.
+        if (S > 0) {
+            zero_floats_kernel<<<blocks_for(S, 256), 256, 0, st>>>(out_cells, S);
+            zero_floats_kernel<<<blocks_for(S, 256), 256, 0, st>>>(out_weights, S);
+            GVC_LAUNCH_CHECK("zero");
+        }
+        GVC_TRY(fill_u32(l.group_records, 0u, (size_t)NG + 1, st));
+        GVC_TRY(fill_u32(l.group_pairs, 0u, (size_t)NG + 1, st));
+        GVC_TRY(exclusive_scan_u32(l.group_records, l.group_offsets, (size_t)NG, true, l.scan_scratch, st));
+        return GVC_OK;
+    }
+    const int fp64 = (prm->flags & GVC_FLAG_FP64_COND) ? 1 : 0;
+    // 1. Morton order of the Gaussians
+    gaussian_codes_kernel<<<blocks_for(P, 256), 256, 0, st>>>(means, P, l.codes, l.idx);
+    GVC_LAUNCH_CHECK("gaussian_codes");
+    bool alt = false;
+    GVC_TRY(radix_sort_u64(l.codes, l.idx, l.codes_alt, l.idx_alt, (size_t)P, 64, l.sort_scratch, &alt, st));
+    const uint64_t* scodes = alt ? l.codes_alt : l.codes;
+    const uint32_t* perm = alt ? l.idx_alt : l.idx;
+    // 2. records
+    record_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, perm, means, scales, rots, values, weights, k, fp64, l.rec,
+                                                      conics);
+    GVC_LAUNCH_CHECK("record");
+    // 3. tree
+    leaf_codes_kernel<<<blocks_for(NL, 256), 256, 0, st>>>(scodes, NL, L, l.leaf_codes);
+    GVC_LAUNCH_CHECK("leaf_codes");
+    if (NL > 1) {
+        karras_kernel<<<blocks_for(NL - 1, 256), 256, 0, st>>>(l.leaf_codes, NL, l.nodes, l.parent_int, l.parent_leaf);
+        GVC_LAUNCH_CHECK("karras");
+        GVC_TRY(fill_u32(l.counters, 0u, (size_t)(NL - 1), st));
+    }
+    refit_kernel<<<blocks_for(NL, 256), 256, 0, st>>>(l.rec, P, NL, L, l.nodes, l.parent_int, l.parent_leaf,
+                                                      l.counters, l.root_box);
+    GVC_LAUNCH_CHECK("refit");
+    // 4. walk
+    if (NG > 0) {
+        const int T = prm->block_threads > 1024 ? 1024 : prm->block_threads;
+        const int wpb = T / 64;
+        const unsigned nb = blocks_for(NG, wpb);
+        if (prm->flags & GVC_FLAG_KAHAN)
+            forward_kernel<true><<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells,
+                                                   out_weights, l.group_records, l.group_pairs, l.err);
+        else
+            forward_kernel<false><<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells,
+                                                    out_weights, l.group_records, l.group_pairs, l.err);
+        GVC_LAUNCH_CHECK("forward");
+    }
+    GVC_TRY(exclusive_scan_u32(l.group_records, l.group_offsets, (size_t)NG, true, l.scan_scratch, st));
+    return GVC_OK;
+}
+
+int forward_stats(const void* state, int P, int S, const gvc_params* prm, uint64_t* n_records, uint64_t* n_pairs,
+                  uint32_t* error_flags, hipStream_t st)
+{
+    GVC_TRY(check_params(prm));
+    Arena a(const_cast<void*>(state), (size_t)-1);
+    ForwardLayout l(a, P, S, prm->leaf_size);
+    stats_kernel<<<1, 256, 0, st>>>(l.group_pairs, l.NG, l.group_offsets, l.stats);
+    GVC_LAUNCH_CHECK("stats");
+    unsigned long long h[2] = { 0, 0 };
+    uint32_t e = 0;
+    GVC_HIP(hipMemcpyAsync(h, l.stats, sizeof(h), hipMemcpyDeviceToHost, st));
+    GVC_HIP(hipMemcpyAsync(&e, l.err, sizeof(e), hipMemcpyDeviceToHost, st));
+    GVC_HIP(hipStreamSynchronize(st));
+    if (n_records) *n_records = h[0];
+    if (n_pairs) *n_pairs = h[1];
+    if (error_flags) *error_flags = e;
+    return GVC_OK;
+}
+
+size_t backward_bytes(int P, uint64_t cap)
+{
+    Arena a(nullptr, 0);
+    BackwardLayout l(a, P, cap);
+    return a.off + 256;
+}
+
+int mixture_backward(int P, const float* means, const float* scales, const float* rots, const float* values,
+                     const float* weights, float k, const void* sample_set, int S, const gvc_params* prm,
+                     const float* out_cells, const float* out_weights, const float* gcells, const float* gweights,
+                     void* state, size_t state_bytes, uint64_t capacity, void* ws, size_t ws_bytes, float* dmeans,
+                     float* dscales, float* drots, float* dvalues, float* dweights, hipStream_t st)
+{
+    (void)means; (void)values; (void)weights;
+    GVC_TRY(check_params(prm));
+    if (P <= 0) return GVC_OK;
+    if (capacity > 0xffffffffull) return set_error(GVC_ERR_INVALID_ARGUMENT, "record capacity above 2^32");
+    const int L = prm->leaf_size;
+    Arena a(state, state_bytes);
+    ForwardLayout l(a, P, S, L);
+    if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "forward state buffer too small");
+    Arena ba(ws, ws_bytes);
+    BackwardLayout bl(ba, P, capacity);
+    if (!ba.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "backward workspace too small");
+    Arena sa(const_cast<void*>(sample_set), sample_set_bytes(S));
+    SampleSetLayout sl(sa, S);
+    const int NG = l.NG, NL = l.NL;
+    if (NG > 0 && capacity > 0) {
+        const int T = prm->block_threads;
+        const unsigned nb = blocks_for(NG, T / 64);
+        backward_kernel<<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells, out_weights,
+                                          gcells, gweights, l.group_offsets, capacity, bl.parts, bl.keys, bl.vals,
+                                          l.err);
+        GVC_LAUNCH_CHECK("backward");
+        pad_keys_kernel<<<blocks_for(capacity, 256), 256, 0, st>>>(bl.keys, bl.vals, l.group_offsets + NG, capacity,
+                                                                  (uint32_t)P);
+        GVC_LAUNCH_CHECK("pad_keys");
+    }
+    GVC_TRY(fill_u32(bl.begin, 0u, (size_t)P, st));
+    GVC_TRY(fill_u32(bl.end, 0u, (size_t)P, st));
+    const uint32_t* skeys = bl.keys;
+    const uint32_t* svals = bl.vals;
+    if (NG > 0 && capacity > 0) {
+        bool alt = false;
+        GVC_TRY(radix_sort_u32(bl.keys, bl.vals, bl.keys_alt, bl.vals_alt, (size_t)capacity,
+                               key_bits_for((uint64_t)P), bl.scratch, &alt, st));
+        skeys = alt ? bl.keys_alt : bl.keys;
+        svals = alt ? bl.vals_alt : bl.vals;
+        segment_kernel<<<blocks_for(capacity, 256), 256, 0, st>>>(skeys, capacity, (uint32_t)P, bl.begin, bl.end);
+        GVC_LAUNCH_CHECK("segment");
+    }
+    reduce_chain_kernel<<<blocks_for(P, 128), 128, 0, st>>>(P, l.rec, bl.begin, bl.end, svals, bl.parts, scales, rots,
+                                                            k, dmeans, dscales, drots, dvalues, dweights);
+    GVC_LAUNCH_CHECK("reduce_chain");
+    return GVC_OK;
+}
+
+int sort_pairs_u64(int n, const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, int key_bits,
+                   void* ws, size_t ws_bytes, hipStream_t st)
+{
+    if (n <= 0) return GVC_OK;
+    Arena a(ws, ws_bytes);
+    uint64_t* k2 = a.take<uint64_t>((size_t)n);
+    uint32_t* v2 = a.take<uint32_t>((size_t)n);
+    uint32_t* scr = a.take<uint32_t>(radix_scratch_elems((size_t)n));
+    if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "sort workspace too small");
+    GVC_HIP(hipMemcpyAsync(kout, kin, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
+    GVC_HIP(hipMemcpyAsync(vout, vin, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    bool alt = false;
+    GVC_TRY(radix_sort_u64(kout, vout, k2, v2, (size_t)n, key_bits, scr, &alt, st));
+    if (alt) {
+        GVC_HIP(hipMemcpyAsync(kout, k2, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
+        GVC_HIP(hipMemcpyAsync(vout, v2, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    }
+    return GVC_OK;
+}
+
+size_t sort_pairs_u64_bytes(int n)
+{
+    Arena a(nullptr, 0);
+    a.take<uint64_t>((size_t)n);
+    a.take<uint32_t>((size_t)n);
+    a.take<uint32_t>(radix_scratch_elems((size_t)n));
+    return a.off + 256;
+}
+
+}  // namespace gvc

@@ -1,0 +1,75 @@
+"""Drop-in for the reference's ``gpu_mesh_sampling`` (GS/gpu_mesh_sampling/__init__.py:1-20),
+running on libgvc instead of Viskores.
+
+``gpu_sample`` and ``gpu_sampleu`` keep their signatures and numpy-in/numpy-out
+behaviour (torch tensors are accepted too and give torch tensors back, without
+leaving the device).  Errors mirror GS/sample_mesh.cu: RuntimeError for a values
+size mismatch, an unsupported values dtype or an unsupported cell type.  Nothing
+is printed.  ``TetMeshSampler`` is the device-resident form for a training loop:
+it keeps the locator and also returns int32 cell ids.
+
+Cell types: tetrahedra (VTK 10) are located exactly; other types of the
+reference's table (GS/sample_mesh.cu:300-336: 1, 3, 5, 9, 12, 13, 14) raise.
+"""
+import numpy as np
+import torch
+
+from gvc.sampling import TetMeshSampler, grid_sample  # noqa: F401
+
+_VTK_KNOWN = {1, 3, 5, 9, 10, 12, 13, 14}
+
+
+def _device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("gpu_mesh_sampling needs a ROCm GPU")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def gpu_sample(dims, origin, spacing, values, samples):
+    """GS/sample_mesh.cu:26-240: trilinear probe of a uniform point grid; -1 outside.
+    Output dtype follows ``values`` (float64, float32, uint16, uint8)."""
+    as_torch = isinstance(samples, torch.Tensor)
+    dev = samples.device if as_torch and samples.is_cuda else _device()
+    if isinstance(values, torch.Tensor):
+        v = values.reshape(-1).to(dev)
+    else:
+        va = np.ascontiguousarray(values)
+        if va.dtype not in (np.float64, np.float32, np.uint16, np.uint8):
+            raise RuntimeError("Unsupported values dtype. Expected float64, float32, uint16, or uint8.")
+        v = torch.from_numpy(va.reshape(-1)).to(dev)
+    s = samples if as_torch else torch.from_numpy(np.ascontiguousarray(samples, dtype=np.float32))
+    out = grid_sample(np.asarray(dims), np.asarray(origin), np.asarray(spacing), v, s.to(dev))
+    return out if as_torch else out.cpu().numpy()
+
+
+def _tets_from_vtk(conn, cell_types, cell_offsets):
+    types = np.asarray(cell_types).astype(np.int64).reshape(-1)
+    C = types.shape[0]
+    conn = np.asarray(conn).astype(np.int64).reshape(-1)
+    offs = np.asarray(cell_offsets).astype(np.int64).reshape(-1)
+    bad = types[~np.isin(types, list(_VTK_KNOWN))]
+    if bad.size:
+        raise RuntimeError(f"Unsupported VTK cell type: {int(bad[0])}")
+    other = types[types != 10]
+    if other.size:
+        raise RuntimeError(f"VTK cell type {int(other[0])} is not supported by this build (tetrahedra only)")
+    start = offs[:C]
+    end = np.where(np.arange(C) + 1 < C, np.append(offs[1:C], 0), conn.shape[0])
+    if C and not np.all(end - start == 4):
+        raise RuntimeError("tetrahedra must have 4 point ids")
+    idx = start[:, None] + np.arange(4)[None, :]
+    return conn[idx]
+
+
+def gpu_sampleu(pts, conn, cell_types, cell_offsets, values, samples):
+    """GS/sample_mesh.cu:242-414: probe an unstructured mesh at the samples;
+    float64 values, -1 outside."""
+    as_torch = isinstance(samples, torch.Tensor)
+    dev = samples.device if as_torch and samples.is_cuda else _device()
+    tets = _tets_from_vtk(conn, cell_types, cell_offsets)
+    p = pts if isinstance(pts, torch.Tensor) else np.ascontiguousarray(pts, dtype=np.float32)
+    v = values if isinstance(values, torch.Tensor) else np.ascontiguousarray(values, dtype=np.float64)
+    sampler = TetMeshSampler(p, tets, v, device=dev)
+    s = samples if as_torch else torch.from_numpy(np.ascontiguousarray(samples, dtype=np.float32))
+    out, _ = sampler.sample(s.to(dev))
+    return out if as_torch else out.cpu().numpy()

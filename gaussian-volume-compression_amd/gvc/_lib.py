@@ -1,0 +1,94 @@
+"""ctypes binding of libgvc.so (the C ABI in include/gvc.h).
+
+There is no fallback: if the library is missing or was built for another GPU,
+every entry point raises.  Build it with ``python -m gvc._build`` (or
+``__graft_entry__.build()``).
+"""
+import ctypes
+import os
+
+from ._build import LIB_PATH
+
+GVC_FLAG_KAHAN = 1
+GVC_FLAG_FP64_COND = 2
+GVC_ERRFLAG_STACK_OVERFLOW = 1
+GVC_ERRFLAG_RECORD_OVERFLOW = 2
+
+
+class GvcError(RuntimeError):
+    pass
+
+
+class gvc_params(ctypes.Structure):
+    _fields_ = [("leaf_size", ctypes.c_int), ("block_threads", ctypes.c_int),
+                ("flags", ctypes.c_uint), ("reserved", ctypes.c_int)]
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_U = ctypes.c_uint
+_F = ctypes.c_float
+_SZ = ctypes.c_size_t
+_U64 = ctypes.c_uint64
+_PP = ctypes.POINTER(gvc_params)
+
+# name: (restype, argtypes)
+_SIGS = {
+    "gvc_version": (_I, []),
+    "gvc_status_string": (ctypes.c_char_p, [_I]),
+    "gvc_last_error": (ctypes.c_char_p, []),
+    "gvc_preprocess": (_I, [_I, _P, _P, _P, _P, _F, _U, _P, _P, _P]),
+    "gvc_morton3d_unit": (_I, [_I, _P, _I, _P, _P]),
+    "gvc_sort_pairs_u64_bytes": (_SZ, [_I]),
+    "gvc_sort_pairs_u64": (_I, [_I, _P, _P, _P, _P, _I, _P, _SZ, _P]),
+    "gvc_sample_set_bytes": (_SZ, [_I]),
+    "gvc_sample_set_build": (_I, [_I, _P, _P, _SZ, _P]),
+    "gvc_forward_state_bytes": (_SZ, [_I, _I, _PP]),
+    "gvc_mixture_forward": (_I, [_I, _P, _P, _P, _P, _P, _F, _P, _I, _PP, _P, _P, _P, _P, _SZ, _P]),
+    "gvc_forward_stats": (_I, [_P, _I, _I, _PP, ctypes.POINTER(_U64), ctypes.POINTER(_U64),
+                               ctypes.POINTER(ctypes.c_uint32), _P]),
+    "gvc_backward_bytes": (_SZ, [_I, _U64]),
+    "gvc_mixture_backward": (_I, [_I, _P, _P, _P, _P, _P, _F, _P, _I, _PP, _P, _P, _P, _P, _P, _SZ, _U64,
+                                  _P, _SZ, _P, _P, _P, _P, _P, _P]),
+    "gvc_tet_locator_bytes": (_SZ, [_I, _I, _U64]),
+    "gvc_tet_locator_build": (_I, [_I, _P, _I, _P, _U64, _P, _SZ, ctypes.POINTER(_U64), _P]),
+    "gvc_tet_locate_interp": (_I, [_I, _P, _I, _P, _SZ, _U64, _P, _I, _P, _P, _P, _P, _P]),
+    "gvc_grid_sample": (_I, [_P, _P, _P, _I, _P, _I, _P, _P, _P]),
+    "gvc_knn3_bytes": (_SZ, [_I]),
+    "gvc_knn3": (_I, [_I, _P, _P, _P, _SZ, _P]),
+}
+
+EXPORTED = tuple(_SIGS.keys())
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GvcError(f"libgvc.so not built ({LIB_PATH}); run __graft_entry__.build() "
+                           "or python -m gvc._build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            if not hasattr(L, name):
+                continue
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, what=""):
+    if status != 0:
+        L = lib()
+        msg = L.gvc_status_string(status).decode()
+        detail = L.gvc_last_error().decode()
+        raise GvcError(f"{what}: {msg} ({detail})" if what else f"{msg} ({detail})")
+
+
+def call(name, *args):
+    f = getattr(lib(), name)
+    st = f(*args)
+    check(st, name)

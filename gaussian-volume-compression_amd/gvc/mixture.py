@@ -1,0 +1,211 @@
+"""torch-level wrappers of the mixture entry points of libgvc.
+
+Device memory comes from torch (caching allocator, so the calls are capturable
+in a torch.cuda.CUDAGraph); every kernel runs on torch's current stream.  No CPU
+fallback exists: tensors must be float32 on a ROCm device.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib
+from ._lib import GVC_FLAG_FP64_COND, GVC_FLAG_KAHAN, call, gvc_params, lib
+
+DEFAULT_LEAF_SIZE = 4
+DEFAULT_BLOCK_THREADS = 256
+DEFAULT_FLAGS = GVC_FLAG_KAHAN
+
+
+@dataclass(frozen=True)
+class MixtureParams:
+    """Launch parameters (the autotuner's search space) and precision flags."""
+    leaf_size: int = DEFAULT_LEAF_SIZE
+    block_threads: int = DEFAULT_BLOCK_THREADS
+    kahan: bool = True
+    fp64_cond: bool = False
+
+    def c(self):
+        flags = (GVC_FLAG_KAHAN if self.kahan else 0) | (GVC_FLAG_FP64_COND if self.fp64_cond else 0)
+        return gvc_params(int(self.leaf_size), int(self.block_threads), flags, 0)
+
+
+_default_params = MixtureParams()
+_record_capacity = None   # fixed backward capacity (graph capture); None = read it
+
+
+def set_default_params(params):
+    global _default_params
+    _default_params = params
+
+
+def default_params():
+    return _default_params
+
+
+def set_record_capacity(n):
+    """Fix the backward's record capacity so no host read is needed (for graph
+    capture).  None restores the exact, synchronising count."""
+    global _record_capacity
+    _record_capacity = None if n is None else int(n)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _stream(dev):
+    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def _as_f32(t, name, shape_tail=None):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name} must be a GPU tensor (got {t.device}); there is no CPU path")
+    if t.dtype != torch.float32:
+        raise TypeError(f"{name} must be float32 (got {t.dtype})")
+    if shape_tail is not None and (t.dim() != 2 or t.shape[1] != shape_tail):
+        if not (shape_tail == 1 and t.dim() == 1):
+            raise ValueError(f"{name} must have shape (P, {shape_tail}), got {tuple(t.shape)}")
+    return t.contiguous()
+
+
+class SampleSet:
+    """Sample points in Morton-sorted 64-point groups (replaces the reference's
+    static sample BVH, DGR/ext.cu:8-65).  Holds a reference to ``samples``."""
+
+    def __init__(self, samples):
+        samples = _as_f32(samples, "samples", 3)
+        self.samples = samples
+        self.device = samples.device
+        self.S = int(samples.shape[0])
+        nbytes = lib().gvc_sample_set_bytes(self.S)
+        self.buffer = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        call("gvc_sample_set_build", self.S, _ptr(samples), _ptr(self.buffer), nbytes, _stream(self.device))
+
+
+class ForwardState:
+    """BVH, records and per-group counts kept from a forward for its backward."""
+
+    def __init__(self, buffer, params, P, S):
+        self.buffer = buffer
+        self.params = params
+        self.P = P
+        self.S = S
+        self._c = params.c()
+
+    def stats(self):
+        """(n_records, n_pairs, error_flags) -- synchronises with the host."""
+        nr = ctypes.c_uint64(0)
+        npairs = ctypes.c_uint64(0)
+        err = ctypes.c_uint32(0)
+        call("gvc_forward_stats", _ptr(self.buffer), self.P, self.S, ctypes.byref(self._c), ctypes.byref(nr),
+             ctypes.byref(npairs), ctypes.byref(err), _stream(self.buffer.device))
+        return int(nr.value), int(npairs.value), int(err.value)
+
+
+def mixture_forward(means3D, scales, rotations, values, weights, scale_modifier, sample_set, params=None,
+                    want_conics=True):
+    """Returns (cells [S], cell_weights [S], conics [6P] or None, ForwardState)."""
+    params = params or _default_params
+    means3D = _as_f32(means3D, "means3D", 3)
+    scales = _as_f32(scales, "scales", 3)
+    rotations = _as_f32(rotations, "rotations", 4)
+    values = _as_f32(values, "values", 1)
+    weights = _as_f32(weights, "weights", 1)
+    P = int(means3D.shape[0])
+    for name, t in (("scales", scales), ("rotations", rotations), ("values", values), ("weights", weights)):
+        if t.shape[0] != P:
+            raise ValueError(f"{name} has {t.shape[0]} rows, means3D has {P}")
+    dev = means3D.device
+    if sample_set.device != dev:
+        raise ValueError("sample set and Gaussians are on different devices")
+    S = sample_set.S
+    cp = params.c()
+    nbytes = lib().gvc_forward_state_bytes(P, S, ctypes.byref(cp))
+    if nbytes == 0:
+        raise _lib.GvcError("invalid mixture parameters")
+    state = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    cells = torch.empty(S, dtype=torch.float32, device=dev)
+    cw = torch.empty(S, dtype=torch.float32, device=dev)
+    conics = torch.empty(6 * P, dtype=torch.float32, device=dev) if want_conics else None
+    call("gvc_mixture_forward", P, _ptr(means3D), _ptr(scales), _ptr(rotations), _ptr(values), _ptr(weights),
+         float(scale_modifier), _ptr(sample_set.buffer), S, ctypes.byref(cp), _ptr(cells), _ptr(cw), _ptr(conics),
+         _ptr(state), nbytes, _stream(dev))
+    return cells, cw, conics, ForwardState(state, params, P, S)
+
+
+def mixture_backward(fstate, means3D, scales, rotations, values, weights, scale_modifier, sample_set, out_cells,
+                     out_weights, grad_cells, grad_weights, record_capacity=None):
+    """Gradients (means3D [P,3], scales [P,3], rotations [P,4], values [P,1], weights [P,1])."""
+    P, S = fstate.P, fstate.S
+    dev = fstate.buffer.device
+    means3D = _as_f32(means3D, "means3D", 3)
+    scales = _as_f32(scales, "scales", 3)
+    rotations = _as_f32(rotations, "rotations", 4)
+    values = _as_f32(values, "values", 1)
+    weights = _as_f32(weights, "weights", 1)
+    out_cells = _as_f32(out_cells, "out_cells")
+    out_weights = _as_f32(out_weights, "out_weights")
+    grad_cells = _as_f32(grad_cells, "grad_cells")
+    grad_weights = _as_f32(grad_weights, "grad_weights")
+    if grad_cells.numel() != S or grad_weights.numel() != S:
+        raise ValueError("upstream gradients must have one entry per sample")
+    gm = torch.empty((P, 3), dtype=torch.float32, device=dev)
+    gs = torch.empty((P, 3), dtype=torch.float32, device=dev)
+    gr = torch.empty((P, 4), dtype=torch.float32, device=dev)
+    gv = torch.empty((P, 1), dtype=torch.float32, device=dev)
+    gw = torch.empty((P, 1), dtype=torch.float32, device=dev)
+    if P == 0:
+        return gm, gs, gr, gv, gw
+    cap = record_capacity if record_capacity is not None else _record_capacity
+    if cap is None:
+        cap = fstate.stats()[0]
+    cp = fstate._c
+    nbytes = lib().gvc_backward_bytes(P, cap)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    call("gvc_mixture_backward", P, _ptr(means3D), _ptr(scales), _ptr(rotations), _ptr(values), _ptr(weights),
+         float(scale_modifier), _ptr(sample_set.buffer), S, ctypes.byref(cp), _ptr(out_cells), _ptr(out_weights),
+         _ptr(grad_cells), _ptr(grad_weights), _ptr(fstate.buffer), fstate.buffer.numel(), ctypes.c_uint64(cap),
+         _ptr(ws), nbytes, _ptr(gm), _ptr(gs), _ptr(gr), _ptr(gv), _ptr(gw), _stream(dev))
+    return gm, gs, gr, gv, gw
+
+
+def preprocess(means3D, scales, rotations, weights, scale_modifier=1.0, fp64_cond=False):
+    """Canonical conic [P,6] and box [P,6] (lo.xyz, hi.xyz)."""
+    means3D = _as_f32(means3D, "means3D", 3)
+    scales = _as_f32(scales, "scales", 3)
+    rotations = _as_f32(rotations, "rotations", 4)
+    weights = _as_f32(weights, "weights", 1)
+    P = int(means3D.shape[0])
+    dev = means3D.device
+    conic = torch.empty((P, 6), dtype=torch.float32, device=dev)
+    box = torch.empty((P, 6), dtype=torch.float32, device=dev)
+    call("gvc_preprocess", P, _ptr(means3D), _ptr(scales), _ptr(rotations), _ptr(weights), float(scale_modifier),
+         GVC_FLAG_FP64_COND if fp64_cond else 0, _ptr(conic), _ptr(box), _stream(dev))
+    return conic, box
+
+
+def morton3d_unit(xyz, clamp=False):
+    """63-bit Morton codes (int64 tensor holding the uint64 bit pattern)."""
+    xyz = _as_f32(xyz, "xyz", 3)
+    n = int(xyz.shape[0])
+    out = torch.empty(n, dtype=torch.int64, device=xyz.device)
+    call("gvc_morton3d_unit", n, _ptr(xyz), int(bool(clamp)), _ptr(out), _stream(xyz.device))
+    return out
+
+
+def sort_pairs_u64(keys, vals, key_bits=64):
+    """Stable sort of (uint64 key as int64 bits, int32 value) pairs."""
+    if keys.dtype != torch.int64 or vals.dtype != torch.int32:
+        raise TypeError("keys must be int64 (uint64 bits), vals int32")
+    n = int(keys.numel())
+    dev = keys.device
+    ko = torch.empty_like(keys)
+    vo = torch.empty_like(vals)
+    nbytes = lib().gvc_sort_pairs_u64_bytes(n)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    call("gvc_sort_pairs_u64", n, _ptr(keys.contiguous()), _ptr(vals.contiguous()), _ptr(ko), _ptr(vo), int(key_bits),
+         _ptr(ws), nbytes, _stream(dev))
+    return ko, vo

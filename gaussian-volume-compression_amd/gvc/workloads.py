@@ -1,0 +1,165 @@
+"""Seeded synthetic workloads of the benchmark configurations (numpy, host side).
+
+Draw order is part of the definition (numpy.random.default_rng(seed)):
+
+config0  2,048 Gaussians: means U[0,1)^3, log-scales N(ln 0.03, 0.3^2) per axis,
+         rotations = normalised N(0,1)^4 (uniform on S^3, stored r,x,y,z),
+         weights U(0,1), values U(0,1); samples = the 32^3 'ij' grid of [0,1]^3
+         (as R/train.py:151-155); target f(x) = sum_k sin(2 pi a_k . x),
+         a_k ~ U{1..4}^3; masked L1 loss (R/train.py:389-394).
+blob_field  sum of random anisotropic blobs with the config0 distributions.
+config1  128^3 blob field (64 blobs) + N(0, 0.01^2) noise; 200,000 Gaussians at
+         random distinct voxel centres (R/scene/dataset_readers.py:29-53),
+         identity rotations, weight 0.1, value = the voxel's field value,
+         scales from the 3-NN distance (R/scene/gaussian_model.py:175-188);
+         2^20 uniform samples per step, trilinear ground truth.
+config2  129^3 lattice, spacing 1/128, interior vertices jittered U(+-0.3/128)
+         (boundary vertices stay on the faces so the domain is [0,1]^3), each cube
+         split into 6 Kuhn tetrahedra along its (0,0,0)-(1,1,1) diagonal;
+         vertex scalar = 32-blob field; 1,000,000 Gaussians at random distinct
+         vertices, identity rotations, weight 0.1, value = vertex scalar; 2^22
+         uniform samples per step located in the tet mesh.
+"""
+import numpy as np
+
+LOG_SCALE_MEAN = float(np.log(0.03))
+LOG_SCALE_STD = 0.3
+
+
+def random_gaussians(rng, P):
+    means = rng.uniform(0.0, 1.0, (P, 3)).astype(np.float32)
+    log_scales = rng.normal(LOG_SCALE_MEAN, LOG_SCALE_STD, (P, 3)).astype(np.float32)
+    q = rng.normal(size=(P, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    rots = q.astype(np.float32)
+    weights = rng.uniform(0.0, 1.0, (P, 1)).astype(np.float32)
+    values = rng.uniform(0.0, 1.0, (P, 1)).astype(np.float32)
+    return dict(means=means, log_scales=log_scales, rotations=rots, weights=weights, values=values)
+
+
+def grid_points(n):
+    g = np.linspace(0.0, 1.0, n, dtype=np.float64)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1).astype(np.float32)
+
+
+def config0(seed=0):
+    rng = np.random.default_rng(seed)
+    g = random_gaussians(rng, 2048)
+    a = rng.integers(1, 5, size=(3, 3))
+    pts = grid_points(32)
+    x = pts.astype(np.float64)
+    target = sum(np.sin(2.0 * np.pi * (x @ a[k].astype(np.float64))) for k in range(3))
+    g.update(samples=pts, target=target.astype(np.float32), freqs=a)
+    return g
+
+
+def _rot_np(q):
+    r, x, y, z = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    return np.stack([
+        1 - 2 * (y * y + z * z), 2 * (x * y - r * z), 2 * (x * z + r * y),
+        2 * (x * y + r * z), 1 - 2 * (x * x + z * z), 2 * (y * z - r * x),
+        2 * (x * z - r * y), 2 * (y * z + r * x), 1 - 2 * (x * x + y * y),
+    ], axis=1).reshape(-1, 3, 3)
+
+
+def blobs(rng, n):
+    g = random_gaussians(rng, n)
+    R = _rot_np(g["rotations"].astype(np.float64))
+    s = np.exp(g["log_scales"].astype(np.float64))
+    cov = R @ (s[:, :, None] ** 2 * np.transpose(R, (0, 2, 1)))
+    inv = np.linalg.inv(cov)
+    return dict(means=g["means"].astype(np.float64), inv=inv, w=g["weights"][:, 0].astype(np.float64))
+
+
+def eval_blobs(b, pts, chunk=1 << 18):
+    """sum_b w_b exp(-0.5 (x - mu_b)^T Sigma_b^-1 (x - mu_b)) in float64."""
+    pts = np.asarray(pts, dtype=np.float64)
+    out = np.zeros(pts.shape[0])
+    for s in range(0, pts.shape[0], chunk):
+        x = pts[s:s + chunk]
+        acc = np.zeros(x.shape[0])
+        for k in range(b["means"].shape[0]):
+            d = x - b["means"][k]
+            q = np.einsum("ni,ij,nj->n", d, b["inv"][k], d)
+            acc += b["w"][k] * np.exp(-0.5 * q)
+        out[s:s + chunk] = acc
+    return out
+
+
+def knn3_mean_sq(points):
+    """Mean squared distance to the 3 nearest other points (scipy KD-tree)."""
+    from scipy.spatial import cKDTree
+    t = cKDTree(points.astype(np.float64))
+    d, _ = t.query(points.astype(np.float64), k=4)
+    return (d[:, 1:] ** 2).mean(axis=1)
+
+
+def config1(seed=0, P=200_000, n=128, S=1 << 20):
+    rng = np.random.default_rng(seed)
+    b = blobs(rng, 64)
+    grid = grid_points(n)
+    field = eval_blobs(b, grid) + rng.normal(0.0, 0.01, grid.shape[0])
+    vox = rng.choice(grid.shape[0], size=P, replace=False)
+    means = grid[vox].astype(np.float32)
+    d2 = np.maximum(knn3_mean_sq(means), 1e-7)
+    log_s = np.log(np.sqrt(d2)).astype(np.float32)
+    log_scales = np.repeat(log_s[:, None], 3, axis=1)
+    rots = np.zeros((P, 4), np.float32)
+    rots[:, 0] = 1.0
+    weights = np.full((P, 1), 0.1, np.float32)
+    values = field[vox].astype(np.float32)[:, None]
+    samples = rng.uniform(0.0, 1.0, (S, 3)).astype(np.float32)
+    return dict(means=means, log_scales=log_scales, rotations=rots, weights=weights, values=values,
+                field=field.astype(np.float32).reshape(n, n, n), samples=samples, dims=(n, n, n))
+
+
+def kuhn_tets(n):
+    """Connectivity [6 n^3, 4] of the Kuhn split of an n^3-cube lattice whose
+    (n+1)^3 vertices are numbered i + (n+1) (j + (n+1) k)."""
+    m = n + 1
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    base = (i + m * (j + m * k)).ravel().astype(np.int64)
+    e = [1, m, m * m]
+    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    tets = []
+    for p in perms:
+        a = base
+        b = a + e[p[0]]
+        c = b + e[p[1]]
+        d = base + e[0] + e[1] + e[2]
+        tets.append(np.stack([a, b, c, d], axis=1))
+    return np.stack(tets, axis=1).reshape(-1, 4)
+
+
+def jittered_lattice(rng, n, jitter=0.3):
+    m = n + 1
+    g = np.arange(m, dtype=np.float64) / n
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pts = np.stack([X.transpose(2, 1, 0).ravel(), Y.transpose(2, 1, 0).ravel(), Z.transpose(2, 1, 0).ravel()], 1)
+    # vertex id = i + m (j + m k): x fastest
+    jit = rng.uniform(-jitter / n, jitter / n, pts.shape)
+    on_face = (pts <= 0.0) | (pts >= 1.0)
+    jit[on_face] = 0.0
+    return (pts + jit).astype(np.float32)
+
+
+def config2(seed=0, n=128, P=1_000_000, S=1 << 22, n_blobs=32):
+    rng = np.random.default_rng(seed)
+    pts = jittered_lattice(rng, n)
+    tets = kuhn_tets(n)
+    b = blobs(rng, n_blobs)
+    vals = eval_blobs(b, pts)
+    vsel = rng.choice(pts.shape[0], size=min(P, pts.shape[0]), replace=False)
+    means = pts[vsel]
+    d2 = np.maximum(knn3_mean_sq(means), 1e-7)
+    log_s = np.log(np.sqrt(d2)).astype(np.float32)
+    log_scales = np.repeat(log_s[:, None], 3, axis=1)
+    Pn = means.shape[0]
+    rots = np.zeros((Pn, 4), np.float32)
+    rots[:, 0] = 1.0
+    weights = np.full((Pn, 1), 0.1, np.float32)
+    values = vals[vsel].astype(np.float32)[:, None]
+    samples = rng.uniform(0.0, 1.0, (S, 3)).astype(np.float32)
+    return dict(means=means, log_scales=log_scales, rotations=rots, weights=weights, values=values,
+                mesh_points=pts, tets=tets, mesh_values=vals, samples=samples)

@@ -1,0 +1,170 @@
+/*
+ * gvc.h -- C ABI of libgvc, the MI355X (gfx950) implementation of the
+ * differentiable Gaussian-mixture evaluator, its mesh sampler and its 3-NN
+ * initialiser.
+ *
+ * Conventions for every entry point:
+ *   - plain device pointers and sizes; no torch or HIP types in the signatures;
+ *     `stream` is a hipStream_t passed as void* (NULL = the null stream);
+ *   - nothing allocates device memory and nothing synchronises with the host,
+ *     except the two *_stats / *_count readers that say so;  scratch memory is a
+ *     caller-owned workspace whose size the matching *_bytes query returns;
+ *   - state lives in caller-owned buffers (a "sample set", a "forward state"),
+ *     never in library statics, so calls on different buffers are independent;
+ *   - return value: GVC_OK or a gvc_status code; gvc_last_error() gives the text
+ *     of the last failure on the calling thread.
+ *
+ * Citations: R/ = the reference repository root, DGR/ = R/diff-gaussian-
+ * rasterization/, CR/ = DGR/cuda_rasterizer/, GS/ = R/gpu_sampling/.
+ */
+#ifndef GVC_H_
+#define GVC_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum gvc_status {
+    GVC_OK = 0,
+    GVC_ERR_INVALID_ARGUMENT = 1,
+    GVC_ERR_WORKSPACE_TOO_SMALL = 2,
+    GVC_ERR_HIP = 3,
+    GVC_ERR_UNSUPPORTED = 4,
+    GVC_ERR_CAPACITY = 5
+};
+
+/* gvc_params.flags */
+#define GVC_FLAG_KAHAN 1u      /* compensated per-point sums in the forward       */
+#define GVC_FLAG_FP64_COND 2u  /* FP64 covariance inverse where cond(Sigma) > 1e6 */
+
+/* Launch/tuning parameters of the mixture kernels (chosen by the autotuner). */
+typedef struct gvc_params {
+    int leaf_size;      /* Gaussians per BVH leaf: 1..64                        */
+    int block_threads;  /* threads per block of the traversal kernels: 64..1024 */
+    unsigned flags;     /* GVC_FLAG_*                                           */
+    int reserved;
+} gvc_params;
+
+/* Error flags reported by gvc_forward_stats (bitwise OR). */
+#define GVC_ERRFLAG_STACK_OVERFLOW 1u
+#define GVC_ERRFLAG_RECORD_OVERFLOW 2u
+
+int gvc_version(void);
+const char *gvc_status_string(int status);
+const char *gvc_last_error(void);
+
+/* Per-Gaussian conic and axis-aligned box, the canonical fp32 preprocess.
+ * Replaces preprocessCUDA, CR/forward.cu:10-114 (launched :285).
+ * conics: [P,6] packed xx xy xz yy yz zz (may be NULL); boxes: [P,6] lo.xyz hi.xyz
+ * (may be NULL).  flags: GVC_FLAG_FP64_COND. */
+int gvc_preprocess(int P, const float *means3D, const float *scales, const float *rotations,
+                   const float *weights, float scale_modifier, unsigned flags, float *conics,
+                   float *boxes, void *stream);
+
+/* 63-bit Morton codes of [n,3] points, the reference's morton3d_unit,
+ * R/gaussian_renderer/__init__.py:45-58 (clamp = 0) and the clamped variant of
+ * R/train.py:300-309 (clamp = 1). */
+int gvc_morton3d_unit(int n, const float *xyz, int clamp, uint64_t *codes, void *stream);
+
+/* Stable sort of (u64 key, u32 value) pairs on the low key_bits bits; the sorted
+ * pairs are written to keys_out/vals_out.  Replaces torch.argsort in
+ * R/gaussian_renderer/__init__.py:77 and R/train.py:311 (those are unstable; this
+ * one orders equal keys by input position). */
+size_t gvc_sort_pairs_u64_bytes(int n);
+int gvc_sort_pairs_u64(int n, const uint64_t *keys_in, const uint32_t *vals_in, uint64_t *keys_out,
+                       uint32_t *vals_out, int key_bits, void *workspace, size_t workspace_bytes,
+                       void *stream);
+
+/* Sample set: the points the mixture is evaluated at, spatially sorted into
+ * 64-point groups.  Replaces BuildBVH/buildBoxes, DGR/ext.cu:13-65, whose static
+ * sample BVH becomes this caller-owned buffer. */
+size_t gvc_sample_set_bytes(int S);
+int gvc_sample_set_build(int S, const float *samples, void *sample_set, size_t sample_set_bytes,
+                         void *stream);
+
+/* Forward: (cells, weights) of the normalised mixture at the S samples.
+ * Replaces _C.rasterize_gaussians = RasterizeGaussiansCUDA, DGR/ext.cu:67-105,
+ * DGR/rasterize_points.cu:17-73, CR/rasterizer_impl.cu:27-239 (kernels K3, K4,
+ * K5/K7, K6: CR/forward.cu:10-274).  out_cells/out_weights: [S]; sentinel (-1, 0)
+ * where the total weight is <= 1e-2; P == 0 leaves zeros (as the reference).
+ * conics: optional [P,6] output (the reference's third forward result).
+ * `state` (gvc_forward_state_bytes) keeps the Gaussian BVH and the per-group
+ * record counts for the backward; it must not be reused until the backward ran. */
+size_t gvc_forward_state_bytes(int P, int S, const gvc_params *params);
+int gvc_mixture_forward(int P, const float *means3D, const float *scales, const float *rotations,
+                        const float *values, const float *weights, float scale_modifier,
+                        const void *sample_set, int S, const gvc_params *params, float *out_cells,
+                        float *out_weights, float *conics, void *state, size_t state_bytes,
+                        void *stream);
+
+/* BLOCKING: copies the forward's counters to the host (one small D2H copy and a
+ * stream synchronisation).  n_records = (Gaussian, 64-point group) partials the
+ * backward will emit; n_pairs = accepted (Gaussian, point) pairs; error_flags =
+ * GVC_ERRFLAG_* raised so far on this state.  Any pointer may be NULL. */
+int gvc_forward_stats(const void *state, int P, int S, const gvc_params *params, uint64_t *n_records,
+                      uint64_t *n_pairs, uint32_t *error_flags, void *stream);
+
+/* Backward: gradients for means3D [P,3], scales [P,3], rotations [P,4], values
+ * [P,1], weights [P,1] from dL/dout_cells, dL/dout_cell_weights [S].  Replaces
+ * _C.rasterize_gaussians_backward, DGR/ext.cu:107-147, DGR/rasterize_points.cu:75-137,
+ * CR/rasterizer_impl.cu:243-452 (kernels K9-K11, CR/backward.cu:9-660); the
+ * covariance chain is derived from the forward formula instead of following
+ * CR/backward.cu:68-124.  Reduction is deterministic: per-(Gaussian, group)
+ * partials are sorted by Gaussian and summed in group order, no float atomics.
+ * record_capacity >= n_records of the forward (gvc_forward_stats) or the call
+ * flags GVC_ERRFLAG_RECORD_OVERFLOW and the gradients are incomplete. */
+size_t gvc_backward_bytes(int P, uint64_t record_capacity);
+int gvc_mixture_backward(int P, const float *means3D, const float *scales, const float *rotations,
+                         const float *values, const float *weights, float scale_modifier,
+                         const void *sample_set, int S, const gvc_params *params,
+                         const float *out_cells, const float *out_weights, const float *dL_dout_cells,
+                         const float *dL_dout_cell_weights, void *state, size_t state_bytes,
+                         uint64_t record_capacity, void *workspace, size_t workspace_bytes,
+                         float *dL_dmeans3D, float *dL_dscales, float *dL_drotations,
+                         float *dL_dvalues, float *dL_dweights, void *stream);
+
+/* ---- mesh sampler ------------------------------------------------------------
+ * Tetrahedral meshes: exact point location (orient3d signs with simulation of
+ * simplicity, so points on shared faces/edges/vertices belong to exactly one tet)
+ * and barycentric interpolation of a per-point field.  Replaces sample_meshu over
+ * Viskores Probe, GS/sample_mesh.cu:242-414 (cells of VTK type 10), and returns the
+ * int32 containing-cell id (-1 outside) besides the value (-1.0 outside, :382).
+ * points [n_points,3] f32, tets [n_tets,4] i32 (device).  The locator buffer
+ * (gvc_tet_locator_bytes) holds a uniform grid of candidate lists with room for
+ * entry_capacity (tet, grid cell) entries.
+ * gvc_tet_locator_build is BLOCKING (two small D2H reads); GVC_ERR_CAPACITY with
+ * *entries_needed set means: rebuild with a capacity >= *entries_needed. */
+size_t gvc_tet_locator_bytes(int n_points, int n_tets, uint64_t entry_capacity);
+int gvc_tet_locator_build(int n_points, const float *points, int n_tets, const int32_t *tets,
+                          uint64_t entry_capacity, void *locator, size_t locator_bytes,
+                          uint64_t *entries_needed, void *stream);
+/* cell_ids [S] i32 (required); values_out [S] f64 (optional, needs point_values
+ * [n_points] f64); bary_out [S,4] f32 (optional). */
+int gvc_tet_locate_interp(int n_points, const float *points, int n_tets, const void *locator,
+                          size_t locator_bytes, uint64_t entry_capacity, const double *point_values,
+                          int S, const float *samples, int32_t *cell_ids, double *values_out,
+                          float *bary_out, void *stream);
+
+/* Uniform point grid (dims nx,ny,nz; origin; spacing -- HOST arrays), trilinear
+ * interpolation, -1 outside cast to the value type (all-ones bits for the
+ * unsigned types).  value_dtype: 0 f64, 1 f32, 2 u16, 3 u8; out has that type.
+ * Replaces sample_mesh over Viskores Probe, GS/sample_mesh.cu:26-240. */
+int gvc_grid_sample(const int64_t *dims, const float *origin, const float *spacing, int value_dtype,
+                    const void *values, int S, const float *samples, void *out, void *stream);
+
+/* ---- 3-NN initialiser ----------------------------------------------------------
+ * distCUDA2 of simple-knn (call sites R/scene/gaussian_model.py:175-178, :730-733):
+ * per point the mean squared distance to its 3 nearest other points.  Exact.
+ * BLOCKING (one small D2H read of the bounding box). */
+size_t gvc_knn3_bytes(int n);
+int gvc_knn3(int n, const float *points, float *mean_sq_dist, void *workspace, size_t workspace_bytes,
+             void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GVC_H_ */

@@ -1,0 +1,216 @@
+"""GPU parity of the mixture path (libgvc through its C ABI) against the oracle.
+
+Bars (stated per test): bit-exact for the integer and predicate work (Morton
+codes, conics and boxes, pair membership counts, sentinel decisions); field
+within 1e-5 relative (max |V - V*| / max(|V*|, 1e-3)) of the FP64 oracle;
+gradients within 1e-4 of the FP64 oracle, measured against the largest
+gradient magnitude of each parameter tensor.
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+import torch  # noqa: E402
+
+import oracle  # noqa: E402
+from gvc import mixture as mx  # noqa: E402
+from gvc import workloads  # noqa: E402
+
+f32 = np.float32
+DEV = "cuda:0"
+
+
+def T(a, dtype=torch.float32):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(DEV, dtype)
+
+
+def field_err(V, W, Vr, Wr):
+    valid = Wr > 0
+    ev = np.abs(V[valid] - Vr[valid]) / np.maximum(np.abs(Vr[valid]), 1e-3)
+    ew = np.abs(W[valid] - Wr[valid]) / np.maximum(np.abs(Wr[valid]), 1e-3)
+    return max(ev.max(initial=0), ew.max(initial=0))
+
+
+def edge_gaussians(seed=0, P=4000):
+    rng = np.random.default_rng(seed)
+    g = workloads.random_gaussians(rng, P)
+    scales = np.exp(g["log_scales"]).astype(f32)
+    w = g["weights"].copy()
+    w[:20, 0] = [0.0, 1e-4, 9.99e-4, 1e-3, 1.0001e-3, 1.0, 2.0, 0.011, 0.0105, 1e-2,
+                 0.5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5]
+    scales[10] = 0.0                       # degenerate: zero covariance
+    scales[11] = [0.03, 1e-5, 0.02]        # cond ~ 1e7
+    scales[12] = [1e-6, 1e-6, 1e-6]
+    rots = g["rotations"].copy()
+    rots[13] = [0, 0, 0, 0]                # zero quaternion
+    rots[14] = [2.0, 0.5, -1.0, 0.1]       # unnormalised
+    means = g["means"].copy()
+    means[15] = [-0.2, 1.3, 0.5]           # outside the unit cube
+    means[16] = [np.nan, 0.5, 0.5]
+    scales[17] = [np.inf, 0.01, 0.01]
+    return means, scales, rots, g["values"], w
+
+
+@pytest.mark.parametrize("fp64_cond", [False, True])
+def test_preprocess_bit_exact(fp64_cond):
+    means, scales, rots, values, w = edge_gaussians(1)
+    c_ref, b_ref, _ = oracle.preprocess(means, scales, rots, w, 1.0, fp64_cond)
+    c, b = mx.preprocess(T(means), T(scales), T(rots), T(w), 1.0, fp64_cond)
+    c, b = c.cpu().numpy(), b.cpu().numpy()
+    # bit-exact (NaN positions must agree too)
+    assert np.array_equal(c.view(np.uint32), c_ref.view(np.uint32)) or np.array_equal(c, c_ref, equal_nan=True)
+    assert np.array_equal(b, b_ref, equal_nan=True)
+    c2, b2 = mx.preprocess(T(means), T(scales), T(rots), T(w), 0.7, fp64_cond)
+    c2r, b2r, _ = oracle.preprocess(means, scales, rots, w, 0.7, fp64_cond)
+    assert np.array_equal(c2.cpu().numpy(), c2r, equal_nan=True)
+    assert np.array_equal(b2.cpu().numpy(), b2r, equal_nan=True)
+
+
+def test_morton_bit_exact():
+    rng = np.random.default_rng(2)
+    pts = np.concatenate([rng.uniform(-0.1, 1.1, (100000, 3)),
+                          [[0, 0, 0], [1, 1, 1], [np.nan, 0.5, 2.0], [1e30, -1e30, 0.5],
+                           [np.nextafter(1, 0), 0.5, 0.25]]]).astype(f32)
+    for clamp in (False, True):
+        ref = oracle.morton3d_unit(pts, clamp)
+        got = mx.morton3d_unit(T(pts), clamp).cpu().numpy().view(np.uint64)
+        assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("n,bits", [(1, 64), (1000, 64), (70001, 64), (300000, 21), (5000, 8)])
+def test_sort_pairs_stable(n, bits):
+    rng = np.random.default_rng(n)
+    if bits == 64:
+        keys = rng.integers(0, 2**63 - 1, n, dtype=np.int64)
+        keys[: n // 3] = keys[0]  # many equal keys: stability matters
+    else:
+        keys = rng.integers(0, 1 << bits, n, dtype=np.int64)
+    vals = np.arange(n, dtype=np.int32)
+    ko, vo = mx.sort_pairs_u64(T(keys, torch.int64), T(vals, torch.int32), bits)
+    order = np.argsort(keys.view(np.uint64) & np.uint64((1 << bits) - 1 if bits < 64 else 2**64 - 1),
+                       kind="stable")
+    assert np.array_equal(vo.cpu().numpy(), vals[order])
+    assert np.array_equal(ko.cpu().numpy(), keys[order])
+
+
+def run_gpu(means, scales, rots, values, w, samples, params, k=1.0):
+    ss = mx.SampleSet(T(samples))
+    cells, cw, conics, st = mx.mixture_forward(T(means), T(scales), T(rots), T(values), T(w), k, ss, params)
+    return ss, cells, cw, conics, st
+
+
+@pytest.mark.parametrize("leaf", [1, 2, 4, 8, 16])
+def test_forward_config0_parity(leaf):
+    g = workloads.config0(0)
+    scales = np.exp(g["log_scales"]).astype(f32)
+    params = mx.MixtureParams(leaf_size=leaf, block_threads=256)
+    ss, cells, cw, conics, st = run_gpu(g["means"], scales, g["rotations"], g["values"], g["weights"],
+                                        g["samples"], params)
+    V, W = cells.cpu().numpy(), cw.cpu().numpy()
+    conic_ref, box_ref, _ = oracle.preprocess(g["means"], scales, g["rotations"], g["weights"])
+    assert np.array_equal(conics.cpu().numpy().reshape(-1, 6), conic_ref)
+    Vr, Wr, cnt = oracle.forward(g["means"], g["values"], g["weights"], conic_ref, box_ref, g["samples"])
+    nrec, npairs, err = st.stats()
+    assert err == 0
+    assert npairs == int(cnt.sum())  # exact pair membership
+    assert np.array_equal(W == 0, Wr == 0)  # sentinel agreement
+    assert np.all(V[W == 0] == -1.0)
+    assert field_err(V, W, Vr, Wr) <= 1e-5
+
+
+def test_forward_edge_cases():
+    means, scales, rots, values, w = edge_gaussians(3, P=3000)
+    rng = np.random.default_rng(4)
+    samples = np.concatenate([rng.uniform(-0.05, 1.05, (10007, 3)), means[:40] ,
+                              [[np.nan, 0.5, 0.5], [0.5, 0.5, 0.5]]]).astype(f32)
+    samples[-2] = [np.nan, 0.5, 0.5]
+    for leaf in (1, 3, 16):
+        params = mx.MixtureParams(leaf_size=leaf)
+        ss, cells, cw, conics, st = run_gpu(means, scales, rots, values, w, samples, params)
+        conic_ref, box_ref, _ = oracle.preprocess(means, scales, rots, w)
+        Vr, Wr, cnt = oracle.forward(means, values, w, conic_ref, box_ref, samples)
+        V, W = cells.cpu().numpy(), cw.cpu().numpy()
+        nrec, npairs, err = st.stats()
+        assert npairs == int(cnt.sum())
+        finite = np.isfinite(Vr) & np.isfinite(V)
+        assert np.array_equal(np.isfinite(V), np.isfinite(Vr))
+        assert np.array_equal((W == 0)[finite], (Wr == 0)[finite])
+        assert field_err(V[finite], W[finite], Vr[finite], Wr[finite]) <= 1e-5
+
+
+def test_empty_and_tiny():
+    rng = np.random.default_rng(5)
+    samples = rng.uniform(0, 1, (100, 3)).astype(f32)
+    z3 = np.zeros((0, 3), f32)
+    ss = mx.SampleSet(T(samples))
+    cells, cw, conics, st = mx.mixture_forward(T(z3), T(z3), T(np.zeros((0, 4), f32)), T(np.zeros((0, 1), f32)),
+                                               T(np.zeros((0, 1), f32)), 1.0, ss)
+    assert torch.all(cells == 0) and torch.all(cw == 0)  # P == 0 leaves zeros (reference)
+    # one Gaussian, samples not a multiple of 64, no samples
+    m = np.array([[0.5, 0.5, 0.5]], f32)
+    s = np.array([[0.2, 0.2, 0.2]], f32)
+    q = np.array([[1, 0, 0, 0]], f32)
+    v = np.array([[0.25]], f32)
+    w = np.array([[1.0]], f32)
+    ss, cells, cw, _, st = run_gpu(m, s, q, v, w, samples, mx.MixtureParams())
+    cr, br, _ = oracle.preprocess(m, s, q, w)
+    Vr, Wr, _ = oracle.forward(m, v, w, cr, br, samples)
+    assert field_err(cells.cpu().numpy(), cw.cpu().numpy(), Vr, Wr) <= 1e-5
+    ss0 = mx.SampleSet(T(np.zeros((0, 3), f32)))
+    c0, w0, _, _ = mx.mixture_forward(T(m), T(s), T(q), T(v), T(w), 1.0, ss0)
+    assert c0.numel() == 0
+
+
+def _grad_check(g, scales, samples, params, tol=1e-4, seed=0):
+    rng = np.random.default_rng(seed)
+    S = samples.shape[0]
+    gV = rng.normal(size=S).astype(f32)
+    gW = rng.normal(size=S).astype(f32)
+    ss, cells, cw, conics, st = run_gpu(g["means"], scales, g["rotations"], g["values"], g["weights"], samples,
+                                        params)
+    grads = mx.mixture_backward(st, T(g["means"]), T(scales), T(g["rotations"]), T(g["values"]), T(g["weights"]),
+                                1.0, ss, cells, cw, T(gV), T(gW))
+    conic_ref, box_ref, _ = oracle.preprocess(g["means"], scales, g["rotations"], g["weights"])
+    cd = oracle.conic_fp64(scales, g["rotations"])
+    Vr, Wr, _ = oracle.forward(g["means"], g["values"], g["weights"], conic_ref, box_ref, samples, cd)
+    ref = oracle.backward(g["means"], scales, g["rotations"], g["values"], g["weights"], conic_ref, box_ref,
+                          samples, Vr, Wr, gV.astype(np.float64), gW.astype(np.float64))
+    names = ["means", "scales", "rotations", "values", "weights"]
+    errs = {}
+    for n, t in zip(names, grads):
+        a = t.cpu().numpy().reshape(-1).astype(np.float64)
+        b = ref[n].reshape(-1)
+        errs[n] = np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+    for n in names:
+        assert errs[n] <= tol, errs
+    return grads
+
+
+@pytest.mark.parametrize("leaf", [1, 4, 16])
+def test_backward_config0_parity(leaf):
+    g = workloads.config0(0)
+    scales = np.exp(g["log_scales"]).astype(f32)
+    # subsample points so the FP64 brute-force oracle stays fast
+    samples = g["samples"][::4].copy()
+    _grad_check(g, scales, samples, mx.MixtureParams(leaf_size=leaf))
+
+
+def test_backward_deterministic():
+    g = workloads.config0(1)
+    scales = np.exp(g["log_scales"]).astype(f32)
+    samples = g["samples"]
+    rng = np.random.default_rng(3)
+    gV = T(rng.normal(size=samples.shape[0]))
+    gW = T(rng.normal(size=samples.shape[0]))
+    outs = []
+    for _ in range(2):
+        ss, cells, cw, conics, st = run_gpu(g["means"], scales, g["rotations"], g["values"], g["weights"], samples,
+                                            mx.MixtureParams())
+        grads = mx.mixture_backward(st, T(g["means"]), T(scales), T(g["rotations"]), T(g["values"]),
+                                    T(g["weights"]), 1.0, ss, cells, cw, gV, gW)
+        outs.append([cells.cpu().numpy(), cw.cpu().numpy()] + [t.cpu().numpy() for t in grads])
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

@@ -1,0 +1,96 @@
+"""Pins the mesh / 3-NN oracle (oracle/mesh_ref.py) on the CPU.
+
+Tet location: by construction (a point drawn inside tet t with barycentrics b
+lies in t and interpolates to sum b v -- the construction of R/train.py:34-61),
+and the partition property of the tie rule (every point of a conforming mesh's
+interior lies in exactly one tet, also on shared faces, edges and vertices).
+Grid probe: trilinear interpolation reproduces affine functions exactly.
+3-NN: against scipy's KD-tree.
+"""
+import numpy as np
+import pytest
+
+from oracle import mesh_ref
+from gvc import workloads
+
+f32 = np.float32
+
+
+def small_mesh(n=3, jitter=0.3, seed=0):
+    rng = np.random.default_rng(seed)
+    pts = workloads.jittered_lattice(rng, n, jitter)
+    tets = workloads.kuhn_tets(n)
+    return pts, tets
+
+
+def test_by_construction_and_values():
+    pts, tets = small_mesh(3, 0.3, 1)
+    rng = np.random.default_rng(2)
+    vals = rng.normal(size=pts.shape[0])
+    t = rng.integers(0, tets.shape[0], 300)
+    u = np.sort(rng.uniform(0.02, 0.98, (300, 3)), axis=1)
+    b = np.stack([u[:, 0], u[:, 1] - u[:, 0], u[:, 2] - u[:, 1], 1 - u[:, 2]], 1)
+    keep = b.min(1) > 0.01
+    t, b = t[keep], b[keep]
+    samples = (b[:, :, None] * pts[tets[t]].astype(np.float64)).sum(1).astype(f32)
+    ids, out = mesh_ref.locate(pts, tets, samples, vals)
+    assert np.array_equal(ids, t)
+    expect = (b * vals[tets[t]]).sum(1)
+    np.testing.assert_allclose(out, expect, rtol=1e-5, atol=1e-5)
+
+
+def test_tie_rule_partitions_the_domain():
+    pts, tets = small_mesh(2, 0.0, 0)  # exact lattice: many points on shared faces
+    g = np.linspace(0.0, 1.0, 9, dtype=f32)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    samples = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    # interior points only: exactly one tet must contain each (SoS perturbation)
+    interior = np.all((samples > 0) & (samples < 1), axis=1)
+    P = pts
+    for p in samples[interior][::3]:
+        n_in = sum(mesh_ref.tet_contains(P, tets[t], p)[0] for t in range(tets.shape[0]))
+        assert n_in == 1, p
+
+
+def test_outside_points():
+    pts, tets = small_mesh(2, 0.2, 3)
+    samples = np.array([[-0.1, 0.5, 0.5], [0.5, 1.2, 0.5], [np.nan, 0.5, 0.5]], f32)
+    ids, out = mesh_ref.locate(pts, tets, samples, np.zeros(pts.shape[0]))
+    assert np.all(ids == -1) and np.all(out == -1.0)
+
+
+def test_grid_probe_reproduces_affine():
+    dims = (5, 7, 4)
+    origin = (0.1, -0.2, 0.3)
+    spacing = (0.25, 0.125, 0.5)
+    ii, jj, kk = np.meshgrid(*[np.arange(d) for d in dims], indexing="ij")
+    x = origin[0] + ii * spacing[0]
+    y = origin[1] + jj * spacing[1]
+    z = origin[2] + kk * spacing[2]
+    f = 2.0 * x - 3.0 * y + 0.5 * z + 1.0
+    values = np.transpose(f, (2, 1, 0)).ravel()  # x fastest
+    rng = np.random.default_rng(4)
+    samples = np.stack([rng.uniform(origin[a] - 0.1, origin[a] + (dims[a] - 1) * spacing[a] + 0.1, 2000)
+                        for a in range(3)], 1).astype(f32)
+    out = mesh_ref.grid_sample(dims, origin, spacing, values, samples)
+    s = samples.astype(np.float64)
+    inside = np.all([(samples[:, a] >= f32(origin[a])) &
+                     (((samples[:, a] - f32(origin[a])) / f32(spacing[a])).astype(f32) <= dims[a] - 1)
+                     for a in range(3)], axis=0)
+    expect = 2.0 * s[:, 0] - 3.0 * s[:, 1] + 0.5 * s[:, 2] + 1.0
+    np.testing.assert_allclose(out[inside], expect[inside], rtol=0, atol=2e-6)
+    assert np.all(out[~inside] == -1)
+    u8 = mesh_ref.grid_sample(dims, origin, spacing, (values * 10).astype(np.uint8), samples)
+    assert np.all(u8[~inside] == 255)
+
+
+def test_knn3_against_kdtree():
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(5)
+    P = rng.uniform(0, 1, (1500, 3)).astype(f32)
+    P[10] = P[11]  # a duplicate pair: distance 0
+    got = mesh_ref.knn3(P)
+    d, _ = cKDTree(P.astype(np.float64)).query(P.astype(np.float64), k=4)
+    expect = (d[:, 1:] ** 2).mean(1)
+    np.testing.assert_allclose(got, expect, rtol=1e-5, atol=1e-9)
+    assert got[10] == got[10] and got[10] < expect.mean()
