@@ -1,0 +1,262 @@
+#!/usr/bin/env python3
+"""Benchmark: one training step of the unstructured-mesh fit on one MI355X.
+
+Workload (default ``--config mesh``, the 1M-Gaussian configuration the metric is
+quoted on): a 129^3 jittered lattice split into 6 Kuhn tetrahedra per cube
+(12,582,912 tets) carrying a 32-blob scalar field; 1,000,000 Gaussians at random
+distinct mesh vertices, initialised like the reference (3-NN scales via
+distCUDA2, identity rotations, weight 0.1, value = vertex value).  Every step:
+
+  1. 2^22 uniform sample points (torch.rand on the device);
+  2. exact tet location + barycentric interpolation of the ground truth (libgvc);
+  3. the sample set (Morton sort into 64-point groups);
+  4. activations and the differentiable mixture forward (libgvc, drop-in
+     bvh_diff_gaussian_rasterization API);
+  5. masked L1 + false-negative term (R/train.py:389-420), backward (libgvc
+     adjoint, deterministic reduction);
+  6. Adam (lr 1e-3).
+
+value = Gaussian-point evaluations per second, forward + backward
+(2 x accepted pairs per step, summed over the timed steps / wall time).
+The roofline object is the dominant kernel (the larger of the forward and
+backward walks): achieved = 44 B x accepted pairs / its mean duration, measured
+with HIP events on the launch stream inside the timed steps.  cpu_baseline is
+the plain fp32 reference formula (brute force, box predicate) from the oracle,
+timed on the host cores on a bounded subset of the same step's points.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "gaussian-volume-compression_amd")
+for _p in (PKG, ROOT):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_PAIR = 44  # mean 12 + conic 24 + weight 4 + value 4
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="mesh", choices=["mesh", "grid"])
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--leaf", type=int, default=None)
+    ap.add_argument("--block", type=int, default=None)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="capture the training step in a hipGraph")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "bench.py measures one GPU; --gpus above 1 is not supported in this build "
+                                   "(the sharding design is described in DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from bvh_diff_gaussian_rasterization import GaussianRasterizationSettings, GaussianRasterizer
+    from gvc import mixture as mx
+    from gvc import workloads
+    from gvc.sampling import TetMeshSampler
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    t_setup = time.perf_counter()
+    if args.config == "mesh":
+        W = workloads.mesh_workload(dev, seed=args.seed)
+        S = 1 << 22
+        sampler = TetMeshSampler(W["mesh_points"], W["tets"], W["mesh_values"], device=dev)
+        workload_name = ("unstructured mesh fit: 1,000,000 Gaussians, 2^22 mesh-sampled points/step, "
+                         "129^3 jittered lattice, 12,582,912 tets")
+    else:
+        W = workloads.grid_workload(dev, seed=args.seed)
+        S = 1 << 20
+        sampler = None
+        workload_name = "structured 128^3 grid fit: 200,000 Gaussians, 2^20 points/step"
+    log(f"setup {time.perf_counter() - t_setup:.1f}s P={W['P']} S={S}")
+    P = W["P"]
+
+    params = mx.default_params()
+    if args.leaf or args.block:
+        params = mx.MixtureParams(leaf_size=args.leaf or params.leaf_size,
+                                  block_threads=args.block or params.block_threads)
+        mx.set_default_params(params)
+
+    xyz = torch.nn.Parameter(W["means"].clone())
+    log_s = torch.nn.Parameter(W["log_scales"].clone())
+    rot = torch.nn.Parameter(W["rotations"].clone())
+    wl = torch.nn.Parameter(torch.logit(W["weights"]).clone())
+    vl = torch.nn.Parameter(torch.logit(W["values"].clamp(1e-4, 1 - 1e-4)).clone())
+    opt = torch.optim.Adam([xyz, log_s, rot, wl, vl], lr=1e-3, eps=1e-15)
+    settings = GaussianRasterizationSettings(volume_mins=(0, 0, 0), volume_maxes=(1, 1, 1), cell_count=128, bg=-1.0,
+                                             scale_modifier=1.0, use_gaussian_bvh=False, debug=False)
+    rast = GaussianRasterizer(settings)
+    fn_reg = 1.0
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed + 1)
+    stage_ev = []
+
+    def step():
+        samples = torch.rand((S, 3), device=dev, generator=gen)
+        if sampler is not None:
+            e0 = torch.cuda.Event(enable_timing=True); e0.record()
+            gt64, _ = sampler.sample(samples)
+            gt = gt64.float()
+            e1 = torch.cuda.Event(enable_timing=True); e1.record()
+            stage_ev.append(("tet_locate", e0, e1))
+        else:
+            gt = W["grid_fn"](samples)
+        e0 = torch.cuda.Event(enable_timing=True); e0.record()
+        rast.build_bvh(samples)
+        e1 = torch.cuda.Event(enable_timing=True); e1.record()
+        stage_ev.append(("sample_set", e0, e1))
+        scales = torch.exp(log_s)
+        rots = torch.nn.functional.normalize(rot)
+        w = torch.sigmoid(wl)
+        v = torch.sigmoid(vl)
+        cells, cw = rast(xyz, scales, rots, v, w)
+        valid = ((gt != -1) & (cells != -1)).float()
+        l1 = ((cells - gt).abs() * valid).sum() / valid.sum().clamp_min(1.0)
+        fn_mask = ((gt != -1) & (cw > 0) & (cw < 0.011)).float()
+        fn_vals = (0.011 - cw).clamp_min(0) * fn_mask
+        fn = fn_reg * fn_vals.sum() / ((fn_vals > 0).float().sum() + 1e-8)
+        loss = l1 + fn
+        loss.backward()
+        opt.step()
+        opt.zero_grad(set_to_none=True)
+        return samples, cells, cw
+
+    # warmup: also sizes the backward's record capacity (no host reads later)
+    mx.set_record_capacity(None)
+    audit = []
+    mx.set_audit(audit)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    max_rec = max(a.stats()[0] for a in audit) if audit else 0
+    mx.set_record_capacity(int(max_rec * 1.25) + 4096)
+    audit.clear()
+    stage_ev.clear()
+    timer = []
+    mx.set_stage_timer(timer)
+    log(f"warmup done, record capacity {int(max_rec * 1.25) + 4096}")
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        last = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    mx.set_stage_timer(None)
+    mx.set_audit(None)
+    wall = t1 - t0
+
+    stats = [a.stats() for a in audit]
+    pairs = [s[1] for s in stats]
+    errs = [s[2] for s in stats]
+    if any(errs):
+        raise RuntimeError(f"mixture error flags during the timed steps: {errs}")
+    total_pairs = sum(pairs)
+    stage_ms = {}
+    for name, a, b in timer + stage_ev:
+        stage_ms.setdefault(name, []).append(a.elapsed_time(b))
+    stage_avg = {k: float(np.mean(v)) for k, v in stage_ms.items()}
+    pairs_per_step = total_pairs / args.steps
+    value = 2.0 * total_pairs / wall
+    dom = max(("fwd_walk", "bwd_walk"), key=lambda k: stage_avg.get(k, 0.0))
+    dom_s = stage_avg[dom] / 1e3
+    achieved = BYTES_PER_PAIR * pairs_per_step / dom_s / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get(dom)
+        except Exception:
+            traffic = None
+
+    out = {
+        "metric": "Gaussian-point evals/s fwd+bwd @1M Gaussians",
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": workload_name, "gaussians": P, "points_per_step": S,
+                   "leaf_size": params.leaf_size, "block_threads": params.block_threads,
+                   "kahan": params.kahan, "parallelism": "single GPU"},
+        "pairs_per_step": pairs_per_step,
+        "stage_ms": stage_avg,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_launch": BYTES_PER_PAIR * pairs_per_step, "mean_ms": stage_avg[dom]},
+    }
+
+    if not args.no_error or not args.no_cpu_baseline:
+        import oracle
+        samples, cells, cw = last
+        with torch.no_grad():
+            m = xyz.detach()
+            sc = torch.exp(log_s).detach()
+            rq = torch.nn.functional.normalize(rot).detach()
+            wv = torch.sigmoid(wl).detach()
+            vv = torch.sigmoid(vl).detach()
+            # the last step's forward ran before its Adam update: re-evaluate at
+            # the current parameters for the error comparison
+            ss = mx.SampleSet(samples)
+            c2, w2, _, _ = mx.mixture_forward(m, sc, rq, vv, wv, 1.0, ss)
+        mh, sh, rh, wh, vh = (t.cpu().numpy() for t in (m, sc, rq, wv, vv))
+        conic, box, _ = oracle.preprocess(mh, sh, rh, wh)
+        sub = np.random.default_rng(0).choice(S, 1024, replace=False)
+        pts = samples.cpu().numpy()[sub]
+        if not args.no_error:
+            Vr, Wr, _ = oracle.forward(mh, vh, wh, conic, box, pts)
+            V, Wg = c2.cpu().numpy()[sub], w2.cpu().numpy()[sub]
+            ok = Wr > 0
+            flips = int(np.sum((Wg > 0) != ok))
+            err = max(float(np.max(np.abs(V[ok] - Vr[ok]) / np.maximum(np.abs(Vr[ok]), 1e-3), initial=0)),
+                      float(np.max(np.abs(Wg[ok] - Wr[ok]) / np.maximum(np.abs(Wr[ok]), 1e-3), initial=0)))
+            out["rel_err_vs_fp64"] = {"field_max_rel": err, "sentinel_flips": flips, "points": int(len(sub))}
+        if not args.no_cpu_baseline:
+            rng = np.random.default_rng(1)
+            allpts = samples.cpu().numpy()
+
+            def run_cpu(n):
+                t2 = time.perf_counter()
+                _, _, _, pr = oracle.cpu_plain_fwdbwd(mh, vh, wh, conic, box, allpts[:n], rng.normal(size=n),
+                                                     rng.normal(size=n))
+                return time.perf_counter() - t2, pr
+
+            n = 256
+            dt, pr = run_cpu(n)
+            n = int(min(S, max(256, n * args.cpu_seconds / max(dt, 1e-3))))
+            dt, pr = run_cpu(n)
+            out["cpu_baseline"] = {"value": 2.0 * pr / dt, "unit": "evals/s", "cores": oracle.num_threads(),
+                                   "kind": "port",
+                                   "sample": f"{n} of the step's {S} points against all {P} Gaussians, plain fp32 "
+                                             f"brute force fwd+bwd with the box predicate ({dt:.1f} s)"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

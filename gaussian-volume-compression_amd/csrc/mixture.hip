@@ -33,7 +33,6 @@ namespace gvc {
 
 namespace {
 
-constexpr int kStackDepth = 128;
 constexpr float kLog2e = 1.4426950408889634f;
 
 // ---------------------------------------------------------------------------
@@ -307,7 +306,18 @@ __global__ void refit_kernel(const GRecord* __restrict__ rec, int P, int NL, int
 }
 
 // ---------------------------------------------------------------------------
-// forward walk
+// group walk: shared by the forward and the backward
+//
+// One wavefront owns one 64-point group.  Candidate Gaussians (those whose box
+// overlaps the group's box) are enumerated in a fixed order:
+//   * wide traversal: each step pops up to 64 tree nodes from a wave-private LDS
+//     stack, one per lane (independent vector loads), tests both child boxes and
+//     compacts the overlapping internal children back onto the stack and the
+//     overlapping leaves into a leaf list (ballot + mbcnt, lane order);
+//   * leaf expansion: the leaves' Gaussians are box-tested 64 at a time and
+//     appended to a candidate ring; every 64 candidates are handed to `chunk`.
+// If the wide stack has no room for a full step, one node is taken off it and
+// its subtree is walked depth first with a second, depth-bounded stack.
 // ---------------------------------------------------------------------------
 struct GroupBox {
     float lx, ly, lz, hx, hy, hz;
@@ -321,49 +331,167 @@ __device__ __forceinline__ GroupBox group_box(float px, float py, float pz)
     return b;
 }
 
-__device__ __forceinline__ bool node_overlaps(const float lo[3], const float hi[3], const GroupBox& b)
+__device__ __forceinline__ bool box_overlaps(float lx, float ly, float lz, float hx, float hy, float hz,
+                                             const GroupBox& b)
 {
-    return boxes_overlap(lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], b.lx, b.ly, b.lz, b.hx, b.hy, b.hz);
+    return boxes_overlap(lx, ly, lz, hx, hy, hz, b.lx, b.ly, b.lz, b.hx, b.hy, b.hz);
 }
 
-// Visits, in a fixed depth-first order, every leaf whose box overlaps the group
-// box and calls leaf_fn(leaf).  Returns false on stack overflow.
-template <class LeafFn>
-__device__ __forceinline__ bool walk_tree(const BNode* __restrict__ nodes, const float* __restrict__ root_box,
-                                          int NL, const GroupBox& gb, int* stack, LeafFn&& leaf_fn)
+constexpr int kWideStack = 256;
+constexpr int kSeqStack = 128;
+constexpr int kRing = 128;
+
+struct WalkLds {
+    int wstack[kWideStack];
+    int sstack[kSeqStack];
+    int leaves[128];
+    int ring[kRing];
+};
+
+__device__ __forceinline__ uint64_t lanemask_lt()
 {
+    const int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+__device__ __forceinline__ void lds_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Candidate enumeration.  chunk(base, n) is called with the ring positions
+// [base, base+n) (n <= 64, modulo kRing) holding Gaussian indices.
+template <class ChunkFn>
+__device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__ rec, const BNode* __restrict__ nodes,
+                                                     const float* __restrict__ root_box, int P, int NL, int L,
+                                                     const GroupBox& gb, WalkLds& s, ChunkFn&& chunk)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lanemask_lt();
+    uint32_t chead = 0, ctail = 0;
+    bool ok = true;
+
+    // expand s.leaves[0..nl) into box-overlapping Gaussians
+    auto expand = [&](int nl) {
+        const int total = nl * L;
+        for (int base = 0; base < total; base += 64) {
+            const int t = base + lane;
+            bool hit = false;
+            int g = 0;
+            if (t < total) {
+                const int leaf = s.leaves[t / L];
+                g = leaf * L + (t - (t / L) * L);
+                if (g < P) {
+                    const float4 b0 = *reinterpret_cast<const float4*>(&rec[g].lox);
+                    const float2 b1 = *reinterpret_cast<const float2*>(&rec[g].hiy);
+                    hit = box_overlaps(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, gb);
+                }
+            }
+            const uint64_t bal = __ballot(hit);
+            if (hit) s.ring[(ctail + (uint32_t)__popcll(bal & lt)) & (kRing - 1)] = g;
+            ctail += (uint32_t)__popcll(bal);
+            lds_fence();
+            while (ctail - chead >= 64u) {
+                chunk(chead, 64);
+                chead += 64;
+            }
+        }
+    };
+
     if (NL <= 0) return true;
     if (NL == 1) {
-        float lo[3] = { root_box[0], root_box[1], root_box[2] };
-        float hi[3] = { root_box[3], root_box[4], root_box[5] };
-        if (node_overlaps(lo, hi, gb)) leaf_fn(0);
-        return true;
-    }
-    int sp = 0;
-    int node = 0;
-    while (true) {
-        const BNode nd = nodes[node];
-        const bool ovl = node_overlaps(nd.llo, nd.lhi, gb);
-        const bool ovr = node_overlaps(nd.rlo, nd.rhi, gb);
-        const int cl = nd.left, cr = nd.right;
-        if (ovl && cl < 0) leaf_fn(~cl);
-        if (ovr && cr < 0) leaf_fn(~cr);
-        const bool dl = ovl && cl >= 0, dr = ovr && cr >= 0;
-        if (dl) {
-            if (dr) {
-                if (sp >= kStackDepth) return false;
-                stack[sp++] = cr;
+        if (box_overlaps(root_box[0], root_box[1], root_box[2], root_box[3], root_box[4], root_box[5], gb)) {
+            if (lane == 0) s.leaves[0] = 0;
+            lds_fence();
+            expand(1);
+        }
+    } else {
+        int sp = 1;
+        if (lane == 0) s.wstack[0] = 0;
+        lds_fence();
+        while (sp > 0) {
+            int k = min(64, min(sp, kWideStack - sp));
+            if (k == 0) {
+                // no room for a wide step: walk the top subtree depth first
+                const int root = uniformi(s.wstack[sp - 1]);
+                sp -= 1;
+                int ssp = 0;
+                int node = root;
+                while (true) {
+                    const BNode nd = nodes[node];
+                    const bool ovl = box_overlaps(nd.llo[0], nd.llo[1], nd.llo[2], nd.lhi[0], nd.lhi[1], nd.lhi[2], gb);
+                    const bool ovr = box_overlaps(nd.rlo[0], nd.rlo[1], nd.rlo[2], nd.rhi[0], nd.rhi[1], nd.rhi[2], gb);
+                    const int cl = nd.left, cr = nd.right;
+                    int nl = 0;
+                    if (ovl && cl < 0) { if (lane == 0) s.leaves[nl] = ~cl; ++nl; }
+                    if (ovr && cr < 0) { if (lane == 0) s.leaves[nl] = ~cr; ++nl; }
+                    if (nl) { lds_fence(); expand(nl); }
+                    const bool dl = ovl && cl >= 0, dr = ovr && cr >= 0;
+                    if (dl) {
+                        if (dr) {
+                            if (ssp >= kSeqStack) { ok = false; break; }
+                            if (lane == 0) s.sstack[ssp] = cr;
+                            ++ssp;
+                            lds_fence();
+                        }
+                        node = cl;
+                    } else if (dr) {
+                        node = cr;
+                    } else {
+                        if (ssp == 0) break;
+                        node = uniformi(s.sstack[--ssp]);
+                    }
+                }
+                if (!ok) break;
+                continue;
             }
-            node = cl;
-        } else if (dr) {
-            node = cr;
-        } else {
-            if (sp == 0) break;
-            node = uniformi(stack[--sp]);
+            const int base = sp - k;
+            int node = -1;
+            if (lane < k) node = s.wstack[base + lane];
+            sp = base;
+            bool ovl = false, ovr = false;
+            int cl = 0, cr = 0;
+            if (lane < k) {
+                const float4* q = reinterpret_cast<const float4*>(&nodes[node]);
+                const float4 a = q[0], b = q[1], c = q[2], d = q[3];
+                // a = llo.xyz lhi.x | b = lhi.yz rlo.xy | c = rlo.z rhi.xyz | d = left right . .
+                ovl = box_overlaps(a.x, a.y, a.z, a.w, b.x, b.y, gb);
+                ovr = box_overlaps(b.z, b.w, c.x, c.y, c.z, c.w, gb);
+                cl = __builtin_bit_cast(int, d.x);
+                cr = __builtin_bit_cast(int, d.y);
+            }
+            const bool pl = ovl && cl >= 0, pr = ovr && cr >= 0;
+            const bool ll = ovl && cl < 0, lr = ovr && cr < 0;
+            const uint64_t bpl = __ballot(pl), bpr = __ballot(pr), bll = __ballot(ll), blr = __ballot(lr);
+            const int npl = __popcll(bpl), npr = __popcll(bpr), nll = __popcll(bll), nlr = __popcll(blr);
+            if (pl) s.wstack[sp + __popcll(bpl & lt)] = cl;
+            if (pr) s.wstack[sp + npl + __popcll(bpr & lt)] = cr;
+            if (ll) s.leaves[__popcll(bll & lt)] = ~cl;
+            if (lr) s.leaves[nll + __popcll(blr & lt)] = ~cr;
+            sp += npl + npr;
+            lds_fence();
+            if (nll + nlr) expand(nll + nlr);
         }
     }
-    return true;
+    if (ctail != chead) chunk(chead, (int)(ctail - chead));
+    return ok;
 }
+
+// ---------------------------------------------------------------------------
+// forward: lane = point.  Per chunk of <= 64 candidates: their records go to
+// LDS; lane j box-tests candidate j against the 64 points (the compare masks are
+// the ballots) giving, per point, the bitmask of candidates whose box holds it;
+// then every lane walks its own set bits in candidate order and accumulates
+// w e^power and w e^power v with Kahan compensation.
+// ---------------------------------------------------------------------------
+struct FwdLds {
+    WalkLds walk;
+    float4 chunk[64][3];   // mean|w, conic c0..c3, c4 c5 v id
+    float4 pts[64];        // group points
+    uint64_t pmask[64];    // per point: candidates whose box holds it
+    int flag[64];          // candidate has an accepted pair
+};
 
 template <bool KAHAN>
 __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict__ rec, const BNode* __restrict__ nodes,
@@ -373,11 +501,11 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
                                                       uint32_t* __restrict__ group_records,
                                                       uint32_t* __restrict__ group_pairs, uint32_t* __restrict__ err)
 {
-    __shared__ int stacks[16][kStackDepth];
-    const int wpb = blockDim.x >> 6;
+    __shared__ FwdLds lds_all[4];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int g = (int)xcd_block(blockIdx.x, gridDim.x) * wpb + wid;
+    const int g = (int)xcd_block(blockIdx.x, gridDim.x) * 4 + wid;
     if (g >= NG) return;
+    FwdLds& s = lds_all[wid];
     const int pi = g * 64 + lane;
     float px = __builtin_nanf(""), py = px, pz = px;
     int orig = -1;
@@ -386,29 +514,47 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
         px = p.x; py = p.y; pz = p.z;
         orig = __builtin_bit_cast(int, p.w);
     }
+    s.pts[lane] = make_float4(px, py, pz, 0.f);
+    const int npts = min(64, S - g * 64);
     const GroupBox gb = group_box(px, py, pz);
+    lds_fence();
 
     float W = 0.f, cW = 0.f, U = 0.f, cU = 0.f;
-    uint32_t npairs = 0;
-    uint32_t nrec = 0;
-    auto leaf = [&](int j) {
-        const int g0 = j * L;
-        const int g1 = min(g0 + L, P);
-        for (int gi = g0; gi < g1; ++gi) {
-            const GRecord& r = rec[gi];
-            if (!boxes_overlap(r.lox, r.loy, r.loz, r.hix, r.hiy, r.hiz, gb.lx, gb.ly, gb.lz, gb.hx, gb.hy, gb.hz))
-                continue;
-            const bool inb = r.lox <= px && px <= r.hix && r.loy <= py && py <= r.hiy && r.loz <= pz && pz <= r.hiz;
-            const float dx = px - r.mx, dy = py - r.my, dz = pz - r.mz;
+    uint32_t npairs = 0, nrec = 0;
+
+    auto chunk = [&](uint32_t base, int n) {
+        // stage the candidates
+        float lox = 1.f, loy = 1.f, loz = 1.f, hix = 0.f, hiy = 0.f, hiz = 0.f;  // empty box for idle lanes
+        if (lane < n) {
+            const int gi = s.walk.ring[(base + lane) & (kRing - 1)];
+            const float4* q = reinterpret_cast<const float4*>(&rec[gi]);
+            const float4 a = q[0], b = q[1], c = q[2], d = q[3], e = q[4];
+            s.chunk[lane][0] = a;
+            s.chunk[lane][1] = b;
+            s.chunk[lane][2] = c;
+            lox = d.x; loy = d.y; loz = d.z; hix = d.w; hiy = e.x; hiz = e.y;
+        }
+        s.flag[lane] = 0;
+        // per point p: which candidates' boxes hold it
+        for (int p = 0; p < npts; ++p) {
+            const float4 q = s.pts[p];
+            const bool inb = lox <= q.x && q.x <= hix && loy <= q.y && q.y <= hiy && loz <= q.z && q.z <= hiz;
+            const uint64_t m = __ballot(inb);
+            if (lane == 0) s.pmask[p] = m;
+        }
+        lds_fence();
+        uint64_t m = (lane < npts) ? s.pmask[lane] : 0ull;
+        while (m) {
+            const int j = __builtin_ctzll(m);
+            m &= m - 1;
+            const float4 a = s.chunk[j][0], b = s.chunk[j][1], c = s.chunk[j][2];
+            const float dx = px - a.x, dy = py - a.y, dz = pz - a.z;
             float r0, r1, r2;
-            const float power = pair_power(r.c0, r.c1, r.c2, r.c3, r.c4, r.c5, dx, dy, dz, r0, r1, r2);
-            const bool acc = inb && in_window(power);
-            if (__ballot(acc) == 0ull) continue;
-            ++nrec;
-            if (acc) {
+            const float power = pair_power(b.x, b.y, b.z, b.w, c.x, c.y, dx, dy, dz, r0, r1, r2);
+            if (in_window(power)) {
                 const float e = __builtin_amdgcn_exp2f(power * kLog2e);
-                const float wgt = r.w * e;
-                const float wv = wgt * r.v;
+                const float wgt = a.w * e;
+                const float wv = wgt * c.z;
                 if (KAHAN) {
                     float y = wgt - cW;
                     float t = W + y;
@@ -423,10 +569,14 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
                     U += wv;
                 }
                 ++npairs;
+                s.flag[j] = 1;
             }
         }
+        lds_fence();
+        nrec += (uint32_t)__popcll(__ballot(lane < n && s.flag[lane] != 0));
+        lds_fence();
     };
-    const bool ok = walk_tree(nodes, root_box, NL, gb, stacks[wid], leaf);
+    const bool ok = enumerate_candidates(rec, nodes, root_box, P, NL, L, gb, s.walk, chunk);
     if (!ok && lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_STACK_OVERFLOW);
     if (orig >= 0) {
         float V, Wo;
@@ -440,6 +590,7 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
         out_cells[orig] = V;
         out_weights[orig] = Wo;
     }
+    (void)npts;
     const uint32_t tp = wave_sumu(npairs);
     if (lane == 0) {
         group_records[g] = nrec;
@@ -448,7 +599,10 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// backward walk
+// backward: lane = candidate.  Per chunk each lane holds one candidate's record,
+// builds the bitmask of the group's points inside its box, then walks those
+// points (LDS) accumulating its Gaussian's 11 partial sums; one record per
+// candidate with an accepted pair, at the position fixed by the forward counts.
 // ---------------------------------------------------------------------------
 struct alignas(16) PartialRecord {
     int gid;           // Gaussian index in BVH order
@@ -458,6 +612,12 @@ struct alignas(16) PartialRecord {
 };
 static_assert(sizeof(PartialRecord) == 48, "PartialRecord layout");
 
+struct BwdLds {
+    WalkLds walk;
+    float4 ptsA[64];  // x, y, z, V
+    float2 ptsB[64];  // gV / W, gW   (zero where W <= cutoff)
+};
+
 __global__ __launch_bounds__(256) void backward_kernel(
     const GRecord* __restrict__ rec, const BNode* __restrict__ nodes, const float* __restrict__ root_box, int P,
     int NL, int L, const float4* __restrict__ pts, int S, int NG, const float* __restrict__ out_cells,
@@ -465,13 +625,11 @@ __global__ __launch_bounds__(256) void backward_kernel(
     const uint32_t* __restrict__ group_offsets, uint64_t capacity, PartialRecord* __restrict__ out_rec,
     uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ err)
 {
-    __shared__ int stacks[16][kStackDepth];
-    __shared__ float4 spA[16][64];  // x, y, z, V
-    __shared__ float2 spB[16][64];  // gV / W, gW   (zero where W <= cutoff)
-    const int wpb = blockDim.x >> 6;
+    __shared__ BwdLds lds_all[4];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int g = (int)xcd_block(blockIdx.x, gridDim.x) * wpb + wid;
+    const int g = (int)xcd_block(blockIdx.x, gridDim.x) * 4 + wid;
     if (g >= NG) return;
+    BwdLds& s = lds_all[wid];
     const int pi = g * 64 + lane;
     float px = __builtin_nanf(""), py = px, pz = px;
     float V = 0.f, gVi = 0.f, gWe = 0.f;
@@ -486,60 +644,66 @@ __global__ __launch_bounds__(256) void backward_kernel(
             gWe = gweights[orig];
         }
     }
-    spA[wid][lane] = make_float4(px, py, pz, V);
-    spB[wid][lane] = make_float2(gVi, gWe);
+    s.ptsA[lane] = make_float4(px, py, pz, V);
+    s.ptsB[lane] = make_float2(gVi, gWe);
     const int npts = min(64, S - g * 64);
     const GroupBox gb = group_box(px, py, pz);
-    __builtin_amdgcn_wave_barrier();
+    lds_fence();
 
-    uint32_t base = group_offsets[g];
+    uint32_t base_out = group_offsets[g];
     const uint32_t limit = group_offsets[g + 1];
-    int qn = 0;
-    int qv = 0;
+    const uint64_t lt = lanemask_lt();
 
-    auto flush = [&]() {
-        const bool active = lane < qn;
-        const int gi = active ? qv : 0;
-        const GRecord r = rec[gi];
+    auto chunk = [&](uint32_t base, int n) {
+        const bool active = lane < n;
+        const int gi = active ? s.walk.ring[(base + lane) & (kRing - 1)] : 0;
+        const float4* q = reinterpret_cast<const float4*>(&rec[gi]);
+        const float4 ra = q[0], rb = q[1], rc = q[2], rd = q[3], re = q[4];
+        // points inside this candidate's box
+        uint64_t m = 0;
+        if (active) {
+            for (int p = 0; p < npts; ++p) {
+                const float4 a = s.ptsA[p];
+                const bool inb = rd.x <= a.x && a.x <= rd.w && rd.y <= a.y && a.y <= re.x && rd.z <= a.z &&
+                                 a.z <= re.y;
+                m |= (uint64_t)inb << p;
+            }
+        }
         float dv = 0.f, dw = 0.f, dmx = 0.f, dmy = 0.f, dmz = 0.f;
         float dc0 = 0.f, dc1 = 0.f, dc2 = 0.f, dc3 = 0.f, dc4 = 0.f, dc5 = 0.f;
         bool any = false;
-        for (int p = 0; p < npts; ++p) {
-            const float4 a = spA[wid][p];
-            const float2 b = spB[wid][p];
-            const bool inb = active && r.lox <= a.x && a.x <= r.hix && r.loy <= a.y && a.y <= r.hiy &&
-                             r.loz <= a.z && a.z <= r.hiz;
-            const float dx = a.x - r.mx, dy = a.y - r.my, dz = a.z - r.mz;
+        while (m) {
+            const int p = __builtin_ctzll(m);
+            m &= m - 1;
+            const float4 a = s.ptsA[p];
+            const float2 b = s.ptsB[p];
+            const float dx = a.x - ra.x, dy = a.y - ra.y, dz = a.z - ra.z;
             float r0, r1, r2;
-            const float power = pair_power(r.c0, r.c1, r.c2, r.c3, r.c4, r.c5, dx, dy, dz, r0, r1, r2);
-            const bool acc = inb && in_window(power);
-            if (__ballot(acc) == 0ull) continue;
-            any |= acc;
-            if (acc) {
-                const float e = __builtin_amdgcn_exp2f(power * kLog2e);
-                const float wgt = r.w * e;
-                dv = __builtin_fmaf(b.x, wgt, dv);
-                const float t = __builtin_fmaf(b.x, r.v - a.w, b.y);
-                dw = __builtin_fmaf(t, e, dw);
-                const float st = wgt * t;
-                dmx = __builtin_fmaf(st, r0, dmx);
-                dmy = __builtin_fmaf(st, r1, dmy);
-                dmz = __builtin_fmaf(st, r2, dmz);
-                const float dq = -0.5f * st;
-                const float h0 = dq * dx, h1 = dq * dy, h2 = dq * dz;
-                const float h0x2 = h0 + h0, h1x2 = h1 + h1;
-                dc0 = __builtin_fmaf(h0, dx, dc0);
-                dc1 = __builtin_fmaf(h0x2, dy, dc1);
-                dc2 = __builtin_fmaf(h0x2, dz, dc2);
-                dc3 = __builtin_fmaf(h1, dy, dc3);
-                dc4 = __builtin_fmaf(h1x2, dz, dc4);
-                dc5 = __builtin_fmaf(h2, dz, dc5);
-            }
+            const float power = pair_power(rb.x, rb.y, rb.z, rb.w, rc.x, rc.y, dx, dy, dz, r0, r1, r2);
+            if (!in_window(power)) continue;
+            any = true;
+            const float e = __builtin_amdgcn_exp2f(power * kLog2e);
+            const float wgt = ra.w * e;
+            dv = __builtin_fmaf(b.x, wgt, dv);
+            const float t = __builtin_fmaf(b.x, rc.z - a.w, b.y);
+            dw = __builtin_fmaf(t, e, dw);
+            const float st = wgt * t;
+            dmx = __builtin_fmaf(st, r0, dmx);
+            dmy = __builtin_fmaf(st, r1, dmy);
+            dmz = __builtin_fmaf(st, r2, dmz);
+            const float dq = -0.5f * st;
+            const float h0 = dq * dx, h1 = dq * dy, h2 = dq * dz;
+            const float h0x2 = h0 + h0, h1x2 = h1 + h1;
+            dc0 = __builtin_fmaf(h0, dx, dc0);
+            dc1 = __builtin_fmaf(h0x2, dy, dc1);
+            dc2 = __builtin_fmaf(h0x2, dz, dc2);
+            dc3 = __builtin_fmaf(h1, dy, dc3);
+            dc4 = __builtin_fmaf(h1x2, dz, dc4);
+            dc5 = __builtin_fmaf(h2, dz, dc5);
         }
         const uint64_t bal = __ballot(any);
-        const uint32_t rank = (uint32_t)__popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
         if (any) {
-            const uint64_t pos = (uint64_t)base + rank;
+            const uint64_t pos = (uint64_t)base_out + (uint32_t)__popcll(bal & lt);
             if (pos < limit && pos < capacity) {
                 PartialRecord o;
                 o.gid = gi;
@@ -553,24 +717,10 @@ __global__ __launch_bounds__(256) void backward_kernel(
                 atomicOr(err, (uint32_t)GVC_ERRFLAG_RECORD_OVERFLOW);
             }
         }
-        base += (uint32_t)__popcll(bal);
-        qn = 0;
+        base_out += (uint32_t)__popcll(bal);
     };
-    auto leaf = [&](int j) {
-        const int g0 = j * L;
-        const int g1 = min(g0 + L, P);
-        for (int gi = g0; gi < g1; ++gi) {
-            const GRecord& r = rec[gi];
-            if (!boxes_overlap(r.lox, r.loy, r.loz, r.hix, r.hiy, r.hiz, gb.lx, gb.ly, gb.lz, gb.hx, gb.hy, gb.hz))
-                continue;
-            if (lane == qn) qv = gi;
-            ++qn;
-            if (qn == 64) flush();
-        }
-    };
-    const bool ok = walk_tree(nodes, root_box, NL, gb, stacks[wid], leaf);
+    const bool ok = enumerate_candidates(rec, nodes, root_box, P, NL, L, gb, s.walk, chunk);
     if (!ok && lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_STACK_OVERFLOW);
-    if (qn > 0) flush();
 }
 
 // Pads the sort input beyond the emitted records with key P (sorts last).
@@ -882,6 +1032,8 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
         return GVC_OK;
     }
     const int fp64 = (prm->flags & GVC_FLAG_FP64_COND) ? 1 : 0;
+    const unsigned stages = prm->stages ? (unsigned)prm->stages : 3u;
+    if (stages & GVC_STAGE_BUILD) {
     // 1. Morton order of the Gaussians
     gaussian_codes_kernel<<<blocks_for(P, 256), 256, 0, st>>>(means, P, l.codes, l.idx);
     GVC_LAUNCH_CHECK("gaussian_codes");
@@ -904,11 +1056,12 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
     refit_kernel<<<blocks_for(NL, 256), 256, 0, st>>>(l.rec, P, NL, L, l.nodes, l.parent_int, l.parent_leaf,
                                                       l.counters, l.root_box);
     GVC_LAUNCH_CHECK("refit");
+    }
+    if (!(stages & GVC_STAGE_WALK)) return GVC_OK;
     // 4. walk
     if (NG > 0) {
-        const int T = prm->block_threads > 1024 ? 1024 : prm->block_threads;
-        const int wpb = T / 64;
-        const unsigned nb = blocks_for(NG, wpb);
+        const int T = 256;
+        const unsigned nb = blocks_for(NG, 4);
         if (prm->flags & GVC_FLAG_KAHAN)
             forward_kernel<true><<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells,
                                                    out_weights, l.group_records, l.group_pairs, l.err);
@@ -967,9 +1120,10 @@ int mixture_backward(int P, const float* means, const float* scales, const float
     Arena sa(const_cast<void*>(sample_set), sample_set_bytes(S));
     SampleSetLayout sl(sa, S);
     const int NG = l.NG, NL = l.NL;
-    if (NG > 0 && capacity > 0) {
-        const int T = prm->block_threads;
-        const unsigned nb = blocks_for(NG, T / 64);
+    const unsigned stages = prm->stages ? (unsigned)prm->stages : 3u;
+    if ((stages & GVC_STAGE_WALK) && NG > 0 && capacity > 0) {
+        const int T = 256;
+        const unsigned nb = blocks_for(NG, 4);
         backward_kernel<<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells, out_weights,
                                           gcells, gweights, l.group_offsets, capacity, bl.parts, bl.keys, bl.vals,
                                           l.err);
@@ -978,6 +1132,7 @@ int mixture_backward(int P, const float* means, const float* scales, const float
                                                                   (uint32_t)P);
         GVC_LAUNCH_CHECK("pad_keys");
     }
+    if (!(stages & GVC_STAGE_REDUCE)) return GVC_OK;
     GVC_TRY(fill_u32(bl.begin, 0u, (size_t)P, st));
     GVC_TRY(fill_u32(bl.end, 0u, (size_t)P, st));
     const uint32_t* skeys = bl.keys;

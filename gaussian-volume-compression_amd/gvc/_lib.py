@@ -13,6 +13,9 @@ GVC_FLAG_KAHAN = 1
 GVC_FLAG_FP64_COND = 2
 GVC_ERRFLAG_STACK_OVERFLOW = 1
 GVC_ERRFLAG_RECORD_OVERFLOW = 2
+GVC_STAGE_BUILD = 1
+GVC_STAGE_WALK = 2
+GVC_STAGE_REDUCE = 1
 
 
 class GvcError(RuntimeError):
@@ -21,7 +24,7 @@ class GvcError(RuntimeError):
 
 class gvc_params(ctypes.Structure):
     _fields_ = [("leaf_size", ctypes.c_int), ("block_threads", ctypes.c_int),
-                ("flags", ctypes.c_uint), ("reserved", ctypes.c_int)]
+                ("flags", ctypes.c_uint), ("stages", ctypes.c_uint)]
 
 
 _lib = None

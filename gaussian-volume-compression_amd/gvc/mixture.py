@@ -32,6 +32,39 @@ class MixtureParams:
 
 _default_params = MixtureParams()
 _record_capacity = None   # fixed backward capacity (graph capture); None = read it
+_stage_timer = None       # list collecting (stage, start_event, end_event) when set
+_audit = None             # list collecting every ForwardState when set
+
+
+def set_stage_timer(lst):
+    """When ``lst`` is a list, every mixture call runs stage by stage and appends
+    (stage name, start event, end event); events are recorded on the launch stream."""
+    global _stage_timer
+    _stage_timer = lst
+
+
+def set_audit(lst):
+    """When ``lst`` is a list, every ForwardState is appended (kept alive) so its
+    counters and error flags can be read after a timed region."""
+    global _audit
+    _audit = lst
+
+
+def _staged_call(name, args_before, cp, args_after, stages):
+    """Runs one mixture entry point, split into its stages when timing."""
+    if _stage_timer is None:
+        cp.stages = 0
+        call(name, *args_before, ctypes.byref(cp), *args_after)
+        return
+    for mask, label in stages:
+        cp.stages = mask
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        call(name, *args_before, ctypes.byref(cp), *args_after)
+        e1.record()
+        _stage_timer.append((label, e0, e1))
+    cp.stages = 0
 
 
 def set_default_params(params):
@@ -130,10 +163,15 @@ def mixture_forward(means3D, scales, rotations, values, weights, scale_modifier,
     cells = torch.empty(S, dtype=torch.float32, device=dev)
     cw = torch.empty(S, dtype=torch.float32, device=dev)
     conics = torch.empty(6 * P, dtype=torch.float32, device=dev) if want_conics else None
-    call("gvc_mixture_forward", P, _ptr(means3D), _ptr(scales), _ptr(rotations), _ptr(values), _ptr(weights),
-         float(scale_modifier), _ptr(sample_set.buffer), S, ctypes.byref(cp), _ptr(cells), _ptr(cw), _ptr(conics),
-         _ptr(state), nbytes, _stream(dev))
-    return cells, cw, conics, ForwardState(state, params, P, S)
+    _staged_call("gvc_mixture_forward",
+                 (P, _ptr(means3D), _ptr(scales), _ptr(rotations), _ptr(values), _ptr(weights),
+                  float(scale_modifier), _ptr(sample_set.buffer), S), cp,
+                 (_ptr(cells), _ptr(cw), _ptr(conics), _ptr(state), nbytes, _stream(dev)),
+                 ((_lib.GVC_STAGE_BUILD, "fwd_build"), (_lib.GVC_STAGE_WALK, "fwd_walk")))
+    fs = ForwardState(state, params, P, S)
+    if _audit is not None:
+        _audit.append(fs)
+    return cells, cw, conics, fs
 
 
 def mixture_backward(fstate, means3D, scales, rotations, values, weights, scale_modifier, sample_set, out_cells,
@@ -165,10 +203,13 @@ def mixture_backward(fstate, means3D, scales, rotations, values, weights, scale_
     cp = fstate._c
     nbytes = lib().gvc_backward_bytes(P, cap)
     ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
-    call("gvc_mixture_backward", P, _ptr(means3D), _ptr(scales), _ptr(rotations), _ptr(values), _ptr(weights),
-         float(scale_modifier), _ptr(sample_set.buffer), S, ctypes.byref(cp), _ptr(out_cells), _ptr(out_weights),
-         _ptr(grad_cells), _ptr(grad_weights), _ptr(fstate.buffer), fstate.buffer.numel(), ctypes.c_uint64(cap),
-         _ptr(ws), nbytes, _ptr(gm), _ptr(gs), _ptr(gr), _ptr(gv), _ptr(gw), _stream(dev))
+    _staged_call("gvc_mixture_backward",
+                 (P, _ptr(means3D), _ptr(scales), _ptr(rotations), _ptr(values), _ptr(weights),
+                  float(scale_modifier), _ptr(sample_set.buffer), S), cp,
+                 (_ptr(out_cells), _ptr(out_weights), _ptr(grad_cells), _ptr(grad_weights), _ptr(fstate.buffer),
+                  fstate.buffer.numel(), ctypes.c_uint64(cap), _ptr(ws), nbytes, _ptr(gm), _ptr(gs), _ptr(gr),
+                  _ptr(gv), _ptr(gw), _stream(dev)),
+                 ((_lib.GVC_STAGE_WALK, "bwd_walk"), (_lib.GVC_STAGE_REDUCE, "bwd_reduce")))
     return gm, gs, gr, gv, gw
 
 

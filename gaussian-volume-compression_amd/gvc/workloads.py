@@ -163,3 +163,73 @@ def config2(seed=0, n=128, P=1_000_000, S=1 << 22, n_blobs=32):
     samples = rng.uniform(0.0, 1.0, (S, 3)).astype(np.float32)
     return dict(means=means, log_scales=log_scales, rotations=rots, weights=weights, values=values,
                 mesh_points=pts, tets=tets, mesh_values=vals, samples=samples)
+
+
+# ---------------------------------------------------------------------------
+# device-side builders used by bench.py (same definitions, torch on the GPU)
+# ---------------------------------------------------------------------------
+def _eval_blobs_torch(b, pts, device):
+    import torch
+    x = torch.as_tensor(pts, dtype=torch.float64, device=device)
+    mu = torch.as_tensor(b["means"], device=device)
+    inv = torch.as_tensor(b["inv"], device=device)
+    w = torch.as_tensor(b["w"], device=device)
+    out = torch.zeros(x.shape[0], dtype=torch.float64, device=device)
+    for k in range(mu.shape[0]):
+        d = x - mu[k]
+        q = ((d @ inv[k]) * d).sum(1)
+        out += w[k] * torch.exp(-0.5 * q)
+    return out
+
+
+def _init_gaussians(means, values, device):
+    """R/scene/gaussian_model.py:175-188: 3-NN scales, identity rotations, w = 0.1."""
+    import torch
+    from simple_knn._C import distCUDA2
+    m = torch.as_tensor(means, dtype=torch.float32, device=device).contiguous()
+    d2 = torch.clamp_min(distCUDA2(m), 1e-7)
+    log_s = torch.log(torch.sqrt(d2))[:, None].repeat(1, 3).contiguous()
+    P = m.shape[0]
+    rots = torch.zeros((P, 4), dtype=torch.float32, device=device)
+    rots[:, 0] = 1.0
+    weights = torch.full((P, 1), 0.1, dtype=torch.float32, device=device)
+    vals = torch.as_tensor(values, dtype=torch.float32, device=device).reshape(P, 1).contiguous()
+    return dict(means=m, log_scales=log_s, rotations=rots, weights=weights, values=vals, P=int(P))
+
+
+def mesh_workload(device, seed=0, n=128, P=1_000_000, n_blobs=32):
+    """config2 on the device: mesh (numpy), Gaussians (torch)."""
+    rng = np.random.default_rng(seed)
+    pts = jittered_lattice(rng, n)
+    tets = kuhn_tets(n).astype(np.int32)
+    b = blobs(rng, n_blobs)
+    vals = _eval_blobs_torch(b, pts, device)
+    vals = (vals - vals.min()) / (vals.max() - vals.min())   # R/scene/dataset_readers.py:10-17
+    vsel = rng.choice(pts.shape[0], size=min(P, pts.shape[0]), replace=False)
+    g = _init_gaussians(pts[vsel], vals.cpu().numpy()[vsel], device)
+    g.update(mesh_points=pts, tets=tets, mesh_values=vals)
+    return g
+
+
+def grid_workload(device, seed=0, n=128, P=200_000):
+    """config1 on the device; grid_fn(samples) is the trilinear ground truth."""
+    import torch
+    from gvc.sampling import grid_sample
+    rng = np.random.default_rng(seed)
+    b = blobs(rng, 64)
+    grid = grid_points(n)
+    field = _eval_blobs_torch(b, grid, device) + torch.as_tensor(rng.normal(0.0, 0.01, grid.shape[0]), device=device)
+    field = field.float()
+    vox = rng.choice(grid.shape[0], size=P, replace=False)
+    g = _init_gaussians(grid[vox], field.cpu().numpy()[vox], device)
+    # grid_points uses 'ij' order (x slowest); the probe wants x fastest
+    f_xfast = field.reshape(n, n, n).permute(2, 1, 0).contiguous().reshape(-1)
+    dims = np.array([n, n, n], np.int64)
+    origin = np.zeros(3, np.float32)
+    spacing = np.full(3, 1.0 / (n - 1), np.float32)
+
+    def grid_fn(samples):
+        return grid_sample(dims, origin, spacing, f_xfast, samples)
+
+    g.update(grid_fn=grid_fn, field=field)
+    return g

@@ -40,12 +40,18 @@ enum gvc_status {
 #define GVC_FLAG_KAHAN 1u      /* compensated per-point sums in the forward       */
 #define GVC_FLAG_FP64_COND 2u  /* FP64 covariance inverse where cond(Sigma) > 1e6 */
 
+/* gvc_params.stages: 0 runs every stage of a call; otherwise only the selected
+ * ones, in order, so a caller can bracket each stage with events. */
+#define GVC_STAGE_BUILD 1u   /* forward: Morton sort, records, tree, refit          */
+#define GVC_STAGE_WALK 2u    /* forward: walk + count scan; backward: partial records */
+#define GVC_STAGE_REDUCE 1u  /* backward: sort, segments, FP64 sums, chain          */
+
 /* Launch/tuning parameters of the mixture kernels (chosen by the autotuner). */
 typedef struct gvc_params {
     int leaf_size;      /* Gaussians per BVH leaf: 1..64                        */
     int block_threads;  /* threads per block of the traversal kernels: 64..1024 */
     unsigned flags;     /* GVC_FLAG_*                                           */
-    int reserved;
+    unsigned stages;    /* GVC_STAGE_* mask, 0 = all                            */
 } gvc_params;
 
 /* Error flags reported by gvc_forward_stats (bitwise OR). */

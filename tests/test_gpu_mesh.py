@@ -1,0 +1,112 @@
+"""GPU parity of the mesh sampler and distCUDA2 against the oracle.
+
+Bars: cell ids bit-exact (int32), interpolated values within 1e-9 relative of the
+exact rational barycentrics; grid probe bit-exact for unsigned types and within
+1e-12 for floating types; distCUDA2 bit-exact (same float operation order).
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+import torch  # noqa: E402
+
+from gvc import workloads  # noqa: E402
+from gvc.sampling import TetMeshSampler  # noqa: E402
+from oracle import mesh_ref  # noqa: E402
+
+f32 = np.float32
+DEV = "cuda:0"
+
+
+def lattice_samples(n, rng, count):
+    """Random points plus points exactly on lattice vertices, edges and faces."""
+    g = np.arange(n + 1, dtype=np.float64) / n
+    special = []
+    for _ in range(count // 4):
+        i, j, k = rng.integers(0, n + 1, 3)
+        a, b = rng.integers(0, n, 2)
+        special.append([g[i], g[j], g[k]])                          # vertex
+        special.append([g[i], (g[a] + g[a + 1]) / 2, g[k]])        # edge midpoint
+        special.append([(g[a] + g[a + 1]) / 2, (g[b] + g[b + 1]) / 2, g[k]])  # face point
+        t = rng.uniform()
+        special.append([g[a] + t / n, g[a] + t / n, g[k]])          # on the x = y diagonal plane
+    rand = rng.uniform(-0.05, 1.05, (count, 3))
+    return np.concatenate([rand, np.array(special)]).astype(f32)
+
+
+@pytest.mark.parametrize("n,jitter", [(4, 0.0), (6, 0.3)])
+def test_tet_locate_bit_exact(n, jitter):
+    rng = np.random.default_rng(n)
+    pts = workloads.jittered_lattice(rng, n, jitter)
+    tets = workloads.kuhn_tets(n)
+    vals = rng.normal(size=pts.shape[0])
+    samples = lattice_samples(n, rng, 1200)
+    sampler = TetMeshSampler(torch.tensor(pts, device=DEV), tets, vals)
+    gv, gid = sampler.sample(torch.tensor(samples, device=DEV))
+    ids, out = mesh_ref.locate(pts, tets, samples, vals)
+    assert np.array_equal(gid.cpu().numpy(), ids)
+    ok = ids >= 0
+    got = gv.cpu().numpy()
+    np.testing.assert_allclose(got[ok], out[ok], rtol=1e-9, atol=1e-12)
+    assert np.all(got[~ok] == -1.0)
+
+
+def test_gpu_sampleu_dropin():
+    from gpu_mesh_sampling import gpu_sampleu
+    rng = np.random.default_rng(7)
+    n = 5
+    pts = workloads.jittered_lattice(rng, n, 0.25)
+    tets = workloads.kuhn_tets(n)
+    C = tets.shape[0]
+    conn = tets.reshape(-1).astype(np.int64)
+    types = np.full(C, 10, np.int64)
+    offsets = np.arange(C + 1, dtype=np.int64) * 4
+    vals = rng.normal(size=pts.shape[0])
+    samples = rng.uniform(-0.1, 1.1, (3000, 3)).astype(f32)
+    out = gpu_sampleu(pts, conn, types, offsets, vals, samples)
+    assert isinstance(out, np.ndarray) and out.dtype == np.float64
+    ids, ref = mesh_ref.locate(pts, tets, samples, vals)
+    np.testing.assert_allclose(out, ref, rtol=1e-9, atol=1e-12)
+    with pytest.raises(RuntimeError):
+        gpu_sampleu(pts, conn, np.full(C, 7, np.int64), offsets, vals, samples)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.uint16, np.uint8])
+def test_grid_sample(dtype):
+    from gpu_mesh_sampling import gpu_sample
+    rng = np.random.default_rng(8)
+    dims = np.array([9, 6, 11], np.int64)
+    origin = np.array([0.1, 0.0, -0.3], f32)
+    spacing = np.array([0.1, 0.2, 0.05], f32)
+    n = int(dims.prod())
+    if dtype in (np.uint8, np.uint16):
+        values = rng.integers(0, np.iinfo(dtype).max, n).astype(dtype)
+    else:
+        values = rng.normal(size=n).astype(dtype)
+    samples = np.stack([rng.uniform(origin[a] - 0.1, origin[a] + (dims[a] - 1) * spacing[a] + 0.1, 5000)
+                        for a in range(3)], 1).astype(f32)
+    samples[:8] = origin  # exact grid points and corners
+    samples[8] = origin + (dims - 1) * spacing
+    out = gpu_sample(dims, origin, spacing, values, samples)
+    ref = mesh_ref.grid_sample(dims, origin, spacing, values, samples)
+    assert out.dtype == values.dtype
+    if dtype in (np.uint8, np.uint16):
+        assert np.array_equal(out, ref)
+    else:
+        np.testing.assert_allclose(out, ref, rtol=1e-12 if dtype == np.float64 else 1e-6, atol=1e-12)
+    with pytest.raises(RuntimeError):
+        gpu_sample(dims, origin, spacing, values[:-1], samples)
+
+
+def test_distcuda2_bit_exact():
+    from simple_knn._C import distCUDA2
+    rng = np.random.default_rng(9)
+    for P in (rng.uniform(0, 1, (5000, 3)), rng.normal(size=(3000, 3)) * [1, 1, 0.001],
+              np.concatenate([rng.uniform(0, 1, (500, 3))] * 2)):
+        P = P.astype(f32)
+        got = distCUDA2(torch.tensor(P, device=DEV)).cpu().numpy()
+        ref = mesh_ref.knn3(P)
+        assert np.array_equal(got, ref)
