@@ -113,18 +113,18 @@ def main():
 
     def step():
         samples = torch.rand((S, 3), device=dev, generator=gen)
+        e0 = torch.cuda.Event(enable_timing=True); e0.record()
+        rast.build_bvh(samples)
+        e1 = torch.cuda.Event(enable_timing=True); e1.record()
+        stage_ev.append(("sample_set", e0, e1))
         if sampler is not None:
             e0 = torch.cuda.Event(enable_timing=True); e0.record()
-            gt64, _ = sampler.sample(samples)
+            gt64, _ = sampler.sample(samples, sample_set=rast.sample_set)
             gt = gt64.float()
             e1 = torch.cuda.Event(enable_timing=True); e1.record()
             stage_ev.append(("tet_locate", e0, e1))
         else:
             gt = W["grid_fn"](samples)
-        e0 = torch.cuda.Event(enable_timing=True); e0.record()
-        rast.build_bvh(samples)
-        e1 = torch.cuda.Event(enable_timing=True); e1.record()
-        stage_ev.append(("sample_set", e0, e1))
         scales = torch.exp(log_s)
         rots = torch.nn.functional.normalize(rot)
         w = torch.sigmoid(wl)
@@ -247,10 +247,12 @@ def main():
                                                      rng.normal(size=n))
                 return time.perf_counter() - t2, pr
 
-            n = 256
-            dt, pr = run_cpu(n)
-            n = int(min(S, max(256, n * args.cpu_seconds / max(dt, 1e-3))))
-            dt, pr = run_cpu(n)
+            n = 512
+            while True:
+                dt, pr = run_cpu(n)
+                if dt >= 0.5 * args.cpu_seconds or n >= S:
+                    break
+                n = int(min(S, n * min(8.0, max(1.5, args.cpu_seconds / max(dt, 1e-3)))))
             out["cpu_baseline"] = {"value": 2.0 * pr / dt, "unit": "evals/s", "cores": oracle.num_threads(),
                                    "kind": "port",
                                    "sample": f"{n} of the step's {S} points against all {P} Gaussians, plain fp32 "

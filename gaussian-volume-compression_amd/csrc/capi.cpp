@@ -49,8 +49,8 @@ size_t tet_locator_bytes(int n_points, int C, unsigned long long cap);
 int tet_locator_build(int n_points, const float* pts, int C, const int* tets, unsigned long long cap, void* loc,
                       size_t bytes, unsigned long long* needed, hipStream_t st);
 int tet_locate_interp(int n_points, const float* pts, int C, const void* loc, size_t bytes, unsigned long long cap,
-                      const double* values, int S, const float* samples, int* cell_ids, double* out_values,
-                      float* bary, hipStream_t st);
+                      const double* values, int S, const float* samples, const void* sample_set, int* cell_ids,
+                      double* out_values, float* bary, hipStream_t st);
 int grid_sample(const long long dims[3], const float origin[3], const float spacing[3], int dtype, const void* values,
                 int S, const float* samples, void* out, hipStream_t st);
 size_t knn3_bytes(int n);
@@ -203,7 +203,20 @@ int gvc_tet_locate_interp(int n_points, const float* points, int n_tets, const v
     GVC_REQUIRE(points && locator && samples && cell_ids, "NULL pointer");
     GVC_REQUIRE(!values_out || point_values, "values_out needs point_values");
     return tet_locate_interp(n_points, points, n_tets, locator, locator_bytes, entry_capacity, point_values, S,
-                             samples, cell_ids, values_out, bary_out, (hipStream_t)stream);
+                             samples, nullptr, cell_ids, values_out, bary_out, (hipStream_t)stream);
+}
+
+int gvc_tet_locate_interp_set(int n_points, const float* points, int n_tets, const void* locator,
+                              size_t locator_bytes, uint64_t entry_capacity, const double* point_values,
+                              const void* sample_set, int S, int32_t* cell_ids, double* values_out, float* bary_out,
+                              void* stream)
+{
+    GVC_REQUIRE(S >= 0, "S < 0");
+    if (S == 0) return GVC_OK;
+    GVC_REQUIRE(points && locator && sample_set && cell_ids, "NULL pointer");
+    GVC_REQUIRE(!values_out || point_values, "values_out needs point_values");
+    return tet_locate_interp(n_points, points, n_tets, locator, locator_bytes, entry_capacity, point_values, S,
+                             nullptr, sample_set, cell_ids, values_out, bary_out, (hipStream_t)stream);
 }
 
 int gvc_grid_sample(const int64_t* dims, const float* origin, const float* spacing, int value_dtype,

@@ -57,6 +57,11 @@ int exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, bool with_to
 int bbox_launch(const float* pts, int n, unsigned* keys, hipStream_t st);
 void bbox_from_keys(const unsigned keys[6], float lo[3], float hi[3]);
 
+// ---- mixture.hip ------------------------------------------------------------
+// Sorted points (x, y, z, original index as int bits) of a sample set, padded with
+// NaN points (index -1) to whole 64-point groups.
+const float4* sample_set_points(const void* sample_set, int S);
+
 // ---- radix_sort.hip --------------------------------------------------------
 // Stable LSD radix sort of (key, value) pairs on the low key_bits bits.
 // Scratch elements (uint32) needed for n items.

@@ -140,15 +140,28 @@ __device__ bool tet_contains(const float* __restrict__ pts, const TetRec& r, con
     return true;
 }
 
+// sorted != nullptr: the samples are taken from a sample set (Morton order, so
+// neighbouring threads share grid cells and tets) and results go to the original
+// index carried in .w.
 __global__ void tet_locate_kernel(const float* __restrict__ pts, const TetRec* __restrict__ recs,
                                   const TetGridHeader* __restrict__ h, const uint32_t* __restrict__ starts,
                                   const int* __restrict__ entries, const double* __restrict__ values, int S,
-                                  const float* __restrict__ samples, int* __restrict__ cell_ids,
-                                  double* __restrict__ out_values, float* __restrict__ bary_out)
+                                  const float* __restrict__ samples, const float4* __restrict__ sorted,
+                                  int* __restrict__ cell_ids, double* __restrict__ out_values,
+                                  float* __restrict__ bary_out)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S) return;
-    const float p[3] = { samples[3 * s], samples[3 * s + 1], samples[3 * s + 2] };
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S) return;
+    int s = i;
+    float p[3];
+    if (sorted) {
+        const float4 q = sorted[i];
+        p[0] = q.x; p[1] = q.y; p[2] = q.z;
+        s = __builtin_bit_cast(int, q.w);
+        if (s < 0) return;
+    } else {
+        p[0] = samples[3 * s]; p[1] = samples[3 * s + 1]; p[2] = samples[3 * s + 2];
+    }
     int best = -1;
     double bb[4] = { 0, 0, 0, 0 };
     bool inside_grid = true;
@@ -312,16 +325,17 @@ int tet_locator_build(int n_points, const float* pts, int C, const int* tets, un
 }
 
 int tet_locate_interp(int n_points, const float* pts, int C, const void* loc, size_t bytes, unsigned long long cap,
-                      const double* values, int S, const float* samples, int* cell_ids, double* out_values,
-                      float* bary, hipStream_t st)
+                      const double* values, int S, const float* samples, const void* sample_set, int* cell_ids,
+                      double* out_values, float* bary, hipStream_t st)
 {
     (void)n_points;
     if (S <= 0) return GVC_OK;
     Arena a(const_cast<void*>(loc), bytes);
     TetLayout l(a, C, max_grid_cells(C), cap);
     if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "tet locator buffer too small");
+    const float4* sorted = sample_set ? sample_set_points(sample_set, S) : nullptr;
     tet_locate_kernel<<<blocks_for(S, 128), 128, 0, st>>>(pts, l.recs, l.header, l.starts, l.entries, values, S,
-                                                          samples, cell_ids, out_values, bary);
+                                                          samples, sorted, cell_ids, out_values, bary);
     GVC_LAUNCH_CHECK("tet_locate");
     return GVC_OK;
 }

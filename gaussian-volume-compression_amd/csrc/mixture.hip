@@ -140,11 +140,18 @@ __global__ void sample_codes_kernel(const float* __restrict__ s, int S, uint32_t
     idx[i] = (uint32_t)i;
 }
 
-__global__ void sample_gather_kernel(const float* __restrict__ s, int S, const uint32_t* __restrict__ perm,
+// Sorted points, padded with NaN points (original index -1) to whole 64-point
+// groups so every group can be read with wave-uniform loads.
+__global__ void sample_gather_kernel(const float* __restrict__ s, int S, int S64, const uint32_t* __restrict__ perm,
                                      float4* __restrict__ pts)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= S) return;
+    if (i >= S64) return;
+    if (i >= S) {
+        const float nan = __builtin_nanf("");
+        pts[i] = make_float4(nan, nan, nan, __builtin_bit_cast(float, -1));
+        return;
+    }
     uint32_t j = perm[i];
     pts[i] = make_float4(s[3 * j], s[3 * j + 1], s[3 * j + 2], __builtin_bit_cast(float, (int)j));
 }
@@ -479,19 +486,36 @@ __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// forward: lane = point.  Per chunk of <= 64 candidates: their records go to
-// LDS; lane j box-tests candidate j against the 64 points (the compare masks are
-// the ballots) giving, per point, the bitmask of candidates whose box holds it;
-// then every lane walks its own set bits in candidate order and accumulates
+// forward: lane = point.  Per chunk of <= 64 candidates: lane j holds candidate
+// j's box; for each of the group's 64 points (wave-uniform scalar loads) the six
+// box compares form a ballot = the candidates whose box holds that point, which
+// v_writelane deposits in that point's lane.  Every lane then walks its own set
+// bits in candidate order (next candidate prefetched from LDS) and accumulates
 // w e^power and w e^power v with Kahan compensation.
 // ---------------------------------------------------------------------------
 struct FwdLds {
     WalkLds walk;
     float4 chunk[64][3];   // mean|w, conic c0..c3, c4 c5 v id
-    float4 pts[64];        // group points
-    uint64_t pmask[64];    // per point: candidates whose box holds it
-    int flag[64];          // candidate has an accepted pair
 };
+
+// lo[lane_c] = vlo, hi[lane_c] = vhi for wave-uniform values and a constant lane.
+// The values usually come straight from a VALU compare, and gfx950 requires wait
+// states between a VALU write of an SGPR and a VALU read of it; the compiler
+// cannot see into the asm, so the s_nop provides them (5 states).
+template <class I>
+__device__ __forceinline__ void writelane2(int& lo, int& hi, uint32_t vlo, uint32_t vhi, I lane_c)
+{
+    asm volatile("s_nop 4\n\tv_writelane_b32 %0, %2, %4\n\tv_writelane_b32 %1, %3, %4"
+                 : "+v"(lo), "+v"(hi)
+                 : "s"(vlo), "s"(vhi), "n"((int)lane_c));
+}
+
+__device__ __forceinline__ uint32_t wave_or32(uint32_t v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v |= (uint32_t)__shfl_xor((int)v, d, 64);
+    return v;
+}
 
 template <bool KAHAN>
 __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict__ rec, const BNode* __restrict__ nodes,
@@ -502,28 +526,20 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
                                                       uint32_t* __restrict__ group_pairs, uint32_t* __restrict__ err)
 {
     __shared__ FwdLds lds_all[4];
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int g = (int)xcd_block(blockIdx.x, gridDim.x) * 4 + wid;
+    const int wid = uniformi(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int g = uniformi((int)xcd_block(blockIdx.x, gridDim.x) * 4 + wid);
     if (g >= NG) return;
     FwdLds& s = lds_all[wid];
-    const int pi = g * 64 + lane;
-    float px = __builtin_nanf(""), py = px, pz = px;
-    int orig = -1;
-    if (pi < S) {
-        const float4 p = pts[pi];
-        px = p.x; py = p.y; pz = p.z;
-        orig = __builtin_bit_cast(int, p.w);
-    }
-    s.pts[lane] = make_float4(px, py, pz, 0.f);
-    const int npts = min(64, S - g * 64);
+    const float4* __restrict__ gp = pts + (size_t)g * 64;
+    const float4 pme = gp[lane];
+    const float px = pme.x, py = pme.y, pz = pme.z;
+    const int orig = __builtin_bit_cast(int, pme.w);
     const GroupBox gb = group_box(px, py, pz);
-    lds_fence();
 
     float W = 0.f, cW = 0.f, U = 0.f, cU = 0.f;
     uint32_t npairs = 0, nrec = 0;
 
     auto chunk = [&](uint32_t base, int n) {
-        // stage the candidates
         float lox = 1.f, loy = 1.f, loz = 1.f, hix = 0.f, hiy = 0.f, hiz = 0.f;  // empty box for idle lanes
         if (lane < n) {
             const int gi = s.walk.ring[(base + lane) & (kRing - 1)];
@@ -534,46 +550,58 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
             s.chunk[lane][2] = c;
             lox = d.x; loy = d.y; loz = d.z; hix = d.w; hiy = e.x; hiz = e.y;
         }
-        s.flag[lane] = 0;
-        // per point p: which candidates' boxes hold it
-        for (int p = 0; p < npts; ++p) {
-            const float4 q = s.pts[p];
+        int mlo = 0, mhi = 0;
+#pragma unroll
+        for (int p = 0; p < 64; ++p) {
+            const float4 q = gp[p];
             const bool inb = lox <= q.x && q.x <= hix && loy <= q.y && q.y <= hiy && loz <= q.z && q.z <= hiz;
-            const uint64_t m = __ballot(inb);
-            if (lane == 0) s.pmask[p] = m;
+            const uint64_t bal = __ballot(inb);
+            writelane2(mlo, mhi, (uint32_t)bal, (uint32_t)(bal >> 32), p);
         }
         lds_fence();
-        uint64_t m = (lane < npts) ? s.pmask[lane] : 0ull;
-        while (m) {
-            const int j = __builtin_ctzll(m);
+        uint64_t m = ((uint64_t)(uint32_t)mhi << 32) | (uint32_t)mlo;
+        uint64_t accm = 0;
+        if (m) {
+            int j = __builtin_ctzll(m);
             m &= m - 1;
-            const float4 a = s.chunk[j][0], b = s.chunk[j][1], c = s.chunk[j][2];
-            const float dx = px - a.x, dy = py - a.y, dz = pz - a.z;
-            float r0, r1, r2;
-            const float power = pair_power(b.x, b.y, b.z, b.w, c.x, c.y, dx, dy, dz, r0, r1, r2);
-            if (in_window(power)) {
-                const float e = __builtin_amdgcn_exp2f(power * kLog2e);
-                const float wgt = a.w * e;
-                const float wv = wgt * c.z;
-                if (KAHAN) {
-                    float y = wgt - cW;
-                    float t = W + y;
-                    cW = (t - W) - y;
-                    W = t;
-                    y = wv - cU;
-                    t = U + y;
-                    cU = (t - U) - y;
-                    U = t;
-                } else {
-                    W += wgt;
-                    U += wv;
+            float4 A = s.chunk[j][0], B = s.chunk[j][1], C = s.chunk[j][2];
+            while (true) {
+                const bool more = m != 0;
+                const int jn = more ? __builtin_ctzll(m) : j;
+                const float4 An = s.chunk[jn][0], Bn = s.chunk[jn][1], Cn = s.chunk[jn][2];
+                const float dx = px - A.x, dy = py - A.y, dz = pz - A.z;
+                float r0, r1, r2;
+                const float power = pair_power(B.x, B.y, B.z, B.w, C.x, C.y, dx, dy, dz, r0, r1, r2);
+                if (in_window(power)) {
+                    const float e = __builtin_amdgcn_exp2f(power * kLog2e);
+                    const float wgt = A.w * e;
+                    const float wv = wgt * C.z;
+                    if (KAHAN) {
+                        float y = wgt - cW;
+                        float t = W + y;
+                        cW = (t - W) - y;
+                        W = t;
+                        y = wv - cU;
+                        t = U + y;
+                        cU = (t - U) - y;
+                        U = t;
+                    } else {
+                        W += wgt;
+                        U += wv;
+                    }
+                    ++npairs;
+                    accm |= 1ull << j;
                 }
-                ++npairs;
-                s.flag[j] = 1;
+                if (!more) break;
+                m &= m - 1;
+                j = jn;
+                A = An;
+                B = Bn;
+                C = Cn;
             }
         }
-        lds_fence();
-        nrec += (uint32_t)__popcll(__ballot(lane < n && s.flag[lane] != 0));
+        const uint32_t olo = wave_or32((uint32_t)accm), ohi = wave_or32((uint32_t)(accm >> 32));
+        nrec += (uint32_t)(__popc(olo) + __popc(ohi));
         lds_fence();
     };
     const bool ok = enumerate_candidates(rec, nodes, root_box, P, NL, L, gb, s.walk, chunk);
@@ -590,7 +618,6 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
         out_cells[orig] = V;
         out_weights[orig] = Wo;
     }
-    (void)npts;
     const uint32_t tp = wave_sumu(npairs);
     if (lane == 0) {
         group_records[g] = nrec;
@@ -600,9 +627,10 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
 
 // ---------------------------------------------------------------------------
 // backward: lane = candidate.  Per chunk each lane holds one candidate's record,
-// builds the bitmask of the group's points inside its box, then walks those
-// points (LDS) accumulating its Gaussian's 11 partial sums; one record per
-// candidate with an accepted pair, at the position fixed by the forward counts.
+// builds the bitmask of the group's points inside its box (points by scalar
+// loads), then walks those points (LDS, next point prefetched) accumulating its
+// Gaussian's 11 partial sums; one record per candidate with an accepted pair, at
+// the position fixed by the forward's per-group counts.
 // ---------------------------------------------------------------------------
 struct alignas(16) PartialRecord {
     int gid;           // Gaussian index in BVH order
@@ -626,17 +654,16 @@ __global__ __launch_bounds__(256) void backward_kernel(
     uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ err)
 {
     __shared__ BwdLds lds_all[4];
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int g = (int)xcd_block(blockIdx.x, gridDim.x) * 4 + wid;
+    const int wid = uniformi(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int g = uniformi((int)xcd_block(blockIdx.x, gridDim.x) * 4 + wid);
     if (g >= NG) return;
     BwdLds& s = lds_all[wid];
-    const int pi = g * 64 + lane;
-    float px = __builtin_nanf(""), py = px, pz = px;
+    const float4* __restrict__ gp = pts + (size_t)g * 64;
+    const float4 pme = gp[lane];
+    const float px = pme.x, py = pme.y, pz = pme.z;
+    const int orig = __builtin_bit_cast(int, pme.w);
     float V = 0.f, gVi = 0.f, gWe = 0.f;
-    if (pi < S) {
-        const float4 p = pts[pi];
-        px = p.x; py = p.y; pz = p.z;
-        const int orig = __builtin_bit_cast(int, p.w);
+    if (orig >= 0) {
         const float Wp = out_weights[orig];
         if ((double)Wp > kWeightCutoff) {
             V = out_cells[orig];
@@ -646,7 +673,6 @@ __global__ __launch_bounds__(256) void backward_kernel(
     }
     s.ptsA[lane] = make_float4(px, py, pz, V);
     s.ptsB[lane] = make_float2(gVi, gWe);
-    const int npts = min(64, S - g * 64);
     const GroupBox gb = group_box(px, py, pz);
     lds_fence();
 
@@ -659,47 +685,61 @@ __global__ __launch_bounds__(256) void backward_kernel(
         const int gi = active ? s.walk.ring[(base + lane) & (kRing - 1)] : 0;
         const float4* q = reinterpret_cast<const float4*>(&rec[gi]);
         const float4 ra = q[0], rb = q[1], rc = q[2], rd = q[3], re = q[4];
-        // points inside this candidate's box
-        uint64_t m = 0;
-        if (active) {
-            for (int p = 0; p < npts; ++p) {
-                const float4 a = s.ptsA[p];
-                const bool inb = rd.x <= a.x && a.x <= rd.w && rd.y <= a.y && a.y <= re.x && rd.z <= a.z &&
-                                 a.z <= re.y;
-                m |= (uint64_t)inb << p;
-            }
+        const float lox = active ? rd.x : 1.f, hix = active ? rd.w : 0.f;
+        uint32_t mlo = 0, mhi = 0;
+#pragma unroll 4
+        for (int p = 0; p < 32; ++p) {
+            const float4 a = s.ptsA[p];
+            const float4 b = s.ptsA[32 + p];
+            const bool ia = lox <= a.x && a.x <= hix && rd.y <= a.y && a.y <= re.x && rd.z <= a.z && a.z <= re.y;
+            const bool ib = lox <= b.x && b.x <= hix && rd.y <= b.y && b.y <= re.x && rd.z <= b.z && b.z <= re.y;
+            mlo |= ia ? (1u << p) : 0u;
+            mhi |= ib ? (1u << p) : 0u;
         }
+        uint64_t m = ((uint64_t)mhi << 32) | mlo;
         float dv = 0.f, dw = 0.f, dmx = 0.f, dmy = 0.f, dmz = 0.f;
         float dc0 = 0.f, dc1 = 0.f, dc2 = 0.f, dc3 = 0.f, dc4 = 0.f, dc5 = 0.f;
         bool any = false;
-        while (m) {
-            const int p = __builtin_ctzll(m);
+        if (m) {
+            int p = __builtin_ctzll(m);
             m &= m - 1;
-            const float4 a = s.ptsA[p];
-            const float2 b = s.ptsB[p];
-            const float dx = a.x - ra.x, dy = a.y - ra.y, dz = a.z - ra.z;
-            float r0, r1, r2;
-            const float power = pair_power(rb.x, rb.y, rb.z, rb.w, rc.x, rc.y, dx, dy, dz, r0, r1, r2);
-            if (!in_window(power)) continue;
-            any = true;
-            const float e = __builtin_amdgcn_exp2f(power * kLog2e);
-            const float wgt = ra.w * e;
-            dv = __builtin_fmaf(b.x, wgt, dv);
-            const float t = __builtin_fmaf(b.x, rc.z - a.w, b.y);
-            dw = __builtin_fmaf(t, e, dw);
-            const float st = wgt * t;
-            dmx = __builtin_fmaf(st, r0, dmx);
-            dmy = __builtin_fmaf(st, r1, dmy);
-            dmz = __builtin_fmaf(st, r2, dmz);
-            const float dq = -0.5f * st;
-            const float h0 = dq * dx, h1 = dq * dy, h2 = dq * dz;
-            const float h0x2 = h0 + h0, h1x2 = h1 + h1;
-            dc0 = __builtin_fmaf(h0, dx, dc0);
-            dc1 = __builtin_fmaf(h0x2, dy, dc1);
-            dc2 = __builtin_fmaf(h0x2, dz, dc2);
-            dc3 = __builtin_fmaf(h1, dy, dc3);
-            dc4 = __builtin_fmaf(h1x2, dz, dc4);
-            dc5 = __builtin_fmaf(h2, dz, dc5);
+            float4 a = s.ptsA[p];
+            float2 b = s.ptsB[p];
+            while (true) {
+                const bool more = m != 0;
+                const int pn = more ? __builtin_ctzll(m) : p;
+                const float4 an = s.ptsA[pn];
+                const float2 bn = s.ptsB[pn];
+                const float dx = a.x - ra.x, dy = a.y - ra.y, dz = a.z - ra.z;
+                float r0, r1, r2;
+                const float power = pair_power(rb.x, rb.y, rb.z, rb.w, rc.x, rc.y, dx, dy, dz, r0, r1, r2);
+                if (in_window(power)) {
+                    any = true;
+                    const float e = __builtin_amdgcn_exp2f(power * kLog2e);
+                    const float wgt = ra.w * e;
+                    dv = __builtin_fmaf(b.x, wgt, dv);
+                    const float t = __builtin_fmaf(b.x, rc.z - a.w, b.y);
+                    dw = __builtin_fmaf(t, e, dw);
+                    const float st = wgt * t;
+                    dmx = __builtin_fmaf(st, r0, dmx);
+                    dmy = __builtin_fmaf(st, r1, dmy);
+                    dmz = __builtin_fmaf(st, r2, dmz);
+                    const float dq = -0.5f * st;
+                    const float h0 = dq * dx, h1 = dq * dy, h2 = dq * dz;
+                    const float h0x2 = h0 + h0, h1x2 = h1 + h1;
+                    dc0 = __builtin_fmaf(h0, dx, dc0);
+                    dc1 = __builtin_fmaf(h0x2, dy, dc1);
+                    dc2 = __builtin_fmaf(h0x2, dz, dc2);
+                    dc3 = __builtin_fmaf(h1, dy, dc3);
+                    dc4 = __builtin_fmaf(h1x2, dz, dc4);
+                    dc5 = __builtin_fmaf(h2, dz, dc5);
+                }
+                if (!more) break;
+                m &= m - 1;
+                p = pn;
+                a = an;
+                b = bn;
+            }
         }
         const uint64_t bal = __ballot(any);
         if (any) {
@@ -874,7 +914,7 @@ struct SampleSetLayout {
     SampleSetLayout(Arena& a, int S)
     {
         header = a.take<SampleSetHeader>(1);
-        pts = a.take<float4>((size_t)S);
+        pts = a.take<float4>(((size_t)S + 63) / 64 * 64);
         keys = a.take<uint32_t>((size_t)S);
         idx = a.take<uint32_t>((size_t)S);
         keys_alt = a.take<uint32_t>((size_t)S);
@@ -987,12 +1027,20 @@ int sample_set_build(int S, const float* samples, void* set, size_t bytes, hipSt
         GVC_LAUNCH_CHECK("sample_codes");
         bool alt = false;
         GVC_TRY(radix_sort_u32(l.keys, l.idx, l.keys_alt, l.idx_alt, (size_t)S, 30, l.scratch, &alt, st));
-        sample_gather_kernel<<<blocks_for(S, 256), 256, 0, st>>>(samples, S, alt ? l.idx_alt : l.idx, l.pts);
+        const int S64 = NG * 64;
+        sample_gather_kernel<<<blocks_for(S64, 256), 256, 0, st>>>(samples, S, S64, alt ? l.idx_alt : l.idx, l.pts);
         GVC_LAUNCH_CHECK("sample_gather");
     }
     sample_header_kernel<<<1, 1, 0, st>>>(l.header, S, NG);
     GVC_LAUNCH_CHECK("sample_header");
     return GVC_OK;
+}
+
+const float4* sample_set_points(const void* sample_set, int S)
+{
+    Arena a(const_cast<void*>(sample_set), sample_set_bytes(S));
+    SampleSetLayout l(a, S);
+    return l.pts;
 }
 
 size_t forward_state_bytes(int P, int S, const gvc_params* prm)

@@ -57,17 +57,27 @@ class TetMeshSampler:
             return
         raise GvcError("tet locator build did not converge")
 
-    def sample(self, samples, with_values=True, with_bary=False):
-        s = torch.as_tensor(samples, dtype=torch.float32).to(self.device).contiguous()
-        if s.dim() != 2 or s.shape[1] != 3:
-            raise ValueError("samples must have shape (S, 3)")
-        S = int(s.shape[0])
+    def sample(self, samples, with_values=True, with_bary=False, sample_set=None):
+        """Locate ``samples`` [S,3]; with ``sample_set`` (a gvc.mixture.SampleSet of
+        the same points) they are processed in its Morton order."""
+        S = int(samples.shape[0]) if sample_set is None else sample_set.S
         ids = torch.empty(S, dtype=torch.int32, device=self.device)
         vals = torch.empty(S, dtype=torch.float64, device=self.device) if (with_values and self.values is not None) \
             else None
         bary = torch.empty((S, 4), dtype=torch.float32, device=self.device) if with_bary else None
-        call("gvc_tet_locate_interp", self.N, _ptr(self.points), self.C, _ptr(self.locator), self.locator_bytes,
-             self.capacity, _ptr(self.values), S, _ptr(s), _ptr(ids), _ptr(vals), _ptr(bary), _stream(self.device))
+        if sample_set is not None:
+            if sample_set.device != self.device:
+                raise ValueError("sample set on another device")
+            call("gvc_tet_locate_interp_set", self.N, _ptr(self.points), self.C, _ptr(self.locator),
+                 self.locator_bytes, self.capacity, _ptr(self.values), _ptr(sample_set.buffer), S, _ptr(ids),
+                 _ptr(vals), _ptr(bary), _stream(self.device))
+        else:
+            s = torch.as_tensor(samples, dtype=torch.float32).to(self.device).contiguous()
+            if s.dim() != 2 or s.shape[1] != 3:
+                raise ValueError("samples must have shape (S, 3)")
+            call("gvc_tet_locate_interp", self.N, _ptr(self.points), self.C, _ptr(self.locator), self.locator_bytes,
+                 self.capacity, _ptr(self.values), S, _ptr(s), _ptr(ids), _ptr(vals), _ptr(bary),
+                 _stream(self.device))
         if with_bary:
             return vals, ids, bary
         return vals, ids

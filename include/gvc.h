@@ -154,6 +154,15 @@ int gvc_tet_locate_interp(int n_points, const float *points, int n_tets, const v
                           int S, const float *samples, int32_t *cell_ids, double *values_out,
                           float *bary_out, void *stream);
 
+/* Same, for the points of a sample set (gvc_sample_set_build): they are processed
+ * in the set's Morton order (cache-friendly) and results land at the original
+ * sample index.  Lets a training step sort its samples once for both the
+ * sampler and the mixture. */
+int gvc_tet_locate_interp_set(int n_points, const float *points, int n_tets, const void *locator,
+                              size_t locator_bytes, uint64_t entry_capacity, const double *point_values,
+                              const void *sample_set, int S, int32_t *cell_ids, double *values_out,
+                              float *bary_out, void *stream);
+
 /* Uniform point grid (dims nx,ny,nz; origin; spacing -- HOST arrays), trilinear
  * interpolation, -1 outside cast to the value type (all-ones bits for the
  * unsigned types).  value_dtype: 0 f64, 1 f32, 2 u16, 3 u8; out has that type.

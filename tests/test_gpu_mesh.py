@@ -52,6 +52,12 @@ def test_tet_locate_bit_exact(n, jitter):
     got = gv.cpu().numpy()
     np.testing.assert_allclose(got[ok], out[ok], rtol=1e-9, atol=1e-12)
     assert np.all(got[~ok] == -1.0)
+    # same answers when the samples are processed in a sample set's Morton order
+    from gvc.mixture import SampleSet
+    st = torch.tensor(samples, device=DEV)
+    gv2, gid2 = sampler.sample(st, sample_set=SampleSet(st))
+    assert np.array_equal(gid2.cpu().numpy(), ids)
+    assert np.array_equal(gv2.cpu().numpy(), got)
 
 
 def test_gpu_sampleu_dropin():
