@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--graph", action="store_true", help="capture the training step in a hipGraph")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (x steps): the calibration launch for the "
+                         "FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
 
 
@@ -68,6 +71,16 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
+
+    if args.copy_probe:
+        n = args.copy_probe << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "launches": args.steps}))
+        return
 
     from bvh_diff_gaussian_rasterization import GaussianRasterizationSettings, GaussianRasterizer
     from gvc import mixture as mx

@@ -10,9 +10,18 @@
 //   exact zeros: p is perturbed to p + (e, e^2, e^3), so a point on a shared face,
 //   edge or vertex belongs to exactly one tet of a conforming mesh.  Among several
 //   containing tets (overlapping meshes) the lowest index wins.
-// Candidates come from a uniform grid over the mesh's bounding box: every tet is
-// listed in each grid cell its bounding box overlaps (CSR; order inside a cell is
-// irrelevant because the lowest containing index is taken).
+//
+// Locator: a uniform grid over the mesh's bounding box (about one cell per eight
+// tets).  Each cell lists every tet whose bounding box overlaps it as an 8-byte
+// entry: the tet id and that bounding box quantised conservatively to 1/32 of the
+// cell per axis (5 bits per bound), so most candidates are rejected from the
+// entry alone.  The survivors are tested with a per-tet barycentric matrix (float,
+// precomputed in double): b = M (p - v3); a decision is taken only when every
+// |b_i| exceeds a margin e_i = 1e-6 sum_j |M_ij d_j| + 2e-9 |M|_1 |d|_1 with
+// d = p - v3: the first term covers the float rounding of each M_ij (16x 2^-24),
+// the second the error of the double inverse, |dM_ij| <= 6 u cond_F(T) |M|_F with
+// u = 2^-53 and cond_F(T) <= 1e6 (a 300x margin).  Otherwise -- and always for
+// tets with cond_F(T) > 1e6 -- the exact predicate decides.  Order inside a cell is irrelevant (lowest index wins).
 #include <math.h>
 #include <string.h>
 
@@ -23,10 +32,10 @@ namespace gvc {
 
 struct alignas(16) TetRec {
     int v[4];
-    float lo[3], hi[3];
-    float pad[2];
+    float m[9];   // rows of inverse([v0-v3 | v1-v3 | v2-v3]); m[0] = NaN: exact test only
+    float v3[3];
 };
-static_assert(sizeof(TetRec) == 48, "TetRec layout");
+static_assert(sizeof(TetRec) == 64, "TetRec layout");
 
 struct alignas(16) TetGridHeader {
     float lo[3];
@@ -34,7 +43,7 @@ struct alignas(16) TetGridHeader {
     int dims[3];
     int n_tets;
     int n_points;
-    int pad[3];
+    float hi[3];    // bounding box of the mesh points: no tet reaches outside it
     unsigned long long n_entries;
     unsigned long long capacity;
 };
@@ -43,24 +52,50 @@ namespace {
 
 inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
+// Per-tet barycentric matrix, computed in double from the float vertices.
 __global__ void tet_records_kernel(const float* __restrict__ pts, const int* __restrict__ tets, int C,
-                                   TetRec* __restrict__ recs)
+                                   TetRec* __restrict__ recs, float* __restrict__ boxes)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= C) return;
     TetRec r;
     for (int k = 0; k < 4; ++k) r.v[k] = tets[4 * t + k];
+    double V[4][3];
+    for (int k = 0; k < 4; ++k)
+        for (int a = 0; a < 3; ++a) V[k][a] = pts[3 * r.v[k] + a];
     for (int a = 0; a < 3; ++a) {
+        r.v3[a] = (float)V[3][a];
         float lo = pts[3 * r.v[0] + a], hi = lo;
         for (int k = 1; k < 4; ++k) {
             float x = pts[3 * r.v[k] + a];
             lo = fminf(lo, x);
             hi = fmaxf(hi, x);
         }
-        r.lo[a] = lo;
-        r.hi[a] = hi;
+        boxes[6 * t + a] = lo;
+        boxes[6 * t + 3 + a] = hi;
     }
-    r.pad[0] = r.pad[1] = 0.f;
+    // T columns: v0-v3, v1-v3, v2-v3
+    double T[3][3];
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) T[a][c] = V[c][a] - V[3][a];
+    const double A00 = T[1][1] * T[2][2] - T[1][2] * T[2][1];
+    const double A01 = T[0][2] * T[2][1] - T[0][1] * T[2][2];
+    const double A02 = T[0][1] * T[1][2] - T[0][2] * T[1][1];
+    const double A10 = T[1][2] * T[2][0] - T[1][0] * T[2][2];
+    const double A11 = T[0][0] * T[2][2] - T[0][2] * T[2][0];
+    const double A12 = T[0][2] * T[1][0] - T[0][0] * T[1][2];
+    const double A20 = T[1][0] * T[2][1] - T[1][1] * T[2][0];
+    const double A21 = T[0][1] * T[2][0] - T[0][0] * T[2][1];
+    const double A22 = T[0][0] * T[1][1] - T[0][1] * T[1][0];
+    const double det = T[0][0] * A00 + T[0][1] * A10 + T[0][2] * A20;
+    const double inv[9] = { A00 / det, A01 / det, A02 / det, A10 / det, A11 / det,
+                            A12 / det, A20 / det, A21 / det, A22 / det };
+    double nT = 0.0, nI = 0.0;
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) nT += T[a][c] * T[a][c];
+    for (int k = 0; k < 9; ++k) nI += inv[k] * inv[k];
+    const bool ok = det != 0.0 && isfinite(det) && sqrt(nT * nI) <= 1e6;
+    for (int k = 0; k < 9; ++k) r.m[k] = ok ? (float)inv[k] : __builtin_nanf("");
     recs[t] = r;
 }
 
@@ -71,17 +106,27 @@ __device__ __forceinline__ int cell_coord(float x, float lo, float cell, int dim
     return c < 0 ? 0 : (c >= dim ? dim - 1 : c);
 }
 
-__global__ void tet_count_kernel(const TetRec* __restrict__ recs, int C, const TetGridHeader* __restrict__ h,
+// 5-bit position of x inside the cell [clo, clo + cell): floor(32 u), clamped.
+// Monotone in x for a fixed cell, so quantised bounds stay conservative.
+__device__ __forceinline__ uint32_t quant5(float x, float clo, float cell)
+{
+    const float u = (x - clo) / cell * 32.0f;
+    if (!(u > 0.0f)) return 0u;
+    if (u >= 31.0f) return 31u;
+    return (uint32_t)u;
+}
+
+__global__ void tet_count_kernel(const float* __restrict__ boxes, int C, const TetGridHeader* __restrict__ h,
                                  uint32_t* __restrict__ counts)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= C) return;
-    const TetRec r = recs[t];
-    if (!(r.lo[0] == r.lo[0] && r.lo[1] == r.lo[1] && r.lo[2] == r.lo[2])) return;
+    const float* b = boxes + 6 * t;
+    if (!(b[0] == b[0] && b[1] == b[1] && b[2] == b[2])) return;
     int c0[3], c1[3];
     for (int a = 0; a < 3; ++a) {
-        c0[a] = cell_coord(r.lo[a], h->lo[a], h->cell[a], h->dims[a]);
-        c1[a] = cell_coord(r.hi[a], h->lo[a], h->cell[a], h->dims[a]);
+        c0[a] = cell_coord(b[a], h->lo[a], h->cell[a], h->dims[a]);
+        c1[a] = cell_coord(b[3 + a], h->lo[a], h->cell[a], h->dims[a]);
     }
     for (int k = c0[2]; k <= c1[2]; ++k)
         for (int j = c0[1]; j <= c1[1]; ++j)
@@ -89,37 +134,43 @@ __global__ void tet_count_kernel(const TetRec* __restrict__ recs, int C, const T
                 atomicAdd(&counts[(size_t)i + (size_t)h->dims[0] * ((size_t)j + (size_t)h->dims[1] * k)], 1u);
 }
 
-__global__ void tet_fill_kernel(const TetRec* __restrict__ recs, int C, const TetGridHeader* __restrict__ h,
+// entry = (tet id, quantised box): bits 0-4 lo.x, 5-9 lo.y, 10-14 lo.z, 15-19 hi.x,
+// 20-24 hi.y, 25-29 hi.z, each as quant5 relative to the entry's cell.
+__global__ void tet_fill_kernel(const float* __restrict__ boxes, int C, const TetGridHeader* __restrict__ h,
                                 const uint32_t* __restrict__ starts, uint32_t* __restrict__ cursor,
-                                int* __restrict__ entries, unsigned long long capacity)
+                                uint2* __restrict__ entries, unsigned long long capacity)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= C) return;
-    const TetRec r = recs[t];
-    if (!(r.lo[0] == r.lo[0] && r.lo[1] == r.lo[1] && r.lo[2] == r.lo[2])) return;
+    const float* b = boxes + 6 * t;
+    if (!(b[0] == b[0] && b[1] == b[1] && b[2] == b[2])) return;
     int c0[3], c1[3];
     for (int a = 0; a < 3; ++a) {
-        c0[a] = cell_coord(r.lo[a], h->lo[a], h->cell[a], h->dims[a]);
-        c1[a] = cell_coord(r.hi[a], h->lo[a], h->cell[a], h->dims[a]);
+        c0[a] = cell_coord(b[a], h->lo[a], h->cell[a], h->dims[a]);
+        c1[a] = cell_coord(b[3 + a], h->lo[a], h->cell[a], h->dims[a]);
     }
     for (int k = c0[2]; k <= c1[2]; ++k)
         for (int j = c0[1]; j <= c1[1]; ++j)
             for (int i = c0[0]; i <= c1[0]; ++i) {
+                const int ci[3] = { i, j, k };
+                uint32_t q = 0;
+                for (int a = 0; a < 3; ++a) {
+                    const float clo = h->lo[a] + (float)ci[a] * h->cell[a];
+                    q |= quant5(b[a], clo, h->cell[a]) << (5 * a);
+                    q |= quant5(b[3 + a], clo, h->cell[a]) << (15 + 5 * a);
+                }
                 const size_t g = (size_t)i + (size_t)h->dims[0] * ((size_t)j + (size_t)h->dims[1] * k);
                 const unsigned long long pos = (unsigned long long)starts[g] + atomicAdd(&cursor[g], 1u);
-                if (pos < capacity) entries[pos] = t;
+                if (pos < capacity) entries[pos] = make_uint2((uint32_t)t, q);
             }
 }
 
-// Containment of p in tet r with the simulation-of-simplicity tie rule; on
-// success writes the four barycentric coordinates.
-__device__ bool tet_contains(const float* __restrict__ pts, const TetRec& r, const float p[3], double bary[4])
+// Exact containment of p in tet r with the simulation-of-simplicity tie rule.
+__device__ __noinline__ bool tet_contains_exact(const float* __restrict__ pts, const TetRec& r, const float* p)
 {
     const float* V[4] = { pts + 3 * r.v[0], pts + 3 * r.v[1], pts + 3 * r.v[2], pts + 3 * r.v[3] };
-    double ovol;
-    const int sT = orient3d_sign(V[0], V[1], V[2], V[3], &ovol);
+    const int sT = orient3d_sign(V[0], V[1], V[2], V[3], nullptr);
     if (sT == 0) return false;
-    double o[4];
     for (int i = 0; i < 4; ++i) {
         // face opposite i, remaining vertices in increasing order; o_i = s_i orient(F, p)
         const float* F[3];
@@ -127,25 +178,70 @@ __device__ bool tet_contains(const float* __restrict__ pts, const TetRec& r, con
         for (int k = 0; k < 4; ++k)
             if (k != i) F[m++] = V[k];
         const int si = ((3 - i) & 1) ? -1 : 1;
-        double val;
-        int sg = orient3d_sign(F[0], F[1], F[2], p, &val);
+        int sg = orient3d_sign(F[0], F[1], F[2], p, nullptr);
         if (sg == 0) {
             // d/dp orient(F, p) = -(F1 - F0) x (F2 - F0); p + (e, e^2, e^3)
             for (int a = 0; a < 3 && sg == 0; ++a) sg = -normal_sign(F[0], F[1], F[2], a);
         }
         if (si * sg != sT) return false;
-        o[i] = si * val;
     }
-    for (int i = 0; i < 4; ++i) bary[i] = o[i] / ovol;
     return true;
 }
 
+__device__ __forceinline__ double orient3d_value(const float* a, const float* b, const float* c, const float* d)
+{
+    const double adx = (double)a[0] - d[0], ady = (double)a[1] - d[1], adz = (double)a[2] - d[2];
+    const double bdx = (double)b[0] - d[0], bdy = (double)b[1] - d[1], bdz = (double)b[2] - d[2];
+    const double cdx = (double)c[0] - d[0], cdy = (double)c[1] - d[1], cdz = (double)c[2] - d[2];
+    return adz * (bdx * cdy - bdy * cdx) + bdz * (cdx * ady - cdy * adx) + cdz * (adx * bdy - ady * bdx);
+}
+
+// Barycentric coordinates of p in r from double orientation volumes.
+__device__ void tet_bary(const float* __restrict__ pts, const TetRec& r, const float* p, double b[4])
+{
+    const float* V[4] = { pts + 3 * r.v[0], pts + 3 * r.v[1], pts + 3 * r.v[2], pts + 3 * r.v[3] };
+    const double vol = orient3d_value(V[0], V[1], V[2], V[3]);
+    b[0] = -orient3d_value(V[1], V[2], V[3], p) / vol;
+    b[1] = orient3d_value(V[0], V[2], V[3], p) / vol;
+    b[2] = -orient3d_value(V[0], V[1], V[3], p) / vol;
+    b[3] = orient3d_value(V[0], V[1], V[2], p) / vol;
+}
+
+// 1 inside, 0 outside, -1 undecided (exact test needed).
+__device__ __forceinline__ int tet_filter(const TetRec& r, const float* p)
+{
+    if (r.m[0] != r.m[0]) return -1;
+    const double dx = (double)p[0] - r.v3[0], dy = (double)p[1] - r.v3[1], dz = (double)p[2] - r.v3[2];
+    float l1m = 0.0f;
+    for (int k = 0; k < 9; ++k) l1m += fabsf(r.m[k]);
+    const double eabs = 2e-9 * (double)l1m * (fabs(dx) + fabs(dy) + fabs(dz));
+    double b[4], e[4];
+    for (int i = 0; i < 3; ++i) {
+        const double t0 = (double)r.m[3 * i] * dx, t1 = (double)r.m[3 * i + 1] * dy, t2 = (double)r.m[3 * i + 2] * dz;
+        b[i] = t0 + t1 + t2;
+        e[i] = 1e-6 * (fabs(t0) + fabs(t1) + fabs(t2)) + eabs;
+    }
+    b[3] = 1.0 - b[0] - b[1] - b[2];
+    e[3] = e[0] + e[1] + e[2] + 1e-15;
+    bool all_in = true;
+    for (int i = 0; i < 4; ++i) {
+        if (b[i] < -e[i]) return 0;
+        all_in = all_in && (b[i] > e[i]);
+    }
+    return all_in ? 1 : -1;
+}
+
+constexpr int kNeedExact = -2147483647 - 1;
+
 // sorted != nullptr: the samples are taken from a sample set (Morton order, so
 // neighbouring threads share grid cells and tets) and results go to the original
-// index carried in .w.
+// index carried in .w.  EXACT = false: candidates the filter cannot decide mark
+// the sample kNeedExact (no result written); EXACT = true: only those samples are
+// redone, the undecided candidates by the exact predicate.
+template <bool EXACT>
 __global__ void tet_locate_kernel(const float* __restrict__ pts, const TetRec* __restrict__ recs,
                                   const TetGridHeader* __restrict__ h, const uint32_t* __restrict__ starts,
-                                  const int* __restrict__ entries, const double* __restrict__ values, int S,
+                                  const uint2* __restrict__ entries, const double* __restrict__ values, int S,
                                   const float* __restrict__ samples, const float4* __restrict__ sorted,
                                   int* __restrict__ cell_ids, double* __restrict__ out_values,
                                   float* __restrict__ bary_out)
@@ -162,34 +258,50 @@ __global__ void tet_locate_kernel(const float* __restrict__ pts, const TetRec* _
     } else {
         p[0] = samples[3 * s]; p[1] = samples[3 * s + 1]; p[2] = samples[3 * s + 2];
     }
+    if (EXACT && cell_ids[s] != kNeedExact) return;
     int best = -1;
-    double bb[4] = { 0, 0, 0, 0 };
+    bool undecided = false;
     bool inside_grid = true;
     int c[3];
+    uint32_t qp[3];
     for (int a = 0; a < 3; ++a) {
+        if (!(p[a] >= h->lo[a] && p[a] <= h->hi[a])) inside_grid = false;
         const float t = (p[a] - h->lo[a]) / h->cell[a];
-        if (!(t >= 0.0f) || !(t <= (float)h->dims[a])) inside_grid = false;
-        int ci = (int)floorf(t);
+        int ci = inside_grid ? (int)floorf(t) : 0;
         c[a] = ci < 0 ? 0 : (ci >= h->dims[a] ? h->dims[a] - 1 : ci);
+        qp[a] = quant5(p[a], h->lo[a] + (float)c[a] * h->cell[a], h->cell[a]);
     }
     if (inside_grid) {
         const size_t g = (size_t)c[0] + (size_t)h->dims[0] * ((size_t)c[1] + (size_t)h->dims[1] * c[2]);
         const uint32_t b = starts[g], e = starts[g + 1];
         for (uint32_t k = b; k < e; ++k) {
-            const int t = entries[k];
+            const uint2 en = entries[k];
+            const int t = (int)en.x;
             if (best >= 0 && t > best) continue;
-            const TetRec r = recs[t];
-            if (p[0] < r.lo[0] || p[0] > r.hi[0] || p[1] < r.lo[1] || p[1] > r.hi[1] || p[2] < r.lo[2] ||
-                p[2] > r.hi[2])
+            const uint32_t q = en.y;
+            if (qp[0] < (q & 31u) || qp[0] > ((q >> 15) & 31u) || qp[1] < ((q >> 5) & 31u) ||
+                qp[1] > ((q >> 20) & 31u) || qp[2] < ((q >> 10) & 31u) || qp[2] > ((q >> 25) & 31u))
                 continue;
-            double bc[4];
-            if (tet_contains(pts, r, p, bc)) {
+            const TetRec r = recs[t];
+            const int f = tet_filter(r, p);
+            if (f == 1) {
                 best = t;
-                for (int i = 0; i < 4; ++i) bb[i] = bc[i];
+            } else if (f == -1) {
+                if (EXACT) {
+                    if (tet_contains_exact(pts, r, p)) best = t;
+                } else {
+                    undecided = true;
+                }
             }
         }
     }
+    if (!EXACT && undecided) {
+        cell_ids[s] = kNeedExact;
+        return;
+    }
     cell_ids[s] = best;
+    double bb[4] = { 0, 0, 0, 0 };
+    if (best >= 0 && (out_values || bary_out)) tet_bary(pts, recs[best], p, bb);
     if (out_values) {
         double v = -1.0;
         if (best >= 0 && values) {
@@ -199,7 +311,7 @@ __global__ void tet_locate_kernel(const float* __restrict__ pts, const TetRec* _
         out_values[s] = v;
     }
     if (bary_out)
-        for (int i = 0; i < 4; ++i) bary_out[4 * s + i] = (float)bb[i];
+        for (int k = 0; k < 4; ++k) bary_out[4 * s + k] = (float)bb[k];
 }
 
 // Layout: everything the locate kernel reads sits at offsets fixed by (C, cap);
@@ -208,25 +320,27 @@ struct TetLayout {
     TetGridHeader* header;
     unsigned* bbox;
     TetRec* recs;
-    int* entries;
+    uint2* entries;
+    float* boxes;
     uint32_t *starts, *cursor, *scan_scratch;
     TetLayout(Arena& a, int C, size_t n_grid_max, unsigned long long cap)
     {
         header = a.take<TetGridHeader>(1);
         bbox = a.take<unsigned>(8);
         recs = a.take<TetRec>((size_t)(C > 0 ? C : 1));
-        entries = a.take<int>((size_t)(cap > 0 ? cap : 1));
+        entries = a.take<uint2>((size_t)(cap > 0 ? cap : 1));
         starts = a.take<uint32_t>(n_grid_max + 1);
         cursor = a.take<uint32_t>(n_grid_max);
         scan_scratch = a.take<uint32_t>(scan_scratch_elems(n_grid_max) + 1);
+        boxes = a.take<float>((size_t)(C > 0 ? C : 1) * 6);
     }
 };
 
-// Grid resolution: about one grid cell per two tets (at most 2^26), cells as
+// Grid resolution: about one grid cell per eight tets (at most 2^26), cells as
 // cubic as the bounding box allows; the cell count never exceeds max_grid_cells.
 double grid_target(int C)
 {
-    double target = C > 2 ? C / 2.0 : 1.0;
+    double target = C > 8 ? C / 8.0 : 1.0;
     return target > 67108864.0 ? 67108864.0 : target;
 }
 size_t max_grid_cells(int C) { return (size_t)(2.0 * grid_target(C)) + 8; }
@@ -285,18 +399,18 @@ int tet_locator_build(int n_points, const float* pts, int C, const int* tets, un
     Arena a(loc, bytes);
     TetLayout l(a, C, max_grid_cells(C), cap);
     if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "tet locator buffer too small");
-    TetLayout& l0 = l;
-    GVC_TRY(bbox_launch(pts, n_points, l0.bbox, st));
+    GVC_TRY(bbox_launch(pts, n_points, l.bbox, st));
     unsigned hb[6];
-    GVC_HIP(hipMemcpyAsync(hb, l0.bbox, sizeof(hb), hipMemcpyDeviceToHost, st));
+    GVC_HIP(hipMemcpyAsync(hb, l.bbox, sizeof(hb), hipMemcpyDeviceToHost, st));
     GVC_HIP(hipStreamSynchronize(st));
     TetGridHeader H;
     memset(&H, 0, sizeof(H));
     float lo[3], hi[3];
     bbox_from_keys(hb, lo, hi);
-    for (int a = 0; a < 3; ++a) {
-        if (!(lo[a] <= hi[a])) return set_error(GVC_ERR_INVALID_ARGUMENT, "mesh points have no finite bounding box");
-        H.lo[a] = lo[a];
+    for (int a2 = 0; a2 < 3; ++a2) {
+        if (!(lo[a2] <= hi[a2])) return set_error(GVC_ERR_INVALID_ARGUMENT, "mesh points have no finite bounding box");
+        H.lo[a2] = lo[a2];
+        H.hi[a2] = hi[a2];
     }
     grid_dims(C, lo, hi, H.dims, H.cell);
     H.n_tets = C;
@@ -304,10 +418,10 @@ int tet_locator_build(int n_points, const float* pts, int C, const int* tets, un
     H.capacity = cap;
     const size_t n_grid = (size_t)H.dims[0] * H.dims[1] * H.dims[2];
     GVC_HIP(hipMemcpyAsync(l.header, &H, sizeof(H), hipMemcpyHostToDevice, st));
-    tet_records_kernel<<<blocks_for(C, 256), 256, 0, st>>>(pts, tets, C, l.recs);
+    tet_records_kernel<<<blocks_for(C, 256), 256, 0, st>>>(pts, tets, C, l.recs, l.boxes);
     GVC_LAUNCH_CHECK("tet_records");
     GVC_TRY(fill_u32(l.starts, 0u, n_grid + 1, st));
-    tet_count_kernel<<<blocks_for(C, 256), 256, 0, st>>>(l.recs, C, l.header, l.starts);
+    tet_count_kernel<<<blocks_for(C, 256), 256, 0, st>>>(l.boxes, C, l.header, l.starts);
     GVC_LAUNCH_CHECK("tet_count");
     GVC_TRY(exclusive_scan_u32(l.starts, l.starts, n_grid, true, l.scan_scratch, st));
     uint32_t total = 0;
@@ -316,7 +430,7 @@ int tet_locator_build(int n_points, const float* pts, int C, const int* tets, un
     if (needed) *needed = total;
     if (total > cap) return set_error(GVC_ERR_CAPACITY, "tet locator entry capacity too small");
     GVC_TRY(fill_u32(l.cursor, 0u, n_grid, st));
-    tet_fill_kernel<<<blocks_for(C, 256), 256, 0, st>>>(l.recs, C, l.header, l.starts, l.cursor, l.entries, cap);
+    tet_fill_kernel<<<blocks_for(C, 256), 256, 0, st>>>(l.boxes, C, l.header, l.starts, l.cursor, l.entries, cap);
     GVC_LAUNCH_CHECK("tet_fill");
     H.n_entries = total;
     GVC_HIP(hipMemcpyAsync(l.header, &H, sizeof(H), hipMemcpyHostToDevice, st));
@@ -334,9 +448,12 @@ int tet_locate_interp(int n_points, const float* pts, int C, const void* loc, si
     TetLayout l(a, C, max_grid_cells(C), cap);
     if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "tet locator buffer too small");
     const float4* sorted = sample_set ? sample_set_points(sample_set, S) : nullptr;
-    tet_locate_kernel<<<blocks_for(S, 128), 128, 0, st>>>(pts, l.recs, l.header, l.starts, l.entries, values, S,
-                                                          samples, sorted, cell_ids, out_values, bary);
+    tet_locate_kernel<false><<<blocks_for(S, 256), 256, 0, st>>>(pts, l.recs, l.header, l.starts, l.entries, values,
+                                                                 S, samples, sorted, cell_ids, out_values, bary);
     GVC_LAUNCH_CHECK("tet_locate");
+    tet_locate_kernel<true><<<blocks_for(S, 256), 256, 0, st>>>(pts, l.recs, l.header, l.starts, l.entries, values,
+                                                                S, samples, sorted, cell_ids, out_values, bary);
+    GVC_LAUNCH_CHECK("tet_locate_exact");
     return GVC_OK;
 }
 
