@@ -63,14 +63,6 @@ __device__ __forceinline__ uint32_t wave_sumu(uint32_t v)
     return v;
 }
 
-// Blocks are dealt round-robin to the 8 XCDs; remap so each XCD (own L2) works on
-// one contiguous, spatially coherent range of groups.
-__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
-{
-    const unsigned x = b & 7u, q = nb >> 3, r = nb & 7u;
-    return x * q + (x < r ? x : r) + (b >> 3);
-}
-
 __device__ __forceinline__ uint32_t spread10(uint32_t v)
 {
     v &= 0x3ffu;
@@ -124,19 +116,27 @@ struct SampleSetHeader {
 };
 constexpr int kSampleMagic = 0x47564353;
 
+// 30-bit Morton key of a sample: coordinates clamped to the unit cube (NaN -> 0),
+// 10 bits per axis.
+__device__ __forceinline__ uint32_t sample_code(float x, float y, float z)
+{
+    const float v[3] = { x, y, z };
+    uint32_t c = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float t = v[a];
+        t = (t == t) ? fminf(fmaxf(t, 0.0f), 1.0f) : 0.0f;
+        c |= spread10((uint32_t)(t * 1023.0f)) << a;
+    }
+    return c;
+}
+
 __global__ void sample_codes_kernel(const float* __restrict__ s, int S, uint32_t* __restrict__ keys,
                                     uint32_t* __restrict__ idx)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= S) return;
-    uint32_t c = 0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        float t = s[3 * i + a];
-        t = (t == t) ? fminf(fmaxf(t, 0.0f), 1.0f) : 0.0f;
-        c |= spread10((uint32_t)(t * 1023.0f)) << a;
-    }
-    keys[i] = c;
+    keys[i] = sample_code(s[3 * i], s[3 * i + 1], s[3 * i + 2]);
     idx[i] = (uint32_t)i;
 }
 
@@ -176,32 +176,47 @@ __global__ void gaussian_codes_kernel(const float* __restrict__ means, int P, ui
     idx[i] = (uint32_t)i;
 }
 
-// Canonical preprocess, written as records in BVH (sorted) order.
+// Canonical preprocess, written as records in BVH (sorted) order.  Also sums the
+// box extents per axis in units of 2^-20 (integer adds: the same total in any
+// order), the walks' estimate of a Gaussian's reach when splitting groups.
 __global__ void record_kernel(int P, const uint32_t* __restrict__ perm, const float* __restrict__ means,
                               const float* __restrict__ scales, const float* __restrict__ rots,
                               const float* __restrict__ values, const float* __restrict__ weights, float k,
-                              int fp64_cond, GRecord* __restrict__ rec, float* __restrict__ conics_out)
+                              int fp64_cond, GRecord* __restrict__ rec, float* __restrict__ conics_out,
+                              unsigned long long* __restrict__ ext_sum)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P) return;
-    const int j = (int)perm[i];
-    float mu[3] = { means[3 * j], means[3 * j + 1], means[3 * j + 2] };
-    float sc[3] = { scales[3 * j], scales[3 * j + 1], scales[3 * j + 2] };
-    float q[4] = { rots[4 * j], rots[4 * j + 1], rots[4 * j + 2], rots[4 * j + 3] };
-    const float w = weights[j];
-    float cn[6], bx[6];
-    preprocess_one(mu, sc, q, w, k, fp64_cond != 0, cn, bx);
-    GRecord r;
-    r.mx = mu[0]; r.my = mu[1]; r.mz = mu[2]; r.w = w;
-    r.c0 = cn[0]; r.c1 = cn[1]; r.c2 = cn[2]; r.c3 = cn[3]; r.c4 = cn[4]; r.c5 = cn[5];
-    r.v = values[j];
-    r.id = j;
-    r.lox = bx[0]; r.loy = bx[1]; r.loz = bx[2];
-    r.hix = bx[3]; r.hiy = bx[4]; r.hiz = bx[5];
-    r.pad0 = 0.f; r.pad1 = 0.f;
-    rec[i] = r;
-    if (conics_out)
-        for (int c = 0; c < 6; ++c) conics_out[6 * j + c] = cn[c];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t ex[3] = { 0, 0, 0 };
+    if (i < P) {
+        const int j = (int)perm[i];
+        float mu[3] = { means[3 * j], means[3 * j + 1], means[3 * j + 2] };
+        float sc[3] = { scales[3 * j], scales[3 * j + 1], scales[3 * j + 2] };
+        float q[4] = { rots[4 * j], rots[4 * j + 1], rots[4 * j + 2], rots[4 * j + 3] };
+        const float w = weights[j];
+        float cn[6], bx[6];
+        preprocess_one(mu, sc, q, w, k, fp64_cond != 0, cn, bx);
+        GRecord r;
+        r.mx = mu[0]; r.my = mu[1]; r.mz = mu[2]; r.w = w;
+        r.c0 = cn[0]; r.c1 = cn[1]; r.c2 = cn[2]; r.c3 = cn[3]; r.c4 = cn[4]; r.c5 = cn[5];
+        r.v = values[j];
+        r.id = j;
+        r.lox = bx[0]; r.loy = bx[1]; r.loz = bx[2];
+        r.hix = bx[3]; r.hiy = bx[4]; r.hiz = bx[5];
+        r.pad0 = 0.f; r.pad1 = 0.f;
+        rec[i] = r;
+        if (conics_out)
+            for (int c = 0; c < 6; ++c) conics_out[6 * j + c] = cn[c];
+        for (int a = 0; a < 3; ++a) {
+            const float e = bx[3 + a] - bx[a];
+            ex[a] = (e >= 0.0f) ? (uint64_t)(fminf(e, 1024.0f) * 1048576.0f) : 0ull;
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        uint64_t v = ex[a];
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) v += (uint64_t)__shfl_xor((long long)v, d, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&ext_sum[a], (unsigned long long)v);
+    }
 }
 
 __global__ void leaf_codes_kernel(const uint64_t* __restrict__ sorted_codes, int NL, int L,
@@ -367,6 +382,100 @@ __device__ __forceinline__ void lds_fence()
     __builtin_amdgcn_wave_barrier();
 }
 
+// Box of the points of the lanes with `in` set (wave-uniform result).
+__device__ __forceinline__ GroupBox masked_box(float px, float py, float pz, bool in)
+{
+    const float inf = __builtin_inff();
+    GroupBox b;
+    b.lx = uniformf(wave_minf(in ? px : inf));
+    b.ly = uniformf(wave_minf(in ? py : inf));
+    b.lz = uniformf(wave_minf(in ? pz : inf));
+    b.hx = uniformf(wave_maxf(in ? px : -inf));
+    b.hy = uniformf(wave_maxf(in ? py : -inf));
+    b.hz = uniformf(wave_maxf(in ? pz : -inf));
+    return b;
+}
+
+// Expected candidate count of a box, up to a constant factor: the volume of the
+// box grown by the mean Gaussian box extent r.
+__device__ __forceinline__ float box_cost(const GroupBox& b, const float r[3])
+{
+    return (b.hx - b.lx + r[0]) * (b.hy - b.ly + r[1]) * (b.hz - b.lz + r[2]);
+}
+
+// Runs: a group's valid lanes [0, nv) are cut into at most 8 runs of consecutive
+// lanes, each walked with its own box.  Consecutive Morton-sorted points can
+// straddle a coarse Morton boundary; their common box is then much larger than
+// the point clouds on either side.  A run [a, b) is cut where its first and last
+// keys first differ (the keys are sorted, so the lanes with that bit set form a
+// suffix) when the two halves' cost is below 3/4 of the whole's.  Returns the
+// run starts as a lane bit mask.  Any partition gives the same sums; this one
+// depends only on the group's points and the mean Gaussian extent.
+__device__ __forceinline__ uint64_t split_runs(float px, float py, float pz, int nv, const float r[3])
+{
+    const int lane = threadIdx.x & 63;
+    if (nv <= 0) return 0ull;
+    uint64_t starts = 1ull;
+    if (nv < 16) return starts;
+    const uint32_t code = lane < nv ? sample_code(px, py, pz) : 0xffffffffu;
+    uint64_t pending = 1ull;
+    int tests = 0;
+    while (pending != 0ull && tests < 7) {
+        const int a = __builtin_ctzll(pending);
+        pending &= pending - 1;
+        const uint64_t after = starts & ~((2ull << a) - 1ull);
+        const int b = after ? __builtin_ctzll(after) : nv;
+        if (b - a < 16) continue;
+        const uint32_t ca = (uint32_t)__builtin_amdgcn_readlane((int)code, a);
+        const uint32_t cb = (uint32_t)__builtin_amdgcn_readlane((int)code, b - 1);
+        if (ca >= cb) continue;
+        ++tests;
+        const int hb = 31 - __builtin_clz(ca ^ cb);
+        const bool inrun = lane >= a && lane < b;
+        const uint64_t upper = __ballot(inrun && ((code >> hb) & 1u));
+        const int sp = upper ? __builtin_ctzll(upper) : b;
+        if (sp <= a || sp >= b) continue;
+        const GroupBox L = masked_box(px, py, pz, inrun && lane < sp);
+        const GroupBox R = masked_box(px, py, pz, inrun && lane >= sp);
+        GroupBox B;
+        B.lx = fminf(L.lx, R.lx); B.ly = fminf(L.ly, R.ly); B.lz = fminf(L.lz, R.lz);
+        B.hx = fmaxf(L.hx, R.hx); B.hy = fmaxf(L.hy, R.hy); B.hz = fmaxf(L.hz, R.hz);
+        if (box_cost(L, r) + box_cost(R, r) < 0.75f * box_cost(B, r)) {
+            starts |= 1ull << sp;
+            pending |= (1ull << a) | (1ull << sp);
+        }
+    }
+    return starts;
+}
+
+// Dynamic group scheduling: the groups are cut into 8 contiguous slices, one per
+// XCD (workgroups are dealt round-robin to the XCDs, so blockIdx & 7 names the
+// wave's XCD); a wave takes the next group of its slice and, once that slice is
+// exhausted, moves on to the following slices.  Returns NG when all are done.
+// queue[8] must be zero at launch.
+__device__ __forceinline__ int next_group(uint32_t* __restrict__ queue, int NG, int& x, int& tries)
+{
+    const int lane = threadIdx.x & 63;
+    while (tries < 8) {
+        const uint32_t b = (uint32_t)(((uint64_t)NG * (uint32_t)x) >> 3);
+        const uint32_t e = (uint32_t)(((uint64_t)NG * (uint32_t)(x + 1)) >> 3);
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(&queue[x], 1u);
+        t = (uint32_t)__shfl((int)t, 0, 64);
+        t = (uint32_t)uniformi((int)t);
+        if (t < e - b) return (int)(b + t);
+        x = (x + 1) & 7;
+        ++tries;
+    }
+    return NG;
+}
+
+__device__ __forceinline__ void mean_extent(const unsigned long long* __restrict__ ext_sum, int P, float r[3])
+{
+    for (int a = 0; a < 3; ++a)
+        r[a] = uniformf((float)((double)ext_sum[a] * (1.0 / 1048576.0) / (double)(P > 0 ? P : 1)));
+}
+
 // Candidate enumeration.  chunk(base, n) is called with the ring positions
 // [base, base+n) (n <= 64, modulo kRing) holding Gaussian indices.
 template <class ChunkFn>
@@ -523,106 +632,124 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
                                                       const float4* __restrict__ pts, int S, int NG,
                                                       float* __restrict__ out_cells, float* __restrict__ out_weights,
                                                       uint32_t* __restrict__ group_records,
-                                                      uint32_t* __restrict__ group_pairs, uint32_t* __restrict__ err)
+                                                      uint32_t* __restrict__ group_pairs, uint32_t* __restrict__ err,
+                                                      uint32_t* __restrict__ queue,
+                                                      const unsigned long long* __restrict__ ext_sum)
 {
     __shared__ FwdLds lds_all[4];
     const int wid = uniformi(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int g = uniformi((int)xcd_block(blockIdx.x, gridDim.x) * 4 + wid);
-    if (g >= NG) return;
     FwdLds& s = lds_all[wid];
-    const float4* __restrict__ gp = pts + (size_t)g * 64;
-    const float4 pme = gp[lane];
-    const float px = pme.x, py = pme.y, pz = pme.z;
-    const int orig = __builtin_bit_cast(int, pme.w);
-    const GroupBox gb = group_box(px, py, pz);
+    float reach[3];
+    mean_extent(ext_sum, P, reach);
+    int x = (int)(blockIdx.x & 7u), tries = 0;
+    bool ok = true;
+    while (true) {
+        const int g = next_group(queue, NG, x, tries);
+        if (g >= NG) break;
+        const float4* __restrict__ gp = pts + (size_t)g * 64;
+        const float4 pme = gp[lane];
+        const float px = pme.x, py = pme.y, pz = pme.z;
+        const int orig = __builtin_bit_cast(int, pme.w);
+        const int nv = __popcll(__ballot(orig >= 0));
+        const uint64_t starts = split_runs(px, py, pz, nv, reach);
 
-    float W = 0.f, cW = 0.f, U = 0.f, cU = 0.f;
-    uint32_t npairs = 0, nrec = 0;
+        float W = 0.f, cW = 0.f, U = 0.f, cU = 0.f;
+        uint32_t npairs = 0, nrec = 0;
+        bool inrun = false;
 
-    auto chunk = [&](uint32_t base, int n) {
-        float lox = 1.f, loy = 1.f, loz = 1.f, hix = 0.f, hiy = 0.f, hiz = 0.f;  // empty box for idle lanes
-        if (lane < n) {
-            const int gi = s.walk.ring[(base + lane) & (kRing - 1)];
-            const float4* q = reinterpret_cast<const float4*>(&rec[gi]);
-            const float4 a = q[0], b = q[1], c = q[2], d = q[3], e = q[4];
-            s.chunk[lane][0] = a;
-            s.chunk[lane][1] = b;
-            s.chunk[lane][2] = c;
-            lox = d.x; loy = d.y; loz = d.z; hix = d.w; hiy = e.x; hiz = e.y;
-        }
-        int mlo = 0, mhi = 0;
-#pragma unroll
-        for (int p = 0; p < 64; ++p) {
-            const float4 q = gp[p];
-            const bool inb = lox <= q.x && q.x <= hix && loy <= q.y && q.y <= hiy && loz <= q.z && q.z <= hiz;
-            const uint64_t bal = __ballot(inb);
-            writelane2(mlo, mhi, (uint32_t)bal, (uint32_t)(bal >> 32), p);
-        }
-        lds_fence();
-        uint64_t m = ((uint64_t)(uint32_t)mhi << 32) | (uint32_t)mlo;
-        uint64_t accm = 0;
-        if (m) {
-            int j = __builtin_ctzll(m);
-            m &= m - 1;
-            float4 A = s.chunk[j][0], B = s.chunk[j][1], C = s.chunk[j][2];
-            while (true) {
-                const bool more = m != 0;
-                const int jn = more ? __builtin_ctzll(m) : j;
-                const float4 An = s.chunk[jn][0], Bn = s.chunk[jn][1], Cn = s.chunk[jn][2];
-                const float dx = px - A.x, dy = py - A.y, dz = pz - A.z;
-                float r0, r1, r2;
-                const float power = pair_power(B.x, B.y, B.z, B.w, C.x, C.y, dx, dy, dz, r0, r1, r2);
-                if (in_window(power)) {
-                    const float e = __builtin_amdgcn_exp2f(power * kLog2e);
-                    const float wgt = A.w * e;
-                    const float wv = wgt * C.z;
-                    if (KAHAN) {
-                        float y = wgt - cW;
-                        float t = W + y;
-                        cW = (t - W) - y;
-                        W = t;
-                        y = wv - cU;
-                        t = U + y;
-                        cU = (t - U) - y;
-                        U = t;
-                    } else {
-                        W += wgt;
-                        U += wv;
-                    }
-                    ++npairs;
-                    accm |= 1ull << j;
-                }
-                if (!more) break;
-                m &= m - 1;
-                j = jn;
-                A = An;
-                B = Bn;
-                C = Cn;
+        auto chunk = [&](uint32_t base, int n) {
+            float lox = 1.f, loy = 1.f, loz = 1.f, hix = 0.f, hiy = 0.f, hiz = 0.f;  // empty box for idle lanes
+            if (lane < n) {
+                const int gi = s.walk.ring[(base + lane) & (kRing - 1)];
+                const float4* q = reinterpret_cast<const float4*>(&rec[gi]);
+                const float4 a = q[0], b = q[1], c = q[2], d = q[3], e = q[4];
+                s.chunk[lane][0] = a;
+                s.chunk[lane][1] = b;
+                s.chunk[lane][2] = c;
+                lox = d.x; loy = d.y; loz = d.z; hix = d.w; hiy = e.x; hiz = e.y;
             }
+            int mlo = 0, mhi = 0;
+#pragma unroll
+            for (int p = 0; p < 64; ++p) {
+                const float4 q = gp[p];
+                const bool inb = lox <= q.x && q.x <= hix && loy <= q.y && q.y <= hiy && loz <= q.z && q.z <= hiz;
+                const uint64_t bal = __ballot(inb);
+                writelane2(mlo, mhi, (uint32_t)bal, (uint32_t)(bal >> 32), p);
+            }
+            lds_fence();
+            uint64_t m = inrun ? (((uint64_t)(uint32_t)mhi << 32) | (uint32_t)mlo) : 0ull;
+            uint64_t accm = 0;
+            if (m) {
+                int j = __builtin_ctzll(m);
+                m &= m - 1;
+                float4 A = s.chunk[j][0], B = s.chunk[j][1], C = s.chunk[j][2];
+                while (true) {
+                    const bool more = m != 0;
+                    const int jn = more ? __builtin_ctzll(m) : j;
+                    const float4 An = s.chunk[jn][0], Bn = s.chunk[jn][1], Cn = s.chunk[jn][2];
+                    const float dx = px - A.x, dy = py - A.y, dz = pz - A.z;
+                    float r0, r1, r2;
+                    const float power = pair_power(B.x, B.y, B.z, B.w, C.x, C.y, dx, dy, dz, r0, r1, r2);
+                    if (in_window(power)) {
+                        const float e = __builtin_amdgcn_exp2f(power * kLog2e);
+                        const float wgt = A.w * e;
+                        const float wv = wgt * C.z;
+                        if (KAHAN) {
+                            float y = wgt - cW;
+                            float t = W + y;
+                            cW = (t - W) - y;
+                            W = t;
+                            y = wv - cU;
+                            t = U + y;
+                            cU = (t - U) - y;
+                            U = t;
+                        } else {
+                            W += wgt;
+                            U += wv;
+                        }
+                        ++npairs;
+                        accm |= 1ull << j;
+                    }
+                    if (!more) break;
+                    m &= m - 1;
+                    j = jn;
+                    A = An;
+                    B = Bn;
+                    C = Cn;
+                }
+            }
+            const uint32_t olo = wave_or32((uint32_t)accm), ohi = wave_or32((uint32_t)(accm >> 32));
+            nrec += (uint32_t)(__popc(olo) + __popc(ohi));
+            lds_fence();
+        };
+        uint64_t rs = starts;
+        while (rs != 0ull) {
+            const int a = __builtin_ctzll(rs);
+            rs &= rs - 1;
+            const int b = rs ? __builtin_ctzll(rs) : nv;
+            inrun = lane >= a && lane < b;
+            const GroupBox gb = masked_box(px, py, pz, inrun);
+            ok = enumerate_candidates(rec, nodes, root_box, P, NL, L, gb, s.walk, chunk) && ok;
         }
-        const uint32_t olo = wave_or32((uint32_t)accm), ohi = wave_or32((uint32_t)(accm >> 32));
-        nrec += (uint32_t)(__popc(olo) + __popc(ohi));
-        lds_fence();
-    };
-    const bool ok = enumerate_candidates(rec, nodes, root_box, P, NL, L, gb, s.walk, chunk);
+        if (orig >= 0) {
+            float V, Wo;
+            if ((double)W <= kWeightCutoff) {
+                V = -1.0f;
+                Wo = 0.0f;
+            } else {
+                V = U / W;
+                Wo = W;
+            }
+            out_cells[orig] = V;
+            out_weights[orig] = Wo;
+        }
+        const uint32_t tp = wave_sumu(npairs);
+        if (lane == 0) {
+            group_records[g] = nrec;
+            group_pairs[g] = tp;
+        }
+    }
     if (!ok && lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_STACK_OVERFLOW);
-    if (orig >= 0) {
-        float V, Wo;
-        if ((double)W <= kWeightCutoff) {
-            V = -1.0f;
-            Wo = 0.0f;
-        } else {
-            V = U / W;
-            Wo = W;
-        }
-        out_cells[orig] = V;
-        out_weights[orig] = Wo;
-    }
-    const uint32_t tp = wave_sumu(npairs);
-    if (lane == 0) {
-        group_records[g] = nrec;
-        group_pairs[g] = tp;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -651,115 +778,133 @@ __global__ __launch_bounds__(256) void backward_kernel(
     int NL, int L, const float4* __restrict__ pts, int S, int NG, const float* __restrict__ out_cells,
     const float* __restrict__ out_weights, const float* __restrict__ gcells, const float* __restrict__ gweights,
     const uint32_t* __restrict__ group_offsets, uint64_t capacity, PartialRecord* __restrict__ out_rec,
-    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ err)
+    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ err, uint32_t* __restrict__ queue,
+    const unsigned long long* __restrict__ ext_sum)
 {
     __shared__ BwdLds lds_all[4];
     const int wid = uniformi(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int g = uniformi((int)xcd_block(blockIdx.x, gridDim.x) * 4 + wid);
-    if (g >= NG) return;
     BwdLds& s = lds_all[wid];
-    const float4* __restrict__ gp = pts + (size_t)g * 64;
-    const float4 pme = gp[lane];
-    const float px = pme.x, py = pme.y, pz = pme.z;
-    const int orig = __builtin_bit_cast(int, pme.w);
-    float V = 0.f, gVi = 0.f, gWe = 0.f;
-    if (orig >= 0) {
-        const float Wp = out_weights[orig];
-        if ((double)Wp > kWeightCutoff) {
-            V = out_cells[orig];
-            gVi = gcells[orig] / Wp;
-            gWe = gweights[orig];
+    const uint64_t lt = lanemask_lt();
+    float reach[3];
+    mean_extent(ext_sum, P, reach);
+    int x = (int)(blockIdx.x & 7u), tries = 0;
+    bool ok = true;
+    while (true) {
+        const int g = next_group(queue, NG, x, tries);
+        if (g >= NG) break;
+        const float4* __restrict__ gp = pts + (size_t)g * 64;
+        const float4 pme = gp[lane];
+        const float px = pme.x, py = pme.y, pz = pme.z;
+        const int orig = __builtin_bit_cast(int, pme.w);
+        float V = 0.f, gVi = 0.f, gWe = 0.f;
+        if (orig >= 0) {
+            const float Wp = out_weights[orig];
+            if ((double)Wp > kWeightCutoff) {
+                V = out_cells[orig];
+                gVi = gcells[orig] / Wp;
+                gWe = gweights[orig];
+            }
+        }
+        lds_fence();
+        s.ptsA[lane] = make_float4(px, py, pz, V);
+        s.ptsB[lane] = make_float2(gVi, gWe);
+        const int nv = __popcll(__ballot(orig >= 0));
+        const uint64_t starts = split_runs(px, py, pz, nv, reach);
+        lds_fence();
+
+        uint32_t base_out = group_offsets[g];
+        const uint32_t limit = group_offsets[g + 1];
+        uint64_t runmask = 0;
+
+        auto chunk = [&](uint32_t base, int n) {
+            const bool active = lane < n;
+            const int gi = active ? s.walk.ring[(base + lane) & (kRing - 1)] : 0;
+            const float4* q = reinterpret_cast<const float4*>(&rec[gi]);
+            const float4 ra = q[0], rb = q[1], rc = q[2], rd = q[3], re = q[4];
+            const float lox = active ? rd.x : 1.f, hix = active ? rd.w : 0.f;
+            uint32_t mlo = 0, mhi = 0;
+#pragma unroll 4
+            for (int p = 0; p < 32; ++p) {
+                const float4 a = s.ptsA[p];
+                const float4 b = s.ptsA[32 + p];
+                const bool ia = lox <= a.x && a.x <= hix && rd.y <= a.y && a.y <= re.x && rd.z <= a.z && a.z <= re.y;
+                const bool ib = lox <= b.x && b.x <= hix && rd.y <= b.y && b.y <= re.x && rd.z <= b.z && b.z <= re.y;
+                mlo |= ia ? (1u << p) : 0u;
+                mhi |= ib ? (1u << p) : 0u;
+            }
+            uint64_t m = (((uint64_t)mhi << 32) | mlo) & runmask;
+            float dv = 0.f, dw = 0.f, dmx = 0.f, dmy = 0.f, dmz = 0.f;
+            float dc0 = 0.f, dc1 = 0.f, dc2 = 0.f, dc3 = 0.f, dc4 = 0.f, dc5 = 0.f;
+            bool any = false;
+            if (m) {
+                int p = __builtin_ctzll(m);
+                m &= m - 1;
+                float4 a = s.ptsA[p];
+                float2 b = s.ptsB[p];
+                while (true) {
+                    const bool more = m != 0;
+                    const int pn = more ? __builtin_ctzll(m) : p;
+                    const float4 an = s.ptsA[pn];
+                    const float2 bn = s.ptsB[pn];
+                    const float dx = a.x - ra.x, dy = a.y - ra.y, dz = a.z - ra.z;
+                    float r0, r1, r2;
+                    const float power = pair_power(rb.x, rb.y, rb.z, rb.w, rc.x, rc.y, dx, dy, dz, r0, r1, r2);
+                    if (in_window(power)) {
+                        any = true;
+                        const float e = __builtin_amdgcn_exp2f(power * kLog2e);
+                        const float wgt = ra.w * e;
+                        dv = __builtin_fmaf(b.x, wgt, dv);
+                        const float t = __builtin_fmaf(b.x, rc.z - a.w, b.y);
+                        dw = __builtin_fmaf(t, e, dw);
+                        const float st = wgt * t;
+                        dmx = __builtin_fmaf(st, r0, dmx);
+                        dmy = __builtin_fmaf(st, r1, dmy);
+                        dmz = __builtin_fmaf(st, r2, dmz);
+                        const float dq = -0.5f * st;
+                        const float h0 = dq * dx, h1 = dq * dy, h2 = dq * dz;
+                        const float h0x2 = h0 + h0, h1x2 = h1 + h1;
+                        dc0 = __builtin_fmaf(h0, dx, dc0);
+                        dc1 = __builtin_fmaf(h0x2, dy, dc1);
+                        dc2 = __builtin_fmaf(h0x2, dz, dc2);
+                        dc3 = __builtin_fmaf(h1, dy, dc3);
+                        dc4 = __builtin_fmaf(h1x2, dz, dc4);
+                        dc5 = __builtin_fmaf(h2, dz, dc5);
+                    }
+                    if (!more) break;
+                    m &= m - 1;
+                    p = pn;
+                    a = an;
+                    b = bn;
+                }
+            }
+            const uint64_t bal = __ballot(any);
+            if (any) {
+                const uint64_t pos = (uint64_t)base_out + (uint32_t)__popcll(bal & lt);
+                if (pos < limit && pos < capacity) {
+                    PartialRecord o;
+                    o.gid = gi;
+                    o.dv = dv; o.dw = dw;
+                    o.dmx = dmx; o.dmy = dmy; o.dmz = dmz;
+                    o.dc[0] = dc0; o.dc[1] = dc1; o.dc[2] = dc2; o.dc[3] = dc3; o.dc[4] = dc4; o.dc[5] = dc5;
+                    out_rec[pos] = o;
+                    keys[pos] = (uint32_t)gi;
+                    vals[pos] = (uint32_t)pos;
+                } else {
+                    atomicOr(err, (uint32_t)GVC_ERRFLAG_RECORD_OVERFLOW);
+                }
+            }
+            base_out += (uint32_t)__popcll(bal);
+        };
+        uint64_t rs = starts;
+        while (rs != 0ull) {
+            const int a = __builtin_ctzll(rs);
+            rs &= rs - 1;
+            const int b = rs ? __builtin_ctzll(rs) : nv;
+            runmask = (b >= 64 ? ~0ull : ((1ull << b) - 1ull)) & ~((1ull << a) - 1ull);
+            const GroupBox gb = masked_box(px, py, pz, lane >= a && lane < b);
+            ok = enumerate_candidates(rec, nodes, root_box, P, NL, L, gb, s.walk, chunk) && ok;
         }
     }
-    s.ptsA[lane] = make_float4(px, py, pz, V);
-    s.ptsB[lane] = make_float2(gVi, gWe);
-    const GroupBox gb = group_box(px, py, pz);
-    lds_fence();
-
-    uint32_t base_out = group_offsets[g];
-    const uint32_t limit = group_offsets[g + 1];
-    const uint64_t lt = lanemask_lt();
-
-    auto chunk = [&](uint32_t base, int n) {
-        const bool active = lane < n;
-        const int gi = active ? s.walk.ring[(base + lane) & (kRing - 1)] : 0;
-        const float4* q = reinterpret_cast<const float4*>(&rec[gi]);
-        const float4 ra = q[0], rb = q[1], rc = q[2], rd = q[3], re = q[4];
-        const float lox = active ? rd.x : 1.f, hix = active ? rd.w : 0.f;
-        uint32_t mlo = 0, mhi = 0;
-#pragma unroll 4
-        for (int p = 0; p < 32; ++p) {
-            const float4 a = s.ptsA[p];
-            const float4 b = s.ptsA[32 + p];
-            const bool ia = lox <= a.x && a.x <= hix && rd.y <= a.y && a.y <= re.x && rd.z <= a.z && a.z <= re.y;
-            const bool ib = lox <= b.x && b.x <= hix && rd.y <= b.y && b.y <= re.x && rd.z <= b.z && b.z <= re.y;
-            mlo |= ia ? (1u << p) : 0u;
-            mhi |= ib ? (1u << p) : 0u;
-        }
-        uint64_t m = ((uint64_t)mhi << 32) | mlo;
-        float dv = 0.f, dw = 0.f, dmx = 0.f, dmy = 0.f, dmz = 0.f;
-        float dc0 = 0.f, dc1 = 0.f, dc2 = 0.f, dc3 = 0.f, dc4 = 0.f, dc5 = 0.f;
-        bool any = false;
-        if (m) {
-            int p = __builtin_ctzll(m);
-            m &= m - 1;
-            float4 a = s.ptsA[p];
-            float2 b = s.ptsB[p];
-            while (true) {
-                const bool more = m != 0;
-                const int pn = more ? __builtin_ctzll(m) : p;
-                const float4 an = s.ptsA[pn];
-                const float2 bn = s.ptsB[pn];
-                const float dx = a.x - ra.x, dy = a.y - ra.y, dz = a.z - ra.z;
-                float r0, r1, r2;
-                const float power = pair_power(rb.x, rb.y, rb.z, rb.w, rc.x, rc.y, dx, dy, dz, r0, r1, r2);
-                if (in_window(power)) {
-                    any = true;
-                    const float e = __builtin_amdgcn_exp2f(power * kLog2e);
-                    const float wgt = ra.w * e;
-                    dv = __builtin_fmaf(b.x, wgt, dv);
-                    const float t = __builtin_fmaf(b.x, rc.z - a.w, b.y);
-                    dw = __builtin_fmaf(t, e, dw);
-                    const float st = wgt * t;
-                    dmx = __builtin_fmaf(st, r0, dmx);
-                    dmy = __builtin_fmaf(st, r1, dmy);
-                    dmz = __builtin_fmaf(st, r2, dmz);
-                    const float dq = -0.5f * st;
-                    const float h0 = dq * dx, h1 = dq * dy, h2 = dq * dz;
-                    const float h0x2 = h0 + h0, h1x2 = h1 + h1;
-                    dc0 = __builtin_fmaf(h0, dx, dc0);
-                    dc1 = __builtin_fmaf(h0x2, dy, dc1);
-                    dc2 = __builtin_fmaf(h0x2, dz, dc2);
-                    dc3 = __builtin_fmaf(h1, dy, dc3);
-                    dc4 = __builtin_fmaf(h1x2, dz, dc4);
-                    dc5 = __builtin_fmaf(h2, dz, dc5);
-                }
-                if (!more) break;
-                m &= m - 1;
-                p = pn;
-                a = an;
-                b = bn;
-            }
-        }
-        const uint64_t bal = __ballot(any);
-        if (any) {
-            const uint64_t pos = (uint64_t)base_out + (uint32_t)__popcll(bal & lt);
-            if (pos < limit && pos < capacity) {
-                PartialRecord o;
-                o.gid = gi;
-                o.dv = dv; o.dw = dw;
-                o.dmx = dmx; o.dmy = dmy; o.dmz = dmz;
-                o.dc[0] = dc0; o.dc[1] = dc1; o.dc[2] = dc2; o.dc[3] = dc3; o.dc[4] = dc4; o.dc[5] = dc5;
-                out_rec[pos] = o;
-                keys[pos] = (uint32_t)gi;
-                vals[pos] = (uint32_t)pos;
-            } else {
-                atomicOr(err, (uint32_t)GVC_ERRFLAG_RECORD_OVERFLOW);
-            }
-        }
-        base_out += (uint32_t)__popcll(bal);
-    };
-    const bool ok = enumerate_candidates(rec, nodes, root_box, P, NL, L, gb, s.walk, chunk);
     if (!ok && lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_STACK_OVERFLOW);
 }
 
@@ -897,6 +1042,24 @@ __global__ void stats_kernel(const uint32_t* __restrict__ group_pairs, int NG, c
 
 inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
+// Grid of a persistent walk: as many 4-wave blocks as fit on the device at once
+// (a multiple of 8, one share per XCD), never more than the groups need.
+unsigned walk_blocks(const void* kernel, int threads, int NG)
+{
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per <= 0 || cus <= 0) {
+        (void)hipGetLastError();
+        per = 1;
+        cus = 256;
+    }
+    unsigned nb = (unsigned)per * (unsigned)cus;
+    nb = nb >= 8 ? nb & ~7u : nb;
+    const unsigned need = blocks_for((size_t)NG, (unsigned)(threads / 64));
+    return nb < need ? nb : need;
+}
+
 inline int key_bits_for(uint64_t maxv)
 {
     int b = 0;
@@ -932,7 +1095,8 @@ struct ForwardLayout {
     uint32_t* counters;
     float* root_box;
     uint32_t *group_records, *group_offsets, *group_pairs, *scan_scratch, *err;
-    unsigned long long* stats;
+    uint32_t* queues;  // 8 forward + 8 backward group queues
+    unsigned long long *stats, *ext_sum;
     int NL, NG;
     ForwardLayout(Arena& a, int P, int S, int L)
     {
@@ -955,7 +1119,9 @@ struct ForwardLayout {
         group_pairs = a.take<uint32_t>((size_t)NG + 1);
         scan_scratch = a.take<uint32_t>(scan_scratch_elems((size_t)NG) + 1);
         err = a.take<uint32_t>(4);
+        queues = a.take<uint32_t>(16);
         stats = a.take<unsigned long long>(2);
+        ext_sum = a.take<unsigned long long>(4);
     }
 };
 
@@ -1090,8 +1256,9 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
     const uint64_t* scodes = alt ? l.codes_alt : l.codes;
     const uint32_t* perm = alt ? l.idx_alt : l.idx;
     // 2. records
+    GVC_TRY(fill_u32(reinterpret_cast<uint32_t*>(l.ext_sum), 0u, 8, st));
     record_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, perm, means, scales, rots, values, weights, k, fp64, l.rec,
-                                                      conics);
+                                                      conics, l.ext_sum);
     GVC_LAUNCH_CHECK("record");
     // 3. tree
     leaf_codes_kernel<<<blocks_for(NL, 256), 256, 0, st>>>(scodes, NL, L, l.leaf_codes);
@@ -1109,13 +1276,18 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
     // 4. walk
     if (NG > 0) {
         const int T = 256;
-        const unsigned nb = blocks_for(NG, 4);
-        if (prm->flags & GVC_FLAG_KAHAN)
+        GVC_TRY(fill_u32(l.queues, 0u, 8, st));
+        if (prm->flags & GVC_FLAG_KAHAN) {
+            const unsigned nb = walk_blocks(reinterpret_cast<const void*>(&forward_kernel<true>), T, NG);
             forward_kernel<true><<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells,
-                                                   out_weights, l.group_records, l.group_pairs, l.err);
-        else
+                                                   out_weights, l.group_records, l.group_pairs, l.err, l.queues,
+                                                   l.ext_sum);
+        } else {
+            const unsigned nb = walk_blocks(reinterpret_cast<const void*>(&forward_kernel<false>), T, NG);
             forward_kernel<false><<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells,
-                                                    out_weights, l.group_records, l.group_pairs, l.err);
+                                                    out_weights, l.group_records, l.group_pairs, l.err, l.queues,
+                                                    l.ext_sum);
+        }
         GVC_LAUNCH_CHECK("forward");
     }
     GVC_TRY(exclusive_scan_u32(l.group_records, l.group_offsets, (size_t)NG, true, l.scan_scratch, st));
@@ -1171,10 +1343,11 @@ int mixture_backward(int P, const float* means, const float* scales, const float
     const unsigned stages = prm->stages ? (unsigned)prm->stages : 3u;
     if ((stages & GVC_STAGE_WALK) && NG > 0 && capacity > 0) {
         const int T = 256;
-        const unsigned nb = blocks_for(NG, 4);
+        GVC_TRY(fill_u32(l.queues + 8, 0u, 8, st));
+        const unsigned nb = walk_blocks(reinterpret_cast<const void*>(&backward_kernel), T, NG);
         backward_kernel<<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells, out_weights,
                                           gcells, gweights, l.group_offsets, capacity, bl.parts, bl.keys, bl.vals,
-                                          l.err);
+                                          l.err, l.queues + 8, l.ext_sum);
         GVC_LAUNCH_CHECK("backward");
         pad_keys_kernel<<<blocks_for(capacity, 256), 256, 0, st>>>(bl.keys, bl.vals, l.group_offsets + NG, capacity,
                                                                   (uint32_t)P);
