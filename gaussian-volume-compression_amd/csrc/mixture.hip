@@ -7,19 +7,22 @@
 //   forward      : 63-bit Morton codes of the means (the reference's morton3d_unit)
 //                  -> stable radix sort -> canonical preprocess written as 80-byte
 //                  records in BVH order -> Karras radix tree over leaves of
-//                  `leaf_size` consecutive records -> bottom-up box refit with
-//                  atomic arrival counters -> one wavefront per group walks the
-//                  tree with the group's box (wave-uniform stack, scalar node and
-//                  record loads), each lane owns one point and accumulates
-//                  w exp(power) and w exp(power) v with Kahan compensation.
-//                  The walk also counts, per group, the Gaussians with at least
-//                  one accepted pair: those are the backward's records.
-//   backward     : the same walk per group; candidate Gaussians are queued in a
-//                  VGPR (one per lane) and each lane loops over the 64 points of
-//                  the group (read from LDS) accumulating its Gaussian's 11
-//                  partial sums -> one record per (Gaussian, group) at a position
-//                  fixed by the forward's counts -> stable radix sort by Gaussian
-//                  -> per-Gaussian FP64 sum in group order -> covariance chain in
+//                  `leaf_size` consecutive records -> node boxes from a sparse
+//                  table of leaf-range boxes -> persistent wavefronts take groups
+//                  from per-XCD queues; a group is cut into runs of consecutive
+//                  lanes where it straddles a coarse Morton boundary, and each run
+//                  walks the tree with its box (wide wave-uniform traversal,
+//                  candidates in 64-wide chunks); each lane owns one point and
+//                  accumulates w exp(power) and w exp(power) v with Kahan
+//                  compensation.  The walk also counts, per group, the Gaussians
+//                  with at least one accepted pair: those are the backward's
+//                  records.
+//   backward     : the same walk per group; each lane takes one candidate
+//                  Gaussian of a chunk and loops over the run's points inside its
+//                  box (LDS) accumulating the Gaussian's 11 partial sums -> one
+//                  record per (Gaussian, group) at a position fixed by the
+//                  forward's counts -> stable radix sort by Gaussian ->
+//                  per-Gaussian FP64 sum in group order -> covariance chain in
 //                  FP64 -> gradients in the caller's Gaussian order.
 // Reference path: CR/forward.cu:10-274, CR/backward.cu:9-660,
 // CR/rasterizer_impl.cu:27-452, DGR/ext.cu:13-147 (CR/ = diff-gaussian-
@@ -177,14 +180,18 @@ __global__ void gaussian_codes_kernel(const float* __restrict__ means, int P, ui
 }
 
 // Canonical preprocess, written as records in BVH (sorted) order.  Also sums the
-// box extents per axis in units of 2^-20 (integer adds: the same total in any
-// order), the walks' estimate of a Gaussian's reach when splitting groups.
-__global__ void record_kernel(int P, const uint32_t* __restrict__ perm, const float* __restrict__ means,
-                              const float* __restrict__ scales, const float* __restrict__ rots,
-                              const float* __restrict__ values, const float* __restrict__ weights, float k,
-                              int fp64_cond, GRecord* __restrict__ rec, float* __restrict__ conics_out,
-                              unsigned long long* __restrict__ ext_sum)
+// box extents per axis in units of 2^-20 per 256-record block (integers: the
+// same totals in any order), the walks' estimate of a Gaussian's reach when
+// splitting groups; ext_reduce_kernel adds the block sums.
+__global__ __launch_bounds__(256) void record_kernel(int P, const uint32_t* __restrict__ perm,
+                                                     const float* __restrict__ means,
+                                                     const float* __restrict__ scales, const float* __restrict__ rots,
+                                                     const float* __restrict__ values,
+                                                     const float* __restrict__ weights, float k, int fp64_cond,
+                                                     GRecord* __restrict__ rec, float* __restrict__ conics_out,
+                                                     unsigned long long* __restrict__ ext_part)
 {
+    __shared__ unsigned long long wsum[4][3];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t ex[3] = { 0, 0, 0 };
     if (i < P) {
@@ -215,7 +222,30 @@ __global__ void record_kernel(int P, const uint32_t* __restrict__ perm, const fl
         uint64_t v = ex[a];
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) v += (uint64_t)__shfl_xor((long long)v, d, 64);
-        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&ext_sum[a], (unsigned long long)v);
+        if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6][a] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int a = threadIdx.x;
+        ext_part[3 * (size_t)blockIdx.x + a] = wsum[0][a] + wsum[1][a] + wsum[2][a] + wsum[3][a];
+    }
+}
+
+__global__ void ext_reduce_kernel(const unsigned long long* __restrict__ part, int nb,
+                                  unsigned long long* __restrict__ ext_sum)
+{
+    __shared__ unsigned long long red[256];
+    for (int a = 0; a < 3; ++a) {
+        unsigned long long v = 0;
+        for (int i = threadIdx.x; i < nb; i += 256) v += part[3 * (size_t)i + a];
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int d = 128; d >= 1; d >>= 1) {
+            if ((int)threadIdx.x < d) red[threadIdx.x] += red[threadIdx.x + d];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) ext_sum[a] = red[0];
+        __syncthreads();
     }
 }
 
@@ -235,8 +265,7 @@ __device__ __forceinline__ int karras_delta(const uint64_t* k, int n, int i, int
 }
 
 // Karras (2012): one thread per internal node.  parent codes: (p << 1) | side.
-__global__ void karras_kernel(const uint64_t* __restrict__ keys, int n, BNode* __restrict__ nodes,
-                              int* __restrict__ parent_int, int* __restrict__ parent_leaf)
+__global__ void karras_kernel(const uint64_t* __restrict__ keys, int n, BNode* __restrict__ nodes)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
@@ -262,69 +291,96 @@ __global__ void karras_kernel(const uint64_t* __restrict__ keys, int n, BNode* _
     nodes[i].right = right;
     nodes[i].pad0 = lo;
     nodes[i].pad1 = hi;
-    if (left < 0) parent_leaf[~left] = (i << 1) | 0; else parent_int[left] = (i << 1) | 0;
-    if (right < 0) parent_leaf[~right] = (i << 1) | 1; else parent_int[right] = (i << 1) | 1;
 }
 
-__device__ __forceinline__ void store_child_box(BNode* nodes, int pcode, const float lo[3], const float hi[3])
+// Node boxes.  A node of the radix tree covers the leaf range [first, last]; its
+// box is the union of the leaves' boxes, read from a sparse table
+// T_k[i] = box of leaves [i, i + 2^k): any range is the union of two (possibly
+// overlapping) table entries.  Unions use fminf/fmaxf, which skip NaN boxes; a
+// leaf whose Gaussians all have NaN boxes keeps a NaN box.
+struct Box6 {
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ Box6 box_union(const Box6& a, const Box6& b)
 {
-    BNode& n = nodes[pcode >> 1];
-    if ((pcode & 1) == 0) {
-        n.llo[0] = lo[0]; n.llo[1] = lo[1]; n.llo[2] = lo[2];
-        n.lhi[0] = hi[0]; n.lhi[1] = hi[1]; n.lhi[2] = hi[2];
-    } else {
-        n.rlo[0] = lo[0]; n.rlo[1] = lo[1]; n.rlo[2] = lo[2];
-        n.rhi[0] = hi[0]; n.rhi[1] = hi[1]; n.rhi[2] = hi[2];
+    Box6 r;
+    for (int c = 0; c < 3; ++c) {
+        r.lo[c] = fminf(a.lo[c], b.lo[c]);
+        r.hi[c] = fmaxf(a.hi[c], b.hi[c]);
     }
+    return r;
 }
 
-// Bottom-up refit: one thread per leaf; the second thread to arrive at a node
-// (atomic counter, acquire/release at agent scope) merges both child boxes and
-// continues upwards.  The root's box goes to root_box.
-__global__ void refit_kernel(const GRecord* __restrict__ rec, int P, int NL, int L, BNode* nodes,
-                             const int* __restrict__ parent_int, const int* __restrict__ parent_leaf,
-                             uint32_t* counters, float* root_box)
+__global__ void leaf_box_kernel(const GRecord* __restrict__ rec, int P, int NL, int L, Box6* __restrict__ t0)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= NL) return;
-    float lo[3] = { __builtin_inff(), __builtin_inff(), __builtin_inff() };
-    float hi[3] = { -__builtin_inff(), -__builtin_inff(), -__builtin_inff() };
+    Box6 b;
+    for (int c = 0; c < 3; ++c) {
+        b.lo[c] = __builtin_inff();
+        b.hi[c] = -__builtin_inff();
+    }
     bool any = false;
     const int g0 = j * L, g1 = min(g0 + L, P);
     for (int g = g0; g < g1; ++g) {
         const GRecord& r = rec[g];
-        // NaN boxes are skipped by fminf/fmaxf; a leaf of only NaN boxes stays NaN.
-        lo[0] = fminf(lo[0], r.lox); lo[1] = fminf(lo[1], r.loy); lo[2] = fminf(lo[2], r.loz);
-        hi[0] = fmaxf(hi[0], r.hix); hi[1] = fmaxf(hi[1], r.hiy); hi[2] = fmaxf(hi[2], r.hiz);
+        b.lo[0] = fminf(b.lo[0], r.lox); b.lo[1] = fminf(b.lo[1], r.loy); b.lo[2] = fminf(b.lo[2], r.loz);
+        b.hi[0] = fmaxf(b.hi[0], r.hix); b.hi[1] = fmaxf(b.hi[1], r.hiy); b.hi[2] = fmaxf(b.hi[2], r.hiz);
         any |= (r.lox == r.lox);
     }
-    if (!any) {
-        lo[0] = lo[1] = lo[2] = __builtin_nanf("");
-        hi[0] = hi[1] = hi[2] = __builtin_nanf("");
+    if (!any)
+        for (int c = 0; c < 3; ++c) b.lo[c] = b.hi[c] = __builtin_nanf("");
+    t0[j] = b;
+}
+
+// T_{k+1}[i] = T_k[i] u T_k[i + 2^k] for i + 2^{k+1} <= NL.
+__global__ void sparse_level_kernel(const Box6* __restrict__ tk, Box6* __restrict__ tk1, int NL, int half)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i + 2 * half > NL) return;
+    tk1[i] = box_union(tk[i], tk[i + half]);
+}
+
+__device__ __forceinline__ Box6 range_box(const Box6* __restrict__ table, int NL, int a, int b)
+{
+    const int k = 31 - __builtin_clz((unsigned)(b - a + 1));
+    const Box6* tk = table + (size_t)k * NL;
+    return box_union(tk[a], tk[b - (1 << k) + 1]);
+}
+
+__global__ void node_box_kernel(const Box6* __restrict__ table, int NL, BNode* __restrict__ nodes,
+                                float* __restrict__ root_box)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NL - 1) return;
+    BNode& n = nodes[i];
+    const int lo = n.pad0, hi = n.pad1, left = n.left;
+    const int gamma = left < 0 ? ~left : left;   // the left child covers [lo, gamma]
+    const Box6 l = range_box(table, NL, lo, gamma);
+    const Box6 r = range_box(table, NL, gamma + 1, hi);
+    for (int c = 0; c < 3; ++c) {
+        n.llo[c] = l.lo[c];
+        n.lhi[c] = l.hi[c];
+        n.rlo[c] = r.lo[c];
+        n.rhi[c] = r.hi[c];
     }
-    if (NL == 1) {
-        for (int c = 0; c < 3; ++c) { root_box[c] = lo[c]; root_box[3 + c] = hi[c]; }
-        return;
-    }
-    int pcode = parent_leaf[j];
-    store_child_box(nodes, pcode, lo, hi);
-    while (true) {
-        const int p = pcode >> 1;
-        const uint32_t old = __hip_atomic_fetch_add(&counters[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0) return;
-        const BNode& n = nodes[p];
-        float nlo[3], nhi[3];
+    if (i == 0) {
+        const Box6 u = box_union(l, r);
         for (int c = 0; c < 3; ++c) {
-            nlo[c] = fminf(n.llo[c], n.rlo[c]);
-            nhi[c] = fmaxf(n.lhi[c], n.rhi[c]);
+            root_box[c] = u.lo[c];
+            root_box[3 + c] = u.hi[c];
         }
-        if (p == 0) {
-            for (int c = 0; c < 3; ++c) { root_box[c] = nlo[c]; root_box[3 + c] = nhi[c]; }
-            return;
-        }
-        pcode = parent_int[p];
-        store_child_box(nodes, pcode, nlo, nhi);
     }
+}
+
+__global__ void single_leaf_root_kernel(const Box6* __restrict__ t0, float* __restrict__ root_box)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        for (int c = 0; c < 3; ++c) {
+            root_box[c] = t0[0].lo[c];
+            root_box[3 + c] = t0[0].hi[c];
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -1042,6 +1098,14 @@ __global__ void stats_kernel(const uint32_t* __restrict__ group_pairs, int NG, c
 
 inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
+// levels of the leaf-range sparse table: 1 + floor(log2(NL))
+inline int sparse_levels(int NL)
+{
+    int k = 1;
+    while (NL > 0 && (2ll << (k - 1)) <= NL) ++k;
+    return k;
+}
+
 // Grid of a persistent walk: as many 4-wave blocks as fit on the device at once
 // (a multiple of 8, one share per XCD), never more than the groups need.
 unsigned walk_blocks(const void* kernel, int threads, int NG)
@@ -1091,12 +1155,11 @@ struct ForwardLayout {
     uint64_t *codes, *codes_alt, *leaf_codes;
     uint32_t *idx, *idx_alt, *sort_scratch;
     BNode* nodes;
-    int *parent_int, *parent_leaf;
-    uint32_t* counters;
+    Box6* sparse;  // levels x NL leaf-range boxes
     float* root_box;
     uint32_t *group_records, *group_offsets, *group_pairs, *scan_scratch, *err;
     uint32_t* queues;  // 8 forward + 8 backward group queues
-    unsigned long long *stats, *ext_sum;
+    unsigned long long *stats, *ext_sum, *ext_part;
     int NL, NG;
     ForwardLayout(Arena& a, int P, int S, int L)
     {
@@ -1110,9 +1173,7 @@ struct ForwardLayout {
         sort_scratch = a.take<uint32_t>(radix_scratch_elems((size_t)P));
         leaf_codes = a.take<uint64_t>((size_t)NL);
         nodes = a.take<BNode>((size_t)(NL > 1 ? NL - 1 : 1));
-        parent_int = a.take<int>((size_t)(NL > 1 ? NL - 1 : 1));
-        parent_leaf = a.take<int>((size_t)(NL > 0 ? NL : 1));
-        counters = a.take<uint32_t>((size_t)(NL > 1 ? NL - 1 : 1));
+        sparse = a.take<Box6>((size_t)sparse_levels(NL) * (size_t)(NL > 0 ? NL : 1));
         root_box = a.take<float>(8);
         group_records = a.take<uint32_t>((size_t)NG + 1);
         group_offsets = a.take<uint32_t>((size_t)NG + 1);
@@ -1122,6 +1183,7 @@ struct ForwardLayout {
         queues = a.take<uint32_t>(16);
         stats = a.take<unsigned long long>(2);
         ext_sum = a.take<unsigned long long>(4);
+        ext_part = a.take<unsigned long long>(3 * (((size_t)P + 255) / 256) + 3);
     }
 };
 
@@ -1256,21 +1318,36 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
     const uint64_t* scodes = alt ? l.codes_alt : l.codes;
     const uint32_t* perm = alt ? l.idx_alt : l.idx;
     // 2. records
-    GVC_TRY(fill_u32(reinterpret_cast<uint32_t*>(l.ext_sum), 0u, 8, st));
     record_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, perm, means, scales, rots, values, weights, k, fp64, l.rec,
-                                                      conics, l.ext_sum);
+                                                      conics, l.ext_part);
     GVC_LAUNCH_CHECK("record");
+    ext_reduce_kernel<<<1, 256, 0, st>>>(l.ext_part, (int)blocks_for(P, 256), l.ext_sum);
+    GVC_LAUNCH_CHECK("ext_reduce");
     // 3. tree
     leaf_codes_kernel<<<blocks_for(NL, 256), 256, 0, st>>>(scodes, NL, L, l.leaf_codes);
     GVC_LAUNCH_CHECK("leaf_codes");
     if (NL > 1) {
-        karras_kernel<<<blocks_for(NL - 1, 256), 256, 0, st>>>(l.leaf_codes, NL, l.nodes, l.parent_int, l.parent_leaf);
+        karras_kernel<<<blocks_for(NL - 1, 256), 256, 0, st>>>(l.leaf_codes, NL, l.nodes);
         GVC_LAUNCH_CHECK("karras");
-        GVC_TRY(fill_u32(l.counters, 0u, (size_t)(NL - 1), st));
     }
-    refit_kernel<<<blocks_for(NL, 256), 256, 0, st>>>(l.rec, P, NL, L, l.nodes, l.parent_int, l.parent_leaf,
-                                                      l.counters, l.root_box);
-    GVC_LAUNCH_CHECK("refit");
+    // 4. boxes: leaf boxes, sparse table of leaf ranges, node boxes
+    leaf_box_kernel<<<blocks_for(NL, 256), 256, 0, st>>>(l.rec, P, NL, L, l.sparse);
+    GVC_LAUNCH_CHECK("leaf_box");
+    if (NL > 1) {
+        const int levels = sparse_levels(NL);
+        for (int kk = 0; kk + 1 < levels; ++kk) {
+            const int half = 1 << kk;
+            const int n_out = NL - 2 * half + 1;
+            sparse_level_kernel<<<blocks_for((size_t)n_out, 256), 256, 0, st>>>(
+                l.sparse + (size_t)kk * NL, l.sparse + (size_t)(kk + 1) * NL, NL, half);
+            GVC_LAUNCH_CHECK("sparse_level");
+        }
+        node_box_kernel<<<blocks_for(NL - 1, 256), 256, 0, st>>>(l.sparse, NL, l.nodes, l.root_box);
+        GVC_LAUNCH_CHECK("node_box");
+    } else {
+        single_leaf_root_kernel<<<1, 64, 0, st>>>(l.sparse, l.root_box);
+        GVC_LAUNCH_CHECK("root_box");
+    }
     }
     if (!(stages & GVC_STAGE_WALK)) return GVC_OK;
     // 4. walk
