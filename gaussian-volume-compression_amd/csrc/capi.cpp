@@ -33,7 +33,7 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
                     float* out_cells, float* out_weights, float* conics, void* state, size_t state_bytes,
                     hipStream_t st);
 int forward_stats(const void* state, int P, int S, const gvc_params* prm, uint64_t* n_records, uint64_t* n_pairs,
-                  uint32_t* error_flags, hipStream_t st);
+                  uint32_t* error_flags, uint64_t* chunk_slots_used, hipStream_t st);
 size_t backward_bytes(int P, uint64_t cap);
 int mixture_backward(int P, const float* means, const float* scales, const float* rots, const float* values,
                      const float* weights, float k, const void* sample_set, int S, const gvc_params* prm,
@@ -67,7 +67,7 @@ using namespace gvc;
 
 extern "C" {
 
-int gvc_version(void) { return 1; }
+int gvc_version(void) { return 2; }
 
 const char* gvc_status_string(int status)
 {
@@ -145,10 +145,11 @@ int gvc_mixture_forward(int P, const float* means3D, const float* scales, const 
 }
 
 int gvc_forward_stats(const void* state, int P, int S, const gvc_params* params, uint64_t* n_records,
-                      uint64_t* n_pairs, uint32_t* error_flags, void* stream)
+                      uint64_t* n_pairs, uint32_t* error_flags, uint64_t* chunk_slots_used, void* stream)
 {
     GVC_REQUIRE(state, "NULL state");
-    return forward_stats(state, P, S, params, n_records, n_pairs, error_flags, (hipStream_t)stream);
+    return forward_stats(state, P, S, params, n_records, n_pairs, error_flags, chunk_slots_used,
+                         (hipStream_t)stream);
 }
 
 size_t gvc_backward_bytes(int P, uint64_t record_capacity)

@@ -533,7 +533,10 @@ __device__ __forceinline__ void mean_extent(const unsigned long long* __restrict
 }
 
 // Candidate enumeration.  chunk(base, n) is called with the ring positions
-// [base, base+n) (n <= 64, modulo kRing) holding Gaussian indices.
+// [base, base+n) (n <= 64, modulo kRing) holding Gaussian indices.  Written as
+// one loop whose every pass does exactly one of: hand a chunk over, expand 64
+// leaf Gaussians into the ring, or take one traversal step; so `chunk` has a
+// single call site and is inlined once.
 template <class ChunkFn>
 __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__ rec, const BNode* __restrict__ nodes,
                                                      const float* __restrict__ root_box, int P, int NL, int L,
@@ -543,17 +546,38 @@ __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__
     const uint64_t lt = lanemask_lt();
     uint32_t chead = 0, ctail = 0;
     bool ok = true;
-
-    // expand s.leaves[0..nl) into box-overlapping Gaussians
-    auto expand = [&](int nl) {
-        const int total = nl * L;
-        for (int base = 0; base < total; base += 64) {
-            const int t = base + lane;
+    int nl = 0, eb = 0;             // leaves s.leaves[0..nl) under expansion; next Gaussian offset
+    int sp = 0;                     // wide stack depth
+    int dnode = -1, ssp = 0;        // depth-first walk of one subtree while dnode >= 0
+    if (NL <= 0) return true;
+    if (NL == 1) {
+        if (box_overlaps(root_box[0], root_box[1], root_box[2], root_box[3], root_box[4], root_box[5], gb)) {
+            if (lane == 0) s.leaves[0] = 0;
+            nl = 1;
+        }
+    } else {
+        if (lane == 0) s.wstack[0] = 0;
+        sp = 1;
+    }
+    lds_fence();
+    while (true) {
+        const uint32_t pending = ctail - chead;
+        const bool exp_done = eb >= nl * L;
+        const bool trav_done = sp == 0 && dnode < 0;
+        if (pending >= 64u || (pending > 0u && exp_done && trav_done)) {
+            const int n = pending >= 64u ? 64 : (int)pending;
+            chunk(chead, n);
+            chead += (uint32_t)n;
+            continue;
+        }
+        if (!exp_done) {
+            // box-test the next 64 Gaussians of the pending leaves
+            const int t = eb + lane;
             bool hit = false;
             int g = 0;
-            if (t < total) {
-                const int leaf = s.leaves[t / L];
-                g = leaf * L + (t - (t / L) * L);
+            if (t < nl * L) {
+                const int q = t / L;
+                g = s.leaves[q] * L + (t - q * L);
                 if (g < P) {
                     const float4 b0 = *reinterpret_cast<const float4*>(&rec[g].lox);
                     const float2 b1 = *reinterpret_cast<const float2*>(&rec[g].hiy);
@@ -563,90 +587,81 @@ __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__
             const uint64_t bal = __ballot(hit);
             if (hit) s.ring[(ctail + (uint32_t)__popcll(bal & lt)) & (kRing - 1)] = g;
             ctail += (uint32_t)__popcll(bal);
+            eb += 64;
             lds_fence();
-            while (ctail - chead >= 64u) {
-                chunk(chead, 64);
-                chead += 64;
-            }
+            continue;
         }
-    };
-
-    if (NL <= 0) return true;
-    if (NL == 1) {
-        if (box_overlaps(root_box[0], root_box[1], root_box[2], root_box[3], root_box[4], root_box[5], gb)) {
-            if (lane == 0) s.leaves[0] = 0;
-            lds_fence();
-            expand(1);
-        }
-    } else {
-        int sp = 1;
-        if (lane == 0) s.wstack[0] = 0;
-        lds_fence();
-        while (sp > 0) {
-            int k = min(64, min(sp, kWideStack - sp));
-            if (k == 0) {
-                // no room for a wide step: walk the top subtree depth first
-                const int root = uniformi(s.wstack[sp - 1]);
-                sp -= 1;
-                int ssp = 0;
-                int node = root;
-                while (true) {
-                    const BNode nd = nodes[node];
-                    const bool ovl = box_overlaps(nd.llo[0], nd.llo[1], nd.llo[2], nd.lhi[0], nd.lhi[1], nd.lhi[2], gb);
-                    const bool ovr = box_overlaps(nd.rlo[0], nd.rlo[1], nd.rlo[2], nd.rhi[0], nd.rhi[1], nd.rhi[2], gb);
-                    const int cl = nd.left, cr = nd.right;
-                    int nl = 0;
-                    if (ovl && cl < 0) { if (lane == 0) s.leaves[nl] = ~cl; ++nl; }
-                    if (ovr && cr < 0) { if (lane == 0) s.leaves[nl] = ~cr; ++nl; }
-                    if (nl) { lds_fence(); expand(nl); }
-                    const bool dl = ovl && cl >= 0, dr = ovr && cr >= 0;
-                    if (dl) {
-                        if (dr) {
-                            if (ssp >= kSeqStack) { ok = false; break; }
-                            if (lane == 0) s.sstack[ssp] = cr;
-                            ++ssp;
-                            lds_fence();
-                        }
-                        node = cl;
-                    } else if (dr) {
-                        node = cr;
+        if (trav_done) break;
+        nl = 0;
+        eb = 0;
+        if (dnode >= 0) {
+            // one node of the depth-first walk
+            const BNode nd = nodes[dnode];
+            const bool ovl = box_overlaps(nd.llo[0], nd.llo[1], nd.llo[2], nd.lhi[0], nd.lhi[1], nd.lhi[2], gb);
+            const bool ovr = box_overlaps(nd.rlo[0], nd.rlo[1], nd.rlo[2], nd.rhi[0], nd.rhi[1], nd.rhi[2], gb);
+            const int cl = nd.left, cr = nd.right;
+            if (ovl && cl < 0) { if (lane == 0) s.leaves[nl] = ~cl; ++nl; }
+            if (ovr && cr < 0) { if (lane == 0) s.leaves[nl] = ~cr; ++nl; }
+            const bool dl = ovl && cl >= 0, dr = ovr && cr >= 0;
+            if (dl) {
+                if (dr) {
+                    if (ssp >= kSeqStack) {
+                        ok = false;
+                        dnode = -1;
+                        sp = 0;
                     } else {
-                        if (ssp == 0) break;
-                        node = uniformi(s.sstack[--ssp]);
+                        if (lane == 0) s.sstack[ssp] = cr;
+                        ++ssp;
+                        dnode = cl;
                     }
+                } else {
+                    dnode = cl;
                 }
-                if (!ok) break;
-                continue;
+            } else if (dr) {
+                dnode = cr;
+            } else {
+                lds_fence();
+                dnode = ssp > 0 ? uniformi(s.sstack[--ssp]) : -1;
             }
-            const int base = sp - k;
-            int node = -1;
-            if (lane < k) node = s.wstack[base + lane];
-            sp = base;
-            bool ovl = false, ovr = false;
-            int cl = 0, cr = 0;
-            if (lane < k) {
-                const float4* q = reinterpret_cast<const float4*>(&nodes[node]);
-                const float4 a = q[0], b = q[1], c = q[2], d = q[3];
-                // a = llo.xyz lhi.x | b = lhi.yz rlo.xy | c = rlo.z rhi.xyz | d = left right . .
-                ovl = box_overlaps(a.x, a.y, a.z, a.w, b.x, b.y, gb);
-                ovr = box_overlaps(b.z, b.w, c.x, c.y, c.z, c.w, gb);
-                cl = __builtin_bit_cast(int, d.x);
-                cr = __builtin_bit_cast(int, d.y);
-            }
-            const bool pl = ovl && cl >= 0, pr = ovr && cr >= 0;
-            const bool ll = ovl && cl < 0, lr = ovr && cr < 0;
-            const uint64_t bpl = __ballot(pl), bpr = __ballot(pr), bll = __ballot(ll), blr = __ballot(lr);
-            const int npl = __popcll(bpl), npr = __popcll(bpr), nll = __popcll(bll), nlr = __popcll(blr);
-            if (pl) s.wstack[sp + __popcll(bpl & lt)] = cl;
-            if (pr) s.wstack[sp + npl + __popcll(bpr & lt)] = cr;
-            if (ll) s.leaves[__popcll(bll & lt)] = ~cl;
-            if (lr) s.leaves[nll + __popcll(blr & lt)] = ~cr;
-            sp += npl + npr;
             lds_fence();
-            if (nll + nlr) expand(nll + nlr);
+            continue;
         }
+        int k = min(64, min(sp, kWideStack - sp));
+        if (k == 0) {
+            // no room for a wide step: walk the top subtree depth first
+            dnode = uniformi(s.wstack[sp - 1]);
+            sp -= 1;
+            ssp = 0;
+            continue;
+        }
+        const int base = sp - k;
+        int node = -1;
+        if (lane < k) node = s.wstack[base + lane];
+        sp = base;
+        bool ovl = false, ovr = false;
+        int cl = 0, cr = 0;
+        if (lane < k) {
+            const float4* q = reinterpret_cast<const float4*>(&nodes[node]);
+            const float4 a = q[0], b = q[1], c = q[2], d = q[3];
+            // a = llo.xyz lhi.x | b = lhi.yz rlo.xy | c = rlo.z rhi.xyz | d = left right . .
+            ovl = box_overlaps(a.x, a.y, a.z, a.w, b.x, b.y, gb);
+            ovr = box_overlaps(b.z, b.w, c.x, c.y, c.z, c.w, gb);
+            cl = __builtin_bit_cast(int, d.x);
+            cr = __builtin_bit_cast(int, d.y);
+        }
+        const bool pl = ovl && cl >= 0, pr = ovr && cr >= 0;
+        const bool ll = ovl && cl < 0, lr = ovr && cr < 0;
+        const uint64_t bpl = __ballot(pl), bpr = __ballot(pr), bll = __ballot(ll), blr = __ballot(lr);
+        const int npl = __popcll(bpl), npr = __popcll(bpr), nll = __popcll(bll);
+        lds_fence();
+        if (pl) s.wstack[sp + __popcll(bpl & lt)] = cl;
+        if (pr) s.wstack[sp + npl + __popcll(bpr & lt)] = cr;
+        if (ll) s.leaves[__popcll(bll & lt)] = ~cl;
+        if (lr) s.leaves[nll + __popcll(blr & lt)] = ~cr;
+        sp += npl + npr;
+        nl = nll + __popcll(blr);
+        lds_fence();
     }
-    if (ctail != chead) chunk(chead, (int)(ctail - chead));
     return ok;
 }
 
@@ -658,6 +673,32 @@ __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__
 // bits in candidate order (next candidate prefetched from LDS) and accumulates
 // w e^power and w e^power v with Kahan compensation.
 // ---------------------------------------------------------------------------
+// Candidate log: the forward appends every chunk it evaluates to a slot of 832
+// bytes {next slot, n, pad | Gaussian index per lane | point-major box-hit mask
+// per point lane}; a group's slots form a linked list headed by group_head[g].
+// Waves reserve slots 8 at a time from one counter.  The backward replays the
+// log instead of walking the tree and box-testing again.
+constexpr uint32_t kSlotBytes = 832;
+constexpr uint32_t kSlotBlock = 8;
+constexpr uint32_t kLogEnd = 0xffffffffu;
+constexpr uint32_t kLogIncomplete = 0xfffffffeu;
+
+struct LogCursor {
+    uint32_t next = 0, end = 0;     // reserved slots [next, end)
+};
+
+__device__ __forceinline__ uint32_t log_take(LogCursor& c, uint32_t* __restrict__ counter)
+{
+    if (c.next == c.end) {
+        uint32_t b = 0;
+        if ((threadIdx.x & 63) == 0) b = atomicAdd(counter, kSlotBlock);
+        b = (uint32_t)uniformi(__shfl((int)b, 0, 64));
+        c.next = b;
+        c.end = b + kSlotBlock;
+    }
+    return c.next++;
+}
+
 struct FwdLds {
     WalkLds walk;
     float4 chunk[64][3];   // mean|w, conic c0..c3, c4 c5 v id
@@ -690,7 +731,10 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
                                                       uint32_t* __restrict__ group_records,
                                                       uint32_t* __restrict__ group_pairs, uint32_t* __restrict__ err,
                                                       uint32_t* __restrict__ queue,
-                                                      const unsigned long long* __restrict__ ext_sum)
+                                                      const unsigned long long* __restrict__ ext_sum,
+                                                      unsigned char* __restrict__ log, uint32_t log_cap,
+                                                      uint32_t* __restrict__ log_counter,
+                                                      uint32_t* __restrict__ group_head)
 {
     __shared__ FwdLds lds_all[4];
     const int wid = uniformi(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -699,6 +743,7 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
     mean_extent(ext_sum, P, reach);
     int x = (int)(blockIdx.x & 7u), tries = 0;
     bool ok = true;
+    LogCursor cur;
     while (true) {
         const int g = next_group(queue, NG, x, tries);
         if (g >= NG) break;
@@ -712,11 +757,14 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
         float W = 0.f, cW = 0.f, U = 0.f, cU = 0.f;
         uint32_t npairs = 0, nrec = 0;
         bool inrun = false;
+        uint32_t head = kLogEnd, prev = kLogEnd;
+        bool logged = true;
 
         auto chunk = [&](uint32_t base, int n) {
             float lox = 1.f, loy = 1.f, loz = 1.f, hix = 0.f, hiy = 0.f, hiz = 0.f;  // empty box for idle lanes
+            int gi = -1;
             if (lane < n) {
-                const int gi = s.walk.ring[(base + lane) & (kRing - 1)];
+                gi = s.walk.ring[(base + lane) & (kRing - 1)];
                 const float4* q = reinterpret_cast<const float4*>(&rec[gi]);
                 const float4 a = q[0], b = q[1], c = q[2], d = q[3], e = q[4];
                 s.chunk[lane][0] = a;
@@ -734,6 +782,23 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
             }
             lds_fence();
             uint64_t m = inrun ? (((uint64_t)(uint32_t)mhi << 32) | (uint32_t)mlo) : 0ull;
+            if (logged) {
+                const uint32_t slot = log_take(cur, log_counter);
+                if (slot < log_cap) {
+                    unsigned char* sb = log + (size_t)slot * kSlotBytes;
+                    reinterpret_cast<int*>(sb + 64)[lane] = gi;
+                    reinterpret_cast<uint2*>(sb + 320)[lane] = make_uint2((uint32_t)m, (uint32_t)(m >> 32));
+                    if (lane == 0) {
+                        reinterpret_cast<uint32_t*>(sb)[0] = kLogEnd;
+                        reinterpret_cast<uint32_t*>(sb)[1] = (uint32_t)n;
+                        if (prev != kLogEnd) reinterpret_cast<uint32_t*>(log + (size_t)prev * kSlotBytes)[0] = slot;
+                    }
+                    if (prev == kLogEnd) head = slot;
+                    prev = slot;
+                } else {
+                    logged = false;
+                }
+            }
             uint64_t accm = 0;
             if (m) {
                 int j = __builtin_ctzll(m);
@@ -803,6 +868,8 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
         if (lane == 0) {
             group_records[g] = nrec;
             group_pairs[g] = tp;
+            group_head[g] = logged ? head : kLogIncomplete;
+            if (!logged) atomicOr(err, (uint32_t)GVC_ERRFLAG_CHUNK_OVERFLOW);
         }
     }
     if (!ok && lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_STACK_OVERFLOW);
@@ -824,33 +891,101 @@ struct alignas(16) PartialRecord {
 static_assert(sizeof(PartialRecord) == 48, "PartialRecord layout");
 
 struct BwdLds {
-    WalkLds walk;
-    float4 ptsA[64];  // x, y, z, V
-    float2 ptsB[64];  // gV / W, gW   (zero where W <= cutoff)
+    float4 ptsA[64];      // x, y, z, V
+    float2 ptsB[64];      // gV / W, gW   (zero where W <= cutoff)
+    float4 slot[64][3];   // one slice's 11 partial sums + accepted flag
+    float4 cand[64][3];   // the chunk's candidates: mean|w, conic c0..c3, c4 c5 v .
+    uint8_t owner[256];   // slice -> candidate lane
 };
 
+// Inclusive prefix sum over the wavefront.
+__device__ __forceinline__ uint32_t wave_inscan_u32(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = (uint32_t)__shfl_up((int)v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+// 64x64 bit-matrix transpose across the wavefront: lane r holds row r (bit c =
+// element (r, c)); returns column `lane`.  Six block-swap stages: at stage j the
+// lanes with bit j clear trade their bits with bit j set against the partner's
+// (lane ^ j) bits with bit j clear.
+__device__ __forceinline__ uint64_t wave_transpose64(uint64_t a)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t lo = (uint32_t)a, hi = (uint32_t)(a >> 32);
+    {
+        const uint32_t plo = (uint32_t)__shfl_xor((int)lo, 32, 64), phi = (uint32_t)__shfl_xor((int)hi, 32, 64);
+        if (lane & 32) lo = phi;
+        else hi = plo;
+    }
+#pragma unroll
+    for (int j = 16; j >= 1; j >>= 1) {
+        const uint32_t m = j == 16 ? 0x0000ffffu : j == 8 ? 0x00ff00ffu : j == 4 ? 0x0f0f0f0fu : j == 2 ? 0x33333333u
+                                                                                                         : 0x55555555u;
+        const uint32_t plo = (uint32_t)__shfl_xor((int)lo, j, 64), phi = (uint32_t)__shfl_xor((int)hi, j, 64);
+        if (lane & j) {
+            lo = (lo & ~m) | ((plo & ~m) >> j);
+            hi = (hi & ~m) | ((phi & ~m) >> j);
+        } else {
+            lo = (lo & m) | ((plo & m) << j);
+            hi = (hi & m) | ((phi & m) << j);
+        }
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// m with its k lowest set bits cleared (k < popcount(m)).
+__device__ __forceinline__ uint64_t drop_low_bits(uint64_t m, uint32_t k)
+{
+    if (k == 0) return m;
+    uint32_t w = (uint32_t)m;
+    int base = 0;
+    const uint32_t c0 = (uint32_t)__popc(w);
+    if (k >= c0) {
+        k -= c0;
+        w = (uint32_t)(m >> 32);
+        base = 32;
+    }
+#pragma unroll
+    for (int st = 16; st >= 1; st >>= 1) {
+        const uint32_t c = (uint32_t)__popc(w & ((1u << st) - 1u));
+        if (k >= c) {
+            k -= c;
+            w >>= st;
+            base += st;
+        }
+    }
+    // bit `base` is the k-th set bit: keep it and everything above
+    return m & ~((1ull << base) - 1ull);
+}
+
 __global__ __launch_bounds__(256) void backward_kernel(
-    const GRecord* __restrict__ rec, const BNode* __restrict__ nodes, const float* __restrict__ root_box, int P,
-    int NL, int L, const float4* __restrict__ pts, int S, int NG, const float* __restrict__ out_cells,
+    const GRecord* __restrict__ rec, const float4* __restrict__ pts, int NG, const float* __restrict__ out_cells,
     const float* __restrict__ out_weights, const float* __restrict__ gcells, const float* __restrict__ gweights,
     const uint32_t* __restrict__ group_offsets, uint64_t capacity, PartialRecord* __restrict__ out_rec,
     uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ err, uint32_t* __restrict__ queue,
-    const unsigned long long* __restrict__ ext_sum)
+    const unsigned char* __restrict__ log, const uint32_t* __restrict__ group_head)
 {
     __shared__ BwdLds lds_all[4];
     const int wid = uniformi(threadIdx.x >> 6), lane = threadIdx.x & 63;
     BwdLds& s = lds_all[wid];
     const uint64_t lt = lanemask_lt();
-    float reach[3];
-    mean_extent(ext_sum, P, reach);
     int x = (int)(blockIdx.x & 7u), tries = 0;
-    bool ok = true;
     while (true) {
         const int g = next_group(queue, NG, x, tries);
         if (g >= NG) break;
-        const float4* __restrict__ gp = pts + (size_t)g * 64;
-        const float4 pme = gp[lane];
-        const float px = pme.x, py = pme.y, pz = pme.z;
+        const uint32_t head = uniformi((int)group_head[g]);
+        if (head == kLogEnd) continue;
+        if (head == kLogIncomplete) {
+            if (lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_CHUNK_OVERFLOW);
+            continue;
+        }
+        const float4 pme = pts[(size_t)g * 64 + lane];
         const int orig = __builtin_bit_cast(int, pme.w);
         float V = 0.f, gVi = 0.f, gWe = 0.f;
         if (orig >= 0) {
@@ -862,75 +997,102 @@ __global__ __launch_bounds__(256) void backward_kernel(
             }
         }
         lds_fence();
-        s.ptsA[lane] = make_float4(px, py, pz, V);
+        s.ptsA[lane] = make_float4(pme.x, pme.y, pme.z, V);
         s.ptsB[lane] = make_float2(gVi, gWe);
-        const int nv = __popcll(__ballot(orig >= 0));
-        const uint64_t starts = split_runs(px, py, pz, nv, reach);
         lds_fence();
 
         uint32_t base_out = group_offsets[g];
         const uint32_t limit = group_offsets[g + 1];
-        uint64_t runmask = 0;
 
-        auto chunk = [&](uint32_t base, int n) {
+        // one logged chunk: n candidates, lane j's Gaussian gi, point-major box hits mrow
+        auto chunk = [&](int n, int gi, uint64_t mrow) {
             const bool active = lane < n;
-            const int gi = active ? s.walk.ring[(base + lane) & (kRing - 1)] : 0;
-            const float4* q = reinterpret_cast<const float4*>(&rec[gi]);
-            const float4 ra = q[0], rb = q[1], rc = q[2], rd = q[3], re = q[4];
-            const float lox = active ? rd.x : 1.f, hix = active ? rd.w : 0.f;
-            uint32_t mlo = 0, mhi = 0;
-#pragma unroll 4
-            for (int p = 0; p < 32; ++p) {
-                const float4 a = s.ptsA[p];
-                const float4 b = s.ptsA[32 + p];
-                const bool ia = lox <= a.x && a.x <= hix && rd.y <= a.y && a.y <= re.x && rd.z <= a.z && a.z <= re.y;
-                const bool ib = lox <= b.x && b.x <= hix && rd.y <= b.y && b.y <= re.x && rd.z <= b.z && b.z <= re.y;
-                mlo |= ia ? (1u << p) : 0u;
-                mhi |= ib ? (1u << p) : 0u;
+            {
+                const float4* q = reinterpret_cast<const float4*>(&rec[active ? gi : 0]);
+                s.cand[lane][0] = q[0];
+                s.cand[lane][1] = q[1];
+                s.cand[lane][2] = q[2];
             }
-            uint64_t m = (((uint64_t)mhi << 32) | mlo) & runmask;
+            const uint64_t tm = wave_transpose64(mrow);
+            const uint64_t m = active ? tm : 0ull;
+            lds_fence();
+
+            // Balanced pair loop: the candidates' point lists are cut into slices of
+            // at most q pairs (q = ceil(H / 128) for H pairs in the chunk, so at most
+            // 192 slices), dealt to the lanes 64 at a time; each slice's sums land in
+            // an LDS slot and the candidate's lane adds its slices in order.
+            const uint32_t h = (uint32_t)__popcll(m);
+            const uint32_t H = uniformi((int)wave_sumu(h));
             float dv = 0.f, dw = 0.f, dmx = 0.f, dmy = 0.f, dmz = 0.f;
             float dc0 = 0.f, dc1 = 0.f, dc2 = 0.f, dc3 = 0.f, dc4 = 0.f, dc5 = 0.f;
             bool any = false;
-            if (m) {
-                int p = __builtin_ctzll(m);
-                m &= m - 1;
-                float4 a = s.ptsA[p];
-                float2 b = s.ptsB[p];
-                while (true) {
-                    const bool more = m != 0;
-                    const int pn = more ? __builtin_ctzll(m) : p;
-                    const float4 an = s.ptsA[pn];
-                    const float2 bn = s.ptsB[pn];
-                    const float dx = a.x - ra.x, dy = a.y - ra.y, dz = a.z - ra.z;
-                    float r0, r1, r2;
-                    const float power = pair_power(rb.x, rb.y, rb.z, rb.w, rc.x, rc.y, dx, dy, dz, r0, r1, r2);
-                    if (in_window(power)) {
-                        any = true;
-                        const float e = __builtin_amdgcn_exp2f(power * kLog2e);
-                        const float wgt = ra.w * e;
-                        dv = __builtin_fmaf(b.x, wgt, dv);
-                        const float t = __builtin_fmaf(b.x, rc.z - a.w, b.y);
-                        dw = __builtin_fmaf(t, e, dw);
-                        const float st = wgt * t;
-                        dmx = __builtin_fmaf(st, r0, dmx);
-                        dmy = __builtin_fmaf(st, r1, dmy);
-                        dmz = __builtin_fmaf(st, r2, dmz);
-                        const float dq = -0.5f * st;
-                        const float h0 = dq * dx, h1 = dq * dy, h2 = dq * dz;
-                        const float h0x2 = h0 + h0, h1x2 = h1 + h1;
-                        dc0 = __builtin_fmaf(h0, dx, dc0);
-                        dc1 = __builtin_fmaf(h0x2, dy, dc1);
-                        dc2 = __builtin_fmaf(h0x2, dz, dc2);
-                        dc3 = __builtin_fmaf(h1, dy, dc3);
-                        dc4 = __builtin_fmaf(h1x2, dz, dc4);
-                        dc5 = __builtin_fmaf(h2, dz, dc5);
+            if (H > 0) {
+                const uint32_t q = H > 128 ? (H + 127) / 128 : 1u;
+                const uint32_t sj = (h + q - 1) / q;
+                const uint32_t incl = wave_inscan_u32(sj);
+                const uint32_t bj = incl - sj;
+                const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+                for (uint32_t t = 0; t < sj; ++t) s.owner[bj + t] = (uint8_t)lane;
+                lds_fence();
+                for (uint32_t r0 = 0; r0 < total; r0 += 64) {
+                    const uint32_t v = r0 + (uint32_t)lane;
+                    const bool live = v < total;
+                    const int j = live ? (int)s.owner[v] : lane;
+                    const uint32_t tb = (uint32_t)__shfl((int)bj, j, 64);
+                    const uint32_t mlo_j = (uint32_t)__shfl((int)(uint32_t)m, j, 64);
+                    const uint32_t mhi_j = (uint32_t)__shfl((int)(uint32_t)(m >> 32), j, 64);
+                    const float4 ca = s.cand[j][0], cb = s.cand[j][1], cc = s.cand[j][2];
+                    const float cmx = ca.x, cmy = ca.y, cmz = ca.z, cw = ca.w;
+                    const float k0 = cb.x, k1 = cb.y, k2 = cb.z, k3 = cb.w, k4 = cc.x, k5 = cc.y, cv = cc.z;
+                    uint64_t mj = live ? drop_low_bits(((uint64_t)mhi_j << 32) | mlo_j, (v - tb) * q) : 0ull;
+                    float sv = 0.f, sw = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+                    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f;
+                    float sany = 0.f;
+                    for (uint32_t it = 0; it < q && mj != 0ull; ++it) {
+                        const int p = __builtin_ctzll(mj);
+                        mj &= mj - 1;
+                        const float4 a = s.ptsA[p];
+                        const float2 b = s.ptsB[p];
+                        const float dx = a.x - cmx, dy = a.y - cmy, dz = a.z - cmz;
+                        float r0_, r1_, r2_;
+                        const float power = pair_power(k0, k1, k2, k3, k4, k5, dx, dy, dz, r0_, r1_, r2_);
+                        if (in_window(power)) {
+                            sany = 1.f;
+                            const float e = __builtin_amdgcn_exp2f(power * kLog2e);
+                            const float wgt = cw * e;
+                            sv = __builtin_fmaf(b.x, wgt, sv);
+                            const float tt = __builtin_fmaf(b.x, cv - a.w, b.y);
+                            sw = __builtin_fmaf(tt, e, sw);
+                            const float st = wgt * tt;
+                            sx = __builtin_fmaf(st, r0_, sx);
+                            sy = __builtin_fmaf(st, r1_, sy);
+                            sz = __builtin_fmaf(st, r2_, sz);
+                            const float dq = -0.5f * st;
+                            const float h0 = dq * dx, h1 = dq * dy, h2 = dq * dz;
+                            const float h0x2 = h0 + h0, h1x2 = h1 + h1;
+                            s0 = __builtin_fmaf(h0, dx, s0);
+                            s1 = __builtin_fmaf(h0x2, dy, s1);
+                            s2 = __builtin_fmaf(h0x2, dz, s2);
+                            s3 = __builtin_fmaf(h1, dy, s3);
+                            s4 = __builtin_fmaf(h1x2, dz, s4);
+                            s5 = __builtin_fmaf(h2, dz, s5);
+                        }
                     }
-                    if (!more) break;
-                    m &= m - 1;
-                    p = pn;
-                    a = an;
-                    b = bn;
+                    s.slot[lane][0] = make_float4(sv, sw, sx, sy);
+                    s.slot[lane][1] = make_float4(sz, s0, s1, s2);
+                    s.slot[lane][2] = make_float4(s3, s4, s5, sany);
+                    lds_fence();
+                    // this candidate's slices inside the round
+                    const uint32_t lo = bj > r0 ? bj : r0;
+                    const uint32_t hi = (bj + sj) < (r0 + 64) ? (bj + sj) : (r0 + 64);
+                    for (uint32_t u = lo; u < hi; ++u) {
+                        const float4 A = s.slot[u - r0][0], B = s.slot[u - r0][1], C = s.slot[u - r0][2];
+                        dv += A.x; dw += A.y; dmx += A.z; dmy += A.w;
+                        dmz += B.x; dc0 += B.y; dc1 += B.z; dc2 += B.w;
+                        dc3 += C.x; dc4 += C.y; dc5 += C.z;
+                        any = any || (C.w != 0.f);
+                    }
+                    lds_fence();
                 }
             }
             const uint64_t bal = __ballot(any);
@@ -951,17 +1113,33 @@ __global__ __launch_bounds__(256) void backward_kernel(
             }
             base_out += (uint32_t)__popcll(bal);
         };
-        uint64_t rs = starts;
-        while (rs != 0ull) {
-            const int a = __builtin_ctzll(rs);
-            rs &= rs - 1;
-            const int b = rs ? __builtin_ctzll(rs) : nv;
-            runmask = (b >= 64 ? ~0ull : ((1ull << b) - 1ull)) & ~((1ull << a) - 1ull);
-            const GroupBox gb = masked_box(px, py, pz, lane >= a && lane < b);
-            ok = enumerate_candidates(rec, nodes, root_box, P, NL, L, gb, s.walk, chunk) && ok;
+        // replay the group's log; the next slot's header and rows are loaded
+        // before the current chunk is processed
+        uint32_t slot = head;
+        const unsigned char* sb = log + (size_t)slot * kSlotBytes;
+        uint32_t nxt = uniformi((int)reinterpret_cast<const uint32_t*>(sb)[0]);
+        int n = uniformi((int)reinterpret_cast<const uint32_t*>(sb)[1]);
+        int gi = reinterpret_cast<const int*>(sb + 64)[lane];
+        uint2 mr = reinterpret_cast<const uint2*>(sb + 320)[lane];
+        while (true) {
+            uint32_t nxt2 = kLogEnd;
+            int n2 = 0, gi2 = -1;
+            uint2 mr2 = make_uint2(0u, 0u);
+            if (nxt != kLogEnd) {
+                const unsigned char* sb2 = log + (size_t)nxt * kSlotBytes;
+                nxt2 = uniformi((int)reinterpret_cast<const uint32_t*>(sb2)[0]);
+                n2 = uniformi((int)reinterpret_cast<const uint32_t*>(sb2)[1]);
+                gi2 = reinterpret_cast<const int*>(sb2 + 64)[lane];
+                mr2 = reinterpret_cast<const uint2*>(sb2 + 320)[lane];
+            }
+            chunk(n, gi, ((uint64_t)mr.y << 32) | mr.x);
+            if (nxt == kLogEnd) break;
+            nxt = nxt2;
+            n = n2;
+            gi = gi2;
+            mr = mr2;
         }
     }
-    if (!ok && lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_STACK_OVERFLOW);
 }
 
 // Pads the sort input beyond the emitted records with key P (sorts last).
@@ -1158,10 +1336,13 @@ struct ForwardLayout {
     Box6* sparse;  // levels x NL leaf-range boxes
     float* root_box;
     uint32_t *group_records, *group_offsets, *group_pairs, *scan_scratch, *err;
-    uint32_t* queues;  // 8 forward + 8 backward group queues
+    uint32_t* queues;  // 8 forward + 8 backward group queues, log counter
     unsigned long long *stats, *ext_sum, *ext_part;
+    uint32_t* group_head;
+    unsigned char* log;
+    uint32_t log_cap;
     int NL, NG;
-    ForwardLayout(Arena& a, int P, int S, int L)
+    ForwardLayout(Arena& a, int P, int S, int L, unsigned chunk_slots)
     {
         NL = P > 0 ? (P + L - 1) / L : 0;
         NG = (S + 63) / 64;
@@ -1180,10 +1361,14 @@ struct ForwardLayout {
         group_pairs = a.take<uint32_t>((size_t)NG + 1);
         scan_scratch = a.take<uint32_t>(scan_scratch_elems((size_t)NG) + 1);
         err = a.take<uint32_t>(4);
-        queues = a.take<uint32_t>(16);
-        stats = a.take<unsigned long long>(2);
+        queues = a.take<uint32_t>(20);
+        stats = a.take<unsigned long long>(4);
         ext_sum = a.take<unsigned long long>(4);
         ext_part = a.take<unsigned long long>(3 * (((size_t)P + 255) / 256) + 3);
+        group_head = a.take<uint32_t>((size_t)NG + 1);
+        const uint64_t cap = (uint64_t)(chunk_slots ? chunk_slots : 16u) * (uint64_t)(NG > 0 ? NG : 1);
+        log_cap = (uint32_t)(cap < 0xfff00000ull ? cap : 0xfff00000ull);
+        log = reinterpret_cast<unsigned char*>(a.take<uint4>((size_t)log_cap * (kSlotBytes / 16)));
     }
 };
 
@@ -1275,7 +1460,7 @@ size_t forward_state_bytes(int P, int S, const gvc_params* prm)
 {
     if (check_params(prm) != GVC_OK) return 0;
     Arena a(nullptr, 0);
-    ForwardLayout l(a, P, S, prm->leaf_size);
+    ForwardLayout l(a, P, S, prm->leaf_size, prm->chunk_slots);
     return a.off + 256;
 }
 
@@ -1288,7 +1473,7 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
     if (P < 0 || S < 0) return set_error(GVC_ERR_INVALID_ARGUMENT, "negative size");
     const int L = prm->leaf_size;
     Arena a(state, state_bytes);
-    ForwardLayout l(a, P, S, L);
+    ForwardLayout l(a, P, S, L, prm->chunk_slots);
     if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "forward state buffer too small");
     Arena sa(const_cast<void*>(sample_set), sample_set_bytes(S));
     SampleSetLayout sl(sa, S);
@@ -1354,16 +1539,17 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
     if (NG > 0) {
         const int T = 256;
         GVC_TRY(fill_u32(l.queues, 0u, 8, st));
+        GVC_TRY(fill_u32(l.queues + 16, 0u, 4, st));
         if (prm->flags & GVC_FLAG_KAHAN) {
             const unsigned nb = walk_blocks(reinterpret_cast<const void*>(&forward_kernel<true>), T, NG);
             forward_kernel<true><<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells,
                                                    out_weights, l.group_records, l.group_pairs, l.err, l.queues,
-                                                   l.ext_sum);
+                                                   l.ext_sum, l.log, l.log_cap, l.queues + 16, l.group_head);
         } else {
             const unsigned nb = walk_blocks(reinterpret_cast<const void*>(&forward_kernel<false>), T, NG);
             forward_kernel<false><<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells,
                                                     out_weights, l.group_records, l.group_pairs, l.err, l.queues,
-                                                    l.ext_sum);
+                                                    l.ext_sum, l.log, l.log_cap, l.queues + 16, l.group_head);
         }
         GVC_LAUNCH_CHECK("forward");
     }
@@ -1372,21 +1558,23 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
 }
 
 int forward_stats(const void* state, int P, int S, const gvc_params* prm, uint64_t* n_records, uint64_t* n_pairs,
-                  uint32_t* error_flags, hipStream_t st)
+                  uint32_t* error_flags, uint64_t* chunk_slots_used, hipStream_t st)
 {
     GVC_TRY(check_params(prm));
     Arena a(const_cast<void*>(state), (size_t)-1);
-    ForwardLayout l(a, P, S, prm->leaf_size);
+    ForwardLayout l(a, P, S, prm->leaf_size, prm->chunk_slots);
     stats_kernel<<<1, 256, 0, st>>>(l.group_pairs, l.NG, l.group_offsets, l.stats);
     GVC_LAUNCH_CHECK("stats");
     unsigned long long h[2] = { 0, 0 };
-    uint32_t e = 0;
+    uint32_t e = 0, used = 0;
     GVC_HIP(hipMemcpyAsync(h, l.stats, sizeof(h), hipMemcpyDeviceToHost, st));
     GVC_HIP(hipMemcpyAsync(&e, l.err, sizeof(e), hipMemcpyDeviceToHost, st));
+    GVC_HIP(hipMemcpyAsync(&used, l.queues + 16, sizeof(used), hipMemcpyDeviceToHost, st));
     GVC_HIP(hipStreamSynchronize(st));
     if (n_records) *n_records = h[0];
     if (n_pairs) *n_pairs = h[1];
     if (error_flags) *error_flags = e;
+    if (chunk_slots_used) *chunk_slots_used = used;
     return GVC_OK;
 }
 
@@ -1409,7 +1597,7 @@ int mixture_backward(int P, const float* means, const float* scales, const float
     if (capacity > 0xffffffffull) return set_error(GVC_ERR_INVALID_ARGUMENT, "record capacity above 2^32");
     const int L = prm->leaf_size;
     Arena a(state, state_bytes);
-    ForwardLayout l(a, P, S, L);
+    ForwardLayout l(a, P, S, L, prm->chunk_slots);
     if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "forward state buffer too small");
     Arena ba(ws, ws_bytes);
     BackwardLayout bl(ba, P, capacity);
@@ -1422,9 +1610,9 @@ int mixture_backward(int P, const float* means, const float* scales, const float
         const int T = 256;
         GVC_TRY(fill_u32(l.queues + 8, 0u, 8, st));
         const unsigned nb = walk_blocks(reinterpret_cast<const void*>(&backward_kernel), T, NG);
-        backward_kernel<<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells, out_weights,
-                                          gcells, gweights, l.group_offsets, capacity, bl.parts, bl.keys, bl.vals,
-                                          l.err, l.queues + 8, l.ext_sum);
+        backward_kernel<<<nb, T, 0, st>>>(l.rec, sl.pts, NG, out_cells, out_weights, gcells, gweights,
+                                          l.group_offsets, capacity, bl.parts, bl.keys, bl.vals, l.err, l.queues + 8,
+                                          l.log, l.group_head);
         GVC_LAUNCH_CHECK("backward");
         pad_keys_kernel<<<blocks_for(capacity, 256), 256, 0, st>>>(bl.keys, bl.vals, l.group_offsets + NG, capacity,
                                                                   (uint32_t)P);

@@ -13,6 +13,7 @@ GVC_FLAG_KAHAN = 1
 GVC_FLAG_FP64_COND = 2
 GVC_ERRFLAG_STACK_OVERFLOW = 1
 GVC_ERRFLAG_RECORD_OVERFLOW = 2
+GVC_ERRFLAG_CHUNK_OVERFLOW = 4
 GVC_STAGE_BUILD = 1
 GVC_STAGE_WALK = 2
 GVC_STAGE_REDUCE = 1
@@ -24,7 +25,7 @@ class GvcError(RuntimeError):
 
 class gvc_params(ctypes.Structure):
     _fields_ = [("leaf_size", ctypes.c_int), ("block_threads", ctypes.c_int),
-                ("flags", ctypes.c_uint), ("stages", ctypes.c_uint)]
+                ("flags", ctypes.c_uint), ("stages", ctypes.c_uint), ("chunk_slots", ctypes.c_uint)]
 
 
 _lib = None
@@ -51,7 +52,7 @@ _SIGS = {
     "gvc_forward_state_bytes": (_SZ, [_I, _I, _PP]),
     "gvc_mixture_forward": (_I, [_I, _P, _P, _P, _P, _P, _F, _P, _I, _PP, _P, _P, _P, _P, _SZ, _P]),
     "gvc_forward_stats": (_I, [_P, _I, _I, _PP, ctypes.POINTER(_U64), ctypes.POINTER(_U64),
-                               ctypes.POINTER(ctypes.c_uint32), _P]),
+                               ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_U64), _P]),
     "gvc_backward_bytes": (_SZ, [_I, _U64]),
     "gvc_mixture_backward": (_I, [_I, _P, _P, _P, _P, _P, _F, _P, _I, _PP, _P, _P, _P, _P, _P, _SZ, _U64,
                                   _P, _SZ, _P, _P, _P, _P, _P, _P]),

@@ -19,17 +19,22 @@ DEFAULT_FLAGS = GVC_FLAG_KAHAN
 
 @dataclass(frozen=True)
 class MixtureParams:
-    """Launch parameters (the autotuner's search space) and precision flags."""
+    """Launch parameters (the autotuner's search space) and precision flags.
+    chunk_slots: candidate-log slots per 64-point group; 0 = the module's
+    high-water mark, which grows whenever a forward ran out of log."""
     leaf_size: int = DEFAULT_LEAF_SIZE
     block_threads: int = DEFAULT_BLOCK_THREADS
     kahan: bool = True
     fp64_cond: bool = False
+    chunk_slots: int = 0
 
     def c(self):
         flags = (GVC_FLAG_KAHAN if self.kahan else 0) | (GVC_FLAG_FP64_COND if self.fp64_cond else 0)
-        return gvc_params(int(self.leaf_size), int(self.block_threads), flags, 0)
+        return gvc_params(int(self.leaf_size), int(self.block_threads), flags, 0,
+                          int(self.chunk_slots or _chunk_slots))
 
 
+_chunk_slots = 16          # candidate-log slots per group used when params.chunk_slots == 0
 _default_params = MixtureParams()
 _record_capacity = None   # fixed backward capacity (graph capture); None = read it
 _stage_timer = None       # list collecting (stage, start_event, end_event) when set
@@ -119,23 +124,36 @@ class SampleSet:
 
 
 class ForwardState:
-    """BVH, records and per-group counts kept from a forward for its backward."""
+    """Records, per-group counts and the candidate log kept from a forward for its
+    backward."""
 
-    def __init__(self, buffer, params, P, S):
+    def __init__(self, buffer, cparams, params, P, S):
         self.buffer = buffer
         self.params = params
         self.P = P
         self.S = S
-        self._c = params.c()
+        self._c = cparams
+        self.chunk_slots_used = 0
 
     def stats(self):
-        """(n_records, n_pairs, error_flags) -- synchronises with the host."""
+        """(n_records, n_pairs, error_flags) -- synchronises with the host; also
+        sets ``chunk_slots_used``."""
         nr = ctypes.c_uint64(0)
         npairs = ctypes.c_uint64(0)
         err = ctypes.c_uint32(0)
+        used = ctypes.c_uint64(0)
         call("gvc_forward_stats", _ptr(self.buffer), self.P, self.S, ctypes.byref(self._c), ctypes.byref(nr),
-             ctypes.byref(npairs), ctypes.byref(err), _stream(self.buffer.device))
+             ctypes.byref(npairs), ctypes.byref(err), ctypes.byref(used), _stream(self.buffer.device))
+        self.chunk_slots_used = int(used.value)
         return int(nr.value), int(npairs.value), int(err.value)
+
+
+def _grow_chunk_slots(fstate):
+    """Raises the module's candidate-log high-water mark to fit ``fstate``'s forward."""
+    global _chunk_slots
+    NG = max(1, (fstate.S + 63) // 64)
+    need = (fstate.chunk_slots_used + NG - 1) // NG
+    _chunk_slots = max(_chunk_slots, need + need // 4 + 2)
 
 
 def mixture_forward(means3D, scales, rotations, values, weights, scale_modifier, sample_set, params=None,
@@ -156,19 +174,28 @@ def mixture_forward(means3D, scales, rotations, values, weights, scale_modifier,
         raise ValueError("sample set and Gaussians are on different devices")
     S = sample_set.S
     cp = params.c()
+    return _forward(P, S, dev, cp, params, means3D, scales, rotations, values, weights, scale_modifier, sample_set,
+                    want_conics)
+
+
+def _forward(P, S, dev, cp, params, means3D, scales, rotations, values, weights, scale_modifier, sample_set,
+             want_conics, outputs=None):
     nbytes = lib().gvc_forward_state_bytes(P, S, ctypes.byref(cp))
     if nbytes == 0:
         raise _lib.GvcError("invalid mixture parameters")
     state = torch.empty(nbytes, dtype=torch.uint8, device=dev)
-    cells = torch.empty(S, dtype=torch.float32, device=dev)
-    cw = torch.empty(S, dtype=torch.float32, device=dev)
+    if outputs is None:
+        cells = torch.empty(S, dtype=torch.float32, device=dev)
+        cw = torch.empty(S, dtype=torch.float32, device=dev)
+    else:
+        cells, cw = outputs
     conics = torch.empty(6 * P, dtype=torch.float32, device=dev) if want_conics else None
     _staged_call("gvc_mixture_forward",
                  (P, _ptr(means3D), _ptr(scales), _ptr(rotations), _ptr(values), _ptr(weights),
                   float(scale_modifier), _ptr(sample_set.buffer), S), cp,
                  (_ptr(cells), _ptr(cw), _ptr(conics), _ptr(state), nbytes, _stream(dev)),
                  ((_lib.GVC_STAGE_BUILD, "fwd_build"), (_lib.GVC_STAGE_WALK, "fwd_walk")))
-    fs = ForwardState(state, params, P, S)
+    fs = ForwardState(state, cp, params, P, S)
     if _audit is not None:
         _audit.append(fs)
     return cells, cw, conics, fs
@@ -199,7 +226,18 @@ def mixture_backward(fstate, means3D, scales, rotations, values, weights, scale_
         return gm, gs, gr, gv, gw
     cap = record_capacity if record_capacity is not None else _record_capacity
     if cap is None:
-        cap = fstate.stats()[0]
+        cap, _, err = fstate.stats()
+        if err & _lib.GVC_ERRFLAG_CHUNK_OVERFLOW:
+            # the candidate log was too small: redo the forward (same inputs, same
+            # results) into a larger state; the outputs go to scratch tensors
+            _grow_chunk_slots(fstate)
+            cp2 = fstate.params.c()
+            scratch = (torch.empty(S, dtype=torch.float32, device=dev), torch.empty(S, dtype=torch.float32, device=dev))
+            _, _, _, fstate = _forward(P, S, dev, cp2, fstate.params, means3D, scales, rotations, values, weights,
+                                       scale_modifier, sample_set, False, outputs=scratch)
+            cap, _, err = fstate.stats()
+            if err & _lib.GVC_ERRFLAG_CHUNK_OVERFLOW:
+                raise _lib.GvcError("candidate log still too small after growing it")
     cp = fstate._c
     nbytes = lib().gvc_backward_bytes(P, cap)
     ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)

@@ -48,15 +48,17 @@ enum gvc_status {
 
 /* Launch/tuning parameters of the mixture kernels (chosen by the autotuner). */
 typedef struct gvc_params {
-    int leaf_size;      /* Gaussians per BVH leaf: 1..64                        */
-    int block_threads;  /* threads per block of the traversal kernels: 64..1024 */
-    unsigned flags;     /* GVC_FLAG_*                                           */
-    unsigned stages;    /* GVC_STAGE_* mask, 0 = all                            */
+    int leaf_size;        /* Gaussians per BVH leaf: 1..64                        */
+    int block_threads;    /* threads per block of the traversal kernels: 64..1024 */
+    unsigned flags;       /* GVC_FLAG_*                                           */
+    unsigned stages;      /* GVC_STAGE_* mask, 0 = all                            */
+    unsigned chunk_slots; /* candidate-log slots per 64-point group, 0 = 16       */
 } gvc_params;
 
 /* Error flags reported by gvc_forward_stats (bitwise OR). */
 #define GVC_ERRFLAG_STACK_OVERFLOW 1u
 #define GVC_ERRFLAG_RECORD_OVERFLOW 2u
+#define GVC_ERRFLAG_CHUNK_OVERFLOW 4u  /* candidate log full: rerun the forward with more chunk_slots */
 
 int gvc_version(void);
 const char *gvc_status_string(int status);
@@ -97,8 +99,11 @@ int gvc_sample_set_build(int S, const float *samples, void *sample_set, size_t s
  * K5/K7, K6: CR/forward.cu:10-274).  out_cells/out_weights: [S]; sentinel (-1, 0)
  * where the total weight is <= 1e-2; P == 0 leaves zeros (as the reference).
  * conics: optional [P,6] output (the reference's third forward result).
- * `state` (gvc_forward_state_bytes) keeps the Gaussian BVH and the per-group
- * record counts for the backward; it must not be reused until the backward ran. */
+ * `state` (gvc_forward_state_bytes) keeps the Gaussian records, the per-group
+ * record counts and the candidate log for the backward: every 64-candidate
+ * chunk a group's walk visited, with its 64x64 box-hit bit mask, in
+ * params->chunk_slots x groups slots of 832 bytes.  It must not be reused until
+ * the backward ran. */
 size_t gvc_forward_state_bytes(int P, int S, const gvc_params *params);
 int gvc_mixture_forward(int P, const float *means3D, const float *scales, const float *rotations,
                         const float *values, const float *weights, float scale_modifier,
@@ -109,9 +114,11 @@ int gvc_mixture_forward(int P, const float *means3D, const float *scales, const 
 /* BLOCKING: copies the forward's counters to the host (one small D2H copy and a
  * stream synchronisation).  n_records = (Gaussian, 64-point group) partials the
  * backward will emit; n_pairs = accepted (Gaussian, point) pairs; error_flags =
- * GVC_ERRFLAG_* raised so far on this state.  Any pointer may be NULL. */
+ * GVC_ERRFLAG_* raised so far on this state; chunk_slots_used = candidate-log
+ * slots the forward asked for (a chunk_slots of ceil(used / groups) suffices
+ * for the same inputs).  Any pointer may be NULL. */
 int gvc_forward_stats(const void *state, int P, int S, const gvc_params *params, uint64_t *n_records,
-                      uint64_t *n_pairs, uint32_t *error_flags, void *stream);
+                      uint64_t *n_pairs, uint32_t *error_flags, uint64_t *chunk_slots_used, void *stream);
 
 /* Backward: gradients for means3D [P,3], scales [P,3], rotations [P,4], values
  * [P,1], weights [P,1] from dL/dout_cells, dL/dout_cell_weights [S].  Replaces
