@@ -21,7 +21,9 @@
 // d = p - v3: the first term covers the float rounding of each M_ij (16x 2^-24),
 // the second the error of the double inverse, |dM_ij| <= 6 u cond_F(T) |M|_F with
 // u = 2^-53 and cond_F(T) <= 1e6 (a 300x margin).  Otherwise -- and always for
-// tets with cond_F(T) > 1e6 -- the exact predicate decides.  Order inside a cell is irrelevant (lowest index wins).
+// tets with cond_F(T) > 1e6 -- the exact predicate decides.  A cell's entries are
+// sorted by tet index, so the scan stops at the first tet that contains the point
+// (lowest index wins) or at the first undecided one (the exact pass redoes it).
 #include <math.h>
 #include <string.h>
 
@@ -165,6 +167,47 @@ __global__ void tet_fill_kernel(const float* __restrict__ boxes, int C, const Te
             }
 }
 
+// Sorts each cell's entries by tet index, so the locate kernel can stop at the
+// first containing tet.  One wavefront per cell: up to kSortCap entries are staged
+// in LDS and each lane places its entries at their rank (count of smaller tet
+// indices; a tet appears at most once per cell); longer cells fall back to an
+// insertion sort by one lane.
+constexpr int kSortCap = 512;
+
+__global__ __launch_bounds__(256) void tet_sort_cells_kernel(const uint32_t* __restrict__ starts, size_t n_grid,
+                                                             uint2* __restrict__ entries)
+{
+    __shared__ uint2 buf[4][kSortCap];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t g = (size_t)blockIdx.x * 4 + wid;
+    if (g >= n_grid) return;
+    const uint32_t b = starts[g], e = starts[g + 1];
+    const uint32_t n = e - b;
+    if (n <= 1) return;
+    if (n <= (uint32_t)kSortCap) {
+        for (uint32_t i = lane; i < n; i += 64) buf[wid][i] = entries[b + i];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t i = lane; i < n; i += 64) {
+            const uint32_t key = buf[wid][i].x;
+            uint32_t rank = 0;
+            for (uint32_t j = 0; j < n; ++j) rank += buf[wid][j].x < key ? 1u : 0u;
+            entries[b + rank] = buf[wid][i];
+        }
+        return;
+    }
+    if (lane != 0) return;
+    for (uint32_t i = b + 1; i < e; ++i) {
+        const uint2 v = entries[i];
+        uint32_t j = i;
+        while (j > b && entries[j - 1].x > v.x) {
+            entries[j] = entries[j - 1];
+            --j;
+        }
+        entries[j] = v;
+    }
+}
+
 // Exact containment of p in tet r with the simulation-of-simplicity tie rule.
 __device__ __noinline__ bool tet_contains_exact(const float* __restrict__ pts, const TetRec& r, const float* p)
 {
@@ -272,25 +315,39 @@ __global__ void tet_locate_kernel(const float* __restrict__ pts, const TetRec* _
         qp[a] = quant5(p[a], h->lo[a] + (float)c[a] * h->cell[a], h->cell[a]);
     }
     if (inside_grid) {
+        // entries of a cell are sorted by tet index: the first tet that contains
+        // p is the answer (lowest index wins), unless an earlier one is undecided
         const size_t g = (size_t)c[0] + (size_t)h->dims[0] * ((size_t)c[1] + (size_t)h->dims[1] * c[2]);
         const uint32_t b = starts[g], e = starts[g + 1];
-        for (uint32_t k = b; k < e; ++k) {
-            const uint2 en = entries[k];
-            const int t = (int)en.x;
-            if (best >= 0 && t > best) continue;
-            const uint32_t q = en.y;
-            if (qp[0] < (q & 31u) || qp[0] > ((q >> 15) & 31u) || qp[1] < ((q >> 5) & 31u) ||
-                qp[1] > ((q >> 20) & 31u) || qp[2] < ((q >> 10) & 31u) || qp[2] > ((q >> 25) & 31u))
-                continue;
-            const TetRec r = recs[t];
-            const int f = tet_filter(r, p);
-            if (f == 1) {
-                best = t;
-            } else if (f == -1) {
-                if (EXACT) {
-                    if (tet_contains_exact(pts, r, p)) best = t;
-                } else {
-                    undecided = true;
+        bool done = false;
+        for (uint32_t k0 = b; k0 < e && !done; k0 += 4) {
+            uint2 en[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) en[u] = (k0 + u < e) ? entries[k0 + u] : make_uint2(0u, 31u);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t q = en[u].y;
+                // per-axis 5-bit fields: lo <= qp <= hi (padding: lo.x = 31 > hi.x = 0, never passes)
+                const bool in = ((q & 31u) <= qp[0]) && (((q >> 5) & 31u) <= qp[1]) && (((q >> 10) & 31u) <= qp[2]) &&
+                                (((q >> 15) & 31u) >= qp[0]) && (((q >> 20) & 31u) >= qp[1]) &&
+                                (((q >> 25) & 31u) >= qp[2]);
+                if (!in || done) continue;
+                const int t = (int)en[u].x;
+                const TetRec r = recs[t];
+                const int f = tet_filter(r, p);
+                if (f == 1) {
+                    best = t;
+                    done = true;
+                } else if (f == -1) {
+                    if (EXACT) {
+                        if (tet_contains_exact(pts, r, p)) {
+                            best = t;
+                            done = true;
+                        }
+                    } else {
+                        undecided = true;
+                        done = true;
+                    }
                 }
             }
         }
@@ -432,6 +489,8 @@ int tet_locator_build(int n_points, const float* pts, int C, const int* tets, un
     GVC_TRY(fill_u32(l.cursor, 0u, n_grid, st));
     tet_fill_kernel<<<blocks_for(C, 256), 256, 0, st>>>(l.boxes, C, l.header, l.starts, l.cursor, l.entries, cap);
     GVC_LAUNCH_CHECK("tet_fill");
+    tet_sort_cells_kernel<<<blocks_for(n_grid, 4), 256, 0, st>>>(l.starts, n_grid, l.entries);
+    GVC_LAUNCH_CHECK("tet_sort_cells");
     H.n_entries = total;
     GVC_HIP(hipMemcpyAsync(l.header, &H, sizeof(H), hipMemcpyHostToDevice, st));
     GVC_HIP(hipStreamSynchronize(st));

@@ -882,13 +882,15 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
 // Gaussian's 11 partial sums; one record per candidate with an accepted pair, at
 // the position fixed by the forward's per-group counts.
 // ---------------------------------------------------------------------------
-struct alignas(16) PartialRecord {
+// One 64-byte line per record, so the reduction's gathers touch one line each.
+struct alignas(64) PartialRecord {
     int gid;           // Gaussian index in BVH order
     float dv, dw;      // d/dvalue, d/dweight
     float dmx, dmy, dmz;
     float dc[6];       // d/dconic, packed, off-diagonals counted once each
+    float pad[4];
 };
-static_assert(sizeof(PartialRecord) == 48, "PartialRecord layout");
+static_assert(sizeof(PartialRecord) == 64, "PartialRecord layout");
 
 struct BwdLds {
     float4 ptsA[64];      // x, y, z, V
@@ -1104,6 +1106,7 @@ __global__ __launch_bounds__(256) void backward_kernel(
                     o.dv = dv; o.dw = dw;
                     o.dmx = dmx; o.dmy = dmy; o.dmz = dmz;
                     o.dc[0] = dc0; o.dc[1] = dc1; o.dc[2] = dc2; o.dc[3] = dc3; o.dc[4] = dc4; o.dc[5] = dc5;
+                    o.pad[0] = o.pad[1] = o.pad[2] = o.pad[3] = 0.f;
                     out_rec[pos] = o;
                     keys[pos] = (uint32_t)gi;
                     vals[pos] = (uint32_t)pos;
@@ -1142,20 +1145,10 @@ __global__ __launch_bounds__(256) void backward_kernel(
     }
 }
 
-// Pads the sort input beyond the emitted records with key P (sorts last).
-__global__ void pad_keys_kernel(uint32_t* keys, uint32_t* vals, const uint32_t* total, uint64_t capacity, uint32_t pad)
+__global__ void segment_kernel(const uint32_t* __restrict__ keys, uint64_t n_max, const uint32_t* __restrict__ n_dev,
+                               uint32_t P, uint32_t* __restrict__ begin, uint32_t* __restrict__ end)
 {
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= capacity) return;
-    if (k >= (uint64_t)*total) {
-        keys[k] = pad;
-        vals[k] = (uint32_t)k;
-    }
-}
-
-__global__ void segment_kernel(const uint32_t* __restrict__ keys, uint64_t n, uint32_t P, uint32_t* __restrict__ begin,
-                               uint32_t* __restrict__ end)
-{
+    const uint64_t n = (uint64_t)*n_dev < n_max ? (uint64_t)*n_dev : n_max;
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t key = keys[k];
@@ -1178,12 +1171,22 @@ __global__ void reduce_chain_kernel(int P, const GRecord* __restrict__ rec, cons
     if (i >= P) return;
     double acc[11] = { 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0 };
     const uint32_t b = begin[i], e = end[i];
+    // loads run one record ahead of the (sequential, group-ordered) sums
+    float4 RA, RB, RC;
+    if (b < e) {
+        const float4* q = reinterpret_cast<const float4*>(&parts[sorted_vals[b]]);
+        RA = q[0]; RB = q[1]; RC = q[2];
+    }
     for (uint32_t k = b; k < e; ++k) {
-        const PartialRecord& pr = parts[sorted_vals[k]];
-        acc[0] += pr.dv; acc[1] += pr.dw;
-        acc[2] += pr.dmx; acc[3] += pr.dmy; acc[4] += pr.dmz;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) acc[5 + c] += pr.dc[c];
+        float4 An = RA, Bn = RB, Cn = RC;
+        if (k + 1 < e) {
+            const float4* q = reinterpret_cast<const float4*>(&parts[sorted_vals[k + 1]]);
+            An = q[0]; Bn = q[1]; Cn = q[2];
+        }
+        // A = gid dv dw dmx | B = dmy dmz dc0 dc1 | C = dc2 dc3 dc4 dc5
+        acc[0] += RA.y; acc[1] += RA.z; acc[2] += RA.w; acc[3] += RB.x; acc[4] += RB.y;
+        acc[5] += RB.z; acc[6] += RB.w; acc[7] += RC.x; acc[8] += RC.y; acc[9] += RC.z; acc[10] += RC.w;
+        RA = An; RB = Bn; RC = Cn;
     }
     const int id = rec[i].id;
     dvalues[id] = (float)acc[0];
@@ -1614,9 +1617,6 @@ int mixture_backward(int P, const float* means, const float* scales, const float
                                           l.group_offsets, capacity, bl.parts, bl.keys, bl.vals, l.err, l.queues + 8,
                                           l.log, l.group_head);
         GVC_LAUNCH_CHECK("backward");
-        pad_keys_kernel<<<blocks_for(capacity, 256), 256, 0, st>>>(bl.keys, bl.vals, l.group_offsets + NG, capacity,
-                                                                  (uint32_t)P);
-        GVC_LAUNCH_CHECK("pad_keys");
     }
     if (!(stages & GVC_STAGE_REDUCE)) return GVC_OK;
     GVC_TRY(fill_u32(bl.begin, 0u, (size_t)P, st));
@@ -1625,11 +1625,13 @@ int mixture_backward(int P, const float* means, const float* scales, const float
     const uint32_t* svals = bl.vals;
     if (NG > 0 && capacity > 0) {
         bool alt = false;
+        // the records emitted: group_offsets[NG] (capped at the capacity)
         GVC_TRY(radix_sort_u32(bl.keys, bl.vals, bl.keys_alt, bl.vals_alt, (size_t)capacity,
-                               key_bits_for((uint64_t)P), bl.scratch, &alt, st));
+                               key_bits_for((uint64_t)P), bl.scratch, &alt, st, l.group_offsets + NG));
         skeys = alt ? bl.keys_alt : bl.keys;
         svals = alt ? bl.vals_alt : bl.vals;
-        segment_kernel<<<blocks_for(capacity, 256), 256, 0, st>>>(skeys, capacity, (uint32_t)P, bl.begin, bl.end);
+        segment_kernel<<<blocks_for(capacity, 256), 256, 0, st>>>(skeys, capacity, l.group_offsets + NG, (uint32_t)P,
+                                                                  bl.begin, bl.end);
         GVC_LAUNCH_CHECK("segment");
     }
     reduce_chain_kernel<<<blocks_for(P, 128), 128, 0, st>>>(P, l.rec, bl.begin, bl.end, svals, bl.parts, scales, rots,

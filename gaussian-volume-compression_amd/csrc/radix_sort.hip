@@ -5,7 +5,9 @@
 //   3. per-tile stable ranking (wave ballot peer masks + cross-wave prefix in LDS,
 //      items taken in striped rounds so round-major order is input order), an LDS
 //      reorder into tile-sorted order, then a coalesced write-out.
-// Stability makes every pass, and therefore the sort, deterministic.  Replaces the
+// Stability makes every pass, and therefore the sort, deterministic.  An optional
+// device-side count (n_dev) sorts only the first min(n, *n_dev) items: the grid is
+// sized for n, tiles past the count stay idle and contribute empty histograms.  Replaces the
 // sorts of R/gaussian_renderer/__init__.py:77 and R/train.py:311 (torch.argsort)
 // and the CUB usage of CR/rasterizer_impl.cu; nothing is allocated inside.
 #include "gvc_internal.h"
@@ -20,10 +22,18 @@ constexpr int kTile = kThreads * kItems;
 constexpr int kWaves = kThreads / 64;
 constexpr int kBins = 256;
 
-template <class K>
-__global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* keys, size_t n, int shift, uint32_t* hist,
-                                                              uint32_t n_tiles)
+__device__ __forceinline__ size_t count_of(size_t n, const uint32_t* n_dev)
 {
+    if (!n_dev) return n;
+    const size_t m = *n_dev;
+    return m < n ? m : n;
+}
+
+template <class K>
+__global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* keys, size_t n_max, const uint32_t* n_dev,
+                                                              int shift, uint32_t* hist, uint32_t n_tiles)
+{
+    const size_t n = count_of(n_max, n_dev);
     __shared__ uint32_t h[kBins];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -39,9 +49,12 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* keys, siz
 
 template <class K>
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* keys, const uint32_t* vals, K* keys_out,
-                                                                 uint32_t* vals_out, size_t n, int shift,
+                                                                 uint32_t* vals_out, size_t n_max,
+                                                                 const uint32_t* n_dev, int shift,
                                                                  const uint32_t* offsets, uint32_t n_tiles)
 {
+    const size_t n = count_of(n_max, n_dev);
+    if ((size_t)blockIdx.x * kTile >= n) return;
     __shared__ uint32_t cnt[kWaves][kBins];
     __shared__ uint32_t run[kBins];
     __shared__ uint32_t start[kBins];
@@ -135,8 +148,8 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* keys, 
 }
 
 template <class K>
-int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, size_t n, int key_bits,
-                    uint32_t* scratch, bool* result_in_alt, hipStream_t st)
+int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, size_t n, const uint32_t* n_dev,
+                    int key_bits, uint32_t* scratch, bool* result_in_alt, hipStream_t st)
 {
     *result_in_alt = false;
     if (n <= 1 || key_bits <= 0) return GVC_OK;
@@ -150,10 +163,11 @@ int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, si
     uint32_t* dst_v = vals_alt;
     bool in_alt = false;
     for (int shift = 0; shift < key_bits; shift += 8) {
-        radix_hist_kernel<K><<<n_tiles, kThreads, 0, st>>>(src_k, n, shift, hist, n_tiles);
+        radix_hist_kernel<K><<<n_tiles, kThreads, 0, st>>>(src_k, n, n_dev, shift, hist, n_tiles);
         GVC_LAUNCH_CHECK("radix_hist");
         GVC_TRY(exclusive_scan_u32(hist, hist, (size_t)kBins * n_tiles, false, scan_scratch, st));
-        radix_scatter_kernel<K><<<n_tiles, kThreads, 0, st>>>(src_k, src_v, dst_k, dst_v, n, shift, hist, n_tiles);
+        radix_scatter_kernel<K><<<n_tiles, kThreads, 0, st>>>(src_k, src_v, dst_k, dst_v, n, n_dev, shift, hist,
+                                                              n_tiles);
         GVC_LAUNCH_CHECK("radix_scatter");
         K* tk = src_k; src_k = dst_k; dst_k = tk;
         uint32_t* tv = src_v; src_v = dst_v; dst_v = tv;
@@ -172,15 +186,17 @@ size_t radix_scratch_elems(size_t n)
 }
 
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, size_t n, int key_bits,
-                   uint32_t* scratch, bool* result_in_alt, hipStream_t st)
+                   uint32_t* scratch, bool* result_in_alt, hipStream_t st, const uint32_t* n_dev)
 {
-    return radix_sort_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, key_bits, scratch, result_in_alt, st);
+    return radix_sort_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, n_dev, key_bits, scratch, result_in_alt,
+                                     st);
 }
 
 int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, size_t n, int key_bits,
                    uint32_t* scratch, bool* result_in_alt, hipStream_t st)
 {
-    return radix_sort_impl<uint64_t>(keys, vals, keys_alt, vals_alt, n, key_bits, scratch, result_in_alt, st);
+    return radix_sort_impl<uint64_t>(keys, vals, keys_alt, vals_alt, n, nullptr, key_bits, scratch, result_in_alt,
+                                     st);
 }
 
 }  // namespace gvc
