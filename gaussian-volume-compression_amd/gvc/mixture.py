@@ -232,6 +232,7 @@ def mixture_backward(fstate, means3D, scales, rotations, values, weights, scale_
             # results) into a larger state; the outputs go to scratch tensors
             _grow_chunk_slots(fstate)
             cp2 = fstate.params.c()
+            cp2.chunk_slots = max(int(cp2.chunk_slots), _chunk_slots)
             scratch = (torch.empty(S, dtype=torch.float32, device=dev), torch.empty(S, dtype=torch.float32, device=dev))
             _, _, _, fstate = _forward(P, S, dev, cp2, fstate.params, means3D, scales, rotations, values, weights,
                                        scale_modifier, sample_set, False, outputs=scratch)

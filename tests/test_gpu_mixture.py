@@ -214,3 +214,55 @@ def test_backward_deterministic():
         outs.append([cells.cpu().numpy(), cw.cpu().numpy()] + [t.cpu().numpy() for t in grads])
     for a, b in zip(*outs):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_groups_straddling_clusters():
+    """Samples in 8 tight corner clusters of 37 points: Morton groups straddle
+    clusters, so groups are cut into runs; pair membership stays exact."""
+    rng = np.random.default_rng(11)
+    corners = np.array([[x, y, z] for x in (0.05, 0.95) for y in (0.05, 0.95) for z in (0.05, 0.95)])
+    samples = np.concatenate([c + rng.normal(scale=0.01, size=(37, 3)) for c in corners]).astype(f32)
+    g = workloads.random_gaussians(rng, 5000)
+    scales = np.exp(g["log_scales"]).astype(f32)
+    for leaf in (1, 4):
+        ss, cells, cw, conics, st = run_gpu(g["means"], scales, g["rotations"], g["values"], g["weights"], samples,
+                                            mx.MixtureParams(leaf_size=leaf))
+        conic_ref, box_ref, _ = oracle.preprocess(g["means"], scales, g["rotations"], g["weights"])
+        Vr, Wr, cnt = oracle.forward(g["means"], g["values"], g["weights"], conic_ref, box_ref, samples)
+        nrec, npairs, err = st.stats()
+        assert err == 0
+        assert npairs == int(cnt.sum())
+        assert field_err(cells.cpu().numpy(), cw.cpu().numpy(), Vr, Wr) <= 1e-5
+    _grad_check(g, scales, samples, mx.MixtureParams())
+
+
+def test_candidate_log_overflow_recovers():
+    """A forward whose candidate log is too small is redone by the (eager)
+    backward with a larger log; the gradients equal those of a roomy log."""
+    g = workloads.config0(2)
+    scales = np.exp(g["log_scales"]).astype(f32)
+    samples = g["samples"]
+    rng = np.random.default_rng(8)
+    gV = T(rng.normal(size=samples.shape[0]))
+    gW = T(rng.normal(size=samples.shape[0]))
+    saved = mx._chunk_slots
+    try:
+        outs = []
+        for slots in (64, 1):
+            ss, cells, cw, conics, st = run_gpu(g["means"], scales, g["rotations"], g["values"], g["weights"],
+                                                samples, mx.MixtureParams(chunk_slots=slots))
+            if slots == 1:
+                assert st.stats()[2] & mx._lib.GVC_ERRFLAG_CHUNK_OVERFLOW
+            grads = mx.mixture_backward(st, T(g["means"]), T(scales), T(g["rotations"]), T(g["values"]),
+                                        T(g["weights"]), 1.0, ss, cells, cw, gV, gW)
+            outs.append([t.cpu().numpy() for t in grads])
+        for a, b in zip(*outs):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        # with a fixed record capacity nothing is re-run: the overflow is reported
+        ss, cells, cw, conics, st = run_gpu(g["means"], scales, g["rotations"], g["values"], g["weights"], samples,
+                                            mx.MixtureParams(chunk_slots=1))
+        mx.mixture_backward(st, T(g["means"]), T(scales), T(g["rotations"]), T(g["values"]), T(g["weights"]), 1.0,
+                            ss, cells, cw, gV, gW, record_capacity=1 << 20)
+        assert st.stats()[2] & mx._lib.GVC_ERRFLAG_CHUNK_OVERFLOW
+    finally:
+        mx._chunk_slots = saved
