@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--graph", action="store_true", help="capture the training step in a hipGraph")
+    ap.add_argument("--recompute", action="store_true",
+                    help="backward walks the tree again instead of replaying the forward's candidate log")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (x steps): the calibration launch for the "
                          "FETCH_SIZE / WRITE_SIZE counters")
@@ -105,9 +107,10 @@ def main():
     P = W["P"]
 
     params = mx.default_params()
-    if args.leaf or args.block:
+    if args.leaf or args.block or args.recompute:
         params = mx.MixtureParams(leaf_size=args.leaf or params.leaf_size,
-                                  block_threads=args.block or params.block_threads)
+                                  block_threads=args.block or params.block_threads,
+                                  candidate_log=not args.recompute)
         mx.set_default_params(params)
 
     xyz = torch.nn.Parameter(W["means"].clone())
@@ -217,7 +220,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload_name, "gaussians": P, "points_per_step": S,
                    "leaf_size": params.leaf_size, "block_threads": params.block_threads,
-                   "kahan": params.kahan, "parallelism": "single GPU"},
+                   "kahan": params.kahan, "candidate_log": params.candidate_log, "parallelism": "single GPU"},
         "pairs_per_step": pairs_per_step,
         "stage_ms": stage_avg,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

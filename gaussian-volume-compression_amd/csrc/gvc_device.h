@@ -220,6 +220,15 @@ __device__ __forceinline__ float pair_power(float c0, float c1, float c2, float 
 
 __device__ __forceinline__ bool in_window(float power) { return !(power < -14.0f || power > 0.0f); }
 
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  For a
+// grid over spatially sorted data, map block b to an index such that every XCD
+// works through one contiguous 1/8 of the data, so neighbours share its L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
+{
+    const unsigned x = b & 7u, q = nb >> 3, r = nb & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 __device__ __forceinline__ bool boxes_overlap(float alx, float aly, float alz, float ahx, float ahy,
                                               float ahz, float blx, float bly, float blz, float bhx,
                                               float bhy, float bhz)

@@ -28,6 +28,7 @@
 #include <string.h>
 
 #include "exact.h"
+#include "gvc_device.h"
 #include "gvc_internal.h"
 
 namespace gvc {
@@ -289,7 +290,7 @@ __global__ void tet_locate_kernel(const float* __restrict__ pts, const TetRec* _
                                   int* __restrict__ cell_ids, double* __restrict__ out_values,
                                   float* __restrict__ bary_out)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= S) return;
     int s = i;
     float p[3];

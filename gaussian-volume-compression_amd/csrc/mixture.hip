@@ -732,7 +732,7 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
                                                       uint32_t* __restrict__ group_pairs, uint32_t* __restrict__ err,
                                                       uint32_t* __restrict__ queue,
                                                       const unsigned long long* __restrict__ ext_sum,
-                                                      unsigned char* __restrict__ log, uint32_t log_cap,
+                                                      int log_on, unsigned char* __restrict__ log, uint32_t log_cap,
                                                       uint32_t* __restrict__ log_counter,
                                                       uint32_t* __restrict__ group_head)
 {
@@ -758,7 +758,7 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
         uint32_t npairs = 0, nrec = 0;
         bool inrun = false;
         uint32_t head = kLogEnd, prev = kLogEnd;
-        bool logged = true;
+        bool logged = log_on != 0;
 
         auto chunk = [&](uint32_t base, int n) {
             float lox = 1.f, loy = 1.f, loz = 1.f, hix = 0.f, hiy = 0.f, hiz = 0.f;  // empty box for idle lanes
@@ -868,8 +868,10 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
         if (lane == 0) {
             group_records[g] = nrec;
             group_pairs[g] = tp;
-            group_head[g] = logged ? head : kLogIncomplete;
-            if (!logged) atomicOr(err, (uint32_t)GVC_ERRFLAG_CHUNK_OVERFLOW);
+            if (log_on) {
+                group_head[g] = logged ? head : kLogIncomplete;
+                if (!logged) atomicOr(err, (uint32_t)GVC_ERRFLAG_CHUNK_OVERFLOW);
+            }
         }
     }
     if (!ok && lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_STACK_OVERFLOW);
@@ -892,12 +894,19 @@ struct alignas(64) PartialRecord {
 };
 static_assert(sizeof(PartialRecord) == 64, "PartialRecord layout");
 
-struct BwdLds {
+struct BwdLdsBase {
     float4 ptsA[64];      // x, y, z, V
     float2 ptsB[64];      // gV / W, gW   (zero where W <= cutoff)
     float4 slot[64][3];   // one slice's 11 partial sums + accepted flag
     float4 cand[64][3];   // the chunk's candidates: mean|w, conic c0..c3, c4 c5 v .
     uint8_t owner[256];   // slice -> candidate lane
+};
+template <bool REPLAY>
+struct BwdLds : BwdLdsBase {
+};
+template <>
+struct BwdLds<false> : BwdLdsBase {
+    WalkLds walk;
 };
 
 // Inclusive prefix sum over the wavefront.
@@ -966,26 +975,38 @@ __device__ __forceinline__ uint64_t drop_low_bits(uint64_t m, uint32_t k)
     return m & ~((1ull << base) - 1ull);
 }
 
+// REPLAY = true: candidates and box hits come from the forward's candidate log.
+// REPLAY = false: the group is walked again (runs, tree, box tests), as the
+// forward did, so nothing but per-group counts is kept between the passes.
+template <bool REPLAY>
 __global__ __launch_bounds__(256) void backward_kernel(
-    const GRecord* __restrict__ rec, const float4* __restrict__ pts, int NG, const float* __restrict__ out_cells,
+    const GRecord* __restrict__ rec, const BNode* __restrict__ nodes, const float* __restrict__ root_box, int P,
+    int NL, int L, const float4* __restrict__ pts, int NG, const float* __restrict__ out_cells,
     const float* __restrict__ out_weights, const float* __restrict__ gcells, const float* __restrict__ gweights,
     const uint32_t* __restrict__ group_offsets, uint64_t capacity, PartialRecord* __restrict__ out_rec,
     uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ err, uint32_t* __restrict__ queue,
-    const unsigned char* __restrict__ log, const uint32_t* __restrict__ group_head)
+    const unsigned long long* __restrict__ ext_sum, const unsigned char* __restrict__ log,
+    const uint32_t* __restrict__ group_head)
 {
-    __shared__ BwdLds lds_all[4];
+    __shared__ BwdLds<REPLAY> lds_all[4];
     const int wid = uniformi(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    BwdLds& s = lds_all[wid];
+    BwdLds<REPLAY>& s = lds_all[wid];
     const uint64_t lt = lanemask_lt();
+    float reach[3] = { 0.f, 0.f, 0.f };
+    if (!REPLAY) mean_extent(ext_sum, P, reach);
     int x = (int)(blockIdx.x & 7u), tries = 0;
+    bool ok = true;
     while (true) {
         const int g = next_group(queue, NG, x, tries);
         if (g >= NG) break;
-        const uint32_t head = uniformi((int)group_head[g]);
-        if (head == kLogEnd) continue;
-        if (head == kLogIncomplete) {
-            if (lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_CHUNK_OVERFLOW);
-            continue;
+        uint32_t head = kLogEnd;
+        if (REPLAY) {
+            head = uniformi((int)group_head[g]);
+            if (head == kLogEnd) continue;
+            if (head == kLogIncomplete) {
+                if (lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_CHUNK_OVERFLOW);
+                continue;
+            }
         }
         const float4 pme = pts[(size_t)g * 64 + lane];
         const int orig = __builtin_bit_cast(int, pme.w);
@@ -1006,18 +1027,9 @@ __global__ __launch_bounds__(256) void backward_kernel(
         uint32_t base_out = group_offsets[g];
         const uint32_t limit = group_offsets[g + 1];
 
-        // one logged chunk: n candidates, lane j's Gaussian gi, point-major box hits mrow
-        auto chunk = [&](int n, int gi, uint64_t mrow) {
-            const bool active = lane < n;
-            {
-                const float4* q = reinterpret_cast<const float4*>(&rec[active ? gi : 0]);
-                s.cand[lane][0] = q[0];
-                s.cand[lane][1] = q[1];
-                s.cand[lane][2] = q[2];
-            }
-            const uint64_t tm = wave_transpose64(mrow);
-            const uint64_t m = active ? tm : 0ull;
-            lds_fence();
+        // one chunk: n candidates, lane j's Gaussian gi and the points inside its box
+        // (bit mask m, lanes j >= n empty); the candidates' records are in s.cand
+        auto process = [&](int n, int gi, uint64_t m) {
 
             // Balanced pair loop: the candidates' point lists are cut into slices of
             // at most q pairs (q = ceil(H / 128) for H pairs in the chunk, so at most
@@ -1116,33 +1128,78 @@ __global__ __launch_bounds__(256) void backward_kernel(
             }
             base_out += (uint32_t)__popcll(bal);
         };
-        // replay the group's log; the next slot's header and rows are loaded
-        // before the current chunk is processed
-        uint32_t slot = head;
-        const unsigned char* sb = log + (size_t)slot * kSlotBytes;
-        uint32_t nxt = uniformi((int)reinterpret_cast<const uint32_t*>(sb)[0]);
-        int n = uniformi((int)reinterpret_cast<const uint32_t*>(sb)[1]);
-        int gi = reinterpret_cast<const int*>(sb + 64)[lane];
-        uint2 mr = reinterpret_cast<const uint2*>(sb + 320)[lane];
-        while (true) {
-            uint32_t nxt2 = kLogEnd;
-            int n2 = 0, gi2 = -1;
-            uint2 mr2 = make_uint2(0u, 0u);
-            if (nxt != kLogEnd) {
-                const unsigned char* sb2 = log + (size_t)nxt * kSlotBytes;
-                nxt2 = uniformi((int)reinterpret_cast<const uint32_t*>(sb2)[0]);
-                n2 = uniformi((int)reinterpret_cast<const uint32_t*>(sb2)[1]);
-                gi2 = reinterpret_cast<const int*>(sb2 + 64)[lane];
-                mr2 = reinterpret_cast<const uint2*>(sb2 + 320)[lane];
+        if constexpr (REPLAY) {
+            // replay the group's log; the next slot's header and rows are loaded
+            // before the current chunk is processed
+            const unsigned char* sb = log + (size_t)head * kSlotBytes;
+            uint32_t nxt = uniformi((int)reinterpret_cast<const uint32_t*>(sb)[0]);
+            int n = uniformi((int)reinterpret_cast<const uint32_t*>(sb)[1]);
+            int gi = reinterpret_cast<const int*>(sb + 64)[lane];
+            uint2 mr = reinterpret_cast<const uint2*>(sb + 320)[lane];
+            while (true) {
+                uint32_t nxt2 = kLogEnd;
+                int n2 = 0, gi2 = -1;
+                uint2 mr2 = make_uint2(0u, 0u);
+                if (nxt != kLogEnd) {
+                    const unsigned char* sb2 = log + (size_t)nxt * kSlotBytes;
+                    nxt2 = uniformi((int)reinterpret_cast<const uint32_t*>(sb2)[0]);
+                    n2 = uniformi((int)reinterpret_cast<const uint32_t*>(sb2)[1]);
+                    gi2 = reinterpret_cast<const int*>(sb2 + 64)[lane];
+                    mr2 = reinterpret_cast<const uint2*>(sb2 + 320)[lane];
+                }
+                const bool active = lane < n;
+                const float4* q = reinterpret_cast<const float4*>(&rec[active ? gi : 0]);
+                s.cand[lane][0] = q[0];
+                s.cand[lane][1] = q[1];
+                s.cand[lane][2] = q[2];
+                const uint64_t tm = wave_transpose64(((uint64_t)mr.y << 32) | mr.x);
+                lds_fence();
+                process(n, gi, active ? tm : 0ull);
+                if (nxt == kLogEnd) break;
+                nxt = nxt2;
+                n = n2;
+                gi = gi2;
+                mr = mr2;
             }
-            chunk(n, gi, ((uint64_t)mr.y << 32) | mr.x);
-            if (nxt == kLogEnd) break;
-            nxt = nxt2;
-            n = n2;
-            gi = gi2;
-            mr = mr2;
+        } else {
+            // walk the group's runs again and box-test each chunk (lane = candidate)
+            const int nv = __popcll(__ballot(orig >= 0));
+            const uint64_t starts = split_runs(pme.x, pme.y, pme.z, nv, reach);
+            uint64_t runmask = 0;
+            auto chunk = [&](uint32_t base, int n) {
+                const bool active = lane < n;
+                const int gi = active ? s.walk.ring[(base + lane) & (kRing - 1)] : 0;
+                const float4* q = reinterpret_cast<const float4*>(&rec[gi]);
+                const float4 rd = q[3], re = q[4];
+                s.cand[lane][0] = q[0];
+                s.cand[lane][1] = q[1];
+                s.cand[lane][2] = q[2];
+                const float lox = active ? rd.x : 1.f, hix = active ? rd.w : 0.f;
+                uint32_t mlo = 0, mhi = 0;
+#pragma unroll 2
+                for (int p = 0; p < 32; ++p) {
+                    const float4 a = s.ptsA[p];
+                    const float4 b = s.ptsA[32 + p];
+                    const bool ia = lox <= a.x && a.x <= hix && rd.y <= a.y && a.y <= re.x && rd.z <= a.z && a.z <= re.y;
+                    const bool ib = lox <= b.x && b.x <= hix && rd.y <= b.y && b.y <= re.x && rd.z <= b.z && b.z <= re.y;
+                    mlo |= ia ? (1u << p) : 0u;
+                    mhi |= ib ? (1u << p) : 0u;
+                }
+                lds_fence();
+                process(n, active ? gi : -1, (((uint64_t)mhi << 32) | mlo) & runmask);
+            };
+            uint64_t rs = starts;
+            while (rs != 0ull) {
+                const int a = __builtin_ctzll(rs);
+                rs &= rs - 1;
+                const int b = rs ? __builtin_ctzll(rs) : nv;
+                runmask = (b >= 64 ? ~0ull : ((1ull << b) - 1ull)) & ~((1ull << a) - 1ull);
+                const GroupBox gb = masked_box(pme.x, pme.y, pme.z, lane >= a && lane < b);
+                ok = enumerate_candidates(rec, nodes, root_box, P, NL, L, gb, s.walk, chunk) && ok;
+            }
         }
     }
+    if (!REPLAY && !ok && lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_STACK_OVERFLOW);
 }
 
 __global__ void segment_kernel(const uint32_t* __restrict__ keys, uint64_t n_max, const uint32_t* __restrict__ n_dev,
@@ -1167,7 +1224,7 @@ __global__ void reduce_chain_kernel(int P, const GRecord* __restrict__ rec, cons
                                     float* __restrict__ dscales, float* __restrict__ drots,
                                     float* __restrict__ dvalues, float* __restrict__ dweights)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= P) return;
     double acc[11] = { 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0 };
     const uint32_t b = begin[i], e = end[i];
@@ -1345,7 +1402,7 @@ struct ForwardLayout {
     unsigned char* log;
     uint32_t log_cap;
     int NL, NG;
-    ForwardLayout(Arena& a, int P, int S, int L, unsigned chunk_slots)
+    ForwardLayout(Arena& a, int P, int S, int L, unsigned flags, unsigned chunk_slots)
     {
         NL = P > 0 ? (P + L - 1) / L : 0;
         NG = (S + 63) / 64;
@@ -1369,7 +1426,9 @@ struct ForwardLayout {
         ext_sum = a.take<unsigned long long>(4);
         ext_part = a.take<unsigned long long>(3 * (((size_t)P + 255) / 256) + 3);
         group_head = a.take<uint32_t>((size_t)NG + 1);
-        const uint64_t cap = (uint64_t)(chunk_slots ? chunk_slots : 16u) * (uint64_t)(NG > 0 ? NG : 1);
+        const uint64_t cap = (flags & GVC_FLAG_CANDIDATE_LOG)
+                                 ? (uint64_t)(chunk_slots ? chunk_slots : 16u) * (uint64_t)(NG > 0 ? NG : 1)
+                                 : 1ull;
         log_cap = (uint32_t)(cap < 0xfff00000ull ? cap : 0xfff00000ull);
         log = reinterpret_cast<unsigned char*>(a.take<uint4>((size_t)log_cap * (kSlotBytes / 16)));
     }
@@ -1463,7 +1522,7 @@ size_t forward_state_bytes(int P, int S, const gvc_params* prm)
 {
     if (check_params(prm) != GVC_OK) return 0;
     Arena a(nullptr, 0);
-    ForwardLayout l(a, P, S, prm->leaf_size, prm->chunk_slots);
+    ForwardLayout l(a, P, S, prm->leaf_size, prm->flags, prm->chunk_slots);
     return a.off + 256;
 }
 
@@ -1476,7 +1535,7 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
     if (P < 0 || S < 0) return set_error(GVC_ERR_INVALID_ARGUMENT, "negative size");
     const int L = prm->leaf_size;
     Arena a(state, state_bytes);
-    ForwardLayout l(a, P, S, L, prm->chunk_slots);
+    ForwardLayout l(a, P, S, L, prm->flags, prm->chunk_slots);
     if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "forward state buffer too small");
     Arena sa(const_cast<void*>(sample_set), sample_set_bytes(S));
     SampleSetLayout sl(sa, S);
@@ -1543,16 +1602,19 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
         const int T = 256;
         GVC_TRY(fill_u32(l.queues, 0u, 8, st));
         GVC_TRY(fill_u32(l.queues + 16, 0u, 4, st));
+        const int log_on = (prm->flags & GVC_FLAG_CANDIDATE_LOG) ? 1 : 0;
         if (prm->flags & GVC_FLAG_KAHAN) {
             const unsigned nb = walk_blocks(reinterpret_cast<const void*>(&forward_kernel<true>), T, NG);
             forward_kernel<true><<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells,
                                                    out_weights, l.group_records, l.group_pairs, l.err, l.queues,
-                                                   l.ext_sum, l.log, l.log_cap, l.queues + 16, l.group_head);
+                                                   l.ext_sum, log_on, l.log, l.log_cap, l.queues + 16,
+                                                   l.group_head);
         } else {
             const unsigned nb = walk_blocks(reinterpret_cast<const void*>(&forward_kernel<false>), T, NG);
             forward_kernel<false><<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells,
                                                     out_weights, l.group_records, l.group_pairs, l.err, l.queues,
-                                                    l.ext_sum, l.log, l.log_cap, l.queues + 16, l.group_head);
+                                                    l.ext_sum, log_on, l.log, l.log_cap, l.queues + 16,
+                                                    l.group_head);
         }
         GVC_LAUNCH_CHECK("forward");
     }
@@ -1565,7 +1627,7 @@ int forward_stats(const void* state, int P, int S, const gvc_params* prm, uint64
 {
     GVC_TRY(check_params(prm));
     Arena a(const_cast<void*>(state), (size_t)-1);
-    ForwardLayout l(a, P, S, prm->leaf_size, prm->chunk_slots);
+    ForwardLayout l(a, P, S, prm->leaf_size, prm->flags, prm->chunk_slots);
     stats_kernel<<<1, 256, 0, st>>>(l.group_pairs, l.NG, l.group_offsets, l.stats);
     GVC_LAUNCH_CHECK("stats");
     unsigned long long h[2] = { 0, 0 };
@@ -1600,7 +1662,7 @@ int mixture_backward(int P, const float* means, const float* scales, const float
     if (capacity > 0xffffffffull) return set_error(GVC_ERR_INVALID_ARGUMENT, "record capacity above 2^32");
     const int L = prm->leaf_size;
     Arena a(state, state_bytes);
-    ForwardLayout l(a, P, S, L, prm->chunk_slots);
+    ForwardLayout l(a, P, S, L, prm->flags, prm->chunk_slots);
     if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "forward state buffer too small");
     Arena ba(ws, ws_bytes);
     BackwardLayout bl(ba, P, capacity);
@@ -1612,10 +1674,19 @@ int mixture_backward(int P, const float* means, const float* scales, const float
     if ((stages & GVC_STAGE_WALK) && NG > 0 && capacity > 0) {
         const int T = 256;
         GVC_TRY(fill_u32(l.queues + 8, 0u, 8, st));
-        const unsigned nb = walk_blocks(reinterpret_cast<const void*>(&backward_kernel), T, NG);
-        backward_kernel<<<nb, T, 0, st>>>(l.rec, sl.pts, NG, out_cells, out_weights, gcells, gweights,
-                                          l.group_offsets, capacity, bl.parts, bl.keys, bl.vals, l.err, l.queues + 8,
-                                          l.log, l.group_head);
+        if (prm->flags & GVC_FLAG_CANDIDATE_LOG) {
+            const unsigned nb = walk_blocks(reinterpret_cast<const void*>(&backward_kernel<true>), T, NG);
+            backward_kernel<true><<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, NG, out_cells,
+                                                    out_weights, gcells, gweights, l.group_offsets, capacity, bl.parts,
+                                                    bl.keys, bl.vals, l.err, l.queues + 8, l.ext_sum, l.log,
+                                                    l.group_head);
+        } else {
+            const unsigned nb = walk_blocks(reinterpret_cast<const void*>(&backward_kernel<false>), T, NG);
+            backward_kernel<false><<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, NG, out_cells,
+                                                     out_weights, gcells, gweights, l.group_offsets, capacity,
+                                                     bl.parts, bl.keys, bl.vals, l.err, l.queues + 8, l.ext_sum,
+                                                     l.log, l.group_head);
+        }
         GVC_LAUNCH_CHECK("backward");
     }
     if (!(stages & GVC_STAGE_REDUCE)) return GVC_OK;

@@ -11,6 +11,7 @@ from ._build import LIB_PATH
 
 GVC_FLAG_KAHAN = 1
 GVC_FLAG_FP64_COND = 2
+GVC_FLAG_CANDIDATE_LOG = 4
 GVC_ERRFLAG_STACK_OVERFLOW = 1
 GVC_ERRFLAG_RECORD_OVERFLOW = 2
 GVC_ERRFLAG_CHUNK_OVERFLOW = 4

@@ -10,7 +10,7 @@ from dataclasses import dataclass
 import torch
 
 from . import _lib
-from ._lib import GVC_FLAG_FP64_COND, GVC_FLAG_KAHAN, call, gvc_params, lib
+from ._lib import GVC_FLAG_CANDIDATE_LOG, GVC_FLAG_FP64_COND, GVC_FLAG_KAHAN, call, gvc_params, lib
 
 DEFAULT_LEAF_SIZE = 4
 DEFAULT_BLOCK_THREADS = 256
@@ -20,16 +20,20 @@ DEFAULT_FLAGS = GVC_FLAG_KAHAN
 @dataclass(frozen=True)
 class MixtureParams:
     """Launch parameters (the autotuner's search space) and precision flags.
-    chunk_slots: candidate-log slots per 64-point group; 0 = the module's
-    high-water mark, which grows whenever a forward ran out of log."""
+    candidate_log: the forward keeps the candidate chunks it visited and the
+    backward replays them; False re-walks the tree in the backward (no
+    candidate storage).  chunk_slots: candidate-log slots per 64-point group;
+    0 = the module's high-water mark, which grows whenever a forward ran out."""
     leaf_size: int = DEFAULT_LEAF_SIZE
     block_threads: int = DEFAULT_BLOCK_THREADS
     kahan: bool = True
     fp64_cond: bool = False
+    candidate_log: bool = True
     chunk_slots: int = 0
 
     def c(self):
-        flags = (GVC_FLAG_KAHAN if self.kahan else 0) | (GVC_FLAG_FP64_COND if self.fp64_cond else 0)
+        flags = ((GVC_FLAG_KAHAN if self.kahan else 0) | (GVC_FLAG_FP64_COND if self.fp64_cond else 0) |
+                 (GVC_FLAG_CANDIDATE_LOG if self.candidate_log else 0))
         return gvc_params(int(self.leaf_size), int(self.block_threads), flags, 0,
                           int(self.chunk_slots or _chunk_slots))
 

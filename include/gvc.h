@@ -39,6 +39,8 @@ enum gvc_status {
 /* gvc_params.flags */
 #define GVC_FLAG_KAHAN 1u      /* compensated per-point sums in the forward       */
 #define GVC_FLAG_FP64_COND 2u  /* FP64 covariance inverse where cond(Sigma) > 1e6 */
+#define GVC_FLAG_CANDIDATE_LOG 4u  /* forward keeps a candidate log that the backward
+                                     * replays; clear: the backward walks the tree again */
 
 /* gvc_params.stages: 0 runs every stage of a call; otherwise only the selected
  * ones, in order, so a caller can bracket each stage with events. */
@@ -99,11 +101,11 @@ int gvc_sample_set_build(int S, const float *samples, void *sample_set, size_t s
  * K5/K7, K6: CR/forward.cu:10-274).  out_cells/out_weights: [S]; sentinel (-1, 0)
  * where the total weight is <= 1e-2; P == 0 leaves zeros (as the reference).
  * conics: optional [P,6] output (the reference's third forward result).
- * `state` (gvc_forward_state_bytes) keeps the Gaussian records, the per-group
- * record counts and the candidate log for the backward: every 64-candidate
- * chunk a group's walk visited, with its 64x64 box-hit bit mask, in
- * params->chunk_slots x groups slots of 832 bytes.  It must not be reused until
- * the backward ran. */
+ * `state` (gvc_forward_state_bytes) keeps the Gaussian records and BVH and the
+ * per-group record counts for the backward, and with GVC_FLAG_CANDIDATE_LOG the
+ * candidate log: every 64-candidate chunk a group's walk visited, with its
+ * 64x64 box-hit bit mask, in params->chunk_slots x groups slots of 832 bytes.
+ * It must not be reused until the backward ran. */
 size_t gvc_forward_state_bytes(int P, int S, const gvc_params *params);
 int gvc_mixture_forward(int P, const float *means3D, const float *scales, const float *rotations,
                         const float *values, const float *weights, float scale_modifier,

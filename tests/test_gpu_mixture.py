@@ -189,13 +189,33 @@ def _grad_check(g, scales, samples, params, tol=1e-4, seed=0):
     return grads
 
 
-@pytest.mark.parametrize("leaf", [1, 4, 16])
-def test_backward_config0_parity(leaf):
+@pytest.mark.parametrize("leaf,log", [(1, True), (4, True), (16, True), (1, False), (4, False)])
+def test_backward_config0_parity(leaf, log):
     g = workloads.config0(0)
     scales = np.exp(g["log_scales"]).astype(f32)
     # subsample points so the FP64 brute-force oracle stays fast
     samples = g["samples"][::4].copy()
-    _grad_check(g, scales, samples, mx.MixtureParams(leaf_size=leaf))
+    _grad_check(g, scales, samples, mx.MixtureParams(leaf_size=leaf, candidate_log=log))
+
+
+def test_backward_replay_equals_rewalk():
+    """The candidate-log replay and the re-walk give bit-identical gradients."""
+    g = workloads.config0(4)
+    scales = np.exp(g["log_scales"]).astype(f32)
+    samples = g["samples"]
+    rng = np.random.default_rng(6)
+    gV = T(rng.normal(size=samples.shape[0]))
+    gW = T(rng.normal(size=samples.shape[0]))
+    outs = []
+    for log in (True, False):
+        ss, cells, cw, conics, st = run_gpu(g["means"], scales, g["rotations"], g["values"], g["weights"], samples,
+                                            mx.MixtureParams(candidate_log=log))
+        grads = mx.mixture_backward(st, T(g["means"]), T(scales), T(g["rotations"]), T(g["values"]),
+                                    T(g["weights"]), 1.0, ss, cells, cw, gV, gW)
+        assert st.stats()[2] == 0
+        outs.append([cells.cpu().numpy()] + [t.cpu().numpy() for t in grads])
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
 def test_backward_deterministic():
