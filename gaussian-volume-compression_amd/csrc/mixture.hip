@@ -1555,7 +1555,7 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
         return GVC_OK;
     }
     const int fp64 = (prm->flags & GVC_FLAG_FP64_COND) ? 1 : 0;
-    const unsigned stages = prm->stages ? (unsigned)prm->stages : 3u;
+    const unsigned stages = prm->stages ? (unsigned)prm->stages : 7u;
     if (stages & GVC_STAGE_BUILD) {
     // 1. Morton order of the Gaussians
     gaussian_codes_kernel<<<blocks_for(P, 256), 256, 0, st>>>(means, P, l.codes, l.idx);
@@ -1596,9 +1596,8 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
         GVC_LAUNCH_CHECK("root_box");
     }
     }
-    if (!(stages & GVC_STAGE_WALK)) return GVC_OK;
     // 4. walk
-    if (NG > 0) {
+    if ((stages & GVC_STAGE_WALK) && NG > 0) {
         const int T = 256;
         GVC_TRY(fill_u32(l.queues, 0u, 8, st));
         GVC_TRY(fill_u32(l.queues + 16, 0u, 4, st));
@@ -1618,7 +1617,9 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
         }
         GVC_LAUNCH_CHECK("forward");
     }
-    GVC_TRY(exclusive_scan_u32(l.group_records, l.group_offsets, (size_t)NG, true, l.scan_scratch, st));
+    // 5. record offsets for the backward
+    if (stages & GVC_STAGE_COUNTS)
+        GVC_TRY(exclusive_scan_u32(l.group_records, l.group_offsets, (size_t)NG, true, l.scan_scratch, st));
     return GVC_OK;
 }
 

@@ -17,6 +17,7 @@ GVC_ERRFLAG_RECORD_OVERFLOW = 2
 GVC_ERRFLAG_CHUNK_OVERFLOW = 4
 GVC_STAGE_BUILD = 1
 GVC_STAGE_WALK = 2
+GVC_STAGE_COUNTS = 4
 GVC_STAGE_REDUCE = 1
 
 

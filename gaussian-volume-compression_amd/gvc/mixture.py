@@ -198,7 +198,8 @@ def _forward(P, S, dev, cp, params, means3D, scales, rotations, values, weights,
                  (P, _ptr(means3D), _ptr(scales), _ptr(rotations), _ptr(values), _ptr(weights),
                   float(scale_modifier), _ptr(sample_set.buffer), S), cp,
                  (_ptr(cells), _ptr(cw), _ptr(conics), _ptr(state), nbytes, _stream(dev)),
-                 ((_lib.GVC_STAGE_BUILD, "fwd_build"), (_lib.GVC_STAGE_WALK, "fwd_walk")))
+                 ((_lib.GVC_STAGE_BUILD, "fwd_build"), (_lib.GVC_STAGE_WALK, "fwd_walk"),
+                  (_lib.GVC_STAGE_COUNTS, "fwd_counts")))
     fs = ForwardState(state, cp, params, P, S)
     if _audit is not None:
         _audit.append(fs)

@@ -44,8 +44,9 @@ enum gvc_status {
 
 /* gvc_params.stages: 0 runs every stage of a call; otherwise only the selected
  * ones, in order, so a caller can bracket each stage with events. */
-#define GVC_STAGE_BUILD 1u   /* forward: Morton sort, records, tree, refit          */
-#define GVC_STAGE_WALK 2u    /* forward: walk + count scan; backward: partial records */
+#define GVC_STAGE_BUILD 1u   /* forward: Morton sort, records, tree, node boxes     */
+#define GVC_STAGE_WALK 2u    /* forward: walk; backward: partial records            */
+#define GVC_STAGE_COUNTS 4u  /* forward: scan of the per-group record counts        */
 #define GVC_STAGE_REDUCE 1u  /* backward: sort, segments, FP64 sums, chain          */
 
 /* Launch/tuning parameters of the mixture kernels (chosen by the autotuner). */
