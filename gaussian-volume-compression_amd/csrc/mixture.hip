@@ -667,11 +667,11 @@ __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__
 
 // ---------------------------------------------------------------------------
 // forward: lane = point.  Per chunk of <= 64 candidates: lane j holds candidate
-// j's box; for each of the group's 64 points (wave-uniform scalar loads) the six
-// box compares form a ballot = the candidates whose box holds that point, which
-// v_writelane deposits in that point's lane.  Every lane then walks its own set
-// bits in candidate order (next candidate prefetched from LDS) and accumulates
-// w e^power and w e^power v with Kahan compensation.
+// j's box and finds the run's points inside it by rank intervals (GroupAxes);
+// a 64x64 bit transpose turns that into each point's candidate mask.  Every
+// lane then walks its own set bits in candidate order (next candidate
+// prefetched from LDS) and accumulates w e^power and w e^power v with Kahan
+// compensation.
 // ---------------------------------------------------------------------------
 // Candidate log: the forward appends every chunk it evaluates to a slot of 832
 // bytes {next slot, n, pad | Gaussian index per lane | point-major box-hit mask
@@ -699,22 +699,130 @@ __device__ __forceinline__ uint32_t log_take(LogCursor& c, uint32_t* __restrict_
     return c.next++;
 }
 
+// 64x64 bit-matrix transpose across the wavefront: lane r holds row r (bit c =
+// element (r, c)); returns column `lane`.  Six block-swap stages: at stage j the
+// lanes with bit j clear trade their bits with bit j set against the partner's
+// (lane ^ j) bits with bit j clear.
+__device__ __forceinline__ uint64_t wave_transpose64(uint64_t a)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t lo = (uint32_t)a, hi = (uint32_t)(a >> 32);
+    {
+        const uint32_t plo = (uint32_t)__shfl_xor((int)lo, 32, 64), phi = (uint32_t)__shfl_xor((int)hi, 32, 64);
+        if (lane & 32) lo = phi;
+        else hi = plo;
+    }
+#pragma unroll
+    for (int j = 16; j >= 1; j >>= 1) {
+        const uint32_t m = j == 16 ? 0x0000ffffu : j == 8 ? 0x00ff00ffu : j == 4 ? 0x0f0f0f0fu : j == 2 ? 0x33333333u
+                                                                                                         : 0x55555555u;
+        const uint32_t plo = (uint32_t)__shfl_xor((int)lo, j, 64), phi = (uint32_t)__shfl_xor((int)hi, j, 64);
+        if (lane & j) {
+            lo = (lo & ~m) | ((plo & ~m) >> j);
+            hi = (hi & ~m) | ((phi & ~m) >> j);
+        } else {
+            lo = (lo & m) | ((plo & m) << j);
+            hi = (hi & m) | ((phi & m) << j);
+        }
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Box-vs-group masks by rank intervals.  Per group, the 64 point coordinates are
+// sorted along each axis (bitonic, key (value, lane); NaN and padding as +inf)
+// and pref[a][k] = the lanes of the k smallest values.  A box [lo, hi] then holds,
+// along axis a, exactly the points of rank r0 <= r < r1 with r0 = #(v < lo) and
+// r1 = #(v <= hi) -- the same comparisons as lo <= v <= hi -- so its point mask
+// is AND_a (pref[a][r1] & ~pref[a][r0]): two 7-step binary searches per axis
+// instead of six compares per point.  Points with a NaN coordinate never pass a
+// comparison and are removed by `finite`; a box with a NaN bound holds nothing.
+struct GroupAxes {
+    float vals[3][64];
+    uint64_t pref[3][65];
+};
+
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int d)
+{
+    const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, d, 64);
+    const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), d, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Returns the mask of lanes whose point has no NaN coordinate among `valid` lanes.
+__device__ __forceinline__ uint64_t build_axes(float px, float py, float pz, bool valid, GroupAxes& ax)
+{
+    const int lane = threadIdx.x & 63;
+    const bool finite = valid && px == px && py == py && pz == pz;
+    const float inf = __builtin_inff();
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float c = a == 0 ? px : (a == 1 ? py : pz);
+        float v = finite ? c : inf;
+        int id = lane;
+#pragma unroll
+        for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const float ov = __shfl_xor(v, j, 64);
+                const int oid = __shfl_xor(id, j, 64);
+                const bool asc = (lane & k) == 0;
+                const bool low = (lane & j) == 0;
+                const bool other_less = (ov < v) || (ov == v && oid < id);
+                if (low == asc ? other_less : !other_less) {
+                    v = ov;
+                    id = oid;
+                }
+            }
+        }
+        ax.vals[a][lane] = v;
+        uint64_t b = 1ull << id;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint64_t o = shfl_up_u64(b, d);
+            if (lane >= d) b |= o;
+        }
+        ax.pref[a][lane + 1] = b;
+        if (lane == 0) ax.pref[a][0] = 0ull;
+    }
+    return __ballot(finite);
+}
+
+// #(vals < x) (strict) or #(vals <= x) over 64 ascending values in LDS.
+template <bool STRICT>
+__device__ __forceinline__ int rank64(const float* __restrict__ vals, float x)
+{
+    int c = 0;
+#pragma unroll
+    for (int st = 32; st >= 1; st >>= 1) {
+        const float v = vals[c + st - 1];
+        if (STRICT ? (v < x) : (v <= x)) c += st;
+    }
+    const float v = vals[c < 63 ? c : 63];
+    if (c < 64 && (STRICT ? (v < x) : (v <= x))) ++c;
+    return c;
+}
+
+// Points of the group inside the box (lox..hiz), before run and finite masks.
+__device__ __forceinline__ uint64_t box_points(const GroupAxes& ax, float lox, float loy, float loz, float hix,
+                                               float hiy, float hiz)
+{
+    const float lo[3] = { lox, loy, loz }, hi[3] = { hix, hiy, hiz };
+    uint64_t m = ~0ull;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int r0 = rank64<true>(ax.vals[a], lo[a]);
+        const int r1 = rank64<false>(ax.vals[a], hi[a]);
+        m &= ax.pref[a][r1] & ~ax.pref[a][r0];
+    }
+    const bool ok = lox == lox && loy == loy && loz == loz && hix == hix && hiy == hiy && hiz == hiz;
+    return ok ? m : 0ull;
+}
+
 struct FwdLds {
     WalkLds walk;
     float4 chunk[64][3];   // mean|w, conic c0..c3, c4 c5 v id
+    GroupAxes axes;
 };
-
-// lo[lane_c] = vlo, hi[lane_c] = vhi for wave-uniform values and a constant lane.
-// The values usually come straight from a VALU compare, and gfx950 requires wait
-// states between a VALU write of an SGPR and a VALU read of it; the compiler
-// cannot see into the asm, so the s_nop provides them (5 states).
-template <class I>
-__device__ __forceinline__ void writelane2(int& lo, int& hi, uint32_t vlo, uint32_t vhi, I lane_c)
-{
-    asm volatile("s_nop 4\n\tv_writelane_b32 %0, %2, %4\n\tv_writelane_b32 %1, %3, %4"
-                 : "+v"(lo), "+v"(hi)
-                 : "s"(vlo), "s"(vhi), "n"((int)lane_c));
-}
 
 __device__ __forceinline__ uint32_t wave_or32(uint32_t v)
 {
@@ -753,10 +861,13 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
         const int orig = __builtin_bit_cast(int, pme.w);
         const int nv = __popcll(__ballot(orig >= 0));
         const uint64_t starts = split_runs(px, py, pz, nv, reach);
+        lds_fence();
+        const uint64_t finite = build_axes(px, py, pz, orig >= 0, s.axes);
+        lds_fence();
 
         float W = 0.f, cW = 0.f, U = 0.f, cU = 0.f;
         uint32_t npairs = 0, nrec = 0;
-        bool inrun = false;
+        uint64_t runmask = 0;
         uint32_t head = kLogEnd, prev = kLogEnd;
         bool logged = log_on != 0;
 
@@ -772,16 +883,11 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
                 s.chunk[lane][2] = c;
                 lox = d.x; loy = d.y; loz = d.z; hix = d.w; hiy = e.x; hiz = e.y;
             }
-            int mlo = 0, mhi = 0;
-#pragma unroll
-            for (int p = 0; p < 64; ++p) {
-                const float4 q = gp[p];
-                const bool inb = lox <= q.x && q.x <= hix && loy <= q.y && q.y <= hiy && loz <= q.z && q.z <= hiz;
-                const uint64_t bal = __ballot(inb);
-                writelane2(mlo, mhi, (uint32_t)bal, (uint32_t)(bal >> 32), p);
-            }
+            // lane j: the run's points inside candidate j's box; transposed, lane p:
+            // the candidates whose box holds point p
+            const uint64_t cm = lane < n ? box_points(s.axes, lox, loy, loz, hix, hiy, hiz) & runmask & finite : 0ull;
+            uint64_t m = wave_transpose64(cm);
             lds_fence();
-            uint64_t m = inrun ? (((uint64_t)(uint32_t)mhi << 32) | (uint32_t)mlo) : 0ull;
             if (logged) {
                 const uint32_t slot = log_take(cur, log_counter);
                 if (slot < log_cap) {
@@ -848,8 +954,8 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
             const int a = __builtin_ctzll(rs);
             rs &= rs - 1;
             const int b = rs ? __builtin_ctzll(rs) : nv;
-            inrun = lane >= a && lane < b;
-            const GroupBox gb = masked_box(px, py, pz, inrun);
+            runmask = (b >= 64 ? ~0ull : ((1ull << b) - 1ull)) & ~((1ull << a) - 1ull);
+            const GroupBox gb = masked_box(px, py, pz, lane >= a && lane < b);
             ok = enumerate_candidates(rec, nodes, root_box, P, NL, L, gb, s.walk, chunk) && ok;
         }
         if (orig >= 0) {
@@ -907,6 +1013,7 @@ struct BwdLds : BwdLdsBase {
 template <>
 struct BwdLds<false> : BwdLdsBase {
     WalkLds walk;
+    GroupAxes axes;
 };
 
 // Inclusive prefix sum over the wavefront.
@@ -919,35 +1026,6 @@ __device__ __forceinline__ uint32_t wave_inscan_u32(uint32_t v)
         if (lane >= d) v += o;
     }
     return v;
-}
-
-// 64x64 bit-matrix transpose across the wavefront: lane r holds row r (bit c =
-// element (r, c)); returns column `lane`.  Six block-swap stages: at stage j the
-// lanes with bit j clear trade their bits with bit j set against the partner's
-// (lane ^ j) bits with bit j clear.
-__device__ __forceinline__ uint64_t wave_transpose64(uint64_t a)
-{
-    const int lane = threadIdx.x & 63;
-    uint32_t lo = (uint32_t)a, hi = (uint32_t)(a >> 32);
-    {
-        const uint32_t plo = (uint32_t)__shfl_xor((int)lo, 32, 64), phi = (uint32_t)__shfl_xor((int)hi, 32, 64);
-        if (lane & 32) lo = phi;
-        else hi = plo;
-    }
-#pragma unroll
-    for (int j = 16; j >= 1; j >>= 1) {
-        const uint32_t m = j == 16 ? 0x0000ffffu : j == 8 ? 0x00ff00ffu : j == 4 ? 0x0f0f0f0fu : j == 2 ? 0x33333333u
-                                                                                                         : 0x55555555u;
-        const uint32_t plo = (uint32_t)__shfl_xor((int)lo, j, 64), phi = (uint32_t)__shfl_xor((int)hi, j, 64);
-        if (lane & j) {
-            lo = (lo & ~m) | ((plo & ~m) >> j);
-            hi = (hi & ~m) | ((phi & ~m) >> j);
-        } else {
-            lo = (lo & m) | ((plo & m) << j);
-            hi = (hi & m) | ((phi & m) << j);
-        }
-    }
-    return ((uint64_t)hi << 32) | lo;
 }
 
 // m with its k lowest set bits cleared (k < popcount(m)).
@@ -1165,6 +1243,8 @@ __global__ __launch_bounds__(256) void backward_kernel(
             // walk the group's runs again and box-test each chunk (lane = candidate)
             const int nv = __popcll(__ballot(orig >= 0));
             const uint64_t starts = split_runs(pme.x, pme.y, pme.z, nv, reach);
+            const uint64_t finite = build_axes(pme.x, pme.y, pme.z, orig >= 0, s.axes);
+            lds_fence();
             uint64_t runmask = 0;
             auto chunk = [&](uint32_t base, int n) {
                 const bool active = lane < n;
@@ -1174,19 +1254,10 @@ __global__ __launch_bounds__(256) void backward_kernel(
                 s.cand[lane][0] = q[0];
                 s.cand[lane][1] = q[1];
                 s.cand[lane][2] = q[2];
-                const float lox = active ? rd.x : 1.f, hix = active ? rd.w : 0.f;
-                uint32_t mlo = 0, mhi = 0;
-#pragma unroll 2
-                for (int p = 0; p < 32; ++p) {
-                    const float4 a = s.ptsA[p];
-                    const float4 b = s.ptsA[32 + p];
-                    const bool ia = lox <= a.x && a.x <= hix && rd.y <= a.y && a.y <= re.x && rd.z <= a.z && a.z <= re.y;
-                    const bool ib = lox <= b.x && b.x <= hix && rd.y <= b.y && b.y <= re.x && rd.z <= b.z && b.z <= re.y;
-                    mlo |= ia ? (1u << p) : 0u;
-                    mhi |= ib ? (1u << p) : 0u;
-                }
+                const uint64_t cm = active ? box_points(s.axes, rd.x, rd.y, rd.z, rd.w, re.x, re.y) & runmask & finite
+                                           : 0ull;
                 lds_fence();
-                process(n, active ? gi : -1, (((uint64_t)mhi << 32) | mlo) & runmask);
+                process(n, active ? gi : -1, cm);
             };
             uint64_t rs = starts;
             while (rs != 0ull) {
