@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--graph", action="store_true", help="capture the training step in a hipGraph")
+    ap.add_argument("--autotune", action="store_true",
+                    help="choose leaf size x run points x block shape on the workload before timing")
     ap.add_argument("--recompute", action="store_true",
                     help="backward walks the tree again instead of replaying the forward's candidate log")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -112,6 +114,23 @@ def main():
                                   block_threads=args.block or params.block_threads,
                                   candidate_log=not args.recompute)
         mx.set_default_params(params)
+
+    tune = None
+    if args.autotune:
+        from gvc.autotune import autotune
+        with torch.no_grad():
+            s0 = torch.rand((S, 3), device=dev, generator=torch.Generator(device=dev).manual_seed(args.seed + 7))
+            ss0 = mx.SampleSet(s0)
+            wts = W["weights"].reshape(-1, 1).contiguous()
+            vls = W["values"].reshape(-1, 1).contiguous()
+            best, table = autotune(W["means"], torch.exp(W["log_scales"]), W["rotations"], vls, wts, ss0,
+                                   base=params, use_cache=False)
+        params = best
+        mx.set_default_params(params)
+        tune = {"chosen": {"leaf_size": best.leaf_size, "run_points": best.run_points,
+                           "block_threads": best.block_threads},
+                "ms_fwd_bwd": {k: (round(v, 3) if v is not None else None) for k, v in table.items()}}
+        log(f"autotune: {tune['chosen']}")
 
     xyz = torch.nn.Parameter(W["means"].clone())
     log_s = torch.nn.Parameter(W["log_scales"].clone())
@@ -220,8 +239,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload_name, "gaussians": P, "points_per_step": S,
                    "leaf_size": params.leaf_size, "block_threads": params.block_threads,
-                   "kahan": params.kahan, "candidate_log": params.candidate_log, "parallelism": "single GPU"},
+                   "run_points": params.run_points, "kahan": params.kahan, "candidate_log": params.candidate_log,
+                   "parallelism": "single GPU"},
         "pairs_per_step": pairs_per_step,
+        "autotune": tune,
         "stage_ms": stage_avg,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -466,15 +466,17 @@ __device__ __forceinline__ float box_cost(const GroupBox& b, const float r[3])
 // keys first differ (the keys are sorted, so the lanes with that bit set form a
 // suffix) when the two halves' cost is below 3/4 of the whole's.  Returns the
 // run starts as a lane bit mask.  Any partition gives the same sums; this one
-// depends only on the group's points and the mean Gaussian extent.
-__device__ __forceinline__ uint64_t split_runs(float px, float py, float pz, int nv, const float r[3])
+// depends only on the group's points, the mean Gaussian extent and run_points
+// (runs never exceed run_points lanes: 16, 32 or 64, a tuning parameter).
+__device__ __forceinline__ uint64_t split_runs(float px, float py, float pz, int nv, const float r[3], int run_points)
 {
     const int lane = threadIdx.x & 63;
     if (nv <= 0) return 0ull;
     uint64_t starts = 1ull;
+    for (int k = run_points; k < nv && k < 64; k += run_points) starts |= 1ull << k;
     if (nv < 16) return starts;
     const uint32_t code = lane < nv ? sample_code(px, py, pz) : 0xffffffffu;
-    uint64_t pending = 1ull;
+    uint64_t pending = starts;
     int tests = 0;
     while (pending != 0ull && tests < 7) {
         const int a = __builtin_ctzll(pending);
@@ -832,7 +834,7 @@ __device__ __forceinline__ uint32_t wave_or32(uint32_t v)
 }
 
 template <bool KAHAN>
-__global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict__ rec, const BNode* __restrict__ nodes,
+__global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict__ rec, const BNode* __restrict__ nodes,
                                                       const float* __restrict__ root_box, int P, int NL, int L,
                                                       const float4* __restrict__ pts, int S, int NG,
                                                       float* __restrict__ out_cells, float* __restrict__ out_weights,
@@ -842,11 +844,11 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
                                                       const unsigned long long* __restrict__ ext_sum,
                                                       int log_on, unsigned char* __restrict__ log, uint32_t log_cap,
                                                       uint32_t* __restrict__ log_counter,
-                                                      uint32_t* __restrict__ group_head)
+                                                      uint32_t* __restrict__ group_head, int run_points)
 {
-    __shared__ FwdLds lds_all[4];
+    extern __shared__ float4 lds_dyn[];
     const int wid = uniformi(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    FwdLds& s = lds_all[wid];
+    FwdLds& s = reinterpret_cast<FwdLds*>(lds_dyn)[wid];
     float reach[3];
     mean_extent(ext_sum, P, reach);
     int x = (int)(blockIdx.x & 7u), tries = 0;
@@ -860,7 +862,7 @@ __global__ __launch_bounds__(256) void forward_kernel(const GRecord* __restrict_
         const float px = pme.x, py = pme.y, pz = pme.z;
         const int orig = __builtin_bit_cast(int, pme.w);
         const int nv = __popcll(__ballot(orig >= 0));
-        const uint64_t starts = split_runs(px, py, pz, nv, reach);
+        const uint64_t starts = split_runs(px, py, pz, nv, reach, run_points);
         lds_fence();
         const uint64_t finite = build_axes(px, py, pz, orig >= 0, s.axes);
         lds_fence();
@@ -1057,18 +1059,18 @@ __device__ __forceinline__ uint64_t drop_low_bits(uint64_t m, uint32_t k)
 // REPLAY = false: the group is walked again (runs, tree, box tests), as the
 // forward did, so nothing but per-group counts is kept between the passes.
 template <bool REPLAY>
-__global__ __launch_bounds__(256) void backward_kernel(
+__global__ __launch_bounds__(1024) void backward_kernel(
     const GRecord* __restrict__ rec, const BNode* __restrict__ nodes, const float* __restrict__ root_box, int P,
     int NL, int L, const float4* __restrict__ pts, int NG, const float* __restrict__ out_cells,
     const float* __restrict__ out_weights, const float* __restrict__ gcells, const float* __restrict__ gweights,
     const uint32_t* __restrict__ group_offsets, uint64_t capacity, PartialRecord* __restrict__ out_rec,
     uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ err, uint32_t* __restrict__ queue,
     const unsigned long long* __restrict__ ext_sum, const unsigned char* __restrict__ log,
-    const uint32_t* __restrict__ group_head)
+    const uint32_t* __restrict__ group_head, int run_points)
 {
-    __shared__ BwdLds<REPLAY> lds_all[4];
+    extern __shared__ float4 lds_dyn[];
     const int wid = uniformi(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    BwdLds<REPLAY>& s = lds_all[wid];
+    BwdLds<REPLAY>& s = reinterpret_cast<BwdLds<REPLAY>*>(lds_dyn)[wid];
     const uint64_t lt = lanemask_lt();
     float reach[3] = { 0.f, 0.f, 0.f };
     if (!REPLAY) mean_extent(ext_sum, P, reach);
@@ -1242,7 +1244,7 @@ __global__ __launch_bounds__(256) void backward_kernel(
         } else {
             // walk the group's runs again and box-test each chunk (lane = candidate)
             const int nv = __popcll(__ballot(orig >= 0));
-            const uint64_t starts = split_runs(pme.x, pme.y, pme.z, nv, reach);
+            const uint64_t starts = split_runs(pme.x, pme.y, pme.z, nv, reach, run_points);
             const uint64_t finite = build_axes(pme.x, pme.y, pme.z, orig >= 0, s.axes);
             lds_fence();
             uint64_t runmask = 0;
@@ -1415,14 +1417,15 @@ inline int sparse_levels(int NL)
     return k;
 }
 
-// Grid of a persistent walk: as many 4-wave blocks as fit on the device at once
-// (a multiple of 8, one share per XCD), never more than the groups need.
-unsigned walk_blocks(const void* kernel, int threads, int NG)
+// Grid of a persistent walk: as many blocks as fit on the device at once (a
+// multiple of 8, one share per XCD), never more than the groups need.
+unsigned walk_blocks(const void* kernel, int threads, size_t lds, int NG)
 {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per <= 0 || cus <= 0) {
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, lds) != hipSuccess || per <= 0 ||
+        cus <= 0) {
         (void)hipGetLastError();
         per = 1;
         cus = 256;
@@ -1432,6 +1435,18 @@ unsigned walk_blocks(const void* kernel, int threads, int NG)
     const unsigned need = blocks_for((size_t)NG, (unsigned)(threads / 64));
     return nb < need ? nb : need;
 }
+
+// Walk kernels take their LDS (per-wave stacks, rings, staging) dynamically:
+// block_threads / 64 waves per block.
+int allow_lds(const void* kernel, size_t bytes)
+{
+    if (bytes > 160u * 1024u) return set_error(GVC_ERR_INVALID_ARGUMENT, "block_threads too large for the LDS");
+    if (bytes > 64u * 1024u)
+        GVC_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return GVC_OK;
+}
+
+int run_points_of(const gvc_params* prm) { return prm->run_points ? (int)prm->run_points : 64; }
 
 inline int key_bits_for(uint64_t maxv)
 {
@@ -1529,6 +1544,8 @@ int check_params(const gvc_params* prm)
         return set_error(GVC_ERR_INVALID_ARGUMENT, "leaf_size must be in [1, 64]");
     if (prm->block_threads < 64 || prm->block_threads > 1024 || (prm->block_threads % 64) != 0)
         return set_error(GVC_ERR_INVALID_ARGUMENT, "block_threads must be a multiple of 64 in [64, 1024]");
+    if (prm->run_points != 0 && prm->run_points != 16 && prm->run_points != 32 && prm->run_points != 64)
+        return set_error(GVC_ERR_INVALID_ARGUMENT, "run_points must be 0, 16, 32 or 64");
     return GVC_OK;
 }
 
@@ -1669,22 +1686,28 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
     }
     // 4. walk
     if ((stages & GVC_STAGE_WALK) && NG > 0) {
-        const int T = 256;
+        const int T = prm->block_threads;
         GVC_TRY(fill_u32(l.queues, 0u, 8, st));
         GVC_TRY(fill_u32(l.queues + 16, 0u, 4, st));
         const int log_on = (prm->flags & GVC_FLAG_CANDIDATE_LOG) ? 1 : 0;
+        const int rp = run_points_of(prm);
+        const size_t lds = (size_t)(T / 64) * sizeof(FwdLds);
         if (prm->flags & GVC_FLAG_KAHAN) {
-            const unsigned nb = walk_blocks(reinterpret_cast<const void*>(&forward_kernel<true>), T, NG);
-            forward_kernel<true><<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells,
-                                                   out_weights, l.group_records, l.group_pairs, l.err, l.queues,
-                                                   l.ext_sum, log_on, l.log, l.log_cap, l.queues + 16,
-                                                   l.group_head);
+            const void* kf = reinterpret_cast<const void*>(&forward_kernel<true>);
+            GVC_TRY(allow_lds(kf, lds));
+            const unsigned nb = walk_blocks(kf, T, lds, NG);
+            forward_kernel<true><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells,
+                                                     out_weights, l.group_records, l.group_pairs, l.err, l.queues,
+                                                     l.ext_sum, log_on, l.log, l.log_cap, l.queues + 16,
+                                                     l.group_head, rp);
         } else {
-            const unsigned nb = walk_blocks(reinterpret_cast<const void*>(&forward_kernel<false>), T, NG);
-            forward_kernel<false><<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells,
-                                                    out_weights, l.group_records, l.group_pairs, l.err, l.queues,
-                                                    l.ext_sum, log_on, l.log, l.log_cap, l.queues + 16,
-                                                    l.group_head);
+            const void* kf = reinterpret_cast<const void*>(&forward_kernel<false>);
+            GVC_TRY(allow_lds(kf, lds));
+            const unsigned nb = walk_blocks(kf, T, lds, NG);
+            forward_kernel<false><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells,
+                                                      out_weights, l.group_records, l.group_pairs, l.err, l.queues,
+                                                      l.ext_sum, log_on, l.log, l.log_cap, l.queues + 16,
+                                                      l.group_head, rp);
         }
         GVC_LAUNCH_CHECK("forward");
     }
@@ -1744,20 +1767,27 @@ int mixture_backward(int P, const float* means, const float* scales, const float
     const int NG = l.NG, NL = l.NL;
     const unsigned stages = prm->stages ? (unsigned)prm->stages : 3u;
     if ((stages & GVC_STAGE_WALK) && NG > 0 && capacity > 0) {
-        const int T = 256;
+        const int T = prm->block_threads;
         GVC_TRY(fill_u32(l.queues + 8, 0u, 8, st));
+        const int rp = run_points_of(prm);
         if (prm->flags & GVC_FLAG_CANDIDATE_LOG) {
-            const unsigned nb = walk_blocks(reinterpret_cast<const void*>(&backward_kernel<true>), T, NG);
-            backward_kernel<true><<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, NG, out_cells,
-                                                    out_weights, gcells, gweights, l.group_offsets, capacity, bl.parts,
-                                                    bl.keys, bl.vals, l.err, l.queues + 8, l.ext_sum, l.log,
-                                                    l.group_head);
+            const void* kf = reinterpret_cast<const void*>(&backward_kernel<true>);
+            const size_t lds = (size_t)(T / 64) * sizeof(BwdLds<true>);
+            GVC_TRY(allow_lds(kf, lds));
+            const unsigned nb = walk_blocks(kf, T, lds, NG);
+            backward_kernel<true><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, NG, out_cells,
+                                                      out_weights, gcells, gweights, l.group_offsets, capacity,
+                                                      bl.parts, bl.keys, bl.vals, l.err, l.queues + 8, l.ext_sum,
+                                                      l.log, l.group_head, rp);
         } else {
-            const unsigned nb = walk_blocks(reinterpret_cast<const void*>(&backward_kernel<false>), T, NG);
-            backward_kernel<false><<<nb, T, 0, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, NG, out_cells,
-                                                     out_weights, gcells, gweights, l.group_offsets, capacity,
-                                                     bl.parts, bl.keys, bl.vals, l.err, l.queues + 8, l.ext_sum,
-                                                     l.log, l.group_head);
+            const void* kf = reinterpret_cast<const void*>(&backward_kernel<false>);
+            const size_t lds = (size_t)(T / 64) * sizeof(BwdLds<false>);
+            GVC_TRY(allow_lds(kf, lds));
+            const unsigned nb = walk_blocks(kf, T, lds, NG);
+            backward_kernel<false><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, NG, out_cells,
+                                                       out_weights, gcells, gweights, l.group_offsets, capacity,
+                                                       bl.parts, bl.keys, bl.vals, l.err, l.queues + 8, l.ext_sum,
+                                                       l.log, l.group_head, rp);
         }
         GVC_LAUNCH_CHECK("backward");
     }

@@ -27,7 +27,8 @@ class GvcError(RuntimeError):
 
 class gvc_params(ctypes.Structure):
     _fields_ = [("leaf_size", ctypes.c_int), ("block_threads", ctypes.c_int),
-                ("flags", ctypes.c_uint), ("stages", ctypes.c_uint), ("chunk_slots", ctypes.c_uint)]
+                ("flags", ctypes.c_uint), ("stages", ctypes.c_uint), ("chunk_slots", ctypes.c_uint),
+                ("run_points", ctypes.c_uint)]
 
 
 _lib = None

@@ -1,0 +1,112 @@
+"""Autotuner for the mixture kernels.
+
+Sweeps BVH leaf size x points walked together (run_points: the wavefront's
+point grouping) x block shape on the caller's own data, times a forward +
+backward per configuration with HIP events, and keeps the fastest.  Results
+are cached per (log2 P, log2 S) bucket in memory and, if ``GVC_TUNE_CACHE``
+names a file, in that JSON file.  Every configuration computes the same
+answer (the pair predicate and the reduction order do not depend on them), so
+tuning never changes results.
+"""
+import json
+import math
+import os
+
+import torch
+
+from . import mixture as mx
+from ._lib import GvcError
+
+LEAF_SIZES = (2, 4, 8, 16)
+RUN_POINTS = (16, 32, 64)
+BLOCKS = (256, 512, 1024)
+
+_cache = {}
+
+
+def _bucket(P, S):
+    return f"{int(round(math.log2(max(P, 1))))}:{int(round(math.log2(max(S, 1))))}"
+
+
+def _load_file():
+    path = os.environ.get("GVC_TUNE_CACHE")
+    if path and os.path.exists(path):
+        try:
+            with open(path) as f:
+                return json.load(f)
+        except (OSError, ValueError):
+            return {}
+    return {}
+
+
+def _save_file(entries):
+    path = os.environ.get("GVC_TUNE_CACHE")
+    if not path:
+        return
+    data = _load_file()
+    data.update(entries)
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+
+
+def _time_config(params, args, sample_set, gV, gW, reps):
+    means, scales, rots, values, weights = args
+    ev = []
+    for r in range(reps + 1):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        cells, cw, _, st = mx.mixture_forward(means, scales, rots, values, weights, 1.0, sample_set, params,
+                                              want_conics=False)
+        mx.mixture_backward(st, means, scales, rots, values, weights, 1.0, sample_set, cells, cw, gV, gW)
+        e1.record()
+        if r > 0:
+            ev.append((e0, e1))
+    torch.cuda.synchronize()
+    return min(a.elapsed_time(b) for a, b in ev)
+
+
+def autotune(means, scales, rots, values, weights, sample_set, base=None, leaf_sizes=LEAF_SIZES,
+             run_points=RUN_POINTS, blocks=BLOCKS, reps=2, use_cache=True):
+    """Returns (best MixtureParams, {config label: ms}).  ``base`` supplies the
+    precision and candidate-log choices, which are not tuned."""
+    base = base or mx.default_params()
+    P, S = int(means.shape[0]), int(sample_set.S)
+    key = _bucket(P, S)
+    if use_cache:
+        if key in _cache:
+            return _cache[key], {}
+        disk = _load_file().get(key)
+        if disk:
+            best = mx.MixtureParams(leaf_size=disk["leaf_size"], block_threads=disk["block_threads"],
+                                    run_points=disk["run_points"], kahan=base.kahan, fp64_cond=base.fp64_cond,
+                                    candidate_log=base.candidate_log)
+            _cache[key] = best
+            return best, {}
+    g = torch.Generator(device=means.device)
+    g.manual_seed(1234)
+    gV = torch.randn(S, device=means.device, generator=g)
+    gW = torch.randn(S, device=means.device, generator=g)
+    args = (means, scales, rots, values, weights)
+    table = {}
+    best, best_ms = None, float("inf")
+    for L in leaf_sizes:
+        for rp in run_points:
+            for bt in blocks:
+                prm = mx.MixtureParams(leaf_size=L, block_threads=bt, run_points=rp, kahan=base.kahan,
+                                       fp64_cond=base.fp64_cond, candidate_log=base.candidate_log)
+                label = f"leaf{L}/run{rp}/block{bt}"
+                try:
+                    ms = _time_config(prm, args, sample_set, gV, gW, reps)
+                except GvcError:
+                    table[label] = None      # infeasible (LDS)
+                    continue
+                table[label] = ms
+                if ms < best_ms:
+                    best, best_ms = prm, ms
+    if best is None:
+        raise GvcError("autotune: no feasible configuration")
+    _cache[key] = best
+    _save_file({key: {"leaf_size": best.leaf_size, "block_threads": best.block_threads,
+                      "run_points": best.run_points, "ms": best_ms}})
+    return best, table

@@ -30,12 +30,13 @@ class MixtureParams:
     fp64_cond: bool = False
     candidate_log: bool = True
     chunk_slots: int = 0
+    run_points: int = 64
 
     def c(self):
         flags = ((GVC_FLAG_KAHAN if self.kahan else 0) | (GVC_FLAG_FP64_COND if self.fp64_cond else 0) |
                  (GVC_FLAG_CANDIDATE_LOG if self.candidate_log else 0))
         return gvc_params(int(self.leaf_size), int(self.block_threads), flags, 0,
-                          int(self.chunk_slots or _chunk_slots))
+                          int(self.chunk_slots or _chunk_slots), int(self.run_points))
 
 
 _chunk_slots = 16          # candidate-log slots per group used when params.chunk_slots == 0

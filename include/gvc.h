@@ -52,10 +52,11 @@ enum gvc_status {
 /* Launch/tuning parameters of the mixture kernels (chosen by the autotuner). */
 typedef struct gvc_params {
     int leaf_size;        /* Gaussians per BVH leaf: 1..64                        */
-    int block_threads;    /* threads per block of the traversal kernels: 64..1024 */
+    int block_threads;    /* threads per block of the walk kernels: 64..1024      */
     unsigned flags;       /* GVC_FLAG_*                                           */
     unsigned stages;      /* GVC_STAGE_* mask, 0 = all                            */
     unsigned chunk_slots; /* candidate-log slots per 64-point group, 0 = 16       */
+    unsigned run_points;  /* most points walked together: 16, 32, 64 (0 = 64)     */
 } gvc_params;
 
 /* Error flags reported by gvc_forward_stats (bitwise OR). */

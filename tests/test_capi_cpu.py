@@ -52,9 +52,9 @@ def test_host_side_queries(L):
     assert L.gvc_status_string(0) == b"ok"
     assert L.gvc_status_string(1) == b"invalid argument"
     assert L.gvc_sample_set_bytes(1000) > 1000 * 16
-    p = _lib.gvc_params(4, 256, 1, 0, 0)
+    p = _lib.gvc_params(4, 256, 1, 0, 0, 0)
     assert L.gvc_forward_state_bytes(1000, 1000, ctypes.byref(p)) > 1000 * 80
-    bad = _lib.gvc_params(0, 256, 1, 0, 0)
+    bad = _lib.gvc_params(0, 256, 1, 0, 0, 0)
     assert L.gvc_forward_state_bytes(1000, 1000, ctypes.byref(bad)) == 0
     assert L.gvc_backward_bytes(1000, 5000) > 5000 * 48
 

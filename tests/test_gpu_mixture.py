@@ -286,3 +286,31 @@ def test_candidate_log_overflow_recovers():
         assert st.stats()[2] & mx._lib.GVC_ERRFLAG_CHUNK_OVERFLOW
     finally:
         mx._chunk_slots = saved
+
+
+@pytest.mark.parametrize("leaf,block,run", [(4, 512, 32), (8, 1024, 16), (2, 256, 16)])
+def test_launch_shapes_parity(leaf, block, run):
+    """Block shape, run length and leaf size change the schedule, never the
+    predicate: exact pair counts and oracle-level fields and gradients."""
+    g = workloads.config0(5)
+    scales = np.exp(g["log_scales"]).astype(f32)
+    params = mx.MixtureParams(leaf_size=leaf, block_threads=block, run_points=run)
+    ss, cells, cw, conics, st = run_gpu(g["means"], scales, g["rotations"], g["values"], g["weights"],
+                                        g["samples"], params)
+    conic_ref, box_ref, _ = oracle.preprocess(g["means"], scales, g["rotations"], g["weights"])
+    Vr, Wr, cnt = oracle.forward(g["means"], g["values"], g["weights"], conic_ref, box_ref, g["samples"])
+    nrec, npairs, err = st.stats()
+    assert err == 0 and npairs == int(cnt.sum())
+    assert field_err(cells.cpu().numpy(), cw.cpu().numpy(), Vr, Wr) <= 1e-5
+    _grad_check(g, scales, g["samples"][::4].copy(), params)
+
+
+def test_autotune_small_sweep():
+    from gvc.autotune import autotune
+    g = workloads.config0(6)
+    scales = np.exp(g["log_scales"]).astype(f32)
+    ss = mx.SampleSet(T(g["samples"]))
+    best, table = autotune(T(g["means"]), T(scales), T(g["rotations"]), T(g["values"]), T(g["weights"]), ss,
+                           leaf_sizes=(2, 4), run_points=(32, 64), blocks=(256,), reps=1, use_cache=False)
+    assert best.leaf_size in (2, 4) and best.run_points in (32, 64)
+    assert len(table) == 4 and all(v is not None and v > 0 for v in table.values())
