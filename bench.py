@@ -8,13 +8,17 @@ distinct mesh vertices, initialised like the reference (3-NN scales via
 distCUDA2, identity rotations, weight 0.1, value = vertex value).  Every step:
 
   1. 2^22 uniform sample points (torch.rand on the device);
-  2. exact tet location + barycentric interpolation of the ground truth (libgvc);
-  3. the sample set (Morton sort into 64-point groups);
-  4. activations and the differentiable mixture forward (libgvc, drop-in
+  2. the sample set (Morton sort into 64-point groups);
+  3. exact tet location + barycentric interpolation of the ground truth (libgvc);
+  4. the reference model's activations (soft-capped exp scales, max_scale 0.02,
+     normalised quaternions, sigmoid weights and values; fused libgvc kernel);
+  5. the differentiable mixture forward (libgvc, drop-in
      bvh_diff_gaussian_rasterization API);
-  5. masked L1 + false-negative term (R/train.py:389-420), backward (libgvc
-     adjoint, deterministic reduction);
-  6. Adam (lr 1e-3).
+  6. masked L1 + false-negative loss (R/train.py:389-420, fused libgvc kernel),
+     backward through loss, mixture (deterministic reduction) and activations;
+  7. Adam (lr 1e-3).
+With --graph, steps 2-7 are captured once in a hipGraph and replayed (the
+samples are drawn into a static buffer before each replay).
 
 value = Gaussian-point evaluations per second, forward + backward
 (2 x accepted pairs per step, summed over the timed steps / wall time).
@@ -53,6 +57,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--graph", action="store_true", help="capture the training step in a hipGraph")
+    ap.add_argument("--max-scale", type=float, default=0.02,
+                    help="soft scale cap of the reference model (R/scene/gaussian_model.py:64); 0 disables it")
     ap.add_argument("--autotune", action="store_true",
                     help="choose leaf size x run points x block shape on the workload before timing")
     ap.add_argument("--recompute", action="store_true",
@@ -88,6 +94,7 @@ def main():
 
     from bvh_diff_gaussian_rasterization import GaussianRasterizationSettings, GaussianRasterizer
     from gvc import mixture as mx
+    from gvc import training
     from gvc import workloads
     from gvc.sampling import TetMeshSampler
 
@@ -133,11 +140,13 @@ def main():
         log(f"autotune: {tune['chosen']}")
 
     xyz = torch.nn.Parameter(W["means"].clone())
+    # raw parameters as the reference model keeps them (log-scales of the capped
+    # scale, logits of weight and value)
     log_s = torch.nn.Parameter(W["log_scales"].clone())
     rot = torch.nn.Parameter(W["rotations"].clone())
     wl = torch.nn.Parameter(torch.logit(W["weights"]).clone())
     vl = torch.nn.Parameter(torch.logit(W["values"].clamp(1e-4, 1 - 1e-4)).clone())
-    opt = torch.optim.Adam([xyz, log_s, rot, wl, vl], lr=1e-3, eps=1e-15)
+    opt = torch.optim.Adam([xyz, log_s, rot, wl, vl], lr=1e-3, eps=1e-15, capturable=args.graph)
     settings = GaussianRasterizationSettings(volume_mins=(0, 0, 0), volume_maxes=(1, 1, 1), cell_count=128, bg=-1.0,
                                              scale_modifier=1.0, use_gaussian_bvh=False, debug=False)
     rast = GaussianRasterizer(settings)
@@ -145,72 +154,101 @@ def main():
     gen = torch.Generator(device=dev)
     gen.manual_seed(args.seed + 1)
     stage_ev = []
+    audit = []
+    acc = torch.zeros(3, dtype=torch.int64, device=dev)   # pairs, records, error flags
+    timing_stages = [False]
 
-    def step():
-        samples = torch.rand((S, 3), device=dev, generator=gen)
-        e0 = torch.cuda.Event(enable_timing=True); e0.record()
-        rast.build_bvh(samples)
-        e1 = torch.cuda.Event(enable_timing=True); e1.record()
-        stage_ev.append(("sample_set", e0, e1))
+    def stamp(name, fn):
+        if not timing_stages[0]:
+            return fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        out = fn()
+        e1.record()
+        stage_ev.append((name, e0, e1))
+        return out
+
+    def body(samples):
+        audit.clear()
+        stamp("sample_set", lambda: rast.build_bvh(samples))
         if sampler is not None:
-            e0 = torch.cuda.Event(enable_timing=True); e0.record()
-            gt64, _ = sampler.sample(samples, sample_set=rast.sample_set)
-            gt = gt64.float()
-            e1 = torch.cuda.Event(enable_timing=True); e1.record()
-            stage_ev.append(("tet_locate", e0, e1))
+            gt = stamp("tet_locate", lambda: sampler.sample(samples, sample_set=rast.sample_set)[0])
         else:
             gt = W["grid_fn"](samples)
-        scales = torch.exp(log_s)
-        rots = torch.nn.functional.normalize(rot)
-        w = torch.sigmoid(wl)
-        v = torch.sigmoid(vl)
+        scales, rots, w, v = stamp("activate", lambda: training.activate(log_s, rot, wl, vl, args.max_scale))
         cells, cw = rast(xyz, scales, rots, v, w)
-        valid = ((gt != -1) & (cells != -1)).float()
-        l1 = ((cells - gt).abs() * valid).sum() / valid.sum().clamp_min(1.0)
-        fn_mask = ((gt != -1) & (cw > 0) & (cw < 0.011)).float()
-        fn_vals = (0.011 - cw).clamp_min(0) * fn_mask
-        fn = fn_reg * fn_vals.sum() / ((fn_vals > 0).float().sum() + 1e-8)
-        loss = l1 + fn
+        loss = stamp("loss", lambda: training.fit_loss(cells, cw, gt, fn_reg))
         loss.backward()
-        opt.step()
+        stamp("adam", lambda: opt.step())
         opt.zero_grad(set_to_none=True)
+        audit[-1].accumulate(acc)
         return samples, cells, cw
 
-    # warmup: also sizes the backward's record capacity (no host reads later)
+    def eager_step():
+        return body(torch.rand((S, 3), device=dev, generator=gen))
+
+    # warmup: sizes the backward's record capacity and the candidate log (no host
+    # reads afterwards)
     mx.set_record_capacity(None)
-    audit = []
     mx.set_audit(audit)
+    max_rec = 0
     for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    max_rec = max(a.stats()[0] for a in audit) if audit else 0
-    mx.set_record_capacity(int(max_rec * 1.25) + 4096)
-    audit.clear()
-    stage_ev.clear()
+        eager_step()
+        torch.cuda.synchronize()
+        max_rec = max(max_rec, audit[-1].stats()[0])
+    cap = int(max_rec * 1.25) + 4096
+    mx.set_record_capacity(cap)
+    log(f"warmup done, record capacity {cap}, candidate-log slots/group {mx._chunk_slots}")
+    # stage profile (untimed steps, HIP events between the stages)
     timer = []
     mx.set_stage_timer(timer)
-    log(f"warmup done, record capacity {int(max_rec * 1.25) + 4096}")
+    timing_stages[0] = True
+    for _ in range(3):
+        eager_step()
+    torch.cuda.synchronize()
+    timing_stages[0] = False
+    mx.set_stage_timer(None)
+    stage_ms = {}
+    for name, e0, e1 in timer + stage_ev:
+        stage_ms.setdefault(name, []).append(e0.elapsed_time(e1))
+    stage_avg = {k: float(np.mean(v)) for k, v in stage_ms.items()}
 
+    graph = None
+    if args.graph:
+        static = torch.empty((S, 3), device=dev)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(2):
+                torch.rand((S, 3), device=dev, generator=gen, out=static)
+                body(static)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        opt.zero_grad(set_to_none=True)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            graph_out = body(static)
+        torch.cuda.synchronize()
+        log("training step captured in a graph")
+
+    acc.zero_()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        last = step()
+        if graph is None:
+            last = eager_step()
+        else:
+            torch.rand((S, 3), device=dev, generator=gen, out=static)
+            graph.replay()
+            last = graph_out
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    mx.set_stage_timer(None)
     mx.set_audit(None)
     wall = t1 - t0
-
-    stats = [a.stats() for a in audit]
-    pairs = [s[1] for s in stats]
-    errs = [s[2] for s in stats]
-    if any(errs):
+    total_pairs, total_records, errs = (int(x) for x in acc.cpu().tolist())
+    if errs:
         raise RuntimeError(f"mixture error flags during the timed steps: {errs}")
-    total_pairs = sum(pairs)
-    stage_ms = {}
-    for name, a, b in timer + stage_ev:
-        stage_ms.setdefault(name, []).append(a.elapsed_time(b))
-    stage_avg = {k: float(np.mean(v)) for k, v in stage_ms.items()}
     pairs_per_step = total_pairs / args.steps
     value = 2.0 * total_pairs / wall
     dom = max(("fwd_walk", "bwd_walk"), key=lambda k: stage_avg.get(k, 0.0))
@@ -254,10 +292,7 @@ def main():
         samples, cells, cw = last
         with torch.no_grad():
             m = xyz.detach()
-            sc = torch.exp(log_s).detach()
-            rq = torch.nn.functional.normalize(rot).detach()
-            wv = torch.sigmoid(wl).detach()
-            vv = torch.sigmoid(vl).detach()
+            sc, rq, wv, vv = (t.detach() for t in training.activate(log_s, rot, wl, vl, args.max_scale))
             # the last step's forward ran before its Adam update: re-evaluate at
             # the current parameters for the error comparison
             ss = mx.SampleSet(samples)

@@ -35,6 +35,8 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
 int forward_stats(const void* state, int P, int S, const gvc_params* prm, uint64_t* n_records, uint64_t* n_pairs,
                   uint32_t* error_flags, uint64_t* chunk_slots_used, hipStream_t st);
 size_t backward_bytes(int P, uint64_t cap);
+int forward_accumulate(const void* state, int P, int S, const gvc_params* prm, unsigned long long* accum,
+                       hipStream_t st);
 int mixture_backward(int P, const float* means, const float* scales, const float* rots, const float* values,
                      const float* weights, float k, const void* sample_set, int S, const gvc_params* prm,
                      const float* out_cells, const float* out_weights, const float* gcells, const float* gweights,
@@ -43,6 +45,19 @@ int mixture_backward(int P, const float* means, const float* scales, const float
 int sort_pairs_u64(int n, const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, int key_bits,
                    void* ws, size_t ws_bytes, hipStream_t st);
 size_t sort_pairs_u64_bytes(int n);
+
+// training.hip
+int activate_forward(int P, const float* ls, const float* q, const float* wl, const float* vl, float max_scale,
+                     float* scales, float* qn, float* w, float* v, hipStream_t st);
+int activate_backward(int P, const float* ls, const float* q, const float* wl, const float* vl, float max_scale,
+                      const float* gs, const float* gq, const float* gw, const float* gv, float* dls, float* dq,
+                      float* dwl, float* dvl, hipStream_t st);
+size_t loss_state_bytes(int S);
+int loss_forward(int S, const float* cells, const float* weights, const float* gt, float fn_reg, float fp_reg,
+                 int surface, float* loss, void* state, size_t bytes, hipStream_t st);
+int loss_backward(int S, const float* cells, const float* weights, const float* gt, float fn_reg, float fp_reg,
+                  int surface, const float* upstream, const void* state, size_t bytes, float* dcells,
+                  float* dweights, hipStream_t st);
 
 // mesh.hip, knn.hip
 size_t tet_locator_bytes(int n_points, int C, unsigned long long cap);
@@ -152,6 +167,12 @@ int gvc_forward_stats(const void* state, int P, int S, const gvc_params* params,
                          (hipStream_t)stream);
 }
 
+int gvc_forward_accumulate(const void* state, int P, int S, const gvc_params* params, uint64_t* accum, void* stream)
+{
+    GVC_REQUIRE(state && accum, "NULL pointer");
+    return forward_accumulate(state, P, S, params, reinterpret_cast<unsigned long long*>(accum), (hipStream_t)stream);
+}
+
 size_t gvc_backward_bytes(int P, uint64_t record_capacity)
 {
     return P >= 0 ? backward_bytes(P, record_capacity) : 0;
@@ -239,6 +260,57 @@ int gvc_knn3(int n, const float* points, float* mean_sq_dist, void* workspace, s
     if (n == 0) return GVC_OK;
     GVC_REQUIRE(points && mean_sq_dist && workspace, "NULL pointer");
     return knn3(n, points, mean_sq_dist, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+int gvc_activate_forward(int P, const float* log_scales, const float* rotations, const float* weight_logits,
+                         const float* value_logits, float max_scale, float* scales, float* rotations_n,
+                         float* weights, float* values, void* stream)
+{
+    GVC_REQUIRE(P >= 0, "P < 0");
+    if (P == 0) return GVC_OK;
+    GVC_REQUIRE(log_scales && rotations && weight_logits && value_logits && scales && rotations_n && weights && values,
+                "NULL pointer");
+    return activate_forward(P, log_scales, rotations, weight_logits, value_logits, max_scale, scales, rotations_n,
+                            weights, values, (hipStream_t)stream);
+}
+
+int gvc_activate_backward(int P, const float* log_scales, const float* rotations, const float* weight_logits,
+                          const float* value_logits, float max_scale, const float* dL_dscales,
+                          const float* dL_drotations_n, const float* dL_dweights, const float* dL_dvalues,
+                          float* dL_dlog_scales, float* dL_drotations, float* dL_dweight_logits,
+                          float* dL_dvalue_logits, void* stream)
+{
+    GVC_REQUIRE(P >= 0, "P < 0");
+    if (P == 0) return GVC_OK;
+    GVC_REQUIRE(log_scales && rotations && weight_logits && value_logits && dL_dscales && dL_drotations_n &&
+                    dL_dweights && dL_dvalues && dL_dlog_scales && dL_drotations && dL_dweight_logits &&
+                    dL_dvalue_logits,
+                "NULL pointer");
+    return activate_backward(P, log_scales, rotations, weight_logits, value_logits, max_scale, dL_dscales,
+                             dL_drotations_n, dL_dweights, dL_dvalues, dL_dlog_scales, dL_drotations,
+                             dL_dweight_logits, dL_dvalue_logits, (hipStream_t)stream);
+}
+
+size_t gvc_loss_state_bytes(int S) { return S >= 0 ? loss_state_bytes(S) : 0; }
+
+int gvc_loss_forward(int S, const float* cells, const float* weights, const float* gt, float fn_reg, float fp_reg,
+                     int encode_surface, float* loss, void* state, size_t state_bytes, void* stream)
+{
+    GVC_REQUIRE(S >= 0, "S < 0");
+    GVC_REQUIRE(loss && state && (S == 0 || (cells && weights && gt)), "NULL pointer");
+    return loss_forward(S, cells, weights, gt, fn_reg, fp_reg, encode_surface, loss, state, state_bytes,
+                        (hipStream_t)stream);
+}
+
+int gvc_loss_backward(int S, const float* cells, const float* weights, const float* gt, float fn_reg,
+                      float fp_reg, int encode_surface, const float* dL_dloss, const void* state,
+                      size_t state_bytes, float* dL_dcells, float* dL_dweights, void* stream)
+{
+    GVC_REQUIRE(S >= 0, "S < 0");
+    if (S == 0) return GVC_OK;
+    GVC_REQUIRE(cells && weights && gt && state && dL_dcells && dL_dweights, "NULL pointer");
+    return loss_backward(S, cells, weights, gt, fn_reg, fp_reg, encode_surface, dL_dloss, state, state_bytes,
+                         dL_dcells, dL_dweights, (hipStream_t)stream);
 }
 
 }  // extern "C"

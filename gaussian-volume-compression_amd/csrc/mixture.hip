@@ -1407,6 +1407,27 @@ __global__ void stats_kernel(const uint32_t* __restrict__ group_pairs, int NG, c
     }
 }
 
+// accum[0] += pairs, accum[1] += records, accum[2] |= error flags (one block:
+// plain read-modify-write, stream-ordered, capturable).
+__global__ void accumulate_kernel(const uint32_t* __restrict__ group_pairs, int NG, const uint32_t* __restrict__ offsets,
+                                  const uint32_t* __restrict__ err, unsigned long long* __restrict__ accum)
+{
+    __shared__ unsigned long long part[256];
+    unsigned long long s = 0;
+    for (int i = threadIdx.x; i < NG; i += blockDim.x) s += group_pairs[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = 128; d >= 1; d >>= 1) {
+        if ((int)threadIdx.x < d) part[threadIdx.x] += part[threadIdx.x + d];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        accum[0] += part[0];
+        accum[1] += offsets[NG];
+        accum[2] |= *err;
+    }
+}
+
 inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 // levels of the leaf-range sparse table: 1 + floor(log2(NL))
@@ -1735,6 +1756,17 @@ int forward_stats(const void* state, int P, int S, const gvc_params* prm, uint64
     if (n_pairs) *n_pairs = h[1];
     if (error_flags) *error_flags = e;
     if (chunk_slots_used) *chunk_slots_used = used;
+    return GVC_OK;
+}
+
+int forward_accumulate(const void* state, int P, int S, const gvc_params* prm, unsigned long long* accum,
+                       hipStream_t st)
+{
+    GVC_TRY(check_params(prm));
+    Arena a(const_cast<void*>(state), (size_t)-1);
+    ForwardLayout l(a, P, S, prm->leaf_size, prm->flags, prm->chunk_slots);
+    accumulate_kernel<<<1, 256, 0, st>>>(l.group_pairs, l.NG, l.group_offsets, l.err, accum);
+    GVC_LAUNCH_CHECK("accumulate");
     return GVC_OK;
 }
 

@@ -57,6 +57,7 @@ _SIGS = {
     "gvc_forward_stats": (_I, [_P, _I, _I, _PP, ctypes.POINTER(_U64), ctypes.POINTER(_U64),
                                ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_U64), _P]),
     "gvc_backward_bytes": (_SZ, [_I, _U64]),
+    "gvc_forward_accumulate": (_I, [_P, _I, _I, _PP, _P, _P]),
     "gvc_mixture_backward": (_I, [_I, _P, _P, _P, _P, _P, _F, _P, _I, _PP, _P, _P, _P, _P, _P, _SZ, _U64,
                                   _P, _SZ, _P, _P, _P, _P, _P, _P]),
     "gvc_tet_locator_bytes": (_SZ, [_I, _I, _U64]),
@@ -66,6 +67,11 @@ _SIGS = {
     "gvc_grid_sample": (_I, [_P, _P, _P, _I, _P, _I, _P, _P, _P]),
     "gvc_knn3_bytes": (_SZ, [_I]),
     "gvc_knn3": (_I, [_I, _P, _P, _P, _SZ, _P]),
+    "gvc_activate_forward": (_I, [_I, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P]),
+    "gvc_activate_backward": (_I, [_I, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "gvc_loss_state_bytes": (_SZ, [_I]),
+    "gvc_loss_forward": (_I, [_I, _P, _P, _P, _F, _F, _I, _P, _P, _SZ, _P]),
+    "gvc_loss_backward": (_I, [_I, _P, _P, _P, _F, _F, _I, _P, _P, _SZ, _P, _P, _P]),
 }
 
 EXPORTED = tuple(_SIGS.keys())

@@ -152,6 +152,12 @@ class ForwardState:
         self.chunk_slots_used = int(used.value)
         return int(nr.value), int(npairs.value), int(err.value)
 
+    def accumulate(self, accum):
+        """Stream-ordered: accum (int64 [3] on the device) += (pairs, records),
+        accum[2] |= error flags.  No host synchronisation (graph-capturable)."""
+        call("gvc_forward_accumulate", _ptr(self.buffer), self.P, self.S, ctypes.byref(self._c), _ptr(accum),
+             _stream(self.buffer.device))
+
 
 def _grow_chunk_slots(fstate):
     """Raises the module's candidate-log high-water mark to fit ``fstate``'s forward."""

@@ -124,6 +124,15 @@ int gvc_mixture_forward(int P, const float *means3D, const float *scales, const 
 int gvc_forward_stats(const void *state, int P, int S, const gvc_params *params, uint64_t *n_records,
                       uint64_t *n_pairs, uint32_t *error_flags, uint64_t *chunk_slots_used, void *stream);
 
+/* Stream-ordered, non-blocking counterpart of gvc_forward_stats for timed or
+ * graph-captured loops: accum (3 x uint64 on the device) gets accum[0] += pairs,
+ * accum[1] += records, accum[2] |= error flags of this state so far (call it after
+ * the backward to include the backward's flags).  No counterpart in the reference:
+ * it replaces the reference's blocking per-call pair-count read-back
+ * (CR/rasterizer_impl.cu:163-166) for callers that must not synchronise. */
+int gvc_forward_accumulate(const void *state, int P, int S, const gvc_params *params, uint64_t *accum,
+                           void *stream);
+
 /* Backward: gradients for means3D [P,3], scales [P,3], rotations [P,4], values
  * [P,1], weights [P,1] from dL/dout_cells, dL/dout_cell_weights [S].  Replaces
  * _C.rasterize_gaussians_backward, DGR/ext.cu:107-147, DGR/rasterize_points.cu:75-137,
@@ -188,6 +197,37 @@ int gvc_grid_sample(const int64_t *dims, const float *origin, const float *spaci
 size_t gvc_knn3_bytes(int n);
 int gvc_knn3(int n, const float *points, float *mean_sq_dist, void *workspace, size_t workspace_bytes,
              void *stream);
+
+/* ---- training-step glue ----------------------------------------------------------
+ * Activations, R/scene/gaussian_model.py:31-48, :71-77, :114-132 (get_scaling with
+ * _apply_cap, get_rotation, get_weight, get_values): scales [P,3] = s * r_soft / r
+ * with s = exp(log_scales), r = |s| + 1e-8, r_soft = max_scale * tanh(r / max_scale)
+ * (max_scale <= 0 disables the cap; the reference uses 0.02); rotations [P,4] =
+ * q / max(|q|, 1e-12); weights, values [P] = sigmoid of the logits.  The backward
+ * maps output gradients to gradients of the raw parameters. */
+int gvc_activate_forward(int P, const float *log_scales, const float *rotations, const float *weight_logits,
+                         const float *value_logits, float max_scale, float *scales, float *rotations_n,
+                         float *weights, float *values, void *stream);
+int gvc_activate_backward(int P, const float *log_scales, const float *rotations, const float *weight_logits,
+                          const float *value_logits, float max_scale, const float *dL_dscales,
+                          const float *dL_drotations_n, const float *dL_dweights, const float *dL_dvalues,
+                          float *dL_dlog_scales, float *dL_drotations, float *dL_dweight_logits,
+                          float *dL_dvalue_logits, void *stream);
+
+/* Fitting loss of R/train.py:389-420 (l1_loss: R/utils/loss_utils.py:19-20):
+ * L1 over (gt != -1) & (cells != -1) as a mean, plus fn_reg x the mean positive
+ * clamp(0.011 - w) over (gt != -1) & (0 < w < t) with t = 0.011 (0.0105 when
+ * encode_surface), plus, when encode_surface, fp_reg x the mean positive w over
+ * (gt == -1) & (w > 0).  loss: one float on the device.  Sums in FP64 in a fixed
+ * order (deterministic); `state` (gvc_loss_state_bytes) keeps them for the
+ * backward, which writes dL/dcells and dL/dweights scaled by *dL_dloss (device
+ * float, NULL = 1). */
+size_t gvc_loss_state_bytes(int S);
+int gvc_loss_forward(int S, const float *cells, const float *weights, const float *gt, float fn_reg, float fp_reg,
+                     int encode_surface, float *loss, void *state, size_t state_bytes, void *stream);
+int gvc_loss_backward(int S, const float *cells, const float *weights, const float *gt, float fn_reg,
+                      float fp_reg, int encode_surface, const float *dL_dloss, const void *state,
+                      size_t state_bytes, float *dL_dcells, float *dL_dweights, void *stream);
 
 #ifdef __cplusplus
 }
