@@ -46,6 +46,27 @@ int sort_pairs_u64(int n, const uint64_t* kin, const uint32_t* vin, uint64_t* ko
                    void* ws, size_t ws_bytes, hipStream_t st);
 size_t sort_pairs_u64_bytes(int n);
 
+// batches (mixture.hip)
+size_t batch_table_bytes(int V);
+int batch_describe(int V, const int* P_counts, const int* S_counts, int L, void* table, size_t bytes, gvc_batch* out,
+                   hipStream_t st);
+size_t sample_set_bytes_batched(const gvc_batch* b);
+int sample_set_build_batched(const gvc_batch* b, const float* samples, void* set, size_t bytes, hipStream_t st);
+size_t forward_state_bytes_batched(const gvc_batch* b, const gvc_params* prm);
+int mixture_forward_batched(const gvc_batch* b, const float* means, const float* scales, const float* rots,
+                            const float* values, const float* weights, float k, const void* sample_set,
+                            const gvc_params* prm, float* out_cells, float* out_weights, float* conics, void* state,
+                            size_t state_bytes, hipStream_t st);
+int forward_stats_batched(const gvc_batch* b, const void* state, const gvc_params* prm, uint64_t* n_records,
+                          uint64_t* n_pairs, uint32_t* error_flags, uint64_t* chunk_slots_used, hipStream_t st);
+int forward_accumulate_batched(const gvc_batch* b, const void* state, const gvc_params* prm,
+                               unsigned long long* accum, hipStream_t st);
+int mixture_backward_batched(const gvc_batch* b, const float* scales, const float* rots, float k,
+                             const void* sample_set, const gvc_params* prm, const float* out_cells,
+                             const float* out_weights, const float* gcells, const float* gweights, void* state,
+                             size_t state_bytes, uint64_t capacity, void* ws, size_t ws_bytes, float* dmeans,
+                             float* dscales, float* drots, float* dvalues, float* dweights, hipStream_t st);
+
 // training.hip
 int activate_forward(int P, const float* ls, const float* q, const float* wl, const float* vl, float max_scale,
                      float* scales, float* qn, float* w, float* v, hipStream_t st);
@@ -260,6 +281,89 @@ int gvc_knn3(int n, const float* points, float* mean_sq_dist, void* workspace, s
     if (n == 0) return GVC_OK;
     GVC_REQUIRE(points && mean_sq_dist && workspace, "NULL pointer");
     return knn3(n, points, mean_sq_dist, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+size_t gvc_batch_table_bytes(int volumes) { return batch_table_bytes(volumes); }
+
+int gvc_batch_describe(int volumes, const int* gaussians_per_volume, const int* samples_per_volume, int leaf_size,
+                       void* table, size_t table_bytes, gvc_batch* out, void* stream)
+{
+    GVC_REQUIRE(gaussians_per_volume && samples_per_volume && table && out, "NULL pointer");
+    return batch_describe(volumes, gaussians_per_volume, samples_per_volume, leaf_size, table, table_bytes, out,
+                          (hipStream_t)stream);
+}
+
+#define GVC_REQUIRE_BATCH(b)                                                                                  \
+    GVC_REQUIRE((b) && (b)->table && (b)->volumes >= 1, "batch not described (gvc_batch_describe)")
+
+size_t gvc_sample_set_bytes_batched(const gvc_batch* batch)
+{
+    return (batch && batch->volumes >= 1) ? sample_set_bytes_batched(batch) : 0;
+}
+
+int gvc_sample_set_build_batched(const gvc_batch* batch, const float* samples, void* sample_set, size_t bytes,
+                                 void* stream)
+{
+    GVC_REQUIRE_BATCH(batch);
+    GVC_REQUIRE(sample_set && (batch->S_total == 0 || samples), "NULL pointer");
+    return sample_set_build_batched(batch, samples, sample_set, bytes, (hipStream_t)stream);
+}
+
+size_t gvc_forward_state_bytes_batched(const gvc_batch* batch, const gvc_params* params)
+{
+    return (batch && batch->volumes >= 1 && params) ? forward_state_bytes_batched(batch, params) : 0;
+}
+
+int gvc_mixture_forward_batched(const gvc_batch* batch, const float* means3D, const float* scales,
+                                const float* rotations, const float* values, const float* weights,
+                                float scale_modifier, const void* sample_set, const gvc_params* params,
+                                float* out_cells, float* out_weights, float* conics, void* state, size_t state_bytes,
+                                void* stream)
+{
+    GVC_REQUIRE_BATCH(batch);
+    GVC_REQUIRE(means3D && scales && rotations && values && weights && sample_set && state && out_cells &&
+                    out_weights,
+                "NULL pointer");
+    return mixture_forward_batched(batch, means3D, scales, rotations, values, weights, scale_modifier, sample_set,
+                                   params, out_cells, out_weights, conics, state, state_bytes, (hipStream_t)stream);
+}
+
+int gvc_forward_stats_batched(const gvc_batch* batch, const void* state, const gvc_params* params,
+                              uint64_t* n_records, uint64_t* n_pairs, uint32_t* error_flags,
+                              uint64_t* chunk_slots_used, void* stream)
+{
+    GVC_REQUIRE_BATCH(batch);
+    GVC_REQUIRE(state, "NULL state");
+    return forward_stats_batched(batch, state, params, n_records, n_pairs, error_flags, chunk_slots_used,
+                                 (hipStream_t)stream);
+}
+
+int gvc_forward_accumulate_batched(const gvc_batch* batch, const void* state, const gvc_params* params,
+                                   uint64_t* accum, void* stream)
+{
+    GVC_REQUIRE_BATCH(batch);
+    GVC_REQUIRE(state && accum, "NULL pointer");
+    return forward_accumulate_batched(batch, state, params, reinterpret_cast<unsigned long long*>(accum),
+                                      (hipStream_t)stream);
+}
+
+int gvc_mixture_backward_batched(const gvc_batch* batch, const float* scales, const float* rotations,
+                                 float scale_modifier, const void* sample_set, const gvc_params* params,
+                                 const float* out_cells, const float* out_weights, const float* dL_dout_cells,
+                                 const float* dL_dout_cell_weights, void* state, size_t state_bytes,
+                                 uint64_t record_capacity, void* workspace, size_t workspace_bytes,
+                                 float* dL_dmeans3D, float* dL_dscales, float* dL_drotations, float* dL_dvalues,
+                                 float* dL_dweights, void* stream)
+{
+    GVC_REQUIRE_BATCH(batch);
+    GVC_REQUIRE(scales && rotations && sample_set && out_cells && out_weights && dL_dout_cells &&
+                    dL_dout_cell_weights && state && workspace && dL_dmeans3D && dL_dscales && dL_drotations &&
+                    dL_dvalues && dL_dweights,
+                "NULL pointer");
+    return mixture_backward_batched(batch, scales, rotations, scale_modifier, sample_set, params, out_cells,
+                                    out_weights, dL_dout_cells, dL_dout_cell_weights, state, state_bytes,
+                                    record_capacity, workspace, workspace_bytes, dL_dmeans3D, dL_dscales,
+                                    dL_drotations, dL_dvalues, dL_dweights, (hipStream_t)stream);
 }
 
 int gvc_activate_forward(int P, const float* log_scales, const float* rotations, const float* weight_logits,

@@ -29,6 +29,8 @@
 // rasterization/cuda_rasterizer/, DGR/ = diff-gaussian-rasterization/).
 #include <math.h>
 
+#include <vector>
+
 #include "gvc_device.h"
 #include "gvc_internal.h"
 
@@ -74,6 +76,54 @@ __device__ __forceinline__ uint32_t spread10(uint32_t v)
     v = (v | (v << 4)) & 0x030c30c3u;
     v = (v | (v << 2)) & 0x09249249u;
     return v;
+}
+
+// ---------------------------------------------------------------------------
+// batches: V independent volumes (Gaussians, samples, groups, leaves and
+// internal nodes each a contiguous range).  The device table holds V, then
+// five arrays of V+1 prefix offsets.  A single volume is the V = 1 table.
+// ---------------------------------------------------------------------------
+struct BatchView {
+    int V;
+    const int* P_off;  // Gaussians
+    const int* S_off;  // samples (input order is volume-major)
+    const int* G_off;  // 64-point groups
+    const int* L_off;  // BVH leaves
+    const int* N_off;  // internal BVH nodes
+};
+
+inline size_t batch_table_ints(int V) { return 4 + 5 * ((size_t)V + 1); }
+
+inline BatchView batch_view(const void* table, int V)
+{
+    const int* t = reinterpret_cast<const int*>(table) + 4;
+    const size_t n = (size_t)V + 1;
+    return BatchView{ V, t, t + n, t + 2 * n, t + 3 * n, t + 4 * n };
+}
+
+// largest v with off[v] <= x (off ascending, off[0] <= x)
+__device__ __forceinline__ int volume_of(const int* __restrict__ off, int V, int x)
+{
+    int lo = 0, hi = V - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// one volume of P Gaussians and S samples with leaves of L Gaussians
+__global__ void single_table_kernel(int* __restrict__ t, int P, int S, int L)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int NL = P > 0 ? (P + L - 1) / L : 0;
+    const int v[5][2] = { { 0, P }, { 0, S }, { 0, (S + 63) / 64 }, { 0, NL }, { 0, NL > 1 ? NL - 1 : 0 } };
+    t[0] = 1; t[1] = t[2] = t[3] = 0;
+    for (int k = 0; k < 5; ++k) {
+        t[4 + 2 * k] = v[k][0];
+        t[4 + 2 * k + 1] = v[k][1];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -143,20 +193,47 @@ __global__ void sample_codes_kernel(const float* __restrict__ s, int S, uint32_t
     idx[i] = (uint32_t)i;
 }
 
-// Sorted points, padded with NaN points (original index -1) to whole 64-point
-// groups so every group can be read with wave-uniform loads.
-__global__ void sample_gather_kernel(const float* __restrict__ s, int S, int S64, const uint32_t* __restrict__ perm,
-                                     float4* __restrict__ pts)
+// Sorted points (volume-major, Morton order within a volume), each volume
+// padded with NaN points (original index -1) to whole 64-point groups, so every
+// group can be read with wave-uniform loads and belongs to one volume.
+__global__ void sample_gather_kernel(const float* __restrict__ s, BatchView b, int S64,
+                                     const uint32_t* __restrict__ perm, float4* __restrict__ pts)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= S64) return;
-    if (i >= S) {
+    const int v = b.V == 1 ? 0 : volume_of(b.G_off, b.V, i >> 6);
+    const int r = i - 64 * b.G_off[v];
+    if (r >= b.S_off[v + 1] - b.S_off[v]) {
         const float nan = __builtin_nanf("");
         pts[i] = make_float4(nan, nan, nan, __builtin_bit_cast(float, -1));
         return;
     }
-    uint32_t j = perm[i];
+    const uint32_t j = perm[b.S_off[v] + r];
     pts[i] = make_float4(s[3 * j], s[3 * j + 1], s[3 * j + 2], __builtin_bit_cast(float, (int)j));
+}
+
+// Volume of each sorted item (the second, stable sort key of a batch).
+__global__ void volume_keys_kernel(const uint32_t* __restrict__ idx, int n, const int* __restrict__ off, int V,
+                                   uint32_t* __restrict__ vkeys, uint32_t* __restrict__ pos)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    vkeys[i] = (uint32_t)volume_of(off, V, (int)idx[i]);
+    pos[i] = (uint32_t)i;
+}
+
+__global__ void gather_u32_kernel(const uint32_t* __restrict__ src, const uint32_t* __restrict__ pos, int n,
+                                  uint32_t* __restrict__ dst)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[pos[i]];
+}
+
+__global__ void gather_u64_kernel(const uint64_t* __restrict__ src, const uint32_t* __restrict__ pos, int n,
+                                  uint64_t* __restrict__ dst)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[pos[i]];
 }
 
 __global__ void sample_header_kernel(SampleSetHeader* h, int S, int NG)
@@ -179,81 +256,83 @@ __global__ void gaussian_codes_kernel(const float* __restrict__ means, int P, ui
     idx[i] = (uint32_t)i;
 }
 
-// Canonical preprocess, written as records in BVH (sorted) order.  Also sums the
-// box extents per axis in units of 2^-20 per 256-record block (integers: the
-// same totals in any order), the walks' estimate of a Gaussian's reach when
-// splitting groups; ext_reduce_kernel adds the block sums.
+// Canonical preprocess, written as records in BVH (sorted) order.
 __global__ __launch_bounds__(256) void record_kernel(int P, const uint32_t* __restrict__ perm,
                                                      const float* __restrict__ means,
                                                      const float* __restrict__ scales, const float* __restrict__ rots,
                                                      const float* __restrict__ values,
                                                      const float* __restrict__ weights, float k, int fp64_cond,
-                                                     GRecord* __restrict__ rec, float* __restrict__ conics_out,
-                                                     unsigned long long* __restrict__ ext_part)
+                                                     GRecord* __restrict__ rec, float* __restrict__ conics_out)
 {
-    __shared__ unsigned long long wsum[4][3];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t ex[3] = { 0, 0, 0 };
-    if (i < P) {
-        const int j = (int)perm[i];
-        float mu[3] = { means[3 * j], means[3 * j + 1], means[3 * j + 2] };
-        float sc[3] = { scales[3 * j], scales[3 * j + 1], scales[3 * j + 2] };
-        float q[4] = { rots[4 * j], rots[4 * j + 1], rots[4 * j + 2], rots[4 * j + 3] };
-        const float w = weights[j];
-        float cn[6], bx[6];
-        preprocess_one(mu, sc, q, w, k, fp64_cond != 0, cn, bx);
-        GRecord r;
-        r.mx = mu[0]; r.my = mu[1]; r.mz = mu[2]; r.w = w;
-        r.c0 = cn[0]; r.c1 = cn[1]; r.c2 = cn[2]; r.c3 = cn[3]; r.c4 = cn[4]; r.c5 = cn[5];
-        r.v = values[j];
-        r.id = j;
-        r.lox = bx[0]; r.loy = bx[1]; r.loz = bx[2];
-        r.hix = bx[3]; r.hiy = bx[4]; r.hiz = bx[5];
-        r.pad0 = 0.f; r.pad1 = 0.f;
-        rec[i] = r;
-        if (conics_out)
-            for (int c = 0; c < 6; ++c) conics_out[6 * j + c] = cn[c];
-        for (int a = 0; a < 3; ++a) {
-            const float e = bx[3 + a] - bx[a];
-            ex[a] = (e >= 0.0f) ? (uint64_t)(fminf(e, 1024.0f) * 1048576.0f) : 0ull;
-        }
-    }
-    for (int a = 0; a < 3; ++a) {
-        uint64_t v = ex[a];
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) v += (uint64_t)__shfl_xor((long long)v, d, 64);
-        if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6][a] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        const int a = threadIdx.x;
-        ext_part[3 * (size_t)blockIdx.x + a] = wsum[0][a] + wsum[1][a] + wsum[2][a] + wsum[3][a];
-    }
+    if (i >= P) return;
+    const int j = (int)perm[i];
+    float mu[3] = { means[3 * j], means[3 * j + 1], means[3 * j + 2] };
+    float sc[3] = { scales[3 * j], scales[3 * j + 1], scales[3 * j + 2] };
+    float q[4] = { rots[4 * j], rots[4 * j + 1], rots[4 * j + 2], rots[4 * j + 3] };
+    const float w = weights[j];
+    float cn[6], bx[6];
+    preprocess_one(mu, sc, q, w, k, fp64_cond != 0, cn, bx);
+    GRecord r;
+    r.mx = mu[0]; r.my = mu[1]; r.mz = mu[2]; r.w = w;
+    r.c0 = cn[0]; r.c1 = cn[1]; r.c2 = cn[2]; r.c3 = cn[3]; r.c4 = cn[4]; r.c5 = cn[5];
+    r.v = values[j];
+    r.id = j;
+    r.lox = bx[0]; r.loy = bx[1]; r.loz = bx[2];
+    r.hix = bx[3]; r.hiy = bx[4]; r.hiz = bx[5];
+    r.pad0 = 0.f; r.pad1 = 0.f;
+    rec[i] = r;
+    if (conics_out)
+        for (int c = 0; c < 6; ++c) conics_out[6 * j + c] = cn[c];
 }
 
-__global__ void ext_reduce_kernel(const unsigned long long* __restrict__ part, int nb,
-                                  unsigned long long* __restrict__ ext_sum)
+// Per volume, the walks' estimate of a Gaussian's reach when splitting groups:
+// the mean box extent per axis over up to 1024 records taken at a fixed stride,
+// summed in units of 2^-20 (integers: the same total in any order).  ext[4v..]
+// = (sum x, sum y, sum z, count).
+__global__ __launch_bounds__(256) void extent_kernel(const GRecord* __restrict__ rec, BatchView bv,
+                                                     unsigned long long* __restrict__ ext)
 {
-    __shared__ unsigned long long red[256];
-    for (int a = 0; a < 3; ++a) {
-        unsigned long long v = 0;
-        for (int i = threadIdx.x; i < nb; i += 256) v += part[3 * (size_t)i + a];
-        red[threadIdx.x] = v;
+    __shared__ unsigned long long red[3][256];
+    const int v = blockIdx.x;
+    const long long g0 = bv.P_off[v], n = bv.P_off[v + 1] - g0;
+    const long long m = n < 1024 ? n : 1024;
+    unsigned long long sum[3] = { 0, 0, 0 };
+    for (long long kk = threadIdx.x; kk < m; kk += 256) {
+        const GRecord& r = rec[g0 + (kk * n) / m];
+        const float e[3] = { r.hix - r.lox, r.hiy - r.loy, r.hiz - r.loz };
+        for (int a = 0; a < 3; ++a)
+            sum[a] += (e[a] >= 0.0f) ? (unsigned long long)(fminf(e[a], 1024.0f) * 1048576.0f) : 0ull;
+    }
+    for (int a = 0; a < 3; ++a) red[a][threadIdx.x] = sum[a];
+    __syncthreads();
+    for (int d = 128; d >= 1; d >>= 1) {
+        if ((int)threadIdx.x < d)
+            for (int a = 0; a < 3; ++a) red[a][threadIdx.x] += red[a][threadIdx.x + d];
         __syncthreads();
-        for (int d = 128; d >= 1; d >>= 1) {
-            if ((int)threadIdx.x < d) red[threadIdx.x] += red[threadIdx.x + d];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) ext_sum[a] = red[0];
-        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < 3; ++a) ext[4 * (size_t)v + a] = red[a][0];
+        ext[4 * (size_t)v + 3] = (unsigned long long)m;
     }
 }
 
-__global__ void leaf_codes_kernel(const uint64_t* __restrict__ sorted_codes, int NL, int L,
+// First Gaussian of leaf j (leaves never span volumes).
+__device__ __forceinline__ int leaf_first(const BatchView& b, int L, int j, int& end)
+{
+    const int v = b.V == 1 ? 0 : volume_of(b.L_off, b.V, j);
+    const int g0 = b.P_off[v] + (j - b.L_off[v]) * L;
+    end = min(g0 + L, b.P_off[v + 1]);
+    return g0;
+}
+
+__global__ void leaf_codes_kernel(const uint64_t* __restrict__ sorted_codes, BatchView b, int NL, int L,
                                   uint64_t* __restrict__ leaf_codes)
 {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < NL) leaf_codes[j] = sorted_codes[(size_t)j * L];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= NL) return;
+    int end;
+    leaf_codes[j] = sorted_codes[leaf_first(b, L, j, end)];
 }
 
 __device__ __forceinline__ int karras_delta(const uint64_t* k, int n, int i, int j)
@@ -264,11 +343,17 @@ __device__ __forceinline__ int karras_delta(const uint64_t* k, int n, int i, int
     return __clzll(a ^ b);
 }
 
-// Karras (2012): one thread per internal node.  parent codes: (p << 1) | side.
-__global__ void karras_kernel(const uint64_t* __restrict__ keys, int n, BNode* __restrict__ nodes)
+// Karras (2012): one thread per internal node, within its volume's leaves;
+// children and leaf ranges are stored as global indices.
+__global__ void karras_kernel(const uint64_t* __restrict__ all_keys, BatchView bv, int NI, BNode* __restrict__ nodes)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= NI) return;
+    const int v = bv.V == 1 ? 0 : volume_of(bv.N_off, bv.V, gi);
+    const int lb = bv.L_off[v], nb = bv.N_off[v];
+    const int n = bv.L_off[v + 1] - lb;
+    const uint64_t* keys = all_keys + lb;
+    const int i = gi - nb;
     const int d = (karras_delta(keys, n, i, i + 1) - karras_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
     const int dmin = karras_delta(keys, n, i, i - d);
     int lmax = 2;
@@ -285,12 +370,12 @@ __global__ void karras_kernel(const uint64_t* __restrict__ keys, int n, BNode* _
     } while (t > 1);
     const int gamma = i + s * d + (d < 0 ? d : 0);
     const int lo = i < j ? i : j, hi = i < j ? j : i;
-    const int left = (lo == gamma) ? ~gamma : gamma;
-    const int right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
-    nodes[i].left = left;
-    nodes[i].right = right;
-    nodes[i].pad0 = lo;
-    nodes[i].pad1 = hi;
+    const int left = (lo == gamma) ? ~(lb + gamma) : nb + gamma;
+    const int right = (hi == gamma + 1) ? ~(lb + gamma + 1) : nb + gamma + 1;
+    nodes[gi].left = left;
+    nodes[gi].right = right;
+    nodes[gi].pad0 = lb + lo;
+    nodes[gi].pad1 = lb + hi;
 }
 
 // Node boxes.  A node of the radix tree covers the leaf range [first, last]; its
@@ -312,7 +397,7 @@ __device__ __forceinline__ Box6 box_union(const Box6& a, const Box6& b)
     return r;
 }
 
-__global__ void leaf_box_kernel(const GRecord* __restrict__ rec, int P, int NL, int L, Box6* __restrict__ t0)
+__global__ void leaf_box_kernel(const GRecord* __restrict__ rec, BatchView bv, int NL, int L, Box6* __restrict__ t0)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= NL) return;
@@ -322,7 +407,8 @@ __global__ void leaf_box_kernel(const GRecord* __restrict__ rec, int P, int NL, 
         b.hi[c] = -__builtin_inff();
     }
     bool any = false;
-    const int g0 = j * L, g1 = min(g0 + L, P);
+    int g1;
+    const int g0 = leaf_first(bv, L, j, g1);
     for (int g = g0; g < g1; ++g) {
         const GRecord& r = rec[g];
         b.lo[0] = fminf(b.lo[0], r.lox); b.lo[1] = fminf(b.lo[1], r.loy); b.lo[2] = fminf(b.lo[2], r.loz);
@@ -349,14 +435,16 @@ __device__ __forceinline__ Box6 range_box(const Box6* __restrict__ table, int NL
     return box_union(tk[a], tk[b - (1 << k) + 1]);
 }
 
-__global__ void node_box_kernel(const Box6* __restrict__ table, int NL, BNode* __restrict__ nodes,
-                                float* __restrict__ root_box)
+__global__ void node_box_kernel(const Box6* __restrict__ table, BatchView bv, int NL, int NI,
+                                BNode* __restrict__ nodes, float* __restrict__ root_box)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= NL - 1) return;
+    if (i >= NI) return;
+    const int v = bv.V == 1 ? 0 : volume_of(bv.N_off, bv.V, i);
     BNode& n = nodes[i];
     const int lo = n.pad0, hi = n.pad1, left = n.left;
-    const int gamma = left < 0 ? ~left : left;   // the left child covers [lo, gamma]
+    // the left child covers [lo, gamma]; an internal left child has local index gamma
+    const int gamma = left < 0 ? ~left : bv.L_off[v] + (left - bv.N_off[v]);
     const Box6 l = range_box(table, NL, lo, gamma);
     const Box6 r = range_box(table, NL, gamma + 1, hi);
     for (int c = 0; c < 3; ++c) {
@@ -365,22 +453,27 @@ __global__ void node_box_kernel(const Box6* __restrict__ table, int NL, BNode* _
         n.rlo[c] = r.lo[c];
         n.rhi[c] = r.hi[c];
     }
-    if (i == 0) {
+    if (i == bv.N_off[v]) {
         const Box6 u = box_union(l, r);
         for (int c = 0; c < 3; ++c) {
-            root_box[c] = u.lo[c];
-            root_box[3 + c] = u.hi[c];
+            root_box[6 * v + c] = u.lo[c];
+            root_box[6 * v + 3 + c] = u.hi[c];
         }
     }
 }
 
-__global__ void single_leaf_root_kernel(const Box6* __restrict__ t0, float* __restrict__ root_box)
+// Root boxes of volumes with a single leaf (no internal node); empty volumes get
+// an empty box.
+__global__ void leaf_root_kernel(const Box6* __restrict__ t0, BatchView bv, float* __restrict__ root_box)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0)
-        for (int c = 0; c < 3; ++c) {
-            root_box[c] = t0[0].lo[c];
-            root_box[3 + c] = t0[0].hi[c];
-        }
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= bv.V) return;
+    const int nl = bv.L_off[v + 1] - bv.L_off[v];
+    if (nl > 1) return;
+    for (int c = 0; c < 3; ++c) {
+        root_box[6 * v + c] = nl == 1 ? t0[bv.L_off[v]].lo[c] : 1.0f;
+        root_box[6 * v + 3 + c] = nl == 1 ? t0[bv.L_off[v]].hi[c] : 0.0f;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -528,10 +621,11 @@ __device__ __forceinline__ int next_group(uint32_t* __restrict__ queue, int NG, 
     return NG;
 }
 
-__device__ __forceinline__ void mean_extent(const unsigned long long* __restrict__ ext_sum, int P, float r[3])
+__device__ __forceinline__ void mean_extent(const unsigned long long* __restrict__ ext, float r[3])
 {
+    const unsigned long long m = ext[3];
     for (int a = 0; a < 3; ++a)
-        r[a] = uniformf((float)((double)ext_sum[a] * (1.0 / 1048576.0) / (double)(P > 0 ? P : 1)));
+        r[a] = uniformf((float)((double)ext[a] * (1.0 / 1048576.0) / (double)(m > 0 ? m : 1)));
 }
 
 // Candidate enumeration.  chunk(base, n) is called with the ring positions
@@ -539,11 +633,32 @@ __device__ __forceinline__ void mean_extent(const unsigned long long* __restrict
 // one loop whose every pass does exactly one of: hand a chunk over, expand 64
 // leaf Gaussians into the ring, or take one traversal step; so `chunk` has a
 // single call site and is inlined once.
+// One volume's tree: Gaussians [g_lo, g_hi), leaves [leaf_base, leaf_base + NL)
+// of L Gaussians, root node `root` (NL > 1) and its box.
+struct TreeView {
+    const float* root_box;
+    int g_lo, g_hi, leaf_base, NL, root;
+};
+
+__device__ __forceinline__ TreeView tree_of(const BatchView& b, const float* __restrict__ root_boxes, int v)
+{
+    TreeView t;
+    t.root_box = root_boxes + 6 * v;
+    t.g_lo = b.P_off[v];
+    t.g_hi = b.P_off[v + 1];
+    t.leaf_base = b.L_off[v];
+    t.NL = b.L_off[v + 1] - b.L_off[v];
+    t.root = b.N_off[v];
+    return t;
+}
+
 template <class ChunkFn>
 __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__ rec, const BNode* __restrict__ nodes,
-                                                     const float* __restrict__ root_box, int P, int NL, int L,
-                                                     const GroupBox& gb, WalkLds& s, ChunkFn&& chunk)
+                                                     const TreeView& tv, int L, const GroupBox& gb, WalkLds& s,
+                                                     ChunkFn&& chunk)
 {
+    const float* __restrict__ root_box = tv.root_box;
+    const int NL = tv.NL;
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lanemask_lt();
     uint32_t chead = 0, ctail = 0;
@@ -554,11 +669,11 @@ __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__
     if (NL <= 0) return true;
     if (NL == 1) {
         if (box_overlaps(root_box[0], root_box[1], root_box[2], root_box[3], root_box[4], root_box[5], gb)) {
-            if (lane == 0) s.leaves[0] = 0;
+            if (lane == 0) s.leaves[0] = tv.leaf_base;
             nl = 1;
         }
     } else {
-        if (lane == 0) s.wstack[0] = 0;
+        if (lane == 0) s.wstack[0] = tv.root;
         sp = 1;
     }
     lds_fence();
@@ -579,8 +694,8 @@ __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__
             int g = 0;
             if (t < nl * L) {
                 const int q = t / L;
-                g = s.leaves[q] * L + (t - q * L);
-                if (g < P) {
+                g = tv.g_lo + (s.leaves[q] - tv.leaf_base) * L + (t - q * L);
+                if (g < tv.g_hi) {
                     const float4 b0 = *reinterpret_cast<const float4*>(&rec[g].lox);
                     const float2 b1 = *reinterpret_cast<const float2*>(&rec[g].hiy);
                     hit = box_overlaps(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, gb);
@@ -714,10 +829,7 @@ __device__ __forceinline__ uint64_t wave_transpose64(uint64_t a)
         if (lane & 32) lo = phi;
         else hi = plo;
     }
-#pragma unroll
-    for (int j = 16; j >= 1; j >>= 1) {
-        const uint32_t m = j == 16 ? 0x0000ffffu : j == 8 ? 0x00ff00ffu : j == 4 ? 0x0f0f0f0fu : j == 2 ? 0x33333333u
-                                                                                                         : 0x55555555u;
+    auto stage = [&](const int j, const uint32_t m) {
         const uint32_t plo = (uint32_t)__shfl_xor((int)lo, j, 64), phi = (uint32_t)__shfl_xor((int)hi, j, 64);
         if (lane & j) {
             lo = (lo & ~m) | ((plo & ~m) >> j);
@@ -726,7 +838,12 @@ __device__ __forceinline__ uint64_t wave_transpose64(uint64_t a)
             lo = (lo & m) | ((plo & m) << j);
             hi = (hi & m) | ((phi & m) << j);
         }
-    }
+    };
+    stage(16, 0x0000ffffu);
+    stage(8, 0x00ff00ffu);
+    stage(4, 0x0f0f0f0fu);
+    stage(2, 0x33333333u);
+    stage(1, 0x55555555u);
     return ((uint64_t)hi << 32) | lo;
 }
 
@@ -835,8 +952,8 @@ __device__ __forceinline__ uint32_t wave_or32(uint32_t v)
 
 template <bool KAHAN>
 __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict__ rec, const BNode* __restrict__ nodes,
-                                                      const float* __restrict__ root_box, int P, int NL, int L,
-                                                      const float4* __restrict__ pts, int S, int NG,
+                                                      const float* __restrict__ root_box, BatchView bv, int P, int L,
+                                                      const float4* __restrict__ pts, int NG,
                                                       float* __restrict__ out_cells, float* __restrict__ out_weights,
                                                       uint32_t* __restrict__ group_records,
                                                       uint32_t* __restrict__ group_pairs, uint32_t* __restrict__ err,
@@ -849,14 +966,16 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
     extern __shared__ float4 lds_dyn[];
     const int wid = uniformi(threadIdx.x >> 6), lane = threadIdx.x & 63;
     FwdLds& s = reinterpret_cast<FwdLds*>(lds_dyn)[wid];
-    float reach[3];
-    mean_extent(ext_sum, P, reach);
     int x = (int)(blockIdx.x & 7u), tries = 0;
     bool ok = true;
     LogCursor cur;
     while (true) {
         const int g = next_group(queue, NG, x, tries);
         if (g >= NG) break;
+        const int vol = bv.V == 1 ? 0 : uniformi(volume_of(bv.G_off, bv.V, g));
+        const TreeView tv = tree_of(bv, root_box, vol);
+        float reach[3];
+        mean_extent(ext_sum + 4 * vol, reach);
         const float4* __restrict__ gp = pts + (size_t)g * 64;
         const float4 pme = gp[lane];
         const float px = pme.x, py = pme.y, pz = pme.z;
@@ -958,7 +1077,7 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
             const int b = rs ? __builtin_ctzll(rs) : nv;
             runmask = (b >= 64 ? ~0ull : ((1ull << b) - 1ull)) & ~((1ull << a) - 1ull);
             const GroupBox gb = masked_box(px, py, pz, lane >= a && lane < b);
-            ok = enumerate_candidates(rec, nodes, root_box, P, NL, L, gb, s.walk, chunk) && ok;
+            ok = enumerate_candidates(rec, nodes, tv, L, gb, s.walk, chunk) && ok;
         }
         if (orig >= 0) {
             float V, Wo;
@@ -1060,8 +1179,8 @@ __device__ __forceinline__ uint64_t drop_low_bits(uint64_t m, uint32_t k)
 // forward did, so nothing but per-group counts is kept between the passes.
 template <bool REPLAY>
 __global__ __launch_bounds__(1024) void backward_kernel(
-    const GRecord* __restrict__ rec, const BNode* __restrict__ nodes, const float* __restrict__ root_box, int P,
-    int NL, int L, const float4* __restrict__ pts, int NG, const float* __restrict__ out_cells,
+    const GRecord* __restrict__ rec, const BNode* __restrict__ nodes, const float* __restrict__ root_box, BatchView bv,
+    int P, int L, const float4* __restrict__ pts, int NG, const float* __restrict__ out_cells,
     const float* __restrict__ out_weights, const float* __restrict__ gcells, const float* __restrict__ gweights,
     const uint32_t* __restrict__ group_offsets, uint64_t capacity, PartialRecord* __restrict__ out_rec,
     uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ err, uint32_t* __restrict__ queue,
@@ -1072,8 +1191,6 @@ __global__ __launch_bounds__(1024) void backward_kernel(
     const int wid = uniformi(threadIdx.x >> 6), lane = threadIdx.x & 63;
     BwdLds<REPLAY>& s = reinterpret_cast<BwdLds<REPLAY>*>(lds_dyn)[wid];
     const uint64_t lt = lanemask_lt();
-    float reach[3] = { 0.f, 0.f, 0.f };
-    if (!REPLAY) mean_extent(ext_sum, P, reach);
     int x = (int)(blockIdx.x & 7u), tries = 0;
     bool ok = true;
     while (true) {
@@ -1244,6 +1361,9 @@ __global__ __launch_bounds__(1024) void backward_kernel(
         } else {
             // walk the group's runs again and box-test each chunk (lane = candidate)
             const int nv = __popcll(__ballot(orig >= 0));
+            const int vol = bv.V == 1 ? 0 : uniformi(volume_of(bv.G_off, bv.V, g));
+            float reach[3];
+            mean_extent(ext_sum + 4 * vol, reach);
             const uint64_t starts = split_runs(pme.x, pme.y, pme.z, nv, reach, run_points);
             const uint64_t finite = build_axes(pme.x, pme.y, pme.z, orig >= 0, s.axes);
             lds_fence();
@@ -1268,7 +1388,8 @@ __global__ __launch_bounds__(1024) void backward_kernel(
                 const int b = rs ? __builtin_ctzll(rs) : nv;
                 runmask = (b >= 64 ? ~0ull : ((1ull << b) - 1ull)) & ~((1ull << a) - 1ull);
                 const GroupBox gb = masked_box(pme.x, pme.y, pme.z, lane >= a && lane < b);
-                ok = enumerate_candidates(rec, nodes, root_box, P, NL, L, gb, s.walk, chunk) && ok;
+                const TreeView tv = tree_of(bv, root_box, vol);
+                ok = enumerate_candidates(rec, nodes, tv, L, gb, s.walk, chunk) && ok;
             }
         }
     }
@@ -1479,50 +1600,76 @@ inline int key_bits_for(uint64_t maxv)
 // ---------------------------------------------------------------------------
 // workspace layouts
 // ---------------------------------------------------------------------------
+// Host-side totals of a batch and its device table.
+struct Batch {
+    int V, P, S, NG, NL, NI, L;
+    const void* table;
+};
+
+Batch single_batch(int P, int S, int L, const void* table)
+{
+    Batch b;
+    b.V = 1; b.P = P; b.S = S; b.L = L;
+    b.NG = (S + 63) / 64;
+    b.NL = P > 0 ? (P + L - 1) / L : 0;
+    b.NI = b.NL > 1 ? b.NL - 1 : 0;
+    b.table = table;
+    return b;
+}
+
 struct SampleSetLayout {
     SampleSetHeader* header;
+    int* table;  // the set's own one-volume table
     float4* pts;
-    uint32_t *keys, *idx, *keys_alt, *idx_alt, *scratch;
-    SampleSetLayout(Arena& a, int S)
+    uint32_t *keys, *idx, *keys_alt, *idx_alt, *vkeys, *pos, *scratch;
+    SampleSetLayout(Arena& a, int S, int NG)
     {
         header = a.take<SampleSetHeader>(1);
-        pts = a.take<float4>(((size_t)S + 63) / 64 * 64);
+        table = a.take<int>(batch_table_ints(1));
+        pts = a.take<float4>((size_t)NG * 64);
         keys = a.take<uint32_t>((size_t)S);
         idx = a.take<uint32_t>((size_t)S);
         keys_alt = a.take<uint32_t>((size_t)S);
         idx_alt = a.take<uint32_t>((size_t)S);
+        vkeys = a.take<uint32_t>((size_t)S);
+        pos = a.take<uint32_t>((size_t)S);
         scratch = a.take<uint32_t>(radix_scratch_elems((size_t)S));
     }
 };
 
 struct ForwardLayout {
+    int* table;  // the state's own one-volume table
     GRecord* rec;
     uint64_t *codes, *codes_alt, *leaf_codes;
-    uint32_t *idx, *idx_alt, *sort_scratch;
+    uint32_t *idx, *idx_alt, *vkeys, *pos, *sort_scratch;
     BNode* nodes;
     Box6* sparse;  // levels x NL leaf-range boxes
-    float* root_box;
+    float* root_box;  // 6 per volume
     uint32_t *group_records, *group_offsets, *group_pairs, *scan_scratch, *err;
     uint32_t* queues;  // 8 forward + 8 backward group queues, log counter
-    unsigned long long *stats, *ext_sum, *ext_part;
+    unsigned long long *stats, *ext_sum;
     uint32_t* group_head;
     unsigned char* log;
     uint32_t log_cap;
     int NL, NG;
-    ForwardLayout(Arena& a, int P, int S, int L, unsigned flags, unsigned chunk_slots)
+    ForwardLayout(Arena& a, const Batch& b, unsigned flags, unsigned chunk_slots)
     {
-        NL = P > 0 ? (P + L - 1) / L : 0;
-        NG = (S + 63) / 64;
+        const int P = b.P;
+        NL = b.NL;
+        NG = b.NG;
+        table = a.take<int>(batch_table_ints(1));
         rec = a.take<GRecord>((size_t)P);
         codes = a.take<uint64_t>((size_t)P);
         codes_alt = a.take<uint64_t>((size_t)P);
         idx = a.take<uint32_t>((size_t)P);
         idx_alt = a.take<uint32_t>((size_t)P);
+        vkeys = a.take<uint32_t>((size_t)P);
+        pos = a.take<uint32_t>((size_t)P);
         sort_scratch = a.take<uint32_t>(radix_scratch_elems((size_t)P));
         leaf_codes = a.take<uint64_t>((size_t)NL);
-        nodes = a.take<BNode>((size_t)(NL > 1 ? NL - 1 : 1));
+        nodes = a.take<BNode>((size_t)(b.NI > 0 ? b.NI : 1));
         sparse = a.take<Box6>((size_t)sparse_levels(NL) * (size_t)(NL > 0 ? NL : 1));
-        root_box = a.take<float>(8);
+        root_box = a.take<float>(6 * (size_t)b.V + 2);
         group_records = a.take<uint32_t>((size_t)NG + 1);
         group_offsets = a.take<uint32_t>((size_t)NG + 1);
         group_pairs = a.take<uint32_t>((size_t)NG + 1);
@@ -1530,8 +1677,7 @@ struct ForwardLayout {
         err = a.take<uint32_t>(4);
         queues = a.take<uint32_t>(20);
         stats = a.take<unsigned long long>(4);
-        ext_sum = a.take<unsigned long long>(4);
-        ext_part = a.take<unsigned long long>(3 * (((size_t)P + 255) / 256) + 3);
+        ext_sum = a.take<unsigned long long>(4 * (size_t)b.V);
         group_head = a.take<uint32_t>((size_t)NG + 1);
         const uint64_t cap = (flags & GVC_FLAG_CANDIDATE_LOG)
                                  ? (uint64_t)(chunk_slots ? chunk_slots : 16u) * (uint64_t)(NG > 0 ? NG : 1)
@@ -1570,49 +1716,46 @@ int check_params(const gvc_params* prm)
     return GVC_OK;
 }
 
-}  // namespace
+// The table a call uses: the batch's, or for one volume the buffer's own.
+const void* table_of(const Batch& b, const int* own) { return b.V == 1 && !b.table ? own : b.table; }
 
-// ---------------------------------------------------------------------------
-// host entry points (called from capi.cpp)
-// ---------------------------------------------------------------------------
-int preprocess_launch(int P, const float* means, const float* scales, const float* rots, const float* weights,
-                      float k, unsigned flags, float* conics, float* boxes, hipStream_t st)
-{
-    if (P <= 0) return GVC_OK;
-    preprocess_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, means, scales, rots, weights, k,
-                                                          (flags & GVC_FLAG_FP64_COND) ? 1 : 0, conics, boxes);
-    GVC_LAUNCH_CHECK("preprocess");
-    return GVC_OK;
-}
-
-int morton_launch(int n, const float* xyz, int clamp, uint64_t* codes, hipStream_t st)
-{
-    if (n <= 0) return GVC_OK;
-    morton63_kernel<<<blocks_for(n, 256), 256, 0, st>>>(xyz, n, clamp, codes);
-    GVC_LAUNCH_CHECK("morton");
-    return GVC_OK;
-}
-
-size_t sample_set_bytes(int S)
+size_t sample_set_bytes_impl(int S, int NG)
 {
     Arena a(nullptr, 0);
-    SampleSetLayout l(a, S);
+    SampleSetLayout l(a, S, NG);
     return a.off + 256;
 }
 
-int sample_set_build(int S, const float* samples, void* set, size_t bytes, hipStream_t st)
+int sample_set_build_impl(const Batch& bt, const float* samples, void* set, size_t bytes, hipStream_t st)
 {
+    const int S = bt.S, NG = bt.NG;
     Arena a(set, bytes);
-    SampleSetLayout l(a, S);
+    SampleSetLayout l(a, S, NG);
     if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "sample set buffer too small");
-    const int NG = (S + 63) / 64;
+    if (!bt.table) {
+        single_table_kernel<<<1, 64, 0, st>>>(l.table, 0, S, 1);
+        GVC_LAUNCH_CHECK("sample_table");
+    }
+    const BatchView bv = batch_view(table_of(bt, l.table), bt.V);
     if (S > 0) {
         sample_codes_kernel<<<blocks_for(S, 256), 256, 0, st>>>(samples, S, l.keys, l.idx);
         GVC_LAUNCH_CHECK("sample_codes");
         bool alt = false;
         GVC_TRY(radix_sort_u32(l.keys, l.idx, l.keys_alt, l.idx_alt, (size_t)S, 30, l.scratch, &alt, st));
-        const int S64 = NG * 64;
-        sample_gather_kernel<<<blocks_for(S64, 256), 256, 0, st>>>(samples, S, S64, alt ? l.idx_alt : l.idx, l.pts);
+        const uint32_t* perm = alt ? l.idx_alt : l.idx;
+        if (bt.V > 1) {
+            // stable second key: the volume, so each volume's samples are contiguous
+            uint32_t* spare = alt ? l.idx : l.idx_alt;
+            volume_keys_kernel<<<blocks_for(S, 256), 256, 0, st>>>(perm, S, bv.S_off, bt.V, l.vkeys, l.pos);
+            GVC_LAUNCH_CHECK("volume_keys");
+            bool alt2 = false;
+            GVC_TRY(radix_sort_u32(l.vkeys, l.pos, l.keys, l.keys_alt, (size_t)S, key_bits_for((uint64_t)bt.V),
+                                   l.scratch, &alt2, st));
+            gather_u32_kernel<<<blocks_for(S, 256), 256, 0, st>>>(perm, alt2 ? l.keys_alt : l.pos, S, spare);
+            GVC_LAUNCH_CHECK("gather");
+            perm = spare;
+        }
+        sample_gather_kernel<<<blocks_for((size_t)NG * 64, 256), 256, 0, st>>>(samples, bv, NG * 64, perm, l.pts);
         GVC_LAUNCH_CHECK("sample_gather");
     }
     sample_header_kernel<<<1, 1, 0, st>>>(l.header, S, NG);
@@ -1620,37 +1763,45 @@ int sample_set_build(int S, const float* samples, void* set, size_t bytes, hipSt
     return GVC_OK;
 }
 
-const float4* sample_set_points(const void* sample_set, int S)
+const float4* sample_set_points_impl(const void* sample_set, int S, int NG)
 {
-    Arena a(const_cast<void*>(sample_set), sample_set_bytes(S));
-    SampleSetLayout l(a, S);
+    Arena a(const_cast<void*>(sample_set), sample_set_bytes_impl(S, NG));
+    SampleSetLayout l(a, S, NG);
     return l.pts;
 }
 
-size_t forward_state_bytes(int P, int S, const gvc_params* prm)
+size_t forward_state_bytes_impl(const Batch& b, const gvc_params* prm)
 {
     if (check_params(prm) != GVC_OK) return 0;
     Arena a(nullptr, 0);
-    ForwardLayout l(a, P, S, prm->leaf_size, prm->flags, prm->chunk_slots);
+    ForwardLayout l(a, b, prm->flags, prm->chunk_slots);
     return a.off + 256;
 }
 
-int mixture_forward(int P, const float* means, const float* scales, const float* rots, const float* values,
-                    const float* weights, float k, const void* sample_set, int S, const gvc_params* prm,
-                    float* out_cells, float* out_weights, float* conics, void* state, size_t state_bytes,
-                    hipStream_t st)
+int mixture_forward_impl(const Batch& bt, const float* means, const float* scales, const float* rots,
+                         const float* values, const float* weights, float k, const void* sample_set,
+                         const gvc_params* prm, float* out_cells, float* out_weights, float* conics, void* state,
+                         size_t state_bytes, hipStream_t st)
 {
     GVC_TRY(check_params(prm));
+    const int P = bt.P, S = bt.S, L = prm->leaf_size;
     if (P < 0 || S < 0) return set_error(GVC_ERR_INVALID_ARGUMENT, "negative size");
-    const int L = prm->leaf_size;
+    if (L != bt.L) return set_error(GVC_ERR_INVALID_ARGUMENT, "params.leaf_size differs from the batch's");
     Arena a(state, state_bytes);
-    ForwardLayout l(a, P, S, L, prm->flags, prm->chunk_slots);
+    ForwardLayout l(a, bt, prm->flags, prm->chunk_slots);
     if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "forward state buffer too small");
-    Arena sa(const_cast<void*>(sample_set), sample_set_bytes(S));
-    SampleSetLayout sl(sa, S);
+    const float4* pts = sample_set_points_impl(sample_set, S, bt.NG);
     const int NG = l.NG, NL = l.NL;
+    const unsigned stages = prm->stages ? (unsigned)prm->stages : 7u;
 
-    GVC_TRY(fill_u32(l.err, 0u, 4, st));
+    if (stages & GVC_STAGE_BUILD) {
+        GVC_TRY(fill_u32(l.err, 0u, 4, st));
+        if (!bt.table) {
+            single_table_kernel<<<1, 64, 0, st>>>(l.table, P, S, L);
+            GVC_LAUNCH_CHECK("table");
+        }
+    }
+    const BatchView bv = batch_view(table_of(bt, l.table), bt.V);
     if (P == 0) {
         // DGR/rasterize_points.cu:51: the reference skips the kernels, leaving zeros.
         if (S > 0) {
@@ -1664,48 +1815,63 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
         return GVC_OK;
     }
     const int fp64 = (prm->flags & GVC_FLAG_FP64_COND) ? 1 : 0;
-    const unsigned stages = prm->stages ? (unsigned)prm->stages : 7u;
     if (stages & GVC_STAGE_BUILD) {
-    // 1. Morton order of the Gaussians
-    gaussian_codes_kernel<<<blocks_for(P, 256), 256, 0, st>>>(means, P, l.codes, l.idx);
-    GVC_LAUNCH_CHECK("gaussian_codes");
-    bool alt = false;
-    GVC_TRY(radix_sort_u64(l.codes, l.idx, l.codes_alt, l.idx_alt, (size_t)P, 64, l.sort_scratch, &alt, st));
-    const uint64_t* scodes = alt ? l.codes_alt : l.codes;
-    const uint32_t* perm = alt ? l.idx_alt : l.idx;
-    // 2. records
-    record_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, perm, means, scales, rots, values, weights, k, fp64, l.rec,
-                                                      conics, l.ext_part);
-    GVC_LAUNCH_CHECK("record");
-    ext_reduce_kernel<<<1, 256, 0, st>>>(l.ext_part, (int)blocks_for(P, 256), l.ext_sum);
-    GVC_LAUNCH_CHECK("ext_reduce");
-    // 3. tree
-    leaf_codes_kernel<<<blocks_for(NL, 256), 256, 0, st>>>(scodes, NL, L, l.leaf_codes);
-    GVC_LAUNCH_CHECK("leaf_codes");
-    if (NL > 1) {
-        karras_kernel<<<blocks_for(NL - 1, 256), 256, 0, st>>>(l.leaf_codes, NL, l.nodes);
-        GVC_LAUNCH_CHECK("karras");
-    }
-    // 4. boxes: leaf boxes, sparse table of leaf ranges, node boxes
-    leaf_box_kernel<<<blocks_for(NL, 256), 256, 0, st>>>(l.rec, P, NL, L, l.sparse);
-    GVC_LAUNCH_CHECK("leaf_box");
-    if (NL > 1) {
-        const int levels = sparse_levels(NL);
-        for (int kk = 0; kk + 1 < levels; ++kk) {
-            const int half = 1 << kk;
-            const int n_out = NL - 2 * half + 1;
-            sparse_level_kernel<<<blocks_for((size_t)n_out, 256), 256, 0, st>>>(
-                l.sparse + (size_t)kk * NL, l.sparse + (size_t)(kk + 1) * NL, NL, half);
-            GVC_LAUNCH_CHECK("sparse_level");
+        // 1. Morton order of the Gaussians (within each volume)
+        gaussian_codes_kernel<<<blocks_for(P, 256), 256, 0, st>>>(means, P, l.codes, l.idx);
+        GVC_LAUNCH_CHECK("gaussian_codes");
+        bool alt = false;
+        GVC_TRY(radix_sort_u64(l.codes, l.idx, l.codes_alt, l.idx_alt, (size_t)P, 64, l.sort_scratch, &alt, st));
+        const uint64_t* scodes = alt ? l.codes_alt : l.codes;
+        const uint32_t* perm = alt ? l.idx_alt : l.idx;
+        if (bt.V > 1) {
+            // stable second key: the volume (codes2 and perm2 are free here)
+            uint64_t* codes2 = alt ? l.codes : l.codes_alt;
+            uint32_t* perm2 = alt ? l.idx : l.idx_alt;
+            volume_keys_kernel<<<blocks_for(P, 256), 256, 0, st>>>(perm, P, bv.P_off, bt.V, l.vkeys, l.pos);
+            GVC_LAUNCH_CHECK("volume_keys");
+            bool alt2 = false;
+            uint32_t* spare = reinterpret_cast<uint32_t*>(codes2);
+            GVC_TRY(radix_sort_u32(l.vkeys, l.pos, perm2, spare, (size_t)P, key_bits_for((uint64_t)bt.V),
+                                   l.sort_scratch, &alt2, st));
+            if (alt2) GVC_HIP(hipMemcpyAsync(l.pos, spare, (size_t)P * 4, hipMemcpyDeviceToDevice, st));
+            gather_u32_kernel<<<blocks_for(P, 256), 256, 0, st>>>(perm, l.pos, P, perm2);
+            gather_u64_kernel<<<blocks_for(P, 256), 256, 0, st>>>(scodes, l.pos, P, codes2);
+            GVC_LAUNCH_CHECK("gather");
+            scodes = codes2;
+            perm = perm2;
         }
-        node_box_kernel<<<blocks_for(NL - 1, 256), 256, 0, st>>>(l.sparse, NL, l.nodes, l.root_box);
-        GVC_LAUNCH_CHECK("node_box");
-    } else {
-        single_leaf_root_kernel<<<1, 64, 0, st>>>(l.sparse, l.root_box);
+        // 2. records
+        record_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, perm, means, scales, rots, values, weights, k, fp64,
+                                                          l.rec, conics);
+        GVC_LAUNCH_CHECK("record");
+        extent_kernel<<<bt.V, 256, 0, st>>>(l.rec, bv, l.ext_sum);
+        GVC_LAUNCH_CHECK("extent");
+        // 3. trees, one per volume
+        leaf_codes_kernel<<<blocks_for(NL, 256), 256, 0, st>>>(scodes, bv, NL, L, l.leaf_codes);
+        GVC_LAUNCH_CHECK("leaf_codes");
+        if (bt.NI > 0) {
+            karras_kernel<<<blocks_for(bt.NI, 256), 256, 0, st>>>(l.leaf_codes, bv, bt.NI, l.nodes);
+            GVC_LAUNCH_CHECK("karras");
+        }
+        // 4. boxes: leaf boxes, sparse table of leaf ranges, node boxes
+        leaf_box_kernel<<<blocks_for(NL, 256), 256, 0, st>>>(l.rec, bv, NL, L, l.sparse);
+        GVC_LAUNCH_CHECK("leaf_box");
+        if (bt.NI > 0) {
+            const int levels = sparse_levels(NL);
+            for (int kk = 0; kk + 1 < levels; ++kk) {
+                const int half = 1 << kk;
+                const int n_out = NL - 2 * half + 1;
+                sparse_level_kernel<<<blocks_for((size_t)n_out, 256), 256, 0, st>>>(
+                    l.sparse + (size_t)kk * NL, l.sparse + (size_t)(kk + 1) * NL, NL, half);
+                GVC_LAUNCH_CHECK("sparse_level");
+            }
+            node_box_kernel<<<blocks_for(bt.NI, 256), 256, 0, st>>>(l.sparse, bv, NL, bt.NI, l.nodes, l.root_box);
+            GVC_LAUNCH_CHECK("node_box");
+        }
+        leaf_root_kernel<<<blocks_for(bt.V, 64), 64, 0, st>>>(l.sparse, bv, l.root_box);
         GVC_LAUNCH_CHECK("root_box");
     }
-    }
-    // 4. walk
+    // 5. walk
     if ((stages & GVC_STAGE_WALK) && NG > 0) {
         const int T = prm->block_threads;
         GVC_TRY(fill_u32(l.queues, 0u, 8, st));
@@ -1717,7 +1883,7 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
             const void* kf = reinterpret_cast<const void*>(&forward_kernel<true>);
             GVC_TRY(allow_lds(kf, lds));
             const unsigned nb = walk_blocks(kf, T, lds, NG);
-            forward_kernel<true><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells,
+            forward_kernel<true><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
                                                      out_weights, l.group_records, l.group_pairs, l.err, l.queues,
                                                      l.ext_sum, log_on, l.log, l.log_cap, l.queues + 16,
                                                      l.group_head, rp);
@@ -1725,25 +1891,25 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
             const void* kf = reinterpret_cast<const void*>(&forward_kernel<false>);
             GVC_TRY(allow_lds(kf, lds));
             const unsigned nb = walk_blocks(kf, T, lds, NG);
-            forward_kernel<false><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, S, NG, out_cells,
+            forward_kernel<false><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
                                                       out_weights, l.group_records, l.group_pairs, l.err, l.queues,
                                                       l.ext_sum, log_on, l.log, l.log_cap, l.queues + 16,
                                                       l.group_head, rp);
         }
         GVC_LAUNCH_CHECK("forward");
     }
-    // 5. record offsets for the backward
+    // 6. record offsets for the backward
     if (stages & GVC_STAGE_COUNTS)
         GVC_TRY(exclusive_scan_u32(l.group_records, l.group_offsets, (size_t)NG, true, l.scan_scratch, st));
     return GVC_OK;
 }
 
-int forward_stats(const void* state, int P, int S, const gvc_params* prm, uint64_t* n_records, uint64_t* n_pairs,
-                  uint32_t* error_flags, uint64_t* chunk_slots_used, hipStream_t st)
+int forward_stats_impl(const Batch& bt, const void* state, const gvc_params* prm, uint64_t* n_records,
+                       uint64_t* n_pairs, uint32_t* error_flags, uint64_t* chunk_slots_used, hipStream_t st)
 {
     GVC_TRY(check_params(prm));
     Arena a(const_cast<void*>(state), (size_t)-1);
-    ForwardLayout l(a, P, S, prm->leaf_size, prm->flags, prm->chunk_slots);
+    ForwardLayout l(a, bt, prm->flags, prm->chunk_slots);
     stats_kernel<<<1, 256, 0, st>>>(l.group_pairs, l.NG, l.group_offsets, l.stats);
     GVC_LAUNCH_CHECK("stats");
     unsigned long long h[2] = { 0, 0 };
@@ -1759,44 +1925,37 @@ int forward_stats(const void* state, int P, int S, const gvc_params* prm, uint64
     return GVC_OK;
 }
 
-int forward_accumulate(const void* state, int P, int S, const gvc_params* prm, unsigned long long* accum,
-                       hipStream_t st)
+int forward_accumulate_impl(const Batch& bt, const void* state, const gvc_params* prm, unsigned long long* accum,
+                            hipStream_t st)
 {
     GVC_TRY(check_params(prm));
     Arena a(const_cast<void*>(state), (size_t)-1);
-    ForwardLayout l(a, P, S, prm->leaf_size, prm->flags, prm->chunk_slots);
+    ForwardLayout l(a, bt, prm->flags, prm->chunk_slots);
     accumulate_kernel<<<1, 256, 0, st>>>(l.group_pairs, l.NG, l.group_offsets, l.err, accum);
     GVC_LAUNCH_CHECK("accumulate");
     return GVC_OK;
 }
 
-size_t backward_bytes(int P, uint64_t cap)
+int mixture_backward_impl(const Batch& bt, const float* scales, const float* rots, float k, const void* sample_set,
+                          const gvc_params* prm, const float* out_cells, const float* out_weights,
+                          const float* gcells, const float* gweights, void* state, size_t state_bytes,
+                          uint64_t capacity, void* ws, size_t ws_bytes, float* dmeans, float* dscales, float* drots,
+                          float* dvalues, float* dweights, hipStream_t st)
 {
-    Arena a(nullptr, 0);
-    BackwardLayout l(a, P, cap);
-    return a.off + 256;
-}
-
-int mixture_backward(int P, const float* means, const float* scales, const float* rots, const float* values,
-                     const float* weights, float k, const void* sample_set, int S, const gvc_params* prm,
-                     const float* out_cells, const float* out_weights, const float* gcells, const float* gweights,
-                     void* state, size_t state_bytes, uint64_t capacity, void* ws, size_t ws_bytes, float* dmeans,
-                     float* dscales, float* drots, float* dvalues, float* dweights, hipStream_t st)
-{
-    (void)means; (void)values; (void)weights;
     GVC_TRY(check_params(prm));
+    const int P = bt.P, S = bt.S, L = prm->leaf_size;
     if (P <= 0) return GVC_OK;
+    if (L != bt.L) return set_error(GVC_ERR_INVALID_ARGUMENT, "params.leaf_size differs from the batch's");
     if (capacity > 0xffffffffull) return set_error(GVC_ERR_INVALID_ARGUMENT, "record capacity above 2^32");
-    const int L = prm->leaf_size;
     Arena a(state, state_bytes);
-    ForwardLayout l(a, P, S, L, prm->flags, prm->chunk_slots);
+    ForwardLayout l(a, bt, prm->flags, prm->chunk_slots);
     if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "forward state buffer too small");
     Arena ba(ws, ws_bytes);
     BackwardLayout bl(ba, P, capacity);
     if (!ba.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "backward workspace too small");
-    Arena sa(const_cast<void*>(sample_set), sample_set_bytes(S));
-    SampleSetLayout sl(sa, S);
-    const int NG = l.NG, NL = l.NL;
+    const float4* pts = sample_set_points_impl(sample_set, S, bt.NG);
+    const BatchView bv = batch_view(table_of(bt, l.table), bt.V);
+    const int NG = l.NG;
     const unsigned stages = prm->stages ? (unsigned)prm->stages : 3u;
     if ((stages & GVC_STAGE_WALK) && NG > 0 && capacity > 0) {
         const int T = prm->block_threads;
@@ -1807,7 +1966,7 @@ int mixture_backward(int P, const float* means, const float* scales, const float
             const size_t lds = (size_t)(T / 64) * sizeof(BwdLds<true>);
             GVC_TRY(allow_lds(kf, lds));
             const unsigned nb = walk_blocks(kf, T, lds, NG);
-            backward_kernel<true><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, NG, out_cells,
+            backward_kernel<true><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
                                                       out_weights, gcells, gweights, l.group_offsets, capacity,
                                                       bl.parts, bl.keys, bl.vals, l.err, l.queues + 8, l.ext_sum,
                                                       l.log, l.group_head, rp);
@@ -1816,7 +1975,7 @@ int mixture_backward(int P, const float* means, const float* scales, const float
             const size_t lds = (size_t)(T / 64) * sizeof(BwdLds<false>);
             GVC_TRY(allow_lds(kf, lds));
             const unsigned nb = walk_blocks(kf, T, lds, NG);
-            backward_kernel<false><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, P, NL, L, sl.pts, NG, out_cells,
+            backward_kernel<false><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
                                                        out_weights, gcells, gweights, l.group_offsets, capacity,
                                                        bl.parts, bl.keys, bl.vals, l.err, l.queues + 8, l.ext_sum,
                                                        l.log, l.group_head, rp);
@@ -1843,6 +2002,185 @@ int mixture_backward(int P, const float* means, const float* scales, const float
                                                             k, dmeans, dscales, drots, dvalues, dweights);
     GVC_LAUNCH_CHECK("reduce_chain");
     return GVC_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host entry points (called from capi.cpp)
+// ---------------------------------------------------------------------------
+int preprocess_launch(int P, const float* means, const float* scales, const float* rots, const float* weights,
+                      float k, unsigned flags, float* conics, float* boxes, hipStream_t st)
+{
+    if (P <= 0) return GVC_OK;
+    preprocess_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, means, scales, rots, weights, k,
+                                                          (flags & GVC_FLAG_FP64_COND) ? 1 : 0, conics, boxes);
+    GVC_LAUNCH_CHECK("preprocess");
+    return GVC_OK;
+}
+
+int morton_launch(int n, const float* xyz, int clamp, uint64_t* codes, hipStream_t st)
+{
+    if (n <= 0) return GVC_OK;
+    morton63_kernel<<<blocks_for(n, 256), 256, 0, st>>>(xyz, n, clamp, codes);
+    GVC_LAUNCH_CHECK("morton");
+    return GVC_OK;
+}
+
+size_t sample_set_bytes(int S) { return sample_set_bytes_impl(S, (S + 63) / 64); }
+
+int sample_set_build(int S, const float* samples, void* set, size_t bytes, hipStream_t st)
+{
+    return sample_set_build_impl(single_batch(0, S, 1, nullptr), samples, set, bytes, st);
+}
+
+const float4* sample_set_points(const void* sample_set, int S)
+{
+    return sample_set_points_impl(sample_set, S, (S + 63) / 64);
+}
+
+size_t forward_state_bytes(int P, int S, const gvc_params* prm)
+{
+    if (check_params(prm) != GVC_OK) return 0;
+    return forward_state_bytes_impl(single_batch(P, S, prm->leaf_size, nullptr), prm);
+}
+
+int mixture_forward(int P, const float* means, const float* scales, const float* rots, const float* values,
+                    const float* weights, float k, const void* sample_set, int S, const gvc_params* prm,
+                    float* out_cells, float* out_weights, float* conics, void* state, size_t state_bytes,
+                    hipStream_t st)
+{
+    GVC_TRY(check_params(prm));
+    return mixture_forward_impl(single_batch(P, S, prm->leaf_size, nullptr), means, scales, rots, values, weights, k,
+                                sample_set, prm, out_cells, out_weights, conics, state, state_bytes, st);
+}
+
+int forward_stats(const void* state, int P, int S, const gvc_params* prm, uint64_t* n_records, uint64_t* n_pairs,
+                  uint32_t* error_flags, uint64_t* chunk_slots_used, hipStream_t st)
+{
+    GVC_TRY(check_params(prm));
+    return forward_stats_impl(single_batch(P, S, prm->leaf_size, nullptr), state, prm, n_records, n_pairs,
+                              error_flags, chunk_slots_used, st);
+}
+
+int forward_accumulate(const void* state, int P, int S, const gvc_params* prm, unsigned long long* accum,
+                       hipStream_t st)
+{
+    GVC_TRY(check_params(prm));
+    return forward_accumulate_impl(single_batch(P, S, prm->leaf_size, nullptr), state, prm, accum, st);
+}
+
+size_t backward_bytes(int P, uint64_t cap)
+{
+    Arena a(nullptr, 0);
+    BackwardLayout l(a, P, cap);
+    return a.off + 256;
+}
+
+int mixture_backward(int P, const float* means, const float* scales, const float* rots, const float* values,
+                     const float* weights, float k, const void* sample_set, int S, const gvc_params* prm,
+                     const float* out_cells, const float* out_weights, const float* gcells, const float* gweights,
+                     void* state, size_t state_bytes, uint64_t capacity, void* ws, size_t ws_bytes, float* dmeans,
+                     float* dscales, float* drots, float* dvalues, float* dweights, hipStream_t st)
+{
+    (void)means; (void)values; (void)weights;
+    GVC_TRY(check_params(prm));
+    return mixture_backward_impl(single_batch(P, S, prm->leaf_size, nullptr), scales, rots, k, sample_set, prm,
+                                 out_cells, out_weights, gcells, gweights, state, state_bytes, capacity, ws, ws_bytes,
+                                 dmeans, dscales, drots, dvalues, dweights, st);
+}
+
+// ---- batches ---------------------------------------------------------------
+size_t batch_table_bytes(int V) { return V > 0 ? batch_table_ints(V) * sizeof(int) + 256 : 0; }
+
+int batch_describe(int V, const int* P_counts, const int* S_counts, int L, void* table, size_t bytes,
+                   gvc_batch* out, hipStream_t st)
+{
+    if (V < 1) return set_error(GVC_ERR_INVALID_ARGUMENT, "a batch needs at least one volume");
+    if (L < 1 || L > 64) return set_error(GVC_ERR_INVALID_ARGUMENT, "leaf_size must be in [1, 64]");
+    if (bytes < batch_table_ints(V) * sizeof(int)) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "batch table too small");
+    std::vector<int> h(batch_table_ints(V), 0);
+    h[0] = V;
+    const size_t n = (size_t)V + 1;
+    int* Po = h.data() + 4;
+    int* So = Po + n;
+    int* Go = So + n;
+    int* Lo = Go + n;
+    int* No = Lo + n;
+    long long p = 0, s = 0, g = 0, l = 0, ni = 0;
+    for (int v = 0; v < V; ++v) {
+        if (P_counts[v] < 1 || S_counts[v] < 0)
+            return set_error(GVC_ERR_INVALID_ARGUMENT, "every volume needs >= 1 Gaussian and >= 0 samples");
+        Po[v] = (int)p; So[v] = (int)s; Go[v] = (int)g; Lo[v] = (int)l; No[v] = (int)ni;
+        const long long nl = (P_counts[v] + L - 1) / L;
+        p += P_counts[v]; s += S_counts[v]; g += (S_counts[v] + 63) / 64; l += nl; ni += nl > 1 ? nl - 1 : 0;
+        if (p > 0x7fffffffll || s > 0x7fffffffll || 64 * g > 0x7fffffffll)
+            return set_error(GVC_ERR_INVALID_ARGUMENT, "batch totals above 2^31");
+    }
+    Po[V] = (int)p; So[V] = (int)s; Go[V] = (int)g; Lo[V] = (int)l; No[V] = (int)ni;
+    GVC_HIP(hipMemcpyAsync(table, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    GVC_HIP(hipStreamSynchronize(st));
+    out->volumes = V;
+    out->leaf_size = L;
+    out->P_total = (int)p;
+    out->S_total = (int)s;
+    out->groups = (int)g;
+    out->leaves = (int)l;
+    out->internal_nodes = (int)ni;
+    out->table = table;
+    return GVC_OK;
+}
+
+static Batch from_c(const gvc_batch* b)
+{
+    Batch r;
+    r.V = b->volumes; r.P = b->P_total; r.S = b->S_total; r.NG = b->groups; r.NL = b->leaves;
+    r.NI = b->internal_nodes; r.L = b->leaf_size; r.table = b->table;
+    return r;
+}
+
+size_t sample_set_bytes_batched(const gvc_batch* b) { return sample_set_bytes_impl(b->S_total, b->groups); }
+
+int sample_set_build_batched(const gvc_batch* b, const float* samples, void* set, size_t bytes, hipStream_t st)
+{
+    return sample_set_build_impl(from_c(b), samples, set, bytes, st);
+}
+
+size_t forward_state_bytes_batched(const gvc_batch* b, const gvc_params* prm)
+{
+    return forward_state_bytes_impl(from_c(b), prm);
+}
+
+int mixture_forward_batched(const gvc_batch* b, const float* means, const float* scales, const float* rots,
+                            const float* values, const float* weights, float k, const void* sample_set,
+                            const gvc_params* prm, float* out_cells, float* out_weights, float* conics, void* state,
+                            size_t state_bytes, hipStream_t st)
+{
+    return mixture_forward_impl(from_c(b), means, scales, rots, values, weights, k, sample_set, prm, out_cells,
+                                out_weights, conics, state, state_bytes, st);
+}
+
+int forward_stats_batched(const gvc_batch* b, const void* state, const gvc_params* prm, uint64_t* n_records,
+                          uint64_t* n_pairs, uint32_t* error_flags, uint64_t* chunk_slots_used, hipStream_t st)
+{
+    return forward_stats_impl(from_c(b), state, prm, n_records, n_pairs, error_flags, chunk_slots_used, st);
+}
+
+int forward_accumulate_batched(const gvc_batch* b, const void* state, const gvc_params* prm,
+                               unsigned long long* accum, hipStream_t st)
+{
+    return forward_accumulate_impl(from_c(b), state, prm, accum, st);
+}
+
+int mixture_backward_batched(const gvc_batch* b, const float* scales, const float* rots, float k,
+                             const void* sample_set, const gvc_params* prm, const float* out_cells,
+                             const float* out_weights, const float* gcells, const float* gweights, void* state,
+                             size_t state_bytes, uint64_t capacity, void* ws, size_t ws_bytes, float* dmeans,
+                             float* dscales, float* drots, float* dvalues, float* dweights, hipStream_t st)
+{
+    return mixture_backward_impl(from_c(b), scales, rots, k, sample_set, prm, out_cells, out_weights, gcells, gweights,
+                                 state, state_bytes, capacity, ws, ws_bytes, dmeans, dscales, drots, dvalues,
+                                 dweights, st);
 }
 
 int sort_pairs_u64(int n, const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, int key_bits,

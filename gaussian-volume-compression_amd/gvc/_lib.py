@@ -25,6 +25,12 @@ class GvcError(RuntimeError):
     pass
 
 
+class gvc_batch(ctypes.Structure):
+    _fields_ = [("volumes", ctypes.c_int), ("leaf_size", ctypes.c_int), ("P_total", ctypes.c_int),
+                ("S_total", ctypes.c_int), ("groups", ctypes.c_int), ("leaves", ctypes.c_int),
+                ("internal_nodes", ctypes.c_int), ("table", ctypes.c_void_p)]
+
+
 class gvc_params(ctypes.Structure):
     _fields_ = [("leaf_size", ctypes.c_int), ("block_threads", ctypes.c_int),
                 ("flags", ctypes.c_uint), ("stages", ctypes.c_uint), ("chunk_slots", ctypes.c_uint),
@@ -40,6 +46,7 @@ _F = ctypes.c_float
 _SZ = ctypes.c_size_t
 _U64 = ctypes.c_uint64
 _PP = ctypes.POINTER(gvc_params)
+_PB = ctypes.POINTER(gvc_batch)
 
 # name: (restype, argtypes)
 _SIGS = {
@@ -67,6 +74,17 @@ _SIGS = {
     "gvc_grid_sample": (_I, [_P, _P, _P, _I, _P, _I, _P, _P, _P]),
     "gvc_knn3_bytes": (_SZ, [_I]),
     "gvc_knn3": (_I, [_I, _P, _P, _P, _SZ, _P]),
+    "gvc_batch_table_bytes": (_SZ, [_I]),
+    "gvc_batch_describe": (_I, [_I, _P, _P, _I, _P, _SZ, _PB, _P]),
+    "gvc_sample_set_bytes_batched": (_SZ, [_PB]),
+    "gvc_sample_set_build_batched": (_I, [_PB, _P, _P, _SZ, _P]),
+    "gvc_forward_state_bytes_batched": (_SZ, [_PB, _PP]),
+    "gvc_mixture_forward_batched": (_I, [_PB, _P, _P, _P, _P, _P, _F, _P, _PP, _P, _P, _P, _P, _SZ, _P]),
+    "gvc_forward_stats_batched": (_I, [_PB, _P, _PP, ctypes.POINTER(_U64), ctypes.POINTER(_U64),
+                                       ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_U64), _P]),
+    "gvc_forward_accumulate_batched": (_I, [_PB, _P, _PP, _P, _P]),
+    "gvc_mixture_backward_batched": (_I, [_PB, _P, _P, _F, _P, _PP, _P, _P, _P, _P, _P, _SZ, _U64, _P, _SZ,
+                                          _P, _P, _P, _P, _P, _P]),
     "gvc_activate_forward": (_I, [_I, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P]),
     "gvc_activate_backward": (_I, [_I, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "gvc_loss_state_bytes": (_SZ, [_I]),

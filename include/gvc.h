@@ -198,6 +198,49 @@ size_t gvc_knn3_bytes(int n);
 int gvc_knn3(int n, const float *points, float *mean_sq_dist, void *workspace, size_t workspace_bytes,
              void *stream);
 
+/* ---- batches of independent volumes ---------------------------------------------
+ * V volumes of uneven size fitted in one persistent launch per pass: each volume
+ * has its own Gaussians, samples, Morton order and BVH; groups never mix volumes.
+ * Inputs are concatenated volume-major (volume 0's Gaussians, then volume 1's,
+ * ...; the same for samples).  No counterpart in the reference, which fits one
+ * volume per process (R/train.py); with V = 1 results equal the single-volume
+ * calls bit for bit.  gvc_batch_describe copies the offset table to the device
+ * (BLOCKING; call once per batch shape, outside captured regions). */
+typedef struct gvc_batch {
+    int volumes;
+    int leaf_size;       /* must equal params->leaf_size of the calls below */
+    int P_total, S_total;
+    int groups;          /* sum of ceil(S_v / 64)                           */
+    int leaves;          /* sum of ceil(P_v / leaf_size)                    */
+    int internal_nodes;  /* sum of max(leaves_v - 1, 0)                     */
+    const void *table;   /* device offset table                             */
+} gvc_batch;
+
+size_t gvc_batch_table_bytes(int volumes);
+int gvc_batch_describe(int volumes, const int *gaussians_per_volume, const int *samples_per_volume, int leaf_size,
+                       void *table, size_t table_bytes, gvc_batch *out, void *stream);
+size_t gvc_sample_set_bytes_batched(const gvc_batch *batch);
+int gvc_sample_set_build_batched(const gvc_batch *batch, const float *samples, void *sample_set, size_t bytes,
+                                 void *stream);
+size_t gvc_forward_state_bytes_batched(const gvc_batch *batch, const gvc_params *params);
+int gvc_mixture_forward_batched(const gvc_batch *batch, const float *means3D, const float *scales,
+                                const float *rotations, const float *values, const float *weights,
+                                float scale_modifier, const void *sample_set, const gvc_params *params,
+                                float *out_cells, float *out_weights, float *conics, void *state,
+                                size_t state_bytes, void *stream);
+int gvc_forward_stats_batched(const gvc_batch *batch, const void *state, const gvc_params *params,
+                              uint64_t *n_records, uint64_t *n_pairs, uint32_t *error_flags,
+                              uint64_t *chunk_slots_used, void *stream);
+int gvc_forward_accumulate_batched(const gvc_batch *batch, const void *state, const gvc_params *params,
+                                   uint64_t *accum, void *stream);
+int gvc_mixture_backward_batched(const gvc_batch *batch, const float *scales, const float *rotations,
+                                 float scale_modifier, const void *sample_set, const gvc_params *params,
+                                 const float *out_cells, const float *out_weights, const float *dL_dout_cells,
+                                 const float *dL_dout_cell_weights, void *state, size_t state_bytes,
+                                 uint64_t record_capacity, void *workspace, size_t workspace_bytes,
+                                 float *dL_dmeans3D, float *dL_dscales, float *dL_drotations, float *dL_dvalues,
+                                 float *dL_dweights, void *stream);
+
 /* ---- training-step glue ----------------------------------------------------------
  * Activations, R/scene/gaussian_model.py:31-48, :71-77, :114-132 (get_scaling with
  * _apply_cap, get_rotation, get_weight, get_values): scales [P,3] = s * r_soft / r
