@@ -1447,60 +1447,10 @@ __global__ void reduce_chain_kernel(int P, const GRecord* __restrict__ rec, cons
     dmeans[3 * id + 2] = (float)acc[4];
 
     // covariance chain
-    const float q4[4] = { rots[4 * id], rots[4 * id + 1], rots[4 * id + 2], rots[4 * id + 3] };
-    double R[3][3];
-    rotation_d(q4[0], q4[1], q4[2], q4[3], R);
-    const double km = kmod;
-    const double s[3] = { km * scales[3 * id], km * scales[3 * id + 1], km * scales[3 * id + 2] };
-    double M[3][3];
-    for (int a = 0; a < 3; ++a)
-        for (int c = 0; c < 3; ++c) M[a][c] = s[a] * R[c][a];
-    double Sg[3][3];
-    for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) Sg[r][c] = M[0][r] * M[0][c] + M[1][r] * M[1][c] + M[2][r] * M[2][c];
-    int jx = 0;
-    double best = fabs(Sg[0][0]);
-    if (fabs(Sg[1][1]) > best) { best = fabs(Sg[1][1]); jx = 1; }
-    if (fabs(Sg[2][2]) > best) { best = fabs(Sg[2][2]); jx = 2; }
-    const double eps = best * 1e-5;
-    const double a_ = Sg[0][0] + eps, b_ = Sg[0][1], c_ = Sg[0][2], d_ = Sg[1][1] + eps, e_ = Sg[1][2],
-                 f_ = Sg[2][2] + eps;
-    const double A00 = d_ * f_ - e_ * e_, A01 = c_ * e_ - b_ * f_, A02 = b_ * e_ - c_ * d_;
-    const double A11 = a_ * f_ - c_ * c_, A12 = b_ * c_ - a_ * e_, A22 = a_ * d_ - b_ * b_;
-    const double det = a_ * A00 + b_ * A01 + c_ * A02;
-    const double C[3][3] = { { A00 / det, A01 / det, A02 / det },
-                             { A01 / det, A11 / det, A12 / det },
-                             { A02 / det, A12 / det, A22 / det } };
-    const double G[3][3] = { { acc[5], 0.5 * acc[6], 0.5 * acc[7] },
-                             { 0.5 * acc[6], acc[8], 0.5 * acc[9] },
-                             { 0.5 * acc[7], 0.5 * acc[9], acc[10] } };
-    double CG[3][3], K[3][3];
-    for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) CG[r][c] = C[r][0] * G[0][c] + C[r][1] * G[1][c] + C[r][2] * G[2][c];
-    for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) K[r][c] = -(CG[r][0] * C[0][c] + CG[r][1] * C[1][c] + CG[r][2] * C[2][c]);
-    const double trH = K[0][0] + K[1][1] + K[2][2];
-    const double sg = Sg[jx][jx] > 0 ? 1.0 : (Sg[jx][jx] < 0 ? -1.0 : 0.0);
-    K[jx][jx] += trH * 1e-5 * sg;
-    double dM[3][3];
-    for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) dM[r][c] = 2.0 * (M[r][0] * K[0][c] + M[r][1] * K[1][c] + M[r][2] * K[2][c]);
-    for (int a = 0; a < 3; ++a) {
-        const double v = dM[a][0] * R[0][a] + dM[a][1] * R[1][a] + dM[a][2] * R[2][a];
-        dscales[3 * id + a] = (float)(km * v);
-    }
-    double Gq[3][3];
-    for (int r = 0; r < 3; ++r)
-        for (int a = 0; a < 3; ++a) Gq[r][a] = dM[a][r] * s[a];
-    const double qr = q4[0], qx = q4[1], qy = q4[2], qz = q4[3];
-    drots[4 * id + 0] = (float)(2.0 * (qz * (Gq[1][0] - Gq[0][1]) + qy * (Gq[0][2] - Gq[2][0]) +
-                                       qx * (Gq[2][1] - Gq[1][2])));
-    drots[4 * id + 1] = (float)(2.0 * (qy * (Gq[0][1] + Gq[1][0]) + qz * (Gq[0][2] + Gq[2][0]) +
-                                       qr * (Gq[2][1] - Gq[1][2])) - 4.0 * qx * (Gq[1][1] + Gq[2][2]));
-    drots[4 * id + 2] = (float)(2.0 * (qx * (Gq[0][1] + Gq[1][0]) + qr * (Gq[0][2] - Gq[2][0]) +
-                                       qz * (Gq[1][2] + Gq[2][1])) - 4.0 * qy * (Gq[0][0] + Gq[2][2]));
-    drots[4 * id + 3] = (float)(2.0 * (qr * (Gq[1][0] - Gq[0][1]) + qx * (Gq[0][2] + Gq[2][0]) +
-                                       qy * (Gq[1][2] + Gq[2][1])) - 4.0 * qz * (Gq[0][0] + Gq[1][1]));
+    double ds[3], dq[4];
+    covariance_chain(acc + 5, scales + 3 * id, rots + 4 * id, kmod, ds, dq);
+    for (int c = 0; c < 3; ++c) dscales[3 * id + c] = (float)ds[c];
+    for (int c = 0; c < 4; ++c) drots[4 * id + c] = (float)dq[c];
 }
 
 __global__ void zero_floats_kernel(float* p, int n)

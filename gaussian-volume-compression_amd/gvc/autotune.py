@@ -49,16 +49,14 @@ def _save_file(entries):
         json.dump(data, f, indent=1, sort_keys=True)
 
 
-def _time_config(params, args, sample_set, gV, gW, reps):
-    means, scales, rots, values, weights = args
+def _time(run, reps):
+    """Best of ``reps`` timed calls of run() (after one untimed call), in ms."""
     ev = []
     for r in range(reps + 1):
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
-        cells, cw, _, st = mx.mixture_forward(means, scales, rots, values, weights, 1.0, sample_set, params,
-                                              want_conics=False)
-        mx.mixture_backward(st, means, scales, rots, values, weights, 1.0, sample_set, cells, cw, gV, gW)
+        run()
         e1.record()
         if r > 0:
             ev.append((e0, e1))
@@ -66,28 +64,8 @@ def _time_config(params, args, sample_set, gV, gW, reps):
     return min(a.elapsed_time(b) for a, b in ev)
 
 
-def autotune(means, scales, rots, values, weights, sample_set, base=None, leaf_sizes=LEAF_SIZES,
-             run_points=RUN_POINTS, blocks=BLOCKS, reps=2, use_cache=True):
-    """Returns (best MixtureParams, {config label: ms}).  ``base`` supplies the
-    precision and candidate-log choices, which are not tuned."""
-    base = base or mx.default_params()
-    P, S = int(means.shape[0]), int(sample_set.S)
-    key = _bucket(P, S)
-    if use_cache:
-        if key in _cache:
-            return _cache[key], {}
-        disk = _load_file().get(key)
-        if disk:
-            best = mx.MixtureParams(leaf_size=disk["leaf_size"], block_threads=disk["block_threads"],
-                                    run_points=disk["run_points"], kahan=base.kahan, fp64_cond=base.fp64_cond,
-                                    candidate_log=base.candidate_log)
-            _cache[key] = best
-            return best, {}
-    g = torch.Generator(device=means.device)
-    g.manual_seed(1234)
-    gV = torch.randn(S, device=means.device, generator=g)
-    gW = torch.randn(S, device=means.device, generator=g)
-    args = (means, scales, rots, values, weights)
+def _sweep(make_run, base, leaf_sizes, run_points, blocks, reps):
+    """Times make_run(params)() for every configuration; returns (best, its ms, table)."""
     table = {}
     best, best_ms = None, float("inf")
     for L in leaf_sizes:
@@ -97,7 +75,7 @@ def autotune(means, scales, rots, values, weights, sample_set, base=None, leaf_s
                                        fp64_cond=base.fp64_cond, candidate_log=base.candidate_log)
                 label = f"leaf{L}/run{rp}/block{bt}"
                 try:
-                    ms = _time_config(prm, args, sample_set, gV, gW, reps)
+                    ms = _time(make_run(prm), reps)
                 except GvcError:
                     table[label] = None      # infeasible (LDS)
                     continue
@@ -106,7 +84,87 @@ def autotune(means, scales, rots, values, weights, sample_set, base=None, leaf_s
                     best, best_ms = prm, ms
     if best is None:
         raise GvcError("autotune: no feasible configuration")
+    return best, best_ms, table
+
+
+def _cached(key, base, use_cache):
+    if not use_cache:
+        return None
+    if key in _cache:
+        return _cache[key]
+    disk = _load_file().get(key)
+    if disk:
+        best = mx.MixtureParams(leaf_size=disk["leaf_size"], block_threads=disk["block_threads"],
+                                run_points=disk["run_points"], kahan=base.kahan, fp64_cond=base.fp64_cond,
+                                candidate_log=base.candidate_log)
+        _cache[key] = best
+        return best
+    return None
+
+
+def _remember(key, best, best_ms):
     _cache[key] = best
     _save_file({key: {"leaf_size": best.leaf_size, "block_threads": best.block_threads,
                       "run_points": best.run_points, "ms": best_ms}})
+
+
+def autotune(means, scales, rots, values, weights, sample_set, base=None, leaf_sizes=LEAF_SIZES,
+             run_points=RUN_POINTS, blocks=BLOCKS, reps=2, use_cache=True):
+    """Returns (best MixtureParams, {config label: ms}).  ``base`` supplies the
+    precision and candidate-log choices, which are not tuned."""
+    base = base or mx.default_params()
+    P, S = int(means.shape[0]), int(sample_set.S)
+    key = _bucket(P, S)
+    hit = _cached(key, base, use_cache)
+    if hit is not None:
+        return hit, {}
+    g = torch.Generator(device=means.device)
+    g.manual_seed(1234)
+    gV = torch.randn(S, device=means.device, generator=g)
+    gW = torch.randn(S, device=means.device, generator=g)
+
+    def make_run(prm):
+        def run():
+            cells, cw, _, st = mx.mixture_forward(means, scales, rots, values, weights, 1.0, sample_set, prm,
+                                                  want_conics=False)
+            mx.mixture_backward(st, means, scales, rots, values, weights, 1.0, sample_set, cells, cw, gV, gW)
+        return run
+
+    best, best_ms, table = _sweep(make_run, base, leaf_sizes, run_points, blocks, reps)
+    _remember(key, best, best_ms)
+    return best, table
+
+
+def autotune_batched(gaussians_per_volume, samples_per_volume, means, scales, rots, values, weights, samples,
+                     base=None, leaf_sizes=LEAF_SIZES, run_points=RUN_POINTS, blocks=BLOCKS, reps=2,
+                     use_cache=True):
+    """The same sweep for a batch of volumes (inputs concatenated volume-major);
+    one Batch and sample set per leaf size.  Returns (best, table)."""
+    from . import batched as bx
+    base = base or mx.default_params()
+    key = "batch:" + _bucket(int(means.shape[0]), int(samples.shape[0])) + f":{len(gaussians_per_volume)}"
+    hit = _cached(key, base, use_cache)
+    if hit is not None:
+        return hit, {}
+    S = int(samples.shape[0])
+    g = torch.Generator(device=means.device)
+    g.manual_seed(1234)
+    gV = torch.randn(S, device=means.device, generator=g)
+    gW = torch.randn(S, device=means.device, generator=g)
+    sets = {}
+
+    def make_run(prm):
+        if prm.leaf_size not in sets:
+            b = bx.Batch(gaussians_per_volume, samples_per_volume, leaf_size=prm.leaf_size, device=means.device)
+            sets[prm.leaf_size] = (b, bx.BatchedSampleSet(b, samples))
+        b, ss = sets[prm.leaf_size]
+
+        def run():
+            cells, cw, _, st = bx.mixture_forward_batched(b, means, scales, rots, values, weights, 1.0, ss, prm,
+                                                          want_conics=False)
+            bx.mixture_backward_batched(st, means, scales, rots, values, weights, 1.0, ss, cells, cw, gV, gW)
+        return run
+
+    best, best_ms, table = _sweep(make_run, base, leaf_sizes, run_points, blocks, reps)
+    _remember(key, best, best_ms)
     return best, table

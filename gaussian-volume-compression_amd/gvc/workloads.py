@@ -233,3 +233,48 @@ def grid_workload(device, seed=0, n=128, P=200_000):
 
     g.update(grid_fn=grid_fn, field=field)
     return g
+
+
+def batch_workload(device, seed=0, volumes=32, total_points=1 << 24, point_batches=2, n_blobs=8):
+    """32 independent synthetic volumes of uneven size, fitted side by side.
+
+    Per volume v (draw order: sides, Gaussian counts, then per volume blobs,
+    Gaussian means, point batches): side_v = round(exp U[ln 32, ln 256]); P_v =
+    round(exp U[ln 1e4, ln 2e6]); points per step proportional to side_v^3 with
+    sum exactly ``total_points`` (largest remainders).  The field of a volume is
+    a sum of ``n_blobs`` random blobs (the small-config distributions), scaled
+    to [0, 1] by its maximum over the volume's points; Gaussians start at
+    uniform random positions with the reference's initialisation (3-NN scales,
+    identity rotations, weight 0.1, value = field at the mean).  Like the
+    reference's precomputed point batches (R/train.py:240-289), ``point_batches``
+    batches of points and their ground truth are drawn once; step k uses batch
+    k mod point_batches.  Every volume lives in its own unit cube."""
+    import torch
+    rng = np.random.default_rng(seed)
+    sides = np.rint(np.exp(rng.uniform(np.log(32.0), np.log(256.0), volumes))).astype(np.int64)
+    P = np.rint(np.exp(rng.uniform(np.log(1e4), np.log(2e6), volumes))).astype(np.int64)
+    vox = sides.astype(np.float64) ** 3
+    share = total_points * vox / vox.sum()
+    S = np.floor(share).astype(np.int64)
+    rest = total_points - int(S.sum())
+    S[np.argsort(-(share - S), kind="stable")[:rest]] += 1
+    parts = []
+    pts = [[] for _ in range(point_batches)]
+    gts = [[] for _ in range(point_batches)]
+    for v in range(volumes):
+        b = blobs(rng, n_blobs)
+        means = rng.uniform(0.0, 1.0, (int(P[v]), 3)).astype(np.float32)
+        xs = [rng.uniform(0.0, 1.0, (int(S[v]), 3)).astype(np.float32) for _ in range(point_batches)]
+        fx = [_eval_blobs_torch(b, x, device) for x in xs]
+        fm = _eval_blobs_torch(b, means, device)
+        top = max([float(f.max()) for f in fx if f.numel()] + [float(fm.max()), 1e-12])
+        parts.append(_init_gaussians(means, (fm / top).float().cpu().numpy(), device))
+        for k in range(point_batches):
+            pts[k].append(torch.as_tensor(xs[k], device=device))
+            gts[k].append((fx[k] / top).float())
+    cat = {k: torch.cat([p[k] for p in parts]).contiguous()
+           for k in ("means", "log_scales", "rotations", "weights", "values")}
+    cat.update(sides=sides, gaussians=P, samples=S, P=int(P.sum()), S=int(S.sum()),
+               point_batches=[torch.cat(x).contiguous() for x in pts],
+               ground_truth=[torch.cat(g).contiguous() for g in gts])
+    return cat
