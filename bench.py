@@ -199,13 +199,15 @@ def main():
     mx.set_record_capacity(None)
     mx.set_audit(audit)
     max_rec = 0
+    lane_groups = 0
     for _ in range(args.warmup):
         eager_step()
         torch.cuda.synchronize()
         max_rec = max(max_rec, audit[-1].stats()[0])
+        lane_groups = audit[-1].lane_groups
     cap = int(max_rec * 1.25) + 4096
     mx.set_record_capacity(cap)
-    log(f"warmup done, record capacity {cap}, candidate-log slots/group {mx._chunk_slots}")
+    log(f"warmup done, record capacity {cap}, candidate-log slots/group {mx.chunk_slots_for((S + 63) // 64)}")
     # stage profile (untimed steps, HIP events between the stages)
     timer = []
     mx.set_stage_timer(timer)
@@ -286,6 +288,7 @@ def main():
                    "run_points": params.run_points, "kahan": params.kahan, "candidate_log": params.candidate_log,
                    "parallelism": "single GPU"},
         "pairs_per_step": pairs_per_step,
+        "groups_walked_per_lane": lane_groups,
         "autotune": tune,
         "stage_ms": stage_avg,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -424,10 +427,12 @@ def main_batch(args):
     mx.set_audit(audit)
     mx.set_record_capacity(None)
     max_rec = 0
+    lane_groups = 0
     for k in range(args.warmup):
         fused_step(k, fp, fopt)
         torch.cuda.synchronize()
         max_rec = max(max_rec, audit[-1].stats()[0])
+        lane_groups = audit[-1].lane_groups
     mx.set_record_capacity(int(max_rec * 1.25) + 4096)
     timer = []
     mx.set_stage_timer(timer)
@@ -489,6 +494,8 @@ def main_batch(args):
                    "block_threads": params.block_threads, "run_points": params.run_points, "kahan": params.kahan,
                    "candidate_log": params.candidate_log, "parallelism": "single GPU"},
         "pairs_per_step": fused_pairs,
+        "groups": int(batch.c.groups),
+        "groups_walked_per_lane": lane_groups,
         "evals_per_s": 2.0 * fused_pairs / fused_s,
         "stage_ms": stage,
         "autotune": tune,

@@ -33,7 +33,8 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
                     float* out_cells, float* out_weights, float* conics, void* state, size_t state_bytes,
                     hipStream_t st);
 int forward_stats(const void* state, int P, int S, const gvc_params* prm, uint64_t* n_records, uint64_t* n_pairs,
-                  uint32_t* error_flags, uint64_t* chunk_slots_used, hipStream_t st);
+                  uint32_t* error_flags, uint64_t* chunk_slots_used, uint64_t* lane_groups,
+                  hipStream_t st);
 size_t backward_bytes(int P, uint64_t cap);
 int forward_accumulate(const void* state, int P, int S, const gvc_params* prm, unsigned long long* accum,
                        hipStream_t st);
@@ -58,7 +59,8 @@ int mixture_forward_batched(const gvc_batch* b, const float* means, const float*
                             const gvc_params* prm, float* out_cells, float* out_weights, float* conics, void* state,
                             size_t state_bytes, hipStream_t st);
 int forward_stats_batched(const gvc_batch* b, const void* state, const gvc_params* prm, uint64_t* n_records,
-                          uint64_t* n_pairs, uint32_t* error_flags, uint64_t* chunk_slots_used, hipStream_t st);
+                          uint64_t* n_pairs, uint32_t* error_flags, uint64_t* chunk_slots_used, uint64_t* lane_groups,
+                          hipStream_t st);
 int forward_accumulate_batched(const gvc_batch* b, const void* state, const gvc_params* prm,
                                unsigned long long* accum, hipStream_t st);
 int mixture_backward_batched(const gvc_batch* b, const float* scales, const float* rots, float k,
@@ -181,11 +183,12 @@ int gvc_mixture_forward(int P, const float* means3D, const float* scales, const 
 }
 
 int gvc_forward_stats(const void* state, int P, int S, const gvc_params* params, uint64_t* n_records,
-                      uint64_t* n_pairs, uint32_t* error_flags, uint64_t* chunk_slots_used, void* stream)
+                      uint64_t* n_pairs, uint32_t* error_flags, uint64_t* chunk_slots_used, uint64_t* lane_groups,
+                      void* stream)
 {
     GVC_REQUIRE(state, "NULL state");
     return forward_stats(state, P, S, params, n_records, n_pairs, error_flags, chunk_slots_used,
-                         (hipStream_t)stream);
+                         lane_groups, (hipStream_t)stream);
 }
 
 int gvc_forward_accumulate(const void* state, int P, int S, const gvc_params* params, uint64_t* accum, void* stream)
@@ -330,12 +333,12 @@ int gvc_mixture_forward_batched(const gvc_batch* batch, const float* means3D, co
 
 int gvc_forward_stats_batched(const gvc_batch* batch, const void* state, const gvc_params* params,
                               uint64_t* n_records, uint64_t* n_pairs, uint32_t* error_flags,
-                              uint64_t* chunk_slots_used, void* stream)
+                              uint64_t* chunk_slots_used, uint64_t* lane_groups, void* stream)
 {
     GVC_REQUIRE_BATCH(batch);
     GVC_REQUIRE(state, "NULL state");
     return forward_stats_batched(batch, state, params, n_records, n_pairs, error_flags, chunk_slots_used,
-                                 (hipStream_t)stream);
+                                 lane_groups, (hipStream_t)stream);
 }
 
 int gvc_forward_accumulate_batched(const gvc_batch* batch, const void* state, const gvc_params* params,

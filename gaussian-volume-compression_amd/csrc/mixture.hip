@@ -256,13 +256,15 @@ __global__ void gaussian_codes_kernel(const float* __restrict__ means, int P, ui
     idx[i] = (uint32_t)i;
 }
 
-// Canonical preprocess, written as records in BVH (sorted) order.
+// Canonical preprocess, written as records in BVH (sorted) order; inv_perm[j] =
+// the record index of the caller's Gaussian j.
 __global__ __launch_bounds__(256) void record_kernel(int P, const uint32_t* __restrict__ perm,
                                                      const float* __restrict__ means,
                                                      const float* __restrict__ scales, const float* __restrict__ rots,
                                                      const float* __restrict__ values,
                                                      const float* __restrict__ weights, float k, int fp64_cond,
-                                                     GRecord* __restrict__ rec, float* __restrict__ conics_out)
+                                                     GRecord* __restrict__ rec, float* __restrict__ conics_out,
+                                                     uint32_t* __restrict__ inv_perm)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
@@ -282,6 +284,7 @@ __global__ __launch_bounds__(256) void record_kernel(int P, const uint32_t* __re
     r.hix = bx[3]; r.hiy = bx[4]; r.hiz = bx[5];
     r.pad0 = 0.f; r.pad1 = 0.f;
     rec[i] = r;
+    inv_perm[j] = (uint32_t)i;
     if (conics_out)
         for (int c = 0; c < 6; ++c) conics_out[6 * j + c] = cn[c];
 }
@@ -599,14 +602,32 @@ __device__ __forceinline__ uint64_t split_runs(float px, float py, float pz, int
     return starts;
 }
 
-// Dynamic group scheduling: the groups are cut into 8 contiguous slices, one per
-// XCD (workgroups are dealt round-robin to the XCDs, so blockIdx & 7 names the
-// wave's XCD); a wave takes the next group of its slice and, once that slice is
-// exhausted, moves on to the following slices.  Returns NG when all are done.
-// queue[8] must be zero at launch.
-__device__ __forceinline__ int next_group(uint32_t* __restrict__ queue, int NG, int& x, int& tries)
+// Dynamic group scheduling.  First the heavy groups (listed in the build by
+// their expected cost, so that the longest walks start at once instead of
+// trailing the pass): every wave takes the next one from one shared cursor.
+// Then the rest: the groups are cut into 8 contiguous slices, one per XCD
+// (workgroups are dealt round-robin to the XCDs, so blockIdx & 7 names the
+// wave's XCD); a wave takes the next group of its slice, skipping heavy ones,
+// and once that slice is exhausted moves on to the following slices.  Returns
+// NG when all are done.  queue[8] and *heavy_cursor must be zero at launch.
+struct GroupSched {
+    const uint32_t* heavy;   // [0] = count, [1..] = heavy groups
+    uint32_t* cursor;
+    const uint8_t* flag;     // 1 = heavy
+    bool heavy_done;
+};
+
+__device__ __forceinline__ int next_group(uint32_t* __restrict__ queue, int NG, int& x, int& tries, GroupSched& sc)
 {
     const int lane = threadIdx.x & 63;
+    if (!sc.heavy_done) {
+        const uint32_t nh = (uint32_t)uniformi((int)sc.heavy[0]);
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(sc.cursor, 1u);
+        t = (uint32_t)uniformi(__shfl((int)t, 0, 64));
+        if (t < nh) return uniformi((int)sc.heavy[1 + t]);
+        sc.heavy_done = true;
+    }
     while (tries < 8) {
         const uint32_t b = (uint32_t)(((uint64_t)NG * (uint32_t)x) >> 3);
         const uint32_t e = (uint32_t)(((uint64_t)NG * (uint32_t)(x + 1)) >> 3);
@@ -614,7 +635,11 @@ __device__ __forceinline__ int next_group(uint32_t* __restrict__ queue, int NG, 
         if (lane == 0) t = atomicAdd(&queue[x], 1u);
         t = (uint32_t)__shfl((int)t, 0, 64);
         t = (uint32_t)uniformi((int)t);
-        if (t < e - b) return (int)(b + t);
+        if (t < e - b) {
+            const int g = (int)(b + t);
+            if (uniformi((int)sc.flag[g]) != 0) continue;
+            return g;
+        }
         x = (x + 1) & 7;
         ++tries;
     }
@@ -783,6 +808,144 @@ __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Lane mode.  Where a group's points are sparse next to its Gaussians, the
+// shared walk of the group's box meets thousands of candidates for a few dozen
+// pairs per point, one 64-candidate chunk after another on one wavefront.  Such
+// a group is walked per lane instead: every lane runs a left-first depth-first
+// walk with its own point as the query (so a point meets its Gaussians in
+// ascending record order).  The lanes step in lockstep -- one node or one
+// Gaussian each per step -- and visit() is called converged after every step.
+// The forward and the backward take the same decision from the same inputs.
+// ---------------------------------------------------------------------------
+constexpr int kLaneStack = 64;
+constexpr float kLaneModeCandidates = 8192.0f;
+
+// Gaussians per unit volume of the volume's root box.
+__device__ __forceinline__ float gaussian_density(const TreeView& tv)
+{
+    const float* __restrict__ rb = tv.root_box;
+    const float vol = fmaxf(rb[3] - rb[0], 1e-12f) * fmaxf(rb[4] - rb[1], 1e-12f) * fmaxf(rb[5] - rb[2], 1e-12f);
+    return (float)(tv.g_hi - tv.g_lo) / vol;
+}
+
+// policy 0: lane mode where the expected candidates of the shared walk exceed
+// kLaneModeCandidates -- Gaussians per unit volume of the volume's root box times
+// the run boxes (starts: the group's runs over lanes [0, nv)) grown by the mean
+// Gaussian extent; 1: never; 2: always.
+__device__ __forceinline__ bool lane_mode_group(const TreeView& tv, float px, float py, float pz, int nv,
+                                                uint64_t starts, const float reach[3], int policy)
+{
+    if (policy != 0) return policy == 2;
+    if (nv <= 0) return false;
+    const int lane = threadIdx.x & 63;
+    float cost = 0.f;
+    uint64_t rs = starts;
+    while (rs != 0ull) {
+        const int a = __builtin_ctzll(rs);
+        rs &= rs - 1;
+        const int b = rs ? __builtin_ctzll(rs) : nv;
+        cost += box_cost(masked_box(px, py, pz, lane >= a && lane < b), reach);
+    }
+    return uniformi(gaussian_density(tv) * cost > kLaneModeCandidates ? 1 : 0) != 0;
+}
+
+// Heavy groups: expected candidates (from four 16-point segment boxes: runs hold
+// at least 16 points) above kHeavyCandidates.  One wave per group; heavy groups
+// are flagged and listed in heavy[1 .. heavy[0]] (heavy[0] zero at launch).
+constexpr float kHeavyCandidates = 4096.0f;
+
+__global__ __launch_bounds__(256) void group_cost_kernel(const float4* __restrict__ pts, int NG, BatchView bv,
+                                                         const float* __restrict__ root_box,
+                                                         const unsigned long long* __restrict__ ext_sum,
+                                                         uint32_t* __restrict__ heavy, uint8_t* __restrict__ flag)
+{
+    const int g = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (g >= NG) return;
+    const int lane = threadIdx.x & 63;
+    const int vol = bv.V == 1 ? 0 : uniformi(volume_of(bv.G_off, bv.V, g));
+    const TreeView tv = tree_of(bv, root_box, vol);
+    float reach[3];
+    mean_extent(ext_sum + 4 * vol, reach);
+    const float4 p = pts[(size_t)g * 64 + lane];
+    const bool valid = __builtin_bit_cast(int, p.w) >= 0 && p.x == p.x && p.y == p.y && p.z == p.z;
+    const float inf = __builtin_inff();
+    float lx = valid ? p.x : inf, ly = valid ? p.y : inf, lz = valid ? p.z : inf;
+    float hx = valid ? p.x : -inf, hy = valid ? p.y : -inf, hz = valid ? p.z : -inf;
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+        lx = fminf(lx, __shfl_xor(lx, d, 64)); ly = fminf(ly, __shfl_xor(ly, d, 64)); lz = fminf(lz, __shfl_xor(lz, d, 64));
+        hx = fmaxf(hx, __shfl_xor(hx, d, 64)); hy = fmaxf(hy, __shfl_xor(hy, d, 64)); hz = fmaxf(hz, __shfl_xor(hz, d, 64));
+    }
+    float c = ((lane & 15) == 0 && hx >= lx) ? (hx - lx + reach[0]) * (hy - ly + reach[1]) * (hz - lz + reach[2]) : 0.f;
+#pragma unroll
+    for (int d = 16; d < 64; d <<= 1) c += __shfl_xor(c, d, 64);
+    const bool is_heavy = gaussian_density(tv) * c > kHeavyCandidates;
+    if (lane == 0) {
+        flag[g] = is_heavy ? 1 : 0;
+        if (is_heavy) heavy[1 + atomicAdd(heavy, 1u)] = (uint32_t)g;
+    }
+}
+
+// visit(hit, gi, A = mean|w, B = conic c0..c3, C = c4 c5 v id, power, r0, r1, r2,
+// dx, dy, dz): hit = Gaussian gi's box holds the point and the power is inside
+// the window (the predicate of the shared walk).  Returns false if a lane's
+// stack overflowed (that lane's walk is then incomplete).
+template <class Visit>
+__device__ __forceinline__ bool lane_walk(const GRecord* __restrict__ rec, const BNode* __restrict__ nodes,
+                                          const TreeView& tv, int L, float px, float py, float pz, bool active,
+                                          Visit&& visit)
+{
+    int stk[kLaneStack];
+    int sp = 0, g = 0, gend = 0;
+    bool ok = true;
+    if (active && tv.NL > 0) stk[sp++] = tv.NL == 1 ? ~tv.leaf_base : tv.root;
+    while (true) {
+        const bool more = g < gend || sp > 0;
+        if (__ballot(more) == 0ull) break;
+        bool hit = false;
+        int gi = -1;
+        float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A, C = A;
+        float power = 0.f, r0 = 0.f, r1 = 0.f, r2 = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
+        if (g < gend) {
+            gi = g++;
+            const float4* q = reinterpret_cast<const float4*>(&rec[gi]);
+            const float4 d = q[3], e = q[4];   // lo.xyz hi.x | hi.yz
+            if (d.x <= px && px <= d.w && d.y <= py && py <= e.x && d.z <= pz && pz <= e.y) {
+                A = q[0];
+                B = q[1];
+                C = q[2];
+                dx = px - A.x;
+                dy = py - A.y;
+                dz = pz - A.z;
+                power = pair_power(B.x, B.y, B.z, B.w, C.x, C.y, dx, dy, dz, r0, r1, r2);
+                hit = in_window(power);
+            }
+        } else if (sp > 0) {
+            const int n = stk[--sp];
+            if (n < 0) {
+                g = tv.g_lo + (~n - tv.leaf_base) * L;
+                gend = min(g + L, tv.g_hi);
+            } else {
+                const float4* q = reinterpret_cast<const float4*>(&nodes[n]);
+                const float4 a = q[0], b = q[1], c = q[2], d = q[3];
+                // a = llo.xyz lhi.x | b = lhi.yz rlo.xy | c = rlo.z rhi.xyz | d = left right . .
+                const bool ovl = a.x <= px && px <= a.w && a.y <= py && py <= b.x && a.z <= pz && pz <= b.y;
+                const bool ovr = b.z <= px && px <= c.y && b.w <= py && py <= c.z && c.x <= pz && pz <= c.w;
+                if (sp + 2 > kLaneStack) {
+                    ok = false;
+                    sp = 0;
+                } else {
+                    if (ovr) stk[sp++] = __builtin_bit_cast(int, d.y);
+                    if (ovl) stk[sp++] = __builtin_bit_cast(int, d.x);
+                }
+            }
+        }
+        visit(hit, gi, A, B, C, power, r0, r1, r2, dx, dy, dz);
+    }
+    return ok;
+}
+
+// ---------------------------------------------------------------------------
 // forward: lane = point.  Per chunk of <= 64 candidates: lane j holds candidate
 // j's box and finds the run's points inside it by rank intervals (GroupAxes);
 // a 64x64 bit transpose turns that into each point's candidate mask.  Every
@@ -799,6 +962,7 @@ constexpr uint32_t kSlotBytes = 832;
 constexpr uint32_t kSlotBlock = 8;
 constexpr uint32_t kLogEnd = 0xffffffffu;
 constexpr uint32_t kLogIncomplete = 0xfffffffeu;
+constexpr uint32_t kLogLaneMode = 0xfffffffdu;   // the group was walked per lane: nothing logged
 
 struct LogCursor {
     uint32_t next = 0, end = 0;     // reserved slots [next, end)
@@ -961,16 +1125,20 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
                                                       const unsigned long long* __restrict__ ext_sum,
                                                       int log_on, unsigned char* __restrict__ log, uint32_t log_cap,
                                                       uint32_t* __restrict__ log_counter,
-                                                      uint32_t* __restrict__ group_head, int run_points)
+                                                      uint32_t* __restrict__ group_head, int run_points,
+                                                      int lane_policy, const uint32_t* __restrict__ heavy,
+                                                      uint32_t* __restrict__ heavy_cursor,
+                                                      const uint8_t* __restrict__ group_flag)
 {
     extern __shared__ float4 lds_dyn[];
     const int wid = uniformi(threadIdx.x >> 6), lane = threadIdx.x & 63;
     FwdLds& s = reinterpret_cast<FwdLds*>(lds_dyn)[wid];
     int x = (int)(blockIdx.x & 7u), tries = 0;
+    GroupSched sched{ heavy, heavy_cursor, group_flag, false };
     bool ok = true;
     LogCursor cur;
     while (true) {
-        const int g = next_group(queue, NG, x, tries);
+        const int g = next_group(queue, NG, x, tries, sched);
         if (g >= NG) break;
         const int vol = bv.V == 1 ? 0 : uniformi(volume_of(bv.G_off, bv.V, g));
         const TreeView tv = tree_of(bv, root_box, vol);
@@ -1070,14 +1238,44 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
             nrec += (uint32_t)(__popc(olo) + __popc(ohi));
             lds_fence();
         };
-        uint64_t rs = starts;
-        while (rs != 0ull) {
-            const int a = __builtin_ctzll(rs);
-            rs &= rs - 1;
-            const int b = rs ? __builtin_ctzll(rs) : nv;
-            runmask = (b >= 64 ? ~0ull : ((1ull << b) - 1ull)) & ~((1ull << a) - 1ull);
-            const GroupBox gb = masked_box(px, py, pz, lane >= a && lane < b);
-            ok = enumerate_candidates(rec, nodes, tv, L, gb, s.walk, chunk) && ok;
+        const bool lane_mode = lane_mode_group(tv, px, py, pz, nv, starts, reach, lane_policy);
+        if (lane_mode) {
+            const bool active = orig >= 0 && px == px && py == py && pz == pz;
+            ok = lane_walk(rec, nodes, tv, L, px, py, pz, active,
+                           [&](bool hit, int, const float4& A, const float4&, const float4& C, float power, float,
+                               float, float, float, float, float) {
+                               if (!hit) return;
+                               const float e = __builtin_amdgcn_exp2f(power * kLog2e);
+                               const float wgt = A.w * e;
+                               const float wv = wgt * C.z;
+                               if (KAHAN) {
+                                   float y = wgt - cW;
+                                   float t = W + y;
+                                   cW = (t - W) - y;
+                                   W = t;
+                                   y = wv - cU;
+                                   t = U + y;
+                                   cU = (t - U) - y;
+                                   U = t;
+                               } else {
+                                   W += wgt;
+                                   U += wv;
+                               }
+                               ++npairs;
+                           }) && ok;
+            nrec = wave_sumu(npairs);   // the backward writes one record per pair
+            head = kLogLaneMode;
+            if (lane == 0) atomicAdd(log_counter + 1, 1u);   // lane-walked groups
+        } else {
+            uint64_t rs = starts;
+            while (rs != 0ull) {
+                const int a = __builtin_ctzll(rs);
+                rs &= rs - 1;
+                const int b = rs ? __builtin_ctzll(rs) : nv;
+                runmask = (b >= 64 ? ~0ull : ((1ull << b) - 1ull)) & ~((1ull << a) - 1ull);
+                const GroupBox gb = masked_box(px, py, pz, lane >= a && lane < b);
+                ok = enumerate_candidates(rec, nodes, tv, L, gb, s.walk, chunk) && ok;
+            }
         }
         if (orig >= 0) {
             float V, Wo;
@@ -1185,16 +1383,18 @@ __global__ __launch_bounds__(1024) void backward_kernel(
     const uint32_t* __restrict__ group_offsets, uint64_t capacity, PartialRecord* __restrict__ out_rec,
     uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ err, uint32_t* __restrict__ queue,
     const unsigned long long* __restrict__ ext_sum, const unsigned char* __restrict__ log,
-    const uint32_t* __restrict__ group_head, int run_points)
+    const uint32_t* __restrict__ group_head, int run_points, int lane_policy, const uint32_t* __restrict__ heavy,
+    uint32_t* __restrict__ heavy_cursor, const uint8_t* __restrict__ group_flag)
 {
     extern __shared__ float4 lds_dyn[];
     const int wid = uniformi(threadIdx.x >> 6), lane = threadIdx.x & 63;
     BwdLds<REPLAY>& s = reinterpret_cast<BwdLds<REPLAY>*>(lds_dyn)[wid];
     const uint64_t lt = lanemask_lt();
     int x = (int)(blockIdx.x & 7u), tries = 0;
+    GroupSched sched{ heavy, heavy_cursor, group_flag, false };
     bool ok = true;
     while (true) {
-        const int g = next_group(queue, NG, x, tries);
+        const int g = next_group(queue, NG, x, tries, sched);
         if (g >= NG) break;
         uint32_t head = kLogEnd;
         if (REPLAY) {
@@ -1207,6 +1407,19 @@ __global__ __launch_bounds__(1024) void backward_kernel(
         }
         const float4 pme = pts[(size_t)g * 64 + lane];
         const int orig = __builtin_bit_cast(int, pme.w);
+        const int vol = bv.V == 1 ? 0 : uniformi(volume_of(bv.G_off, bv.V, g));
+        const TreeView tv = tree_of(bv, root_box, vol);
+        float reach[3] = { 0.f, 0.f, 0.f };
+        int nv = 0;
+        uint64_t starts = 0ull;
+        if (!REPLAY) {
+            mean_extent(ext_sum + 4 * vol, reach);
+            nv = __popcll(__ballot(orig >= 0));
+            starts = split_runs(pme.x, pme.y, pme.z, nv, reach, run_points);
+        }
+        const bool lane_mode =
+            REPLAY ? head == kLogLaneMode
+                   : lane_mode_group(tv, pme.x, pme.y, pme.z, nv, starts, reach, lane_policy);
         float V = 0.f, gVi = 0.f, gWe = 0.f;
         if (orig >= 0) {
             const float Wp = out_weights[orig];
@@ -1325,7 +1538,41 @@ __global__ __launch_bounds__(1024) void backward_kernel(
             }
             base_out += (uint32_t)__popcll(bal);
         };
-        if constexpr (REPLAY) {
+        if (lane_mode) {
+            // one record per accepted pair, in step order then lane order
+            const bool active = orig >= 0 && pme.x == pme.x && pme.y == pme.y && pme.z == pme.z;
+            ok = lane_walk(rec, nodes, tv, L, pme.x, pme.y, pme.z, active,
+                           [&](bool hit, int gi, const float4& A, const float4&, const float4& C, float power,
+                               float r0_, float r1_, float r2_, float dx, float dy, float dz) {
+                               const uint64_t bal = __ballot(hit);
+                               if (hit) {
+                                   const uint64_t pos = (uint64_t)base_out + (uint32_t)__popcll(bal & lt);
+                                   if (pos < limit && pos < capacity) {
+                                       const float e = __builtin_amdgcn_exp2f(power * kLog2e);
+                                       const float wgt = A.w * e;
+                                       const float tt = __builtin_fmaf(gVi, C.z - V, gWe);
+                                       const float st = wgt * tt;
+                                       const float dq = -0.5f * st;
+                                       const float h0 = dq * dx, h1 = dq * dy, h2 = dq * dz;
+                                       const float h0x2 = h0 + h0, h1x2 = h1 + h1;
+                                       PartialRecord o;
+                                       o.gid = gi;
+                                       o.dv = gVi * wgt;
+                                       o.dw = tt * e;
+                                       o.dmx = st * r0_; o.dmy = st * r1_; o.dmz = st * r2_;
+                                       o.dc[0] = h0 * dx; o.dc[1] = h0x2 * dy; o.dc[2] = h0x2 * dz;
+                                       o.dc[3] = h1 * dy; o.dc[4] = h1x2 * dz; o.dc[5] = h2 * dz;
+                                       o.pad[0] = o.pad[1] = o.pad[2] = o.pad[3] = 0.f;
+                                       out_rec[pos] = o;
+                                       keys[pos] = (uint32_t)gi;
+                                       vals[pos] = (uint32_t)pos;
+                                   } else {
+                                       atomicOr(err, (uint32_t)GVC_ERRFLAG_RECORD_OVERFLOW);
+                                   }
+                               }
+                               base_out += (uint32_t)__popcll(bal);
+                           }) && ok;
+        } else if constexpr (REPLAY) {
             // replay the group's log; the next slot's header and rows are loaded
             // before the current chunk is processed
             const unsigned char* sb = log + (size_t)head * kSlotBytes;
@@ -1360,11 +1607,6 @@ __global__ __launch_bounds__(1024) void backward_kernel(
             }
         } else {
             // walk the group's runs again and box-test each chunk (lane = candidate)
-            const int nv = __popcll(__ballot(orig >= 0));
-            const int vol = bv.V == 1 ? 0 : uniformi(volume_of(bv.G_off, bv.V, g));
-            float reach[3];
-            mean_extent(ext_sum + 4 * vol, reach);
-            const uint64_t starts = split_runs(pme.x, pme.y, pme.z, nv, reach, run_points);
             const uint64_t finite = build_axes(pme.x, pme.y, pme.z, orig >= 0, s.axes);
             lds_fence();
             uint64_t runmask = 0;
@@ -1388,12 +1630,11 @@ __global__ __launch_bounds__(1024) void backward_kernel(
                 const int b = rs ? __builtin_ctzll(rs) : nv;
                 runmask = (b >= 64 ? ~0ull : ((1ull << b) - 1ull)) & ~((1ull << a) - 1ull);
                 const GroupBox gb = masked_box(pme.x, pme.y, pme.z, lane >= a && lane < b);
-                const TreeView tv = tree_of(bv, root_box, vol);
                 ok = enumerate_candidates(rec, nodes, tv, L, gb, s.walk, chunk) && ok;
             }
         }
     }
-    if (!REPLAY && !ok && lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_STACK_OVERFLOW);
+    if (!ok && lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_STACK_OVERFLOW);
 }
 
 __global__ void segment_kernel(const uint32_t* __restrict__ keys, uint64_t n_max, const uint32_t* __restrict__ n_dev,
@@ -1408,19 +1649,21 @@ __global__ void segment_kernel(const uint32_t* __restrict__ keys, uint64_t n_max
     if (k + 1 == n || keys[k + 1] != key) end[key] = (uint32_t)(k + 1);
 }
 
-// Per Gaussian: FP64 sum of its partial records in group order, then the
-// covariance chain (conic gradient -> Sigma -> M -> scales, rotations), all in
-// double from the float inputs.
-__global__ void reduce_chain_kernel(int P, const GRecord* __restrict__ rec, const uint32_t* __restrict__ begin,
+// Per Gaussian, in the caller's order (coalesced parameter reads and gradient
+// writes): FP64 sum of its partial records in group order, then the covariance
+// chain (conic gradient -> Sigma -> M -> scales, rotations), all in double from
+// the float inputs.
+__global__ void reduce_chain_kernel(int P, const uint32_t* __restrict__ inv_perm, const uint32_t* __restrict__ begin,
                                     const uint32_t* __restrict__ end, const uint32_t* __restrict__ sorted_vals,
                                     const PartialRecord* __restrict__ parts, const float* __restrict__ scales,
                                     const float* __restrict__ rots, float kmod, float* __restrict__ dmeans,
                                     float* __restrict__ dscales, float* __restrict__ drots,
                                     float* __restrict__ dvalues, float* __restrict__ dweights)
 {
-    const int i = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= P) return;
+    const int id = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (id >= P) return;
     double acc[11] = { 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+    const uint32_t i = inv_perm[id];
     const uint32_t b = begin[i], e = end[i];
     // loads run one record ahead of the (sequential, group-ordered) sums
     float4 RA, RB, RC;
@@ -1439,7 +1682,6 @@ __global__ void reduce_chain_kernel(int P, const GRecord* __restrict__ rec, cons
         acc[5] += RB.z; acc[6] += RB.w; acc[7] += RC.x; acc[8] += RC.y; acc[9] += RC.z; acc[10] += RC.w;
         RA = An; RB = Bn; RC = Cn;
     }
-    const int id = rec[i].id;
     dvalues[id] = (float)acc[0];
     dweights[id] = (float)acc[1];
     dmeans[3 * id + 0] = (float)acc[2];
@@ -1459,44 +1701,50 @@ __global__ void zero_floats_kernel(float* p, int n)
     if (i < n) p[i] = 0.f;
 }
 
-__global__ void stats_kernel(const uint32_t* __restrict__ group_pairs, int NG, const uint32_t* __restrict__ offsets,
-                             unsigned long long* __restrict__ out)
+// Sum of the per-group pair counts: each block adds its share to *total (integer
+// atomics: the same total in any order).
+__device__ __forceinline__ void add_group_pairs(const uint32_t* __restrict__ group_pairs, int NG,
+                                                unsigned long long* __restrict__ total)
 {
-    // out[0] = records, out[1] = pairs
     __shared__ unsigned long long part[256];
     unsigned long long s = 0;
-    for (int i = threadIdx.x; i < NG; i += blockDim.x) s += group_pairs[i];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < NG; i += gridDim.x * blockDim.x) s += group_pairs[i];
     part[threadIdx.x] = s;
     __syncthreads();
     for (int d = 128; d >= 1; d >>= 1) {
         if ((int)threadIdx.x < d) part[threadIdx.x] += part[threadIdx.x + d];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        out[0] = offsets[NG];
-        out[1] = part[0];
-    }
+    if (threadIdx.x == 0 && part[0]) atomicAdd(total, part[0]);
 }
 
-// accum[0] += pairs, accum[1] += records, accum[2] |= error flags (one block:
-// plain read-modify-write, stream-ordered, capturable).
-__global__ void accumulate_kernel(const uint32_t* __restrict__ group_pairs, int NG, const uint32_t* __restrict__ offsets,
-                                  const uint32_t* __restrict__ err, unsigned long long* __restrict__ accum)
+// out[0] = records, out[1] = pairs (out[1] cleared by the caller)
+__global__ __launch_bounds__(256) void stats_kernel(const uint32_t* __restrict__ group_pairs, int NG,
+                                                    const uint32_t* __restrict__ offsets,
+                                                    unsigned long long* __restrict__ out)
 {
-    __shared__ unsigned long long part[256];
-    unsigned long long s = 0;
-    for (int i = threadIdx.x; i < NG; i += blockDim.x) s += group_pairs[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int d = 128; d >= 1; d >>= 1) {
-        if ((int)threadIdx.x < d) part[threadIdx.x] += part[threadIdx.x + d];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        accum[0] += part[0];
+    add_group_pairs(group_pairs, NG, out + 1);
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = offsets[NG];
+}
+
+// accum[0] += pairs, accum[1] += records, accum[2] |= error flags
+// (stream-ordered, capturable).
+__global__ __launch_bounds__(256) void accumulate_kernel(const uint32_t* __restrict__ group_pairs, int NG,
+                                                         const uint32_t* __restrict__ offsets,
+                                                         const uint32_t* __restrict__ err,
+                                                         unsigned long long* __restrict__ accum)
+{
+    add_group_pairs(group_pairs, NG, accum);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         accum[1] += offsets[NG];
         accum[2] |= *err;
     }
+}
+
+inline unsigned reduce_blocks(int NG)
+{
+    const long long b = ((long long)NG + 1023) / 1024;
+    return (unsigned)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
 }
 
 inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
@@ -1539,6 +1787,10 @@ int allow_lds(const void* kernel, size_t bytes)
 }
 
 int run_points_of(const gvc_params* prm) { return prm->run_points ? (int)prm->run_points : 64; }
+int lane_policy_of(const gvc_params* prm)
+{
+    return (prm->flags & GVC_FLAG_LANE_ALWAYS) ? 2 : ((prm->flags & GVC_FLAG_LANE_NEVER) ? 1 : 0);
+}
 
 inline int key_bits_for(uint64_t maxv)
 {
@@ -1592,6 +1844,7 @@ struct ForwardLayout {
     GRecord* rec;
     uint64_t *codes, *codes_alt, *leaf_codes;
     uint32_t *idx, *idx_alt, *vkeys, *pos, *sort_scratch;
+    uint32_t* inv_perm;  // caller's Gaussian index -> record (BVH) index
     BNode* nodes;
     Box6* sparse;  // levels x NL leaf-range boxes
     float* root_box;  // 6 per volume
@@ -1599,6 +1852,8 @@ struct ForwardLayout {
     uint32_t* queues;  // 8 forward + 8 backward group queues, log counter
     unsigned long long *stats, *ext_sum;
     uint32_t* group_head;
+    uint32_t* heavy;        // [0] = count, [1..] = heavy groups (walked first)
+    uint8_t* group_flag;    // 1 = heavy
     unsigned char* log;
     uint32_t log_cap;
     int NL, NG;
@@ -1615,6 +1870,7 @@ struct ForwardLayout {
         idx_alt = a.take<uint32_t>((size_t)P);
         vkeys = a.take<uint32_t>((size_t)P);
         pos = a.take<uint32_t>((size_t)P);
+        inv_perm = a.take<uint32_t>((size_t)P);
         sort_scratch = a.take<uint32_t>(radix_scratch_elems((size_t)P));
         leaf_codes = a.take<uint64_t>((size_t)NL);
         nodes = a.take<BNode>((size_t)(b.NI > 0 ? b.NI : 1));
@@ -1629,6 +1885,8 @@ struct ForwardLayout {
         stats = a.take<unsigned long long>(4);
         ext_sum = a.take<unsigned long long>(4 * (size_t)b.V);
         group_head = a.take<uint32_t>((size_t)NG + 1);
+        heavy = a.take<uint32_t>((size_t)NG + 1);
+        group_flag = reinterpret_cast<uint8_t*>(a.take<uint32_t>(((size_t)NG + 3) / 4 + 1));
         const uint64_t cap = (flags & GVC_FLAG_CANDIDATE_LOG)
                                  ? (uint64_t)(chunk_slots ? chunk_slots : 16u) * (uint64_t)(NG > 0 ? NG : 1)
                                  : 1ull;
@@ -1663,6 +1921,8 @@ int check_params(const gvc_params* prm)
         return set_error(GVC_ERR_INVALID_ARGUMENT, "block_threads must be a multiple of 64 in [64, 1024]");
     if (prm->run_points != 0 && prm->run_points != 16 && prm->run_points != 32 && prm->run_points != 64)
         return set_error(GVC_ERR_INVALID_ARGUMENT, "run_points must be 0, 16, 32 or 64");
+    if ((prm->flags & GVC_FLAG_LANE_NEVER) && (prm->flags & GVC_FLAG_LANE_ALWAYS))
+        return set_error(GVC_ERR_INVALID_ARGUMENT, "GVC_FLAG_LANE_NEVER and GVC_FLAG_LANE_ALWAYS are exclusive");
     return GVC_OK;
 }
 
@@ -1792,7 +2052,7 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
         }
         // 2. records
         record_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, perm, means, scales, rots, values, weights, k, fp64,
-                                                          l.rec, conics);
+                                                          l.rec, conics, l.inv_perm);
         GVC_LAUNCH_CHECK("record");
         extent_kernel<<<bt.V, 256, 0, st>>>(l.rec, bv, l.ext_sum);
         GVC_LAUNCH_CHECK("extent");
@@ -1820,6 +2080,13 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
         }
         leaf_root_kernel<<<blocks_for(bt.V, 64), 64, 0, st>>>(l.sparse, bv, l.root_box);
         GVC_LAUNCH_CHECK("root_box");
+        // 4b. heavy groups, walked first
+        GVC_TRY(fill_u32(l.heavy, 0u, 1, st));
+        if (NG > 0) {
+            group_cost_kernel<<<blocks_for((size_t)NG, 4), 256, 0, st>>>(pts, NG, bv, l.root_box, l.ext_sum, l.heavy,
+                                                                        l.group_flag);
+            GVC_LAUNCH_CHECK("group_cost");
+        }
     }
     // 5. walk
     if ((stages & GVC_STAGE_WALK) && NG > 0) {
@@ -1828,6 +2095,7 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
         GVC_TRY(fill_u32(l.queues + 16, 0u, 4, st));
         const int log_on = (prm->flags & GVC_FLAG_CANDIDATE_LOG) ? 1 : 0;
         const int rp = run_points_of(prm);
+        const int lp = lane_policy_of(prm);
         const size_t lds = (size_t)(T / 64) * sizeof(FwdLds);
         if (prm->flags & GVC_FLAG_KAHAN) {
             const void* kf = reinterpret_cast<const void*>(&forward_kernel<true>);
@@ -1836,7 +2104,7 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
             forward_kernel<true><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
                                                      out_weights, l.group_records, l.group_pairs, l.err, l.queues,
                                                      l.ext_sum, log_on, l.log, l.log_cap, l.queues + 16,
-                                                     l.group_head, rp);
+                                                     l.group_head, rp, lp, l.heavy, l.queues + 18, l.group_flag);
         } else {
             const void* kf = reinterpret_cast<const void*>(&forward_kernel<false>);
             GVC_TRY(allow_lds(kf, lds));
@@ -1844,7 +2112,7 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
             forward_kernel<false><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
                                                       out_weights, l.group_records, l.group_pairs, l.err, l.queues,
                                                       l.ext_sum, log_on, l.log, l.log_cap, l.queues + 16,
-                                                      l.group_head, rp);
+                                                      l.group_head, rp, lp, l.heavy, l.queues + 18, l.group_flag);
         }
         GVC_LAUNCH_CHECK("forward");
     }
@@ -1855,23 +2123,26 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
 }
 
 int forward_stats_impl(const Batch& bt, const void* state, const gvc_params* prm, uint64_t* n_records,
-                       uint64_t* n_pairs, uint32_t* error_flags, uint64_t* chunk_slots_used, hipStream_t st)
+                       uint64_t* n_pairs, uint32_t* error_flags, uint64_t* chunk_slots_used, uint64_t* lane_groups,
+                       hipStream_t st)
 {
     GVC_TRY(check_params(prm));
     Arena a(const_cast<void*>(state), (size_t)-1);
     ForwardLayout l(a, bt, prm->flags, prm->chunk_slots);
-    stats_kernel<<<1, 256, 0, st>>>(l.group_pairs, l.NG, l.group_offsets, l.stats);
+    GVC_TRY(fill_u32(reinterpret_cast<uint32_t*>(l.stats), 0u, 4, st));
+    stats_kernel<<<reduce_blocks(l.NG), 256, 0, st>>>(l.group_pairs, l.NG, l.group_offsets, l.stats);
     GVC_LAUNCH_CHECK("stats");
     unsigned long long h[2] = { 0, 0 };
-    uint32_t e = 0, used = 0;
+    uint32_t e = 0, used[2] = { 0, 0 };
     GVC_HIP(hipMemcpyAsync(h, l.stats, sizeof(h), hipMemcpyDeviceToHost, st));
     GVC_HIP(hipMemcpyAsync(&e, l.err, sizeof(e), hipMemcpyDeviceToHost, st));
-    GVC_HIP(hipMemcpyAsync(&used, l.queues + 16, sizeof(used), hipMemcpyDeviceToHost, st));
+    GVC_HIP(hipMemcpyAsync(used, l.queues + 16, sizeof(used), hipMemcpyDeviceToHost, st));
     GVC_HIP(hipStreamSynchronize(st));
     if (n_records) *n_records = h[0];
     if (n_pairs) *n_pairs = h[1];
     if (error_flags) *error_flags = e;
-    if (chunk_slots_used) *chunk_slots_used = used;
+    if (chunk_slots_used) *chunk_slots_used = used[0];
+    if (lane_groups) *lane_groups = used[1];
     return GVC_OK;
 }
 
@@ -1881,7 +2152,7 @@ int forward_accumulate_impl(const Batch& bt, const void* state, const gvc_params
     GVC_TRY(check_params(prm));
     Arena a(const_cast<void*>(state), (size_t)-1);
     ForwardLayout l(a, bt, prm->flags, prm->chunk_slots);
-    accumulate_kernel<<<1, 256, 0, st>>>(l.group_pairs, l.NG, l.group_offsets, l.err, accum);
+    accumulate_kernel<<<reduce_blocks(l.NG), 256, 0, st>>>(l.group_pairs, l.NG, l.group_offsets, l.err, accum);
     GVC_LAUNCH_CHECK("accumulate");
     return GVC_OK;
 }
@@ -1910,7 +2181,9 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
     if ((stages & GVC_STAGE_WALK) && NG > 0 && capacity > 0) {
         const int T = prm->block_threads;
         GVC_TRY(fill_u32(l.queues + 8, 0u, 8, st));
+        GVC_TRY(fill_u32(l.queues + 19, 0u, 1, st));
         const int rp = run_points_of(prm);
+        const int lp = lane_policy_of(prm);
         if (prm->flags & GVC_FLAG_CANDIDATE_LOG) {
             const void* kf = reinterpret_cast<const void*>(&backward_kernel<true>);
             const size_t lds = (size_t)(T / 64) * sizeof(BwdLds<true>);
@@ -1919,7 +2192,8 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
             backward_kernel<true><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
                                                       out_weights, gcells, gweights, l.group_offsets, capacity,
                                                       bl.parts, bl.keys, bl.vals, l.err, l.queues + 8, l.ext_sum,
-                                                      l.log, l.group_head, rp);
+                                                      l.log, l.group_head, rp, lp, l.heavy, l.queues + 19,
+                                                      l.group_flag);
         } else {
             const void* kf = reinterpret_cast<const void*>(&backward_kernel<false>);
             const size_t lds = (size_t)(T / 64) * sizeof(BwdLds<false>);
@@ -1928,7 +2202,8 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
             backward_kernel<false><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
                                                        out_weights, gcells, gweights, l.group_offsets, capacity,
                                                        bl.parts, bl.keys, bl.vals, l.err, l.queues + 8, l.ext_sum,
-                                                       l.log, l.group_head, rp);
+                                                       l.log, l.group_head, rp, lp, l.heavy, l.queues + 19,
+                                                       l.group_flag);
         }
         GVC_LAUNCH_CHECK("backward");
     }
@@ -1948,8 +2223,8 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
                                                                   bl.begin, bl.end);
         GVC_LAUNCH_CHECK("segment");
     }
-    reduce_chain_kernel<<<blocks_for(P, 128), 128, 0, st>>>(P, l.rec, bl.begin, bl.end, svals, bl.parts, scales, rots,
-                                                            k, dmeans, dscales, drots, dvalues, dweights);
+    reduce_chain_kernel<<<blocks_for(P, 128), 128, 0, st>>>(P, l.inv_perm, bl.begin, bl.end, svals, bl.parts, scales,
+                                                            rots, k, dmeans, dscales, drots, dvalues, dweights);
     GVC_LAUNCH_CHECK("reduce_chain");
     return GVC_OK;
 }
@@ -2006,11 +2281,12 @@ int mixture_forward(int P, const float* means, const float* scales, const float*
 }
 
 int forward_stats(const void* state, int P, int S, const gvc_params* prm, uint64_t* n_records, uint64_t* n_pairs,
-                  uint32_t* error_flags, uint64_t* chunk_slots_used, hipStream_t st)
+                  uint32_t* error_flags, uint64_t* chunk_slots_used, uint64_t* lane_groups,
+                       hipStream_t st)
 {
     GVC_TRY(check_params(prm));
     return forward_stats_impl(single_batch(P, S, prm->leaf_size, nullptr), state, prm, n_records, n_pairs,
-                              error_flags, chunk_slots_used, st);
+                              error_flags, chunk_slots_used, lane_groups, st);
 }
 
 int forward_accumulate(const void* state, int P, int S, const gvc_params* prm, unsigned long long* accum,
@@ -2111,9 +2387,11 @@ int mixture_forward_batched(const gvc_batch* b, const float* means, const float*
 }
 
 int forward_stats_batched(const gvc_batch* b, const void* state, const gvc_params* prm, uint64_t* n_records,
-                          uint64_t* n_pairs, uint32_t* error_flags, uint64_t* chunk_slots_used, hipStream_t st)
+                          uint64_t* n_pairs, uint32_t* error_flags, uint64_t* chunk_slots_used, uint64_t* lane_groups,
+                       hipStream_t st)
 {
-    return forward_stats_impl(from_c(b), state, prm, n_records, n_pairs, error_flags, chunk_slots_used, st);
+    return forward_stats_impl(from_c(b), state, prm, n_records, n_pairs, error_flags, chunk_slots_used, lane_groups,
+                              st);
 }
 
 int forward_accumulate_batched(const gvc_batch* b, const void* state, const gvc_params* prm,

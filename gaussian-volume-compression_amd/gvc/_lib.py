@@ -15,6 +15,8 @@ GVC_FLAG_CANDIDATE_LOG = 4
 GVC_ERRFLAG_STACK_OVERFLOW = 1
 GVC_ERRFLAG_RECORD_OVERFLOW = 2
 GVC_ERRFLAG_CHUNK_OVERFLOW = 4
+GVC_FLAG_LANE_NEVER = 8
+GVC_FLAG_LANE_ALWAYS = 16
 GVC_STAGE_BUILD = 1
 GVC_STAGE_WALK = 2
 GVC_STAGE_COUNTS = 4
@@ -62,7 +64,7 @@ _SIGS = {
     "gvc_forward_state_bytes": (_SZ, [_I, _I, _PP]),
     "gvc_mixture_forward": (_I, [_I, _P, _P, _P, _P, _P, _F, _P, _I, _PP, _P, _P, _P, _P, _SZ, _P]),
     "gvc_forward_stats": (_I, [_P, _I, _I, _PP, ctypes.POINTER(_U64), ctypes.POINTER(_U64),
-                               ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_U64), _P]),
+                               ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_U64), ctypes.POINTER(_U64), _P]),
     "gvc_backward_bytes": (_SZ, [_I, _U64]),
     "gvc_forward_accumulate": (_I, [_P, _I, _I, _PP, _P, _P]),
     "gvc_mixture_backward": (_I, [_I, _P, _P, _P, _P, _P, _F, _P, _I, _PP, _P, _P, _P, _P, _P, _SZ, _U64,
@@ -81,7 +83,8 @@ _SIGS = {
     "gvc_forward_state_bytes_batched": (_SZ, [_PB, _PP]),
     "gvc_mixture_forward_batched": (_I, [_PB, _P, _P, _P, _P, _P, _F, _P, _PP, _P, _P, _P, _P, _SZ, _P]),
     "gvc_forward_stats_batched": (_I, [_PB, _P, _PP, ctypes.POINTER(_U64), ctypes.POINTER(_U64),
-                                       ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_U64), _P]),
+                                       ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_U64), ctypes.POINTER(_U64),
+                                       _P]),
     "gvc_forward_accumulate_batched": (_I, [_PB, _P, _PP, _P, _P]),
     "gvc_mixture_backward_batched": (_I, [_PB, _P, _P, _F, _P, _PP, _P, _P, _P, _P, _P, _SZ, _U64, _P, _SZ,
                                           _P, _P, _P, _P, _P, _P]),

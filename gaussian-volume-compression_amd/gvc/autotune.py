@@ -4,9 +4,10 @@ Sweeps BVH leaf size x points walked together (run_points: the wavefront's
 point grouping) x block shape on the caller's own data, times a forward +
 backward per configuration with HIP events, and keeps the fastest.  Results
 are cached per (log2 P, log2 S) bucket in memory and, if ``GVC_TUNE_CACHE``
-names a file, in that JSON file.  Every configuration computes the same
-answer (the pair predicate and the reduction order do not depend on them), so
-tuning never changes results.
+names a file, in that JSON file.  Every configuration accepts exactly the
+same (Gaussian, point) pairs; the order in which a point's compensated sum
+visits its Gaussians follows the traversal, so fields and gradients of two
+configurations agree to the accuracy of the compensated sums, not bit for bit.
 """
 import json
 import math
@@ -72,7 +73,8 @@ def _sweep(make_run, base, leaf_sizes, run_points, blocks, reps):
         for rp in run_points:
             for bt in blocks:
                 prm = mx.MixtureParams(leaf_size=L, block_threads=bt, run_points=rp, kahan=base.kahan,
-                                       fp64_cond=base.fp64_cond, candidate_log=base.candidate_log)
+                                       fp64_cond=base.fp64_cond, candidate_log=base.candidate_log,
+                                       lane_mode=base.lane_mode)
                 label = f"leaf{L}/run{rp}/block{bt}"
                 try:
                     ms = _time(make_run(prm), reps)
@@ -96,7 +98,7 @@ def _cached(key, base, use_cache):
     if disk:
         best = mx.MixtureParams(leaf_size=disk["leaf_size"], block_threads=disk["block_threads"],
                                 run_points=disk["run_points"], kahan=base.kahan, fp64_cond=base.fp64_cond,
-                                candidate_log=base.candidate_log)
+                                candidate_log=base.candidate_log, lane_mode=base.lane_mode)
         _cache[key] = best
         return best
     return None

@@ -78,17 +78,21 @@ class BatchedForwardState:
         self.P = batch.P
         self.S = batch.S
         self.chunk_slots_used = 0
+        self.lane_groups = 0
 
     def stats(self):
-        """(n_records, n_pairs, error_flags) over the whole batch; synchronises."""
+        """(n_records, n_pairs, error_flags) over the whole batch; synchronises;
+        also sets ``chunk_slots_used`` and ``lane_groups``."""
         nr = ctypes.c_uint64(0)
         npairs = ctypes.c_uint64(0)
         err = ctypes.c_uint32(0)
         used = ctypes.c_uint64(0)
+        lanes = ctypes.c_uint64(0)
         call("gvc_forward_stats_batched", ctypes.byref(self.batch.c), _ptr(self.buffer), ctypes.byref(self._c),
-             ctypes.byref(nr), ctypes.byref(npairs), ctypes.byref(err), ctypes.byref(used),
+             ctypes.byref(nr), ctypes.byref(npairs), ctypes.byref(err), ctypes.byref(used), ctypes.byref(lanes),
              _stream(self.buffer.device))
         self.chunk_slots_used = int(used.value)
+        self.lane_groups = int(lanes.value)
         return int(nr.value), int(npairs.value), int(err.value)
 
     def accumulate(self, accum):
@@ -144,7 +148,7 @@ def mixture_forward_batched(batch, means3D, scales, rotations, values, weights, 
     if sample_set.batch is not batch:
         raise ValueError("the sample set was built for another batch")
     inputs = _check_inputs(batch, means3D, scales, rotations, values, weights)
-    return _forward(batch, params.c(), params, inputs, scale_modifier, sample_set, want_conics)
+    return _forward(batch, params.c(batch.c.groups), params, inputs, scale_modifier, sample_set, want_conics)
 
 
 def mixture_backward_batched(fstate, means3D, scales, rotations, values, weights, scale_modifier, sample_set,
@@ -167,17 +171,15 @@ def mixture_backward_batched(fstate, means3D, scales, rotations, values, weights
     gw = torch.empty((P, 1), dtype=torch.float32, device=dev)
     cap = record_capacity if record_capacity is not None else mx._record_capacity
     if cap is None:
-        cap, _, err = fstate.stats()
-        if err & _lib.GVC_ERRFLAG_CHUNK_OVERFLOW:
-            mx._grow_chunk_slots(fstate)
-            cp2 = fstate.params.c()
-            cp2.chunk_slots = max(int(cp2.chunk_slots), mx._chunk_slots)
-            scratch = (torch.empty(S, dtype=torch.float32, device=dev), torch.empty(S, dtype=torch.float32, device=dev))
-            _, _, _, fstate = _forward(batch, cp2, fstate.params, inputs, scale_modifier, sample_set, False,
-                                       outputs=scratch)
-            cap, _, err = fstate.stats()
-            if err & _lib.GVC_ERRFLAG_CHUNK_OVERFLOW:
-                raise _lib.GvcError("candidate log still too small after growing it")
+        scratch = []
+
+        def rerun(cp2):
+            if not scratch:
+                scratch.extend(torch.empty(S, dtype=torch.float32, device=dev) for _ in range(2))
+            return _forward(batch, cp2, fstate.params, inputs, scale_modifier, sample_set, False,
+                            outputs=tuple(scratch))[3]
+
+        fstate, cap = mx._redo_until_log_fits(fstate, batch.c.groups, rerun)
     cp = fstate._c
     nbytes = lib().gvc_backward_bytes(P, cap)
     ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)

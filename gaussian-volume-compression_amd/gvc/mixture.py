@@ -23,7 +23,10 @@ class MixtureParams:
     candidate_log: the forward keeps the candidate chunks it visited and the
     backward replays them; False re-walks the tree in the backward (no
     candidate storage).  chunk_slots: candidate-log slots per 64-point group;
-    0 = the module's high-water mark, which grows whenever a forward ran out."""
+    0 = the module's high-water mark for that number of groups, which grows
+    whenever a forward ran out.  lane_mode: "auto" walks a group per lane (each
+    lane its own point) where its points are sparse next to its Gaussians;
+    "never" / "always" force one way for every group."""
     leaf_size: int = DEFAULT_LEAF_SIZE
     block_threads: int = DEFAULT_BLOCK_THREADS
     kahan: bool = True
@@ -31,15 +34,18 @@ class MixtureParams:
     candidate_log: bool = True
     chunk_slots: int = 0
     run_points: int = 64
+    lane_mode: str = "auto"
 
-    def c(self):
+    def c(self, groups=0):
         flags = ((GVC_FLAG_KAHAN if self.kahan else 0) | (GVC_FLAG_FP64_COND if self.fp64_cond else 0) |
-                 (GVC_FLAG_CANDIDATE_LOG if self.candidate_log else 0))
+                 (GVC_FLAG_CANDIDATE_LOG if self.candidate_log else 0) |
+                 {"auto": 0, "never": _lib.GVC_FLAG_LANE_NEVER, "always": _lib.GVC_FLAG_LANE_ALWAYS}[self.lane_mode])
         return gvc_params(int(self.leaf_size), int(self.block_threads), flags, 0,
-                          int(self.chunk_slots or _chunk_slots), int(self.run_points))
+                          int(self.chunk_slots or chunk_slots_for(groups)), int(self.run_points))
 
 
-_chunk_slots = 16          # candidate-log slots per group used when params.chunk_slots == 0
+DEFAULT_CHUNK_SLOTS = 16
+_chunk_slots = {}          # bit length of the group count -> candidate-log slots per group (high-water mark)
 _default_params = MixtureParams()
 _record_capacity = None   # fixed backward capacity (graph capture); None = read it
 _stage_timer = None       # list collecting (stage, start_event, end_event) when set
@@ -139,17 +145,21 @@ class ForwardState:
         self.S = S
         self._c = cparams
         self.chunk_slots_used = 0
+        self.lane_groups = 0
 
     def stats(self):
         """(n_records, n_pairs, error_flags) -- synchronises with the host; also
-        sets ``chunk_slots_used``."""
+        sets ``chunk_slots_used`` and ``lane_groups`` (groups walked per lane)."""
         nr = ctypes.c_uint64(0)
         npairs = ctypes.c_uint64(0)
         err = ctypes.c_uint32(0)
         used = ctypes.c_uint64(0)
+        lanes = ctypes.c_uint64(0)
         call("gvc_forward_stats", _ptr(self.buffer), self.P, self.S, ctypes.byref(self._c), ctypes.byref(nr),
-             ctypes.byref(npairs), ctypes.byref(err), ctypes.byref(used), _stream(self.buffer.device))
+             ctypes.byref(npairs), ctypes.byref(err), ctypes.byref(used), ctypes.byref(lanes),
+             _stream(self.buffer.device))
         self.chunk_slots_used = int(used.value)
+        self.lane_groups = int(lanes.value)
         return int(nr.value), int(npairs.value), int(err.value)
 
     def accumulate(self, accum):
@@ -159,12 +169,38 @@ class ForwardState:
              _stream(self.buffer.device))
 
 
-def _grow_chunk_slots(fstate):
-    """Raises the module's candidate-log high-water mark to fit ``fstate``'s forward."""
-    global _chunk_slots
-    NG = max(1, (fstate.S + 63) // 64)
+def chunk_slots_for(groups):
+    """Candidate-log slots per group used for a forward over ``groups`` groups
+    when params.chunk_slots is 0."""
+    return _chunk_slots.get(max(int(groups), 1).bit_length(), DEFAULT_CHUNK_SLOTS)
+
+
+def _grow_chunk_slots(fstate, groups):
+    """After a forward whose candidate log ran out: a larger slots-per-group
+    value (at least double), also kept as the high-water mark for this many
+    groups.  The slots counted by an overflowing forward stop where groups gave
+    up logging, so the count is a lower bound: hence the doubling."""
+    NG = max(1, int(groups))
     need = (fstate.chunk_slots_used + NG - 1) // NG
-    _chunk_slots = max(_chunk_slots, need + need // 4 + 2)
+    new = max(2 * int(fstate._c.chunk_slots), need + need // 4 + 2)
+    key = NG.bit_length()
+    _chunk_slots[key] = max(_chunk_slots.get(key, DEFAULT_CHUNK_SLOTS), new)
+    return new
+
+
+def _redo_until_log_fits(fstate, groups, rerun, attempts=12):
+    """Returns (state, n_records) of a forward whose candidate log held every
+    chunk: ``fstate`` itself, or the state of rerun(cparams) with a grown log
+    (same inputs, same results)."""
+    for _ in range(attempts):
+        cap, _, err = fstate.stats()
+        if not err & _lib.GVC_ERRFLAG_CHUNK_OVERFLOW:
+            return fstate, cap
+        slots = _grow_chunk_slots(fstate, groups)
+        cp2 = fstate.params.c(groups)
+        cp2.chunk_slots = max(int(cp2.chunk_slots), slots)
+        fstate = rerun(cp2)
+    raise _lib.GvcError("candidate log still too small after growing it")
 
 
 def mixture_forward(means3D, scales, rotations, values, weights, scale_modifier, sample_set, params=None,
@@ -184,7 +220,7 @@ def mixture_forward(means3D, scales, rotations, values, weights, scale_modifier,
     if sample_set.device != dev:
         raise ValueError("sample set and Gaussians are on different devices")
     S = sample_set.S
-    cp = params.c()
+    cp = params.c((S + 63) // 64)
     return _forward(P, S, dev, cp, params, means3D, scales, rotations, values, weights, scale_modifier, sample_set,
                     want_conics)
 
@@ -238,19 +274,17 @@ def mixture_backward(fstate, means3D, scales, rotations, values, weights, scale_
         return gm, gs, gr, gv, gw
     cap = record_capacity if record_capacity is not None else _record_capacity
     if cap is None:
-        cap, _, err = fstate.stats()
-        if err & _lib.GVC_ERRFLAG_CHUNK_OVERFLOW:
-            # the candidate log was too small: redo the forward (same inputs, same
-            # results) into a larger state; the outputs go to scratch tensors
-            _grow_chunk_slots(fstate)
-            cp2 = fstate.params.c()
-            cp2.chunk_slots = max(int(cp2.chunk_slots), _chunk_slots)
-            scratch = (torch.empty(S, dtype=torch.float32, device=dev), torch.empty(S, dtype=torch.float32, device=dev))
-            _, _, _, fstate = _forward(P, S, dev, cp2, fstate.params, means3D, scales, rotations, values, weights,
-                                       scale_modifier, sample_set, False, outputs=scratch)
-            cap, _, err = fstate.stats()
-            if err & _lib.GVC_ERRFLAG_CHUNK_OVERFLOW:
-                raise _lib.GvcError("candidate log still too small after growing it")
+        # a forward whose candidate log was too small is redone (same inputs,
+        # same results) into a larger state; its outputs go to scratch tensors
+        scratch = []
+
+        def rerun(cp2):
+            if not scratch:
+                scratch.extend(torch.empty(S, dtype=torch.float32, device=dev) for _ in range(2))
+            return _forward(P, S, dev, cp2, fstate.params, means3D, scales, rotations, values, weights,
+                            scale_modifier, sample_set, False, outputs=tuple(scratch))[3]
+
+        fstate, cap = _redo_until_log_fits(fstate, (S + 63) // 64, rerun)
     cp = fstate._c
     nbytes = lib().gvc_backward_bytes(P, cap)
     ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)

@@ -41,6 +41,12 @@ enum gvc_status {
 #define GVC_FLAG_FP64_COND 2u  /* FP64 covariance inverse where cond(Sigma) > 1e6 */
 #define GVC_FLAG_CANDIDATE_LOG 4u  /* forward keeps a candidate log that the backward
                                      * replays; clear: the backward walks the tree again */
+/* How a 64-point group is walked.  Default: the group shares one walk of its
+ * box, except where its points are sparse next to its Gaussians (expected
+ * candidates of the shared walk above 8192), where every lane walks the tree for
+ * its own point.  These two flags force one way for every group (not both). */
+#define GVC_FLAG_LANE_NEVER 8u
+#define GVC_FLAG_LANE_ALWAYS 16u
 
 /* gvc_params.stages: 0 runs every stage of a call; otherwise only the selected
  * ones, in order, so a caller can bracket each stage with events. */
@@ -120,9 +126,11 @@ int gvc_mixture_forward(int P, const float *means3D, const float *scales, const 
  * backward will emit; n_pairs = accepted (Gaussian, point) pairs; error_flags =
  * GVC_ERRFLAG_* raised so far on this state; chunk_slots_used = candidate-log
  * slots the forward asked for (a chunk_slots of ceil(used / groups) suffices
- * for the same inputs).  Any pointer may be NULL. */
+ * for the same inputs; a lower bound when the log overflowed); lane_groups =
+ * groups walked per lane.  Any pointer may be NULL. */
 int gvc_forward_stats(const void *state, int P, int S, const gvc_params *params, uint64_t *n_records,
-                      uint64_t *n_pairs, uint32_t *error_flags, uint64_t *chunk_slots_used, void *stream);
+                      uint64_t *n_pairs, uint32_t *error_flags, uint64_t *chunk_slots_used,
+                      uint64_t *lane_groups, void *stream);
 
 /* Stream-ordered, non-blocking counterpart of gvc_forward_stats for timed or
  * graph-captured loops: accum (3 x uint64 on the device) gets accum[0] += pairs,
@@ -230,7 +238,7 @@ int gvc_mixture_forward_batched(const gvc_batch *batch, const float *means3D, co
                                 size_t state_bytes, void *stream);
 int gvc_forward_stats_batched(const gvc_batch *batch, const void *state, const gvc_params *params,
                               uint64_t *n_records, uint64_t *n_pairs, uint32_t *error_flags,
-                              uint64_t *chunk_slots_used, void *stream);
+                              uint64_t *chunk_slots_used, uint64_t *lane_groups, void *stream);
 int gvc_forward_accumulate_batched(const gvc_batch *batch, const void *state, const gvc_params *params,
                                    uint64_t *accum, void *stream);
 int gvc_mixture_backward_batched(const gvc_batch *batch, const float *scales, const float *rotations,

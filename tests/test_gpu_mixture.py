@@ -102,11 +102,12 @@ def run_gpu(means, scales, rots, values, w, samples, params, k=1.0):
     return ss, cells, cw, conics, st
 
 
-@pytest.mark.parametrize("leaf", [1, 2, 4, 8, 16])
-def test_forward_config0_parity(leaf):
+@pytest.mark.parametrize("leaf,lane", [(1, "auto"), (2, "auto"), (4, "auto"), (8, "auto"), (16, "auto"),
+                                       (1, "always"), (4, "always"), (16, "always"), (4, "never")])
+def test_forward_config0_parity(leaf, lane):
     g = workloads.config0(0)
     scales = np.exp(g["log_scales"]).astype(f32)
-    params = mx.MixtureParams(leaf_size=leaf, block_threads=256)
+    params = mx.MixtureParams(leaf_size=leaf, block_threads=256, lane_mode=lane)
     ss, cells, cw, conics, st = run_gpu(g["means"], scales, g["rotations"], g["values"], g["weights"],
                                         g["samples"], params)
     V, W = cells.cpu().numpy(), cw.cpu().numpy()
@@ -127,8 +128,8 @@ def test_forward_edge_cases():
     samples = np.concatenate([rng.uniform(-0.05, 1.05, (10007, 3)), means[:40] ,
                               [[np.nan, 0.5, 0.5], [0.5, 0.5, 0.5]]]).astype(f32)
     samples[-2] = [np.nan, 0.5, 0.5]
-    for leaf in (1, 3, 16):
-        params = mx.MixtureParams(leaf_size=leaf)
+    for leaf, lane in ((1, "auto"), (3, "auto"), (16, "auto"), (3, "always"), (16, "always")):
+        params = mx.MixtureParams(leaf_size=leaf, lane_mode=lane)
         ss, cells, cw, conics, st = run_gpu(means, scales, rots, values, w, samples, params)
         conic_ref, box_ref, _ = oracle.preprocess(means, scales, rots, w)
         Vr, Wr, cnt = oracle.forward(means, values, w, conic_ref, box_ref, samples)
@@ -189,13 +190,65 @@ def _grad_check(g, scales, samples, params, tol=1e-4, seed=0):
     return grads
 
 
-@pytest.mark.parametrize("leaf,log", [(1, True), (4, True), (16, True), (1, False), (4, False)])
-def test_backward_config0_parity(leaf, log):
+@pytest.mark.parametrize("leaf,log,lane", [(1, True, "auto"), (4, True, "auto"), (16, True, "auto"),
+                                           (1, False, "auto"), (4, False, "auto"), (4, True, "always"),
+                                           (16, False, "always"), (2, True, "never")])
+def test_backward_config0_parity(leaf, log, lane):
     g = workloads.config0(0)
     scales = np.exp(g["log_scales"]).astype(f32)
     # subsample points so the FP64 brute-force oracle stays fast
     samples = g["samples"][::4].copy()
-    _grad_check(g, scales, samples, mx.MixtureParams(leaf_size=leaf, candidate_log=log))
+    _grad_check(g, scales, samples, mx.MixtureParams(leaf_size=leaf, candidate_log=log, lane_mode=lane))
+
+
+def sparse_and_dense(seed=12, P=40000, dense=1500, sparse=300):
+    """Small Gaussians over the unit cube; points in one dense cluster (groups
+    walked together) and scattered thinly (groups walked per lane)."""
+    rng = np.random.default_rng(seed)
+    g = workloads.random_gaussians(rng, P)
+    scales = (np.exp(g["log_scales"]) * 0.3).astype(f32)
+    samples = np.concatenate([rng.uniform(0.40, 0.45, (dense, 3)), rng.uniform(0.0, 1.0, (sparse, 3))]).astype(f32)
+    return g, scales, samples
+
+
+def test_lane_mode_sparse_groups():
+    """Automatic lane mode: the pair set is the same under every policy, fields
+    and gradients meet the oracle bars, replay and re-walk agree bit for bit, and
+    lane-walked groups write one backward record per pair."""
+    g, scales, samples = sparse_and_dense()
+    conic_ref, box_ref, _ = oracle.preprocess(g["means"], scales, g["rotations"], g["weights"])
+    Vr, Wr, cnt = oracle.forward(g["means"], g["values"], g["weights"], conic_ref, box_ref, samples)
+    recs, lanes = {}, {}
+    NG = (samples.shape[0] + 63) // 64
+    for lane in ("never", "auto", "always"):
+        ss, cells, cw, conics, st = run_gpu(g["means"], scales, g["rotations"], g["values"], g["weights"], samples,
+                                            mx.MixtureParams(lane_mode=lane))
+        nrec, npairs, err = st.stats()
+        # walked together, the sparse groups outgrow the default candidate log
+        # (the eager backward redoes such a forward with a larger one)
+        assert (err & ~mx._lib.GVC_ERRFLAG_CHUNK_OVERFLOW) == 0 and npairs == int(cnt.sum())
+        V, W = cells.cpu().numpy(), cw.cpu().numpy()
+        assert np.array_equal(W == 0, Wr == 0)
+        assert field_err(V, W, Vr, Wr) <= 1e-5
+        recs[lane], lanes[lane] = nrec, st.lane_groups
+    assert recs["always"] == int(cnt.sum())            # per lane: one record per pair
+    assert lanes["never"] == 0 and lanes["always"] == NG
+    assert 0 < lanes["auto"] < NG                      # only the sparse groups are walked per lane
+    for lane in ("auto", "always"):
+        _grad_check(g, scales, samples, mx.MixtureParams(lane_mode=lane))
+        rng = np.random.default_rng(3)
+        gV = T(rng.normal(size=samples.shape[0]))
+        gW = T(rng.normal(size=samples.shape[0]))
+        outs = []
+        for log in (True, False):
+            ss, cells, cw, conics, st = run_gpu(g["means"], scales, g["rotations"], g["values"], g["weights"],
+                                                samples, mx.MixtureParams(lane_mode=lane, candidate_log=log))
+            grads = mx.mixture_backward(st, T(g["means"]), T(scales), T(g["rotations"]), T(g["values"]),
+                                        T(g["weights"]), 1.0, ss, cells, cw, gV, gW)
+            assert st.stats()[2] == 0
+            outs.append([cells.cpu().numpy()] + [t.cpu().numpy() for t in grads])
+        for a, b in zip(*outs):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
 def test_backward_replay_equals_rewalk():
@@ -265,7 +318,7 @@ def test_candidate_log_overflow_recovers():
     rng = np.random.default_rng(8)
     gV = T(rng.normal(size=samples.shape[0]))
     gW = T(rng.normal(size=samples.shape[0]))
-    saved = mx._chunk_slots
+    saved = dict(mx._chunk_slots)
     try:
         outs = []
         for slots in (64, 1):
@@ -285,7 +338,8 @@ def test_candidate_log_overflow_recovers():
                             ss, cells, cw, gV, gW, record_capacity=1 << 20)
         assert st.stats()[2] & mx._lib.GVC_ERRFLAG_CHUNK_OVERFLOW
     finally:
-        mx._chunk_slots = saved
+        mx._chunk_slots.clear()
+        mx._chunk_slots.update(saved)
 
 
 @pytest.mark.parametrize("leaf,block,run", [(4, 512, 32), (8, 1024, 16), (2, 256, 16)])
