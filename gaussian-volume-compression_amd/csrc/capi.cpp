@@ -91,6 +91,8 @@ int tet_locate_interp(int n_points, const float* pts, int C, const void* loc, si
                       double* out_values, float* bary, hipStream_t st);
 int grid_sample(const long long dims[3], const float origin[3], const float spacing[3], int dtype, const void* values,
                 int S, const float* samples, void* out, hipStream_t st);
+int cell_interp(int S, const float* samples, const int* cell_ids, const uint8_t* types, const int* offsets,
+                const int* conn, const float* pts, const double* values, double* out, hipStream_t st);
 size_t knn3_bytes(int n);
 int knn3(int n, const float* pts, float* out, void* ws, size_t bytes, hipStream_t st);
 
@@ -263,6 +265,19 @@ int gvc_tet_locate_interp_set(int n_points, const float* points, int n_tets, con
     GVC_REQUIRE(!values_out || point_values, "values_out needs point_values");
     return tet_locate_interp(n_points, points, n_tets, locator, locator_bytes, entry_capacity, point_values, S,
                              nullptr, sample_set, cell_ids, values_out, bary_out, (hipStream_t)stream);
+}
+
+int gvc_cell_interp(int S, const float* samples, const int32_t* cell_ids, const uint8_t* cell_types,
+                    const int32_t* cell_offsets, const int32_t* connectivity, const float* points,
+                    const double* point_values, double* values_out, void* stream)
+{
+    GVC_REQUIRE(S >= 0, "S < 0");
+    if (S == 0) return GVC_OK;
+    GVC_REQUIRE(samples && cell_ids && cell_types && cell_offsets && connectivity && points && point_values &&
+                    values_out,
+                "NULL pointer");
+    return cell_interp(S, samples, cell_ids, cell_types, cell_offsets, connectivity, points, point_values, values_out,
+                       (hipStream_t)stream);
 }
 
 int gvc_grid_sample(const int64_t* dims, const float* origin, const float* spacing, int value_dtype,

@@ -439,6 +439,118 @@ void grid_dims(int C, const float lo[3], const float hi[3], int dims[3], float c
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Non-tetrahedral cells of a mixed mesh (gpu_sampleu over VTK hexahedra,
+// wedges and pyramids).  The cell holding a sample is found by the tetrahedral
+// locator on the cells' tetrahedral splits; its value is the cell's own
+// interpolant: the VTK shape functions N_i at the parametric coordinates of the
+// sample (the isoparametric map inverted by Newton's method in double, from
+// the cell centre), value = sum N_i v_i.  Shape functions, parametric (r, s, t):
+//   hexahedron (12), points 0-3 bottom, 4-7 top:
+//     (1-r)(1-s)(1-t), r(1-s)(1-t), rs(1-t), (1-r)s(1-t), (1-r)(1-s)t, r(1-s)t, rst, (1-r)st
+//   wedge (13), points 0-2 one triangle, 3-5 the other:
+//     (1-r-s)(1-t), r(1-t), s(1-t), (1-r-s)t, rt, st
+//   pyramid (14), points 0-3 base, 4 apex:
+//     (1-r)(1-s)(1-t), r(1-s)(1-t), rs(1-t), (1-r)s(1-t), t
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int shape_functions(int type, double r, double s, double t, double N[8], double dN[8][3])
+{
+    const double rm = 1.0 - r, sm = 1.0 - s, tm = 1.0 - t;
+    if (type == 12) {
+        const double a[8] = { rm * sm, r * sm, r * s, rm * s, rm * sm, r * sm, r * s, rm * s };
+        const double dr[8] = { -sm, sm, s, -s, -sm, sm, s, -s };
+        const double ds[8] = { -rm, -r, r, rm, -rm, -r, r, rm };
+        for (int i = 0; i < 8; ++i) {
+            const double w = i < 4 ? tm : t, dw = i < 4 ? -1.0 : 1.0;
+            N[i] = a[i] * w;
+            dN[i][0] = dr[i] * w;
+            dN[i][1] = ds[i] * w;
+            dN[i][2] = a[i] * dw;
+        }
+        return 8;
+    }
+    if (type == 13) {
+        const double a[3] = { 1.0 - r - s, r, s };
+        const double dr[3] = { -1.0, 1.0, 0.0 };
+        const double ds[3] = { -1.0, 0.0, 1.0 };
+        for (int i = 0; i < 6; ++i) {
+            const int k = i % 3;
+            const double w = i < 3 ? tm : t, dw = i < 3 ? -1.0 : 1.0;
+            N[i] = a[k] * w;
+            dN[i][0] = dr[k] * w;
+            dN[i][1] = ds[k] * w;
+            dN[i][2] = a[k] * dw;
+        }
+        return 6;
+    }
+    // pyramid
+    const double a[4] = { rm * sm, r * sm, r * s, rm * s };
+    const double dr[4] = { -sm, sm, s, -s };
+    const double ds[4] = { -rm, -r, r, rm };
+    for (int i = 0; i < 4; ++i) {
+        N[i] = a[i] * tm;
+        dN[i][0] = dr[i] * tm;
+        dN[i][1] = ds[i] * tm;
+        dN[i][2] = -a[i];
+    }
+    N[4] = t;
+    dN[4][0] = 0.0;
+    dN[4][1] = 0.0;
+    dN[4][2] = 1.0;
+    return 5;
+}
+
+__global__ void cell_interp_kernel(int S, const float* __restrict__ samples, const int* __restrict__ cell,
+                                   const uint8_t* __restrict__ types, const int* __restrict__ offsets,
+                                   const int* __restrict__ conn, const float* __restrict__ pts,
+                                   const double* __restrict__ values, double* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S) return;
+    const int c = cell[i];
+    if (c < 0) return;
+    const int type = types[c];
+    if (type != 12 && type != 13 && type != 14) return;
+    const int* ids = conn + offsets[c];
+    const int nv = type == 12 ? 8 : (type == 13 ? 6 : 5);
+    double X[8][3];
+    for (int k = 0; k < nv; ++k)
+        for (int a = 0; a < 3; ++a) X[k][a] = (double)pts[3 * ids[k] + a];
+    const double p[3] = { (double)samples[3 * i], (double)samples[3 * i + 1], (double)samples[3 * i + 2] };
+    double xi[3] = { 0.5, 0.5, 0.5 };
+    if (type == 13) xi[0] = xi[1] = 1.0 / 3.0;
+    if (type == 14) xi[2] = 0.2;
+    double N[8], dN[8][3];
+    for (int it = 0; it < 32; ++it) {
+        shape_functions(type, xi[0], xi[1], xi[2], N, dN);
+        double F[3] = { -p[0], -p[1], -p[2] }, J[3][3] = { { 0, 0, 0 }, { 0, 0, 0 }, { 0, 0, 0 } };
+        for (int k = 0; k < nv; ++k)
+            for (int a = 0; a < 3; ++a) {
+                F[a] += N[k] * X[k][a];
+                for (int b = 0; b < 3; ++b) J[a][b] += dN[k][b] * X[k][a];
+            }
+        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
+                     c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+        if (det == 0.0 || det != det) break;
+        const double inv = 1.0 / det;
+        const double d0 = (F[0] * c00 + F[1] * (J[0][2] * J[2][1] - J[0][1] * J[2][2]) +
+                           F[2] * (J[0][1] * J[1][2] - J[0][2] * J[1][1])) * inv;
+        const double d1 = (F[0] * c01 + F[1] * (J[0][0] * J[2][2] - J[0][2] * J[2][0]) +
+                           F[2] * (J[0][2] * J[1][0] - J[0][0] * J[1][2])) * inv;
+        const double d2 = (F[0] * c02 + F[1] * (J[0][1] * J[2][0] - J[0][0] * J[2][1]) +
+                           F[2] * (J[0][0] * J[1][1] - J[0][1] * J[1][0])) * inv;
+        xi[0] -= d0;
+        xi[1] -= d1;
+        xi[2] -= d2;
+        if (fabs(d0) + fabs(d1) + fabs(d2) < 1e-14) break;
+    }
+    shape_functions(type, xi[0], xi[1], xi[2], N, dN);
+    double v = 0.0;
+    for (int k = 0; k < nv; ++k) v += N[k] * values[ids[k]];
+    out[i] = v;
+}
 }  // namespace
 
 size_t tet_locator_bytes(int n_points, int C, unsigned long long cap)
@@ -495,6 +607,16 @@ int tet_locator_build(int n_points, const float* pts, int C, const int* tets, un
     H.n_entries = total;
     GVC_HIP(hipMemcpyAsync(l.header, &H, sizeof(H), hipMemcpyHostToDevice, st));
     GVC_HIP(hipStreamSynchronize(st));
+    return GVC_OK;
+}
+
+int cell_interp(int S, const float* samples, const int* cell_ids, const uint8_t* types, const int* offsets,
+                const int* conn, const float* pts, const double* values, double* out, hipStream_t st)
+{
+    if (S <= 0) return GVC_OK;
+    cell_interp_kernel<<<blocks_for(S, 256), 256, 0, st>>>(S, samples, cell_ids, types, offsets, conn, pts, values,
+                                                           out);
+    GVC_LAUNCH_CHECK("cell_interp");
     return GVC_OK;
 }
 

@@ -8,15 +8,16 @@ size mismatch, an unsupported values dtype or an unsupported cell type.  Nothing
 is printed.  ``TetMeshSampler`` is the device-resident form for a training loop:
 it keeps the locator and also returns int32 cell ids.
 
-Cell types: tetrahedra (VTK 10) are located exactly; other types of the
-reference's table (GS/sample_mesh.cu:300-336: 1, 3, 5, 9, 12, 13, 14) raise.
+Cell types: the reference's table (GS/sample_mesh.cu:300-336).  Tetrahedra (VTK
+10) are located exactly and interpolated by barycentrics; hexahedra (12), wedges
+(13) and pyramids (14) are located through tetrahedral splits (exact for planar
+faces) and interpolated by their VTK shape functions; vertices, lines, triangles
+and quads (1, 3, 5, 9) have no volume and hold no sample.  Any other type raises.
 """
 import numpy as np
 import torch
 
-from gvc.sampling import TetMeshSampler, grid_sample  # noqa: F401
-
-_VTK_KNOWN = {1, 3, 5, 9, 10, 12, 13, 14}
+from gvc.sampling import CellMeshSampler, TetMeshSampler, grid_sample  # noqa: F401
 
 
 def _device():
@@ -42,34 +43,14 @@ def gpu_sample(dims, origin, spacing, values, samples):
     return out if as_torch else out.cpu().numpy()
 
 
-def _tets_from_vtk(conn, cell_types, cell_offsets):
-    types = np.asarray(cell_types).astype(np.int64).reshape(-1)
-    C = types.shape[0]
-    conn = np.asarray(conn).astype(np.int64).reshape(-1)
-    offs = np.asarray(cell_offsets).astype(np.int64).reshape(-1)
-    bad = types[~np.isin(types, list(_VTK_KNOWN))]
-    if bad.size:
-        raise RuntimeError(f"Unsupported VTK cell type: {int(bad[0])}")
-    other = types[types != 10]
-    if other.size:
-        raise RuntimeError(f"VTK cell type {int(other[0])} is not supported by this build (tetrahedra only)")
-    start = offs[:C]
-    end = np.where(np.arange(C) + 1 < C, np.append(offs[1:C], 0), conn.shape[0])
-    if C and not np.all(end - start == 4):
-        raise RuntimeError("tetrahedra must have 4 point ids")
-    idx = start[:, None] + np.arange(4)[None, :]
-    return conn[idx]
-
-
 def gpu_sampleu(pts, conn, cell_types, cell_offsets, values, samples):
     """GS/sample_mesh.cu:242-414: probe an unstructured mesh at the samples;
     float64 values, -1 outside."""
     as_torch = isinstance(samples, torch.Tensor)
     dev = samples.device if as_torch and samples.is_cuda else _device()
-    tets = _tets_from_vtk(conn, cell_types, cell_offsets)
     p = pts if isinstance(pts, torch.Tensor) else np.ascontiguousarray(pts, dtype=np.float32)
     v = values if isinstance(values, torch.Tensor) else np.ascontiguousarray(values, dtype=np.float64)
-    sampler = TetMeshSampler(p, tets, v, device=dev)
+    sampler = CellMeshSampler(p, conn, cell_types, cell_offsets, v, device=dev)
     s = samples if as_torch else torch.from_numpy(np.ascontiguousarray(samples, dtype=np.float32))
     out, _ = sampler.sample(s.to(dev))
     return out if as_torch else out.cpu().numpy()

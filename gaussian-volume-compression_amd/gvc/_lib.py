@@ -76,6 +76,7 @@ _SIGS = {
     "gvc_grid_sample": (_I, [_P, _P, _P, _I, _P, _I, _P, _P, _P]),
     "gvc_knn3_bytes": (_SZ, [_I]),
     "gvc_knn3": (_I, [_I, _P, _P, _P, _SZ, _P]),
+    "gvc_cell_interp": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "gvc_batch_table_bytes": (_SZ, [_I]),
     "gvc_batch_describe": (_I, [_I, _P, _P, _I, _P, _SZ, _PB, _P]),
     "gvc_sample_set_bytes_batched": (_SZ, [_PB]),

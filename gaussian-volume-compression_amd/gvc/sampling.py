@@ -83,6 +83,105 @@ class TetMeshSampler:
         return vals, ids
 
 
+# Tetrahedral splits used to locate the other 3D VTK cells (local point indices).
+# Planar faces make the split exact; a point on a shared face goes to one cell.
+_SPLITS = {
+    12: np.array([[0, 1, 3, 4], [1, 2, 3, 6], [1, 3, 4, 6], [1, 4, 5, 6], [3, 4, 6, 7]], np.int64),
+    13: np.array([[0, 1, 2, 3], [1, 2, 3, 4], [2, 3, 4, 5]], np.int64),
+    14: np.array([[0, 1, 2, 4], [0, 2, 3, 4]], np.int64),
+}
+_NPTS = {1: 1, 3: 2, 5: 3, 9: 4, 10: 4, 12: 8, 13: 6, 14: 5}
+VTK_CELL_TYPES = frozenset(_NPTS)
+
+
+def split_cells(conn, cell_types, cell_offsets):
+    """VTK cells -> (tets [T,4] int64, parent cell [T] int64).  Tetrahedra map to
+    themselves; hexahedra, wedges and pyramids to tetrahedral splits; vertices,
+    lines, triangles and quads (no volume) to nothing.  Raises RuntimeError for a
+    type outside the reference's table or a cell with the wrong point count."""
+    types = np.asarray(cell_types).astype(np.int64).reshape(-1)
+    conn = np.asarray(conn).astype(np.int64).reshape(-1)
+    offs = np.asarray(cell_offsets).astype(np.int64).reshape(-1)
+    C = types.shape[0]
+    bad = types[~np.isin(types, list(_NPTS))]
+    if bad.size:
+        raise RuntimeError(f"Unsupported VTK cell type: {int(bad[0])}")
+    start = offs[:C]
+    end = np.append(offs[1:C], conn.shape[0]) if offs.shape[0] <= C else offs[1:C + 1]
+    count = end - start
+    want = np.vectorize(_NPTS.get, otypes=[np.int64])(types) if C else np.zeros(0, np.int64)
+    if C and not np.array_equal(count, want):
+        k = int(np.nonzero(count != want)[0][0])
+        raise RuntimeError(f"cell {k} of VTK type {int(types[k])} has {int(count[k])} point ids, expected {int(want[k])}")
+    tets, parent = [], []
+    for t, local in ((10, np.arange(4)[None, :]),) + tuple(_SPLITS.items()):
+        cells = np.nonzero(types == t)[0]
+        if cells.size == 0:
+            continue
+        ids = conn[start[cells][:, None, None] + local[None, :, :]]          # [n, k, 4]
+        tets.append(ids.reshape(-1, 4))
+        parent.append(np.repeat(cells, local.shape[0]))
+    if not tets:
+        return np.zeros((0, 4), np.int64), np.zeros(0, np.int64)
+    tets = np.concatenate(tets)
+    parent = np.concatenate(parent)
+    order = np.argsort(parent, kind="stable")      # tets in cell order: lower cell ids win ties
+    return tets[order], parent[order]
+
+
+class CellMeshSampler:
+    """Device-resident mixed VTK mesh (types 1, 3, 5, 9, 10, 12, 13, 14) with its
+    locator: the reference's gpu_sampleu over Viskores Probe (GS/sample_mesh.cu:
+    242-414).  A sample is located in the cells' tetrahedral splits; tetrahedra
+    interpolate by exact barycentrics, hexahedra / wedges / pyramids by their VTK
+    shape functions at the sample's parametric coordinates.  Cells without volume
+    never hold a sample.  ``sample(samples)`` -> (values [S] float64 with -1
+    outside, cell ids [S] int32 with -1 outside)."""
+
+    def __init__(self, points, conn, cell_types, cell_offsets, values=None, device="cuda"):
+        tets, parent = split_cells(conn, cell_types, cell_offsets)
+        self.tet_sampler = TetMeshSampler(points, tets, values, device=device)
+        dev = self.tet_sampler.device
+        self.device = dev
+        types = np.asarray(cell_types).astype(np.int64).reshape(-1)
+        self.mixed = bool(np.any(np.isin(types, (12, 13, 14))))
+        self.parent = torch.as_tensor(parent, dtype=torch.int32).to(dev)
+        self.types = torch.as_tensor(types.astype(np.uint8)).to(dev)
+        self.offsets = torch.as_tensor(np.asarray(cell_offsets).astype(np.int64).reshape(-1)[:types.shape[0]]
+                                       .astype(np.int32)).to(dev)
+        self.conn = torch.as_tensor(np.asarray(conn).astype(np.int64).reshape(-1).astype(np.int32)).to(dev)
+
+    def sample(self, samples, with_values=True):
+        s = torch.as_tensor(samples, dtype=torch.float32).to(self.device).contiguous()
+        vals, tet_ids = self.tet_sampler.sample(s, with_values=with_values)
+        ids = torch.where(tet_ids >= 0, self.parent[tet_ids.clamp_min(0).long()], tet_ids).contiguous()
+        if with_values and vals is not None and self.mixed:
+            t = self.tet_sampler
+            call("gvc_cell_interp", int(s.shape[0]), _ptr(s), _ptr(ids), _ptr(self.types), _ptr(self.offsets),
+                 _ptr(self.conn), _ptr(t.points), _ptr(t.values), _ptr(vals), _stream(self.device))
+        return vals, ids
+
+
+def sample_mesh_points(points, tets, num_batches, batch_size, generator=None, return_cells=False):
+    """R/train.py:34-61: random points inside a tetrahedral mesh -- a uniformly
+    chosen tetrahedron, barycentrics (u0, u1 - u0, u2 - u1, 1 - u2) from three
+    sorted uniforms, the point their combination of its vertices.  Returns
+    [num_batches, batch_size, 3] float32 (and, with return_cells, the chosen
+    tetrahedron per point [num_batches, batch_size] int64: the cell that holds
+    the point by construction)."""
+    pts = torch.as_tensor(points, dtype=torch.float32)
+    cells = torch.as_tensor(tets, dtype=torch.long).to(pts.device)
+    total = int(num_batches) * int(batch_size)
+    idx = torch.randint(0, cells.shape[0], (total,), device=pts.device, generator=generator)
+    verts = pts[cells[idx]]
+    u = torch.rand(total, 3, device=pts.device, generator=generator).sort(dim=1).values
+    bary = torch.stack([u[:, 0], u[:, 1] - u[:, 0], u[:, 2] - u[:, 1], 1.0 - u[:, 2]], dim=1)
+    out = (bary.unsqueeze(2) * verts).sum(dim=1).reshape(num_batches, batch_size, 3)
+    if return_cells:
+        return out, idx.reshape(num_batches, batch_size)
+    return out
+
+
 def grid_sample(dims, origin, spacing, values, samples):
     """Trilinear probe of a uniform point grid on the GPU (torch tensors in/out)."""
     dev = samples.device

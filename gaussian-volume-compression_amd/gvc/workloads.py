@@ -278,3 +278,52 @@ def batch_workload(device, seed=0, volumes=32, total_points=1 << 24, point_batch
                point_batches=[torch.cat(x).contiguous() for x in pts],
                ground_truth=[torch.cat(g).contiguous() for g in gts])
     return cat
+
+
+def mixed_cell_mesh(n=4, seed=0):
+    """A VTK mesh of mixed 3D cells with planar faces: an n^3-cube lattice under a
+    fixed affine map, each cube (uniformly at random) one hexahedron (12), two
+    wedges (13), six pyramids around an added centre point (14) or six Kuhn
+    tetrahedra (10).  Returns (points [N,3] float32, conn int64, types int64,
+    offsets int64 [C] (start of each cell in conn))."""
+    rng = np.random.default_rng(seed)
+    g = np.arange(n + 1, dtype=np.float64) / n
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    base = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    A = np.array([[1.0, 0.15, -0.1], [0.05, 0.9, 0.2], [-0.1, 0.1, 1.1]])
+    b = np.array([0.05, -0.02, 0.03])
+    pts = [base @ A.T + b]
+
+    def vid(i, j, k):
+        return (i * (n + 1) + j) * (n + 1) + k
+
+    conn, types = [], []
+    extra = 0
+    N0 = base.shape[0]
+    kuhn = [[0, 1, 3, 7], [0, 1, 5, 7], [0, 2, 3, 7], [0, 2, 6, 7], [0, 4, 5, 7], [0, 4, 6, 7]]
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                # binary corner order c = dx + 2 dy + 4 dz, and VTK hexahedron order
+                c = [vid(i + (q & 1), j + ((q >> 1) & 1), k + ((q >> 2) & 1)) for q in range(8)]
+                hexa = [c[0], c[1], c[3], c[2], c[4], c[5], c[7], c[6]]
+                kind = int(rng.integers(0, 4))
+                if kind == 0:
+                    conn.append(hexa); types.append(12)
+                elif kind == 1:
+                    conn.append([hexa[0], hexa[1], hexa[2], hexa[4], hexa[5], hexa[6]]); types.append(13)
+                    conn.append([hexa[0], hexa[2], hexa[3], hexa[4], hexa[6], hexa[7]]); types.append(13)
+                elif kind == 2:
+                    centre = N0 + extra
+                    extra += 1
+                    pts.append((np.mean(base[c], axis=0) @ A.T + b)[None, :])
+                    for face in ([0, 1, 2, 3], [4, 5, 6, 7], [0, 1, 5, 4], [3, 2, 6, 7], [0, 3, 7, 4],
+                                 [1, 2, 6, 5]):
+                        conn.append([hexa[f] for f in face] + [centre]); types.append(14)
+                else:
+                    for t in kuhn:
+                        conn.append([c[q] for q in t]); types.append(10)
+    flat = np.concatenate([np.asarray(x, np.int64) for x in conn])
+    lens = np.array([len(x) for x in conn], np.int64)
+    offsets = np.concatenate([[0], np.cumsum(lens)[:-1]])
+    return np.concatenate(pts).astype(np.float32), flat, np.asarray(types, np.int64), offsets

@@ -191,6 +191,18 @@ int gvc_tet_locate_interp_set(int n_points, const float *points, int n_tets, con
                               const void *sample_set, int S, int32_t *cell_ids, double *values_out,
                               float *bary_out, void *stream);
 
+/* Mixed meshes (the rest of sample_meshu's cell table, GS/sample_mesh.cu:300-336,
+ * interpolated like Viskores Probe): for each sample whose cell (cell_ids[i], e.g.
+ * the parent of the tetrahedron gvc_tet_locate_interp found in the cells'
+ * tetrahedral splits) is a VTK hexahedron (12), wedge (13) or pyramid (14),
+ * values_out[i] = sum_k N_k(r,s,t) point_values[conn[cell_offsets[c] + k]] with
+ * the VTK shape functions at the sample's parametric coordinates (Newton's
+ * method in double).  Other samples are left untouched.  cell_types uint8 [C],
+ * cell_offsets int32 [C], connectivity int32. */
+int gvc_cell_interp(int S, const float *samples, const int32_t *cell_ids, const uint8_t *cell_types,
+                    const int32_t *cell_offsets, const int32_t *connectivity, const float *points,
+                    const double *point_values, double *values_out, void *stream);
+
 /* Uniform point grid (dims nx,ny,nz; origin; spacing -- HOST arrays), trilinear
  * interpolation, -1 outside cast to the value type (all-ones bits for the
  * unsigned types).  value_dtype: 0 f64, 1 f32, 2 u16, 3 u8; out has that type.

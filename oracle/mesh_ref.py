@@ -155,3 +155,133 @@ def knn3(points):
             b = np.append(b, np.float32(np.inf))
         out[i] = np.float32((b[0] + b[1]) + b[2]) / np.float32(3.0)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Mixed VTK cells (GS/sample_mesh.cu:300-336 cell table, Viskores Probe
+# semantics): a sample is in a cell iff its parametric coordinates lie in the
+# cell's parametric domain (tetrahedra: barycentrics >= 0; hexahedra: [0,1]^3;
+# wedges: r, s >= 0, r + s <= 1, 0 <= t <= 1; pyramids: [0,1]^3), within a
+# tolerance; the lowest containing cell id wins; the value is the cell's
+# interpolant sum_k N_k v_k.  Parametric coordinates by Newton's method on the
+# isoparametric map (numpy, float64, vectorised over candidate pairs).  This is
+# an independent restatement: it decides containment in parametric space, not
+# through the tetrahedral splits the GPU locates with.  Pinned by: exact
+# reproduction of linear fields (the isoparametric completeness property), by
+# cells known by construction, and on pure-tetrahedron meshes by the exact
+# orientation oracle above.
+# ---------------------------------------------------------------------------
+_HEX_CORNERS = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]],
+                        np.float64)
+
+
+def _shape(kind, xi):
+    """N [m, k] and dN/dxi [m, k, 3] for parametric points xi [m, 3]."""
+    r, s, t = xi[:, 0], xi[:, 1], xi[:, 2]
+    one = np.ones_like(r)
+    if kind == 12:
+        # tensor-product linear: N_k = prod_a (c_ka xi_a + (1 - c_ka)(1 - xi_a))
+        c = _HEX_CORNERS[None, :, :]
+        f = c * xi[:, None, :] + (1 - c) * (1 - xi[:, None, :])          # [m, 8, 3]
+        df = 2 * c - 1
+        N = f.prod(axis=2)
+        dN = np.stack([df[..., a] * np.prod(np.delete(f, a, axis=2), axis=2) for a in range(3)], axis=2)
+        return N, dN
+    if kind == 13:
+        tri = np.stack([1 - r - s, r, s], 1)
+        dtri = np.array([[-1.0, -1.0], [1.0, 0.0], [0.0, 1.0]])
+        N = np.concatenate([tri * (1 - t)[:, None], tri * t[:, None]], 1)
+        dN = np.zeros((xi.shape[0], 6, 3))
+        for k in range(3):
+            for h, w, dw in ((0, 1 - t, -one), (1, t, one)):
+                dN[:, 3 * h + k, 0] = dtri[k, 0] * w
+                dN[:, 3 * h + k, 1] = dtri[k, 1] * w
+                dN[:, 3 * h + k, 2] = tri[:, k] * dw
+        return N, dN
+    if kind == 14:
+        quad = np.stack([(1 - r) * (1 - s), r * (1 - s), r * s, (1 - r) * s], 1)
+        dq_r = np.stack([-(1 - s), 1 - s, s, -s], 1)
+        dq_s = np.stack([-(1 - r), -r, r, 1 - r], 1)
+        N = np.concatenate([quad * (1 - t)[:, None], t[:, None]], 1)
+        dN = np.zeros((xi.shape[0], 5, 3))
+        dN[:, :4, 0] = dq_r * (1 - t)[:, None]
+        dN[:, :4, 1] = dq_s * (1 - t)[:, None]
+        dN[:, :4, 2] = -quad
+        dN[:, 4, 2] = 1.0
+        return N, dN
+    raise ValueError(kind)
+
+
+def _inside(kind, xi, tol):
+    r, s, t = xi[:, 0], xi[:, 1], xi[:, 2]
+    if kind == 13:
+        return (r >= -tol) & (s >= -tol) & (r + s <= 1 + tol) & (t >= -tol) & (t <= 1 + tol)
+    return np.all((xi >= -tol) & (xi <= 1 + tol), axis=1)
+
+
+def _parametric(kind, X, p, iters=60):
+    """Newton on sum_k N_k(xi) X_k = p; X [m, k, 3], p [m, 3] -> xi [m, 3]."""
+    xi = np.tile({12: [0.5, 0.5, 0.5], 13: [1 / 3, 1 / 3, 0.5], 14: [0.5, 0.5, 0.2]}[kind], (p.shape[0], 1))
+    for _ in range(iters):
+        N, dN = _shape(kind, xi)
+        F = np.einsum("mk,mka->ma", N, X) - p
+        J = np.einsum("mkb,mka->mab", dN, X)
+        ok = np.abs(np.linalg.det(J)) > 1e-300
+        step = np.zeros_like(xi)
+        step[ok] = np.linalg.solve(J[ok], F[ok][..., None])[..., 0]
+        xi = xi - step
+        if np.max(np.abs(step), initial=0.0) < 1e-15:
+            break
+    return xi
+
+
+def locate_cells(pts, conn, cell_types, cell_offsets, samples, values=None, tol=1e-9):
+    """(cell ids [S] int32, values [S] float64 or None, parametric margin [S]):
+    margin = how far inside the winning cell's domain the sample lies (small =
+    near a face, where two cells may both claim it)."""
+    P = np.asarray(pts, np.float64)
+    conn = np.asarray(conn, np.int64).reshape(-1)
+    types = np.asarray(cell_types, np.int64).reshape(-1)
+    offs = np.asarray(cell_offsets, np.int64).reshape(-1)[:types.shape[0]]
+    smp = np.asarray(samples, np.float64)
+    S = smp.shape[0]
+    npts = {10: 4, 12: 8, 13: 6, 14: 5}
+    best = np.full(S, np.iinfo(np.int64).max)
+    val = np.full(S, -1.0)
+    margin = np.zeros(S)
+    for kind, k in npts.items():
+        cells = np.nonzero(types == kind)[0]
+        if cells.size == 0:
+            continue
+        ids = conn[offs[cells][:, None] + np.arange(k)[None, :]]
+        X = P[ids]                                             # [c, k, 3]
+        lo, hi = X.min(axis=1), X.max(axis=1)
+        inbox = np.all((smp[:, None, :] >= lo[None] - 1e-12) & (smp[:, None, :] <= hi[None] + 1e-12), axis=2)
+        si, ci = np.nonzero(inbox)
+        if si.size == 0:
+            continue
+        Xp, pp = X[ci], smp[si]
+        if kind == 10:
+            M = np.stack([Xp[:, 1] - Xp[:, 0], Xp[:, 2] - Xp[:, 0], Xp[:, 3] - Xp[:, 0]], 2)
+            lam = np.linalg.solve(M, (pp - Xp[:, 0])[..., None])[..., 0]
+            bary = np.concatenate([1 - lam.sum(1, keepdims=True), lam], 1)
+            ins = np.all(bary >= -tol, axis=1)
+            N = bary
+            mg = bary.min(axis=1)
+        else:
+            xi = _parametric(kind, Xp, pp)
+            ins = _inside(kind, xi, tol)
+            N, _ = _shape(kind, xi)
+            if kind == 13:
+                mg = np.minimum.reduce([xi[:, 0], xi[:, 1], 1 - xi[:, 0] - xi[:, 1], xi[:, 2], 1 - xi[:, 2]])
+            else:
+                mg = np.minimum(xi, 1 - xi).min(axis=1)
+        for q in np.nonzero(ins)[0]:
+            s_, c_ = si[q], cells[ci[q]]
+            if c_ < best[s_]:
+                best[s_] = c_
+                margin[s_] = mg[q]
+                if values is not None:
+                    val[s_] = float(np.dot(N[q], np.asarray(values, np.float64)[ids[ci[q]]]))
+    out_ids = np.where(best == np.iinfo(np.int64).max, -1, best).astype(np.int32)
+    return out_ids, (val if values is not None else None), margin

@@ -116,3 +116,67 @@ def test_distcuda2_bit_exact():
         got = distCUDA2(torch.tensor(P, device=DEV)).cpu().numpy()
         ref = mesh_ref.knn3(P)
         assert np.array_equal(got, ref)
+
+
+def test_mixed_cells_against_oracle():
+    """Hexahedra, wedges, pyramids and tetrahedra (planar faces): the GPU picks the
+    oracle's cell wherever the oracle's choice is clear of a face, interpolates
+    within 1e-9 of it, and reproduces a linear field exactly."""
+    from gvc.sampling import CellMeshSampler
+    p, c, t, o = workloads.mixed_cell_mesh(4, 2)
+    rng = np.random.default_rng(11)
+    smp = rng.uniform(-0.2, 1.3, (6000, 3)).astype(f32)
+    vals = rng.normal(size=p.shape[0])
+    sampler = CellMeshSampler(p, c, t, o, vals, device=DEV)
+    gv, gid = sampler.sample(torch.tensor(smp, device=DEV))
+    gv, gid = gv.cpu().numpy(), gid.cpu().numpy()
+    ids, ref, margin = mesh_ref.locate_cells(p, c, t, o, smp, vals)
+    clear = (margin > 1e-6) | (ids < 0)
+    assert clear.mean() > 0.99
+    assert np.array_equal(gid[clear], ids[clear])
+    ok = clear & (ids >= 0)
+    np.testing.assert_allclose(gv[ok], ref[ok], rtol=1e-9, atol=1e-12)
+    assert np.all(gv[gid < 0] == -1.0)
+    # linear field: exact up to rounding for every located sample, clear or not
+    a = np.array([0.3, -1.2, 0.7])
+    lin = CellMeshSampler(p, c, t, o, p.astype(np.float64) @ a + 0.25, device=DEV)
+    lv, lid = lin.sample(torch.tensor(smp, device=DEV))
+    lv, lid = lv.cpu().numpy(), lid.cpu().numpy()
+    inside = lid >= 0
+    np.testing.assert_allclose(lv[inside], smp[inside].astype(np.float64) @ a + 0.25, atol=1e-9)
+    # the numpy drop-in gives the same values
+    from gpu_mesh_sampling import gpu_sampleu
+    out = gpu_sampleu(p, c, t, o, vals, smp)
+    assert np.array_equal(out, gv)
+
+
+def test_sample_mesh_points_by_construction():
+    """R/train.py:34-61 sampling: each point lies in the tetrahedron that made it;
+    the locator agrees wherever the point is not within rounding of that
+    tetrahedron's faces, and interpolates its barycentric combination."""
+    from gvc.sampling import sample_mesh_points
+    rng = np.random.default_rng(12)
+    n = 6
+    pts = workloads.jittered_lattice(rng, n, 0.3)
+    tets = workloads.kuhn_tets(n)
+    vals = rng.normal(size=pts.shape[0])
+    gen = torch.Generator(device=DEV)
+    gen.manual_seed(5)
+    P = torch.tensor(pts, device=DEV)
+    s, made = sample_mesh_points(P, torch.tensor(tets, device=DEV), 3, 4000, generator=gen, return_cells=True)
+    assert s.shape == (3, 4000, 3) and s.dtype == torch.float32
+    s = s.reshape(-1, 3)
+    made = made.reshape(-1).cpu().numpy()
+    sampler = TetMeshSampler(P, tets, vals)
+    gv, gid = sampler.sample(s)
+    gv, gid = gv.cpu().numpy(), gid.cpu().numpy()
+    X = pts.astype(np.float64)[tets[made]]
+    x = s.cpu().numpy().astype(np.float64)
+    M = np.stack([X[:, 1] - X[:, 0], X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]], 2)
+    lam = np.linalg.solve(M, (x - X[:, 0])[..., None])[..., 0]
+    bary = np.concatenate([1 - lam.sum(1, keepdims=True), lam], 1)
+    clear = bary.min(axis=1) > 1e-5
+    assert clear.mean() > 0.99
+    assert np.array_equal(gid[clear], made[clear].astype(np.int32))
+    ref = (bary * vals[tets[made]]).sum(1)
+    np.testing.assert_allclose(gv[clear], ref[clear], rtol=1e-9, atol=1e-12)

@@ -94,3 +94,70 @@ def test_knn3_against_kdtree():
     expect = (d[:, 1:] ** 2).mean(1)
     np.testing.assert_allclose(got, expect, rtol=1e-5, atol=1e-9)
     assert got[10] == got[10] and got[10] < expect.mean()
+
+
+def test_mixed_cells_reproduce_linear_fields():
+    """Isoparametric completeness: every cell type interpolates a linear field
+    exactly, wherever the sample is located."""
+    p, c, t, o = workloads.mixed_cell_mesh(3, 0)
+    assert set(np.unique(t)) == {10, 12, 13, 14}
+    rng = np.random.default_rng(1)
+    smp = rng.uniform(-0.2, 1.3, (3000, 3)).astype(np.float32)
+    a = np.array([0.3, -1.2, 0.7])
+    f = p.astype(np.float64) @ a + 0.25
+    ids, vals, _ = mesh_ref.locate_cells(p, c, t, o, smp, f)
+    ok = ids >= 0
+    assert ok.sum() > 500 and (~ok).sum() > 500
+    np.testing.assert_allclose(vals[ok], smp[ok].astype(np.float64) @ a + 0.25, atol=1e-12)
+
+
+def test_mixed_cells_by_construction():
+    """Points made inside a known cell (trilinear / wedge / pyramid / barycentric
+    maps written out here) are located in that cell."""
+    p, c, t, o = workloads.mixed_cell_mesh(3, 4)
+    P = p.astype(np.float64)
+    rng = np.random.default_rng(2)
+    pts, want = [], []
+    for cell in range(t.shape[0]):
+        k = {10: 4, 12: 8, 13: 6, 14: 5}[int(t[cell])]
+        X = P[c[o[cell]:o[cell] + k]]
+        for _ in range(3):
+            r, s, u = rng.uniform(0.05, 0.95, 3)
+            if t[cell] == 10:
+                w = rng.dirichlet(np.ones(4)) * 0.8 + 0.05
+                x = w @ X
+            elif t[cell] == 12:
+                x = (X[0] * (1 - r) * (1 - s) * (1 - u) + X[1] * r * (1 - s) * (1 - u) + X[2] * r * s * (1 - u) +
+                     X[3] * (1 - r) * s * (1 - u) + X[4] * (1 - r) * (1 - s) * u + X[5] * r * (1 - s) * u +
+                     X[6] * r * s * u + X[7] * (1 - r) * s * u)
+            elif t[cell] == 13:
+                r, s = r * 0.45, s * 0.45
+                bot = X[0] * (1 - r - s) + X[1] * r + X[2] * s
+                top = X[3] * (1 - r - s) + X[4] * r + X[5] * s
+                x = bot * (1 - u) + top * u
+            else:
+                u = u * 0.9
+                base = X[0] * (1 - r) * (1 - s) + X[1] * r * (1 - s) + X[2] * r * s + X[3] * (1 - r) * s
+                x = base * (1 - u) + X[4] * u
+            pts.append(x)
+            want.append(cell)
+    ids, _, margin = mesh_ref.locate_cells(p, c, t, o, np.asarray(pts, np.float32))
+    assert np.array_equal(ids, np.asarray(want, np.int32))
+    assert margin.min() > 1e-4
+
+
+def test_mixed_oracle_agrees_with_exact_tets():
+    """On a tetrahedra-only mesh the parametric oracle and the exact orientation
+    oracle pick the same cells away from faces."""
+    rng = np.random.default_rng(3)
+    n = 4
+    pts = workloads.jittered_lattice(rng, n, 0.25)
+    tets = workloads.kuhn_tets(n)
+    conn = tets.reshape(-1)
+    types = np.full(tets.shape[0], 10)
+    offs = np.arange(tets.shape[0]) * 4
+    smp = rng.uniform(-0.05, 1.05, (1500, 3)).astype(np.float32)
+    ids, _, margin = mesh_ref.locate_cells(pts, conn, types, offs, smp)
+    ref, _ = mesh_ref.locate(pts, tets, smp, np.zeros(pts.shape[0]))
+    clear = (margin > 1e-6) | (ids < 0)
+    assert np.array_equal(ids[clear], ref[clear])
