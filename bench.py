@@ -49,9 +49,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="mesh", choices=["mesh", "grid", "batch32"],
+    ap.add_argument("--config", default="mesh", choices=["mesh", "grid", "batch32", "precision"],
                     help="mesh: the 1M-Gaussian mesh fit (the headline line); grid: the 200k-Gaussian grid fit; "
-                         "batch32: 32 uneven volumes in one batched launch per pass beside the plain sequence")
+                         "batch32: 32 uneven volumes in one batched launch per pass beside the plain sequence; "
+                         "precision: 4M Gaussians on a 512^3 field, naive / Kahan / FP64-cond against the FP64 "
+                         "oracle on 65,536 points, plus timed steps")
     ap.add_argument("--plain-steps", type=int, default=3, help="batch32: timed steps of the plain sequence")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--leaf", type=int, default=None)
@@ -113,20 +115,29 @@ def main():
         sampler = TetMeshSampler(W["mesh_points"], W["tets"], W["mesh_values"], device=dev)
         workload_name = ("unstructured mesh fit: 1,000,000 Gaussians, 2^22 mesh-sampled points/step, "
                          "129^3 jittered lattice, 12,582,912 tets")
-    else:
+    elif args.config == "grid":
         W = workloads.grid_workload(dev, seed=args.seed)
         S = 1 << 20
         sampler = None
         workload_name = "structured 128^3 grid fit: 200,000 Gaussians, 2^20 points/step"
+    else:
+        W = workloads.precision_workload(dev, seed=args.seed)
+        W["grid_fn"] = W["field_fn"]
+        S = 1 << 23
+        sampler = None
+        workload_name = ("precision study fit: 4,000,000 Gaussians (2% with one log-scale ln 1e-4, 0.5% ln 1e-6), "
+                         "512^3 field of 256 blobs + 3 sinusoid octaves, 2^23 points/step")
     log(f"setup {time.perf_counter() - t_setup:.1f}s P={W['P']} S={S}")
     P = W["P"]
 
     params = mx.default_params()
-    if args.leaf or args.block or args.recompute:
+    if args.leaf or args.block or args.recompute or args.config == "precision":
         params = mx.MixtureParams(leaf_size=args.leaf or params.leaf_size,
                                   block_threads=args.block or params.block_threads,
-                                  candidate_log=not args.recompute)
+                                  candidate_log=not args.recompute, fp64_cond=args.config == "precision")
         mx.set_default_params(params)
+
+    study = precision_study(args, W, dev) if args.config == "precision" else None
 
     tune = None
     if args.autotune:
@@ -152,7 +163,7 @@ def main():
     rot = torch.nn.Parameter(W["rotations"].clone())
     wl = torch.nn.Parameter(torch.logit(W["weights"]).clone())
     vl = torch.nn.Parameter(torch.logit(W["values"].clamp(1e-4, 1 - 1e-4)).clone())
-    opt = torch.optim.Adam([xyz, log_s, rot, wl, vl], lr=1e-3, eps=1e-15, capturable=args.graph)
+    opt = torch.optim.Adam([xyz, log_s, rot, wl, vl], lr=1e-3, eps=1e-15, capturable=args.graph, fused=True)
     settings = GaussianRasterizationSettings(volume_mins=(0, 0, 0), volume_maxes=(1, 1, 1), cell_count=128, bg=-1.0,
                                              scale_modifier=1.0, use_gaussian_bvh=False, debug=False)
     rast = GaussianRasterizer(settings)
@@ -264,7 +275,7 @@ def main():
     achieved = BYTES_PER_PAIR * pairs_per_step / dom_s / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-    if os.path.exists(tf):
+    if args.config == "mesh" and os.path.exists(tf):
         try:
             traffic = json.load(open(tf)).get(dom)
         except Exception:
@@ -286,9 +297,10 @@ def main():
         "config": {"workload": workload_name, "gaussians": P, "points_per_step": S,
                    "leaf_size": params.leaf_size, "block_threads": params.block_threads,
                    "run_points": params.run_points, "kahan": params.kahan, "candidate_log": params.candidate_log,
-                   "parallelism": "single GPU"},
+                   "fp64_cond": params.fp64_cond, "graph": bool(args.graph), "parallelism": "single GPU"},
         "pairs_per_step": pairs_per_step,
         "groups_walked_per_lane": lane_groups,
+        "precision_study": study,
         "autotune": tune,
         "stage_ms": stage_avg,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -339,6 +351,68 @@ def main():
                                    "sample": f"{n} of the step's {S} points against all {P} Gaussians, plain fp32 "
                                              f"brute force fwd+bwd with the box predicate ({dt:.1f} s)"}
     print(json.dumps(out))
+
+
+def precision_study(args, W, dev, n_points=65536, n_stress=16384):
+    """Field error of the three accumulation / inversion modes against the FP64
+    oracle (FP64 conics, the mode's own canonical predicate) at the workload's
+    initial parameters, on two fixed point sets: n_points uniform points, and
+    n_stress points drawn inside the boxes of the Gaussians whose covariance takes
+    the FP64 path (scale ratio above 1000, cond(Sigma) > 1e6), where the fp32
+    inverse is least accurate."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from gvc import mixture as mx
+    from gvc import training
+
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        sc, rq, wv, vv = training.activate(W["log_scales"], W["rotations"], torch.logit(W["weights"]),
+                                           torch.logit(W["values"].clamp(1e-4, 1 - 1e-4)), args.max_scale)
+        m = W["means"]
+    mh, sh, rh, wh, vh = (t.detach().cpu().numpy() for t in (m, sc, rq, wv, vv))
+    cd = oracle.conic_fp64(sh, rh)
+    pre = {fp64: oracle.preprocess(mh, sh, rh, wh, 1.0, fp64) for fp64 in (False, True)}
+    ill = np.asarray(pre[True][2]).reshape(-1).astype(bool)
+    rng = np.random.default_rng(args.seed + 12)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed + 11)
+    uniform = torch.rand((n_points, 3), device=dev, generator=gen)
+    box_ill = pre[True][1][ill]
+    pick = rng.integers(0, max(int(ill.sum()), 1), n_stress)
+    stress = (box_ill[pick, :3] + rng.uniform(0, 1, (n_stress, 3)) * (box_ill[pick, 3:] - box_ill[pick, :3])
+              if ill.any() else rng.uniform(0, 1, (n_stress, 3))).astype(np.float32)
+    modes = {"fp32_naive": mx.MixtureParams(kahan=False), "fp32_kahan": mx.MixtureParams(kahan=True),
+             "fp32_kahan_fp64_cond": mx.MixtureParams(kahan=True, fp64_cond=True)}
+    out = {"gaussians": int(mh.shape[0]), "fp64_path_gaussians": int(ill.sum()),
+           "oracle": "FP64 sums with FP64 conics, the mode's own (bit-exact) predicate", "sets": {}}
+    for set_name, pts_t in (("uniform", uniform), ("inside_fp64_path_boxes", torch.tensor(stress, device=dev))):
+        pts = pts_t.cpu().numpy()
+        with torch.no_grad():
+            ss = mx.SampleSet(pts_t)
+            gpu = {}
+            for name, prm in modes.items():
+                cells, cw, _, st = mx.mixture_forward(m, sc, rq, vv, wv, 1.0, ss, prm, want_conics=False)
+                gpu[name] = (cells.cpu().numpy(), cw.cpu().numpy(), st.stats()[1])
+        ref = {fp64: oracle.forward(mh, vh, wh, pre[fp64][0], pre[fp64][1], pts, cd) for fp64 in (False, True)}
+        res = {"points": int(pts.shape[0]), "modes": {}}
+        for name, (V, Wg, pairs) in gpu.items():
+            Vr, Wr, cnt = ref[name.endswith("fp64_cond")]
+            ok = (Wr > 0) & (Wg > 0)
+            err = np.maximum(np.abs(V - Vr) / np.maximum(np.abs(Vr), 1e-3),
+                             np.abs(Wg - Wr) / np.maximum(np.abs(Wr), 1e-3))
+            err = np.where(ok, err, 0.0)
+            res["modes"][name] = {
+                "field_max_rel": float(err.max(initial=0.0)),
+                "field_p999_rel": float(np.quantile(err[ok], 0.999)) if ok.any() else 0.0,
+                "sentinel_flips": int(np.sum((Wg > 0) != (Wr > 0))),
+                "pairs": int(pairs), "pairs_match_oracle": int(pairs) == int(cnt.sum())}
+        out["sets"][set_name] = res
+    out["seconds"] = time.perf_counter() - t0
+    log(f"precision study: {json.dumps(out['sets'])}")
+    return out
 
 
 def main_batch(args):
@@ -394,7 +468,7 @@ def main_batch(args):
         rot = torch.nn.Parameter(W["rotations"].clone())
         wl = torch.nn.Parameter(torch.logit(W["weights"]).clone())
         vl = torch.nn.Parameter(torch.logit(W["values"].clamp(1e-4, 1 - 1e-4)).clone())
-        return [xyz, log_s, rot, wl, vl], torch.optim.Adam([xyz, log_s, rot, wl, vl], lr=1e-3, eps=1e-15)
+        return [xyz, log_s, rot, wl, vl], torch.optim.Adam([xyz, log_s, rot, wl, vl], lr=1e-3, eps=1e-15, fused=True)
 
     audit = []
     acc = torch.zeros(3, dtype=torch.int64, device=dev)

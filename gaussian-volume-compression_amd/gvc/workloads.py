@@ -327,3 +327,66 @@ def mixed_cell_mesh(n=4, seed=0):
     lens = np.array([len(x) for x in conn], np.int64)
     offsets = np.concatenate([[0], np.cumsum(lens)[:-1]])
     return np.concatenate(pts).astype(np.float32), flat, np.asarray(types, np.int64), offsets
+
+
+def precision_workload(device, seed=0, n=512, P=4_000_000, n_blobs=256, degenerate=0.02, stress=0.005):
+    """The precision-study workload: an n^3 float32 field (a sum of ``n_blobs``
+    random blobs, scaled to [0, 1], plus three sinusoid octaves of amplitude 0.1,
+    0.05, 0.025 at 4, 8, 16 cycles per unit along random unit directions with
+    random phases) and P Gaussians at uniform random positions initialised like
+    the reference (3-NN scales, identity rotations, weight 0.1, value = field at
+    the mean).  A fraction ``degenerate`` of them gets one log-scale set to
+    ln 1e-4 (scale ratio about 35, below the FP64 threshold); a further
+    ``stress`` fraction gets ln 1e-6 (ratio above 1000, i.e. cond(Sigma) > 1e6);
+    both sets get random rotations.
+    Returns the Gaussians (torch), the field (x fastest, for gvc_grid_sample) and
+    the grid geometry; field_fn(samples) samples it trilinearly."""
+    import torch
+    from gvc.sampling import grid_sample
+    rng = np.random.default_rng(seed)
+    b = blobs(rng, n_blobs)
+    dirs = rng.normal(size=(3, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    phases = rng.uniform(0, 2 * np.pi, 3)
+    g = torch.arange(n, device=device, dtype=torch.float32) / (n - 1)
+    field = torch.zeros((n, n, n), dtype=torch.float32, device=device)   # [z, y, x]
+    mu = torch.as_tensor(b["means"], dtype=torch.float32, device=device)
+    inv = torch.as_tensor(b["inv"], dtype=torch.float32, device=device)
+    w = torch.as_tensor(b["w"], dtype=torch.float32, device=device)
+    Z, Y, X = torch.meshgrid(g, g, g, indexing="ij")
+    for k in range(n_blobs):
+        dx, dy, dz = X - mu[k, 0], Y - mu[k, 1], Z - mu[k, 2]
+        q = (inv[k, 0, 0] * dx * dx + inv[k, 1, 1] * dy * dy + inv[k, 2, 2] * dz * dz +
+             2 * (inv[k, 0, 1] * dx * dy + inv[k, 0, 2] * dx * dz + inv[k, 1, 2] * dy * dz))
+        field += w[k] * torch.exp(-0.5 * q)
+    field = (field - field.min()) / (field.max() - field.min())
+    for k, (amp, freq) in enumerate(((0.1, 4.0), (0.05, 8.0), (0.025, 16.0))):
+        d = dirs[k]
+        field += amp * torch.sin(2 * np.pi * freq * (float(d[0]) * X + float(d[1]) * Y + float(d[2]) * Z) +
+                                 float(phases[k]))
+    del X, Y, Z
+    flat = field.reshape(-1).contiguous()
+    dims = np.array([n, n, n], np.int64)
+    origin = np.zeros(3, np.float32)
+    spacing = np.full(3, 1.0 / (n - 1), np.float32)
+
+    def field_fn(samples):
+        return grid_sample(dims, origin, spacing, flat, samples)
+
+    means = rng.uniform(0.0, 1.0, (P, 3)).astype(np.float32)
+    m = torch.as_tensor(means, device=device)
+    vals = field_fn(m).double().cpu().numpy()
+    out = _init_gaussians(means, vals, device)
+    nd, ns = int(round(degenerate * P)), int(round(stress * P))
+    pick = rng.permutation(P)[:nd + ns]
+    axis = torch.as_tensor(rng.integers(0, 3, nd + ns), device=device)
+    rows = torch.as_tensor(pick, device=device)
+    target = torch.full((nd + ns,), float(np.log(1e-4)), device=device)
+    target[nd:] = float(np.log(1e-6))
+    out["log_scales"][rows, axis] = target
+    # rotated, as after training: a diagonal covariance inverts exactly in fp32
+    q = rng.normal(size=(nd + ns, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    out["rotations"][rows] = torch.as_tensor(q, dtype=torch.float32, device=device)
+    out.update(field=flat, field_fn=field_fn, degenerate=pick[:nd], stress=pick[nd:])
+    return out

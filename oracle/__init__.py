@@ -51,6 +51,10 @@ def lib():
         L.orc_preprocess.argtypes = [I, P, P, P, P, F, I, P, P, P]
         L.orc_forward.restype = None
         L.orc_forward.argtypes = [I, I, P, P, P, P, P, P, P, P, P, P]
+        L.orc_forward_binned.restype = I
+        L.orc_forward_binned.argtypes = [I, I, P, P, P, P, P, P, P, I, ctypes.c_longlong, P, P, P]
+        L.orc_conic_fp64_n.restype = None
+        L.orc_conic_fp64_n.argtypes = [I, P, P, ctypes.c_float, P]
         L.orc_backward.restype = None
         L.orc_backward.argtypes = [I, I, P, P, P, P, P, F, P, P, P, P, P, P, P, P, P, P, P, P, P]
         L.orc_cov_chain.restype = None
@@ -107,16 +111,18 @@ def preprocess(means, scales, rots, weights, scale_modifier=1.0, fp64_cond=False
 
 def conic_fp64(scales, rots, scale_modifier=1.0):
     scales, rots = _f32(scales), _f32(rots)
+    scales, rots = _f32(scales), _f32(rots)
     P = scales.shape[0]
     out = np.zeros((P, 6), np.float64)
-    L = lib()
-    for i in range(P):
-        L.orc_conic_fp64(_p(scales[i]), _p(rots[i]), float(scale_modifier), _p(out[i]))
+    lib().orc_conic_fp64_n(P, _p(scales), _p(rots), float(scale_modifier), _p(out))
     return out
 
 
-def forward(means, values, weights, conic, box, samples, conic_d=None):
-    """Brute-force FP64 forward with the canonical predicate.  Returns V, W, counts."""
+def forward(means, values, weights, conic, box, samples, conic_d=None, binned=None):
+    """FP64 forward with the canonical predicate.  Returns V, W, counts.  Brute
+    force, or (binned=True, default for large problems) the same sums with the
+    Gaussians binned in a uniform grid -- identical results, every accepted pair
+    visited in ascending Gaussian order."""
     means, values, weights = _f32(means), _f32(values).reshape(-1), _f32(weights).reshape(-1)
     conic, box, samples = _f32(conic), _f32(box), _f32(samples)
     cd = _f64(conic_d) if conic_d is not None else None
@@ -124,6 +130,19 @@ def forward(means, values, weights, conic, box, samples, conic_d=None):
     V = np.zeros(S, np.float64)
     W = np.zeros(S, np.float64)
     cnt = np.zeros(S, np.int32)
+    if binned is None:
+        binned = P * S > 200_000_000
+    if binned and P > 0:
+        ext = (box[:, 3:] - box[:, :3]).astype(np.float64)
+        fin = np.all(np.isfinite(box), axis=1)
+        span = np.nanmax(np.where(fin[:, None], box[:, 3:], -np.inf), axis=0) - \
+            np.nanmin(np.where(fin[:, None], box[:, :3], np.inf), axis=0) if fin.any() else np.ones(3)
+        width = np.median(ext[fin], axis=0) if fin.any() else np.ones(3)
+        G = int(np.clip(np.min(np.asarray(span) / np.maximum(width, 1e-12)), 1, 256))
+        rc = lib().orc_forward_binned(P, S, _p(means), _p(values), _p(weights), _p(conic), _p(box), _p(cd),
+                                      _p(samples), G, 1_500_000_000, _p(V), _p(W), _p(cnt))
+        if rc == 0:
+            return V, W, cnt
     lib().orc_forward(P, S, _p(means), _p(values), _p(weights), _p(conic), _p(box), _p(cd),
                       _p(samples), _p(V), _p(W), _p(cnt))
     return V, W, cnt

@@ -293,6 +293,38 @@ void orc_pair_mask(int P, int S, const float *means, const float *conic, const f
  * Outputs V (cells) and W (weights) in double; sentinel (-1, 0) when W <= 1e-2.
  * count_out[s] = number of accepted pairs at point s (may be NULL).
  */
+/* FP64 contribution of an accepted pair to (W, U). */
+static inline void orc_pair_add(int i, const float *x, const float *means, const float *values,
+                                const float *weights, const float *conic, const double *conic_d,
+                                double *W, double *U)
+{
+    const float *cf = conic + 6 * i;
+    double c[6];
+    for (int k = 0; k < 6; ++k) c[k] = conic_d ? conic_d[6 * i + k] : (double)cf[k];
+    double dx = (double)x[0] - means[3 * i + 0];
+    double dy = (double)x[1] - means[3 * i + 1];
+    double dz = (double)x[2] - means[3 * i + 2];
+    double quad = dx * (c[0] * dx + c[1] * dy + c[2] * dz) +
+                  dy * (c[1] * dx + c[3] * dy + c[4] * dz) +
+                  dz * (c[2] * dx + c[4] * dy + c[5] * dz);
+    double wgt = (double)weights[i] * exp(-0.5 * quad);
+    *W += wgt;
+    *U += wgt * (double)values[i];
+}
+
+static inline void orc_finish(double W, double U, int cnt, int s, double *V_out, double *W_out,
+                              int32_t *count_out)
+{
+    if (W <= ORC_WEIGHT_CUTOFF) {
+        V_out[s] = -1.0;
+        W_out[s] = 0.0;
+    } else {
+        V_out[s] = U / W;
+        W_out[s] = W;
+    }
+    if (count_out) count_out[s] = cnt;
+}
+
 void orc_forward(int P, int S, const float *means, const float *values, const float *weights,
                  const float *conic, const float *box, const double *conic_d,
                  const float *samples, double *V_out, double *W_out, int32_t *count_out)
@@ -305,29 +337,90 @@ void orc_forward(int P, int S, const float *means, const float *values, const fl
         for (int i = 0; i < P; ++i) {
             float pw;
             if (!orc_pair_accept(box + 6 * i, means + 3 * i, conic + 6 * i, x, &pw)) continue;
-            const float *cf = conic + 6 * i;
-            double c[6];
-            for (int k = 0; k < 6; ++k) c[k] = conic_d ? conic_d[6 * i + k] : (double)cf[k];
-            double dx = (double)x[0] - means[3 * i + 0];
-            double dy = (double)x[1] - means[3 * i + 1];
-            double dz = (double)x[2] - means[3 * i + 2];
-            double quad = dx * (c[0] * dx + c[1] * dy + c[2] * dz) +
-                          dy * (c[1] * dx + c[3] * dy + c[4] * dz) +
-                          dz * (c[2] * dx + c[4] * dy + c[5] * dz);
-            double wgt = (double)weights[i] * exp(-0.5 * quad);
-            W += wgt;
-            U += wgt * (double)values[i];
+            orc_pair_add(i, x, means, values, weights, conic, conic_d, &W, &U);
             ++cnt;
         }
-        if (W <= ORC_WEIGHT_CUTOFF) {
-            V_out[s] = -1.0;
-            W_out[s] = 0.0;
-        } else {
-            V_out[s] = U / W;
-            W_out[s] = W;
-        }
-        if (count_out) count_out[s] = cnt;
+        orc_finish(W, U, cnt, s, V_out, W_out, count_out);
     }
+}
+
+/* The same forward with the Gaussians binned in a uniform grid of G^3 cells over
+ * the finite part of their boxes (a box reaching past the grid is clamped to it;
+ * a box with a NaN bound holds nothing and is not binned).  Each cell lists its
+ * Gaussians in ascending index, so every sample visits its accepted pairs in the
+ * order of orc_forward and the FP64 sums are identical.  Returns 0, or -1 when
+ * the bins would exceed max_entries. */
+int orc_forward_binned(int P, int S, const float *means, const float *values, const float *weights,
+                       const float *conic, const float *box, const double *conic_d,
+                       const float *samples, int G, long long max_entries, double *V_out,
+                       double *W_out, int32_t *count_out)
+{
+    double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+    for (int i = 0; i < P; ++i)
+        for (int a = 0; a < 3; ++a) {
+            double l = box[6 * i + a], h = box[6 * i + 3 + a];
+            if (isfinite(l) && l < lo[a]) lo[a] = l;
+            if (isfinite(h) && h > hi[a]) hi[a] = h;
+        }
+    for (int a = 0; a < 3; ++a)
+        if (!(lo[a] <= hi[a])) { lo[a] = 0.0; hi[a] = 1.0; }
+    if (G < 1) G = 1;
+    const double w[3] = { (hi[0] - lo[0]) / G + 1e-300, (hi[1] - lo[1]) / G + 1e-300, (hi[2] - lo[2]) / G + 1e-300 };
+#define ORC_CELL(v, a) ((int)fmin(fmax(floor(((v) - lo[a]) / w[a]), 0.0), (double)(G - 1)))
+    const size_t nc = (size_t)G * G * G;
+    long long *start = (long long *)calloc(nc + 1, sizeof(long long));
+    if (!start) return -1;
+    for (int pass = 0; pass < 2; ++pass) {
+        int *list = NULL;
+        long long *fill = NULL;
+        if (pass == 1) {
+            for (size_t c = 0; c < nc; ++c) start[c + 1] += start[c];
+            if (start[nc] > max_entries) { free(start); return -1; }
+            list = (int *)malloc(sizeof(int) * (size_t)(start[nc] > 0 ? start[nc] : 1));
+            fill = (long long *)malloc(sizeof(long long) * nc);
+            if (!list || !fill) { free(list); free(fill); free(start); return -1; }
+            memcpy(fill, start, sizeof(long long) * nc);
+        }
+        for (int i = 0; i < P; ++i) {
+            const float *b = box + 6 * i;
+            if (b[0] != b[0] || b[1] != b[1] || b[2] != b[2] || b[3] != b[3] || b[4] != b[4] || b[5] != b[5])
+                continue;
+            const int x0 = ORC_CELL(b[0], 0), x1 = ORC_CELL(b[3], 0);
+            const int y0 = ORC_CELL(b[1], 1), y1 = ORC_CELL(b[4], 1);
+            const int z0 = ORC_CELL(b[2], 2), z1 = ORC_CELL(b[5], 2);
+            for (int z = z0; z <= z1; ++z)
+                for (int y = y0; y <= y1; ++y)
+                    for (int x = x0; x <= x1; ++x) {
+                        const size_t c = ((size_t)z * G + y) * G + x;
+                        if (pass == 0) ++start[c + 1];
+                        else list[fill[c]++] = i;
+                    }
+        }
+        if (pass == 1) {
+#pragma omp parallel for schedule(dynamic, 64)
+            for (int s = 0; s < S; ++s) {
+                const float *x = samples + 3 * s;
+                double Wv = 0.0, U = 0.0;
+                int cnt = 0;
+                if (x[0] == x[0] && x[1] == x[1] && x[2] == x[2]) {
+                    const size_t c = ((size_t)ORC_CELL(x[2], 2) * G + ORC_CELL(x[1], 1)) * G + ORC_CELL(x[0], 0);
+                    for (long long k = start[c]; k < start[c + 1]; ++k) {
+                        const int i = list[k];
+                        float pw;
+                        if (!orc_pair_accept(box + 6 * i, means + 3 * i, conic + 6 * i, x, &pw)) continue;
+                        orc_pair_add(i, x, means, values, weights, conic, conic_d, &Wv, &U);
+                        ++cnt;
+                    }
+                }
+                orc_finish(Wv, U, cnt, s, V_out, W_out, count_out);
+            }
+            free(list);
+            free(fill);
+        }
+    }
+#undef ORC_CELL
+    free(start);
+    return 0;
 }
 
 /*
@@ -417,6 +510,12 @@ void orc_conic_fp64(const float *scale3, const float *q4, float scale_modifier, 
     double det = a * A00 + b * A01 + c * A02;
     C6[0] = A00 / det; C6[1] = A01 / det; C6[2] = A02 / det;
     C6[3] = A11 / det; C6[4] = A12 / det; C6[5] = A22 / det;
+}
+
+void orc_conic_fp64_n(int P, const float *scales, const float *rots, float scale_modifier, double *out)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < P; ++i) orc_conic_fp64(scales + 3 * i, rots + 4 * i, scale_modifier, out + 6 * i);
 }
 
 /*
