@@ -1194,48 +1194,46 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
                     logged = false;
                 }
             }
-            uint64_t accm = 0;
-            if (m) {
-                int j = __builtin_ctzll(m);
-                m &= m - 1;
-                float4 A = s.chunk[j][0], B = s.chunk[j][1], C = s.chunk[j][2];
-                while (true) {
-                    const bool more = m != 0;
-                    const int jn = more ? __builtin_ctzll(m) : j;
-                    const float4 An = s.chunk[jn][0], Bn = s.chunk[jn][1], Cn = s.chunk[jn][2];
-                    const float dx = px - A.x, dy = py - A.y, dz = pz - A.z;
-                    float r0, r1, r2;
-                    const float power = pair_power(B.x, B.y, B.z, B.w, C.x, C.y, dx, dy, dz, r0, r1, r2);
-                    if (in_window(power)) {
-                        const float e = __builtin_amdgcn_exp2f(power * kLog2e);
-                        const float wgt = A.w * e;
-                        const float wv = wgt * C.z;
-                        if (KAHAN) {
-                            float y = wgt - cW;
-                            float t = W + y;
-                            cW = (t - W) - y;
-                            W = t;
-                            y = wv - cU;
-                            t = U + y;
-                            cU = (t - U) - y;
-                            U = t;
-                        } else {
-                            W += wgt;
-                            U += wv;
-                        }
-                        ++npairs;
-                        accm |= 1ull << j;
+            // the backward writes one record per candidate with a box hit in the run
+            nrec += (uint32_t)__popcll(__ballot(cm != 0ull));
+            // each point lane walks its candidates in ascending order, two per pass
+            // (both loads issued before either pair is evaluated)
+            const float4* __restrict__ cb = &s.chunk[0][0];
+            auto pair = [&](const float4& A, const float4& B, const float4& C) {
+                const float dx = px - A.x, dy = py - A.y, dz = pz - A.z;
+                float r0, r1, r2;
+                const float power = pair_power(B.x, B.y, B.z, B.w, C.x, C.y, dx, dy, dz, r0, r1, r2);
+                if (in_window(power)) {
+                    const float e = __builtin_amdgcn_exp2f(power * kLog2e);
+                    const float wgt = A.w * e;
+                    const float wv = wgt * C.z;
+                    if (KAHAN) {
+                        float y = wgt - cW;
+                        float t = W + y;
+                        cW = (t - W) - y;
+                        W = t;
+                        y = wv - cU;
+                        t = U + y;
+                        cU = (t - U) - y;
+                        U = t;
+                    } else {
+                        W += wgt;
+                        U += wv;
                     }
-                    if (!more) break;
-                    m &= m - 1;
-                    j = jn;
-                    A = An;
-                    B = Bn;
-                    C = Cn;
+                    ++npairs;
                 }
+            };
+            while (m != 0ull) {
+                const uint32_t j0 = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                const bool two = m != 0ull;
+                const uint32_t j1 = two ? (uint32_t)__builtin_ctzll(m) : j0;
+                m &= two ? m - 1 : m;
+                const float4 A0 = cb[3 * j0], B0 = cb[3 * j0 + 1], C0 = cb[3 * j0 + 2];
+                const float4 A1 = cb[3 * j1], B1 = cb[3 * j1 + 1], C1 = cb[3 * j1 + 2];
+                pair(A0, B0, C0);
+                if (two) pair(A1, B1, C1);
             }
-            const uint32_t olo = wave_or32((uint32_t)accm), ohi = wave_or32((uint32_t)(accm >> 32));
-            nrec += (uint32_t)(__popc(olo) + __popc(ohi));
             lds_fence();
         };
         const bool lane_mode = lane_mode_group(tv, px, py, pz, nv, starts, reach, lane_policy);
@@ -1449,7 +1447,6 @@ __global__ __launch_bounds__(1024) void backward_kernel(
             const uint32_t H = uniformi((int)wave_sumu(h));
             float dv = 0.f, dw = 0.f, dmx = 0.f, dmy = 0.f, dmz = 0.f;
             float dc0 = 0.f, dc1 = 0.f, dc2 = 0.f, dc3 = 0.f, dc4 = 0.f, dc5 = 0.f;
-            bool any = false;
             if (H > 0) {
                 const uint32_t q = H > 128 ? (H + 127) / 128 : 1u;
                 const uint32_t sj = (h + q - 1) / q;
@@ -1471,7 +1468,6 @@ __global__ __launch_bounds__(1024) void backward_kernel(
                     uint64_t mj = live ? drop_low_bits(((uint64_t)mhi_j << 32) | mlo_j, (v - tb) * q) : 0ull;
                     float sv = 0.f, sw = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
                     float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f;
-                    float sany = 0.f;
                     for (uint32_t it = 0; it < q && mj != 0ull; ++it) {
                         const int p = __builtin_ctzll(mj);
                         mj &= mj - 1;
@@ -1481,7 +1477,6 @@ __global__ __launch_bounds__(1024) void backward_kernel(
                         float r0_, r1_, r2_;
                         const float power = pair_power(k0, k1, k2, k3, k4, k5, dx, dy, dz, r0_, r1_, r2_);
                         if (in_window(power)) {
-                            sany = 1.f;
                             const float e = __builtin_amdgcn_exp2f(power * kLog2e);
                             const float wgt = cw * e;
                             sv = __builtin_fmaf(b.x, wgt, sv);
@@ -1504,7 +1499,7 @@ __global__ __launch_bounds__(1024) void backward_kernel(
                     }
                     s.slot[lane][0] = make_float4(sv, sw, sx, sy);
                     s.slot[lane][1] = make_float4(sz, s0, s1, s2);
-                    s.slot[lane][2] = make_float4(s3, s4, s5, sany);
+                    s.slot[lane][2] = make_float4(s3, s4, s5, 0.f);
                     lds_fence();
                     // this candidate's slices inside the round
                     const uint32_t lo = bj > r0 ? bj : r0;
@@ -1514,13 +1509,14 @@ __global__ __launch_bounds__(1024) void backward_kernel(
                         dv += A.x; dw += A.y; dmx += A.z; dmy += A.w;
                         dmz += B.x; dc0 += B.y; dc1 += B.z; dc2 += B.w;
                         dc3 += C.x; dc4 += C.y; dc5 += C.z;
-                        any = any || (C.w != 0.f);
                     }
                     lds_fence();
                 }
             }
-            const uint64_t bal = __ballot(any);
-            if (any) {
+            // one record per candidate with a box hit (the forward counted these)
+            const bool has = h != 0u;
+            const uint64_t bal = __ballot(has);
+            if (has) {
                 const uint64_t pos = (uint64_t)base_out + (uint32_t)__popcll(bal & lt);
                 if (pos < limit && pos < capacity) {
                     PartialRecord o;
