@@ -605,11 +605,25 @@ __device__ __forceinline__ uint64_t split_runs(float px, float py, float pz, int
 // Dynamic group scheduling.  First the heavy groups (listed in the build by
 // their expected cost, so that the longest walks start at once instead of
 // trailing the pass): every wave takes the next one from one shared cursor.
-// Then the rest: the groups are cut into 8 contiguous slices, one per XCD
+// Then the rest: the groups are cut into kQueues contiguous slices, kQueues/8 per XCD
 // (workgroups are dealt round-robin to the XCDs, so blockIdx & 7 names the
 // wave's XCD); a wave takes the next group of its slice, skipping heavy ones,
-// and once that slice is exhausted moves on to the following slices.  Returns
-// NG when all are done.  queue[8] and *heavy_cursor must be zero at launch.
+// and once that slice is exhausted moves on to the following slices (its own
+// XCD's first).  Each slice's counter is on its own cache line.  Returns
+// NG when all are done.  The 8 queue words (queue[x * kLine]) and *heavy_cursor must
+// be zero at launch.
+// Device counters of the walks, each on its own 128-byte line: atomics on one
+// line serialise in one L2 channel, and every group claim is one atomic.
+constexpr int kLine = 32;                  // uint32 words per 128-byte line
+constexpr int kQueues = 8;                 // group queues per pass, a multiple of the 8 XCDs
+constexpr int kQFwd = 0;                   // forward group queues, a line each
+constexpr int kQBwd = kQueues * kLine;     // backward group queues
+constexpr int kQLog = 2 * kQueues * kLine; // candidate-log slot counter
+constexpr int kQLane = kQLog + kLine;      // lane-walked groups
+constexpr int kQHeavyF = kQLog + 2 * kLine;  // heavy-list cursor, forward
+constexpr int kQHeavyB = kQLog + 3 * kLine;  // heavy-list cursor, backward
+constexpr int kQWords = kQLog + 4 * kLine;
+
 struct GroupSched {
     const uint32_t* heavy;   // [0] = count, [1..] = heavy groups
     uint32_t* cursor;
@@ -628,11 +642,11 @@ __device__ __forceinline__ int next_group(uint32_t* __restrict__ queue, int NG, 
         if (t < nh) return uniformi((int)sc.heavy[1 + t]);
         sc.heavy_done = true;
     }
-    while (tries < 8) {
-        const uint32_t b = (uint32_t)(((uint64_t)NG * (uint32_t)x) >> 3);
-        const uint32_t e = (uint32_t)(((uint64_t)NG * (uint32_t)(x + 1)) >> 3);
+    while (tries < kQueues) {
+        const uint32_t b = (uint32_t)(((uint64_t)NG * (uint32_t)x) / kQueues);
+        const uint32_t e = (uint32_t)(((uint64_t)NG * (uint32_t)(x + 1)) / kQueues);
         uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(&queue[x], 1u);
+        if (lane == 0) t = atomicAdd(&queue[x * kLine], 1u);
         t = (uint32_t)__shfl((int)t, 0, 64);
         t = (uint32_t)uniformi((int)t);
         if (t < e - b) {
@@ -640,10 +654,18 @@ __device__ __forceinline__ int next_group(uint32_t* __restrict__ queue, int NG, 
             if (uniformi((int)sc.flag[g]) != 0) continue;
             return g;
         }
-        x = (x + 1) & 7;
+        x = (x + 1) & (kQueues - 1);
         ++tries;
     }
     return NG;
+}
+
+// A wave's first queue: one of the slices of its XCD's eighth of the groups
+// (blockIdx & 7 names the XCD).
+__device__ __forceinline__ int home_queue()
+{
+    constexpr unsigned per = kQueues / 8;
+    return (int)((blockIdx.x & 7u) * per + (blockIdx.x >> 3) % per);
 }
 
 __device__ __forceinline__ void mean_extent(const unsigned long long* __restrict__ ext, float r[3])
@@ -1133,7 +1155,7 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
     extern __shared__ float4 lds_dyn[];
     const int wid = uniformi(threadIdx.x >> 6), lane = threadIdx.x & 63;
     FwdLds& s = reinterpret_cast<FwdLds*>(lds_dyn)[wid];
-    int x = (int)(blockIdx.x & 7u), tries = 0;
+    int x = home_queue(), tries = 0;
     GroupSched sched{ heavy, heavy_cursor, group_flag, false };
     bool ok = true;
     LogCursor cur;
@@ -1263,7 +1285,7 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
                            }) && ok;
             nrec = wave_sumu(npairs);   // the backward writes one record per pair
             head = kLogLaneMode;
-            if (lane == 0) atomicAdd(log_counter + 1, 1u);   // lane-walked groups
+            if (lane == 0) atomicAdd(log_counter + (kQLane - kQLog), 1u);   // lane-walked groups
         } else {
             uint64_t rs = starts;
             while (rs != 0ull) {
@@ -1388,7 +1410,7 @@ __global__ __launch_bounds__(1024) void backward_kernel(
     const int wid = uniformi(threadIdx.x >> 6), lane = threadIdx.x & 63;
     BwdLds<REPLAY>& s = reinterpret_cast<BwdLds<REPLAY>*>(lds_dyn)[wid];
     const uint64_t lt = lanemask_lt();
-    int x = (int)(blockIdx.x & 7u), tries = 0;
+    int x = home_queue(), tries = 0;
     GroupSched sched{ heavy, heavy_cursor, group_flag, false };
     bool ok = true;
     while (true) {
@@ -1845,7 +1867,7 @@ struct ForwardLayout {
     Box6* sparse;  // levels x NL leaf-range boxes
     float* root_box;  // 6 per volume
     uint32_t *group_records, *group_offsets, *group_pairs, *scan_scratch, *err;
-    uint32_t* queues;  // 8 forward + 8 backward group queues, log counter
+    uint32_t* queues;  // walk counters, kQWords words (kQ* offsets)
     unsigned long long *stats, *ext_sum;
     uint32_t* group_head;
     uint32_t* heavy;        // [0] = count, [1..] = heavy groups (walked first)
@@ -1877,7 +1899,7 @@ struct ForwardLayout {
         group_pairs = a.take<uint32_t>((size_t)NG + 1);
         scan_scratch = a.take<uint32_t>(scan_scratch_elems((size_t)NG) + 1);
         err = a.take<uint32_t>(4);
-        queues = a.take<uint32_t>(20);
+        queues = a.take<uint32_t>(kQWords);
         stats = a.take<unsigned long long>(4);
         ext_sum = a.take<unsigned long long>(4 * (size_t)b.V);
         group_head = a.take<uint32_t>((size_t)NG + 1);
@@ -2087,8 +2109,8 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
     // 5. walk
     if ((stages & GVC_STAGE_WALK) && NG > 0) {
         const int T = prm->block_threads;
-        GVC_TRY(fill_u32(l.queues, 0u, 8, st));
-        GVC_TRY(fill_u32(l.queues + 16, 0u, 4, st));
+        GVC_TRY(fill_u32(l.queues + kQFwd, 0u, kQueues * kLine, st));
+        GVC_TRY(fill_u32(l.queues + kQLog, 0u, kQWords - kQLog, st));
         const int log_on = (prm->flags & GVC_FLAG_CANDIDATE_LOG) ? 1 : 0;
         const int rp = run_points_of(prm);
         const int lp = lane_policy_of(prm);
@@ -2098,17 +2120,19 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
             GVC_TRY(allow_lds(kf, lds));
             const unsigned nb = walk_blocks(kf, T, lds, NG);
             forward_kernel<true><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
-                                                     out_weights, l.group_records, l.group_pairs, l.err, l.queues,
-                                                     l.ext_sum, log_on, l.log, l.log_cap, l.queues + 16,
-                                                     l.group_head, rp, lp, l.heavy, l.queues + 18, l.group_flag);
+                                                     out_weights, l.group_records, l.group_pairs, l.err,
+                                                     l.queues + kQFwd, l.ext_sum, log_on, l.log, l.log_cap,
+                                                     l.queues + kQLog, l.group_head, rp, lp, l.heavy,
+                                                     l.queues + kQHeavyF, l.group_flag);
         } else {
             const void* kf = reinterpret_cast<const void*>(&forward_kernel<false>);
             GVC_TRY(allow_lds(kf, lds));
             const unsigned nb = walk_blocks(kf, T, lds, NG);
             forward_kernel<false><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
-                                                      out_weights, l.group_records, l.group_pairs, l.err, l.queues,
-                                                      l.ext_sum, log_on, l.log, l.log_cap, l.queues + 16,
-                                                      l.group_head, rp, lp, l.heavy, l.queues + 18, l.group_flag);
+                                                      out_weights, l.group_records, l.group_pairs, l.err,
+                                                      l.queues + kQFwd, l.ext_sum, log_on, l.log, l.log_cap,
+                                                      l.queues + kQLog, l.group_head, rp, lp, l.heavy,
+                                                      l.queues + kQHeavyF, l.group_flag);
         }
         GVC_LAUNCH_CHECK("forward");
     }
@@ -2132,7 +2156,8 @@ int forward_stats_impl(const Batch& bt, const void* state, const gvc_params* prm
     uint32_t e = 0, used[2] = { 0, 0 };
     GVC_HIP(hipMemcpyAsync(h, l.stats, sizeof(h), hipMemcpyDeviceToHost, st));
     GVC_HIP(hipMemcpyAsync(&e, l.err, sizeof(e), hipMemcpyDeviceToHost, st));
-    GVC_HIP(hipMemcpyAsync(used, l.queues + 16, sizeof(used), hipMemcpyDeviceToHost, st));
+    GVC_HIP(hipMemcpyAsync(used, l.queues + kQLog, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    GVC_HIP(hipMemcpyAsync(used + 1, l.queues + kQLane, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     GVC_HIP(hipStreamSynchronize(st));
     if (n_records) *n_records = h[0];
     if (n_pairs) *n_pairs = h[1];
@@ -2176,8 +2201,8 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
     const unsigned stages = prm->stages ? (unsigned)prm->stages : 3u;
     if ((stages & GVC_STAGE_WALK) && NG > 0 && capacity > 0) {
         const int T = prm->block_threads;
-        GVC_TRY(fill_u32(l.queues + 8, 0u, 8, st));
-        GVC_TRY(fill_u32(l.queues + 19, 0u, 1, st));
+        GVC_TRY(fill_u32(l.queues + kQBwd, 0u, kQueues * kLine, st));
+        GVC_TRY(fill_u32(l.queues + kQHeavyB, 0u, kLine, st));
         const int rp = run_points_of(prm);
         const int lp = lane_policy_of(prm);
         if (prm->flags & GVC_FLAG_CANDIDATE_LOG) {
@@ -2187,8 +2212,8 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
             const unsigned nb = walk_blocks(kf, T, lds, NG);
             backward_kernel<true><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
                                                       out_weights, gcells, gweights, l.group_offsets, capacity,
-                                                      bl.parts, bl.keys, bl.vals, l.err, l.queues + 8, l.ext_sum,
-                                                      l.log, l.group_head, rp, lp, l.heavy, l.queues + 19,
+                                                      bl.parts, bl.keys, bl.vals, l.err, l.queues + kQBwd, l.ext_sum,
+                                                      l.log, l.group_head, rp, lp, l.heavy, l.queues + kQHeavyB,
                                                       l.group_flag);
         } else {
             const void* kf = reinterpret_cast<const void*>(&backward_kernel<false>);
@@ -2197,8 +2222,8 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
             const unsigned nb = walk_blocks(kf, T, lds, NG);
             backward_kernel<false><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
                                                        out_weights, gcells, gweights, l.group_offsets, capacity,
-                                                       bl.parts, bl.keys, bl.vals, l.err, l.queues + 8, l.ext_sum,
-                                                       l.log, l.group_head, rp, lp, l.heavy, l.queues + 19,
+                                                       bl.parts, bl.keys, bl.vals, l.err, l.queues + kQBwd, l.ext_sum,
+                                                       l.log, l.group_head, rp, lp, l.heavy, l.queues + kQHeavyB,
                                                        l.group_flag);
         }
         GVC_LAUNCH_CHECK("backward");
