@@ -1401,7 +1401,7 @@ __global__ __launch_bounds__(1024) void backward_kernel(
     int P, int L, const float4* __restrict__ pts, int NG, const float* __restrict__ out_cells,
     const float* __restrict__ out_weights, const float* __restrict__ gcells, const float* __restrict__ gweights,
     const uint32_t* __restrict__ group_offsets, uint64_t capacity, PartialRecord* __restrict__ out_rec,
-    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ err, uint32_t* __restrict__ queue,
+    uint32_t* __restrict__ keys, uint32_t* __restrict__ err, uint32_t* __restrict__ queue,
     const unsigned long long* __restrict__ ext_sum, const unsigned char* __restrict__ log,
     const uint32_t* __restrict__ group_head, int run_points, int lane_policy, const uint32_t* __restrict__ heavy,
     uint32_t* __restrict__ heavy_cursor, const uint8_t* __restrict__ group_flag)
@@ -1549,7 +1549,6 @@ __global__ __launch_bounds__(1024) void backward_kernel(
                     o.pad[0] = o.pad[1] = o.pad[2] = o.pad[3] = 0.f;
                     out_rec[pos] = o;
                     keys[pos] = (uint32_t)gi;
-                    vals[pos] = (uint32_t)pos;
                 } else {
                     atomicOr(err, (uint32_t)GVC_ERRFLAG_RECORD_OVERFLOW);
                 }
@@ -1583,7 +1582,6 @@ __global__ __launch_bounds__(1024) void backward_kernel(
                                        o.pad[0] = o.pad[1] = o.pad[2] = o.pad[3] = 0.f;
                                        out_rec[pos] = o;
                                        keys[pos] = (uint32_t)gi;
-                                       vals[pos] = (uint32_t)pos;
                                    } else {
                                        atomicOr(err, (uint32_t)GVC_ERRFLAG_RECORD_OVERFLOW);
                                    }
@@ -1667,51 +1665,84 @@ __global__ void segment_kernel(const uint32_t* __restrict__ keys, uint64_t n_max
     if (k + 1 == n || keys[k + 1] != key) end[key] = (uint32_t)(k + 1);
 }
 
-// Per Gaussian, in the caller's order (coalesced parameter reads and gradient
-// writes): FP64 sum of its partial records in group order, then the covariance
-// chain (conic gradient -> Sigma -> M -> scales, rotations), all in double from
-// the float inputs.
-__global__ void reduce_chain_kernel(int P, const uint32_t* __restrict__ inv_perm, const uint32_t* __restrict__ begin,
-                                    const uint32_t* __restrict__ end, const uint32_t* __restrict__ sorted_vals,
-                                    const PartialRecord* __restrict__ parts, const float* __restrict__ scales,
-                                    const float* __restrict__ rots, float kmod, float* __restrict__ dmeans,
-                                    float* __restrict__ dscales, float* __restrict__ drots,
-                                    float* __restrict__ dvalues, float* __restrict__ dweights)
+// Per Gaussian, in BVH order (neighbouring Gaussians have segments of similar
+// length): a team of TEAM lanes sums the Gaussian's partial records in FP64,
+// lane j of the team taking records begin+j, begin+j+TEAM, ... in order, then a
+// fixed butterfly over the team (every lane ends with the same sums).  The order
+// depends only on the segment, so a Gaussian's gradients do not depend on where
+// its segment lies in the sorted records.  A wave covers 64 consecutive
+// Gaussians in TEAM passes, hands lane L the sums of Gaussian L, and every lane
+// runs the covariance chain (conic gradient -> Sigma -> M -> scales, rotations)
+// in double from the float inputs and writes the caller-order gradients.
+template <int TEAM>
+__global__ __launch_bounds__(256) void reduce_chain_kernel(int P, const GRecord* __restrict__ rec,
+                                                           const uint32_t* __restrict__ begin,
+                                                           const uint32_t* __restrict__ end,
+                                                           const uint32_t* __restrict__ sorted_vals,
+                                                           const PartialRecord* __restrict__ parts,
+                                                           const float* __restrict__ scales,
+                                                           const float* __restrict__ rots, float kmod,
+                                                           float* __restrict__ dmeans, float* __restrict__ dscales,
+                                                           float* __restrict__ drots, float* __restrict__ dvalues,
+                                                           float* __restrict__ dweights)
 {
-    const int id = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (id >= P) return;
-    double acc[11] = { 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0 };
-    const uint32_t i = inv_perm[id];
-    const uint32_t b = begin[i], e = end[i];
-    // loads run one record ahead of the (sequential, group-ordered) sums
-    float4 RA, RB, RC;
-    if (b < e) {
-        const float4* q = reinterpret_cast<const float4*>(&parts[sorted_vals[b]]);
-        RA = q[0]; RB = q[1]; RC = q[2];
-    }
-    for (uint32_t k = b; k < e; ++k) {
-        float4 An = RA, Bn = RB, Cn = RC;
-        if (k + 1 < e) {
-            const float4* q = reinterpret_cast<const float4*>(&parts[sorted_vals[k + 1]]);
-            An = q[0]; Bn = q[1]; Cn = q[2];
-        }
+    static_assert(TEAM >= 1 && TEAM <= 64 && (TEAM & (TEAM - 1)) == 0, "team size");
+    constexpr int kPer = 64 / TEAM;  // Gaussians per pass
+    const int lane = threadIdx.x & 63;
+    const int g0 = ((int)xcd_block(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6)) * 64;
+    if (g0 >= P) return;
+    const int team = lane / TEAM, j = lane % TEAM;
+    double mine[11];
+    for (int c = 0; c < 11; ++c) mine[c] = 0.0;
+    for (int p = 0; p < TEAM; ++p) {
+        const int i = g0 + p * kPer + team;
+        double acc[11];
+        for (int c = 0; c < 11; ++c) acc[c] = 0.0;
         // A = gid dv dw dmx | B = dmy dmz dc0 dc1 | C = dc2 dc3 dc4 dc5
-        acc[0] += RA.y; acc[1] += RA.z; acc[2] += RA.w; acc[3] += RB.x; acc[4] += RB.y;
-        acc[5] += RB.z; acc[6] += RB.w; acc[7] += RC.x; acc[8] += RC.y; acc[9] += RC.z; acc[10] += RC.w;
-        RA = An; RB = Bn; RC = Cn;
+        auto add = [&](const float4& A, const float4& B, const float4& C) {
+            acc[0] += A.y; acc[1] += A.z; acc[2] += A.w; acc[3] += B.x; acc[4] += B.y;
+            acc[5] += B.z; acc[6] += B.w; acc[7] += C.x; acc[8] += C.y; acc[9] += C.z; acc[10] += C.w;
+        };
+        if (i < P) {
+            const uint32_t e = end[i];
+            uint32_t k = begin[i] + (uint32_t)j;
+            for (; k + TEAM < e; k += 2 * TEAM) {
+                const float4* q0 = reinterpret_cast<const float4*>(&parts[sorted_vals[k]]);
+                const float4* q1 = reinterpret_cast<const float4*>(&parts[sorted_vals[k + TEAM]]);
+                const float4 A0 = q0[0], B0 = q0[1], C0 = q0[2];
+                const float4 A1 = q1[0], B1 = q1[1], C1 = q1[2];
+                add(A0, B0, C0);
+                add(A1, B1, C1);
+            }
+            if (k < e) {
+                const float4* q0 = reinterpret_cast<const float4*>(&parts[sorted_vals[k]]);
+                add(q0[0], q0[1], q0[2]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 11; ++c) {
+#pragma unroll
+            for (int o = 1; o < TEAM; o <<= 1) acc[c] += __shfl_xor(acc[c], o, 64);
+            const double x = __shfl(acc[c], (lane % kPer) * TEAM, 64);
+            if (lane / kPer == p) mine[c] = x;
+        }
     }
-    dvalues[id] = (float)acc[0];
-    dweights[id] = (float)acc[1];
-    dmeans[3 * id + 0] = (float)acc[2];
-    dmeans[3 * id + 1] = (float)acc[3];
-    dmeans[3 * id + 2] = (float)acc[4];
-
-    // covariance chain
+    const int i = g0 + lane;
+    if (i >= P) return;
+    const int id = rec[i].id;
+    dvalues[id] = (float)mine[0];
+    dweights[id] = (float)mine[1];
+    dmeans[3 * id + 0] = (float)mine[2];
+    dmeans[3 * id + 1] = (float)mine[3];
+    dmeans[3 * id + 2] = (float)mine[4];
     double ds[3], dq[4];
-    covariance_chain(acc + 5, scales + 3 * id, rots + 4 * id, kmod, ds, dq);
+    covariance_chain(mine + 5, scales + 3 * id, rots + 4 * id, kmod, ds, dq);
     for (int c = 0; c < 3; ++c) dscales[3 * id + c] = (float)ds[c];
     for (int c = 0; c < 4; ++c) drots[4 * id + c] = (float)dq[c];
 }
+
+// Team size of the gradient reduction (lanes per Gaussian).
+constexpr int kReduceTeam = 4;
 
 __global__ void zero_floats_kernel(float* p, int n)
 {
@@ -2212,7 +2243,7 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
             const unsigned nb = walk_blocks(kf, T, lds, NG);
             backward_kernel<true><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
                                                       out_weights, gcells, gweights, l.group_offsets, capacity,
-                                                      bl.parts, bl.keys, bl.vals, l.err, l.queues + kQBwd, l.ext_sum,
+                                                      bl.parts, bl.keys, l.err, l.queues + kQBwd, l.ext_sum,
                                                       l.log, l.group_head, rp, lp, l.heavy, l.queues + kQHeavyB,
                                                       l.group_flag);
         } else {
@@ -2222,7 +2253,7 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
             const unsigned nb = walk_blocks(kf, T, lds, NG);
             backward_kernel<false><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
                                                        out_weights, gcells, gweights, l.group_offsets, capacity,
-                                                       bl.parts, bl.keys, bl.vals, l.err, l.queues + kQBwd, l.ext_sum,
+                                                       bl.parts, bl.keys, l.err, l.queues + kQBwd, l.ext_sum,
                                                        l.log, l.group_head, rp, lp, l.heavy, l.queues + kQHeavyB,
                                                        l.group_flag);
         }
@@ -2236,16 +2267,18 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
     if (NG > 0 && capacity > 0) {
         bool alt = false;
         // the records emitted: group_offsets[NG] (capped at the capacity)
+        // values: the record index (generated by the first pass)
         GVC_TRY(radix_sort_u32(bl.keys, bl.vals, bl.keys_alt, bl.vals_alt, (size_t)capacity,
-                               key_bits_for((uint64_t)P), bl.scratch, &alt, st, l.group_offsets + NG));
+                               key_bits_for((uint64_t)P), bl.scratch, &alt, st, l.group_offsets + NG, true));
         skeys = alt ? bl.keys_alt : bl.keys;
         svals = alt ? bl.vals_alt : bl.vals;
         segment_kernel<<<blocks_for(capacity, 256), 256, 0, st>>>(skeys, capacity, l.group_offsets + NG, (uint32_t)P,
                                                                   bl.begin, bl.end);
         GVC_LAUNCH_CHECK("segment");
     }
-    reduce_chain_kernel<<<blocks_for(P, 128), 128, 0, st>>>(P, l.inv_perm, bl.begin, bl.end, svals, bl.parts, scales,
-                                                            rots, k, dmeans, dscales, drots, dvalues, dweights);
+    reduce_chain_kernel<kReduceTeam><<<blocks_for(P, 256), 256, 0, st>>>(P, l.rec, bl.begin, bl.end, svals, bl.parts,
+                                                                         scales, rots, k, dmeans, dscales, drots,
+                                                                         dvalues, dweights);
     GVC_LAUNCH_CHECK("reduce_chain");
     return GVC_OK;
 }

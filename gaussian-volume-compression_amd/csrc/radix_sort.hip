@@ -1,26 +1,32 @@
-// radix_sort.hip -- stable LSD radix sort of (key, uint32 value) pairs, 8-bit
-// digits, 2048-item tiles per 256-thread block.  Per pass:
-//   1. per-tile digit histogram (LDS integer atomics: order-independent counts)
+// radix_sort.hip -- stable LSD radix sort of (key, uint32 value) pairs.
+// Digits of at most 8 bits, as few passes as the key width needs (20-bit keys:
+// three 7-bit passes); 8192-item tiles per 512-thread block.  Per pass:
+//   1. per-tile digit histogram (per-wave LDS counters, integer atomics: the
+//      counts do not depend on order)
 //   2. exclusive scan of the [digit][tile] histogram -> global digit offsets
-//   3. per-tile stable ranking (wave ballot peer masks + cross-wave prefix in LDS,
-//      items taken in striped rounds so round-major order is input order), an LDS
-//      reorder into tile-sorted order, then a coalesced write-out.
+//   3. per-tile stable ranking: each wave owns 1024 consecutive items and ranks
+//      them 64 at a time (ballot peer masks; the lowest peer lane updates the
+//      wave's private digit counter), a cross-wave prefix per digit, an LDS
+//      reorder into tile-sorted order, then a write-out in digit runs.
 // Stability makes every pass, and therefore the sort, deterministic.  An optional
 // device-side count (n_dev) sorts only the first min(n, *n_dev) items: the grid is
-// sized for n, tiles past the count stay idle and contribute empty histograms.  Replaces the
-// sorts of R/gaussian_renderer/__init__.py:77 and R/train.py:311 (torch.argsort)
-// and the CUB usage of CR/rasterizer_impl.cu; nothing is allocated inside.
+// sized for n, tiles past the count stay idle and contribute empty histograms.
+// Optional identity values: the first pass takes item i's value to be i (the
+// caller's value array is then only written).  Replaces the sorts of
+// R/gaussian_renderer/__init__.py:77 and R/train.py:311 (torch.argsort) and the
+// CUB usage of CR/rasterizer_impl.cu; nothing is allocated inside.
 #include "gvc_internal.h"
 
 namespace gvc {
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kItems = 8;
-constexpr int kTile = kThreads * kItems;
+constexpr int kThreads = 512;
 constexpr int kWaves = kThreads / 64;
-constexpr int kBins = 256;
+constexpr int kItems = 16;                  // per thread (rounds per wave)
+constexpr int kTile = kThreads * kItems;    // 8192
+constexpr int kWaveItems = 64 * kItems;     // consecutive items a wave ranks
+constexpr int kMaxBins = 256;
 
 __device__ __forceinline__ size_t count_of(size_t n, const uint32_t* n_dev)
 {
@@ -30,61 +36,76 @@ __device__ __forceinline__ size_t count_of(size_t n, const uint32_t* n_dev)
 }
 
 template <class K>
-__global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* keys, size_t n_max, const uint32_t* n_dev,
-                                                              int shift, uint32_t* hist, uint32_t n_tiles)
+__global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* __restrict__ keys, size_t n_max,
+                                                              const uint32_t* __restrict__ n_dev, int shift,
+                                                              uint32_t mask, uint32_t* __restrict__ hist,
+                                                              uint32_t n_tiles)
 {
     const size_t n = count_of(n_max, n_dev);
-    __shared__ uint32_t h[kBins];
-    h[threadIdx.x] = 0;
+    __shared__ uint32_t h[kWaves][kMaxBins];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (int i = t; i < kWaves * kMaxBins; i += kThreads) (&h[0][0])[i] = 0u;
     __syncthreads();
-    const size_t base = (size_t)blockIdx.x * kTile;
-#pragma unroll
+    const size_t base = (size_t)blockIdx.x * kTile + (size_t)w * kWaveItems + lane;
+#pragma unroll 4
     for (int r = 0; r < kItems; ++r) {
-        size_t i = base + (size_t)r * kThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 0xffu], 1u);
+        const size_t i = base + (size_t)r * 64;
+        if (i < n) atomicAdd(&h[w][(uint32_t)(keys[i] >> shift) & mask], 1u);
     }
     __syncthreads();
-    hist[(size_t)threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
+    for (uint32_t d = t; d <= mask; d += kThreads) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int v = 0; v < kWaves; ++v) s += h[v][d];
+        hist[(size_t)d * n_tiles + blockIdx.x] = s;
+    }
 }
 
+// Dynamic LDS: kTile keys, kTile values.
 template <class K>
-__global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* keys, const uint32_t* vals, K* keys_out,
-                                                                 uint32_t* vals_out, size_t n_max,
-                                                                 const uint32_t* n_dev, int shift,
-                                                                 const uint32_t* offsets, uint32_t n_tiles)
+constexpr size_t scatter_lds() { return (size_t)kTile * (sizeof(K) + sizeof(uint32_t)); }
+
+template <class K>
+__global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __restrict__ keys,
+                                                                 const uint32_t* __restrict__ vals,
+                                                                 K* __restrict__ keys_out,
+                                                                 uint32_t* __restrict__ vals_out, size_t n_max,
+                                                                 const uint32_t* __restrict__ n_dev, int shift,
+                                                                 uint32_t mask, const uint32_t* __restrict__ offsets,
+                                                                 uint32_t n_tiles)
 {
     const size_t n = count_of(n_max, n_dev);
-    if ((size_t)blockIdx.x * kTile >= n) return;
-    __shared__ uint32_t cnt[kWaves][kBins];
-    __shared__ uint32_t run[kBins];
-    __shared__ uint32_t start[kBins];
-    __shared__ K skeys[kTile];
-    __shared__ uint32_t svals[kTile];
-    __shared__ uint32_t wtot[kWaves];
-
-    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     const size_t base = (size_t)blockIdx.x * kTile;
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    if (base >= n) return;
+    extern __shared__ __align__(16) unsigned char dyn[];
+    K* skeys = reinterpret_cast<K*>(dyn);
+    uint32_t* svals = reinterpret_cast<uint32_t*>(dyn + (size_t)kTile * sizeof(K));
+    __shared__ uint32_t cnt[kWaves][kMaxBins];
+    __shared__ uint32_t start[kMaxBins];
+    __shared__ uint32_t gofs[kMaxBins];
+    __shared__ uint32_t wsum[kWaves];
 
-    run[t] = 0;
-    K k_reg[kItems];
-    uint32_t v_reg[kItems];
-    uint32_t d_reg[kItems];
-    uint32_t rank[kItems];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int i = t; i < kWaves * kMaxBins; i += kThreads) (&cnt[0][0])[i] = 0u;
+    const size_t wbase = base + (size_t)w * kWaveItems + lane;
+
+    K kr[kItems];
+    uint32_t vr[kItems];
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
-        size_t i = base + (size_t)r * kThreads + t;
-        bool ok = i < n;
-        k_reg[r] = ok ? keys[i] : K(0);
-        v_reg[r] = ok ? vals[i] : 0u;
-        d_reg[r] = ok ? ((uint32_t)(k_reg[r] >> shift) & 0xffu) : 256u;
+        const size_t i = wbase + (size_t)r * 64;
+        const bool ok = i < n;
+        kr[r] = ok ? keys[i] : K(0);
+        vr[r] = ok ? (vals ? vals[i] : (uint32_t)i) : 0u;
     }
+    __syncthreads();
+    // wave-private stable ranks: round r holds the wave's items r*64 .. r*64+63
+    uint32_t rk[kItems];
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) cnt[w][t] = 0;
-        __syncthreads();
-        const uint32_t d = d_reg[r];
+        const bool ok = wbase + (size_t)r * 64 < n;
+        const uint32_t d = ok ? ((uint32_t)(kr[r] >> shift) & mask) : (uint32_t)kMaxBins;
         uint64_t peers = ~0ull;
 #pragma unroll
         for (int b = 0; b < 9; ++b) {
@@ -92,85 +113,117 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* keys, 
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
         }
-        const uint32_t below = (uint32_t)__popcll(peers & lt_mask);
-        if (d < 256u && below == 0) cnt[wid][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        {
-            uint32_t acc = run[t];
-#pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                uint32_t c = cnt[w][t];
-                cnt[w][t] = acc;
-                acc += c;
-            }
-            run[t] = acc;
+        const int leader = __builtin_ctzll(peers);
+        uint32_t old = 0u;
+        if (ok && lane == leader) {
+            old = cnt[w][d];
+            cnt[w][d] = old + (uint32_t)__popcll(peers);
         }
-        __syncthreads();
-        rank[r] = (d < 256u) ? cnt[wid][d] + below : 0u;
-        __syncthreads();
+        old = __shfl(old, leader, 64);
+        rk[r] = old + (uint32_t)__popcll(peers & lt_mask);
     }
-    // tile-local exclusive scan of the digit counts
+    __syncthreads();
+    // per digit: exclusive prefix over the waves, then over the digits
     {
-        uint32_t x = run[t];
-        uint32_t inc = x;
+        const uint32_t d = (uint32_t)t;
+        uint32_t tot = 0u;
+        if (d < (uint32_t)kMaxBins) {
+#pragma unroll
+            for (int v = 0; v < kWaves; ++v) {
+                const uint32_t c = cnt[v][d];
+                cnt[v][d] = tot;
+                tot += c;
+            }
+        }
+        uint32_t inc = tot;
 #pragma unroll
         for (int dd = 1; dd < 64; dd <<= 1) {
-            uint32_t y = __shfl_up(inc, dd, 64);
+            const uint32_t y = __shfl_up(inc, dd, 64);
             if (lane >= dd) inc += y;
         }
-        if (lane == 63) wtot[wid] = inc;
+        if (lane == 63) wsum[w] = inc;
         __syncthreads();
-        uint32_t b = 0;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w)
-            if (w < wid) b += wtot[w];
-        start[t] = b + inc - x;
+        if (d < (uint32_t)kMaxBins) {
+            uint32_t b = 0u;
+            for (int v = 0; v < w; ++v) b += wsum[v];
+            const uint32_t s0 = b + inc - tot;
+            start[d] = s0;
+            gofs[d] = (d <= mask) ? offsets[(size_t)d * n_tiles + blockIdx.x] - s0 : 0u;
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
-        const uint32_t d = d_reg[r];
-        if (d < 256u) {
-            const uint32_t pos = start[d] + rank[r];
-            skeys[pos] = k_reg[r];
-            svals[pos] = v_reg[r];
+        if (wbase + (size_t)r * 64 < n) {
+            const uint32_t d = (uint32_t)(kr[r] >> shift) & mask;
+            const uint32_t pos = start[d] + cnt[w][d] + rk[r];
+            skeys[pos] = kr[r];
+            svals[pos] = vr[r];
         }
     }
     __syncthreads();
-    const size_t tile_n = (n - base) < (size_t)kTile ? (n - base) : (size_t)kTile;
+    const uint32_t tile_n = (uint32_t)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
     for (uint32_t k = t; k < tile_n; k += kThreads) {
         const K key = skeys[k];
-        const uint32_t d = (uint32_t)(key >> shift) & 0xffu;
-        const size_t g = (size_t)offsets[(size_t)d * n_tiles + blockIdx.x] + (k - start[d]);
+        const uint32_t d = (uint32_t)(key >> shift) & mask;
+        const size_t g = (size_t)(gofs[d] + k);
         keys_out[g] = key;
         vals_out[g] = svals[k];
     }
 }
 
 template <class K>
+int allow_scatter_lds()
+{
+    static bool done = false;
+    if (!done) {
+        GVC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&radix_scatter_kernel<K>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)scatter_lds<K>()));
+        done = true;
+    }
+    return GVC_OK;
+}
+
+template <class K>
 int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, size_t n, const uint32_t* n_dev,
-                    int key_bits, uint32_t* scratch, bool* result_in_alt, hipStream_t st)
+                    int key_bits, uint32_t* scratch, bool* result_in_alt, hipStream_t st, bool vals_identity)
 {
     *result_in_alt = false;
-    if (n <= 1 || key_bits <= 0) return GVC_OK;
+    if (n <= 1 || key_bits <= 0) {
+        // nothing to reorder; identity values still have to be written
+        if (vals_identity && n > 1)
+            return set_error(GVC_ERR_INVALID_ARGUMENT, "radix sort: identity values need a key to sort on");
+        if (vals_identity && n == 1) GVC_TRY(fill_u32(vals, 0u, 1, st));
+        return GVC_OK;
+    }
     if (n > 0xffffffffull) return set_error(GVC_ERR_INVALID_ARGUMENT, "radix sort: more than 2^32 items");
+    if (key_bits > (int)(8 * sizeof(K))) key_bits = (int)(8 * sizeof(K));
+    GVC_TRY(allow_scatter_lds<K>());
     const uint32_t n_tiles = (uint32_t)((n + kTile - 1) / kTile);
     uint32_t* hist = scratch;
-    uint32_t* scan_scratch = scratch + (size_t)kBins * n_tiles;
+    uint32_t* scan_scratch = scratch + (size_t)kMaxBins * n_tiles;
+    const int passes = (key_bits + 7) / 8;
+    const int width = (key_bits + passes - 1) / passes;
     K* src_k = keys;
-    uint32_t* src_v = vals;
+    const uint32_t* src_v = vals_identity ? nullptr : vals;
     K* dst_k = keys_alt;
     uint32_t* dst_v = vals_alt;
+    uint32_t* other_v = vals;
     bool in_alt = false;
-    for (int shift = 0; shift < key_bits; shift += 8) {
-        radix_hist_kernel<K><<<n_tiles, kThreads, 0, st>>>(src_k, n, n_dev, shift, hist, n_tiles);
+    for (int shift = 0; shift < key_bits; shift += width) {
+        const int bits = (key_bits - shift) < width ? (key_bits - shift) : width;
+        const uint32_t mask = (1u << bits) - 1u;
+        radix_hist_kernel<K><<<n_tiles, kThreads, 0, st>>>(src_k, n, n_dev, shift, mask, hist, n_tiles);
         GVC_LAUNCH_CHECK("radix_hist");
-        GVC_TRY(exclusive_scan_u32(hist, hist, (size_t)kBins * n_tiles, false, scan_scratch, st));
-        radix_scatter_kernel<K><<<n_tiles, kThreads, 0, st>>>(src_k, src_v, dst_k, dst_v, n, n_dev, shift, hist,
-                                                              n_tiles);
+        GVC_TRY(exclusive_scan_u32(hist, hist, (size_t)(mask + 1) * n_tiles, false, scan_scratch, st));
+        radix_scatter_kernel<K><<<n_tiles, kThreads, scatter_lds<K>(), st>>>(src_k, src_v, dst_k, dst_v, n, n_dev,
+                                                                            shift, mask, hist, n_tiles);
         GVC_LAUNCH_CHECK("radix_scatter");
         K* tk = src_k; src_k = dst_k; dst_k = tk;
-        uint32_t* tv = src_v; src_v = dst_v; dst_v = tv;
+        uint32_t* nv = other_v;
+        other_v = dst_v;
+        src_v = dst_v;
+        dst_v = nv;
         in_alt = !in_alt;
     }
     *result_in_alt = in_alt;
@@ -182,21 +235,21 @@ int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, si
 size_t radix_scratch_elems(size_t n)
 {
     const size_t n_tiles = (n + kTile - 1) / kTile;
-    return (size_t)kBins * n_tiles + scan_scratch_elems((size_t)kBins * n_tiles) + 1;
+    return (size_t)kMaxBins * n_tiles + scan_scratch_elems((size_t)kMaxBins * n_tiles) + 1;
 }
 
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, size_t n, int key_bits,
-                   uint32_t* scratch, bool* result_in_alt, hipStream_t st, const uint32_t* n_dev)
+                   uint32_t* scratch, bool* result_in_alt, hipStream_t st, const uint32_t* n_dev, bool vals_identity)
 {
     return radix_sort_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, n_dev, key_bits, scratch, result_in_alt,
-                                     st);
+                                     st, vals_identity);
 }
 
 int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, size_t n, int key_bits,
                    uint32_t* scratch, bool* result_in_alt, hipStream_t st)
 {
     return radix_sort_impl<uint64_t>(keys, vals, keys_alt, vals_alt, n, nullptr, key_bits, scratch, result_in_alt,
-                                     st);
+                                     st, false);
 }
 
 }  // namespace gvc
