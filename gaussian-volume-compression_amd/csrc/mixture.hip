@@ -1665,26 +1665,24 @@ __global__ void segment_kernel(const uint32_t* __restrict__ keys, uint64_t n_max
     if (k + 1 == n || keys[k + 1] != key) end[key] = (uint32_t)(k + 1);
 }
 
+// Gradient sums of one Gaussian, in BVH order: 11 doubles and a pad (96 bytes).
+constexpr int kSumStride = 12;
+
 // Per Gaussian, in BVH order (neighbouring Gaussians have segments of similar
 // length): a team of TEAM lanes sums the Gaussian's partial records in FP64,
 // lane j of the team taking records begin+j, begin+j+TEAM, ... in order, then a
 // fixed butterfly over the team (every lane ends with the same sums).  The order
 // depends only on the segment, so a Gaussian's gradients do not depend on where
 // its segment lies in the sorted records.  A wave covers 64 consecutive
-// Gaussians in TEAM passes, hands lane L the sums of Gaussian L, and every lane
-// runs the covariance chain (conic gradient -> Sigma -> M -> scales, rotations)
-// in double from the float inputs and writes the caller-order gradients.
+// Gaussians in TEAM passes (lane L first loads Gaussian L's segment bounds; a
+// pass whose teams have no records is skipped), hands lane L the sums of
+// Gaussian L and writes them to sums[] in BVH order.
 template <int TEAM>
-__global__ __launch_bounds__(256) void reduce_chain_kernel(int P, const GRecord* __restrict__ rec,
-                                                           const uint32_t* __restrict__ begin,
-                                                           const uint32_t* __restrict__ end,
-                                                           const uint32_t* __restrict__ sorted_vals,
-                                                           const PartialRecord* __restrict__ parts,
-                                                           const float* __restrict__ scales,
-                                                           const float* __restrict__ rots, float kmod,
-                                                           float* __restrict__ dmeans, float* __restrict__ dscales,
-                                                           float* __restrict__ drots, float* __restrict__ dvalues,
-                                                           float* __restrict__ dweights)
+__global__ __launch_bounds__(256) void reduce_sums_kernel(int P, const uint32_t* __restrict__ begin,
+                                                          const uint32_t* __restrict__ end,
+                                                          const uint32_t* __restrict__ sorted_vals,
+                                                          const PartialRecord* __restrict__ parts,
+                                                          double* __restrict__ sums)
 {
     static_assert(TEAM >= 1 && TEAM <= 64 && (TEAM & (TEAM - 1)) == 0, "team size");
     constexpr int kPer = 64 / TEAM;  // Gaussians per pass
@@ -1692,10 +1690,17 @@ __global__ __launch_bounds__(256) void reduce_chain_kernel(int P, const GRecord*
     const int g0 = ((int)xcd_block(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6)) * 64;
     if (g0 >= P) return;
     const int team = lane / TEAM, j = lane % TEAM;
+    const int gi = g0 + lane;
+    const bool valid = gi < P;
+    const uint32_t my_b = valid ? begin[gi] : 0u;
+    const uint32_t my_e = valid ? end[gi] : 0u;
     double mine[11];
     for (int c = 0; c < 11; ++c) mine[c] = 0.0;
     for (int p = 0; p < TEAM; ++p) {
-        const int i = g0 + p * kPer + team;
+        const int src = p * kPer + team;
+        const uint32_t b = (uint32_t)__shfl((int)my_b, src, 64);
+        const uint32_t e = (uint32_t)__shfl((int)my_e, src, 64);
+        if (__ballot(e > b) == 0ull) continue;
         double acc[11];
         for (int c = 0; c < 11; ++c) acc[c] = 0.0;
         // A = gid dv dw dmx | B = dmy dmz dc0 dc1 | C = dc2 dc3 dc4 dc5
@@ -1703,21 +1708,18 @@ __global__ __launch_bounds__(256) void reduce_chain_kernel(int P, const GRecord*
             acc[0] += A.y; acc[1] += A.z; acc[2] += A.w; acc[3] += B.x; acc[4] += B.y;
             acc[5] += B.z; acc[6] += B.w; acc[7] += C.x; acc[8] += C.y; acc[9] += C.z; acc[10] += C.w;
         };
-        if (i < P) {
-            const uint32_t e = end[i];
-            uint32_t k = begin[i] + (uint32_t)j;
-            for (; k + TEAM < e; k += 2 * TEAM) {
-                const float4* q0 = reinterpret_cast<const float4*>(&parts[sorted_vals[k]]);
-                const float4* q1 = reinterpret_cast<const float4*>(&parts[sorted_vals[k + TEAM]]);
-                const float4 A0 = q0[0], B0 = q0[1], C0 = q0[2];
-                const float4 A1 = q1[0], B1 = q1[1], C1 = q1[2];
-                add(A0, B0, C0);
-                add(A1, B1, C1);
-            }
-            if (k < e) {
-                const float4* q0 = reinterpret_cast<const float4*>(&parts[sorted_vals[k]]);
-                add(q0[0], q0[1], q0[2]);
-            }
+        uint32_t k = b + (uint32_t)j;
+        for (; k + TEAM < e; k += 2 * TEAM) {
+            const float4* q0 = reinterpret_cast<const float4*>(&parts[sorted_vals[k]]);
+            const float4* q1 = reinterpret_cast<const float4*>(&parts[sorted_vals[k + TEAM]]);
+            const float4 A0 = q0[0], B0 = q0[1], C0 = q0[2];
+            const float4 A1 = q1[0], B1 = q1[1], C1 = q1[2];
+            add(A0, B0, C0);
+            add(A1, B1, C1);
+        }
+        if (k < e) {
+            const float4* q0 = reinterpret_cast<const float4*>(&parts[sorted_vals[k]]);
+            add(q0[0], q0[1], q0[2]);
         }
 #pragma unroll
         for (int c = 0; c < 11; ++c) {
@@ -1727,16 +1729,46 @@ __global__ __launch_bounds__(256) void reduce_chain_kernel(int P, const GRecord*
             if (lane / kPer == p) mine[c] = x;
         }
     }
-    const int i = g0 + lane;
-    if (i >= P) return;
-    const int id = rec[i].id;
-    dvalues[id] = (float)mine[0];
-    dweights[id] = (float)mine[1];
-    dmeans[3 * id + 0] = (float)mine[2];
-    dmeans[3 * id + 1] = (float)mine[3];
-    dmeans[3 * id + 2] = (float)mine[4];
-    double ds[3], dq[4];
-    covariance_chain(mine + 5, scales + 3 * id, rots + 4 * id, kmod, ds, dq);
+    if (!valid) return;
+    double2* o = reinterpret_cast<double2*>(sums + (size_t)kSumStride * gi);
+    o[0] = make_double2(mine[0], mine[1]);
+    o[1] = make_double2(mine[2], mine[3]);
+    o[2] = make_double2(mine[4], mine[5]);
+    o[3] = make_double2(mine[6], mine[7]);
+    o[4] = make_double2(mine[8], mine[9]);
+    o[5] = make_double2(mine[10], 0.0);
+}
+
+// Per Gaussian, in the caller's order (coalesced parameter reads and gradient
+// writes; one 96-byte read of the BVH-order sums): gradients of value, weight
+// and mean, and the covariance chain (conic gradient -> Sigma -> M -> scales,
+// rotations) in double from the float inputs.  A zero conic gradient (no
+// records) gives zero scale and rotation gradients without the chain.
+__global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __restrict__ inv_perm,
+                                                    const double* __restrict__ sums, const float* __restrict__ scales,
+                                                    const float* __restrict__ rots, float kmod,
+                                                    float* __restrict__ dmeans, float* __restrict__ dscales,
+                                                    float* __restrict__ drots, float* __restrict__ dvalues,
+                                                    float* __restrict__ dweights)
+{
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= P) return;
+    const double2* s2 = reinterpret_cast<const double2*>(sums + (size_t)kSumStride * inv_perm[id]);
+    double m[12];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        const double2 v = s2[c];
+        m[2 * c] = v.x;
+        m[2 * c + 1] = v.y;
+    }
+    dvalues[id] = (float)m[0];
+    dweights[id] = (float)m[1];
+    dmeans[3 * id + 0] = (float)m[2];
+    dmeans[3 * id + 1] = (float)m[3];
+    dmeans[3 * id + 2] = (float)m[4];
+    double ds[3] = { 0.0, 0.0, 0.0 }, dq[4] = { 0.0, 0.0, 0.0, 0.0 };
+    const bool any = m[5] != 0.0 || m[6] != 0.0 || m[7] != 0.0 || m[8] != 0.0 || m[9] != 0.0 || m[10] != 0.0;
+    if (any) covariance_chain(m + 5, scales + 3 * id, rots + 4 * id, kmod, ds, dq);
     for (int c = 0; c < 3; ++c) dscales[3 * id + c] = (float)ds[c];
     for (int c = 0; c < 4; ++c) drots[4 * id + c] = (float)dq[c];
 }
@@ -1947,6 +1979,7 @@ struct ForwardLayout {
 struct BackwardLayout {
     PartialRecord* parts;
     uint32_t *keys, *vals, *keys_alt, *vals_alt, *scratch, *begin, *end;
+    double* sums;  // kSumStride per Gaussian, BVH order
     BackwardLayout(Arena& a, int P, uint64_t cap)
     {
         const size_t c = (size_t)(cap > 0 ? cap : 1);
@@ -1958,6 +1991,7 @@ struct BackwardLayout {
         scratch = a.take<uint32_t>(radix_scratch_elems(c));
         begin = a.take<uint32_t>((size_t)(P > 0 ? P : 1));
         end = a.take<uint32_t>((size_t)(P > 0 ? P : 1));
+        sums = a.take<double>((size_t)kSumStride * (size_t)(P > 0 ? P : 1));
     }
 };
 
@@ -2276,10 +2310,12 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
                                                                   bl.begin, bl.end);
         GVC_LAUNCH_CHECK("segment");
     }
-    reduce_chain_kernel<kReduceTeam><<<blocks_for(P, 256), 256, 0, st>>>(P, l.rec, bl.begin, bl.end, svals, bl.parts,
-                                                                         scales, rots, k, dmeans, dscales, drots,
-                                                                         dvalues, dweights);
-    GVC_LAUNCH_CHECK("reduce_chain");
+    reduce_sums_kernel<kReduceTeam><<<blocks_for(P, 256), 256, 0, st>>>(P, bl.begin, bl.end, svals, bl.parts,
+                                                                        bl.sums);
+    GVC_LAUNCH_CHECK("reduce_sums");
+    chain_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, bl.sums, scales, rots, k, dmeans, dscales, drots,
+                                                     dvalues, dweights);
+    GVC_LAUNCH_CHECK("chain");
     return GVC_OK;
 }
 
