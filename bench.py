@@ -299,6 +299,7 @@ def main():
                    "run_points": params.run_points, "kahan": params.kahan, "candidate_log": params.candidate_log,
                    "fp64_cond": params.fp64_cond, "graph": bool(args.graph), "parallelism": "single GPU"},
         "pairs_per_step": pairs_per_step,
+        "records_per_step": total_records / args.steps,
         "groups_walked_per_lane": lane_groups,
         "precision_study": study,
         "autotune": tune,
@@ -526,7 +527,7 @@ def main_batch(args):
     torch.cuda.synchronize()
     fused_s = (time.perf_counter() - t0) / args.steps
     mx.set_audit(None)
-    pairs, _, errs = (int(x) for x in acc.cpu().tolist())
+    pairs, records, errs = (int(x) for x in acc.cpu().tolist())
     if errs:
         raise RuntimeError(f"mixture error flags during the timed steps: {errs}")
     fused_pairs = pairs / args.steps
@@ -568,6 +569,7 @@ def main_batch(args):
                    "block_threads": params.block_threads, "run_points": params.run_points, "kahan": params.kahan,
                    "candidate_log": params.candidate_log, "parallelism": "single GPU"},
         "pairs_per_step": fused_pairs,
+        "records_per_step": records / args.steps,
         "groups": int(batch.c.groups),
         "groups_walked_per_lane": lane_groups,
         "evals_per_s": 2.0 * fused_pairs / fused_s,

@@ -275,34 +275,61 @@ __device__ __forceinline__ int tet_filter(const TetRec& r, const float* p)
     return all_in ? 1 : -1;
 }
 
-constexpr int kNeedExact = -2147483647 - 1;
+// A sample the filter pass leaves undecided holds cell_ids[s] = -2 - i, i = its
+// position in that pass's order (decided samples hold >= -1).
+__device__ __forceinline__ int mark_undecided(int i) { return -2 - i; }
 
-// sorted != nullptr: the samples are taken from a sample set (Morton order, so
-// neighbouring threads share grid cells and tets) and results go to the original
-// index carried in .w.  EXACT = false: candidates the filter cannot decide mark
-// the sample kNeedExact (no result written); EXACT = true: only those samples are
-// redone, the undecided candidates by the exact predicate.
-template <bool EXACT>
-__global__ void tet_locate_kernel(const float* __restrict__ pts, const TetRec* __restrict__ recs,
-                                  const TetGridHeader* __restrict__ h, const uint32_t* __restrict__ starts,
-                                  const uint2* __restrict__ entries, const double* __restrict__ values, int S,
-                                  const float* __restrict__ samples, const float4* __restrict__ sorted,
-                                  int* __restrict__ cell_ids, double* __restrict__ out_values,
-                                  float* __restrict__ bary_out)
+constexpr int kLocateThreads = 256;
+constexpr int kCandBatch = 8;  // candidates a lane collects before testing them
+
+// Per-axis 5-bit fields of an entry: lo <= qp <= hi (padding: lo.x = 31 > hi.x =
+// 0, never passes).
+__device__ __forceinline__ bool entry_box_in(uint32_t q, const uint32_t qp[3])
 {
-    const int i = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= S) return;
-    int s = i;
+    return ((q & 31u) <= qp[0]) && (((q >> 5) & 31u) <= qp[1]) && (((q >> 10) & 31u) <= qp[2]) &&
+           (((q >> 15) & 31u) >= qp[0]) && (((q >> 20) & 31u) >= qp[1]) && (((q >> 25) & 31u) >= qp[2]);
+}
+
+// EXACT = false (filter pass): threads follow the sample set's Morton order when
+// one is given (sorted != nullptr; neighbouring threads share grid cells and
+// tets; results go to the original index carried in .w), else the caller's
+// order; a sample whose first undecided candidate comes before any decided one
+// is marked (mark_undecided, no result written).  EXACT = true: threads follow
+// the caller's order (coalesced reads of the marks); only marked samples are
+// redone, undecided candidates by the exact predicate.
+// A lane first collects the entries of its grid cell that pass the quantised box
+// test (up to kCandBatch, in tet order, in LDS), then tests those with the
+// filter, the next candidate's record in flight, so a wave runs the filter once
+// per candidate of its busiest lane rather than once per entry of its longest
+// cell.
+template <bool EXACT>
+__global__ __launch_bounds__(kLocateThreads) void tet_locate_kernel(
+    const float* __restrict__ pts, const TetRec* __restrict__ recs, const TetGridHeader* __restrict__ h,
+    const uint32_t* __restrict__ starts, const uint2* __restrict__ entries, const double* __restrict__ values, int S,
+    const float* __restrict__ samples, const float4* __restrict__ sorted, int* __restrict__ cell_ids,
+    double* __restrict__ out_values, float* __restrict__ bary_out)
+{
+    __shared__ int cand[kCandBatch][kLocateThreads];
+    const int tid = threadIdx.x;
+    const int i0 = (int)xcd_block(blockIdx.x, gridDim.x) * kLocateThreads + tid;
+    if (i0 >= S) return;
+    int s = i0, i = i0;
+    if (EXACT) {
+        const int mark = cell_ids[s];
+        if (mark > -2) return;
+        i = -2 - mark;
+    }
     float p[3];
     if (sorted) {
         const float4 q = sorted[i];
         p[0] = q.x; p[1] = q.y; p[2] = q.z;
-        s = __builtin_bit_cast(int, q.w);
-        if (s < 0) return;
+        if (!EXACT) {
+            s = __builtin_bit_cast(int, q.w);
+            if (s < 0) return;
+        }
     } else {
-        p[0] = samples[3 * s]; p[1] = samples[3 * s + 1]; p[2] = samples[3 * s + 2];
+        p[0] = samples[3 * i]; p[1] = samples[3 * i + 1]; p[2] = samples[3 * i + 2];
     }
-    if (EXACT && cell_ids[s] != kNeedExact) return;
     int best = -1;
     bool undecided = false;
     bool inside_grid = true;
@@ -319,42 +346,50 @@ __global__ void tet_locate_kernel(const float* __restrict__ pts, const TetRec* _
         // entries of a cell are sorted by tet index: the first tet that contains
         // p is the answer (lowest index wins), unless an earlier one is undecided
         const size_t g = (size_t)c[0] + (size_t)h->dims[0] * ((size_t)c[1] + (size_t)h->dims[1] * c[2]);
-        const uint32_t b = starts[g], e = starts[g + 1];
+        uint32_t k = starts[g];
+        const uint32_t e = starts[g + 1];
         bool done = false;
-        for (uint32_t k0 = b; k0 < e && !done; k0 += 4) {
-            uint2 en[4];
+        while (!done && k < e) {
+            int n = 0;
+            while (k < e && n <= kCandBatch - 4) {
+                uint2 en[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) en[u] = (k0 + u < e) ? entries[k0 + u] : make_uint2(0u, 31u);
+                for (int u = 0; u < 4; ++u) en[u] = (k + u < e) ? entries[k + u] : make_uint2(0u, 31u);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const uint32_t q = en[u].y;
-                // per-axis 5-bit fields: lo <= qp <= hi (padding: lo.x = 31 > hi.x = 0, never passes)
-                const bool in = ((q & 31u) <= qp[0]) && (((q >> 5) & 31u) <= qp[1]) && (((q >> 10) & 31u) <= qp[2]) &&
-                                (((q >> 15) & 31u) >= qp[0]) && (((q >> 20) & 31u) >= qp[1]) &&
-                                (((q >> 25) & 31u) >= qp[2]);
-                if (!in || done) continue;
-                const int t = (int)en[u].x;
-                const TetRec r = recs[t];
+                for (int u = 0; u < 4; ++u)
+                    if (entry_box_in(en[u].y, qp)) cand[n++][tid] = (int)en[u].x;
+                k += 4;
+            }
+            TetRec rn;
+            if (n > 0) rn = recs[cand[0][tid]];
+            for (int j = 0; j < n; ++j) {
+                const int t = cand[j][tid];
+                const TetRec r = rn;
+                if (j + 1 < n) rn = recs[cand[j + 1][tid]];
                 const int f = tet_filter(r, p);
                 if (f == 1) {
                     best = t;
                     done = true;
-                } else if (f == -1) {
+                    break;
+                }
+                if (f == -1) {
                     if (EXACT) {
                         if (tet_contains_exact(pts, r, p)) {
                             best = t;
                             done = true;
+                            break;
                         }
                     } else {
                         undecided = true;
                         done = true;
+                        break;
                     }
                 }
             }
         }
     }
     if (!EXACT && undecided) {
-        cell_ids[s] = kNeedExact;
+        cell_ids[s] = mark_undecided(i);
         return;
     }
     cell_ids[s] = best;
@@ -630,11 +665,13 @@ int tet_locate_interp(int n_points, const float* pts, int C, const void* loc, si
     TetLayout l(a, C, max_grid_cells(C), cap);
     if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "tet locator buffer too small");
     const float4* sorted = sample_set ? sample_set_points(sample_set, S) : nullptr;
-    tet_locate_kernel<false><<<blocks_for(S, 256), 256, 0, st>>>(pts, l.recs, l.header, l.starts, l.entries, values,
-                                                                 S, samples, sorted, cell_ids, out_values, bary);
+    if ((size_t)S > (size_t)INT32_MAX - 2) return set_error(GVC_ERR_INVALID_ARGUMENT, "too many samples");
+    const unsigned nb = blocks_for((size_t)S, kLocateThreads);
+    tet_locate_kernel<false><<<nb, kLocateThreads, 0, st>>>(pts, l.recs, l.header, l.starts, l.entries, values, S,
+                                                            samples, sorted, cell_ids, out_values, bary);
     GVC_LAUNCH_CHECK("tet_locate");
-    tet_locate_kernel<true><<<blocks_for(S, 256), 256, 0, st>>>(pts, l.recs, l.header, l.starts, l.entries, values,
-                                                                S, samples, sorted, cell_ids, out_values, bary);
+    tet_locate_kernel<true><<<nb, kLocateThreads, 0, st>>>(pts, l.recs, l.header, l.starts, l.entries, values, S,
+                                                           samples, sorted, cell_ids, out_values, bary);
     GVC_LAUNCH_CHECK("tet_locate_exact");
     return GVC_OK;
 }
