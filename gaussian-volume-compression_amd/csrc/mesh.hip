@@ -16,11 +16,12 @@
 // entry: the tet id and that bounding box quantised conservatively to 1/32 of the
 // cell per axis (5 bits per bound), so most candidates are rejected from the
 // entry alone.  The survivors are tested with a per-tet barycentric matrix (float,
-// precomputed in double): b = M (p - v3); a decision is taken only when every
-// |b_i| exceeds a margin e_i = 1e-6 sum_j |M_ij d_j| + 2e-9 |M|_1 |d|_1 with
-// d = p - v3: the first term covers the float rounding of each M_ij (16x 2^-24),
-// the second the error of the double inverse, |dM_ij| <= 6 u cond_F(T) |M|_F with
-// u = 2^-53 and cond_F(T) <= 1e6 (a 300x margin).  Otherwise -- and always for
+// precomputed in double): b = M (p - v3), evaluated in float; a decision is taken
+// only when every |b_i| exceeds a margin e_i = 2e-6 sum_j |M_ij d_j| + 2e-9
+// |M|_1 |d|_1 with d = p - v3 (e_3 also 6e-7 (1 + sum |b_i|)): the first term
+// covers the float rounding of each M_ij and of the evaluation (<= 5 x 2^-24, a
+// 6x margin), the second the error of the double inverse, |dM_ij| <= 6 u
+// cond_F(T) |M|_F with u = 2^-53 and cond_F(T) <= 1e6 (a 300x margin).  Otherwise -- and always for
 // tets with cond_F(T) > 1e6 -- the exact predicate decides.  A cell's entries are
 // sorted by tet index, so the scan stops at the first tet that contains the point
 // (lowest index wins) or at the first undecided one (the exact pass redoes it).

@@ -256,19 +256,26 @@ __global__ void gaussian_codes_kernel(const float* __restrict__ means, int P, ui
     idx[i] = (uint32_t)i;
 }
 
-// Canonical preprocess, written as records in BVH (sorted) order; inv_perm[j] =
-// the record index of the caller's Gaussian j.
-__global__ __launch_bounds__(256) void record_kernel(int P, const uint32_t* __restrict__ perm,
+// inv_perm[perm[i]] = i: the record (BVH) index of each caller's Gaussian.
+__global__ __launch_bounds__(256) void inv_perm_kernel(int P, const uint32_t* __restrict__ perm,
+                                                       uint32_t* __restrict__ inv_perm)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P) inv_perm[perm[i]] = (uint32_t)i;
+}
+
+// Canonical preprocess, in the caller's order (coalesced parameter reads and
+// conic writes), written as the record of BVH index inv_perm[j].
+__global__ __launch_bounds__(256) void record_kernel(int P, const uint32_t* __restrict__ inv_perm,
                                                      const float* __restrict__ means,
                                                      const float* __restrict__ scales, const float* __restrict__ rots,
                                                      const float* __restrict__ values,
                                                      const float* __restrict__ weights, float k, int fp64_cond,
-                                                     GRecord* __restrict__ rec, float* __restrict__ conics_out,
-                                                     uint32_t* __restrict__ inv_perm)
+                                                     GRecord* __restrict__ rec, float* __restrict__ conics_out)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P) return;
-    const int j = (int)perm[i];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P) return;
+    const uint32_t i = inv_perm[j];
     float mu[3] = { means[3 * j], means[3 * j + 1], means[3 * j + 2] };
     float sc[3] = { scales[3 * j], scales[3 * j + 1], scales[3 * j + 2] };
     float q[4] = { rots[4 * j], rots[4 * j + 1], rots[4 * j + 2], rots[4 * j + 3] };
@@ -284,7 +291,6 @@ __global__ __launch_bounds__(256) void record_kernel(int P, const uint32_t* __re
     r.hix = bx[3]; r.hiy = bx[4]; r.hiz = bx[5];
     r.pad0 = 0.f; r.pad1 = 0.f;
     rec[i] = r;
-    inv_perm[j] = (uint32_t)i;
     if (conics_out)
         for (int c = 0; c < 6; ++c) conics_out[6 * j + c] = cn[c];
 }
@@ -2134,8 +2140,9 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
             perm = perm2;
         }
         // 2. records
-        record_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, perm, means, scales, rots, values, weights, k, fp64,
-                                                          l.rec, conics, l.inv_perm);
+        inv_perm_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, perm, l.inv_perm);
+        record_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, means, scales, rots, values, weights, k, fp64,
+                                                          l.rec, conics);
         GVC_LAUNCH_CHECK("record");
         extent_kernel<<<bt.V, 256, 0, st>>>(l.rec, bv, l.ext_sum);
         GVC_LAUNCH_CHECK("extent");
