@@ -4,9 +4,10 @@
 //   sample set   : 30-bit Morton keys of the samples -> stable radix sort -> float4
 //                  points (x, y, z, original index) in Morton order; every 64
 //                  consecutive points form one group = one wavefront.
-//   forward      : 63-bit Morton codes of the means (the reference's morton3d_unit)
-//                  -> stable radix sort -> canonical preprocess written as 80-byte
-//                  records in BVH order -> Karras radix tree over leaves of
+//   forward      : top 48 bits of the means' 63-bit Morton codes (the reference's
+//                  morton3d_unit) -> stable radix sort -> canonical preprocess in
+//                  the caller's order written as 80-byte records at their BVH
+//                  positions -> Karras radix tree over leaves of
 //                  `leaf_size` consecutive records -> node boxes from a sparse
 //                  table of leaf-range boxes -> persistent wavefronts take groups
 //                  from per-XCD queues; a group is cut into runs of consecutive
@@ -22,8 +23,9 @@
 //                  box (LDS) accumulating the Gaussian's 11 partial sums -> one
 //                  record per (Gaussian, group) at a position fixed by the
 //                  forward's counts -> stable radix sort by Gaussian ->
-//                  per-Gaussian FP64 sum in group order -> covariance chain in
-//                  FP64 -> gradients in the caller's Gaussian order.
+//                  per-Gaussian FP64 sums by lane teams in BVH order (order
+//                  fixed by the segment alone) -> covariance chain in FP64 ->
+//                  gradients in the caller's Gaussian order.
 // Reference path: CR/forward.cu:10-274, CR/backward.cu:9-660,
 // CR/rasterizer_impl.cu:27-452, DGR/ext.cu:13-147 (CR/ = diff-gaussian-
 // rasterization/cuda_rasterizer/, DGR/ = diff-gaussian-rasterization/).
@@ -247,12 +249,18 @@ __global__ void sample_header_kernel(SampleSetHeader* h, int S, int NG)
 // ---------------------------------------------------------------------------
 // Gaussian BVH
 // ---------------------------------------------------------------------------
+// The tree orders Gaussians by the top 48 bits of their 63-bit Morton codes (16
+// bits per axis, a 65536^3 lattice): six 8-bit radix passes instead of eight;
+// equal keys keep the caller's order (stable sort) and the Karras build breaks
+// ties by index.
+constexpr int kGaussianCodeBits = 48;
+
 __global__ void gaussian_codes_kernel(const float* __restrict__ means, int P, uint64_t* __restrict__ codes,
                                       uint32_t* __restrict__ idx)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
-    codes[i] = morton3d_unit(means[3 * i], means[3 * i + 1], means[3 * i + 2], false);
+    codes[i] = morton3d_unit(means[3 * i], means[3 * i + 1], means[3 * i + 2], false) >> (63 - kGaussianCodeBits);
     idx[i] = (uint32_t)i;
 }
 
@@ -2119,7 +2127,8 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
         gaussian_codes_kernel<<<blocks_for(P, 256), 256, 0, st>>>(means, P, l.codes, l.idx);
         GVC_LAUNCH_CHECK("gaussian_codes");
         bool alt = false;
-        GVC_TRY(radix_sort_u64(l.codes, l.idx, l.codes_alt, l.idx_alt, (size_t)P, 64, l.sort_scratch, &alt, st));
+        GVC_TRY(radix_sort_u64(l.codes, l.idx, l.codes_alt, l.idx_alt, (size_t)P, kGaussianCodeBits, l.sort_scratch,
+                               &alt, st));
         const uint64_t* scodes = alt ? l.codes_alt : l.codes;
         const uint32_t* perm = alt ? l.idx_alt : l.idx;
         if (bt.V > 1) {
