@@ -120,6 +120,24 @@ __device__ __forceinline__ uint32_t quant5(float x, float clo, float cell)
     return (uint32_t)u;
 }
 
+// 64-bit total of the per-cell entry counts (the scan that turns them into
+// offsets is 32-bit, so the build refuses totals near 2^32; below that bound the
+// scans' index steps cannot wrap).
+__global__ __launch_bounds__(256) void count_total_kernel(const uint32_t* __restrict__ counts, size_t n,
+                                                          unsigned long long* __restrict__ total)
+{
+    __shared__ unsigned long long part[256];
+    unsigned long long v = 0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) v += counts[i];
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 128; d >= 1; d >>= 1) {
+        if ((int)threadIdx.x < d) part[threadIdx.x] += part[threadIdx.x + d];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && part[0]) atomicAdd(total, part[0]);
+}
+
 __global__ void tet_count_kernel(const float* __restrict__ boxes, int C, const TetGridHeader* __restrict__ h,
                                  uint32_t* __restrict__ counts)
 {
@@ -629,10 +647,18 @@ int tet_locator_build(int n_points, const float* pts, int C, const int* tets, un
     GVC_TRY(fill_u32(l.starts, 0u, n_grid + 1, st));
     tet_count_kernel<<<blocks_for(C, 256), 256, 0, st>>>(l.boxes, C, l.header, l.starts);
     GVC_LAUNCH_CHECK("tet_count");
-    GVC_TRY(exclusive_scan_u32(l.starts, l.starts, n_grid, true, l.scan_scratch, st));
-    uint32_t total = 0;
-    GVC_HIP(hipMemcpyAsync(&total, l.starts + n_grid, sizeof(total), hipMemcpyDeviceToHost, st));
+    // the bounding-box words are free again: 64-bit entry total
+    unsigned long long* total64 = reinterpret_cast<unsigned long long*>(l.bbox);
+    GVC_TRY(fill_u32(l.bbox, 0u, 2, st));
+    count_total_kernel<<<256, 256, 0, st>>>(l.starts, n_grid, total64);
+    GVC_LAUNCH_CHECK("count_total");
+    unsigned long long total_all = 0;
+    GVC_HIP(hipMemcpyAsync(&total_all, total64, sizeof(total_all), hipMemcpyDeviceToHost, st));
     GVC_HIP(hipStreamSynchronize(st));
+    if (total_all > 0xffffffffull - 64ull)
+        return set_error(GVC_ERR_INVALID_ARGUMENT, "mesh too large for the tet locator (more than 2^32 - 64 grid entries)");
+    GVC_TRY(exclusive_scan_u32(l.starts, l.starts, n_grid, true, l.scan_scratch, st));
+    const uint32_t total = (uint32_t)total_all;
     if (needed) *needed = total;
     if (total > cap) return set_error(GVC_ERR_CAPACITY, "tet locator entry capacity too small");
     GVC_TRY(fill_u32(l.cursor, 0u, n_grid, st));
