@@ -1682,21 +1682,67 @@ __global__ void segment_kernel(const uint32_t* __restrict__ keys, uint64_t n_max
 // Gradient sums of one Gaussian, in BVH order: 11 doubles and a pad (96 bytes).
 constexpr int kSumStride = 12;
 
+// Segments longer than this are summed by a whole wave each (reduce_long_kernel):
+// in sparse volumes Gaussians carry hundreds of records, and a few hundred waves
+// walking them with 4-lane teams would trail the launch.
+constexpr uint32_t kLongSegment = 128;
+
+// FP64 sums of records [b, e): lane j of a team of TEAM lanes adds records b+j,
+// b+j+TEAM, ... in order, then a fixed butterfly over the team leaves the same
+// sums in every lane.  The order depends only on the segment and TEAM.
+template <int TEAM>
+__device__ __forceinline__ void team_sums(uint32_t b, uint32_t e, int j, const uint32_t* __restrict__ sorted_vals,
+                                          const PartialRecord* __restrict__ parts, double acc[11])
+{
+    for (int c = 0; c < 11; ++c) acc[c] = 0.0;
+    // A = gid dv dw dmx | B = dmy dmz dc0 dc1 | C = dc2 dc3 dc4 dc5
+    auto add = [&](const float4& A, const float4& B, const float4& C) {
+        acc[0] += A.y; acc[1] += A.z; acc[2] += A.w; acc[3] += B.x; acc[4] += B.y;
+        acc[5] += B.z; acc[6] += B.w; acc[7] += C.x; acc[8] += C.y; acc[9] += C.z; acc[10] += C.w;
+    };
+    uint32_t k = b + (uint32_t)j;
+    for (; k + TEAM < e; k += 2 * TEAM) {
+        const float4* q0 = reinterpret_cast<const float4*>(&parts[sorted_vals[k]]);
+        const float4* q1 = reinterpret_cast<const float4*>(&parts[sorted_vals[k + TEAM]]);
+        const float4 A0 = q0[0], B0 = q0[1], C0 = q0[2];
+        const float4 A1 = q1[0], B1 = q1[1], C1 = q1[2];
+        add(A0, B0, C0);
+        add(A1, B1, C1);
+    }
+    if (k < e) {
+        const float4* q0 = reinterpret_cast<const float4*>(&parts[sorted_vals[k]]);
+        add(q0[0], q0[1], q0[2]);
+    }
+#pragma unroll
+    for (int c = 0; c < 11; ++c) {
+#pragma unroll
+        for (int o = 1; o < TEAM; o <<= 1) acc[c] += __shfl_xor(acc[c], o, 64);
+    }
+}
+
+__device__ __forceinline__ void store_sums(double* __restrict__ sums, int gi, const double m[11])
+{
+    double2* o = reinterpret_cast<double2*>(sums + (size_t)kSumStride * gi);
+    o[0] = make_double2(m[0], m[1]);
+    o[1] = make_double2(m[2], m[3]);
+    o[2] = make_double2(m[4], m[5]);
+    o[3] = make_double2(m[6], m[7]);
+    o[4] = make_double2(m[8], m[9]);
+    o[5] = make_double2(m[10], 0.0);
+}
+
 // Per Gaussian, in BVH order (neighbouring Gaussians have segments of similar
-// length): a team of TEAM lanes sums the Gaussian's partial records in FP64,
-// lane j of the team taking records begin+j, begin+j+TEAM, ... in order, then a
-// fixed butterfly over the team (every lane ends with the same sums).  The order
-// depends only on the segment, so a Gaussian's gradients do not depend on where
-// its segment lies in the sorted records.  A wave covers 64 consecutive
-// Gaussians in TEAM passes (lane L first loads Gaussian L's segment bounds; a
-// pass whose teams have no records is skipped), hands lane L the sums of
-// Gaussian L and writes them to sums[] in BVH order.
+// length): segments of at most kLongSegment records are summed by a team of TEAM
+// lanes (team_sums); a wave covers 64 consecutive Gaussians in TEAM passes (lane
+// L first loads Gaussian L's segment bounds; a pass whose teams have no records
+// is skipped), hands lane L the sums of Gaussian L and writes them to sums[] in
+// BVH order.  Longer segments are listed for reduce_long_kernel.
 template <int TEAM>
 __global__ __launch_bounds__(256) void reduce_sums_kernel(int P, const uint32_t* __restrict__ begin,
                                                           const uint32_t* __restrict__ end,
                                                           const uint32_t* __restrict__ sorted_vals,
                                                           const PartialRecord* __restrict__ parts,
-                                                          double* __restrict__ sums)
+                                                          double* __restrict__ sums, uint32_t* __restrict__ long_list)
 {
     static_assert(TEAM >= 1 && TEAM <= 64 && (TEAM & (TEAM - 1)) == 0, "team size");
     constexpr int kPer = 64 / TEAM;  // Gaussians per pass
@@ -1706,8 +1752,19 @@ __global__ __launch_bounds__(256) void reduce_sums_kernel(int P, const uint32_t*
     const int team = lane / TEAM, j = lane % TEAM;
     const int gi = g0 + lane;
     const bool valid = gi < P;
-    const uint32_t my_b = valid ? begin[gi] : 0u;
-    const uint32_t my_e = valid ? end[gi] : 0u;
+    uint32_t my_b = valid ? begin[gi] : 0u;
+    uint32_t my_e = valid ? end[gi] : 0u;
+    const bool is_long = my_e - my_b > kLongSegment;
+    const uint64_t longs = __ballot(is_long);
+    if (longs != 0ull) {
+        uint32_t slot = 0;
+        if (lane == 0) slot = atomicAdd(long_list, (uint32_t)__popcll(longs));
+        slot = (uint32_t)__shfl((int)slot, 0, 64);
+        if (is_long) {
+            long_list[1 + slot + (uint32_t)__popcll(longs & lanemask_lt())] = (uint32_t)gi;
+            my_e = my_b;  // summed by reduce_long_kernel
+        }
+    }
     double mine[11];
     for (int c = 0; c < 11; ++c) mine[c] = 0.0;
     for (int p = 0; p < TEAM; ++p) {
@@ -1716,41 +1773,35 @@ __global__ __launch_bounds__(256) void reduce_sums_kernel(int P, const uint32_t*
         const uint32_t e = (uint32_t)__shfl((int)my_e, src, 64);
         if (__ballot(e > b) == 0ull) continue;
         double acc[11];
-        for (int c = 0; c < 11; ++c) acc[c] = 0.0;
-        // A = gid dv dw dmx | B = dmy dmz dc0 dc1 | C = dc2 dc3 dc4 dc5
-        auto add = [&](const float4& A, const float4& B, const float4& C) {
-            acc[0] += A.y; acc[1] += A.z; acc[2] += A.w; acc[3] += B.x; acc[4] += B.y;
-            acc[5] += B.z; acc[6] += B.w; acc[7] += C.x; acc[8] += C.y; acc[9] += C.z; acc[10] += C.w;
-        };
-        uint32_t k = b + (uint32_t)j;
-        for (; k + TEAM < e; k += 2 * TEAM) {
-            const float4* q0 = reinterpret_cast<const float4*>(&parts[sorted_vals[k]]);
-            const float4* q1 = reinterpret_cast<const float4*>(&parts[sorted_vals[k + TEAM]]);
-            const float4 A0 = q0[0], B0 = q0[1], C0 = q0[2];
-            const float4 A1 = q1[0], B1 = q1[1], C1 = q1[2];
-            add(A0, B0, C0);
-            add(A1, B1, C1);
-        }
-        if (k < e) {
-            const float4* q0 = reinterpret_cast<const float4*>(&parts[sorted_vals[k]]);
-            add(q0[0], q0[1], q0[2]);
-        }
+        team_sums<TEAM>(b, e, j, sorted_vals, parts, acc);
 #pragma unroll
         for (int c = 0; c < 11; ++c) {
-#pragma unroll
-            for (int o = 1; o < TEAM; o <<= 1) acc[c] += __shfl_xor(acc[c], o, 64);
             const double x = __shfl(acc[c], (lane % kPer) * TEAM, 64);
             if (lane / kPer == p) mine[c] = x;
         }
     }
-    if (!valid) return;
-    double2* o = reinterpret_cast<double2*>(sums + (size_t)kSumStride * gi);
-    o[0] = make_double2(mine[0], mine[1]);
-    o[1] = make_double2(mine[2], mine[3]);
-    o[2] = make_double2(mine[4], mine[5]);
-    o[3] = make_double2(mine[6], mine[7]);
-    o[4] = make_double2(mine[8], mine[9]);
-    o[5] = make_double2(mine[10], 0.0);
+    if (!valid || is_long) return;
+    store_sums(sums, gi, mine);
+}
+
+// One wave per listed Gaussian (long segments): team_sums with 64 lanes.  A
+// fixed grid takes the list with a grid-stride loop over its device-side count.
+__global__ __launch_bounds__(256) void reduce_long_kernel(const uint32_t* __restrict__ begin,
+                                                          const uint32_t* __restrict__ end,
+                                                          const uint32_t* __restrict__ sorted_vals,
+                                                          const PartialRecord* __restrict__ parts,
+                                                          const uint32_t* __restrict__ long_list,
+                                                          double* __restrict__ sums)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t n = long_list[0];
+    const uint32_t waves = gridDim.x * (blockDim.x / 64);
+    for (uint32_t w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); w < n; w += waves) {
+        const int gi = (int)long_list[1 + w];
+        double acc[11];
+        team_sums<64>(begin[gi], end[gi], lane, sorted_vals, parts, acc);
+        if (lane == 0) store_sums(sums, gi, acc);
+    }
 }
 
 // Per Gaussian, in the caller's order (coalesced parameter reads and gradient
@@ -1994,6 +2045,7 @@ struct BackwardLayout {
     PartialRecord* parts;
     uint32_t *keys, *vals, *keys_alt, *vals_alt, *scratch, *begin, *end;
     double* sums;  // kSumStride per Gaussian, BVH order
+    uint32_t* long_list;  // [0] = count, [1..] = Gaussians with long segments
     BackwardLayout(Arena& a, int P, uint64_t cap)
     {
         const size_t c = (size_t)(cap > 0 ? cap : 1);
@@ -2006,6 +2058,7 @@ struct BackwardLayout {
         begin = a.take<uint32_t>((size_t)(P > 0 ? P : 1));
         end = a.take<uint32_t>((size_t)(P > 0 ? P : 1));
         sums = a.take<double>((size_t)kSumStride * (size_t)(P > 0 ? P : 1));
+        long_list = a.take<uint32_t>((size_t)P + 1);
     }
 };
 
@@ -2326,9 +2379,12 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
                                                                   bl.begin, bl.end);
         GVC_LAUNCH_CHECK("segment");
     }
+    GVC_TRY(fill_u32(bl.long_list, 0u, 1, st));
     reduce_sums_kernel<kReduceTeam><<<blocks_for(P, 256), 256, 0, st>>>(P, bl.begin, bl.end, svals, bl.parts,
-                                                                        bl.sums);
+                                                                        bl.sums, bl.long_list);
     GVC_LAUNCH_CHECK("reduce_sums");
+    reduce_long_kernel<<<1024, 256, 0, st>>>(bl.begin, bl.end, svals, bl.parts, bl.long_list, bl.sums);
+    GVC_LAUNCH_CHECK("reduce_long");
     chain_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, bl.sums, scales, rots, k, dmeans, dscales, drots,
                                                      dvalues, dweights);
     GVC_LAUNCH_CHECK("chain");
