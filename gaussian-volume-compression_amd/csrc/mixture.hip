@@ -438,11 +438,24 @@ __global__ void leaf_box_kernel(const GRecord* __restrict__ rec, BatchView bv, i
 }
 
 // T_{k+1}[i] = T_k[i] u T_k[i + 2^k] for i + 2^{k+1} <= NL.
-__global__ void sparse_level_kernel(const Box6* __restrict__ tk, Box6* __restrict__ tk1, int NL, int half)
+// Up to three levels of the sparse table per launch: from level k (ranges of
+// h = 2^k leaves), level k+1+m at index i is the union of level k at i, i + h, ...,
+// i + (2^(m+1) - 1) h; each level is written where its range fits in [0, NL).
+__global__ void sparse_level_kernel(const Box6* __restrict__ tk, Box6* __restrict__ out, int NL, int half, int nlev)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i + 2 * half > NL) return;
-    tk1[i] = box_union(tk[i], tk[i + half]);
+    Box6 u = box_union(tk[i], tk[i + half]);
+    out[i] = u;
+    for (int m = 1; m < nlev; ++m) {
+        const int span = half << m;  // level k+m box covers [i, i + span)
+        if (i + 2 * span > NL) return;
+        // level k+m at i + span: the union of level k at i + span, ..., i + 2 span - h
+        Box6 v = tk[i + span];
+        for (int t = span + half; t < 2 * span; t += half) v = box_union(v, tk[i + t]);
+        u = box_union(u, v);
+        out[(size_t)m * NL + i] = u;
+    }
 }
 
 __device__ __forceinline__ Box6 range_box(const Box6* __restrict__ table, int NL, int a, int b)
@@ -2220,11 +2233,12 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
         GVC_LAUNCH_CHECK("leaf_box");
         if (bt.NI > 0) {
             const int levels = sparse_levels(NL);
-            for (int kk = 0; kk + 1 < levels; ++kk) {
+            for (int kk = 0; kk + 1 < levels; kk += 3) {
                 const int half = 1 << kk;
                 const int n_out = NL - 2 * half + 1;
+                const int nlev = (levels - 1 - kk) < 3 ? (levels - 1 - kk) : 3;
                 sparse_level_kernel<<<blocks_for((size_t)n_out, 256), 256, 0, st>>>(
-                    l.sparse + (size_t)kk * NL, l.sparse + (size_t)(kk + 1) * NL, NL, half);
+                    l.sparse + (size_t)kk * NL, l.sparse + (size_t)(kk + 1) * NL, NL, half, nlev);
                 GVC_LAUNCH_CHECK("sparse_level");
             }
             node_box_kernel<<<blocks_for(bt.NI, 256), 256, 0, st>>>(l.sparse, bv, NL, bt.NI, l.nodes, l.root_box);
