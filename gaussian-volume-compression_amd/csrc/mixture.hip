@@ -1248,40 +1248,41 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
             // each point lane walks its candidates in ascending order, two per pass
             // (both loads issued before either pair is evaluated)
             const float4* __restrict__ cb = &s.chunk[0][0];
-            auto pair = [&](const float4& A, const float4& B, const float4& C) {
+            // Branch-free: a rejected pair (or the missing second one) adds zero,
+            // which leaves the compensated sum W - cW unchanged.
+            auto pair = [&](const float4& A, const float4& B, const float4& C, bool live) {
                 const float dx = px - A.x, dy = py - A.y, dz = pz - A.z;
                 float r0, r1, r2;
                 const float power = pair_power(B.x, B.y, B.z, B.w, C.x, C.y, dx, dy, dz, r0, r1, r2);
-                if (in_window(power)) {
-                    const float e = __builtin_amdgcn_exp2f(power * kLog2e);
-                    const float wgt = A.w * e;
-                    const float wv = wgt * C.z;
-                    if (KAHAN) {
-                        float y = wgt - cW;
-                        float t = W + y;
-                        cW = (t - W) - y;
-                        W = t;
-                        y = wv - cU;
-                        t = U + y;
-                        cU = (t - U) - y;
-                        U = t;
-                    } else {
-                        W += wgt;
-                        U += wv;
-                    }
-                    ++npairs;
+                const bool in = live && in_window(power);
+                const float e = __builtin_amdgcn_exp2f(power * kLog2e);
+                const float wgt = in ? A.w * e : 0.0f;
+                const float wv = in ? wgt * C.z : 0.0f;
+                if (KAHAN) {
+                    float y = wgt - cW;
+                    float t = W + y;
+                    cW = (t - W) - y;
+                    W = t;
+                    y = wv - cU;
+                    t = U + y;
+                    cU = (t - U) - y;
+                    U = t;
+                } else {
+                    W += wgt;
+                    U += wv;
                 }
+                npairs += in ? 1u : 0u;
             };
             while (m != 0ull) {
                 const uint32_t j0 = (uint32_t)__builtin_ctzll(m);
                 m &= m - 1;
                 const bool two = m != 0ull;
                 const uint32_t j1 = two ? (uint32_t)__builtin_ctzll(m) : j0;
-                m &= two ? m - 1 : m;
+                m &= m - 1;  // stays 0 when there was no second bit
                 const float4 A0 = cb[3 * j0], B0 = cb[3 * j0 + 1], C0 = cb[3 * j0 + 2];
                 const float4 A1 = cb[3 * j1], B1 = cb[3 * j1 + 1], C1 = cb[3 * j1 + 2];
-                pair(A0, B0, C0);
-                if (two) pair(A1, B1, C1);
+                pair(A0, B0, C0, true);
+                pair(A1, B1, C1, two);
             }
             lds_fence();
         };
