@@ -740,6 +740,8 @@ __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__
     int nl = 0, eb = 0;             // leaves s.leaves[0..nl) under expansion; next Gaussian offset
     int sp = 0;                     // wide stack depth
     int dnode = -1, ssp = 0;        // depth-first walk of one subtree while dnode >= 0
+    // t / L as umulhi(t, ceil(2^32 / L)): exact for t < 2^32 / L (here t < 128 L)
+    const uint32_t l_recip = L > 1 ? (uint32_t)((0x100000000ull + (uint64_t)L - 1u) / (uint64_t)L) : 0u;
     if (NL <= 0) return true;
     if (NL == 1) {
         if (box_overlaps(root_box[0], root_box[1], root_box[2], root_box[3], root_box[4], root_box[5], gb)) {
@@ -767,7 +769,7 @@ __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__
             bool hit = false;
             int g = 0;
             if (t < nl * L) {
-                const int q = t / L;
+                const int q = L > 1 ? (int)__umulhi((uint32_t)t, l_recip) : t;
                 g = tv.g_lo + (s.leaves[q] - tv.leaf_base) * L + (t - q * L);
                 if (g < tv.g_hi) {
                     const float4 b0 = *reinterpret_cast<const float4*>(&rec[g].lox);
