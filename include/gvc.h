@@ -147,7 +147,8 @@ int gvc_forward_accumulate(const void *state, int P, int S, const gvc_params *pa
  * CR/rasterizer_impl.cu:243-452 (kernels K9-K11, CR/backward.cu:9-660); the
  * covariance chain is derived from the forward formula instead of following
  * CR/backward.cu:68-124.  Reduction is deterministic: per-(Gaussian, group)
- * partials are sorted by Gaussian and summed in group order, no float atomics.
+ * partials are sorted by Gaussian (stable) and each Gaussian's are summed in FP64
+ * in an order fixed by its own sorted partials; no float atomics.
  * record_capacity >= n_records of the forward (gvc_forward_stats) or the call
  * flags GVC_ERRFLAG_RECORD_OVERFLOW and the gradients are incomplete. */
 size_t gvc_backward_bytes(int P, uint64_t record_capacity);
