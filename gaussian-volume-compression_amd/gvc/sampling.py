@@ -182,6 +182,82 @@ def sample_mesh_points(points, tets, num_batches, batch_size, generator=None, re
     return out
 
 
+class BoundarySurface:
+    """The boundary triangles of a tetrahedral mesh with outward unit normals,
+    for R/train.py:62-92 (sample_exterior_points), which takes them from
+    pyvista's extract_surface().triangulate().compute_normals().  Here: a face
+    of a tetrahedron is on the boundary iff no other tetrahedron has it (the 4C
+    faces sorted by their sorted vertex triples, equal neighbours removed); its
+    normal, computed in float64 from the vertices, points away from the owning
+    tetrahedron's opposite vertex.  Faces are ordered by their sorted vertex
+    triples (pyvista's order differs, so samples agree in distribution, not draw
+    for draw).  Everything stays on the points' device."""
+
+    def __init__(self, points, tets):
+        pts = torch.as_tensor(points, dtype=torch.float32)
+        dev = pts.device
+        cells = torch.as_tensor(tets, dtype=torch.long).to(dev)
+        if cells.dim() != 2 or cells.shape[1] != 4:
+            raise ValueError("tets must have shape (C, 4)")
+        N = int(pts.shape[0])
+        # face k of a tetrahedron is opposite its vertex k
+        opp = torch.tensor([[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]], device=dev)
+        faces = cells[:, opp].reshape(-1, 3)                       # (4C, 3), row 4t+k
+        apex = cells.reshape(-1)                                   # opposite vertex of row 4t+k
+        key = faces.sort(dim=1).values
+        # lexicographic order of (k0, k1, k2): stable sorts on (k1, k2), then k0
+        order = torch.sort(key[:, 1] * N + key[:, 2], stable=True).indices
+        order = order[torch.sort(key[order, 0], stable=True).indices]
+        ks = key[order]
+        same_next = torch.zeros(ks.shape[0], dtype=torch.bool, device=dev)
+        if ks.shape[0] > 1:
+            eq = (ks[1:] == ks[:-1]).all(dim=1)
+            same_next[:-1] |= eq
+            same_next[1:] |= eq
+        rows = order[~same_next]
+        tri = faces[rows]
+        a, b, c = (pts[tri[:, i]].double() for i in range(3))
+        d = pts[apex[rows]].double()
+        n = torch.linalg.cross(b - a, c - a)
+        flip = ((n * (d - a)).sum(dim=1) > 0).unsqueeze(1)
+        n = torch.where(flip, -n, n)
+        n = n / n.norm(dim=1, keepdim=True).clamp_min(1e-300)
+        self.points = pts
+        self.faces = tri
+        self.normals = n.float()
+
+    @property
+    def count(self):
+        return int(self.faces.shape[0])
+
+    def sample(self, num_batches, batch_size, offset=0.01, generator=None, return_faces=False):
+        """Points just outside the surface, as R/train.py:62-92 draws them: a
+        uniformly chosen boundary triangle, barycentrics (1 - sqrt(u0),
+        sqrt(u0)(1 - u1), sqrt(u0) u1), then a step of U[0, offset) along the
+        outward normal.  [num_batches, batch_size, 3] float32 (and, with
+        return_faces, the chosen triangle per point)."""
+        dev = self.points.device
+        total = int(num_batches) * int(batch_size)
+        idx = torch.randint(0, self.count, (total,), device=dev, generator=generator)
+        v = self.points[self.faces[idx]]
+        u = torch.rand(total, 2, device=dev, generator=generator)
+        su = u[:, 0].sqrt()
+        bary = torch.stack([1 - su, su * (1 - u[:, 1]), su * u[:, 1]], dim=1)
+        surf = (bary.unsqueeze(2) * v).sum(dim=1)
+        dist = torch.rand(total, 1, device=dev, generator=generator) * offset
+        out = (surf + self.normals[idx] * dist).reshape(num_batches, batch_size, 3)
+        if return_faces:
+            return out, idx.reshape(num_batches, batch_size)
+        return out
+
+
+def sample_exterior_points(points, tets, num_batches, batch_size, offset=0.01, generator=None):
+    """R/train.py:62-92 for a tetrahedral mesh (points [N,3], tets [C,4]); see
+    BoundarySurface.  Build the surface once and call its sample() to draw
+    repeatedly."""
+    return BoundarySurface(points, tets).sample(num_batches, batch_size, offset, generator)
+
+
 def grid_sample(dims, origin, spacing, values, samples):
     """Trilinear probe of a uniform point grid on the GPU (torch tensors in/out)."""
     dev = samples.device

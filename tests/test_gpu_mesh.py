@@ -180,3 +180,31 @@ def test_sample_mesh_points_by_construction():
     assert np.array_equal(gid[clear], made[clear].astype(np.int32))
     ref = (bary * vals[tets[made]]).sum(1)
     np.testing.assert_allclose(gv[clear], ref[clear], rtol=1e-9, atol=1e-12)
+
+
+def test_exterior_points_locate_outside():
+    """R/train.py:62-92 sampling on the device: points lifted off the mesh's
+    boundary along the outward normals are outside the mesh for the exact
+    locator (id -1, value -1); points left on a boundary face are inside it or
+    on it."""
+    from gvc.sampling import BoundarySurface
+    rng = np.random.default_rng(13)
+    n = 6
+    pts = workloads.jittered_lattice(rng, n, 0.3)
+    tets = workloads.kuhn_tets(n)
+    vals = rng.normal(size=pts.shape[0])
+    P = torch.tensor(pts, device=DEV)
+    surf = BoundarySurface(P, torch.tensor(tets, device=DEV))
+    assert surf.faces.device.type == "cuda" and surf.count == 6 * n * n * 2
+    gen = torch.Generator(device=DEV)
+    gen.manual_seed(9)
+    s, f = surf.sample(2, 5000, offset=0.01, generator=gen, return_faces=True)
+    s = s.reshape(-1, 3)
+    f = f.reshape(-1)
+    tri = surf.points[surf.faces[f]]
+    h = ((s - tri[:, 0]) * surf.normals[f]).sum(1).cpu().numpy()
+    gv, gid = TetMeshSampler(P, tets, vals).sample(s)
+    gv, gid = gv.cpu().numpy(), gid.cpu().numpy()
+    lifted = h > 1e-5
+    assert lifted.mean() > 0.99
+    assert np.all(gid[lifted] == -1) and np.all(gv[lifted] == -1.0)
