@@ -96,6 +96,22 @@ int cell_interp(int S, const float* samples, const int* cell_ids, const uint8_t*
 size_t knn3_bytes(int n);
 int knn3(int n, const float* pts, float* out, void* ws, size_t bytes, hipStream_t st);
 
+// optim.hip
+int adam_step(int n, const gvc_adam_tensor* tensors, gvc_train_clock* clock, double beta1, double beta2,
+              double eps, hipStream_t st);
+int lr_advance(gvc_train_clock* clock, const gvc_lr_schedule* s, int lr_index, int64_t set_iteration,
+               hipStream_t st);
+size_t error_order_bytes(int S);
+int error_order(int S, const float* cells, const float* gt, uint32_t* order, uint32_t* valid_count, void* ws,
+                size_t bytes, hipStream_t st);
+size_t densify_bytes(int total);
+int densify_in_empty(int count, int k, const gvc_store* s, const float* samples, const float* gt,
+                     const uint32_t* idx, float dist_scale, float init_weight, void* ws, size_t bytes,
+                     hipStream_t st);
+size_t prune_bytes(int count);
+int prune(int count, const gvc_store* s, const uint8_t* mask, float min_weight, int* new_count, void* ws,
+          size_t bytes, hipStream_t st);
+
 }  // namespace gvc
 
 using namespace gvc;
@@ -433,6 +449,71 @@ int gvc_loss_backward(int S, const float* cells, const float* weights, const flo
     GVC_REQUIRE(cells && weights && gt && state && dL_dcells && dL_dweights, "NULL pointer");
     return loss_backward(S, cells, weights, gt, fn_reg, fp_reg, encode_surface, dL_dloss, state, state_bytes,
                          dL_dcells, dL_dweights, (hipStream_t)stream);
+}
+
+int gvc_lr_advance(gvc_train_clock* clock, const gvc_lr_schedule* schedule, int lr_index, int64_t set_iteration,
+                   void* stream)
+{
+    GVC_REQUIRE(clock && schedule, "NULL pointer");
+    GVC_REQUIRE(lr_index >= 0 && lr_index < GVC_ADAM_MAX_TENSORS, "lr_index out of range");
+    GVC_REQUIRE(schedule->max_steps > 0, "max_steps <= 0");
+    return lr_advance(clock, schedule, lr_index, set_iteration, (hipStream_t)stream);
+}
+
+int gvc_adam_step(int n_tensors, const gvc_adam_tensor* tensors, gvc_train_clock* clock, double beta1,
+                  double beta2, double eps, void* stream)
+{
+    GVC_REQUIRE(n_tensors >= 0 && n_tensors <= GVC_ADAM_MAX_TENSORS, "n_tensors out of range");
+    if (n_tensors == 0) return GVC_OK;
+    GVC_REQUIRE(tensors && clock, "NULL pointer");
+    GVC_REQUIRE(beta1 >= 0.0 && beta1 < 1.0 && beta2 >= 0.0 && beta2 < 1.0, "betas outside [0, 1)");
+    return adam_step(n_tensors, tensors, clock, beta1, beta2, eps, (hipStream_t)stream);
+}
+
+size_t gvc_error_order_bytes(int S) { return S >= 0 ? error_order_bytes(S) : 0; }
+
+int gvc_error_order(int S, const float* cells, const float* gt, uint32_t* order, uint32_t* valid_count,
+                    void* workspace, size_t workspace_bytes, void* stream)
+{
+    GVC_REQUIRE(S >= 0, "S < 0");
+    GVC_REQUIRE(valid_count && workspace && (S == 0 || (cells && gt && order)), "NULL pointer");
+    return error_order(S, cells, gt, order, valid_count, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+static bool store_ok(const gvc_store* s)
+{
+    if (!s || !s->xyz || !s->weight || !s->scaling || !s->rotation || !s->value) return false;
+    // moments all present or all absent
+    int have = 0;
+    for (int k = 0; k < 5; ++k) have += (s->exp_avg[k] != nullptr) + (s->exp_avg_sq[k] != nullptr);
+    return have == 0 || have == 10;
+}
+
+size_t gvc_densify_bytes(int total) { return total >= 0 ? densify_bytes(total) : 0; }
+
+int gvc_densify_in_empty(int count, int k, const gvc_store* store, const float* samples, const float* gt,
+                         const uint32_t* idx, float dist_scale, float init_weight, void* workspace,
+                         size_t workspace_bytes, void* stream)
+{
+    GVC_REQUIRE(count >= 0 && k >= 0, "count < 0 or k < 0");
+    GVC_REQUIRE(store_ok(store), "incomplete gvc_store");
+    GVC_REQUIRE((long long)count + k <= store->capacity, "count + k exceeds the store capacity");
+    if (k == 0) return GVC_OK;
+    GVC_REQUIRE(samples && gt && workspace, "NULL pointer");
+    return densify_in_empty(count, k, store, samples, gt, idx, dist_scale, init_weight, workspace, workspace_bytes,
+                            (hipStream_t)stream);
+}
+
+size_t gvc_prune_bytes(int count) { return count >= 0 ? prune_bytes(count) : 0; }
+
+int gvc_prune(int count, const gvc_store* store, const uint8_t* prune_mask, float min_weight, int* new_count,
+              void* workspace, size_t workspace_bytes, void* stream)
+{
+    GVC_REQUIRE(count >= 0, "count < 0");
+    GVC_REQUIRE(store_ok(store), "incomplete gvc_store");
+    GVC_REQUIRE(count <= store->capacity, "count exceeds the store capacity");
+    GVC_REQUIRE(new_count && workspace, "NULL pointer");
+    return prune(count, store, prune_mask, min_weight, new_count, workspace, workspace_bytes, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -39,6 +39,31 @@ class gvc_params(ctypes.Structure):
                 ("run_points", ctypes.c_uint)]
 
 
+GVC_ADAM_MAX_TENSORS = 8
+
+
+class gvc_train_clock(ctypes.Structure):
+    _fields_ = [("iteration", ctypes.c_int64), ("step", ctypes.c_int64),
+                ("lr", ctypes.c_double * GVC_ADAM_MAX_TENSORS), ("step_size", ctypes.c_float * GVC_ADAM_MAX_TENSORS),
+                ("bc2_sqrt", ctypes.c_float), ("reserved", ctypes.c_float * 7)]
+
+
+class gvc_lr_schedule(ctypes.Structure):
+    _fields_ = [("lr_init", ctypes.c_double), ("lr_final", ctypes.c_double), ("delay_mult", ctypes.c_double),
+                ("delay_steps", ctypes.c_int64), ("max_steps", ctypes.c_int64)]
+
+
+class gvc_adam_tensor(ctypes.Structure):
+    _fields_ = [("param", ctypes.c_void_p), ("grad", ctypes.c_void_p), ("exp_avg", ctypes.c_void_p),
+                ("exp_avg_sq", ctypes.c_void_p), ("numel", ctypes.c_uint64), ("lr_index", ctypes.c_int)]
+
+
+class gvc_store(ctypes.Structure):
+    _fields_ = [("capacity", ctypes.c_int), ("xyz", ctypes.c_void_p), ("weight", ctypes.c_void_p),
+                ("scaling", ctypes.c_void_p), ("rotation", ctypes.c_void_p), ("value", ctypes.c_void_p),
+                ("exp_avg", ctypes.c_void_p * 5), ("exp_avg_sq", ctypes.c_void_p * 5)]
+
+
 _lib = None
 
 _P = ctypes.c_void_p
@@ -94,6 +119,15 @@ _SIGS = {
     "gvc_loss_state_bytes": (_SZ, [_I]),
     "gvc_loss_forward": (_I, [_I, _P, _P, _P, _F, _F, _I, _P, _P, _SZ, _P]),
     "gvc_loss_backward": (_I, [_I, _P, _P, _P, _F, _F, _I, _P, _P, _SZ, _P, _P, _P]),
+    "gvc_lr_advance": (_I, [_P, ctypes.POINTER(gvc_lr_schedule), _I, ctypes.c_int64, _P]),
+    "gvc_adam_step": (_I, [_I, ctypes.POINTER(gvc_adam_tensor), _P, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_double, _P]),
+    "gvc_error_order_bytes": (_SZ, [_I]),
+    "gvc_error_order": (_I, [_I, _P, _P, _P, _P, _P, _SZ, _P]),
+    "gvc_densify_bytes": (_SZ, [_I]),
+    "gvc_densify_in_empty": (_I, [_I, _I, ctypes.POINTER(gvc_store), _P, _P, _P, _F, _F, _P, _SZ, _P]),
+    "gvc_prune_bytes": (_SZ, [_I]),
+    "gvc_prune": (_I, [_I, ctypes.POINTER(gvc_store), _P, _F, _P, _P, _SZ, _P]),
 }
 
 EXPORTED = tuple(_SIGS.keys())

@@ -293,6 +293,103 @@ int gvc_loss_backward(int S, const float *cells, const float *weights, const flo
                       float fp_reg, int encode_surface, const float *dL_dloss, const void *state,
                       size_t state_bytes, float *dL_dcells, float *dL_dweights, void *stream);
 
+/* ---- optimiser and model surgery ---------------------------------------------------
+ * The training loop's optimiser and densify/prune (R/scene/gaussian_model.py:207-256,
+ * :396-612, :727-799; R/train.py:367, :551-597), over parameter buffers of a fixed
+ * capacity: nothing is reallocated, rows [0, count) are the live Gaussians. */
+#define GVC_ADAM_MAX_TENSORS 8
+
+/* Device clock of a training loop (caller-owned device memory, zero-initialised,
+ * lr[] set by the caller).  `iteration` is advanced by gvc_lr_advance, `step` (the
+ * Adam step count) by gvc_adam_step; both on the device, so a captured step keeps
+ * counting on replay. */
+typedef struct gvc_train_clock {
+    int64_t iteration;
+    int64_t step;
+    double lr[GVC_ADAM_MAX_TENSORS];           /* learning rate per lr_index */
+    float step_size[GVC_ADAM_MAX_TENSORS];     /* written by gvc_adam_step */
+    float bc2_sqrt;                            /* written by gvc_adam_step */
+    float reserved[7];
+} gvc_train_clock;
+
+/* get_expon_lr_func (R/utils/general_utils.py:35-69). */
+typedef struct gvc_lr_schedule {
+    double lr_init, lr_final, delay_mult;
+    int64_t delay_steps, max_steps;
+} gvc_lr_schedule;
+
+/* update_learning_rate (R/scene/gaussian_model.py:249-255): clock.iteration =
+ * set_iteration (>= 0) or clock.iteration + 1 (set_iteration < 0), then
+ * clock.lr[lr_index] = schedule(iteration).  Stream-ordered, no host sync. */
+int gvc_lr_advance(gvc_train_clock *clock, const gvc_lr_schedule *schedule, int lr_index, int64_t set_iteration,
+                   void *stream);
+
+/* One tensor of an Adam step: numel floats of parameter, gradient and both moments.
+ * grad == NULL skips the tensor (torch skips a parameter without a gradient). */
+typedef struct gvc_adam_tensor {
+    float *param;
+    const float *grad;
+    float *exp_avg;
+    float *exp_avg_sq;
+    uint64_t numel;
+    int lr_index;
+} gvc_adam_tensor;
+
+/* optimizer.step() of torch.optim.Adam(betas, eps) as the reference configures it
+ * (R/scene/gaussian_model.py:240, stepped at R/train.py:593-595), for up to
+ * GVC_ADAM_MAX_TENSORS tensors in one launch: clock.step += 1, then per element
+ * m = lerp(m, g, 1 - beta1), v = v*beta2 + (1 - beta2)*g*g,
+ * p += -lr/(1 - beta1^t) * m / (sqrt(v)/sqrt(1 - beta2^t) + eps).  No step is
+ * counted when every grad is NULL.  beta1, beta2, eps are doubles, as torch holds
+ * them: 1 - beta and the bias corrections are formed in double, then rounded to
+ * float.  Stream-ordered, no host sync. */
+int gvc_adam_step(int n_tensors, const gvc_adam_tensor *tensors, gvc_train_clock *clock, double beta1,
+                  double beta2, double eps, void *stream);
+
+/* The densification pick of R/train.py:562-578 (torch.topk of |cells - gt| over
+ * gt != -1): order[0..S) = the indices of the valid samples by descending error,
+ * ties in ascending index, then the invalid ones; *valid_count (device uint32) =
+ * the number of valid samples.  Stream-ordered. */
+size_t gvc_error_order_bytes(int S);
+int gvc_error_order(int S, const float *cells, const float *gt, uint32_t *order, uint32_t *valid_count,
+                    void *workspace, size_t workspace_bytes, void *stream);
+
+/* The trainable model: the five raw parameter arrays of capacity rows (means
+ * [C,3], weight logits [C], log-scales [C,3], quaternions [C,4], value logits [C]:
+ * the reference optimiser's groups xyz, weight, scaling, rotation, value, in that
+ * order) and their Adam moments in the same order and shapes (NULL: no moments yet,
+ * as before the first step). */
+typedef struct gvc_store {
+    int capacity;
+    float *xyz, *weight, *scaling, *rotation, *value;
+    float *exp_avg[5];
+    float *exp_avg_sq[5];
+} gvc_store;
+
+/* densify_in_empty (R/scene/gaussian_model.py:727-763) and the initial cloud of
+ * create_from_pcd (:139-205): rows [count, count + k) take positions
+ * samples[idx[j]] (idx NULL: samples[j]), log-scales log(sqrt(max(d * dist_scale,
+ * 1e-7))) on all three axes with d = the mean squared distance to the 3 nearest
+ * of the count + k positions (distCUDA2), rotation (1, 0, 0, 0), weight logit
+ * log(w0 / (1 - w0)) for w0 = init_weight, value logit log(g / (1 - g)) for
+ * g = gt[idx[j]], and zero moments.  The reference uses dist_scale 0.1 and w0 0.01
+ * when densifying, 1 and 0.1 for the initial cloud.  count + k <= capacity; every
+ * idx[j] must index `samples` and `gt` (not checked on the device).
+ * BLOCKING (the 3-NN reads back a bounding box). */
+size_t gvc_densify_bytes(int total);
+int gvc_densify_in_empty(int count, int k, const gvc_store *store, const float *samples, const float *gt,
+                         const uint32_t *idx, float dist_scale, float init_weight, void *workspace,
+                         size_t workspace_bytes, void *stream);
+
+/* prune_points / the prune of densify_and_prune (R/scene/gaussian_model.py:524-554,
+ * :786): remove the rows with prune_mask[i] != 0, or (prune_mask NULL) with
+ * sigmoid(weight logit) < min_weight (a NaN weight is kept), from every parameter
+ * and moment; the survivors keep their order.  *new_count (device int) = the rows
+ * kept.  Stream-ordered (reading new_count is the caller's sync). */
+size_t gvc_prune_bytes(int count);
+int gvc_prune(int count, const gvc_store *store, const uint8_t *prune_mask, float min_weight, int *new_count,
+              void *workspace, size_t workspace_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

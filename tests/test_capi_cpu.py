@@ -66,3 +66,31 @@ def test_invalid_arguments_rejected_without_gpu(L):
     assert b"P < 0" in L.gvc_last_error()
     st = L.gvc_sample_set_build(10, None, None, 0, None)
     assert st == 1
+
+
+def test_training_structs_and_validation(L):
+    from gvc import _lib
+    # the C layouts: 8 + 8 + 8*8 + 8*4 + 4 + 7*4 bytes; pointers after the int capacity
+    assert ctypes.sizeof(_lib.gvc_train_clock) == 144
+    assert ctypes.sizeof(_lib.gvc_lr_schedule) == 40
+    assert _lib.gvc_store.xyz.offset == 8 and ctypes.sizeof(_lib.gvc_store) == 8 + 15 * 8
+    assert L.gvc_error_order_bytes(100000) > 100000 * 12
+    assert L.gvc_prune_bytes(1000) > 1000 * 12 * 3 * 4
+    assert L.gvc_densify_bytes(1000) > 1000 * 4
+    many = (_lib.gvc_adam_tensor * 9)()
+    assert L.gvc_adam_step(9, many, None, 0.9, 0.999, 1e-15, None) == 1
+    assert b"n_tensors" in L.gvc_last_error()
+    one = (_lib.gvc_adam_tensor * 1)()
+    assert L.gvc_adam_step(1, one, ctypes.c_void_p(8), 1.0, 0.999, 1e-15, None) == 1
+    assert b"betas" in L.gvc_last_error()
+    s = _lib.gvc_store()
+    s.capacity = 10
+    s.xyz = s.weight = s.scaling = s.rotation = s.value = 256
+    assert L.gvc_densify_in_empty(8, 3, ctypes.byref(s), None, None, None, 0.1, 0.01, None, 0, None) == 1
+    assert b"capacity" in L.gvc_last_error()
+    s.exp_avg[0] = 256          # moments must be all present or all absent
+    assert L.gvc_prune(5, ctypes.byref(s), None, 0.005, ctypes.c_void_p(8), ctypes.c_void_p(8), 0, None) == 1
+    assert b"gvc_store" in L.gvc_last_error()
+    sch = _lib.gvc_lr_schedule(1e-4, 1e-6, 1.0, 0, 0)
+    assert L.gvc_lr_advance(ctypes.c_void_p(8), ctypes.byref(sch), 0, -1, None) == 1
+    assert b"max_steps" in L.gvc_last_error()

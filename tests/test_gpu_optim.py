@@ -1,0 +1,347 @@
+"""The training loop's optimiser and model surgery on the GPU (libgvc optim.hip
+through gvc.model / gvc.trainer) against torch.optim.Adam, torch.topk-ordered
+picks and the oracle (oracle/train_ref.py).
+
+Bars: Adam parameters within 1e-6 relative (+1e-8) of torch's Adam after several
+steps (the accumulated updates within 1e-3 of theirs), moments within 1e-6 of
+their largest magnitude; the error order, the
+densified positions, rotations, the untouched rows and the prune bit-exact;
+log-scales and logits within 2 ulp-scale tolerances (logf vs numpy's log); the
+graph-replayed loop bit-identical to the eager loop; the loop within 1e-5 of the
+reference's procedure run with torch's Adam and tensor surgery.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+import torch  # noqa: E402
+
+from oracle import train_ref as tr  # noqa: E402
+from gvc.model import GROUPS, WIDTHS, GaussianModel, OptimizationParams  # noqa: E402
+from gvc.trainer import LoopArgs, Trainer  # noqa: E402
+
+DEV = "cuda:0"
+
+
+def random_params(n, seed):
+    rng = np.random.default_rng(seed)
+    return {"xyz": rng.uniform(0, 1, (n, 3)).astype(np.float32),
+            "weight": rng.normal(-1, 1.5, (n, 1)).astype(np.float32),
+            "scaling": rng.normal(-4, 0.5, (n, 3)).astype(np.float32),
+            "rotation": rng.normal(size=(n, 4)).astype(np.float32),
+            "value": rng.normal(size=(n, 1)).astype(np.float32)}
+
+
+def make_model(params, capacity, setup=True):
+    m = GaussianModel(capacity, DEV)
+    m.load_raw(*(torch.as_tensor(params[g]) for g in GROUPS))
+    if setup:
+        m.training_setup(OptimizationParams())
+    return m
+
+
+def buf(m, g, rows=None):
+    n = m.count if rows is None else rows
+    return m._buf[g][:n].detach().cpu().numpy()
+
+
+def moments(m, g, rows=None):
+    n = m.count if rows is None else rows
+    return m._m[g][:n].cpu().numpy(), m._v[g][:n].cpu().numpy()
+
+
+def lrs_of(opt):
+    return [opt.position_lr_init, opt.weight_lr, opt.scaling_lr, opt.rotation_lr, opt.values_lr]
+
+
+def random_grads(rng, n):
+    return {g: torch.as_tensor((rng.normal(size=(n, w)) * 10.0 ** rng.integers(-5, 1)).astype(np.float32), device=DEV)
+            for g, w in zip(GROUPS, WIDTHS)}
+
+
+def test_adam_matches_torch_adam():
+    n = 3000
+    p = random_params(n, 0)
+    m = make_model(p, 4096)
+    opt = OptimizationParams()
+    ref = tr.RefModel(p, lrs_of(opt), DEV)
+    rng = np.random.default_rng(1)
+    for it in range(1, 7):
+        m.update_learning_rate(it)
+        ref.opt.param_groups[0]["lr"] = tr.expon_lr(it, opt.position_lr_init, opt.position_lr_final,
+                                                     max_steps=opt.position_lr_max_steps)
+        grads = random_grads(rng, n)
+        for g in GROUPS:
+            m._params[g].grad = grads[g].clone()
+            ref.p[g].grad = grads[g].clone()
+        m.optimizer_step()
+        m.zero_grad()
+        ref.opt.step()
+    mom = ref.moments()
+    for g in GROUPS:
+        np.testing.assert_allclose(buf(m, g), ref.p[g].detach().cpu().numpy(), rtol=1e-6, atol=1e-8, err_msg=g)
+        # the accumulated update itself, not only the parameter it moved
+        du, dr = buf(m, g) - p[g], ref.p[g].detach().cpu().numpy() - p[g]
+        assert np.abs(du - dr).max() <= 1e-3 * np.abs(dr).max(), g   # p rounds: a few ulp of p
+        mm, vv = moments(m, g)
+        rm, rv = mom[g][0].cpu().numpy(), mom[g][1].cpu().numpy()
+        assert np.abs(mm - rm).max() <= 1e-6 * np.abs(rm).max(), g
+        assert np.abs(vv - rv).max() <= 1e-6 * np.abs(rv).max(), g
+        assert mom[g][2] == 6.0
+    it, steps, lrs = m.clock()
+    assert (it, steps) == (6, 6)
+    assert lrs[0] == pytest.approx(tr.expon_lr(6, opt.position_lr_init, opt.position_lr_final, max_steps=30000),
+                                   rel=1e-12)
+    assert lrs[1:] == [opt.weight_lr, opt.scaling_lr, opt.rotation_lr, opt.values_lr]
+
+
+def test_adam_skips_tensors_without_gradient():
+    n = 500
+    p = random_params(n, 2)
+    m = make_model(p, 512)
+    rng = np.random.default_rng(3)
+    grads = random_grads(rng, n)
+    m._params["xyz"].grad = grads["xyz"]
+    m.optimizer_step()
+    assert m.clock()[1] == 1
+    assert not np.array_equal(buf(m, "xyz"), p["xyz"])
+    for g in GROUPS[1:]:
+        assert np.array_equal(buf(m, g), p[g]), g
+        assert not moments(m, g)[0].any() and not moments(m, g)[1].any()
+    m.zero_grad()
+    m.optimizer_step()                  # no gradients at all: no step counted (a densification iteration)
+    assert m.clock()[1] == 1
+
+
+@pytest.mark.parametrize("S", [1, 1000, 100003])
+def test_error_order(S):
+    rng = np.random.default_rng(S)
+    cells = np.round(rng.normal(size=S) * 20) / 20          # many ties
+    gt = np.round(rng.normal(size=S) * 20) / 20
+    gt[rng.random(S) < 0.3] = -1.0
+    cells = cells.astype(np.float32)
+    gt = gt.astype(np.float32)
+    if S > 10:
+        cells[7] = np.nan
+        gt[7] = 0.5
+    m = GaussianModel(8, DEV)
+    order, valid = m.error_order(torch.as_tensor(cells, device=DEV), torch.as_tensor(gt, device=DEV))
+    ref, nvalid = tr.error_order(cells, gt)
+    assert int(valid.item()) == nvalid
+    o = order.cpu().numpy()
+    assert np.array_equal(o[:nvalid], ref)
+    assert np.array_equal(np.sort(o[nvalid:]), np.flatnonzero(gt == -1.0))
+
+
+def test_densify_in_empty_rows():
+    n, k, S = 1500, 700, 4000
+    p = random_params(n, 4)
+    m = make_model(p, 3000)
+    rng = np.random.default_rng(5)
+    grads = random_grads(rng, n)
+    for g in GROUPS:
+        m._params[g].grad = grads[g]
+    m.optimizer_step()                  # non-zero moments on the old rows
+    m.zero_grad()
+    old = {g: (buf(m, g).copy(), *[a.copy() for a in moments(m, g)]) for g in GROUPS}
+    samples = rng.uniform(0, 1, (S, 3)).astype(np.float32)
+    gt = rng.uniform(0.01, 0.99, S).astype(np.float32)
+    idx = rng.permutation(S)[:k].astype(np.int32)
+    gt[idx[3]] = 1.0                    # a saturated value logit (+inf), as trained models hold
+    m.densify_in_empty(torch.as_tensor(samples, device=DEV), torch.as_tensor(gt, device=DEV),
+                       idx=torch.as_tensor(idx, device=DEV))
+    assert m.count == n + k
+    for g in GROUPS:
+        assert np.array_equal(buf(m, g, n), old[g][0]), g
+        mm, vv = moments(m, g)
+        assert np.array_equal(mm[:n], old[g][1]) and np.array_equal(vv[:n], old[g][2]), g
+        assert not mm[n:].any() and not vv[n:].any(), g
+    all_xyz = np.concatenate([old["xyz"][0], samples[idx]])
+    assert np.array_equal(buf(m, "xyz"), all_xyz)
+    want = tr.new_rows(all_xyz, n, gt[idx], 0.1, 0.01)
+    np.testing.assert_allclose(buf(m, "scaling")[n:], want["scaling"], rtol=3e-7, atol=1e-6)
+    assert np.array_equal(buf(m, "rotation")[n:], want["rotation"])
+    np.testing.assert_allclose(buf(m, "weight")[n:], want["weight"], rtol=3e-7)
+    got_v = buf(m, "value")[n:]
+    assert np.isposinf(got_v[3, 0]) and np.isposinf(want["value"][3, 0])
+    fin = np.isfinite(want["value"])
+    np.testing.assert_allclose(got_v[fin], want["value"][fin], rtol=3e-7, atol=1e-6)
+
+
+def test_create_from_pcd():
+    rng = np.random.default_rng(6)
+    pts = rng.uniform(0, 1, (900, 3)).astype(np.float32)
+    vals = rng.uniform(0.05, 0.95, 900).astype(np.float32)
+    m = GaussianModel(1000, DEV)
+    m.create_from_pcd(torch.as_tensor(pts, device=DEV), torch.as_tensor(vals, device=DEV))
+    want = tr.new_rows(pts, 0, vals, 1.0, 0.1)
+    assert m.count == 900
+    assert np.array_equal(buf(m, "xyz"), pts)
+    np.testing.assert_allclose(buf(m, "scaling"), want["scaling"], rtol=3e-7, atol=1e-6)
+    np.testing.assert_allclose(buf(m, "weight"), want["weight"], rtol=3e-7)
+    np.testing.assert_allclose(buf(m, "value"), want["value"], rtol=3e-7, atol=1e-6)
+    with pytest.raises(RuntimeError, match="capacity"):
+        m.densify_in_empty(torch.as_tensor(pts, device=DEV), torch.as_tensor(vals, device=DEV))
+
+
+@pytest.mark.parametrize("rule", ["mask", "weight"])
+def test_prune(rule):
+    n = 20000
+    p = random_params(n, 7)
+    thr = math.log(0.005 / 0.995)
+    w = p["weight"][:, 0]
+    w[np.abs(w - thr) < 1e-3] += 0.01   # no logit within rounding of the threshold
+    w[:2000] = thr - np.abs(w[:2000]) - 0.01
+    w[5] = np.nan                       # a NaN weight is kept
+    m = make_model(p, n)
+    rng = np.random.default_rng(8)
+    grads = random_grads(rng, n)
+    for g in GROUPS:
+        m._params[g].grad = grads[g]
+    m.optimizer_step()
+    m.zero_grad()
+    before = {g: (buf(m, g).copy(), *[a.copy() for a in moments(m, g)]) for g in GROUPS}
+    if rule == "mask":
+        mask = rng.random(n) < 0.4
+        m.prune_points(torch.as_tensor(mask, device=DEV))
+        keep = ~mask
+    else:
+        m.prune_points(min_weight=0.005)
+        wl = before["weight"][0][:, 0]
+        keep = ~(1.0 / (1.0 + np.exp(-wl.astype(np.float64))) < 0.005)
+        assert keep[5] and keep.sum() < n - 1500
+    assert m.count == int(keep.sum())
+    for g in GROUPS:
+        assert np.array_equal(buf(m, g), before[g][0][keep], equal_nan=True), g
+        mm, vv = moments(m, g)
+        assert np.array_equal(mm, before[g][1][keep]) and np.array_equal(vv, before[g][2][keep]), g
+    # survivors train on: a step after the surgery still works and counts
+    for g in GROUPS:
+        m._params[g].grad = torch.ones_like(m._params[g])
+    m.optimizer_step()
+    assert m.clock()[1] == 2
+
+
+def field(x):
+    r = np.linalg.norm(x - 0.5, axis=1)
+    v = 0.5 + 0.4 * np.sin(2 * np.pi * x[:, 0]) * np.cos(2 * np.pi * x[:, 1]) * x[:, 2]
+    return np.where(r < 0.45, v, -1.0).astype(np.float32)
+
+
+def loop_problem(seed=0, B=3, S=8192, P0=1500):
+    rng = np.random.default_rng(seed)
+    samples = rng.uniform(0, 1, (B, S, 3)).astype(np.float32)
+    gt = np.stack([field(s) for s in samples])
+    pts = (0.5 + (rng.uniform(-1, 1, (P0, 3)) * 0.3)).astype(np.float32)
+    vals = np.clip(field(pts), 0.05, 0.95)
+    opt = OptimizationParams(iterations=26, densify_from_iter=10, densification_interval=5, densify_until_iter=26)
+    loop = LoopArgs(cap_max=2600, min_weight=0.005, fn_reg=1.0, fn_reg2=2.0)
+    return samples, gt, pts, vals, opt, loop
+
+
+def fresh_model(pts, vals, opt, capacity=8000):
+    m = GaussianModel(capacity, DEV)
+    m.create_from_pcd(torch.as_tensor(pts, device=DEV), torch.as_tensor(vals, device=DEV))
+    m.training_setup(opt)
+    return m
+
+
+def test_trainer_graph_equals_eager():
+    samples, gt, pts, vals, opt, loop = loop_problem()
+    runs = []
+    for graph in (False, True):
+        m = fresh_model(pts, vals, opt)
+        t = Trainer(m, torch.as_tensor(samples), torch.as_tensor(gt), opt, loop, graph=graph, seed=3)
+        t.run()
+        runs.append((m, t))
+    (me, te), (mg, tg) = runs
+    assert te.densifications == tg.densifications == 4          # iterations 10, 15, 20, 25
+    # captured at 2, again after the first surgery and after the fn_reg switch (a
+    # later densification that adds and removes nothing keeps the graph)
+    assert tg.replays > 15 and tg.captures >= 3
+    assert me.count == mg.count and me.count != 1500
+    assert me.clock() == mg.clock()
+    assert me.clock()[:2] == (26, 26 - 1 - 4)                    # the last iteration and densifications take no step
+    for g in GROUPS:
+        assert np.array_equal(buf(me, g), buf(mg, g)), g
+        assert all(np.array_equal(a, b) for a, b in zip(moments(me, g), moments(mg, g))), g
+
+
+def test_trainer_follows_reference_procedure():
+    """Twelve iterations (a densification at 10) against the reference's
+    procedure: torch.optim.Adam, tensor surgery, torch.topk, torch.sigmoid; the
+    mixture, loss and 3-NN are the (separately pinned) libgvc ones."""
+    from bvh_diff_gaussian_rasterization import GaussianRasterizationSettings, GaussianRasterizer
+    from simple_knn._C import distCUDA2
+    from gvc.training import activate, fit_loss
+
+    samples, gt, pts, vals, opt, loop = loop_problem(seed=1)
+    opt = OptimizationParams(iterations=12, densify_from_iter=10, densification_interval=5, densify_until_iter=12)
+    m = fresh_model(pts, vals, opt)
+    init = {g: buf(m, g).copy() for g in GROUPS}
+    t = Trainer(m, torch.as_tensor(samples), torch.as_tensor(gt), opt, loop, graph=True, seed=5)
+    t.run()
+
+    ref = tr.RefModel(init, lrs_of(opt), DEV)
+    rast = GaussianRasterizer(GaussianRasterizationSettings(volume_mins=(0, 0, 0), volume_maxes=(1, 1, 1),
+                                                            cell_count=128, bg=-1.0, scale_modifier=1.0,
+                                                            use_gaussian_bvh=False, debug=False))
+    S_t = torch.as_tensor(samples, device=DEV)
+    G_t = torch.as_tensor(gt, device=DEV)
+    rs = np.random.RandomState(5)
+    fn_reg = loop.fn_reg
+    for it in range(1, opt.iterations + 1):
+        j = rs.randint(0, samples.shape[0])
+        rast.build_bvh(S_t[j])
+        ref.opt.param_groups[0]["lr"] = tr.expon_lr(it, opt.position_lr_init, opt.position_lr_final,
+                                                     max_steps=opt.position_lr_max_steps)
+        sc, rq, w, v = activate(ref.p["scaling"], ref.p["rotation"], ref.p["weight"], ref.p["value"])
+        cells, cw = rast(ref.p["xyz"], sc, rq, v, w)
+        fit_loss(cells, cw, G_t[j], fn_reg).backward()
+        if it > opt.densify_from_iter:
+            fn_reg = loop.fn_reg2
+        if it == 10:
+            with torch.no_grad():
+                cf, gf = cells.reshape(-1), G_t[j].reshape(-1)
+                mask = gf != -1
+                diff = torch.abs(cf[mask] - gf[mask])
+                n = ref.p["xyz"].shape[0]
+                low = torch.count_nonzero(torch.sigmoid(ref.p["weight"]) <= 0.005).item()
+                k = min(min(500000, loop.cap_max - n + low + 1000), diff.numel())
+                loss_idx = torch.nonzero(mask).squeeze(1)[torch.topk(diff, k).indices]
+                new_xyz = S_t[j][loss_idx]
+                allp = torch.cat([ref.p["xyz"].detach(), new_xyz])
+                d2 = torch.clamp_min(distCUDA2(allp) * 0.1, 0.0000001)[n:]
+                ls = torch.log(torch.sqrt(d2))[..., None].repeat(1, 3)
+                rot = torch.zeros((k, 4), device=DEV)
+                rot[:, 0] = 1
+                w0 = 0.01 * torch.ones((k, 1), device=DEV)
+                g = G_t[j][loss_idx].reshape(-1, 1)
+                ref.cat({"xyz": new_xyz, "weight": torch.log(w0 / (1 - w0)), "scaling": ls, "rotation": rot,
+                         "value": torch.log(g / (1 - g))})
+                ref.prune(~(torch.sigmoid(ref.p["weight"]) < loop.min_weight).squeeze())
+        elif it < opt.iterations:
+            ref.opt.step()
+        ref.opt.zero_grad(set_to_none=True)
+    assert m.count == ref.p["xyz"].shape[0]
+    # rows are matched by position: torch.topk orders near-equal errors as its
+    # float rounding falls, the appended block may be permuted; the picks and the
+    # surviving rows must be the same
+    from scipy.spatial import cKDTree
+    rx = ref.p["xyz"].detach().cpu().numpy()
+    d, match = cKDTree(buf(m, "xyz")).query(rx)
+    assert d.max() < 1e-5 and np.unique(match).size == match.size
+    n0 = init["xyz"].shape[0]
+    survivors = match[match < n0]
+    assert np.array_equal(survivors, np.sort(survivors))          # the prune keeps the order
+    for g in GROUPS:
+        a, b = buf(m, g)[match], ref.p[g].detach().cpu().numpy()
+        fin = np.isfinite(b)
+        assert np.array_equal(np.isfinite(a), fin), g
+        err = np.abs(a[fin] - b[fin]).max() / max(np.abs(b[fin]).max(), 1e-30)
+        assert err <= 1e-5, (g, err)
