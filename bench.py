@@ -16,7 +16,10 @@ distCUDA2, identity rotations, weight 0.1, value = vertex value).  Every step:
      bvh_diff_gaussian_rasterization API);
   6. masked L1 + false-negative loss (R/train.py:389-420, fused libgvc kernel),
      backward through loss, mixture (deterministic reduction) and activations;
-  7. Adam (lr 1e-3).
+  7. the position learning-rate schedule and Adam with the reference's per-group
+     rates (R/scene/gaussian_model.py:207-255): one fused libgvc launch over the
+     fixed-capacity parameter buffers of gvc.model.GaussianModel, step count and
+     rates in a device clock.
 With --graph, steps 2-7 are captured once in a hipGraph and replayed (the
 samples are drawn into a static buffer before each replay).
 
@@ -42,6 +45,19 @@ for _p in (PKG, ROOT):
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PAIR = 44  # mean 12 + conic 24 + weight 4 + value 4
+ADAM_BYTES_PER_GAUSSIAN = 12 * 7 * 4  # 12 floats: p, g, m, v read; p, m, v written
+
+
+def make_model(W, max_scale, dev, capacity=None):
+    """A gvc.model.GaussianModel holding the workload's raw parameters, with the
+    reference's optimiser settings."""
+    import torch
+    from gvc.model import GaussianModel, OptimizationParams
+    model = GaussianModel(capacity or W["means"].shape[0], dev, max_scale=max_scale)
+    model.load_raw(W["means"], torch.logit(W["weights"]), W["log_scales"], W["rotations"],
+                   torch.logit(W["values"].clamp(1e-4, 1 - 1e-4)))
+    model.training_setup(OptimizationParams())
+    return model
 
 
 def parse():
@@ -49,11 +65,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="mesh", choices=["mesh", "grid", "batch32", "precision"],
+    ap.add_argument("--config", default="mesh", choices=["mesh", "grid", "batch32", "precision", "train"],
                     help="mesh: the 1M-Gaussian mesh fit (the headline line); grid: the 200k-Gaussian grid fit; "
                          "batch32: 32 uneven volumes in one batched launch per pass beside the plain sequence; "
                          "precision: 4M Gaussians on a 512^3 field, naive / Kahan / FP64-cond against the FP64 "
-                         "oracle on 65,536 points, plus timed steps")
+                         "oracle on 65,536 points, plus timed steps; train: the fitting loop of the mesh "
+                         "workload with densification every 100 iterations (gvc.trainer)")
     ap.add_argument("--plain-steps", type=int, default=3, help="batch32: timed steps of the plain sequence")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--leaf", type=int, default=None)
@@ -99,6 +116,8 @@ def main():
 
     if args.config == "batch32":
         return main_batch(args)
+    if args.config == "train":
+        return main_train(args)
 
     from bvh_diff_gaussian_rasterization import GaussianRasterizationSettings, GaussianRasterizer
     from gvc import mixture as mx
@@ -156,14 +175,9 @@ def main():
                 "ms_fwd_bwd": {k: (round(v, 3) if v is not None else None) for k, v in table.items()}}
         log(f"autotune: {tune['chosen']}")
 
-    xyz = torch.nn.Parameter(W["means"].clone())
     # raw parameters as the reference model keeps them (log-scales of the capped
-    # scale, logits of weight and value)
-    log_s = torch.nn.Parameter(W["log_scales"].clone())
-    rot = torch.nn.Parameter(W["rotations"].clone())
-    wl = torch.nn.Parameter(torch.logit(W["weights"]).clone())
-    vl = torch.nn.Parameter(torch.logit(W["values"].clamp(1e-4, 1 - 1e-4)).clone())
-    opt = torch.optim.Adam([xyz, log_s, rot, wl, vl], lr=1e-3, eps=1e-15, capturable=args.graph, fused=True)
+    # scale, logits of weight and value), in the model's fixed-capacity buffers
+    model = make_model(W, args.max_scale, dev)
     settings = GaussianRasterizationSettings(volume_mins=(0, 0, 0), volume_maxes=(1, 1, 1), cell_count=128, bg=-1.0,
                                              scale_modifier=1.0, use_gaussian_bvh=False, debug=False)
     rast = GaussianRasterizer(settings)
@@ -193,12 +207,13 @@ def main():
             gt = stamp("tet_locate", lambda: sampler.sample(samples, sample_set=rast.sample_set)[0])
         else:
             gt = W["grid_fn"](samples)
-        scales, rots, w, v = stamp("activate", lambda: training.activate(log_s, rot, wl, vl, args.max_scale))
-        cells, cw = rast(xyz, scales, rots, v, w)
+        scales, rots, w, v = stamp("activate", lambda: model.activated())
+        cells, cw = rast(model.get_xyz, scales, rots, v, w)
         loss = stamp("loss", lambda: training.fit_loss(cells, cw, gt, fn_reg))
         loss.backward()
-        stamp("adam", lambda: opt.step())
-        opt.zero_grad(set_to_none=True)
+        model.update_learning_rate()
+        stamp("adam", lambda: model.optimizer_step())
+        model.zero_grad()
         audit[-1].accumulate(acc)
         return samples, cells, cw
 
@@ -244,7 +259,7 @@ def main():
                 body(static)
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
-        opt.zero_grad(set_to_none=True)
+        model.zero_grad()
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
             graph_out = body(static)
@@ -304,6 +319,11 @@ def main():
         "precision_study": study,
         "autotune": tune,
         "stage_ms": stage_avg,
+        # fused Adam: p, g, m, v read and p, m, v written, 12 floats per Gaussian
+        "optimizer": {"kernel": "adam", "bytes_per_launch": ADAM_BYTES_PER_GAUSSIAN * P,
+                      "ms": stage_avg.get("adam"),
+                      "GB/s": (ADAM_BYTES_PER_GAUSSIAN * P / (stage_avg["adam"] * 1e6)
+                               if stage_avg.get("adam") else None)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": BYTES_PER_PAIR * pairs_per_step, "mean_ms": stage_avg[dom]},
@@ -313,8 +333,8 @@ def main():
         import oracle
         samples, cells, cw = last
         with torch.no_grad():
-            m = xyz.detach()
-            sc, rq, wv, vv = (t.detach() for t in training.activate(log_s, rot, wl, vl, args.max_scale))
+            m = model.get_xyz.detach()
+            sc, rq, wv, vv = (t.detach() for t in model.activated())
             # the last step's forward ran before its Adam update: re-evaluate at
             # the current parameters for the error comparison
             ss = mx.SampleSet(samples)
@@ -416,6 +436,88 @@ def precision_study(args, W, dev, n_points=65536, n_stress=16384):
     return out
 
 
+def main_train(args):
+    """The fitting loop (gvc.trainer.Trainer, R/train.py:343-597) on the mesh
+    workload: 1,000,000 Gaussians, 4 precomputed batches of 2^22 mesh-sampled
+    points, the reference's learning rates, densification (error-ordered
+    densify_in_empty + weight prune) every 100 iterations from iteration 100 with
+    cap_max 1,100,000.  Steady iterations are graph replays; a densification
+    iteration runs eagerly and the next one captures again.  Timed: iterations
+    warmup+1 .. warmup+steps (the default 300 include 3 densifications)."""
+    import torch
+
+    from gvc import workloads
+    from gvc.model import OptimizationParams
+    from gvc.sampling import TetMeshSampler
+    from gvc.trainer import LoopArgs, Trainer
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    t_setup = time.perf_counter()
+    W = workloads.mesh_workload(dev, seed=args.seed)
+    S, B = 1 << 22, 4
+    sampler = TetMeshSampler(W["mesh_points"], W["tets"], W["mesh_values"], device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed + 1)
+    samples = torch.rand((B, S, 3), device=dev, generator=gen)
+    gt = torch.stack([sampler.sample(samples[b])[0].float() for b in range(B)])
+    del sampler
+    P0 = W["P"]
+    cap = 1_100_000
+    model = make_model(W, args.max_scale, dev, capacity=cap + 500_000)
+    warm = max(args.warmup, 2)
+    steps = args.steps if args.steps != 20 else 300
+    opt = OptimizationParams(iterations=warm + steps + 1, densify_from_iter=100, densification_interval=100,
+                             densify_until_iter=warm + steps)
+    model.training_setup(opt)
+    trainer = Trainer(model, samples, gt, opt, LoopArgs(cap_max=cap, min_weight=0.005), graph=True, seed=args.seed)
+    log(f"setup {time.perf_counter() - t_setup:.1f}s P={P0} S={S} batches={B}")
+    dens_ms = []
+    orig = model.densify_from_errors
+
+    def timed_densify(*a, **k):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = orig(*a, **k)
+        torch.cuda.synchronize()
+        dens_ms.append(1e3 * (time.perf_counter() - t))
+        log(f"densified: k={out}, Gaussians {model.count}")
+        return out
+
+    model.densify_from_errors = timed_densify
+    trainer.run(1, warm)
+    torch.cuda.synchronize()
+    dens_ms.clear()
+    r0, c0 = trainer.replays, trainer.captures
+    t0 = time.perf_counter()
+    trainer.run(warm + 1, warm + steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = {
+        "metric": "fitting-loop iterations/s (fwd+bwd+Adam, densification every 100 iterations)",
+        "value": steps / wall,
+        "unit": "iterations/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": 1e3 * wall / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "fitting loop on the unstructured mesh fit: 1,000,000 initial Gaussians, 4 batches of "
+                               "2^22 mesh-sampled points, densify every 100 iterations, cap_max 1,100,000",
+                   "parallelism": "single GPU"},
+        "densifications": len(dens_ms),
+        "densify_ms": [round(x, 2) for x in dens_ms],
+        "graph_replays": trainer.replays - r0,
+        "graph_captures": trainer.captures - c0,
+        "gaussians_final": model.count,
+    }
+    print(json.dumps(out))
+
+
 def main_batch(args):
     """32 independent volumes of uneven size (sides log-uniform in [32, 256],
     Gaussians log-uniform in [1e4, 2e6], 2^24 points per step split in
@@ -464,38 +566,33 @@ def main_batch(args):
     nb = len(W["point_batches"])
 
     def make_params():
-        xyz = torch.nn.Parameter(W["means"].clone())
-        log_s = torch.nn.Parameter(W["log_scales"].clone())
-        rot = torch.nn.Parameter(W["rotations"].clone())
-        wl = torch.nn.Parameter(torch.logit(W["weights"]).clone())
-        vl = torch.nn.Parameter(torch.logit(W["values"].clamp(1e-4, 1 - 1e-4)).clone())
-        return [xyz, log_s, rot, wl, vl], torch.optim.Adam([xyz, log_s, rot, wl, vl], lr=1e-3, eps=1e-15, fused=True)
+        return make_model(W, args.max_scale, dev), None
 
     audit = []
     acc = torch.zeros(3, dtype=torch.int64, device=dev)
     plain_pairs = torch.zeros(1, dtype=torch.int64, device=dev)
 
-    def fused_step(k, prm, opt):
-        xyz, log_s, rot, wl, vl = prm
+    def fused_step(k, model, _):
         pts, gt = W["point_batches"][k % nb], W["ground_truth"][k % nb]
         audit.clear()
         ss = bx.BatchedSampleSet(batch, pts)
-        scales, rots, w, v = training.activate(log_s, rot, wl, vl, args.max_scale)
-        cells, cw = bx.rasterize_batch(xyz, scales, rots, v, w, ss, params=params)
+        scales, rots, w, v = model.activated()
+        cells, cw = bx.rasterize_batch(model.get_xyz, scales, rots, v, w, ss, params=params)
         training.fit_loss(cells, cw, gt, 1.0).backward()
-        opt.step()
-        opt.zero_grad(set_to_none=True)
+        model.update_learning_rate()
+        model.optimizer_step()
+        model.zero_grad()
         audit[-1].accumulate(acc)
 
-    def plain_step(k, prm, opt):
-        xyz, log_s, rot, wl, vl = prm
+    def plain_step(k, model, _):
         pts, gt = W["point_batches"][k % nb], W["ground_truth"][k % nb]
         trees = [plain.PlainSampleTree(pts[batch.sample_slice(v)]) for v in range(V)]
-        scales, rots, w, v = training.activate(log_s, rot, wl, vl, args.max_scale)
-        cells, cw = plain.plain_sequence(xyz, scales, rots, v, w, g_off, trees, plain_pairs)
+        scales, rots, w, v = model.activated()
+        cells, cw = plain.plain_sequence(model.get_xyz, scales, rots, v, w, g_off, trees, plain_pairs)
         training.fit_loss(cells, cw, gt, 1.0).backward()
-        opt.step()
-        opt.zero_grad(set_to_none=True)
+        model.update_learning_rate()
+        model.optimizer_step()
+        model.zero_grad()
 
     # fused: warm-up sizes the record capacity, then stage profile, then timed
     fp, fopt = make_params()
