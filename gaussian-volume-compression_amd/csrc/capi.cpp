@@ -102,8 +102,8 @@ int adam_step(int n, const gvc_adam_tensor* tensors, gvc_train_clock* clock, dou
 int lr_advance(gvc_train_clock* clock, const gvc_lr_schedule* s, int lr_index, int64_t set_iteration,
                hipStream_t st);
 size_t error_order_bytes(int S);
-int error_order(int S, const float* cells, const float* gt, uint32_t* order, uint32_t* valid_count, void* ws,
-                size_t bytes, hipStream_t st);
+int error_order(int S, const float* cells, const float* gt, int count, const float* weight_logits,
+                float low_weight, uint32_t* order, uint32_t* counts, void* ws, size_t bytes, hipStream_t st);
 size_t densify_bytes(int total);
 int densify_in_empty(int count, int k, const gvc_store* s, const float* samples, const float* gt,
                      const uint32_t* idx, float dist_scale, float init_weight, void* ws, size_t bytes,
@@ -472,12 +472,15 @@ int gvc_adam_step(int n_tensors, const gvc_adam_tensor* tensors, gvc_train_clock
 
 size_t gvc_error_order_bytes(int S) { return S >= 0 ? error_order_bytes(S) : 0; }
 
-int gvc_error_order(int S, const float* cells, const float* gt, uint32_t* order, uint32_t* valid_count,
-                    void* workspace, size_t workspace_bytes, void* stream)
+int gvc_error_order(int S, const float* cells, const float* gt, int count, const float* weight_logits,
+                    float low_weight, uint32_t* order, uint32_t* counts, void* workspace, size_t workspace_bytes,
+                    void* stream)
 {
-    GVC_REQUIRE(S >= 0, "S < 0");
-    GVC_REQUIRE(valid_count && workspace && (S == 0 || (cells && gt && order)), "NULL pointer");
-    return error_order(S, cells, gt, order, valid_count, workspace, workspace_bytes, (hipStream_t)stream);
+    GVC_REQUIRE(S >= 0 && count >= 0, "S < 0 or count < 0");
+    GVC_REQUIRE(counts && workspace && (S == 0 || (cells && gt && order)), "NULL pointer");
+    GVC_REQUIRE(count == 0 || weight_logits, "NULL weight_logits");
+    return error_order(S, cells, gt, count, weight_logits, low_weight, order, counts, workspace, workspace_bytes,
+                       (hipStream_t)stream);
 }
 
 static bool store_ok(const gvc_store* s)

@@ -17,7 +17,9 @@
 //
 // Surgery (R/scene/gaussian_model.py:396-612, :727-799; R/train.py:551-590):
 //   - the error order: indices of the samples with gt != -1 by descending
-//     |cells - gt| (ties: ascending index), the reference's torch.topk pick;
+//     |cells - gt| (ties: ascending index), the reference's torch.topk pick, with
+//     the two counts its pick size needs (valid samples, Gaussians of weight
+//     <= 0.005);
 //   - densify_in_empty: rows [count, count + k) of every parameter take the
 //     picked samples' positions, log-scales log(sqrt(max(knn3 * f, 1e-7))) on all
 //     three axes (knn3 over the count + k positions; f = 0.1 there, 1 for the
@@ -129,20 +131,47 @@ __global__ void lr_advance_kernel(gvc_train_clock* clock, gvc_lr_schedule s, int
 
 // Ascending key = descending |cells - gt| over the valid samples (31 magnitude
 // bits; NaN above +inf, as torch.topk ranks it); invalid samples (gt == -1) key
-// 2^31, after every valid one.
-__global__ void error_key_kernel(int S, const float* __restrict__ cells, const float* __restrict__ gt,
-                                 uint32_t* __restrict__ keys, uint32_t* __restrict__ valid_count)
+// 2^31, after every valid one.  Also counts the valid samples and the Gaussians
+// with weight sigmoid(logit) <= low_weight: per block in LDS, one atomic per
+// block and counter (integer sums: the counts do not depend on order).
+constexpr int kKeyThreads = 256;
+constexpr unsigned kKeyBlocks = 1024;
+
+__device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+__global__ __launch_bounds__(kKeyThreads) void error_key_kernel(int S, const float* __restrict__ cells,
+                                                                const float* __restrict__ gt,
+                                                                uint32_t* __restrict__ keys, int count,
+                                                                const float* __restrict__ weight_logits,
+                                                                float low_weight, uint32_t* __restrict__ counts)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = false;
-    if (i < S) {
-        const float g = gt[i];
-        valid = g != -1.0f;
-        const uint32_t mag = __float_as_uint(fabsf(cells[i] - g)) & 0x7fffffffu;
-        keys[i] = valid ? 0x7fffffffu - mag : 0x80000000u;
+    __shared__ uint32_t part[2][kKeyThreads / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t nvalid = 0, nlow = 0;
+    const int n = S > count ? S : count;
+    for (int i0 = blockIdx.x * kKeyThreads; i0 < n; i0 += gridDim.x * kKeyThreads) {
+        const int i = i0 + threadIdx.x;
+        bool valid = false, low = false;
+        if (i < S) {
+            const float g = gt[i];
+            valid = g != -1.0f;
+            const uint32_t mag = __float_as_uint(fabsf(cells[i] - g)) & 0x7fffffffu;
+            keys[i] = valid ? 0x7fffffffu - mag : 0x80000000u;
+        }
+        if (weight_logits && i < count) low = sigmoidf(weight_logits[i]) <= low_weight;
+        nvalid += (uint32_t)__popcll(__ballot(valid));
+        nlow += (uint32_t)__popcll(__ballot(low));
     }
-    const uint64_t bal = __ballot(valid);
-    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(valid_count, (uint32_t)__popcll(bal));
+    if (lane == 0) {
+        part[0][w] = nvalid;
+        part[1][w] = nlow;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t t = 0;
+        for (int k = 0; k < kKeyThreads / 64; ++k) t += part[threadIdx.x][k];
+        if (t) atomicAdd(&counts[threadIdx.x], t);
+    }
 }
 
 // ---- densify ---------------------------------------------------------------------
@@ -197,8 +226,6 @@ __global__ void init_rows_kernel(int count, int k, StoreArgs s, const float* __r
 }
 
 // ---- prune ------------------------------------------------------------------------
-
-__device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
 
 __global__ void keep_kernel(int count, const float* __restrict__ weight_logits, const uint8_t* __restrict__ mask,
                             float min_weight, uint32_t* __restrict__ keep)
@@ -336,16 +363,19 @@ size_t error_order_bytes(int S)
     return a.off + 256;
 }
 
-int error_order(int S, const float* cells, const float* gt, uint32_t* order, uint32_t* valid_count, void* ws,
-                size_t bytes, hipStream_t st)
+int error_order(int S, const float* cells, const float* gt, int count, const float* weight_logits,
+                float low_weight, uint32_t* order, uint32_t* counts, void* ws, size_t bytes, hipStream_t st)
 {
     Arena a(ws, bytes);
     ErrorOrderLayout l(a, S);
     if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "error order workspace too small");
-    GVC_TRY(fill_u32(valid_count, 0u, 1, st));
-    if (S == 0) return GVC_OK;
-    error_key_kernel<<<blocks_for(S, 256), 256, 0, st>>>(S, cells, gt, l.keys, valid_count);
+    GVC_TRY(fill_u32(counts, 0u, 2, st));
+    const int n = S > count ? S : count;
+    if (n == 0) return GVC_OK;
+    const unsigned nb = blocks_for((size_t)n, kKeyThreads) < kKeyBlocks ? blocks_for((size_t)n, kKeyThreads) : kKeyBlocks;
+    error_key_kernel<<<nb, kKeyThreads, 0, st>>>(S, cells, gt, l.keys, count, weight_logits, low_weight, counts);
     GVC_LAUNCH_CHECK("error_key");
+    if (S == 0) return GVC_OK;
     bool alt = false;
     GVC_TRY(radix_sort_u32(l.keys, order, l.keys_alt, l.vals_alt, (size_t)S, 32, l.scratch, &alt, st, nullptr,
                            true));

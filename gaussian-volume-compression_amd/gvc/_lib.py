@@ -123,7 +123,7 @@ _SIGS = {
     "gvc_adam_step": (_I, [_I, ctypes.POINTER(gvc_adam_tensor), _P, ctypes.c_double, ctypes.c_double,
                                ctypes.c_double, _P]),
     "gvc_error_order_bytes": (_SZ, [_I]),
-    "gvc_error_order": (_I, [_I, _P, _P, _P, _P, _P, _SZ, _P]),
+    "gvc_error_order": (_I, [_I, _P, _P, _I, _P, _F, _P, _P, _P, _SZ, _P]),
     "gvc_densify_bytes": (_SZ, [_I]),
     "gvc_densify_in_empty": (_I, [_I, _I, ctypes.POINTER(gvc_store), _P, _P, _P, _F, _F, _P, _SZ, _P]),
     "gvc_prune_bytes": (_SZ, [_I]),

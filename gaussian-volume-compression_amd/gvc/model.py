@@ -217,14 +217,14 @@ class GaussianModel:
 
     # ---- surgery ------------------------------------------------------------------------
 
-    def _append(self, samples, gt, idx, dist_scale, init_weight):
+    def _append(self, samples, gt, idx, dist_scale, init_weight, trusted=False):
         k = int(idx.numel()) if idx is not None else int(samples.shape[0])
         self._check_rows(k)
         samples = samples.to(self.device, torch.float32).contiguous()
         gt = gt.to(self.device, torch.float32).contiguous().reshape(-1)
         if idx is not None:
             idx = idx.to(self.device, torch.int32).contiguous()
-            if k and not bool(((idx >= 0) & (idx < samples.shape[0])).all().item()):
+            if k and not trusted and not bool(((idx >= 0) & (idx < samples.shape[0])).all().item()):
                 raise IndexError("densify indices outside the sample array")
         if gt.numel() != samples.shape[0]:
             raise ValueError(f"{gt.numel()} values for {samples.shape[0]} samples")
@@ -268,32 +268,36 @@ class GaussianModel:
         self.densify_in_empty(empty_points, empty_values, new_scale, idx=idx)
         self.prune_points(min_weight=min_weight)
 
-    def error_order(self, cells, gt):
-        """(order, valid count): sample indices with gt != -1 by descending
-        |cells - gt| (ties in ascending index), then the rest."""
+    def error_order(self, cells, gt, low_weight=0.005):
+        """(order, counts): sample indices with gt != -1 by descending
+        |cells - gt| (ties in ascending index), then the rest; counts (device
+        int32 [2]) = the valid samples and the Gaussians of weight <= low_weight."""
         cells = cells.detach().reshape(-1).to(torch.float32).contiguous()
         gt = gt.reshape(-1).to(torch.float32).contiguous()
         S = int(cells.numel())
         order = torch.empty(S, dtype=torch.int32, device=self.device)
-        valid = torch.empty(1, dtype=torch.int32, device=self.device)
+        counts = torch.empty(2, dtype=torch.int32, device=self.device)
         nb = lib().gvc_error_order_bytes(S)
         ws = torch.empty(nb, dtype=torch.uint8, device=self.device)
-        call("gvc_error_order", S, _ptr(cells), _ptr(gt), _ptr(order), _ptr(valid), _ptr(ws), nb,
-             _stream(self.device))
-        return order, valid
+        call("gvc_error_order", S, _ptr(cells), _ptr(gt), self.count, _ptr(self._buf["weight"]), float(low_weight),
+             _ptr(order), _ptr(counts), _ptr(ws), nb, _stream(self.device))
+        return order, counts
 
     def densify_from_errors(self, cells, gt, samples, cap_max, min_weight, max_k=500000):
         """The densification step of R/train.py:562-590: k = min(max_k, cap_max -
         count + #(weight <= 0.005) + 1000, #(gt != -1)) new Gaussians at the samples
-        of largest |cells - gt|, then the weight prune.  Returns k.  Reads two
-        counts back, as the reference does."""
+        of largest |cells - gt|, then the weight prune.  Returns k.  Reads the two
+        counts back (one small copy), then the pruned count, as the reference reads
+        its shapes and counts."""
         with torch.no_grad():
-            order, valid = self.error_order(cells, gt)
-            low = int(torch.count_nonzero(self.get_weight <= 0.005).item())
+            order, counts = self.error_order(cells, gt)
+            valid, low = (int(x) for x in counts.cpu().tolist())
             k = min(int(max_k), int(cap_max) - self.count + low + 1000)
-            k = min(k, int(valid.item()))
+            k = min(k, valid)
             if k < 0:
                 raise RuntimeError(f"selected index k out of range (k = {k}; is cap_max set?)")
-            self.densify_and_prune(0.0, min_weight, None, samples.reshape(-1, 3), gt.reshape(-1), k,
-                                   idx=order[:k])
+            # indices from the error order are in range by construction
+            self._append(samples.reshape(-1, 3), gt.reshape(-1), order[:k], dist_scale=0.1, init_weight=0.01,
+                         trusted=True)
+            self.prune_points(min_weight=min_weight)
         return k

@@ -347,12 +347,16 @@ int gvc_adam_step(int n_tensors, const gvc_adam_tensor *tensors, gvc_train_clock
                   double beta2, double eps, void *stream);
 
 /* The densification pick of R/train.py:562-578 (torch.topk of |cells - gt| over
- * gt != -1): order[0..S) = the indices of the valid samples by descending error,
- * ties in ascending index, then the invalid ones; *valid_count (device uint32) =
- * the number of valid samples.  Stream-ordered. */
+ * gt != -1, k from the Gaussian count, cap_max and the low-weight count):
+ * order[0..S) = the indices of the valid samples by descending error, ties in
+ * ascending index, then the invalid ones; counts (device uint32 [2]) = the number
+ * of valid samples and the number of the `count` Gaussians whose weight
+ * sigmoid(weight_logits[i]) <= low_weight (the reference uses 0.005).
+ * Stream-ordered. */
 size_t gvc_error_order_bytes(int S);
-int gvc_error_order(int S, const float *cells, const float *gt, uint32_t *order, uint32_t *valid_count,
-                    void *workspace, size_t workspace_bytes, void *stream);
+int gvc_error_order(int S, const float *cells, const float *gt, int count, const float *weight_logits,
+                    float low_weight, uint32_t *order, uint32_t *counts, void *workspace, size_t workspace_bytes,
+                    void *stream);
 
 /* The trainable model: the five raw parameter arrays of capacity rows (means
  * [C,3], weight logits [C], log-scales [C,3], quaternions [C,4], value logits [C]:

@@ -129,10 +129,11 @@ def test_error_order(S):
     if S > 10:
         cells[7] = np.nan
         gt[7] = 0.5
-    m = GaussianModel(8, DEV)
-    order, valid = m.error_order(torch.as_tensor(cells, device=DEV), torch.as_tensor(gt, device=DEV))
+    wl = rng.normal(-4, 2, 3000).astype(np.float32)
+    m = make_model(random_params(3000, S) | {"weight": wl[:, None]}, 3000, setup=False)
+    order, counts = m.error_order(torch.as_tensor(cells, device=DEV), torch.as_tensor(gt, device=DEV))
     ref, nvalid = tr.error_order(cells, gt)
-    assert int(valid.item()) == nvalid
+    assert counts.cpu().tolist() == [nvalid, int((1 / (1 + np.exp(-wl.astype(np.float64))) <= 0.005).sum())]
     o = order.cpu().numpy()
     assert np.array_equal(o[:nvalid], ref)
     assert np.array_equal(np.sort(o[nvalid:]), np.flatnonzero(gt == -1.0))
