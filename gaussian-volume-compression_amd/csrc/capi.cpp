@@ -109,6 +109,9 @@ int densify_in_empty(int count, int k, const gvc_store* s, const float* samples,
                      const uint32_t* idx, float dist_scale, float init_weight, void* ws, size_t bytes,
                      hipStream_t st);
 size_t prune_bytes(int count);
+// metrics.hip
+size_t volume_metrics_bytes(int S);
+int volume_metrics(int S, const float* cells, const float* gt, double* out, void* ws, size_t bytes, hipStream_t st);
 int prune(int count, const gvc_store* s, const uint8_t* mask, float min_weight, int* new_count, void* ws,
           size_t bytes, hipStream_t st);
 
@@ -517,6 +520,16 @@ int gvc_prune(int count, const gvc_store* store, const uint8_t* prune_mask, floa
     GVC_REQUIRE(count <= store->capacity, "count exceeds the store capacity");
     GVC_REQUIRE(new_count && workspace, "NULL pointer");
     return prune(count, store, prune_mask, min_weight, new_count, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+size_t gvc_volume_metrics_bytes(int S) { return S >= 0 ? volume_metrics_bytes(S) : 0; }
+
+int gvc_volume_metrics(int S, const float* cells, const float* gt, double* out, void* workspace,
+                       size_t workspace_bytes, void* stream)
+{
+    GVC_REQUIRE(S >= 0, "S < 0");
+    GVC_REQUIRE(out && workspace && (S == 0 || (cells && gt)), "NULL pointer");
+    return volume_metrics(S, cells, gt, out, workspace, workspace_bytes, (hipStream_t)stream);
 }
 
 }  // extern "C"

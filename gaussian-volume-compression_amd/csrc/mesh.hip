@@ -5,11 +5,13 @@
 // :155-167, :380-390; GS/ = gpu_sampling/).  Differences by design: device
 // pointers in and out (no host staging), int32 cell ids are returned, nothing is
 // printed, and location is exact:
-//   point p is in tet T = (v0,v1,v2,v3) iff, for every face, orient3d of the face
-//   and p has the sign of orient3d(T), with simulation of simplicity resolving
-//   exact zeros: p is perturbed to p + (e, e^2, e^3), so a point on a shared face,
-//   edge or vertex belongs to exactly one tet of a conforming mesh.  Among several
-//   containing tets (overlapping meshes) the lowest index wins.
+//   point p is in the closed tet T = (v0,v1,v2,v3) iff, for every face, orient3d
+//   of the face and p is zero or has the sign of orient3d(T) (degenerate tets
+//   hold nothing); among several containing tets the lowest index wins.  As
+//   Probe's cells are closed (parametric coordinates within [0, 1] up to a
+//   tolerance), a point on the mesh boundary is inside -- the reference's own
+//   evaluation grid clips samples onto the boundary planes -- and a point on a
+//   face, edge or vertex shared by several tets takes the lowest of them.
 //
 // Locator: a uniform grid over the mesh's bounding box (about one cell per eight
 // tets).  Each cell lists every tet whose bounding box overlaps it as an 8-byte
@@ -228,7 +230,7 @@ __global__ __launch_bounds__(256) void tet_sort_cells_kernel(const uint32_t* __r
     }
 }
 
-// Exact containment of p in tet r with the simulation-of-simplicity tie rule.
+// Exact containment of p in the closed tet r.
 __device__ __noinline__ bool tet_contains_exact(const float* __restrict__ pts, const TetRec& r, const float* p)
 {
     const float* V[4] = { pts + 3 * r.v[0], pts + 3 * r.v[1], pts + 3 * r.v[2], pts + 3 * r.v[3] };
@@ -241,12 +243,8 @@ __device__ __noinline__ bool tet_contains_exact(const float* __restrict__ pts, c
         for (int k = 0; k < 4; ++k)
             if (k != i) F[m++] = V[k];
         const int si = ((3 - i) & 1) ? -1 : 1;
-        int sg = orient3d_sign(F[0], F[1], F[2], p, nullptr);
-        if (sg == 0) {
-            // d/dp orient(F, p) = -(F1 - F0) x (F2 - F0); p + (e, e^2, e^3)
-            for (int a = 0; a < 3 && sg == 0; ++a) sg = -normal_sign(F[0], F[1], F[2], a);
-        }
-        if (si * sg != sT) return false;
+        const int sg = orient3d_sign(F[0], F[1], F[2], p, nullptr);
+        if (sg != 0 && si * sg != sT) return false;
     }
     return true;
 }

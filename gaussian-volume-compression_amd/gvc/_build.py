@@ -15,7 +15,7 @@ CSRC = os.path.join(ROOT, "csrc")
 INCLUDE = os.path.join(os.path.dirname(ROOT), "include")
 LIB_PATH = os.path.join(PKG_DIR, "libgvc.so")
 
-SOURCES = ["util.hip", "radix_sort.hip", "mixture.hip", "mesh.hip", "knn.hip", "training.hip", "optim.hip", "capi.cpp"]
+SOURCES = ["util.hip", "radix_sort.hip", "mixture.hip", "mesh.hip", "knn.hip", "training.hip", "optim.hip", "metrics.hip", "capi.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -128,6 +128,8 @@ _SIGS = {
     "gvc_densify_in_empty": (_I, [_I, _I, ctypes.POINTER(gvc_store), _P, _P, _P, _F, _F, _P, _SZ, _P]),
     "gvc_prune_bytes": (_SZ, [_I]),
     "gvc_prune": (_I, [_I, ctypes.POINTER(gvc_store), _P, _F, _P, _P, _SZ, _P]),
+    "gvc_volume_metrics_bytes": (_SZ, [_I]),
+    "gvc_volume_metrics": (_I, [_I, _P, _P, _P, _P, _SZ, _P]),
 }
 
 EXPORTED = tuple(_SIGS.keys())

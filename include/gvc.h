@@ -394,6 +394,19 @@ size_t gvc_prune_bytes(int count);
 int gvc_prune(int count, const gvc_store *store, const uint8_t *prune_mask, float min_weight, int *new_count,
               void *workspace, size_t workspace_bytes, void *stream);
 
+/* ---- evaluation ----------------------------------------------------------------
+ * The sums behind the volume evaluation of R/volume_metrics.py:318-369 (L1, L2, the
+ * five PSNR variants, false negatives and positives), in one pass over S (cells,
+ * gt) pairs, FP64, deterministic: out (device double [GVC_METRIC_SUMS]) =
+ *   [0] sum |c-g|  [1] sum (c-g)^2  [2] sum (c-g)^2 over g != -1 & c != -1  [3] its count
+ *   [4] sum g^2  [5] sum (c0-g)^2 with c0 = (c == -1 ? 0 : c)  [6] that over g != -1
+ *   [7] count g != -1  [8] count g == -1  [9] count c == -1 & g != -1  [10] count c != -1 & g == -1
+ * Stream-ordered. */
+#define GVC_METRIC_SUMS 11
+size_t gvc_volume_metrics_bytes(int S);
+int gvc_volume_metrics(int S, const float *cells, const float *gt, double *out, void *workspace,
+                       size_t workspace_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

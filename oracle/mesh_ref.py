@@ -59,7 +59,8 @@ def _normal_exact(f0, f1, f2):
 
 
 def tet_contains(P, tet, p):
-    """(inside, barycentric float64[4]) under the simulation-of-simplicity rule."""
+    """(inside, barycentric float64[4]) for the closed tet: every face orientation
+    zero or of the tet's sign; a degenerate tet holds nothing."""
     V = [P[int(i)] for i in tet]
     sT = orient_sign(V[0], V[1], V[2], V[3])
     if sT == 0:
@@ -69,13 +70,7 @@ def tet_contains(P, tet, p):
         F = [V[k] for k in range(4) if k != i]
         si = -1 if ((3 - i) & 1) else 1
         sg = orient_sign(F[0], F[1], F[2], p)
-        if sg == 0:
-            n = _normal_exact(*F)
-            for comp in n:
-                if comp != 0:
-                    sg = -_sign(comp)
-                    break
-        if si * sg != sT:
+        if sg != 0 and si * sg != sT:
             return False, None
         signs.append(si)
     vol = _orient_exact(V[0], V[1], V[2], V[3])

@@ -208,3 +208,25 @@ def test_exterior_points_locate_outside():
     lifted = h > 1e-5
     assert lifted.mean() > 0.99
     assert np.all(gid[lifted] == -1) and np.all(gv[lifted] == -1.0)
+
+
+def test_boundary_points_are_inside():
+    """Points clipped onto the mesh's boundary planes -- what the reference's
+    evaluation grid does (np.clip to the box, R/volume_metrics.py:222-226) -- lie
+    in closed boundary tets and are located, as Probe's closed cells locate them."""
+    from gvc.evaluation import evaluation_grid
+    rng = np.random.default_rng(21)
+    n = 6
+    pts = workloads.jittered_lattice(rng, n, 0.3)
+    tets = workloads.kuhn_tets(n)
+    vals = 2.0 * pts[:, 0] - pts[:, 1] + 0.25 * pts[:, 2]
+    grid, _ = evaluation_grid([0, 0, 0], [1, 1, 1], 24, np.random.RandomState(1))
+    g32 = grid.astype(f32)
+    on_face = np.any((g32 == 0) | (g32 == 1), axis=1)
+    assert on_face.sum() > 1000
+    gv, gid = TetMeshSampler(torch.tensor(pts, device=DEV), tets, vals).sample(torch.tensor(g32, device=DEV))
+    gid = gid.cpu().numpy()
+    assert np.all(gid >= 0)
+    ids, ref = mesh_ref.locate(pts, tets, g32[on_face][::5], vals)
+    assert np.array_equal(gid[on_face][::5], ids)
+    np.testing.assert_allclose(gv.cpu().numpy()[on_face][::5], ref, rtol=1e-9, atol=1e-12)

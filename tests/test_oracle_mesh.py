@@ -2,8 +2,9 @@
 
 Tet location: by construction (a point drawn inside tet t with barycentrics b
 lies in t and interpolates to sum b v -- the construction of R/train.py:34-61),
-and the partition property of the tie rule (every point of a conforming mesh's
-interior lies in exactly one tet, also on shared faces, edges and vertices).
+closed cells (every point of the closed domain, boundary included, is located;
+the lowest containing tet wins; shared faces, edges and vertices interpolate an
+affine field exactly).
 Grid probe: trilinear interpolation reproduces affine functions exactly.
 3-NN: against scipy's KD-tree.
 """
@@ -39,17 +40,30 @@ def test_by_construction_and_values():
     np.testing.assert_allclose(out, expect, rtol=1e-5, atol=1e-5)
 
 
-def test_tie_rule_partitions_the_domain():
-    pts, tets = small_mesh(2, 0.0, 0)  # exact lattice: many points on shared faces
+def test_closed_tets_cover_the_domain():
+    pts, tets = small_mesh(2, 0.0, 0)  # exact lattice: many points on shared faces, edges, vertices
     g = np.linspace(0.0, 1.0, 9, dtype=f32)
     X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
     samples = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
-    # interior points only: exactly one tet must contain each (SoS perturbation)
-    interior = np.all((samples > 0) & (samples < 1), axis=1)
     P = pts
-    for p in samples[interior][::3]:
-        n_in = sum(mesh_ref.tet_contains(P, tets[t], p)[0] for t in range(tets.shape[0]))
-        assert n_in == 1, p
+    vals = 1.0 + 2.0 * pts[:, 0] - pts[:, 1] + 0.5 * pts[:, 2]
+    ids, out = mesh_ref.locate(P, tets, samples, vals)
+    # every point of the closed domain is located, boundary points included (the
+    # reference's evaluation grid clips samples onto the boundary planes)
+    assert np.all(ids >= 0)
+    # the lowest containing tet wins, and it really contains the point
+    for s_, p in enumerate(samples[::7]):
+        holders = [t for t in range(tets.shape[0]) if mesh_ref.tet_contains(P, tets[t], p)[0]]
+        assert holders and ids[7 * s_] == holders[0]
+    # points on shared faces, edges and vertices interpolate the (continuous) affine field exactly
+    s64 = samples.astype(np.float64)
+    np.testing.assert_allclose(out, 1.0 + 2.0 * s64[:, 0] - s64[:, 1] + 0.5 * s64[:, 2], rtol=0, atol=1e-12)
+    # an interior lattice vertex is held by exactly the tets it is a vertex of
+    centre = np.array([0.5, 0.5, 0.5], f32)
+    vid = int(np.nonzero(np.all(P == centre, axis=1))[0][0])
+    around = int(np.any(tets == vid, axis=1).sum())
+    assert around > 6
+    assert sum(mesh_ref.tet_contains(P, tets[t], centre)[0] for t in range(tets.shape[0])) == around
 
 
 def test_outside_points():
