@@ -99,16 +99,4 @@ __host__ __device__ inline int orient3d_sign(const float* a, const float* b, con
     return exact_sum_sign(terms, n);
 }
 
-// Sign of the exact value of a.y b.z - a.z b.y style 2x2 minor built from
-// differences: n = (f1 - f0) x (f2 - f0), component `axis`.
-__host__ __device__ inline int normal_sign(const float* f0, const float* f1, const float* f2, int axis)
-{
-    const int u = (axis + 1) % 3, v = (axis + 2) % 3;
-    // n_axis = (f1u - f0u)(f2v - f0v) - (f1v - f0v)(f2u - f0u)
-    //        = f1u f2v - f1u f0v - f0u f2v - f1v f2u + f1v f0u + f0v f2u   (f0u f0v cancels)
-    double t[6] = { (double)f1[u] * f2[v], -(double)f1[u] * f0[v], -(double)f0[u] * f2[v],
-                    -(double)f1[v] * f2[u], (double)f1[v] * f0[u], (double)f0[v] * f2[u] };
-    return exact_sum_sign(t, 6);
-}
-
 }  // namespace gvc

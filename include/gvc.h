@@ -162,9 +162,10 @@ int gvc_mixture_backward(int P, const float *means3D, const float *scales, const
                          float *dL_dvalues, float *dL_dweights, void *stream);
 
 /* ---- mesh sampler ------------------------------------------------------------
- * Tetrahedral meshes: exact point location (orient3d signs with simulation of
- * simplicity, so points on shared faces/edges/vertices belong to exactly one tet)
- * and barycentric interpolation of a per-point field.  Replaces sample_meshu over
+ * Tetrahedral meshes: exact point location in closed tets (exact orient3d signs; a
+ * point on a face, edge or vertex is inside, on the mesh boundary too, as Probe's
+ * closed cells hold it; where several tets hold a point the lowest index wins) and
+ * barycentric interpolation of a per-point field.  Replaces sample_meshu over
  * Viskores Probe, GS/sample_mesh.cu:242-414 (cells of VTK type 10), and returns the
  * int32 containing-cell id (-1 outside) besides the value (-1.0 outside, :382).
  * points [n_points,3] f32, tets [n_tets,4] i32 (device).  The locator buffer

@@ -1,9 +1,9 @@
 """CPU oracle for the mesh sampler and the 3-NN initialiser -- TEST INFRASTRUCTURE ONLY.
 
 Tet location (GS/sample_mesh.cu:242-414 semantics; GS/ = R/gpu_sampling/): a point
-is in tet (v0,v1,v2,v3) iff every face orientation of the point agrees with the
-tet's orientation, with exact zeros resolved by simulation of simplicity (the point
-perturbed to p + (e, e^2, e^3)); the lowest containing tet index wins; -1 outside.
+is in the closed tet (v0,v1,v2,v3) iff every face orientation of the point is zero
+or agrees with the tet's orientation (degenerate tets hold nothing); the lowest
+containing tet index wins; -1 outside.
 Orientation signs are exact: Python ``fractions.Fraction`` over the float32 inputs
 (filtered by a float64 evaluation with a generous error bound).  This is an
 independent implementation of the predicate the GPU evaluates with double
@@ -47,15 +47,6 @@ def orient_sign(a, b, c, d):
     if abs(det) > bound:
         return int(np.sign(det))
     return _sign(_orient_exact(a, b, c, d))
-
-
-def _normal_exact(f0, f1, f2):
-    F0 = [Fraction(float(x)) for x in f0]
-    F1 = [Fraction(float(x)) for x in f1]
-    F2 = [Fraction(float(x)) for x in f2]
-    u = [F1[i] - F0[i] for i in range(3)]
-    v = [F2[i] - F0[i] for i in range(3)]
-    return [u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]]
 
 
 def tet_contains(P, tet, p):
