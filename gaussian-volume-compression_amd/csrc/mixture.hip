@@ -4,7 +4,7 @@
 //   sample set   : 30-bit Morton keys of the samples -> stable radix sort -> float4
 //                  points (x, y, z, original index) in Morton order; every 64
 //                  consecutive points form one group = one wavefront.
-//   forward      : top 48 bits of the means' 63-bit Morton codes (the reference's
+//   forward      : top 32 bits of the means' 63-bit Morton codes (the reference's
 //                  morton3d_unit) -> stable radix sort -> canonical preprocess in
 //                  the caller's order written as 80-byte records at their BVH
 //                  positions -> Karras radix tree over leaves of
@@ -231,13 +231,6 @@ __global__ void gather_u32_kernel(const uint32_t* __restrict__ src, const uint32
     if (i < n) dst[i] = src[pos[i]];
 }
 
-__global__ void gather_u64_kernel(const uint64_t* __restrict__ src, const uint32_t* __restrict__ pos, int n,
-                                  uint64_t* __restrict__ dst)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[pos[i]];
-}
-
 __global__ void sample_header_kernel(SampleSetHeader* h, int S, int NG)
 {
     h->S = S;
@@ -249,19 +242,19 @@ __global__ void sample_header_kernel(SampleSetHeader* h, int S, int NG)
 // ---------------------------------------------------------------------------
 // Gaussian BVH
 // ---------------------------------------------------------------------------
-// The tree orders Gaussians by the top 48 bits of their 63-bit Morton codes (16
-// bits per axis, a 65536^3 lattice): six 8-bit radix passes instead of eight;
-// equal keys keep the caller's order (stable sort) and the Karras build breaks
-// ties by index.
-constexpr int kGaussianCodeBits = 48;
+// The tree orders Gaussians by the top 32 bits of their 63-bit Morton codes
+// (about 11 bits per axis, a lattice of ~2^32 cells, far finer than the Gaussians
+// of a volume): four 8-bit radix passes on 32-bit keys instead of eight on 64-bit
+// ones; equal keys keep the caller's order (stable sort) and the Karras build
+// breaks ties by index.
+constexpr int kGaussianCodeBits = 32;
 
-__global__ void gaussian_codes_kernel(const float* __restrict__ means, int P, uint64_t* __restrict__ codes,
-                                      uint32_t* __restrict__ idx)
+__global__ void gaussian_codes_kernel(const float* __restrict__ means, int P, uint32_t* __restrict__ codes)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
-    codes[i] = morton3d_unit(means[3 * i], means[3 * i + 1], means[3 * i + 2], false) >> (63 - kGaussianCodeBits);
-    idx[i] = (uint32_t)i;
+    codes[i] = (uint32_t)(morton3d_unit(means[3 * i], means[3 * i + 1], means[3 * i + 2], false) >>
+                          (63 - kGaussianCodeBits));
 }
 
 // inv_perm[perm[i]] = i: the record (BVH) index of each caller's Gaussian.
@@ -343,7 +336,7 @@ __device__ __forceinline__ int leaf_first(const BatchView& b, int L, int j, int&
     return g0;
 }
 
-__global__ void leaf_codes_kernel(const uint64_t* __restrict__ sorted_codes, BatchView b, int NL, int L,
+__global__ void leaf_codes_kernel(const uint32_t* __restrict__ sorted_codes, BatchView b, int NL, int L,
                                   uint64_t* __restrict__ leaf_codes)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2006,7 +1999,8 @@ struct SampleSetLayout {
 struct ForwardLayout {
     int* table;  // the state's own one-volume table
     GRecord* rec;
-    uint64_t *codes, *codes_alt, *leaf_codes;
+    uint32_t *codes, *codes_alt;
+    uint64_t* leaf_codes;
     uint32_t *idx, *idx_alt, *vkeys, *pos, *sort_scratch;
     uint32_t* inv_perm;  // caller's Gaussian index -> record (BVH) index
     BNode* nodes;
@@ -2028,8 +2022,8 @@ struct ForwardLayout {
         NG = b.NG;
         table = a.take<int>(batch_table_ints(1));
         rec = a.take<GRecord>((size_t)P);
-        codes = a.take<uint64_t>((size_t)P);
-        codes_alt = a.take<uint64_t>((size_t)P);
+        codes = a.take<uint32_t>((size_t)P);
+        codes_alt = a.take<uint32_t>((size_t)P);
         idx = a.take<uint32_t>((size_t)P);
         idx_alt = a.take<uint32_t>((size_t)P);
         vkeys = a.take<uint32_t>((size_t)P);
@@ -2195,26 +2189,26 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
     const int fp64 = (prm->flags & GVC_FLAG_FP64_COND) ? 1 : 0;
     if (stages & GVC_STAGE_BUILD) {
         // 1. Morton order of the Gaussians (within each volume)
-        gaussian_codes_kernel<<<blocks_for(P, 256), 256, 0, st>>>(means, P, l.codes, l.idx);
+        gaussian_codes_kernel<<<blocks_for(P, 256), 256, 0, st>>>(means, P, l.codes);
         GVC_LAUNCH_CHECK("gaussian_codes");
         bool alt = false;
-        GVC_TRY(radix_sort_u64(l.codes, l.idx, l.codes_alt, l.idx_alt, (size_t)P, kGaussianCodeBits, l.sort_scratch,
-                               &alt, st));
-        const uint64_t* scodes = alt ? l.codes_alt : l.codes;
+        GVC_TRY(radix_sort_u32(l.codes, l.idx, l.codes_alt, l.idx_alt, (size_t)P, kGaussianCodeBits, l.sort_scratch,
+                               &alt, st, nullptr, true));
+        const uint32_t* scodes = alt ? l.codes_alt : l.codes;
         const uint32_t* perm = alt ? l.idx_alt : l.idx;
         if (bt.V > 1) {
             // stable second key: the volume (codes2 and perm2 are free here)
-            uint64_t* codes2 = alt ? l.codes : l.codes_alt;
+            uint32_t* codes2 = alt ? l.codes : l.codes_alt;
             uint32_t* perm2 = alt ? l.idx : l.idx_alt;
             volume_keys_kernel<<<blocks_for(P, 256), 256, 0, st>>>(perm, P, bv.P_off, bt.V, l.vkeys, l.pos);
             GVC_LAUNCH_CHECK("volume_keys");
             bool alt2 = false;
-            uint32_t* spare = reinterpret_cast<uint32_t*>(codes2);
+            uint32_t* spare = codes2;
             GVC_TRY(radix_sort_u32(l.vkeys, l.pos, perm2, spare, (size_t)P, key_bits_for((uint64_t)bt.V),
                                    l.sort_scratch, &alt2, st));
             if (alt2) GVC_HIP(hipMemcpyAsync(l.pos, spare, (size_t)P * 4, hipMemcpyDeviceToDevice, st));
             gather_u32_kernel<<<blocks_for(P, 256), 256, 0, st>>>(perm, l.pos, P, perm2);
-            gather_u64_kernel<<<blocks_for(P, 256), 256, 0, st>>>(scodes, l.pos, P, codes2);
+            gather_u32_kernel<<<blocks_for(P, 256), 256, 0, st>>>(scodes, l.pos, P, codes2);
             GVC_LAUNCH_CHECK("gather");
             scodes = codes2;
             perm = perm2;
