@@ -74,6 +74,8 @@ def parse():
                          "evaluation of the 1M-Gaussian mesh model on a jittered 128^3 grid (gvc.evaluation)")
     ap.add_argument("--plain-steps", type=int, default=3, help="batch32: timed steps of the plain sequence")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--fp64-ratio", type=float, default=0.0,
+                    help="precision: scale ratio above which Gaussians take the FP64 path (0 = 1000, cond 1e6)")
     ap.add_argument("--leaf", type=int, default=None)
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -156,7 +158,8 @@ def main():
     if args.leaf or args.block or args.recompute or args.config == "precision":
         params = mx.MixtureParams(leaf_size=args.leaf or params.leaf_size,
                                   block_threads=args.block or params.block_threads,
-                                  candidate_log=not args.recompute, fp64_cond=args.config == "precision")
+                                  candidate_log=not args.recompute, fp64_cond=args.config == "precision",
+                                  fp64_scale_ratio=args.fp64_ratio if args.config == "precision" else 0.0)
         mx.set_default_params(params)
 
     study = precision_study(args, W, dev) if args.config == "precision" else None
@@ -315,7 +318,9 @@ def main():
         "config": {"workload": workload_name, "gaussians": P, "points_per_step": S,
                    "leaf_size": params.leaf_size, "block_threads": params.block_threads,
                    "run_points": params.run_points, "kahan": params.kahan, "candidate_log": params.candidate_log,
-                   "fp64_cond": params.fp64_cond, "graph": bool(args.graph), "parallelism": "single GPU"},
+                   "fp64_cond": params.fp64_cond,
+                   "fp64_scale_ratio": (params.fp64_scale_ratio or 1000.0) if params.fp64_cond else None,
+                   "graph": bool(args.graph), "parallelism": "single GPU"},
         "pairs_per_step": pairs_per_step,
         "records_per_step": total_records / args.steps,
         "groups_walked_per_lane": lane_groups,
@@ -378,12 +383,15 @@ def main():
 
 
 def precision_study(args, W, dev, n_points=65536, n_stress=16384):
-    """Field error of the three accumulation / inversion modes against the FP64
-    oracle (FP64 conics, the mode's own canonical predicate) at the workload's
-    initial parameters, on two fixed point sets: n_points uniform points, and
-    n_stress points drawn inside the boxes of the Gaussians whose covariance takes
-    the FP64 path (scale ratio above 1000, cond(Sigma) > 1e6), where the fp32
-    inverse is least accurate."""
+    """Field error of the accumulation / inversion modes against the FP64 oracle
+    (FP64 conics, the mode's own canonical predicate) at the workload's initial
+    parameters, on two fixed point sets: n_points uniform points, and n_stress
+    points drawn inside the boxes of the Gaussians whose covariance takes the FP64
+    path at the default threshold (scale ratio above 1000, cond(Sigma) > 1e6),
+    where the fp32 inverse is least accurate.  Modes: fp32 conic form with naive
+    or Kahan sums; Kahan with the FP64 path (FP64 inverse, pairs evaluated from
+    its Cholesky factor in compensated fp32) above scale ratio 1000 (the default)
+    and above ratio 10."""
     import numpy as np
     import torch
 
@@ -398,19 +406,22 @@ def precision_study(args, W, dev, n_points=65536, n_stress=16384):
         m = W["means"]
     mh, sh, rh, wh, vh = (t.detach().cpu().numpy() for t in (m, sc, rq, wv, vv))
     cd = oracle.conic_fp64(sh, rh)
-    pre = {fp64: oracle.preprocess(mh, sh, rh, wh, 1.0, fp64) for fp64 in (False, True)}
-    ill = np.asarray(pre[True][2]).reshape(-1).astype(bool)
+    modes = {"fp32_naive": mx.MixtureParams(kahan=False), "fp32_kahan": mx.MixtureParams(kahan=True),
+             "fp32_kahan_fp64_cond": mx.MixtureParams(kahan=True, fp64_cond=True),
+             "fp32_kahan_fp64_ratio10": mx.MixtureParams(kahan=True, fp64_cond=True, fp64_scale_ratio=10.0)}
+    pre = {name: oracle.preprocess(mh, sh, rh, wh, 1.0, prm.fp64_cond, prm.fp64_scale_ratio, factor=True)
+           for name, prm in modes.items()}
+    ill = np.asarray(pre["fp32_kahan_fp64_cond"][2]).reshape(-1).astype(bool)
     rng = np.random.default_rng(args.seed + 12)
     gen = torch.Generator(device=dev)
     gen.manual_seed(args.seed + 11)
     uniform = torch.rand((n_points, 3), device=dev, generator=gen)
-    box_ill = pre[True][1][ill]
+    box_ill = pre["fp32_kahan_fp64_cond"][1][ill]
     pick = rng.integers(0, max(int(ill.sum()), 1), n_stress)
     stress = (box_ill[pick, :3] + rng.uniform(0, 1, (n_stress, 3)) * (box_ill[pick, 3:] - box_ill[pick, :3])
               if ill.any() else rng.uniform(0, 1, (n_stress, 3))).astype(np.float32)
-    modes = {"fp32_naive": mx.MixtureParams(kahan=False), "fp32_kahan": mx.MixtureParams(kahan=True),
-             "fp32_kahan_fp64_cond": mx.MixtureParams(kahan=True, fp64_cond=True)}
     out = {"gaussians": int(mh.shape[0]), "fp64_path_gaussians": int(ill.sum()),
+           "fp64_path_gaussians_ratio10": int(np.asarray(pre["fp32_kahan_fp64_ratio10"][2]).sum()),
            "oracle": "FP64 sums with FP64 conics, the mode's own (bit-exact) predicate", "sets": {}}
     for set_name, pts_t in (("uniform", uniform), ("inside_fp64_path_boxes", torch.tensor(stress, device=dev))):
         pts = pts_t.cpu().numpy()
@@ -420,10 +431,11 @@ def precision_study(args, W, dev, n_points=65536, n_stress=16384):
             for name, prm in modes.items():
                 cells, cw, _, st = mx.mixture_forward(m, sc, rq, vv, wv, 1.0, ss, prm, want_conics=False)
                 gpu[name] = (cells.cpu().numpy(), cw.cpu().numpy(), st.stats()[1])
-        ref = {fp64: oracle.forward(mh, vh, wh, pre[fp64][0], pre[fp64][1], pts, cd) for fp64 in (False, True)}
+        ref = {name: oracle.forward(mh, vh, wh, pre[name][0], pre[name][1], pts, cd,
+                                    factor=pre[name][3] if modes[name].fp64_cond else None) for name in modes}
         res = {"points": int(pts.shape[0]), "modes": {}}
         for name, (V, Wg, pairs) in gpu.items():
-            Vr, Wr, cnt = ref[name.endswith("fp64_cond")]
+            Vr, Wr, cnt = ref[name]
             ok = (Wr > 0) & (Wg > 0)
             err = np.maximum(np.abs(V - Vr) / np.maximum(np.abs(Vr), 1e-3),
                              np.abs(Wg - Wr) / np.maximum(np.abs(Wr), 1e-3))

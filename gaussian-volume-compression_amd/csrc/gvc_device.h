@@ -14,7 +14,8 @@
 namespace gvc {
 
 constexpr double kWeightCutoff = 1e-2;  // CR/config.h:9
-constexpr float kCondRatio = 1000.0f;   // cond(Sigma) = (s_max/s_min)^2 > 1e6
+constexpr float kCondRatio = 1000.0f;   // default FP64 threshold: cond(Sigma) = (s_max/s_min)^2 > 1e6
+constexpr uint32_t kPreciseBit = 0x80000000u;  // GRecord::id bit: the Gaussian is on the precise path
 
 // Gaussian record, 80 bytes, stored in BVH (Morton) order.  One scalar
 // s_load_dwordx16 + s_load_dwordx4 fetches it for a wave-uniform candidate.
@@ -22,7 +23,7 @@ struct alignas(16) GRecord {
     float mx, my, mz, w;       // mean, weight
     float c0, c1, c2, c3;      // conic xx xy xz yy
     float c4, c5, v;           // conic yz zz, value
-    int id;                    // original Gaussian index
+    int id;                    // original Gaussian index | kPreciseBit on the precise path
     float lox, loy, loz, hix;  // box
     float hiy, hiz;
     float pad0, pad1;
@@ -115,24 +116,54 @@ __device__ __forceinline__ void rotation_d(float r_, float x_, float y_, float z
     R[2][2] = 1.0 - 2.0 * (x * x + y * y);
 }
 
-__device__ __forceinline__ bool ill_conditioned(float s0, float s1, float s2, float k)
+__device__ __forceinline__ bool ill_conditioned(float s0, float s1, float s2, float k, float ratio)
 {
     float a = fabsf(k * s0), b = fabsf(k * s1), c = fabsf(k * s2);
     float smax = fmaxf(fmaxf(a, b), c);
     float smin = fminf(fminf(a, b), c);
-    return smax > kCondRatio * smin;
+    return smax > ratio * smin;
+}
+
+// Upper Cholesky factor U of an FP64 conic (packed xx xy xz yy yz zz), conic =
+// U^T U, as hi + lo float pairs (u00 u01 u02 u11 u12 u22).  False when the conic
+// is not numerically positive definite (the Gaussian then keeps its conic).
+__device__ __forceinline__ bool conic_factor(const double C[6], float hi[6], float lo[6])
+{
+    if (!(C[0] > 0.0)) return false;
+    const double u00 = __builtin_sqrt(C[0]);
+    const double u01 = C[1] / u00, u02 = C[2] / u00;
+    const double a1 = C[3] - u01 * u01;
+    if (!(a1 > 0.0)) return false;
+    const double u11 = __builtin_sqrt(a1);
+    const double u12 = (C[4] - u01 * u02) / u11;
+    const double a2 = (C[5] - u02 * u02) - u12 * u12;
+    if (!(a2 > 0.0)) return false;
+    const double u[6] = { u00, u01, u02, u11, u12, __builtin_sqrt(a2) };
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        hi[k] = (float)u[k];
+        if (!(fabsf(hi[k]) <= 3.4e38f)) return false;
+        lo[k] = (float)(u[k] - (double)hi[k]);
+    }
+    return true;
 }
 
 // CR/forward.cu:10-114 with the canonical operation order.  conic = packed
 // inverse of Sigma + eps I (xx xy xz yy yz zz); box = lo.xyz hi.xyz.
-__device__ __forceinline__ void preprocess_one(const float mu[3], const float sc[3], const float q[4],
-                                               float w, float k, bool fp64_cond, float conic[6],
-                                               float box[6])
+// fp64_ratio > 0: Gaussians whose scale ratio exceeds it (cond(Sigma) above its
+// square) take the FP64 covariance inverse; with fac_hi/fac_lo given, such a
+// Gaussian also gets the Cholesky factor of that FP64 conic and returns true
+// (the precise path: its pairs are evaluated from the factor, precise_power).
+__device__ __forceinline__ bool preprocess_one(const float mu[3], const float sc[3], const float q[4],
+                                               float w, float k, float fp64_ratio, float conic[6],
+                                               float box[6], float* fac_hi = nullptr,
+                                               float* fac_lo = nullptr)
 {
     float R[3][3];
     rotation(q[0], q[1], q[2], q[3], R);
     float s[3] = { k * sc[0], k * sc[1], k * sc[2] };
-    bool ill = fp64_cond && ill_conditioned(sc[0], sc[1], sc[2], k);
+    bool ill = fp64_ratio > 0.f && ill_conditioned(sc[0], sc[1], sc[2], k, fp64_ratio);
+    bool precise = false;
     if (!ill) {
         float M[3][3];
 #pragma unroll
@@ -184,12 +215,10 @@ __device__ __forceinline__ void preprocess_one(const float mu[3], const float sc
         double A00 = d * f - e * e, A01 = c * e - b * f, A02 = b * e - c * d;
         double A11 = a * f - c * c, A12 = b * c - a * e, A22 = a * d - b * b;
         double det = a * A00 + b * A01 + c * A02;
-        conic[0] = (float)(A00 / det);
-        conic[1] = (float)(A01 / det);
-        conic[2] = (float)(A02 / det);
-        conic[3] = (float)(A11 / det);
-        conic[4] = (float)(A12 / det);
-        conic[5] = (float)(A22 / det);
+        const double cd[6] = { A00 / det, A01 / det, A02 / det, A11 / det, A12 / det, A22 / det };
+#pragma unroll
+        for (int t = 0; t < 6; ++t) conic[t] = (float)cd[t];
+        if (fac_hi) precise = conic_factor(cd, fac_hi, fac_lo);
     }
     float m = box_radius(w);
     float sS[3] = { s[0] * m, s[1] * m, s[2] * m };
@@ -204,6 +233,7 @@ __device__ __forceinline__ void preprocess_one(const float mu[3], const float sc
         box[c] = fminf(p, lo);
         box[3 + c] = fmaxf(p, hi);
     }
+    return precise;
 }
 
 // Adjoint of the covariance chain in double: from the gradient of the packed
@@ -211,9 +241,13 @@ __device__ __forceinline__ void preprocess_one(const float mu[3], const float sc
 // (r x y z).  The forward it differentiates: M = diag(k s) R^T, Sigma = M^T M,
 // conic = (Sigma + eps I)^-1 with eps = 1e-5 max_i |Sigma_ii| (CR/forward.cu:26-75;
 // the eps term's own derivative is kept on the largest diagonal entry).
+// yframe: g6 is the gradient with respect to the factor-frame outer products
+// (sum of dquad y y^T, y = U d, conic = U^T U: the precise path's backward), and
+// dL/dSigma = -C G C is formed as -U^T G_y U (exact algebra with G = U^-1 G_y U^-T;
+// no cancellation against the conic's large entries).
 __device__ __forceinline__ void covariance_chain(const double* __restrict__ g6, const float* __restrict__ sc,
                                                  const float* __restrict__ q, float kmod, double ds[3],
-                                                 double dq[4])
+                                                 double dq[4], bool yframe = false)
 {
     const float q4[4] = { q[0], q[1], q[2], q[3] };
     double R[3][3];
@@ -243,10 +277,25 @@ __device__ __forceinline__ void covariance_chain(const double* __restrict__ g6, 
                              { 0.5 * g6[1], g6[3], 0.5 * g6[4] },
                              { 0.5 * g6[2], 0.5 * g6[4], g6[5] } };
     double CG[3][3], K[3][3];
-    for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) CG[r][c] = C[r][0] * G[0][c] + C[r][1] * G[1][c] + C[r][2] * G[2][c];
-    for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) K[r][c] = -(CG[r][0] * C[0][c] + CG[r][1] * C[1][c] + CG[r][2] * C[2][c]);
+    if (yframe) {
+        // upper Cholesky factor of C (the preprocess's conic_factor in double)
+        double U[3][3] = { { 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0 } };
+        U[0][0] = __builtin_sqrt(C[0][0]);
+        U[0][1] = C[0][1] / U[0][0];
+        U[0][2] = C[0][2] / U[0][0];
+        U[1][1] = __builtin_sqrt(C[1][1] - U[0][1] * U[0][1]);
+        U[1][2] = (C[1][2] - U[0][1] * U[0][2]) / U[1][1];
+        U[2][2] = __builtin_sqrt((C[2][2] - U[0][2] * U[0][2]) - U[1][2] * U[1][2]);
+        for (int r = 0; r < 3; ++r)   // CG = U^T G_y
+            for (int c = 0; c < 3; ++c) CG[r][c] = U[0][r] * G[0][c] + U[1][r] * G[1][c] + U[2][r] * G[2][c];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) K[r][c] = -(CG[r][0] * U[0][c] + CG[r][1] * U[1][c] + CG[r][2] * U[2][c]);
+    } else {
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) CG[r][c] = C[r][0] * G[0][c] + C[r][1] * G[1][c] + C[r][2] * G[2][c];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) K[r][c] = -(CG[r][0] * C[0][c] + CG[r][1] * C[1][c] + CG[r][2] * C[2][c]);
+    }
     const double trH = K[0][0] + K[1][1] + K[2][2];
     const double sg = Sg[jx][jx] > 0 ? 1.0 : (Sg[jx][jx] < 0 ? -1.0 : 0.0);
     K[jx][jx] += trH * 1e-5 * sg;
@@ -283,6 +332,100 @@ __device__ __forceinline__ float pair_power(float c0, float c1, float c2, float 
 }
 
 __device__ __forceinline__ bool in_window(float power) { return !(power < -14.0f || power > 0.0f); }
+
+// Knuth's TwoSum: s + e == a + b exactly (round to nearest, no contraction).
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
+{
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// (s, c) += (uh + ul)(dh + dl): uh dh exactly (product and its fma residual, the
+// product added by TwoSum), the cross terms uh dl + ul dh rounded once, ul dl dropped.
+__device__ __forceinline__ void ff_madd(float& s, float& c, float uh, float ul, float dh, float dl)
+{
+    const float p = uh * dh;
+    const float pe = __builtin_fmaf(uh, dh, -p);
+    float t, e;
+    two_sum(s, p, t, e);
+    s = t;
+    c = c + (e + (pe + __builtin_fmaf(uh, dl, ul * dh)));
+}
+
+// Power of a pair on the precise path.  The conic is U^T U with U upper
+// triangular (u00 u01 u02 u11 u12 u22 = h*, each entry h + l); d = x - mu exactly
+// as dh + dl; y = U d in float-float arithmetic, each component rounded once;
+// power = -0.5 |y|^2.  The fp32 conic form d^T C d cancels against entries up to
+// cond times larger than its value; here the cancellation is carried exactly.
+// y feeds precise_conic_d.  Window and use as for pair_power.
+__device__ __forceinline__ float precise_power(float px, float py, float pz, float mx, float my, float mz,
+                                               float h0, float h1, float h2, float h3, float h4, float h5,
+                                               const float4& l0123, const float4& l45, float& y0, float& y1,
+                                               float& y2)
+{
+    float xh, xl, yh, yl, zh, zl;
+    two_sum(px, -mx, xh, xl);
+    two_sum(py, -my, yh, yl);
+    two_sum(pz, -mz, zh, zl);
+    float s = 0.f, c = 0.f;
+    ff_madd(s, c, h0, l0123.x, xh, xl);
+    ff_madd(s, c, h1, l0123.y, yh, yl);
+    ff_madd(s, c, h2, l0123.z, zh, zl);
+    y0 = s + c;
+    s = 0.f; c = 0.f;
+    ff_madd(s, c, h3, l0123.w, yh, yl);
+    ff_madd(s, c, h4, l45.x, zh, zl);
+    y1 = s + c;
+    s = 0.f; c = 0.f;
+    ff_madd(s, c, h5, l45.y, zh, zl);
+    y2 = s + c;
+    const float quad = __builtin_fmaf(y2, y2, __builtin_fmaf(y1, y1, y0 * y0));
+    return -0.5f * quad;
+}
+
+// conic d = U^T y on the precise path (the mean's gradient term of pair_power's r).
+__device__ __forceinline__ void precise_conic_d(float h0, float h1, float h2, float h3, float h4, float h5,
+                                                float y0, float y1, float y2, float& r0, float& r1, float& r2)
+{
+    r0 = h0 * y0;
+    r1 = __builtin_fmaf(h3, y1, h1 * y0);
+    r2 = __builtin_fmaf(h5, y2, __builtin_fmaf(h4, y1, h2 * y0));
+}
+
+// Pair evaluation of a record's candidate (A = mean|w, B = c0..c3, C = c4 c5 v id):
+// the canonical fp32 form, or with PRECISE and the id's precise bit set, the
+// factor form (hi parts in the conic slots, lo parts in fac_lo[2 j], [2 j + 1] for
+// the Gaussian's caller index j, the id without kPreciseBit).
+// r = conic d, and (dx, dy, dz) = the vector whose outer products the adjoint sums
+// for the conic's gradient: d = x - mu (fp32) on the conic form, y = U d on the
+// precise path (the chain then takes that gradient in the factor frame).
+template <bool PRECISE>
+__device__ __forceinline__ float record_power(const float4& A, const float4& B, const float4& C,
+                                              const float4* __restrict__ fac_lo, float px, float py,
+                                              float pz, float& dx, float& dy, float& dz, float& r0, float& r1,
+                                              float& r2)
+{
+    dx = px - A.x;
+    dy = py - A.y;
+    dz = pz - A.z;
+    if constexpr (PRECISE) {
+        const int id = __builtin_bit_cast(int, C.w);
+        if (id < 0) {
+            const size_t j = (size_t)(uint32_t)(id & 0x7fffffff);
+            const float4 l0 = fac_lo[2 * j], l1 = fac_lo[2 * j + 1];
+            float y0, y1, y2;
+            const float pw = precise_power(px, py, pz, A.x, A.y, A.z, B.x, B.y, B.z, B.w, C.x, C.y, l0, l1, y0,
+                                           y1, y2);
+            precise_conic_d(B.x, B.y, B.z, B.w, C.x, C.y, y0, y1, y2, r0, r1, r2);
+            dx = y0;
+            dy = y1;
+            dz = y2;
+            return pw;
+        }
+    }
+    return pair_power(B.x, B.y, B.z, B.w, C.x, C.y, dx, dy, dz, r0, r1, r2);
+}
 
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  For a
 // grid over spatially sorted data, map block b to an index such that every XCD

@@ -32,6 +32,7 @@
 #include <math.h>
 
 #include <vector>
+#include <type_traits>
 
 #include "gvc_device.h"
 #include "gvc_internal.h"
@@ -145,7 +146,7 @@ __global__ void morton63_kernel(const float* __restrict__ xyz, int n, int clamp,
 
 __global__ void preprocess_kernel(int P, const float* __restrict__ means, const float* __restrict__ scales,
                                   const float* __restrict__ rots, const float* __restrict__ weights, float k,
-                                  int fp64_cond, float* __restrict__ conics, float* __restrict__ boxes)
+                                  float fp64_ratio, float* __restrict__ conics, float* __restrict__ boxes)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
@@ -153,7 +154,7 @@ __global__ void preprocess_kernel(int P, const float* __restrict__ means, const 
     float sc[3] = { scales[3 * i], scales[3 * i + 1], scales[3 * i + 2] };
     float q[4] = { rots[4 * i], rots[4 * i + 1], rots[4 * i + 2], rots[4 * i + 3] };
     float cn[6], bx[6];
-    preprocess_one(mu, sc, q, weights[i], k, fp64_cond != 0, cn, bx);
+    preprocess_one(mu, sc, q, weights[i], k, fp64_ratio, cn, bx);
     if (conics)
         for (int c = 0; c < 6; ++c) conics[6 * i + c] = cn[c];
     if (boxes)
@@ -266,13 +267,18 @@ __global__ __launch_bounds__(256) void inv_perm_kernel(int P, const uint32_t* __
 }
 
 // Canonical preprocess, in the caller's order (coalesced parameter reads and
-// conic writes), written as the record of BVH index inv_perm[j].
+// conic writes), written as the record of BVH index inv_perm[j].  With fac_lo
+// (FP64 mode), a Gaussian on the precise path carries the hi parts of its conic's
+// Cholesky factor in the conic slots and kPreciseBit in its id; the lo parts go
+// to fac_lo[2j], [2j + 1], j = the caller's index (every Gaussian's pair is
+// written: no stale entries).
 __global__ __launch_bounds__(256) void record_kernel(int P, const uint32_t* __restrict__ inv_perm,
                                                      const float* __restrict__ means,
                                                      const float* __restrict__ scales, const float* __restrict__ rots,
                                                      const float* __restrict__ values,
-                                                     const float* __restrict__ weights, float k, int fp64_cond,
-                                                     GRecord* __restrict__ rec, float* __restrict__ conics_out)
+                                                     const float* __restrict__ weights, float k, float fp64_ratio,
+                                                     GRecord* __restrict__ rec, float4* __restrict__ fac_lo,
+                                                     float* __restrict__ conics_out)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= P) return;
@@ -281,17 +287,22 @@ __global__ __launch_bounds__(256) void record_kernel(int P, const uint32_t* __re
     float sc[3] = { scales[3 * j], scales[3 * j + 1], scales[3 * j + 2] };
     float q[4] = { rots[4 * j], rots[4 * j + 1], rots[4 * j + 2], rots[4 * j + 3] };
     const float w = weights[j];
-    float cn[6], bx[6];
-    preprocess_one(mu, sc, q, w, k, fp64_cond != 0, cn, bx);
+    float cn[6], bx[6], fh[6], fl[6];
+    const bool precise = preprocess_one(mu, sc, q, w, k, fp64_ratio, cn, bx, fac_lo ? fh : nullptr, fl);
+    const float* cr = precise ? fh : cn;
     GRecord r;
     r.mx = mu[0]; r.my = mu[1]; r.mz = mu[2]; r.w = w;
-    r.c0 = cn[0]; r.c1 = cn[1]; r.c2 = cn[2]; r.c3 = cn[3]; r.c4 = cn[4]; r.c5 = cn[5];
+    r.c0 = cr[0]; r.c1 = cr[1]; r.c2 = cr[2]; r.c3 = cr[3]; r.c4 = cr[4]; r.c5 = cr[5];
     r.v = values[j];
-    r.id = j;
+    r.id = (int)((uint32_t)j | (precise ? kPreciseBit : 0u));
     r.lox = bx[0]; r.loy = bx[1]; r.loz = bx[2];
     r.hix = bx[3]; r.hiy = bx[4]; r.hiz = bx[5];
     r.pad0 = 0.f; r.pad1 = 0.f;
     rec[i] = r;
+    if (fac_lo) {
+        fac_lo[2 * (size_t)j] = precise ? make_float4(fl[0], fl[1], fl[2], fl[3]) : make_float4(0.f, 0.f, 0.f, 0.f);
+        fac_lo[2 * (size_t)j + 1] = precise ? make_float4(fl[4], fl[5], 0.f, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     if (conics_out)
         for (int c = 0; c < 6; ++c) conics_out[6 * j + c] = cn[c];
 }
@@ -934,10 +945,10 @@ __global__ __launch_bounds__(256) void group_cost_kernel(const float4* __restric
 // dx, dy, dz): hit = Gaussian gi's box holds the point and the power is inside
 // the window (the predicate of the shared walk).  Returns false if a lane's
 // stack overflowed (that lane's walk is then incomplete).
-template <class Visit>
-__device__ __forceinline__ bool lane_walk(const GRecord* __restrict__ rec, const BNode* __restrict__ nodes,
-                                          const TreeView& tv, int L, float px, float py, float pz, bool active,
-                                          Visit&& visit)
+template <bool PRECISE, class Visit>
+__device__ __forceinline__ bool lane_walk(const GRecord* __restrict__ rec, const float4* __restrict__ fac_lo,
+                                          const BNode* __restrict__ nodes, const TreeView& tv, int L, float px,
+                                          float py, float pz, bool active, Visit&& visit)
 {
     int stk[kLaneStack];
     int sp = 0, g = 0, gend = 0;
@@ -958,10 +969,7 @@ __device__ __forceinline__ bool lane_walk(const GRecord* __restrict__ rec, const
                 A = q[0];
                 B = q[1];
                 C = q[2];
-                dx = px - A.x;
-                dy = py - A.y;
-                dz = pz - A.z;
-                power = pair_power(B.x, B.y, B.z, B.w, C.x, C.y, dx, dy, dz, r0, r1, r2);
+                power = record_power<PRECISE>(A, B, C, fac_lo, px, py, pz, dx, dy, dz, r0, r1, r2);
                 hit = in_window(power);
             }
         } else if (sp > 0) {
@@ -1158,8 +1166,10 @@ __device__ __forceinline__ uint32_t wave_or32(uint32_t v)
     return v;
 }
 
-template <bool KAHAN>
-__global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict__ rec, const BNode* __restrict__ nodes,
+template <bool KAHAN, bool PRECISE>
+__global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict__ rec,
+                                                      const float4* __restrict__ fac_lo,
+                                                      const BNode* __restrict__ nodes,
                                                       const float* __restrict__ root_box, BatchView bv, int P, int L,
                                                       const float4* __restrict__ pts, int NG,
                                                       float* __restrict__ out_cells, float* __restrict__ out_weights,
@@ -1246,9 +1256,8 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
             // Branch-free: a rejected pair (or the missing second one) adds zero,
             // which leaves the compensated sum W - cW unchanged.
             auto pair = [&](const float4& A, const float4& B, const float4& C, bool live) {
-                const float dx = px - A.x, dy = py - A.y, dz = pz - A.z;
-                float r0, r1, r2;
-                const float power = pair_power(B.x, B.y, B.z, B.w, C.x, C.y, dx, dy, dz, r0, r1, r2);
+                float dx, dy, dz, r0, r1, r2;
+                const float power = record_power<PRECISE>(A, B, C, fac_lo, px, py, pz, dx, dy, dz, r0, r1, r2);
                 const bool in = live && in_window(power);
                 const float e = __builtin_amdgcn_exp2f(power * kLog2e);
                 const float wgt = in ? A.w * e : 0.0f;
@@ -1284,7 +1293,7 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
         const bool lane_mode = lane_mode_group(tv, px, py, pz, nv, starts, reach, lane_policy);
         if (lane_mode) {
             const bool active = orig >= 0 && px == px && py == py && pz == pz;
-            ok = lane_walk(rec, nodes, tv, L, px, py, pz, active,
+            ok = lane_walk<PRECISE>(rec, fac_lo, nodes, tv, L, px, py, pz, active,
                            [&](bool hit, int, const float4& A, const float4&, const float4& C, float power, float,
                                float, float, float, float, float) {
                                if (!hit) return;
@@ -1357,7 +1366,7 @@ struct alignas(64) PartialRecord {
     int gid;           // Gaussian index in BVH order
     float dv, dw;      // d/dvalue, d/dweight
     float dmx, dmy, dmz;
-    float dc[6];       // d/dconic, packed, off-diagonals counted once each
+    float dc[6];       // d/dconic, packed, off-diagonals counted once each (factor frame on the precise path)
     float pad[4];
 };
 static_assert(sizeof(PartialRecord) == 64, "PartialRecord layout");
@@ -1418,9 +1427,9 @@ __device__ __forceinline__ uint64_t drop_low_bits(uint64_t m, uint32_t k)
 // REPLAY = true: candidates and box hits come from the forward's candidate log.
 // REPLAY = false: the group is walked again (runs, tree, box tests), as the
 // forward did, so nothing but per-group counts is kept between the passes.
-template <bool REPLAY>
+template <bool REPLAY, bool PRECISE>
 __global__ __launch_bounds__(1024) void backward_kernel(
-    const GRecord* __restrict__ rec, const BNode* __restrict__ nodes, const float* __restrict__ root_box, BatchView bv,
+    const GRecord* __restrict__ rec, const float4* __restrict__ fac_lo, const BNode* __restrict__ nodes, const float* __restrict__ root_box, BatchView bv,
     int P, int L, const float4* __restrict__ pts, int NG, const float* __restrict__ out_cells,
     const float* __restrict__ out_weights, const float* __restrict__ gcells, const float* __restrict__ gweights,
     const uint32_t* __restrict__ group_offsets, uint64_t capacity, PartialRecord* __restrict__ out_rec,
@@ -1510,8 +1519,7 @@ __global__ __launch_bounds__(1024) void backward_kernel(
                     const uint32_t mlo_j = (uint32_t)__shfl((int)(uint32_t)m, j, 64);
                     const uint32_t mhi_j = (uint32_t)__shfl((int)(uint32_t)(m >> 32), j, 64);
                     const float4 ca = s.cand[j][0], cb = s.cand[j][1], cc = s.cand[j][2];
-                    const float cmx = ca.x, cmy = ca.y, cmz = ca.z, cw = ca.w;
-                    const float k0 = cb.x, k1 = cb.y, k2 = cb.z, k3 = cb.w, k4 = cc.x, k5 = cc.y, cv = cc.z;
+                    const float cw = ca.w, cv = cc.z;
                     uint64_t mj = live ? drop_low_bits(((uint64_t)mhi_j << 32) | mlo_j, (v - tb) * q) : 0ull;
                     float sv = 0.f, sw = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
                     float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f;
@@ -1520,9 +1528,9 @@ __global__ __launch_bounds__(1024) void backward_kernel(
                         mj &= mj - 1;
                         const float4 a = s.ptsA[p];
                         const float2 b = s.ptsB[p];
-                        const float dx = a.x - cmx, dy = a.y - cmy, dz = a.z - cmz;
-                        float r0_, r1_, r2_;
-                        const float power = pair_power(k0, k1, k2, k3, k4, k5, dx, dy, dz, r0_, r1_, r2_);
+                        float dx, dy, dz, r0_, r1_, r2_;
+                        const float power = record_power<PRECISE>(ca, cb, cc, fac_lo, a.x, a.y, a.z, dx, dy, dz,
+                                                                  r0_, r1_, r2_);
                         if (in_window(power)) {
                             const float e = __builtin_amdgcn_exp2f(power * kLog2e);
                             const float wgt = cw * e;
@@ -1583,7 +1591,7 @@ __global__ __launch_bounds__(1024) void backward_kernel(
         if (lane_mode) {
             // one record per accepted pair, in step order then lane order
             const bool active = orig >= 0 && pme.x == pme.x && pme.y == pme.y && pme.z == pme.z;
-            ok = lane_walk(rec, nodes, tv, L, pme.x, pme.y, pme.z, active,
+            ok = lane_walk<PRECISE>(rec, fac_lo, nodes, tv, L, pme.x, pme.y, pme.z, active,
                            [&](bool hit, int gi, const float4& A, const float4&, const float4& C, float power,
                                float r0_, float r1_, float r2_, float dx, float dy, float dz) {
                                const uint64_t bal = __ballot(hit);
@@ -1820,7 +1828,10 @@ __global__ __launch_bounds__(256) void reduce_long_kernel(const uint32_t* __rest
 // and mean, and the covariance chain (conic gradient -> Sigma -> M -> scales,
 // rotations) in double from the float inputs.  A zero conic gradient (no
 // records) gives zero scale and rotation gradients without the chain.
+// rec (FP64 mode only, else null): a record on the precise path has its conic
+// gradient summed in the factor frame.
 __global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __restrict__ inv_perm,
+                                                    const GRecord* __restrict__ rec,
                                                     const double* __restrict__ sums, const float* __restrict__ scales,
                                                     const float* __restrict__ rots, float kmod,
                                                     float* __restrict__ dmeans, float* __restrict__ dscales,
@@ -1844,7 +1855,8 @@ __global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __res
     dmeans[3 * id + 2] = (float)m[4];
     double ds[3] = { 0.0, 0.0, 0.0 }, dq[4] = { 0.0, 0.0, 0.0, 0.0 };
     const bool any = m[5] != 0.0 || m[6] != 0.0 || m[7] != 0.0 || m[8] != 0.0 || m[9] != 0.0 || m[10] != 0.0;
-    if (any) covariance_chain(m + 5, scales + 3 * id, rots + 4 * id, kmod, ds, dq);
+    const bool yframe = rec && rec[inv_perm[id]].id < 0;
+    if (any) covariance_chain(m + 5, scales + 3 * id, rots + 4 * id, kmod, ds, dq, yframe);
     for (int c = 0; c < 3; ++c) dscales[3 * id + c] = (float)ds[c];
     for (int c = 0; c < 4; ++c) drots[4 * id + c] = (float)dq[c];
 }
@@ -1999,6 +2011,7 @@ struct SampleSetLayout {
 struct ForwardLayout {
     int* table;  // the state's own one-volume table
     GRecord* rec;
+    float4* fac_lo;  // FP64 mode: lo parts of the precise-path factors, 2 per Gaussian (caller order)
     uint32_t *codes, *codes_alt;
     uint64_t* leaf_codes;
     uint32_t *idx, *idx_alt, *vkeys, *pos, *sort_scratch;
@@ -2022,6 +2035,7 @@ struct ForwardLayout {
         NG = b.NG;
         table = a.take<int>(batch_table_ints(1));
         rec = a.take<GRecord>((size_t)P);
+        fac_lo = (flags & GVC_FLAG_FP64_COND) ? a.take<float4>(2 * (size_t)P) : nullptr;
         codes = a.take<uint32_t>((size_t)P);
         codes_alt = a.take<uint32_t>((size_t)P);
         idx = a.take<uint32_t>((size_t)P);
@@ -2085,8 +2099,13 @@ int check_params(const gvc_params* prm)
         return set_error(GVC_ERR_INVALID_ARGUMENT, "run_points must be 0, 16, 32 or 64");
     if ((prm->flags & GVC_FLAG_LANE_NEVER) && (prm->flags & GVC_FLAG_LANE_ALWAYS))
         return set_error(GVC_ERR_INVALID_ARGUMENT, "GVC_FLAG_LANE_NEVER and GVC_FLAG_LANE_ALWAYS are exclusive");
+    if (!(prm->fp64_scale_ratio == 0.f || (prm->fp64_scale_ratio >= 1.f && prm->fp64_scale_ratio <= 1e18f)))
+        return set_error(GVC_ERR_INVALID_ARGUMENT, "fp64_scale_ratio must be 0 (default) or in [1, 1e18]");
     return GVC_OK;
 }
+
+// The scale ratio above which a Gaussian takes the FP64 path (0 = the default).
+float fp64_ratio_of(const gvc_params* prm) { return prm->fp64_scale_ratio > 0.f ? prm->fp64_scale_ratio : kCondRatio; }
 
 // The table a call uses: the batch's, or for one volume the buffer's own.
 const void* table_of(const Batch& b, const int* own) { return b.V == 1 && !b.table ? own : b.table; }
@@ -2186,7 +2205,7 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
         GVC_TRY(exclusive_scan_u32(l.group_records, l.group_offsets, (size_t)NG, true, l.scan_scratch, st));
         return GVC_OK;
     }
-    const int fp64 = (prm->flags & GVC_FLAG_FP64_COND) ? 1 : 0;
+    const float fp64_ratio = (prm->flags & GVC_FLAG_FP64_COND) ? fp64_ratio_of(prm) : 0.f;
     if (stages & GVC_STAGE_BUILD) {
         // 1. Morton order of the Gaussians (within each volume)
         gaussian_codes_kernel<<<blocks_for(P, 256), 256, 0, st>>>(means, P, l.codes);
@@ -2215,8 +2234,8 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
         }
         // 2. records
         inv_perm_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, perm, l.inv_perm);
-        record_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, means, scales, rots, values, weights, k, fp64,
-                                                          l.rec, conics);
+        record_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, means, scales, rots, values, weights, k,
+                                                          fp64_ratio, l.rec, l.fac_lo, conics);
         GVC_LAUNCH_CHECK("record");
         extent_kernel<<<bt.V, 256, 0, st>>>(l.rec, bv, l.ext_sum);
         GVC_LAUNCH_CHECK("extent");
@@ -2262,25 +2281,23 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
         const int rp = run_points_of(prm);
         const int lp = lane_policy_of(prm);
         const size_t lds = (size_t)(T / 64) * sizeof(FwdLds);
-        if (prm->flags & GVC_FLAG_KAHAN) {
-            const void* kf = reinterpret_cast<const void*>(&forward_kernel<true>);
+        auto launch = [&](auto kahan, auto precise) -> int {
+            constexpr bool K = decltype(kahan)::value, PR = decltype(precise)::value;
+            const void* kf = reinterpret_cast<const void*>(&forward_kernel<K, PR>);
             GVC_TRY(allow_lds(kf, lds));
             const unsigned nb = walk_blocks(kf, T, lds, NG);
-            forward_kernel<true><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
-                                                     out_weights, l.group_records, l.group_pairs, l.err,
-                                                     l.queues + kQFwd, l.ext_sum, log_on, l.log, l.log_cap,
-                                                     l.queues + kQLog, l.group_head, rp, lp, l.heavy,
-                                                     l.queues + kQHeavyF, l.group_flag);
-        } else {
-            const void* kf = reinterpret_cast<const void*>(&forward_kernel<false>);
-            GVC_TRY(allow_lds(kf, lds));
-            const unsigned nb = walk_blocks(kf, T, lds, NG);
-            forward_kernel<false><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
-                                                      out_weights, l.group_records, l.group_pairs, l.err,
+            forward_kernel<K, PR><<<nb, T, lds, st>>>(l.rec, l.fac_lo, l.nodes, l.root_box, bv, P, L, pts, NG,
+                                                      out_cells, out_weights, l.group_records, l.group_pairs, l.err,
                                                       l.queues + kQFwd, l.ext_sum, log_on, l.log, l.log_cap,
                                                       l.queues + kQLog, l.group_head, rp, lp, l.heavy,
                                                       l.queues + kQHeavyF, l.group_flag);
-        }
+            return GVC_OK;
+        };
+        using T1 = std::true_type;
+        using F0 = std::false_type;
+        const bool kahan = (prm->flags & GVC_FLAG_KAHAN) != 0, precise = l.fac_lo != nullptr;
+        GVC_TRY(kahan ? (precise ? launch(T1{}, T1{}) : launch(T1{}, F0{}))
+                      : (precise ? launch(F0{}, T1{}) : launch(F0{}, F0{})));
         GVC_LAUNCH_CHECK("forward");
     }
     // 6. record offsets for the backward
@@ -2352,27 +2369,24 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
         GVC_TRY(fill_u32(l.queues + kQHeavyB, 0u, kLine, st));
         const int rp = run_points_of(prm);
         const int lp = lane_policy_of(prm);
-        if (prm->flags & GVC_FLAG_CANDIDATE_LOG) {
-            const void* kf = reinterpret_cast<const void*>(&backward_kernel<true>);
-            const size_t lds = (size_t)(T / 64) * sizeof(BwdLds<true>);
+        auto launch = [&](auto replay, auto precise) -> int {
+            constexpr bool RE = decltype(replay)::value, PR = decltype(precise)::value;
+            const void* kf = reinterpret_cast<const void*>(&backward_kernel<RE, PR>);
+            const size_t lds = (size_t)(T / 64) * sizeof(BwdLds<RE>);
             GVC_TRY(allow_lds(kf, lds));
             const unsigned nb = walk_blocks(kf, T, lds, NG);
-            backward_kernel<true><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
-                                                      out_weights, gcells, gweights, l.group_offsets, capacity,
-                                                      bl.parts, bl.keys, l.err, l.queues + kQBwd, l.ext_sum,
-                                                      l.log, l.group_head, rp, lp, l.heavy, l.queues + kQHeavyB,
-                                                      l.group_flag);
-        } else {
-            const void* kf = reinterpret_cast<const void*>(&backward_kernel<false>);
-            const size_t lds = (size_t)(T / 64) * sizeof(BwdLds<false>);
-            GVC_TRY(allow_lds(kf, lds));
-            const unsigned nb = walk_blocks(kf, T, lds, NG);
-            backward_kernel<false><<<nb, T, lds, st>>>(l.rec, l.nodes, l.root_box, bv, P, L, pts, NG, out_cells,
-                                                       out_weights, gcells, gweights, l.group_offsets, capacity,
-                                                       bl.parts, bl.keys, l.err, l.queues + kQBwd, l.ext_sum,
-                                                       l.log, l.group_head, rp, lp, l.heavy, l.queues + kQHeavyB,
-                                                       l.group_flag);
-        }
+            backward_kernel<RE, PR><<<nb, T, lds, st>>>(l.rec, l.fac_lo, l.nodes, l.root_box, bv, P, L, pts, NG,
+                                                        out_cells, out_weights, gcells, gweights, l.group_offsets,
+                                                        capacity, bl.parts, bl.keys, l.err, l.queues + kQBwd,
+                                                        l.ext_sum, l.log, l.group_head, rp, lp, l.heavy,
+                                                        l.queues + kQHeavyB, l.group_flag);
+            return GVC_OK;
+        };
+        using T1 = std::true_type;
+        using F0 = std::false_type;
+        const bool replay = (prm->flags & GVC_FLAG_CANDIDATE_LOG) != 0, precise = l.fac_lo != nullptr;
+        GVC_TRY(replay ? (precise ? launch(T1{}, T1{}) : launch(T1{}, F0{}))
+                       : (precise ? launch(F0{}, T1{}) : launch(F0{}, F0{})));
         GVC_LAUNCH_CHECK("backward");
     }
     if (!(stages & GVC_STAGE_REDUCE)) return GVC_OK;
@@ -2398,8 +2412,8 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
     GVC_LAUNCH_CHECK("reduce_sums");
     reduce_long_kernel<<<1024, 256, 0, st>>>(bl.begin, bl.end, svals, bl.parts, bl.long_list, bl.sums);
     GVC_LAUNCH_CHECK("reduce_long");
-    chain_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, bl.sums, scales, rots, k, dmeans, dscales, drots,
-                                                     dvalues, dweights);
+    chain_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, l.fac_lo ? l.rec : nullptr, bl.sums, scales,
+                                                     rots, k, dmeans, dscales, drots, dvalues, dweights);
     GVC_LAUNCH_CHECK("chain");
     return GVC_OK;
 }
@@ -2414,7 +2428,8 @@ int preprocess_launch(int P, const float* means, const float* scales, const floa
 {
     if (P <= 0) return GVC_OK;
     preprocess_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, means, scales, rots, weights, k,
-                                                          (flags & GVC_FLAG_FP64_COND) ? 1 : 0, conics, boxes);
+                                                          (flags & GVC_FLAG_FP64_COND) ? kCondRatio : 0.f, conics,
+                                                          boxes);
     GVC_LAUNCH_CHECK("preprocess");
     return GVC_OK;
 }

@@ -36,7 +36,7 @@ class gvc_batch(ctypes.Structure):
 class gvc_params(ctypes.Structure):
     _fields_ = [("leaf_size", ctypes.c_int), ("block_threads", ctypes.c_int),
                 ("flags", ctypes.c_uint), ("stages", ctypes.c_uint), ("chunk_slots", ctypes.c_uint),
-                ("run_points", ctypes.c_uint)]
+                ("run_points", ctypes.c_uint), ("fp64_scale_ratio", ctypes.c_float)]
 
 
 GVC_ADAM_MAX_TENSORS = 8

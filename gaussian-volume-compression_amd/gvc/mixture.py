@@ -20,6 +20,9 @@ DEFAULT_FLAGS = GVC_FLAG_KAHAN
 @dataclass(frozen=True)
 class MixtureParams:
     """Launch parameters (the autotuner's search space) and precision flags.
+    fp64_cond: Gaussians whose scale ratio exceeds fp64_scale_ratio (0 = 1000,
+    i.e. cond(Sigma) > 1e6) take the FP64 covariance inverse, and their pairs are
+    evaluated from the Cholesky factor of that conic in compensated fp32.
     candidate_log: the forward keeps the candidate chunks it visited and the
     backward replays them; False re-walks the tree in the backward (no
     candidate storage).  chunk_slots: candidate-log slots per 64-point group;
@@ -35,13 +38,15 @@ class MixtureParams:
     chunk_slots: int = 0
     run_points: int = 64
     lane_mode: str = "auto"
+    fp64_scale_ratio: float = 0.0
 
     def c(self, groups=0):
         flags = ((GVC_FLAG_KAHAN if self.kahan else 0) | (GVC_FLAG_FP64_COND if self.fp64_cond else 0) |
                  (GVC_FLAG_CANDIDATE_LOG if self.candidate_log else 0) |
                  {"auto": 0, "never": _lib.GVC_FLAG_LANE_NEVER, "always": _lib.GVC_FLAG_LANE_ALWAYS}[self.lane_mode])
         return gvc_params(int(self.leaf_size), int(self.block_threads), flags, 0,
-                          int(self.chunk_slots or chunk_slots_for(groups)), int(self.run_points))
+                          int(self.chunk_slots or chunk_slots_for(groups)), int(self.run_points),
+                          float(self.fp64_scale_ratio))
 
 
 DEFAULT_CHUNK_SLOTS = 16

@@ -38,7 +38,11 @@ enum gvc_status {
 
 /* gvc_params.flags */
 #define GVC_FLAG_KAHAN 1u      /* compensated per-point sums in the forward       */
-#define GVC_FLAG_FP64_COND 2u  /* FP64 covariance inverse where cond(Sigma) > 1e6 */
+#define GVC_FLAG_FP64_COND 2u  /* FP64 covariance inverse where cond(Sigma) > 1e6
+                                  (gvc_params.fp64_scale_ratio squared); in the
+                                  mixture calls those Gaussians' pairs are then
+                                  evaluated from the Cholesky factor of that FP64
+                                  conic in compensated fp32 (the precise path)   */
 #define GVC_FLAG_CANDIDATE_LOG 4u  /* forward keeps a candidate log that the backward
                                      * replays; clear: the backward walks the tree again */
 /* How a 64-point group is walked.  Default: the group shares one walk of its
@@ -63,6 +67,8 @@ typedef struct gvc_params {
     unsigned stages;      /* GVC_STAGE_* mask, 0 = all                            */
     unsigned chunk_slots; /* candidate-log slots per 64-point group, 0 = 16       */
     unsigned run_points;  /* most points walked together: 16, 32, 64 (0 = 64)     */
+    float fp64_scale_ratio; /* GVC_FLAG_FP64_COND threshold on s_max / s_min:
+                               0 = 1000 (cond(Sigma) > 1e6), else in [1, 1e18]   */
 } gvc_params;
 
 /* Error flags reported by gvc_forward_stats (bitwise OR). */

@@ -48,15 +48,19 @@ def lib():
         L.orc_box_radius.restype = F
         L.orc_box_radius.argtypes = [F]
         L.orc_preprocess.restype = None
-        L.orc_preprocess.argtypes = [I, P, P, P, P, F, I, P, P, P]
+        L.orc_preprocess.argtypes = [I, P, P, P, P, F, I, F, P, P, P, P]
         L.orc_forward.restype = None
-        L.orc_forward.argtypes = [I, I, P, P, P, P, P, P, P, P, P, P]
+        L.orc_forward.argtypes = [I, I, P, P, P, P, P, P, P, P, P, P, P]
         L.orc_forward_binned.restype = I
-        L.orc_forward_binned.argtypes = [I, I, P, P, P, P, P, P, P, I, ctypes.c_longlong, P, P, P]
+        L.orc_forward_binned.argtypes = [I, I, P, P, P, P, P, P, P, P, I, ctypes.c_longlong, P, P, P]
+        L.orc_precise_power.restype = F
+        L.orc_precise_power.argtypes = [P, P, P]
+        L.orc_conic_factor.restype = I
+        L.orc_conic_factor.argtypes = [P, P, P]
         L.orc_conic_fp64_n.restype = None
         L.orc_conic_fp64_n.argtypes = [I, P, P, ctypes.c_float, P]
         L.orc_backward.restype = None
-        L.orc_backward.argtypes = [I, I, P, P, P, P, P, F, P, P, P, P, P, P, P, P, P, P, P, P, P]
+        L.orc_backward.argtypes = [I, I, P, P, P, P, P, F, P, P, P, P, P, P, P, P, P, P, P, P, P, P]
         L.orc_cov_chain.restype = None
         L.orc_cov_chain.argtypes = [P, P, F, P, P, P, P]
         L.orc_conic_fp64.restype = None
@@ -68,9 +72,9 @@ def lib():
         L.orc_cpu_plain_fwdbwd.restype = ctypes.c_longlong
         L.orc_cpu_plain_fwdbwd.argtypes = [I, I, P, P, P, P, P, P, P, P, P, P, P]
         L.orc_pair_accept_ext.restype = I
-        L.orc_pair_accept_ext.argtypes = [P, P, P, P, P]
+        L.orc_pair_accept_ext.argtypes = [P, P, P, P, P, P]
         L.orc_pair_mask.restype = None
-        L.orc_pair_mask.argtypes = [I, I, P, P, P, P, P, P]
+        L.orc_pair_mask.argtypes = [I, I, P, P, P, P, P, P, P]
         L.orc_num_threads.restype = I
         L.orc_num_threads.argtypes = []
         _lib = L
@@ -97,16 +101,35 @@ def box_radius(w):
     return np.float32(lib().orc_box_radius(float(np.float32(w))))
 
 
-def preprocess(means, scales, rots, weights, scale_modifier=1.0, fp64_cond=False):
-    """Canonical conic (P,6) and box (P,6: lo.xyz hi.xyz) -- CR/forward.cu:10-114."""
+def preprocess(means, scales, rots, weights, scale_modifier=1.0, fp64_cond=False, fp64_ratio=0.0,
+               factor=False):
+    """Canonical conic (P,6) and box (P,6: lo.xyz hi.xyz) -- CR/forward.cu:10-114 --
+    and the FP64-path mask (scale ratio above fp64_ratio, 0 = 1000).  factor=True
+    also returns the precise-path factors (P,12: hi[6] lo[6] of the Cholesky factor
+    of the FP64 conic, NaN rows off the path), for forward/backward/pair_mask."""
     means, scales, rots, weights = _f32(means), _f32(scales), _f32(rots), _f32(weights).reshape(-1)
     P = means.shape[0]
     conic = np.zeros((P, 6), np.float32)
     box = np.zeros((P, 6), np.float32)
     ill = np.zeros(P, np.int32)
+    fac = np.zeros((P, 12), np.float32) if factor else None
     lib().orc_preprocess(P, _p(means), _p(scales), _p(rots), _p(weights), float(scale_modifier),
-                         int(bool(fp64_cond)), _p(conic), _p(box), _p(ill))
+                         int(bool(fp64_cond)), float(fp64_ratio), _p(conic), _p(box), _p(ill), _p(fac))
+    if factor:
+        return conic, box, ill.astype(bool), fac
     return conic, box, ill.astype(bool)
+
+
+def precise_power(fac, mean, x):
+    """fp32 power of one pair on the precise path (fac: 12 floats)."""
+    return float(lib().orc_precise_power(_p(_f32(fac)), _p(_f32(mean)), _p(_f32(x))))
+
+
+def conic_factor(conic6_d):
+    """(ok, hi[6], lo[6]) Cholesky factor of one FP64 conic."""
+    hi = np.zeros(6, np.float32); lo = np.zeros(6, np.float32)
+    ok = lib().orc_conic_factor(_p(_f64(conic6_d)), _p(hi), _p(lo))
+    return bool(ok), hi, lo
 
 
 def conic_fp64(scales, rots, scale_modifier=1.0):
@@ -118,7 +141,7 @@ def conic_fp64(scales, rots, scale_modifier=1.0):
     return out
 
 
-def forward(means, values, weights, conic, box, samples, conic_d=None, binned=None):
+def forward(means, values, weights, conic, box, samples, conic_d=None, binned=None, factor=None):
     """FP64 forward with the canonical predicate.  Returns V, W, counts.  Brute
     force, or (binned=True, default for large problems) the same sums with the
     Gaussians binned in a uniform grid -- identical results, every accepted pair
@@ -126,6 +149,7 @@ def forward(means, values, weights, conic, box, samples, conic_d=None, binned=No
     means, values, weights = _f32(means), _f32(values).reshape(-1), _f32(weights).reshape(-1)
     conic, box, samples = _f32(conic), _f32(box), _f32(samples)
     cd = _f64(conic_d) if conic_d is not None else None
+    fac = _f32(factor) if factor is not None else None
     P, S = means.shape[0], samples.shape[0]
     V = np.zeros(S, np.float64)
     W = np.zeros(S, np.float64)
@@ -140,17 +164,19 @@ def forward(means, values, weights, conic, box, samples, conic_d=None, binned=No
         width = np.median(ext[fin], axis=0) if fin.any() else np.ones(3)
         G = int(np.clip(np.min(np.asarray(span) / np.maximum(width, 1e-12)), 1, 256))
         rc = lib().orc_forward_binned(P, S, _p(means), _p(values), _p(weights), _p(conic), _p(box), _p(cd),
-                                      _p(samples), G, 1_500_000_000, _p(V), _p(W), _p(cnt))
+                                      _p(fac), _p(samples), G, 1_500_000_000, _p(V), _p(W), _p(cnt))
         if rc == 0:
             return V, W, cnt
-    lib().orc_forward(P, S, _p(means), _p(values), _p(weights), _p(conic), _p(box), _p(cd),
+    lib().orc_forward(P, S, _p(means), _p(values), _p(weights), _p(conic), _p(box), _p(cd), _p(fac),
                       _p(samples), _p(V), _p(W), _p(cnt))
     return V, W, cnt
 
 
 def backward(means, scales, rots, values, weights, conic, box, samples, V, W, gV, gW,
-             scale_modifier=1.0):
-    """Brute-force FP64 backward (frozen canonical predicate, FP64 conic)."""
+             scale_modifier=1.0, factor=None):
+    """Brute-force FP64 backward (frozen canonical predicate -- the precise path's
+    where factor is given --, FP64 conic)."""
+    fac = _f32(factor) if factor is not None else None
     means, scales, rots = _f32(means), _f32(scales), _f32(rots)
     values, weights = _f32(values).reshape(-1), _f32(weights).reshape(-1)
     conic, box, samples = _f32(conic), _f32(box), _f32(samples)
@@ -159,7 +185,7 @@ def backward(means, scales, rots, values, weights, conic, box, samples, V, W, gV
     dm = np.zeros((P, 3)); ds = np.zeros((P, 3)); dr = np.zeros((P, 4))
     dv = np.zeros(P); dw = np.zeros(P); dc = np.zeros((P, 6))
     lib().orc_backward(P, S, _p(means), _p(scales), _p(rots), _p(values), _p(weights),
-                       float(scale_modifier), _p(conic), _p(box), _p(samples), _p(V), _p(W),
+                       float(scale_modifier), _p(conic), _p(box), _p(fac), _p(samples), _p(V), _p(W),
                        _p(gV), _p(gW), _p(dm), _p(ds), _p(dr), _p(dv), _p(dw), _p(dc))
     return dict(means=dm, scales=ds, rotations=dr, values=dv, weights=dw, conic=dc)
 
@@ -206,13 +232,14 @@ def cpu_plain_fwdbwd(means, values, weights, conic, box, samples, gV, gW):
     return V, W, g, int(pairs)
 
 
-def pair_mask(means, conic, box, samples):
+def pair_mask(means, conic, box, samples, factor=None):
     """Dense canonical membership mask (P, S) bool and the fp32 powers."""
     means, conic, box, samples = _f32(means), _f32(conic), _f32(box), _f32(samples)
+    fac = _f32(factor) if factor is not None else None
     P, S = means.shape[0], samples.shape[0]
     mask = np.zeros((P, S), np.uint8)
     power = np.zeros((P, S), np.float32)
-    lib().orc_pair_mask(P, S, _p(means), _p(conic), _p(box), _p(samples), _p(mask), _p(power))
+    lib().orc_pair_mask(P, S, _p(means), _p(conic), _p(box), _p(fac), _p(samples), _p(mask), _p(power))
     return mask.astype(bool), power
 
 

@@ -143,37 +143,116 @@ static void orc_rotation_d(const float q[4], double R[3][3])
     R[2][2] = 1.0 - 2.0 * (x * x + y * y);
 }
 
-/* True when cond(Sigma) = (s_max/s_min)^2 exceeds 1e6 (unregularised Sigma, unit q). */
-int orc_ill_conditioned(const float s[3], float scale_modifier)
+/* True when the scale ratio s_max/s_min exceeds ratio (cond(Sigma) = ratio^2 on the
+ * unregularised Sigma with unit q; the default ratio 1000 is cond > 1e6). */
+int orc_ill_conditioned(const float s[3], float scale_modifier, float ratio)
 {
     float a = fabsf(scale_modifier * s[0]);
     float b = fabsf(scale_modifier * s[1]);
     float c = fabsf(scale_modifier * s[2]);
     float smax = fmaxf(fmaxf(a, b), c);
     float smin = fminf(fminf(a, b), c);
-    return smax > ORC_COND_RATIO * smin;
+    return smax > ratio * smin;
+}
+
+/*
+ * Upper Cholesky factor U of an FP64 conic (packed xx xy xz yy yz zz), conic =
+ * U^T U, entries u00 u01 u02 u11 u12 u22 as hi + lo floats.  Returns 0 when the
+ * conic is not numerically positive definite (the Gaussian keeps its conic).
+ */
+int orc_conic_factor(const double C[6], float hi[6], float lo[6])
+{
+    if (!(C[0] > 0.0)) return 0;
+    const double u00 = sqrt(C[0]);
+    const double u01 = C[1] / u00, u02 = C[2] / u00;
+    const double a1 = C[3] - u01 * u01;
+    if (!(a1 > 0.0)) return 0;
+    const double u11 = sqrt(a1);
+    const double u12 = (C[4] - u01 * u02) / u11;
+    const double a2 = (C[5] - u02 * u02) - u12 * u12;
+    if (!(a2 > 0.0)) return 0;
+    const double u[6] = { u00, u01, u02, u11, u12, sqrt(a2) };
+    for (int k = 0; k < 6; ++k) {
+        hi[k] = (float)u[k];
+        if (!(fabsf(hi[k]) <= 3.4e38f)) return 0;
+        lo[k] = (float)(u[k] - (double)hi[k]);
+    }
+    return 1;
+}
+
+/* Knuth's TwoSum: s + e == a + b exactly. */
+static inline void orc_two_sum(float a, float b, float *s, float *e)
+{
+    float t = a + b;
+    float bb = t - a;
+    *e = (a - (t - bb)) + (b - bb);
+    *s = t;
+}
+
+/* (s, c) += (uh + ul)(dh + dl): uh dh exactly (product, fma residual, TwoSum),
+ * cross terms uh dl + ul dh rounded once, ul dl dropped. */
+static inline void orc_ff_madd(float *s, float *c, float uh, float ul, float dh, float dl)
+{
+    float p = uh * dh;
+    float pe = fmaf(uh, dh, -p);
+    float t, e;
+    orc_two_sum(*s, p, &t, &e);
+    *s = t;
+    *c = *c + (e + (pe + fmaf(uh, dl, ul * dh)));
+}
+
+/* Power of a pair on the precise path: d = x - mu exactly as dh + dl (TwoSum),
+ * y = U d in float-float arithmetic, each component rounded once to float,
+ * power = -0.5 |y|^2 with quad = fma(y2, y2, fma(y1, y1, y0 y0)).  fac = hi[6] lo[6]. */
+float orc_precise_power(const float *fac, const float *mu, const float *x)
+{
+    const float *h = fac, *l = fac + 6;
+    float xh, xl, yh, yl, zh, zl;
+    orc_two_sum(x[0], -mu[0], &xh, &xl);
+    orc_two_sum(x[1], -mu[1], &yh, &yl);
+    orc_two_sum(x[2], -mu[2], &zh, &zl);
+    float s = 0.f, c = 0.f;
+    orc_ff_madd(&s, &c, h[0], l[0], xh, xl);
+    orc_ff_madd(&s, &c, h[1], l[1], yh, yl);
+    orc_ff_madd(&s, &c, h[2], l[2], zh, zl);
+    float y0 = s + c;
+    s = 0.f; c = 0.f;
+    orc_ff_madd(&s, &c, h[3], l[3], yh, yl);
+    orc_ff_madd(&s, &c, h[4], l[4], zh, zl);
+    float y1 = s + c;
+    s = 0.f; c = 0.f;
+    orc_ff_madd(&s, &c, h[5], l[5], zh, zl);
+    float y2 = s + c;
+    float quad = fmaf(y2, y2, fmaf(y1, y1, y0 * y0));
+    return -0.5f * quad;
 }
 
 /*
  * CR/forward.cu:10-114 (preprocessCUDA).  Outputs per Gaussian:
  *   conic[6] = packed inverse of (Sigma + eps I): xx xy xz yy yz zz
  *   box[6]   = lo.xyz, hi.xyz
- * fp64_cond != 0: Gaussians with cond(Sigma) > 1e6 get their covariance, epsilon,
- * determinant and adjugate in double (plain ops), conic = (float)(adj/det).
+ * fp64_cond != 0: Gaussians whose scale ratio exceeds fp64_ratio (0 = 1000, i.e.
+ * cond(Sigma) > 1e6) get their covariance, epsilon, determinant and adjugate in
+ * double (plain ops), conic = (float)(adj/det).  factor_out (P x 12, may be NULL):
+ * for those, the Cholesky factor of the double conic as hi[6] lo[6]
+ * (orc_conic_factor) -- the precise path; NaN in every other row.
  */
 void orc_preprocess(int P, const float *means, const float *scales, const float *rots,
-                    const float *weights, float scale_modifier, int fp64_cond,
-                    float *conic_out, float *box_out, int32_t *ill_out)
+                    const float *weights, float scale_modifier, int fp64_cond, float fp64_ratio,
+                    float *conic_out, float *box_out, int32_t *ill_out, float *factor_out)
 {
+    const float ratio = fp64_ratio > 0.f ? fp64_ratio : ORC_COND_RATIO;
     for (int i = 0; i < P; ++i) {
         const float *q = rots + 4 * i;
         float s[3] = { scale_modifier * scales[3 * i + 0], scale_modifier * scales[3 * i + 1],
                        scale_modifier * scales[3 * i + 2] };
         float R[3][3];
         orc_rotation(q, R);
-        int ill = fp64_cond ? orc_ill_conditioned(scales + 3 * i, scale_modifier) : 0;
+        int ill = fp64_cond ? orc_ill_conditioned(scales + 3 * i, scale_modifier, ratio) : 0;
         if (ill_out) ill_out[i] = ill;
         float *cn = conic_out + 6 * i;
+        if (factor_out)
+            for (int k = 0; k < 12; ++k) factor_out[12 * i + k] = NAN;
         if (!ill) {
             float M[3][3]; /* M[a][i] = s_a * Rstd[i][a] */
             for (int a = 0; a < 3; ++a)
@@ -222,12 +301,16 @@ void orc_preprocess(int P, const float *means, const float *scales, const float 
             double A00 = d * f - e * e, A01 = c * e - b * f, A02 = b * e - c * d;
             double A11 = a * f - c * c, A12 = b * c - a * e, A22 = a * d - b * b;
             double det = a * A00 + b * A01 + c * A02;
-            cn[0] = (float)(A00 / det);
-            cn[1] = (float)(A01 / det);
-            cn[2] = (float)(A02 / det);
-            cn[3] = (float)(A11 / det);
-            cn[4] = (float)(A12 / det);
-            cn[5] = (float)(A22 / det);
+            const double cd[6] = { A00 / det, A01 / det, A02 / det, A11 / det, A12 / det, A22 / det };
+            for (int k = 0; k < 6; ++k) cn[k] = (float)cd[k];
+            if (factor_out) {
+                float hi[6], lo[6];
+                if (orc_conic_factor(cd, hi, lo))
+                    for (int k = 0; k < 6; ++k) {
+                        factor_out[12 * i + k] = hi[k];
+                        factor_out[12 * i + 6 + k] = lo[k];
+                    }
+            }
         }
         /* AABB: CR/forward.cu:79-113 */
         float m = orc_box_radius(weights[i]);
@@ -247,13 +330,20 @@ void orc_preprocess(int P, const float *means, const float *scales, const float 
 }
 
 /* Canonical per-pair predicate.  Returns 1 if (Gaussian, point) contributes and
- * writes the fp32 power and the fp32 offsets used for it. */
-static inline int orc_pair_accept(const float *box, const float *mu, const float *cn,
+ * writes the fp32 power.  fac (12 floats or NULL): a Gaussian on the precise path
+ * (fac[0] not NaN) takes its power from orc_precise_power. */
+static inline int orc_pair_accept(const float *box, const float *mu, const float *cn, const float *fac,
                                   const float *x, float *power_out)
 {
     if (!(box[0] <= x[0] && x[0] <= box[3] && box[1] <= x[1] && x[1] <= box[4] &&
           box[2] <= x[2] && x[2] <= box[5]))
         return 0;
+    if (fac && fac[0] == fac[0]) {
+        float pw = orc_precise_power(fac, mu, x);
+        if (pw < -14.0f || pw > 0.0f) return 0;
+        *power_out = pw;
+        return 1;
+    }
     float dx = x[0] - mu[0], dy = x[1] - mu[1], dz = x[2] - mu[2];
     float r0 = fmaf(cn[2], dz, fmaf(cn[1], dy, cn[0] * dx));
     float r1 = fmaf(cn[4], dz, fmaf(cn[3], dy, cn[1] * dx));
@@ -265,22 +355,23 @@ static inline int orc_pair_accept(const float *box, const float *mu, const float
     return 1;
 }
 
-int orc_pair_accept_ext(const float *box, const float *mu, const float *cn, const float *x,
+int orc_pair_accept_ext(const float *box, const float *mu, const float *cn, const float *fac, const float *x,
                         float *power_out)
 {
-    return orc_pair_accept(box, mu, cn, x, power_out);
+    return orc_pair_accept(box, mu, cn, fac, x, power_out);
 }
 
 /* Dense membership mask (small problems only): mask[i*S + s] = 1 iff the canonical
  * predicate accepts (Gaussian i, point s); power[i*S + s] = its fp32 power. */
 void orc_pair_mask(int P, int S, const float *means, const float *conic, const float *box,
-                   const float *samples, uint8_t *mask, float *power)
+                   const float *factor, const float *samples, uint8_t *mask, float *power)
 {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < P; ++i)
         for (int s = 0; s < S; ++s) {
             float pw = 0.f;
-            int a = orc_pair_accept(box + 6 * i, means + 3 * i, conic + 6 * i, samples + 3 * s, &pw);
+            int a = orc_pair_accept(box + 6 * i, means + 3 * i, conic + 6 * i, factor ? factor + 12 * i : NULL,
+                                    samples + 3 * s, &pw);
             mask[(size_t)i * S + s] = (uint8_t)a;
             if (power) power[(size_t)i * S + s] = a ? pw : 0.f;
         }
@@ -326,7 +417,7 @@ static inline void orc_finish(double W, double U, int cnt, int s, double *V_out,
 }
 
 void orc_forward(int P, int S, const float *means, const float *values, const float *weights,
-                 const float *conic, const float *box, const double *conic_d,
+                 const float *conic, const float *box, const double *conic_d, const float *factor,
                  const float *samples, double *V_out, double *W_out, int32_t *count_out)
 {
 #pragma omp parallel for schedule(dynamic, 64)
@@ -336,7 +427,9 @@ void orc_forward(int P, int S, const float *means, const float *values, const fl
         int cnt = 0;
         for (int i = 0; i < P; ++i) {
             float pw;
-            if (!orc_pair_accept(box + 6 * i, means + 3 * i, conic + 6 * i, x, &pw)) continue;
+            if (!orc_pair_accept(box + 6 * i, means + 3 * i, conic + 6 * i, factor ? factor + 12 * i : NULL, x,
+                                 &pw))
+                continue;
             orc_pair_add(i, x, means, values, weights, conic, conic_d, &W, &U);
             ++cnt;
         }
@@ -351,7 +444,7 @@ void orc_forward(int P, int S, const float *means, const float *values, const fl
  * order of orc_forward and the FP64 sums are identical.  Returns 0, or -1 when
  * the bins would exceed max_entries. */
 int orc_forward_binned(int P, int S, const float *means, const float *values, const float *weights,
-                       const float *conic, const float *box, const double *conic_d,
+                       const float *conic, const float *box, const double *conic_d, const float *factor,
                        const float *samples, int G, long long max_entries, double *V_out,
                        double *W_out, int32_t *count_out)
 {
@@ -407,7 +500,9 @@ int orc_forward_binned(int P, int S, const float *means, const float *values, co
                     for (long long k = start[c]; k < start[c + 1]; ++k) {
                         const int i = list[k];
                         float pw;
-                        if (!orc_pair_accept(box + 6 * i, means + 3 * i, conic + 6 * i, x, &pw)) continue;
+                        if (!orc_pair_accept(box + 6 * i, means + 3 * i, conic + 6 * i,
+                                             factor ? factor + 12 * i : NULL, x, &pw))
+                            continue;
                         orc_pair_add(i, x, means, values, weights, conic, conic_d, &Wv, &U);
                         ++cnt;
                     }
@@ -530,8 +625,8 @@ void orc_conic_fp64_n(int P, const float *scales, const float *rots, float scale
  */
 void orc_backward(int P, int S, const float *means, const float *scales, const float *rots,
                   const float *values, const float *weights, float scale_modifier,
-                  const float *conic, const float *box, const float *samples, const double *V,
-                  const double *W, const double *gV, const double *gW, double *dmeans,
+                  const float *conic, const float *box, const float *factor, const float *samples,
+                  const double *V, const double *W, const double *gV, const double *gW, double *dmeans,
                   double *dscales, double *drots, double *dvalues, double *dweights,
                   double *dconic_out)
 {
@@ -543,7 +638,9 @@ void orc_backward(int P, int S, const float *means, const float *scales, const f
         for (int s = 0; s < S; ++s) {
             const float *x = samples + 3 * s;
             float pw;
-            if (!orc_pair_accept(box + 6 * i, means + 3 * i, conic + 6 * i, x, &pw)) continue;
+            if (!orc_pair_accept(box + 6 * i, means + 3 * i, conic + 6 * i, factor ? factor + 12 * i : NULL, x,
+                                 &pw))
+                continue;
             if (W[s] <= ORC_WEIGHT_CUTOFF) continue;
             double dx = (double)x[0] - means[3 * i + 0];
             double dy = (double)x[1] - means[3 * i + 1];
@@ -684,7 +781,7 @@ long long orc_cpu_plain_fwdbwd(int P, int S, const float *means, const float *va
             float W = 0.f, U = 0.f;
             for (int i = 0; i < P; ++i) {
                 float pw;
-                if (!orc_pair_accept(box + 6 * i, means + 3 * i, conic + 6 * i, x, &pw)) continue;
+                if (!orc_pair_accept(box + 6 * i, means + 3 * i, conic + 6 * i, NULL, x, &pw)) continue;
                 float wgt = weights[i] * expf(pw);
                 W += wgt;
                 U += wgt * values[i];
@@ -698,7 +795,7 @@ long long orc_cpu_plain_fwdbwd(int P, int S, const float *means, const float *va
             float invW = 1.f / W;
             for (int i = 0; i < P; ++i) {
                 float pw;
-                if (!orc_pair_accept(box + 6 * i, means + 3 * i, conic + 6 * i, x, &pw)) continue;
+                if (!orc_pair_accept(box + 6 * i, means + 3 * i, conic + 6 * i, NULL, x, &pw)) continue;
                 const float *cn = conic + 6 * i;
                 float dx = x[0] - means[3 * i], dy = x[1] - means[3 * i + 1], dz = x[2] - means[3 * i + 2];
                 float e = expf(pw), wgt = weights[i] * e;

@@ -201,6 +201,70 @@ def test_backward_config0_parity(leaf, log, lane):
     _grad_check(g, scales, samples, mx.MixtureParams(leaf_size=leaf, candidate_log=log, lane_mode=lane))
 
 
+def ill_conditioned_set(seed=31, frac=0.15, divisors=(35.0, 3500.0)):
+    """Config-0 Gaussians with a fraction made anisotropic (one scale divided by
+    each of ``divisors`` in turn) and randomly rotated, and points half from the
+    regular grid, half inside the anisotropic Gaussians' boxes."""
+    g = workloads.config0(0)
+    rng = np.random.default_rng(seed)
+    P = g["means"].shape[0]
+    scales = np.exp(g["log_scales"]).astype(f32)
+    pick = rng.permutation(P)[:int(frac * P)]
+    div = np.asarray(divisors)[np.arange(pick.size) % len(divisors)]
+    ax = rng.integers(0, 3, pick.size)
+    scales[pick, ax] = (scales[pick, ax] / div).astype(f32)
+    q = rng.normal(size=(pick.size, 4))
+    rots = g["rotations"].copy()
+    rots[pick] = (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(f32)
+    _, box, _ = oracle.preprocess(g["means"], scales, rots, g["weights"])
+    k = pick[rng.integers(0, pick.size, 4096)]
+    inside = box[k, :3] + rng.uniform(0, 1, (4096, 3)) * (box[k, 3:] - box[k, :3])
+    samples = np.concatenate([g["samples"][::8], inside]).astype(f32)
+    return dict(g, rotations=rots), scales, samples
+
+
+@pytest.mark.parametrize("ratio,leaf,log,lane", [(0.0, 4, True, "auto"), (10.0, 4, True, "auto"),
+                                                 (10.0, 1, False, "auto"), (10.0, 16, True, "always"),
+                                                 (10.0, 2, True, "never")])
+def test_precise_path_parity(ratio, leaf, log, lane):
+    """FP64 mode: Gaussians above the scale ratio take the FP64 inverse and their
+    pairs are evaluated from the Cholesky factor (compensated fp32).  Bars: conics
+    bit-exact, pair count equal to the oracle's (which evaluates the same factor
+    form), field within 1e-5 and gradients within 1e-4 of the FP64 oracle.  At the
+    default threshold (ratio 1000) every anisotropic Gaussian is above it; a
+    Gaussian below it keeps the fp32 conic form, whose error at ratio 35 exceeds
+    the field bar."""
+    g, scales, samples = ill_conditioned_set(divisors=(3500.0,) if ratio == 0.0 else (35.0, 3500.0))
+    params = mx.MixtureParams(leaf_size=leaf, candidate_log=log, lane_mode=lane, fp64_cond=True,
+                              fp64_scale_ratio=ratio)
+    ss, cells, cw, conics, st = run_gpu(g["means"], scales, g["rotations"], g["values"], g["weights"], samples,
+                                        params)
+    conic_ref, box_ref, ill, fac = oracle.preprocess(g["means"], scales, g["rotations"], g["weights"], 1.0, True,
+                                                     ratio, factor=True)
+    assert ill.sum() == int(0.15 * 2048)
+    assert np.array_equal(conics.cpu().numpy().reshape(-1, 6), conic_ref)
+    cd = oracle.conic_fp64(scales, g["rotations"])
+    Vr, Wr, cnt = oracle.forward(g["means"], g["values"], g["weights"], conic_ref, box_ref, samples, cd,
+                                 factor=fac)
+    V, W = cells.cpu().numpy(), cw.cpu().numpy()
+    nrec, npairs, err = st.stats()
+    assert err == 0
+    assert npairs == int(cnt.sum())
+    assert np.array_equal(W == 0, Wr == 0)
+    assert field_err(V, W, Vr, Wr) <= 1e-5
+    rng = np.random.default_rng(1)
+    gV = rng.normal(size=samples.shape[0]).astype(f32)
+    gW = rng.normal(size=samples.shape[0]).astype(f32)
+    grads = mx.mixture_backward(st, T(g["means"]), T(scales), T(g["rotations"]), T(g["values"]), T(g["weights"]),
+                                1.0, ss, cells, cw, T(gV), T(gW))
+    ref = oracle.backward(g["means"], scales, g["rotations"], g["values"], g["weights"], conic_ref, box_ref,
+                          samples, Vr, Wr, gV.astype(np.float64), gW.astype(np.float64), factor=fac)
+    for n, t in zip(["means", "scales", "rotations", "values", "weights"], grads):
+        a = t.cpu().numpy().reshape(-1).astype(np.float64)
+        b = ref[n].reshape(-1)
+        assert np.abs(a - b).max() / max(np.abs(b).max(), 1e-30) <= 1e-4, n
+
+
 def sparse_and_dense(seed=12, P=40000, dense=1500, sparse=300):
     """Small Gaussians over the unit cube; points in one dense cluster (groups
     walked together) and scattered thinly (groups walked per lane)."""

@@ -146,6 +146,108 @@ def test_fp64_path_for_ill_conditioned():
     assert err32.max() > 10 * err64.max()  # the fp32 adjugate loses digits here
 
 
+# ---------------------------------------------------------------- precise path
+def _ill_set(P, ratio_axis, seed):
+    rng = np.random.default_rng(seed)
+    q = rng.normal(size=(P, 4))
+    q = (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(f32)
+    s = np.exp(rng.normal(np.log(0.005), 0.2, (P, 3))).astype(f32)
+    s[np.arange(P), rng.integers(0, 3, P)] = (s[:, 0] / ratio_axis).astype(f32)
+    mu = rng.uniform(0.1, 0.9, (P, 3)).astype(f32)
+    w = rng.uniform(0.2, 1.0, (P, 1)).astype(f32)
+    return mu, s, q, w
+
+
+def _sym(c):
+    return np.array([[c[0], c[1], c[2]], [c[1], c[3], c[4]], [c[2], c[4], c[5]]], np.float64)
+
+
+def test_conic_factor_reconstructs_and_rejects():
+    # U^T U with U = hi + lo reproduces the FP64 conic to double-float accuracy
+    mu, s, q, w = _ill_set(300, 3000.0, 21)
+    cd = oracle.conic_fp64(s, q)
+    for c in cd:
+        ok, hi, lo = oracle.conic_factor(c)
+        assert ok
+        U = np.zeros((3, 3))
+        U[np.triu_indices(3)] = hi.astype(np.float64) + lo.astype(np.float64)
+        C = _sym(c)
+        assert np.abs(U.T @ U - C).max() <= 1e-13 * np.abs(C).max()
+        assert np.all(np.abs(lo) <= np.spacing(np.abs(hi)))   # lo below half an ulp of hi
+        assert U[0, 0] > 0 and U[1, 1] > 0 and U[2, 2] > 0
+    # diagonal conic: U = diag(sqrt(c)), lo = the float rounding error of sqrt
+    ok, hi, lo = oracle.conic_factor([4.0, 0, 0, 9.0, 0, 2.0])
+    assert ok and np.array_equal(hi, np.array([2, 0, 0, 3, 0, np.float32(math.sqrt(2))], f32))
+    assert lo[5] == np.float32(math.sqrt(2) - float(np.float32(math.sqrt(2))))
+    # not positive definite
+    assert not oracle.conic_factor([1.0, 2.0, 0, 1.0, 0, 1.0])[0]
+    assert not oracle.conic_factor([-1.0, 0, 0, 1.0, 0, 1.0])[0]
+    assert not oracle.conic_factor([np.nan, 0, 0, 1.0, 0, 1.0])[0]
+
+
+@pytest.mark.parametrize("ratio", [35.0, 3500.0])
+def test_precise_power_against_exact_quadratic_form(ratio):
+    # power from the factor agrees with -0.5 d^T C d evaluated exactly (double,
+    # exact d) to a few fp32 ulps of the power; the fp32 conic form of the same
+    # Gaussian is orders of magnitude off, and dropping the lo parts (a planted
+    # mistake) breaks the bound.
+    mu, s, q, w = _ill_set(60, ratio, 22)
+    conic, box, ill, fac = oracle.preprocess(mu, s, q, w, fp64_cond=True, fp64_ratio=10.0, factor=True)
+    assert ill.all() and np.isfinite(fac).all()
+    cd = oracle.conic_fp64(s, q)
+    rng = np.random.default_rng(23)
+    worst = worst_conic = worst_nolo = 0.0
+    fac_nolo = fac.copy()
+    fac_nolo[:, 6:] = 0.0
+    for i in range(len(mu)):
+        C = _sym(cd[i])
+        K = np.linalg.cholesky(np.linalg.inv(C))       # points with quad in [0, 28]
+        y = rng.normal(size=(200, 3))
+        y *= (rng.uniform(0, 28, 200) ** 0.5 / np.linalg.norm(y, axis=1))[:, None]
+        X = (mu[i].astype(np.float64) + y @ K.T).astype(f32)
+        d = X.astype(np.float64) - mu[i].astype(np.float64)
+        exact = -0.5 * np.einsum('ni,ij,nj->n', d, C, d)
+        for x, e in zip(X, exact):
+            tol = 4 * float(np.spacing(f32(max(abs(e), 1.0))))
+            worst = max(worst, abs(oracle.precise_power(fac[i], mu[i], x) - e) / tol)
+            worst_nolo = max(worst_nolo, abs(oracle.precise_power(fac_nolo[i], mu[i], x) - e) / tol)
+            dx = (x - mu[i]).astype(f32)
+            c = conic[i]
+            r = [c[0] * dx[0] + c[1] * dx[1] + c[2] * dx[2], c[1] * dx[0] + c[3] * dx[1] + c[4] * dx[2],
+                 c[2] * dx[0] + c[4] * dx[1] + c[5] * dx[2]]
+            worst_conic = max(worst_conic, abs(-0.5 * float(np.dot(dx, r)) - e) / tol)
+    assert worst <= 1.0
+    assert worst_nolo > 1.0
+    assert worst_conic > 10.0
+
+
+def test_precise_path_forward_and_mask():
+    # the forward with the factor matches FP64 sums at the same pairs, and the
+    # pair set is exactly the points whose precise power lies in the window
+    mu, s, q, w = _ill_set(40, 3000.0, 24)
+    v = np.random.default_rng(25).uniform(0, 1, (40, 1)).astype(f32)
+    conic, box, ill, fac = oracle.preprocess(mu, s, q, w, fp64_cond=True, factor=True)
+    assert ill.all()
+    rng = np.random.default_rng(26)
+    k = rng.integers(0, 40, 3000)
+    X = (box[k, :3] + rng.uniform(0, 1, (3000, 3)) * (box[k, 3:] - box[k, :3])).astype(f32)
+    mask, power = oracle.pair_mask(mu, conic, box, X, factor=fac)
+    for i in range(0, 40, 7):
+        for j in np.flatnonzero(mask[i])[:20]:
+            assert power[i, j] == np.float32(oracle.precise_power(fac[i], mu[i], X[j]))
+    V, W, cnt = oracle.forward(mu, v, w, conic, box, X, oracle.conic_fp64(s, q), factor=fac)
+    assert np.array_equal(cnt, mask.sum(axis=0))
+    cd = oracle.conic_fp64(s, q)
+    Wr = np.zeros(len(X)); Ur = np.zeros(len(X))
+    for i in range(40):
+        d = X[mask[i]].astype(np.float64) - mu[i].astype(np.float64)
+        e = np.exp(-0.5 * np.einsum('ni,ij,nj->n', d, _sym(cd[i]), d))
+        Wr[mask[i]] += float(w[i, 0]) * e
+        Ur[mask[i]] += float(w[i, 0]) * e * float(v[i, 0])
+    on = Wr > 1e-2
+    assert np.allclose(W[on], Wr[on], rtol=1e-12) and np.allclose(V[on], Ur[on] / Wr[on], rtol=1e-12)
+
+
 # ---------------------------------------------------------------- Morton
 def test_morton_known_answers():
     pts = np.array([[0, 0, 0], [1, 1, 1], [1, 0, 0], [0, 1, 0], [0, 0, 1],
