@@ -446,9 +446,61 @@ def precision_study(args, W, dev, n_points=65536, n_stress=16384):
                 "sentinel_flips": int(np.sum((Wg > 0) != (Wr > 0))),
                 "pairs": int(pairs), "pairs_match_oracle": int(pairs) == int(cnt.sum())}
         out["sets"][set_name] = res
+    out["gradients"] = precision_gradients(W, modes, pre, cd, (mh, sh, rh, wh, vh), (m, sc, rq, wv, vv), uniform,
+                                           ill, args.seed)
     out["seconds"] = time.perf_counter() - t0
     log(f"precision study: {json.dumps(out['sets'])}")
+    log(f"precision study gradients: {json.dumps(out['gradients'])}")
     return out
+
+
+def precision_gradients(W, modes, pre, cd, host, dev_params, pts_t, ill, seed, n_each=256):
+    """Gradient error of each Kahan mode against the FP64 oracle backward (the
+    mode's own predicate, FP64 conics) for a forward + backward over the uniform
+    point set with random upstream gradients.  A Gaussian's gradient depends only
+    on its own parameters and on V, W and the upstream gradients at the points, so
+    the oracle evaluates a subset: n_each Gaussians each from the FP64-path set
+    (scale ratio above 1000), the ratio-35 set and the rest.  Error per parameter
+    tensor: max |g - g*| / max |g*| over the subset, as in the parity tests."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from gvc import mixture as mx
+
+    mh, sh, rh, wh, vh = host
+    m, sc, rq, wv, vv = dev_params
+    rng = np.random.default_rng(seed + 13)
+    deg = np.zeros(mh.shape[0], bool)
+    deg[np.asarray(W["degenerate"])] = True
+    sub = np.concatenate([rng.choice(np.flatnonzero(mask), n_each, replace=False)
+                          for mask in (ill, deg, ~(ill | deg)) if mask.any()])
+    pts = pts_t.cpu().numpy()
+    gV = rng.normal(size=pts.shape[0]).astype(np.float32)
+    gW = rng.normal(size=pts.shape[0]).astype(np.float32)
+    res = {"gaussians": int(sub.size), "points": int(pts.shape[0]), "modes": {}}
+    names = ["means", "scales", "rotations", "values", "weights"]
+    for name, prm in modes.items():
+        if not prm.kahan:
+            continue
+        fac = pre[name][3] if prm.fp64_cond else None
+        with torch.no_grad():
+            ss = mx.SampleSet(pts_t)
+            cells, cw, _, st = mx.mixture_forward(m, sc, rq, vv, wv, 1.0, ss, prm, want_conics=False)
+            grads = mx.mixture_backward(st, m, sc, rq, vv, wv, 1.0, ss, cells, cw,
+                                        torch.as_tensor(gV, device=pts_t.device),
+                                        torch.as_tensor(gW, device=pts_t.device))
+        Vr, Wr, _ = oracle.forward(mh, vh, wh, pre[name][0], pre[name][1], pts, cd, factor=fac)
+        ref = oracle.backward(mh[sub], sh[sub], rh[sub], vh[sub], wh[sub], pre[name][0][sub], pre[name][1][sub], pts,
+                              Vr, Wr, gV.astype(np.float64), gW.astype(np.float64),
+                              factor=fac[sub] if fac is not None else None)
+        errs = {}
+        for n, t in zip(names, grads):
+            a = t.detach().cpu().numpy().reshape(mh.shape[0], -1)[sub].astype(np.float64)
+            b = ref[n].reshape(sub.size, -1)
+            errs[n] = float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+        res["modes"][name] = {"max_rel_by_tensor": errs, "max_rel": max(errs.values())}
+    return res
 
 
 def main_eval(args):

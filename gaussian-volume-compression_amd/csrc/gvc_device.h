@@ -127,22 +127,32 @@ __device__ __forceinline__ bool ill_conditioned(float s0, float s1, float s2, fl
 // Upper Cholesky factor U of an FP64 conic (packed xx xy xz yy yz zz), conic =
 // U^T U, as hi + lo float pairs (u00 u01 u02 u11 u12 u22).  False when the conic
 // is not numerically positive definite (the Gaussian then keeps its conic).
-__device__ __forceinline__ bool conic_factor(const double C[6], float hi[6], float lo[6])
+__device__ __forceinline__ bool conic_factor_d(const double C[6], double u[6])
 {
     if (!(C[0] > 0.0)) return false;
-    const double u00 = __builtin_sqrt(C[0]);
-    const double u01 = C[1] / u00, u02 = C[2] / u00;
-    const double a1 = C[3] - u01 * u01;
+    u[0] = __builtin_sqrt(C[0]);
+    u[1] = C[1] / u[0];
+    u[2] = C[2] / u[0];
+    const double a1 = C[3] - u[1] * u[1];
     if (!(a1 > 0.0)) return false;
-    const double u11 = __builtin_sqrt(a1);
-    const double u12 = (C[4] - u01 * u02) / u11;
-    const double a2 = (C[5] - u02 * u02) - u12 * u12;
+    u[3] = __builtin_sqrt(a1);
+    u[4] = (C[4] - u[1] * u[2]) / u[3];
+    const double a2 = (C[5] - u[2] * u[2]) - u[4] * u[4];
     if (!(a2 > 0.0)) return false;
-    const double u[6] = { u00, u01, u02, u11, u12, __builtin_sqrt(a2) };
+    u[5] = __builtin_sqrt(a2);
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        if (!(fabsf((float)u[k]) <= 3.4e38f)) return false;
+    return true;
+}
+
+__device__ __forceinline__ bool conic_factor(const double C[6], float hi[6], float lo[6])
+{
+    double u[6];
+    if (!conic_factor_d(C, u)) return false;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         hi[k] = (float)u[k];
-        if (!(fabsf(hi[k]) <= 3.4e38f)) return false;
         lo[k] = (float)(u[k] - (double)hi[k]);
     }
     return true;
@@ -241,13 +251,14 @@ __device__ __forceinline__ bool preprocess_one(const float mu[3], const float sc
 // (r x y z).  The forward it differentiates: M = diag(k s) R^T, Sigma = M^T M,
 // conic = (Sigma + eps I)^-1 with eps = 1e-5 max_i |Sigma_ii| (CR/forward.cu:26-75;
 // the eps term's own derivative is kept on the largest diagonal entry).
-// yframe: g6 is the gradient with respect to the factor-frame outer products
-// (sum of dquad y y^T, y = U d, conic = U^T U: the precise path's backward), and
-// dL/dSigma = -C G C is formed as -U^T G_y U (exact algebra with G = U^-1 G_y U^-T;
-// no cancellation against the conic's large entries).
+// fp64_ratio > 0: a Gaussian on the precise path (the preprocess's test: scale
+// ratio above fp64_ratio and a positive definite factor of the same FP64 conic)
+// has g6 with respect to the factor-frame outer products (sum of dquad y y^T,
+// y = U d, conic = U^T U), and dL/dSigma = -C G C is formed as -U^T G_y U (exact
+// algebra with G = U^-1 G_y U^-T; no cancellation against the conic's large entries).
 __device__ __forceinline__ void covariance_chain(const double* __restrict__ g6, const float* __restrict__ sc,
                                                  const float* __restrict__ q, float kmod, double ds[3],
-                                                 double dq[4], bool yframe = false)
+                                                 double dq[4], float fp64_ratio = 0.f)
 {
     const float q4[4] = { q[0], q[1], q[2], q[3] };
     double R[3][3];
@@ -277,15 +288,10 @@ __device__ __forceinline__ void covariance_chain(const double* __restrict__ g6, 
                              { 0.5 * g6[1], g6[3], 0.5 * g6[4] },
                              { 0.5 * g6[2], 0.5 * g6[4], g6[5] } };
     double CG[3][3], K[3][3];
-    if (yframe) {
-        // upper Cholesky factor of C (the preprocess's conic_factor in double)
-        double U[3][3] = { { 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0 } };
-        U[0][0] = __builtin_sqrt(C[0][0]);
-        U[0][1] = C[0][1] / U[0][0];
-        U[0][2] = C[0][2] / U[0][0];
-        U[1][1] = __builtin_sqrt(C[1][1] - U[0][1] * U[0][1]);
-        U[1][2] = (C[1][2] - U[0][1] * U[0][2]) / U[1][1];
-        U[2][2] = __builtin_sqrt((C[2][2] - U[0][2] * U[0][2]) - U[1][2] * U[1][2]);
+    const double c6[6] = { C[0][0], C[0][1], C[0][2], C[1][1], C[1][2], C[2][2] };
+    double u6[6];
+    if (fp64_ratio > 0.f && ill_conditioned(sc[0], sc[1], sc[2], kmod, fp64_ratio) && conic_factor_d(c6, u6)) {
+        const double U[3][3] = { { u6[0], u6[1], u6[2] }, { 0.0, u6[3], u6[4] }, { 0.0, 0.0, u6[5] } };
         for (int r = 0; r < 3; ++r)   // CG = U^T G_y
             for (int c = 0; c < 3; ++c) CG[r][c] = U[0][r] * G[0][c] + U[1][r] * G[1][c] + U[2][r] * G[2][c];
         for (int r = 0; r < 3; ++r)

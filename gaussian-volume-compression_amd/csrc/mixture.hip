@@ -270,8 +270,9 @@ __global__ __launch_bounds__(256) void inv_perm_kernel(int P, const uint32_t* __
 // conic writes), written as the record of BVH index inv_perm[j].  With fac_lo
 // (FP64 mode), a Gaussian on the precise path carries the hi parts of its conic's
 // Cholesky factor in the conic slots and kPreciseBit in its id; the lo parts go
-// to fac_lo[2j], [2j + 1], j = the caller's index (every Gaussian's pair is
-// written: no stale entries).
+// to fac_lo[2j], [2j + 1], j = the caller's index.  Only those entries are
+// written: a pair loop reads an entry only for an id with kPreciseBit, which
+// this same launch sets.
 __global__ __launch_bounds__(256) void record_kernel(int P, const uint32_t* __restrict__ inv_perm,
                                                      const float* __restrict__ means,
                                                      const float* __restrict__ scales, const float* __restrict__ rots,
@@ -299,9 +300,9 @@ __global__ __launch_bounds__(256) void record_kernel(int P, const uint32_t* __re
     r.hix = bx[3]; r.hiy = bx[4]; r.hiz = bx[5];
     r.pad0 = 0.f; r.pad1 = 0.f;
     rec[i] = r;
-    if (fac_lo) {
-        fac_lo[2 * (size_t)j] = precise ? make_float4(fl[0], fl[1], fl[2], fl[3]) : make_float4(0.f, 0.f, 0.f, 0.f);
-        fac_lo[2 * (size_t)j + 1] = precise ? make_float4(fl[4], fl[5], 0.f, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (precise) {
+        fac_lo[2 * (size_t)j] = make_float4(fl[0], fl[1], fl[2], fl[3]);
+        fac_lo[2 * (size_t)j + 1] = make_float4(fl[4], fl[5], 0.f, 0.f);
     }
     if (conics_out)
         for (int c = 0; c < 6; ++c) conics_out[6 * j + c] = cn[c];
@@ -1828,10 +1829,9 @@ __global__ __launch_bounds__(256) void reduce_long_kernel(const uint32_t* __rest
 // and mean, and the covariance chain (conic gradient -> Sigma -> M -> scales,
 // rotations) in double from the float inputs.  A zero conic gradient (no
 // records) gives zero scale and rotation gradients without the chain.
-// rec (FP64 mode only, else null): a record on the precise path has its conic
-// gradient summed in the factor frame.
-__global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __restrict__ inv_perm,
-                                                    const GRecord* __restrict__ rec,
+// fp64_ratio > 0 (FP64 mode): Gaussians on the precise path have their conic
+// gradient summed in the factor frame (covariance_chain repeats the test).
+__global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __restrict__ inv_perm, float fp64_ratio,
                                                     const double* __restrict__ sums, const float* __restrict__ scales,
                                                     const float* __restrict__ rots, float kmod,
                                                     float* __restrict__ dmeans, float* __restrict__ dscales,
@@ -1855,8 +1855,7 @@ __global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __res
     dmeans[3 * id + 2] = (float)m[4];
     double ds[3] = { 0.0, 0.0, 0.0 }, dq[4] = { 0.0, 0.0, 0.0, 0.0 };
     const bool any = m[5] != 0.0 || m[6] != 0.0 || m[7] != 0.0 || m[8] != 0.0 || m[9] != 0.0 || m[10] != 0.0;
-    const bool yframe = rec && rec[inv_perm[id]].id < 0;
-    if (any) covariance_chain(m + 5, scales + 3 * id, rots + 4 * id, kmod, ds, dq, yframe);
+    if (any) covariance_chain(m + 5, scales + 3 * id, rots + 4 * id, kmod, ds, dq, fp64_ratio);
     for (int c = 0; c < 3; ++c) dscales[3 * id + c] = (float)ds[c];
     for (int c = 0; c < 4; ++c) drots[4 * id + c] = (float)dq[c];
 }
@@ -2412,8 +2411,9 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
     GVC_LAUNCH_CHECK("reduce_sums");
     reduce_long_kernel<<<1024, 256, 0, st>>>(bl.begin, bl.end, svals, bl.parts, bl.long_list, bl.sums);
     GVC_LAUNCH_CHECK("reduce_long");
-    chain_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, l.fac_lo ? l.rec : nullptr, bl.sums, scales,
-                                                     rots, k, dmeans, dscales, drots, dvalues, dweights);
+    const float fp64_ratio = (prm->flags & GVC_FLAG_FP64_COND) ? fp64_ratio_of(prm) : 0.f;
+    chain_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, fp64_ratio, bl.sums, scales, rots, k, dmeans,
+                                                     dscales, drots, dvalues, dweights);
     GVC_LAUNCH_CHECK("chain");
     return GVC_OK;
 }
