@@ -348,17 +348,22 @@ def main():
             ss = mx.SampleSet(samples)
             c2, w2, _, _ = mx.mixture_forward(m, sc, rq, vv, wv, 1.0, ss)
         mh, sh, rh, wh, vh = (t.cpu().numpy() for t in (m, sc, rq, wv, vv))
-        conic, box, _ = oracle.preprocess(mh, sh, rh, wh)
+        # the oracle in the step's own mode (FP64 path and precise evaluation when on)
+        conic, box, _, fac = oracle.preprocess(mh, sh, rh, wh, 1.0, params.fp64_cond, params.fp64_scale_ratio,
+                                               factor=True)
+        fac = fac if params.fp64_cond else None
         sub = np.random.default_rng(0).choice(S, 1024, replace=False)
         pts = samples.cpu().numpy()[sub]
         if not args.no_error:
-            Vr, Wr, _ = oracle.forward(mh, vh, wh, conic, box, pts)
+            Vr, Wr, _ = oracle.forward(mh, vh, wh, conic, box, pts, oracle.conic_fp64(sh, rh), factor=fac)
             V, Wg = c2.cpu().numpy()[sub], w2.cpu().numpy()[sub]
             ok = Wr > 0
             flips = int(np.sum((Wg > 0) != ok))
             err = max(float(np.max(np.abs(V[ok] - Vr[ok]) / np.maximum(np.abs(Vr[ok]), 1e-3), initial=0)),
                       float(np.max(np.abs(Wg[ok] - Wr[ok]) / np.maximum(np.abs(Wr[ok]), 1e-3), initial=0)))
             out["rel_err_vs_fp64"] = {"field_max_rel": err, "sentinel_flips": flips, "points": int(len(sub))}
+            out["rel_err_vs_fp64"]["gradients"] = step_gradients((m, sc, rq, wv, vv), samples,
+                                                                 (mh, sh, rh, wh, vh), conic, box, fac, params)
         if not args.no_cpu_baseline:
             rng = np.random.default_rng(1)
             allpts = samples.cpu().numpy()
@@ -452,6 +457,50 @@ def precision_study(args, W, dev, n_points=65536, n_stress=16384):
     log(f"precision study: {json.dumps(out['sets'])}")
     log(f"precision study gradients: {json.dumps(out['gradients'])}")
     return out
+
+
+def step_gradients(dev_params, samples, host, conic, box, fac, params, n_check=256, seed=2):
+    """Gradient error of one forward + backward over the step's full point set
+    (random upstream gradients) against the FP64 oracle backward, for n_check
+    random Gaussians.  A Gaussian's gradient sees only the points inside its box,
+    so the oracle's V and W are evaluated at every step point inside the checked
+    Gaussians' boxes (gathered on the device), against all Gaussians.  Error per
+    parameter tensor: max |g - g*| / max |g*| over the checked Gaussians."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from gvc import mixture as mx
+
+    m, sc, rq, wv, vv = dev_params
+    mh, sh, rh, wh, vh = host
+    P, S, dev = mh.shape[0], samples.shape[0], samples.device
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    gV = torch.randn(S, device=dev, generator=gen)
+    gW = torch.randn(S, device=dev, generator=gen)
+    with torch.no_grad():
+        ss = mx.SampleSet(samples)
+        cells, cw, _, st = mx.mixture_forward(m, sc, rq, vv, wv, 1.0, ss, params, want_conics=False)
+        grads = mx.mixture_backward(st, m, sc, rq, vv, wv, 1.0, ss, cells, cw, gV, gW)
+        sub = np.random.default_rng(seed).choice(P, n_check, replace=False)
+        bx = torch.as_tensor(box[sub], device=dev)
+        inside = torch.zeros(S, dtype=torch.bool, device=dev)
+        for k in range(n_check):
+            inside |= ((samples >= bx[k, :3]) & (samples <= bx[k, 3:])).all(dim=1)
+        idx = inside.nonzero().squeeze(1)
+    pts = samples[idx].cpu().numpy()
+    gv, gw = gV[idx].double().cpu().numpy(), gW[idx].double().cpu().numpy()
+    Vr, Wr, _ = oracle.forward(mh, vh, wh, conic, box, pts, oracle.conic_fp64(sh, rh), factor=fac)
+    ref = oracle.backward(mh[sub], sh[sub], rh[sub], vh[sub], wh[sub], conic[sub], box[sub], pts, Vr, Wr, gv, gw,
+                          factor=fac[sub] if fac is not None else None)
+    errs = {}
+    for n, t in zip(["means", "scales", "rotations", "values", "weights"], grads):
+        a = t.detach().cpu().numpy().reshape(P, -1)[sub].astype(np.float64)
+        b = ref[n].reshape(n_check, -1)
+        errs[n] = float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+    return {"gaussians": int(n_check), "points": int(pts.shape[0]), "max_rel_by_tensor": errs,
+            "max_rel": max(errs.values())}
 
 
 def precision_gradients(W, modes, pre, cd, host, dev_params, pts_t, ill, seed, n_each=256):
