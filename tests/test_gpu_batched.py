@@ -103,6 +103,24 @@ def test_batch_equals_single_calls(log):
     check_against_singles(vols, mx.MixtureParams(candidate_log=log))
 
 
+def test_batch_precise_path_equals_single_calls():
+    """FP64 mode with Gaussians on the precise path in every volume (a tenth made
+    anisotropic, one scale divided by 3500 or 35, randomly rotated; threshold at
+    scale ratio 10): the batched launch reads each volume's factors at its own
+    Gaussians' entries, bit-identical to one call per volume."""
+    vols = [make_volume(*spec) for spec in VOLUMES]
+    for k, g in enumerate(vols):
+        rng = np.random.default_rng(100 + k)
+        P = g["means"].shape[0]
+        pick = rng.permutation(P)[:max(1, P // 10)]
+        ax = rng.integers(0, 3, pick.size)
+        div = np.where(np.arange(pick.size) % 2 == 0, 3500.0, 35.0)
+        g["scales"][pick, ax] = (g["scales"][pick, ax] / div).astype(f32)
+        q = rng.normal(size=(pick.size, 4))
+        g["rotations"][pick] = (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(f32)
+    check_against_singles(vols, mx.MixtureParams(fp64_cond=True, fp64_scale_ratio=10.0))
+
+
 @pytest.mark.parametrize("leaf,block", [(2, 512), (8, 1024)])
 def test_batch_launch_shapes(leaf, block):
     vols = [make_volume(*spec) for spec in VOLUMES[:2] + VOLUMES[3:]]
