@@ -52,9 +52,27 @@ def rasterize_gaussians(means3D, scales, rotations, values, weights, scale_modif
         torch.cuda.synchronize(ss.device)
         nrec, npairs, err = state.stats()
         print(f"Forward time: {1e3 * (time.perf_counter() - t0):.3f} ms, pairs {npairs}, partial records {nrec}")
+        if state.params.candidate_log:
+            _print_intersections(state, ss, use_gaussian_bvh)
     if return_state:
         return cells, cw, conics, state
     return cells, cw, conics
+
+
+def _print_intersections(state, ss, use_gaussian_bvh):
+    """The reference's debug statistics of accepted pairs (CR/rasterizer_impl.cu:
+    175-216): per sample with use_gaussian_bvh, else per Gaussian -- the maximum,
+    its first index and the mean."""
+    per_sample, per_gaussian = state.pair_counts(ss)
+    counts, what = (per_sample, "samples") if use_gaussian_bvh else (per_gaussian, "gaussians")
+    n = counts.numel()
+    if n == 0:
+        print(f"Intersections: max=0 ({what[:-1]} -1), avg=0.000 over 0 {what}")
+        return
+    mx = int(counts.max())
+    idx = int(torch.argmax(counts)) if mx > 0 else -1
+    avg = float(counts.to(torch.float64).sum()) / n
+    print(f"Intersections: max={mx} ({what[:-1]} {idx}), avg={avg:.3f} over {n} {what}")
 
 
 def rasterize_gaussians_backward(means3D, scales, rotations, conics, values, weights, out_cells, out_weights,

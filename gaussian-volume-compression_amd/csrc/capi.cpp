@@ -36,6 +36,10 @@ int forward_stats(const void* state, int P, int S, const gvc_params* prm, uint64
                   uint32_t* error_flags, uint64_t* chunk_slots_used, uint64_t* lane_groups,
                   hipStream_t st);
 size_t backward_bytes(int P, uint64_t cap);
+int forward_tree(const void* state, int P, int S, const gvc_params* prm, uint32_t* order, uint32_t* keys,
+                 int32_t* links, float* boxes, hipStream_t st);
+int forward_pair_counts(const void* state, int P, int S, const gvc_params* prm, const void* sample_set,
+                        uint32_t* sample_counts, uint32_t* gaussian_counts, hipStream_t st);
 int forward_accumulate(const void* state, int P, int S, const gvc_params* prm, unsigned long long* accum,
                        hipStream_t st);
 int mixture_backward(int P, const float* means, const float* scales, const float* rots, const float* values,
@@ -63,6 +67,8 @@ int forward_stats_batched(const gvc_batch* b, const void* state, const gvc_param
                           hipStream_t st);
 int forward_accumulate_batched(const gvc_batch* b, const void* state, const gvc_params* prm,
                                unsigned long long* accum, hipStream_t st);
+int forward_pair_counts_batched(const gvc_batch* b, const void* state, const gvc_params* prm, const void* sample_set,
+                                uint32_t* sample_counts, uint32_t* gaussian_counts, hipStream_t st);
 int mixture_backward_batched(const gvc_batch* b, const float* scales, const float* rots, float k,
                              const void* sample_set, const gvc_params* prm, const float* out_cells,
                              const float* out_weights, const float* gcells, const float* gweights, void* state,
@@ -216,6 +222,21 @@ int gvc_forward_accumulate(const void* state, int P, int S, const gvc_params* pa
 {
     GVC_REQUIRE(state && accum, "NULL pointer");
     return forward_accumulate(state, P, S, params, reinterpret_cast<unsigned long long*>(accum), (hipStream_t)stream);
+}
+
+int gvc_forward_tree(const void* state, int P, int S, const gvc_params* params, uint32_t* order, uint32_t* keys,
+                     int32_t* nodes, float* boxes, void* stream)
+{
+    GVC_REQUIRE(state && params, "NULL pointer");
+    return forward_tree(state, P, S, params, order, keys, nodes, boxes, (hipStream_t)stream);
+}
+
+int gvc_forward_pair_counts(const void* state, int P, int S, const gvc_params* params, const void* sample_set,
+                            uint32_t* sample_counts, uint32_t* gaussian_counts, void* stream)
+{
+    GVC_REQUIRE(state && params && sample_set && (S == 0 || sample_counts) && (P == 0 || gaussian_counts),
+                "NULL pointer");
+    return forward_pair_counts(state, P, S, params, sample_set, sample_counts, gaussian_counts, (hipStream_t)stream);
 }
 
 size_t gvc_backward_bytes(int P, uint64_t record_capacity)
@@ -373,6 +394,18 @@ int gvc_forward_stats_batched(const gvc_batch* batch, const void* state, const g
     GVC_REQUIRE(state, "NULL state");
     return forward_stats_batched(batch, state, params, n_records, n_pairs, error_flags, chunk_slots_used,
                                  lane_groups, (hipStream_t)stream);
+}
+
+int gvc_forward_pair_counts_batched(const gvc_batch* batch, const void* state, const gvc_params* params,
+                                    const void* sample_set, uint32_t* sample_counts, uint32_t* gaussian_counts,
+                                    void* stream)
+{
+    GVC_REQUIRE_BATCH(batch);
+    GVC_REQUIRE(state && params && sample_set && (batch->S_total == 0 || sample_counts) &&
+                    (batch->P_total == 0 || gaussian_counts),
+                "NULL pointer");
+    return forward_pair_counts_batched(batch, state, params, sample_set, sample_counts, gaussian_counts,
+                                       (hipStream_t)stream);
 }
 
 int gvc_forward_accumulate_batched(const gvc_batch* batch, const void* state, const gvc_params* params,

@@ -511,6 +511,39 @@ __global__ void leaf_root_kernel(const Box6* __restrict__ t0, BatchView bv, floa
     }
 }
 
+// The tree of a forward state, copied out (gvc_forward_tree): per BVH position
+// the caller's index and the 32-bit Morton key (recomputed from the record's
+// mean, as gaussian_codes_kernel formed it); per internal node its links, leaf
+// range and both child boxes.
+__global__ void tree_export_kernel(const GRecord* __restrict__ rec, int P, const BNode* __restrict__ nodes, int NI,
+                                   uint32_t* __restrict__ order, uint32_t* __restrict__ keys,
+                                   int32_t* __restrict__ links, float* __restrict__ boxes)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P) {
+        const GRecord& r = rec[i];
+        if (order) order[i] = (uint32_t)r.id & ~kPreciseBit;
+        if (keys)
+            keys[i] = (uint32_t)(morton3d_unit(r.mx, r.my, r.mz, false) >> (63 - kGaussianCodeBits));
+    }
+    if (i < NI) {
+        const BNode& n = nodes[i];
+        if (links) {
+            links[4 * (size_t)i] = n.left;
+            links[4 * (size_t)i + 1] = n.right;
+            links[4 * (size_t)i + 2] = n.pad0;
+            links[4 * (size_t)i + 3] = n.pad1;
+        }
+        if (boxes)
+            for (int c = 0; c < 3; ++c) {
+                boxes[12 * (size_t)i + c] = n.llo[c];
+                boxes[12 * (size_t)i + 3 + c] = n.lhi[c];
+                boxes[12 * (size_t)i + 6 + c] = n.rlo[c];
+                boxes[12 * (size_t)i + 9 + c] = n.rhi[c];
+            }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // group walk: shared by the forward and the backward
 //
@@ -1353,6 +1386,80 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
         }
     }
     if (!ok && lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_STACK_OVERFLOW);
+}
+
+// ---------------------------------------------------------------------------
+// Debug counts (gvc_forward_pair_counts): accepted pairs per sample and per
+// Gaussian, the reference's count_intersections (CR/forward.cu:180, :257)
+// printed by its debug mode (CR/rasterizer_impl.cu:175-216).  One wave per group
+// replays the group's candidate log -- lane p reads its point's row of box hits
+// and evaluates the predicate for each hit candidate -- or, for a lane-walked
+// group, walks the tree per lane again.  Integer atomics only (the totals do not
+// depend on their order).  Not on the hot path: the walks count nothing per
+// Gaussian.
+// ---------------------------------------------------------------------------
+template <bool PRECISE>
+__global__ __launch_bounds__(256) void pair_count_kernel(const GRecord* __restrict__ rec,
+                                                         const float4* __restrict__ fac_lo,
+                                                         const BNode* __restrict__ nodes,
+                                                         const float* __restrict__ root_box, BatchView bv, int L,
+                                                         const float4* __restrict__ pts, int NG,
+                                                         const unsigned char* __restrict__ log,
+                                                         const uint32_t* __restrict__ group_head,
+                                                         uint32_t* __restrict__ sample_counts,
+                                                         uint32_t* __restrict__ gaussian_counts,
+                                                         uint32_t* __restrict__ err)
+{
+    __shared__ float4 cand[4][64][3];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = (int)blockIdx.x * 4 + wid;
+    if (g >= NG) return;
+    const uint32_t head = (uint32_t)uniformi((int)group_head[g]);
+    const float4 pme = pts[(size_t)g * 64 + lane];
+    const int orig = __builtin_bit_cast(int, pme.w);
+    uint32_t count = 0;
+    auto tally = [&](bool hit, const float4& C) {
+        if (!hit) return;
+        ++count;
+        atomicAdd(&gaussian_counts[(uint32_t)__builtin_bit_cast(int, C.w) & ~kPreciseBit], 1u);
+    };
+    if (head == kLogIncomplete) {
+        if (lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_CHUNK_OVERFLOW);
+    } else if (head == kLogLaneMode) {
+        const int vol = bv.V == 1 ? 0 : uniformi(volume_of(bv.G_off, bv.V, g));
+        const TreeView tv = tree_of(bv, root_box, vol);
+        const bool active = orig >= 0 && pme.x == pme.x && pme.y == pme.y && pme.z == pme.z;
+        const bool ok = lane_walk<PRECISE>(rec, fac_lo, nodes, tv, L, pme.x, pme.y, pme.z, active,
+                                           [&](bool hit, int, const float4&, const float4&, const float4& C, float,
+                                               float, float, float, float, float, float) { tally(hit, C); });
+        if (!ok) atomicOr(err, (uint32_t)GVC_ERRFLAG_STACK_OVERFLOW);
+    } else {
+        for (uint32_t slot = head; slot != kLogEnd;) {
+            const unsigned char* sb = log + (size_t)slot * kSlotBytes;
+            const uint32_t nxt = (uint32_t)uniformi((int)reinterpret_cast<const uint32_t*>(sb)[0]);
+            const int n = uniformi((int)reinterpret_cast<const uint32_t*>(sb)[1]);
+            if (lane < n) {
+                const float4* q = reinterpret_cast<const float4*>(&rec[reinterpret_cast<const int*>(sb + 64)[lane]]);
+                cand[wid][lane][0] = q[0];
+                cand[wid][lane][1] = q[1];
+                cand[wid][lane][2] = q[2];
+            }
+            const uint2 row = reinterpret_cast<const uint2*>(sb + 320)[lane];
+            uint64_t m = ((uint64_t)row.y << 32) | row.x;
+            lds_fence();
+            while (m != 0ull) {
+                const int j = __builtin_ctzll(m);
+                m &= m - 1;
+                const float4 A = cand[wid][j][0], B = cand[wid][j][1], C = cand[wid][j][2];
+                float dx, dy, dz, r0, r1, r2;
+                const float power = record_power<PRECISE>(A, B, C, fac_lo, pme.x, pme.y, pme.z, dx, dy, dz, r0, r1, r2);
+                tally(in_window(power), C);
+            }
+            lds_fence();
+            slot = nxt;
+        }
+    }
+    if (orig >= 0) sample_counts[orig] = count;
 }
 
 // ---------------------------------------------------------------------------
@@ -2330,6 +2437,30 @@ int forward_stats_impl(const Batch& bt, const void* state, const gvc_params* prm
     return GVC_OK;
 }
 
+int forward_pair_counts_impl(const Batch& bt, const void* state, const gvc_params* prm, const void* sample_set,
+                             uint32_t* sample_counts, uint32_t* gaussian_counts, hipStream_t st)
+{
+    GVC_TRY(check_params(prm));
+    if (!(prm->flags & GVC_FLAG_CANDIDATE_LOG))
+        return set_error(GVC_ERR_UNSUPPORTED, "pair counts replay the candidate log: the forward needs GVC_FLAG_CANDIDATE_LOG");
+    Arena a(const_cast<void*>(state), (size_t)-1);
+    ForwardLayout l(a, bt, prm->flags, prm->chunk_slots);
+    if (bt.S > 0) GVC_TRY(fill_u32(sample_counts, 0u, (size_t)bt.S, st));
+    if (bt.P > 0) GVC_TRY(fill_u32(gaussian_counts, 0u, (size_t)bt.P, st));
+    if (bt.P == 0 || bt.NG == 0) return GVC_OK;
+    const float4* pts = sample_set_points_impl(sample_set, bt.S, bt.NG);
+    const BatchView bv = batch_view(table_of(bt, l.table), bt.V);
+    const unsigned nb = blocks_for((size_t)bt.NG, 4);
+    if (l.fac_lo)
+        pair_count_kernel<true><<<nb, 256, 0, st>>>(l.rec, l.fac_lo, l.nodes, l.root_box, bv, bt.L, pts, bt.NG, l.log,
+                                                   l.group_head, sample_counts, gaussian_counts, l.err);
+    else
+        pair_count_kernel<false><<<nb, 256, 0, st>>>(l.rec, l.fac_lo, l.nodes, l.root_box, bv, bt.L, pts, bt.NG, l.log,
+                                                    l.group_head, sample_counts, gaussian_counts, l.err);
+    GVC_LAUNCH_CHECK("pair_count");
+    return GVC_OK;
+}
+
 int forward_accumulate_impl(const Batch& bt, const void* state, const gvc_params* prm, unsigned long long* accum,
                             hipStream_t st)
 {
@@ -2486,6 +2617,32 @@ int forward_accumulate(const void* state, int P, int S, const gvc_params* prm, u
     return forward_accumulate_impl(single_batch(P, S, prm->leaf_size, nullptr), state, prm, accum, st);
 }
 
+int forward_pair_counts(const void* state, int P, int S, const gvc_params* prm, const void* sample_set,
+                        uint32_t* sample_counts, uint32_t* gaussian_counts, hipStream_t st)
+{
+    GVC_TRY(check_params(prm));
+    if (P < 0 || S < 0) return set_error(GVC_ERR_INVALID_ARGUMENT, "negative size");
+    return forward_pair_counts_impl(single_batch(P, S, prm->leaf_size, nullptr), state, prm, sample_set,
+                                    sample_counts, gaussian_counts, st);
+}
+
+int forward_tree(const void* state, int P, int S, const gvc_params* prm, uint32_t* order, uint32_t* keys,
+                 int32_t* links, float* boxes, hipStream_t st)
+{
+    GVC_TRY(check_params(prm));
+    if (P < 0 || S < 0) return set_error(GVC_ERR_INVALID_ARGUMENT, "negative size");
+    const Batch bt = single_batch(P, S, prm->leaf_size, nullptr);
+    Arena a(const_cast<void*>(state), forward_state_bytes_impl(bt, prm));
+    ForwardLayout l(a, bt, prm->flags, prm->chunk_slots);
+    const int n = P > bt.NI ? P : bt.NI;
+    if (n > 0) {
+        tree_export_kernel<<<blocks_for((size_t)n, 256), 256, 0, st>>>(l.rec, P, l.nodes, bt.NI, order, keys, links,
+                                                                       boxes);
+        GVC_LAUNCH_CHECK("tree_export");
+    }
+    return GVC_OK;
+}
+
 size_t backward_bytes(int P, uint64_t cap)
 {
     Arena a(nullptr, 0);
@@ -2582,6 +2739,14 @@ int forward_stats_batched(const gvc_batch* b, const void* state, const gvc_param
 {
     return forward_stats_impl(from_c(b), state, prm, n_records, n_pairs, error_flags, chunk_slots_used, lane_groups,
                               st);
+}
+
+int forward_pair_counts_batched(const gvc_batch* b, const void* state, const gvc_params* prm, const void* sample_set,
+                                uint32_t* sample_counts, uint32_t* gaussian_counts, hipStream_t st)
+{
+    if (prm->leaf_size != b->leaf_size)
+        return set_error(GVC_ERR_INVALID_ARGUMENT, "params.leaf_size differs from the batch's");
+    return forward_pair_counts_impl(from_c(b), state, prm, sample_set, sample_counts, gaussian_counts, st);
 }
 
 int forward_accumulate_batched(const gvc_batch* b, const void* state, const gvc_params* prm,

@@ -8,7 +8,8 @@
 // values = sigmoid.  One thread per Gaussian, forward and backward.
 //
 // Loss (R/train.py:389-420, R/utils/loss_utils.py:19-20):
-//   recon = (gt != -1) & (cells != -1);  l1 = mean over recon of |cells - gt|
+//   recon = (gt != -1) & (cells != -1);  l1 = mean over recon of |cells - gt|,
+//           or, when encode_surface, the mean over every sample (R/train.py:391-392)
 //   fn    = (gt != -1) & (w > 0) & (w < t_fn),  t_fn = 0.0105 (surface) / 0.011
 //   FN    = fn_reg * sum(clamp(0.011 - w[fn], 0)) / (#(vals > 0) + 1e-8)
 //   FP    = fp_reg * sum(w[(gt == -1) & (w > 0)]) / (#(vals > 0) + 1e-8)   (surface only)
@@ -110,10 +111,13 @@ struct LossTerms {
     double l1, fnv, fpv;
 };
 
+// recon: the sample enters the L1 mean -- every sample when encode_surface
+// (torch.abs(cells - gt).mean(), R/train.py:391-392), else the masked ones
+// (R/train.py:389, :394).
 __device__ __forceinline__ LossTerms loss_terms(float c, float w, float g, int surface)
 {
     LossTerms t;
-    t.recon = (g != -1.0f) && (c != -1.0f);
+    t.recon = surface || ((g != -1.0f) && (c != -1.0f));
     t.l1 = t.recon ? fabs((double)(c - g)) : 0.0;
     const float tf = surface ? 0.0105f : 0.011f;
     t.fn = (g != -1.0f) && (w > 0.0f) && (w < tf);

@@ -1,7 +1,8 @@
 """Drop-in for the reference's ``gpu_mesh_sampling`` (GS/gpu_mesh_sampling/__init__.py:1-20),
 running on libgvc instead of Viskores.
 
-``gpu_sample`` and ``gpu_sampleu`` keep their signatures and numpy-in/numpy-out
+``gpu_sample`` and ``gpu_sampleu``, and the native module's ``sample_mesh`` and
+``sample_meshu`` (``_gpu_mesh_sampling``), keep their signatures and numpy-in/numpy-out
 behaviour (torch tensors are accepted too and give torch tensors back, without
 leaving the device).  Errors mirror GS/sample_mesh.cu: RuntimeError for a values
 size mismatch, an unsupported values dtype or an unsupported cell type.  Nothing
@@ -14,43 +15,16 @@ Cell types: the reference's table (GS/sample_mesh.cu:300-336).  Tetrahedra (VTK
 faces) and interpolated by their VTK shape functions; vertices, lines, triangles
 and quads (1, 3, 5, 9) have no volume and hold no sample.  Any other type raises.
 """
-import numpy as np
-import torch
-
 from gvc.sampling import CellMeshSampler, TetMeshSampler, grid_sample  # noqa: F401
 
-
-def _device():
-    if not torch.cuda.is_available():
-        raise RuntimeError("gpu_mesh_sampling needs a ROCm GPU")
-    return torch.device("cuda", torch.cuda.current_device())
+from ._gpu_mesh_sampling import sample_mesh, sample_meshu  # noqa: F401  (GS/gpu_mesh_sampling/__init__.py:1)
 
 
 def gpu_sample(dims, origin, spacing, values, samples):
-    """GS/sample_mesh.cu:26-240: trilinear probe of a uniform point grid; -1 outside.
-    Output dtype follows ``values`` (float64, float32, uint16, uint8)."""
-    as_torch = isinstance(samples, torch.Tensor)
-    dev = samples.device if as_torch and samples.is_cuda else _device()
-    if isinstance(values, torch.Tensor):
-        v = values.reshape(-1).to(dev)
-    else:
-        va = np.ascontiguousarray(values)
-        if va.dtype not in (np.float64, np.float32, np.uint16, np.uint8):
-            raise RuntimeError("Unsupported values dtype. Expected float64, float32, uint16, or uint8.")
-        v = torch.from_numpy(va.reshape(-1)).to(dev)
-    s = samples if as_torch else torch.from_numpy(np.ascontiguousarray(samples, dtype=np.float32))
-    out = grid_sample(np.asarray(dims), np.asarray(origin), np.asarray(spacing), v, s.to(dev))
-    return out if as_torch else out.cpu().numpy()
+    """GS/gpu_mesh_sampling/__init__.py:4-11: ``sample_mesh`` of the native module."""
+    return sample_mesh(dims, origin, spacing, values, samples)
 
 
 def gpu_sampleu(pts, conn, cell_types, cell_offsets, values, samples):
-    """GS/sample_mesh.cu:242-414: probe an unstructured mesh at the samples;
-    float64 values, -1 outside."""
-    as_torch = isinstance(samples, torch.Tensor)
-    dev = samples.device if as_torch and samples.is_cuda else _device()
-    p = pts if isinstance(pts, torch.Tensor) else np.ascontiguousarray(pts, dtype=np.float32)
-    v = values if isinstance(values, torch.Tensor) else np.ascontiguousarray(values, dtype=np.float64)
-    sampler = CellMeshSampler(p, conn, cell_types, cell_offsets, v, device=dev)
-    s = samples if as_torch else torch.from_numpy(np.ascontiguousarray(samples, dtype=np.float32))
-    out, _ = sampler.sample(s.to(dev))
-    return out if as_torch else out.cpu().numpy()
+    """GS/gpu_mesh_sampling/__init__.py:13-21: ``sample_meshu`` of the native module."""
+    return sample_meshu(pts, conn, cell_types, cell_offsets, values, samples)

@@ -90,6 +90,8 @@ _SIGS = {
     "gvc_mixture_forward": (_I, [_I, _P, _P, _P, _P, _P, _F, _P, _I, _PP, _P, _P, _P, _P, _SZ, _P]),
     "gvc_forward_stats": (_I, [_P, _I, _I, _PP, ctypes.POINTER(_U64), ctypes.POINTER(_U64),
                                ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_U64), ctypes.POINTER(_U64), _P]),
+    "gvc_forward_tree": (_I, [_P, _I, _I, _PP, _P, _P, _P, _P, _P]),
+    "gvc_forward_pair_counts": (_I, [_P, _I, _I, _PP, _P, _P, _P, _P]),
     "gvc_backward_bytes": (_SZ, [_I, _U64]),
     "gvc_forward_accumulate": (_I, [_P, _I, _I, _PP, _P, _P]),
     "gvc_mixture_backward": (_I, [_I, _P, _P, _P, _P, _P, _F, _P, _I, _PP, _P, _P, _P, _P, _P, _SZ, _U64,
@@ -112,6 +114,7 @@ _SIGS = {
                                        ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_U64), ctypes.POINTER(_U64),
                                        _P]),
     "gvc_forward_accumulate_batched": (_I, [_PB, _P, _PP, _P, _P]),
+    "gvc_forward_pair_counts_batched": (_I, [_PB, _P, _PP, _P, _P, _P, _P]),
     "gvc_mixture_backward_batched": (_I, [_PB, _P, _P, _F, _P, _PP, _P, _P, _P, _P, _P, _SZ, _U64, _P, _SZ,
                                           _P, _P, _P, _P, _P, _P]),
     "gvc_activate_forward": (_I, [_I, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P]),

@@ -95,6 +95,16 @@ class BatchedForwardState:
         self.lane_groups = int(lanes.value)
         return int(nr.value), int(npairs.value), int(err.value)
 
+    def pair_counts(self, sample_set):
+        """Accepted pairs per sample [S_total] and per Gaussian [P_total] (int32
+        device tensors, volume-major positions; needs the candidate log)."""
+        dev = self.buffer.device
+        sc = torch.empty(self.S, dtype=torch.int32, device=dev)
+        gc = torch.empty(self.P, dtype=torch.int32, device=dev)
+        call("gvc_forward_pair_counts_batched", ctypes.byref(self.batch.c), _ptr(self.buffer),
+             ctypes.byref(self._c), _ptr(sample_set.buffer), _ptr(sc), _ptr(gc), _stream(dev))
+        return sc, gc
+
     def accumulate(self, accum):
         """Stream-ordered accum (int64 [3]) += (pairs, records), accum[2] |= flags."""
         call("gvc_forward_accumulate_batched", ctypes.byref(self.batch.c), _ptr(self.buffer), ctypes.byref(self._c),

@@ -167,6 +167,33 @@ class ForwardState:
         self.lane_groups = int(lanes.value)
         return int(nr.value), int(npairs.value), int(err.value)
 
+    def pair_counts(self, sample_set):
+        """Accepted pairs per sample [S] and per Gaussian [P] (int32 device
+        tensors; gvc_forward_pair_counts, needs the candidate log).  Debug only."""
+        dev = self.buffer.device
+        sc = torch.empty(self.S, dtype=torch.int32, device=dev)
+        gc = torch.empty(self.P, dtype=torch.int32, device=dev)
+        call("gvc_forward_pair_counts", _ptr(self.buffer), self.P, self.S, ctypes.byref(self._c),
+             _ptr(sample_set.buffer), _ptr(sc), _ptr(gc), _stream(dev))
+        return sc, gc
+
+    def tree(self):
+        """The Gaussian BVH of this forward (gvc_forward_tree): dict of device
+        tensors order [P] int32 (caller index per BVH position), keys [P] int64
+        (the 32-bit Morton keys), nodes [NL-1, 4] int32 (left, right, first leaf,
+        last leaf; a child < 0 is leaf ~c), boxes [NL-1, 12] float32 (left child's
+        lo.xyz hi.xyz, right child's lo.xyz hi.xyz).  Stream-ordered."""
+        dev = self.buffer.device
+        NL = (self.P + self._c.leaf_size - 1) // self._c.leaf_size if self.P > 0 else 0
+        NI = max(NL - 1, 0)
+        order = torch.empty(self.P, dtype=torch.int32, device=dev)
+        keys = torch.empty(self.P, dtype=torch.int32, device=dev)
+        nodes = torch.empty((NI, 4), dtype=torch.int32, device=dev)
+        boxes = torch.empty((NI, 12), dtype=torch.float32, device=dev)
+        call("gvc_forward_tree", _ptr(self.buffer), self.P, self.S, ctypes.byref(self._c), _ptr(order), _ptr(keys),
+             _ptr(nodes), _ptr(boxes), _stream(dev))
+        return dict(order=order, keys=keys.to(torch.int64) & 0xffffffff, nodes=nodes, boxes=boxes)
+
     def accumulate(self, accum):
         """Stream-ordered: accum (int64 [3] on the device) += (pairs, records),
         accum[2] |= error flags.  No host synchronisation (graph-capturable)."""

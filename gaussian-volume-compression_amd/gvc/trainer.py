@@ -4,8 +4,9 @@ replayed from a graph.
 Per iteration, as the reference: pick one of the precomputed sample batches at
 random (``np.random.randint``, :354-362), build the sample set (``build_bvh``),
 advance the position learning rate, evaluate the mixture, take the fitting loss
-(L1 over the reconstructed samples + fn_reg x the false-negative term, + fp_reg x
-the false-positive term when encoding the surface), back-propagate, switch fn_reg
+(L1 over the reconstructed samples -- over every sample when encoding the
+surface -- + fn_reg x the false-negative term, + fp_reg x the false-positive term
+when encoding the surface), back-propagate, switch fn_reg
 to fn_reg2 after densify_from_iter, densify and prune at the densification
 interval (:551-590), else take an Adam step (:593-595; at a densification
 iteration the reference's parameters were replaced, so its step has no gradients

@@ -147,6 +147,36 @@ int gvc_forward_stats(const void *state, int P, int S, const gvc_params *params,
 int gvc_forward_accumulate(const void *state, int P, int S, const gvc_params *params, uint64_t *accum,
                            void *stream);
 
+/* The Gaussian BVH a forward built into `state` (single volume), copied out for
+ * inspection: the counterpart of the reference's static Gaussian BVH
+ * (DGR/ext.cu:8-10, :39-42), which cuBQL's radixBuilder builds over the Gaussian
+ * boxes (CR/rasterizer_impl.cu:78-83).  Here the tree is a Karras (2012) radix tree
+ * over leaves of params->leaf_size consecutive Gaussians in Morton order.
+ *   order [P] u32: the caller's index of the Gaussian at each BVH position (stable
+ *         sort of the keys, equal keys in caller order); leaf j holds positions
+ *         [j L, min((j + 1) L, P));
+ *   keys  [P] u32: the sorted keys, the top 32 bits of morton3d_unit of the means;
+ *   nodes [NL - 1][4] i32 (NL = ceil(P / L) leaves; none when NL < 2): left child,
+ *         right child (>= 0 internal node, < 0 leaf ~j), first and last leaf under
+ *         the node; node 0 is the root.  Ties between equal leaf keys are broken by
+ *         leaf index;
+ *   boxes [NL - 1][12] f32: the left child's box lo.xyz hi.xyz, then the right
+ *         child's: the union (fminf/fmaxf, NaN boxes skipped) of the boxes of the
+ *         Gaussians under it.
+ * Any output may be NULL.  Stream-ordered, no host synchronisation. */
+int gvc_forward_tree(const void *state, int P, int S, const gvc_params *params, uint32_t *order, uint32_t *keys,
+                     int32_t *nodes, float *boxes, void *stream);
+
+/* Debug counts of a forward (state from a forward with GVC_FLAG_CANDIDATE_LOG,
+ * with the same sample set): sample_counts [S] u32 = accepted pairs per sample
+ * (original sample order), gaussian_counts [P] u32 = accepted pairs per Gaussian
+ * (caller order).  The reference's count_intersections (CR/forward.cu:180, :257),
+ * which its debug mode reduces to max/mean (CR/rasterizer_impl.cu:175-216).  Replays
+ * the candidate log (or re-walks lane-walked groups); stream-ordered;
+ * GVC_ERR_UNSUPPORTED without the candidate log. */
+int gvc_forward_pair_counts(const void *state, int P, int S, const gvc_params *params, const void *sample_set,
+                            uint32_t *sample_counts, uint32_t *gaussian_counts, void *stream);
+
 /* Backward: gradients for means3D [P,3], scales [P,3], rotations [P,4], values
  * [P,1], weights [P,1] from dL/dout_cells, dL/dout_cell_weights [S].  Replaces
  * _C.rasterize_gaussians_backward, DGR/ext.cu:107-147, DGR/rasterize_points.cu:75-137,
@@ -261,6 +291,11 @@ int gvc_forward_stats_batched(const gvc_batch *batch, const void *state, const g
                               uint64_t *chunk_slots_used, uint64_t *lane_groups, void *stream);
 int gvc_forward_accumulate_batched(const gvc_batch *batch, const void *state, const gvc_params *params,
                                    uint64_t *accum, void *stream);
+/* gvc_forward_pair_counts over a batch: counts at the concatenated (volume-major)
+ * sample and Gaussian positions. */
+int gvc_forward_pair_counts_batched(const gvc_batch *batch, const void *state, const gvc_params *params,
+                                    const void *sample_set, uint32_t *sample_counts, uint32_t *gaussian_counts,
+                                    void *stream);
 int gvc_mixture_backward_batched(const gvc_batch *batch, const float *scales, const float *rotations,
                                  float scale_modifier, const void *sample_set, const gvc_params *params,
                                  const float *out_cells, const float *out_weights, const float *dL_dout_cells,
@@ -286,7 +321,8 @@ int gvc_activate_backward(int P, const float *log_scales, const float *rotations
                           float *dL_dvalue_logits, void *stream);
 
 /* Fitting loss of R/train.py:389-420 (l1_loss: R/utils/loss_utils.py:19-20):
- * L1 over (gt != -1) & (cells != -1) as a mean, plus fn_reg x the mean positive
+ * L1 over (gt != -1) & (cells != -1) as a mean (over every sample when
+ * encode_surface, R/train.py:391-392), plus fn_reg x the mean positive
  * clamp(0.011 - w) over (gt != -1) & (0 < w < t) with t = 0.011 (0.0105 when
  * encode_surface), plus, when encode_surface, fp_reg x the mean positive w over
  * (gt == -1) & (w > 0).  loss: one float on the device.  Sums in FP64 in a fixed

@@ -13,7 +13,12 @@ Layout:
                    autograd gradients and frozen-predicate central differences.
   mesh_ref.py      exact tet location (rational orient3d + simulation of
                    simplicity), trilinear grid sampling, brute-force 3-NN.
-  workloads.py     seeded synthetic inputs of the benchmark configurations.
+  train_ref.py     the reference's training-loop procedures on torch tensors
+                   (learning-rate schedule, densify/prune surgery).
+  eval_ref.py      the reference's evaluation grid construction and metrics.
+
+The seeded synthetic inputs of the benchmark configurations are product code
+(``gvc/workloads.py``), shared by the tests and ``bench.py``.
 
 The reference itself cannot be built here (it needs nvcc, cuBQL, glm and
 Viskores), so there is no ``_ref/`` build.

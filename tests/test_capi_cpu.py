@@ -40,6 +40,22 @@ def test_every_declared_symbol_is_exported(L):
     assert not missing, missing
 
 
+def test_dropin_package_names():
+    """The names the reference's three Python packages export (DGR/bvh_diff_gaussian_
+    rasterization/__init__.py, GS/gpu_mesh_sampling/__init__.py:1-21, GS/ext.cpp:4-7,
+    simple-knn's _C.distCUDA2) import without a GPU."""
+    import bvh_diff_gaussian_rasterization as dgr
+    import gpu_mesh_sampling as gms
+    from gpu_mesh_sampling import _gpu_mesh_sampling
+    from simple_knn._C import distCUDA2  # noqa: F401
+    for n in ("GaussianRasterizationSettings", "GaussianRasterizer", "rasterize_gaussians"):
+        assert hasattr(dgr, n)
+    for n in ("gpu_sample", "gpu_sampleu", "sample_mesh", "sample_meshu"):
+        assert callable(getattr(gms, n))
+    assert gms.sample_mesh is _gpu_mesh_sampling.sample_mesh
+    assert gms.sample_meshu is _gpu_mesh_sampling.sample_meshu
+
+
 def test_every_declared_symbol_has_a_binding():
     from gvc import _lib
     missing = [n for n in declared_functions() if n not in _lib.EXPORTED]

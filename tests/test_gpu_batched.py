@@ -1,18 +1,21 @@
 """Batches of independent volumes: one launch per pass over V volumes.
 
-Bar: bit-exact.  Every volume's cells, cell weights, conics and gradients in a
-batch equal the single-volume calls on that volume's inputs, bit for bit (the
-single-volume path is itself checked against the oracle in test_gpu_mixture.py),
-and pair counts add up exactly.
+Bars: bit-exact.  Every volume's cells, cell weights, conics and gradients in a
+batch equal the single-volume calls on that volume's inputs, bit for bit, and
+pair counts add up exactly.  Each volume's batched results are also checked
+against the FP64 oracle directly (the precise path's factors included): pair
+count exact, field within 1e-5 relative, gradients within 1e-5 normwise.
 """
 import numpy as np
 import pytest
 
-from conftest import gpu_available
+from conftest import assert_grads, gpu_available
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
 
 import torch  # noqa: E402
+
+import oracle  # noqa: E402
 
 from gvc import batched as bx  # noqa: E402
 from gvc import mixture as mx  # noqa: E402
@@ -81,12 +84,39 @@ def upstream(vols, seed):
     return gVs, gWs
 
 
+def check_against_oracle(g, params, outs, gV, gW):
+    """One volume's batched cells and gradients against the FP64 oracle (with the
+    precise path's factors when params.fp64_cond)."""
+    if g["means"].shape[0] == 0:
+        return
+    conic, box, _, fac = oracle.preprocess(g["means"], g["scales"], g["rotations"], g["weights"], 1.0,
+                                           params.fp64_cond, params.fp64_scale_ratio, factor=True)
+    fac = fac if params.fp64_cond else None
+    cd = oracle.conic_fp64(g["scales"], g["rotations"])
+    Vr, Wr, cnt = oracle.forward(g["means"], g["values"], g["weights"], conic, box, g["samples"], cd, factor=fac)
+    V, W = outs[0].cpu().numpy(), outs[1].cpu().numpy()
+    assert np.array_equal(W > 0, Wr > 0)
+    ok = Wr > 0
+    err = np.maximum(np.abs(V[ok] - Vr[ok]) / np.maximum(np.abs(Vr[ok]), 1e-3),
+                     np.abs(W[ok] - Wr[ok]) / np.maximum(np.abs(Wr[ok]), 1e-3))
+    assert err.max(initial=0.0) <= 1e-5
+    ref = oracle.backward(g["means"], g["scales"], g["rotations"], g["values"], g["weights"], conic, box,
+                          g["samples"], Vr, Wr, gV.double().cpu().numpy(), gW.double().cpu().numpy(), factor=fac)
+    if any(np.abs(ref[n]).max() > 0 for n in ("means", "values")):
+        assert_grads(outs[2:], ref)
+    return int(cnt.sum())
+
+
 def check_against_singles(vols, params, seed=11):
     gVs, gWs = upstream(vols, seed)
     batch, outs, npairs = batched(vols, params, gVs, gWs)
     total = 0
     for v, g in enumerate(vols):
+        ss_, gs_ = batch.sample_slice(v), batch.gaussian_slice(v)
+        n_or = check_against_oracle(g, params, [outs[0][ss_], outs[1][ss_]] + [t[gs_] for t in outs[3:]],
+                                    gVs[v], gWs[v])
         ref, np_v = single(g, params, gVs[v], gWs[v])
+        assert n_or is None or n_or == np_v, f"pairs of volume {v}"
         total += np_v
         ss, gs = batch.sample_slice(v), batch.gaussian_slice(v)
         assert np.array_equal(bits(outs[0][ss]), bits(ref[0])), f"cells of volume {v}"

@@ -76,6 +76,9 @@ def test_gpu_sampleu_dropin():
     assert isinstance(out, np.ndarray) and out.dtype == np.float64
     ids, ref = mesh_ref.locate(pts, tets, samples, vals)
     np.testing.assert_allclose(out, ref, rtol=1e-9, atol=1e-12)
+    # the native module's name (GS/ext.cpp:6, re-exported by GS/gpu_mesh_sampling/__init__.py:1)
+    from gpu_mesh_sampling import sample_meshu
+    assert np.array_equal(sample_meshu(pts, conn, types, offsets, vals, samples), out)
     with pytest.raises(RuntimeError):
         gpu_sampleu(pts, conn, np.full(C, 7, np.int64), offsets, vals, samples)
 
@@ -98,6 +101,8 @@ def test_grid_sample(dtype):
     samples[8] = origin + (dims - 1) * spacing
     out = gpu_sample(dims, origin, spacing, values, samples)
     ref = mesh_ref.grid_sample(dims, origin, spacing, values, samples)
+    from gpu_mesh_sampling import sample_mesh  # the native module's name (GS/ext.cpp:5)
+    assert np.array_equal(sample_mesh(dims, origin, spacing, values, samples), out)
     assert out.dtype == values.dtype
     if dtype in (np.uint8, np.uint16):
         assert np.array_equal(out, ref)

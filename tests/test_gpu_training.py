@@ -62,7 +62,10 @@ def ref_loss(cells, weights, gt, fn_reg, fp_reg, encode_surface):
     float32 tensors, as the reference's fp32 training tensors compare."""
     c32, w32, g32 = cells.detach().float(), weights.detach().float(), gt.float()
     recon = torch.logical_and(g32 != -1, c32 != -1)
-    l1 = torch.abs(cells[recon] - gt[recon]).mean()
+    if encode_surface:
+        l1 = torch.abs(cells - gt).mean()
+    else:
+        l1 = torch.abs(cells[recon] - gt[recon]).mean()
     t = 0.0105 if encode_surface else 0.011
     fn_mask = torch.logical_and(torch.logical_and(g32 != -1, w32 > 0.0), w32 < t)
     fn_vals = torch.clamp(0.011 - weights[fn_mask], min=0)
