@@ -118,6 +118,7 @@ size_t prune_bytes(int count);
 // metrics.hip
 size_t volume_metrics_bytes(int S);
 int volume_metrics(int S, const float* cells, const float* gt, double* out, void* ws, size_t bytes, hipStream_t st);
+int volume_metrics_noalloc(int S, const float* cells, const float* gt, double* out, hipStream_t st);
 int prune(int count, const gvc_store* s, const uint8_t* mask, float min_weight, int* new_count, void* ws,
           size_t bytes, hipStream_t st);
 
@@ -563,6 +564,13 @@ int gvc_volume_metrics(int S, const float* cells, const float* gt, double* out, 
     GVC_REQUIRE(S >= 0, "S < 0");
     GVC_REQUIRE(out && workspace && (S == 0 || (cells && gt)), "NULL pointer");
     return volume_metrics(S, cells, gt, out, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+int gvc_volume_metrics_noalloc(int S, const float* cells, const float* gt, double* out, void* stream)
+{
+    GVC_REQUIRE(S >= 0, "S < 0");
+    GVC_REQUIRE(out && (S == 0 || (cells && gt)), "NULL pointer");
+    return volume_metrics_noalloc(S, cells, gt, out, (hipStream_t)stream);
 }
 
 }  // extern "C"

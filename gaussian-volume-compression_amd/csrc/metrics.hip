@@ -13,6 +13,8 @@
 //   6  sum (c0 - g)^2 over g != -1       7  count g != -1
 //   8  count g == -1                     9  count c == -1 & g != -1   (false negatives)
 //   10 count c != -1 & g == -1   (false positives)
+#include <mutex>
+
 #include "gvc_internal.h"
 
 namespace gvc {
@@ -70,7 +72,54 @@ __global__ __launch_bounds__(kMetricThreads) void metrics_kernel(int n, const fl
     if (threadIdx.x < kSums) out[(size_t)kSums * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// Scratch of the allocation-free entry point (gvc_volume_metrics_noalloc): the
+// block partials of up to kStaticBlocks blocks, i.e. samples up to
+// kStaticBlocks * kMetricThreads = 2^24, in a file-scope device array (5.8 MB
+// per GPU, held until the process ends).  Host code never passes `scratch`
+// itself to a kernel: its device address is looked up per device once.
+constexpr unsigned kStaticBlocks = 65536;
+constexpr int kStaticMaxSamples = (int)(kStaticBlocks * kMetricThreads);
+static __device__ __align__(256) double scratch[(size_t)kSums * kStaticBlocks];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+double* scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch` (hipGetSymbolAddress with that device
+// current, once per device).
+int scratch_of_current_device(double** out)
+{
+    int dev = 0;
+    GVC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_error(GVC_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        GVC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = scratch_ptr[dev];
+    return GVC_OK;
+}
+
 }  // namespace
+
+int volume_metrics_noalloc(int S, const float* cells, const float* gt, double* out, hipStream_t st)
+{
+    if (S > kStaticMaxSamples)
+        return set_error(GVC_ERR_CAPACITY, "gvc_volume_metrics_noalloc: more than 2^24 samples (its static scratch "
+                                           "holds 65536 block partials); use gvc_volume_metrics with a workspace");
+    double* part = nullptr;
+    GVC_TRY(scratch_of_current_device(&part));
+    const unsigned nb = blocks_for((size_t)(S > 0 ? S : 1), kMetricThreads);
+    // every partial the final pass reads is written by this call's first pass:
+    // nothing has to start at zero
+    metrics_kernel<false><<<nb, kMetricThreads, 0, st>>>(S, cells, gt, nullptr, part);
+    GVC_LAUNCH_CHECK("metrics");
+    metrics_kernel<true><<<1, kMetricThreads, 0, st>>>((int)nb, nullptr, nullptr, part, out);
+    GVC_LAUNCH_CHECK("metrics_final");
+    return GVC_OK;
+}
 
 size_t volume_metrics_bytes(int S)
 {

@@ -9,6 +9,12 @@ import os
 
 from ._build import LIB_PATH
 
+GVC_ERR_INVALID_ARGUMENT = 1
+GVC_ERR_WORKSPACE_TOO_SMALL = 2
+GVC_ERR_HIP = 3
+GVC_ERR_UNSUPPORTED = 4
+GVC_ERR_CAPACITY = 5
+GVC_METRIC_SUMS = 11
 GVC_FLAG_KAHAN = 1
 GVC_FLAG_FP64_COND = 2
 GVC_FLAG_CANDIDATE_LOG = 4
@@ -133,6 +139,7 @@ _SIGS = {
     "gvc_prune": (_I, [_I, ctypes.POINTER(gvc_store), _P, _F, _P, _P, _SZ, _P]),
     "gvc_volume_metrics_bytes": (_SZ, [_I]),
     "gvc_volume_metrics": (_I, [_I, _P, _P, _P, _P, _SZ, _P]),
+    "gvc_volume_metrics_noalloc": (_I, [_I, _P, _P, _P, _P]),
 }
 
 EXPORTED = tuple(_SIGS.keys())

@@ -450,6 +450,16 @@ size_t gvc_volume_metrics_bytes(int S);
 int gvc_volume_metrics(int S, const float *cells, const float *gt, double *out, void *workspace,
                        size_t workspace_bytes, void *stream);
 
+/* gvc_volume_metrics without a workspace and without allocating: no counterpart
+ * in the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight evaluation loops, harnesses that forbid hipMalloc).
+ * Its block partials live in a static device array of the library (5.8 MB per
+ * GPU, kept until the process ends), so S is limited to 2^24 samples
+ * (GVC_ERR_CAPACITY beyond).  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (different streams, host threads) race and must be ordered
+ * by the caller.  Same results as gvc_volume_metrics, bit for bit. */
+int gvc_volume_metrics_noalloc(int S, const float *cells, const float *gt, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
