@@ -350,7 +350,7 @@ def main():
         mh, sh, rh, wh, vh = (t.cpu().numpy() for t in (m, sc, rq, wv, vv))
         # the oracle in the step's own mode (FP64 path and precise evaluation when on)
         conic, box, _, fac = oracle.preprocess(mh, sh, rh, wh, 1.0, params.fp64_cond, params.fp64_scale_ratio,
-                                               factor=True)
+                                               factor=True, factor_ratio=params.factor_scale_ratio)
         fac = fac if params.fp64_cond else None
         sub = np.random.default_rng(0).choice(S, 1024, replace=False)
         pts = samples.cpu().numpy()[sub]
@@ -394,9 +394,11 @@ def precision_study(args, W, dev, n_points=65536, n_stress=16384):
     points drawn inside the boxes of the Gaussians whose covariance takes the FP64
     path at the default threshold (scale ratio above 1000, cond(Sigma) > 1e6),
     where the fp32 inverse is least accurate.  Modes: fp32 conic form with naive
-    or Kahan sums; Kahan with the FP64 path (FP64 inverse, pairs evaluated from
-    its Cholesky factor in compensated fp32) above scale ratio 1000 (the default)
-    and above ratio 10."""
+    or Kahan sums; Kahan with the precise path at the default thresholds (pairs
+    evaluated from the conic's Cholesky factor in compensated fp32: the factor of
+    the FP64 inverse above scale ratio 1000, the float-float factor above 10);
+    the same with the float-float band switched off (FP64 factor above 1000
+    only); and the FP64 factor above ratio 10."""
     import numpy as np
     import torch
 
@@ -413,8 +415,10 @@ def precision_study(args, W, dev, n_points=65536, n_stress=16384):
     cd = oracle.conic_fp64(sh, rh)
     modes = {"fp32_naive": mx.MixtureParams(kahan=False), "fp32_kahan": mx.MixtureParams(kahan=True),
              "fp32_kahan_fp64_cond": mx.MixtureParams(kahan=True, fp64_cond=True),
+             "fp32_kahan_fp64_only": mx.MixtureParams(kahan=True, fp64_cond=True, factor_scale_ratio=1000.0),
              "fp32_kahan_fp64_ratio10": mx.MixtureParams(kahan=True, fp64_cond=True, fp64_scale_ratio=10.0)}
-    pre = {name: oracle.preprocess(mh, sh, rh, wh, 1.0, prm.fp64_cond, prm.fp64_scale_ratio, factor=True)
+    pre = {name: oracle.preprocess(mh, sh, rh, wh, 1.0, prm.fp64_cond, prm.fp64_scale_ratio, factor=True,
+                                   factor_ratio=prm.factor_scale_ratio)
            for name, prm in modes.items()}
     ill = np.asarray(pre["fp32_kahan_fp64_cond"][2]).reshape(-1).astype(bool)
     rng = np.random.default_rng(args.seed + 12)
@@ -425,7 +429,9 @@ def precision_study(args, W, dev, n_points=65536, n_stress=16384):
     pick = rng.integers(0, max(int(ill.sum()), 1), n_stress)
     stress = (box_ill[pick, :3] + rng.uniform(0, 1, (n_stress, 3)) * (box_ill[pick, 3:] - box_ill[pick, :3])
               if ill.any() else rng.uniform(0, 1, (n_stress, 3))).astype(np.float32)
+    fac_def = pre["fp32_kahan_fp64_cond"][3]
     out = {"gaussians": int(mh.shape[0]), "fp64_path_gaussians": int(ill.sum()),
+           "float_float_factor_gaussians": int((np.isfinite(fac_def[:, 0]) & ~ill).sum()),
            "fp64_path_gaussians_ratio10": int(np.asarray(pre["fp32_kahan_fp64_ratio10"][2]).sum()),
            "oracle": "FP64 sums with FP64 conics, the mode's own (bit-exact) predicate", "sets": {}}
     for set_name, pts_t in (("uniform", uniform), ("inside_fp64_path_boxes", torch.tensor(stress, device=dev))):

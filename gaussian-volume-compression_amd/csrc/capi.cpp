@@ -24,6 +24,9 @@ int hip_status(hipError_t e, const char* what)
 // mixture.hip
 int preprocess_launch(int P, const float* means, const float* scales, const float* rots, const float* weights,
                       float k, unsigned flags, float* conics, float* boxes, hipStream_t st);
+int preprocess_factors_launch(int P, const float* means, const float* scales, const float* rots, const float* weights,
+                              float k, const gvc_params* prm, float* conics, float* boxes, float* factors,
+                              hipStream_t st);
 int morton_launch(int n, const float* xyz, int clamp, uint64_t* codes, hipStream_t st);
 size_t sample_set_bytes(int S);
 int sample_set_build(int S, const float* samples, void* set, size_t bytes, hipStream_t st);
@@ -158,6 +161,18 @@ int gvc_preprocess(int P, const float* means3D, const float* scales, const float
     GVC_REQUIRE(means3D && scales && rotations && weights, "NULL input");
     return preprocess_launch(P, means3D, scales, rotations, weights, scale_modifier, flags, conics, boxes,
                              (hipStream_t)stream);
+}
+
+int gvc_preprocess_factors(int P, const float* means3D, const float* scales, const float* rotations,
+                           const float* weights, float scale_modifier, const gvc_params* params, float* conics,
+                           float* boxes, float* factors, void* stream)
+{
+    GVC_REQUIRE(P >= 0, "P < 0");
+    GVC_REQUIRE(params, "NULL params");
+    if (P == 0) return GVC_OK;
+    GVC_REQUIRE(means3D && scales && rotations && weights, "NULL input");
+    return preprocess_factors_launch(P, means3D, scales, rotations, weights, scale_modifier, params, conics, boxes,
+                                     factors, (hipStream_t)stream);
 }
 
 int gvc_morton3d_unit(int n, const float* xyz, int clamp, uint64_t* codes, void* stream)

@@ -15,6 +15,7 @@ namespace gvc {
 
 constexpr double kWeightCutoff = 1e-2;  // CR/config.h:9
 constexpr float kCondRatio = 1000.0f;   // default FP64 threshold: cond(Sigma) = (s_max/s_min)^2 > 1e6
+constexpr float kFactorRatio = 10.0f;   // default float-float factor threshold (FP64 mode): cond(Sigma) > 100
 constexpr uint32_t kPreciseBit = 0x80000000u;  // GRecord::id bit: the Gaussian is on the precise path
 
 // Gaussian record, 80 bytes, stored in BVH (Morton) order.  One scalar
@@ -158,16 +159,162 @@ __device__ __forceinline__ bool conic_factor(const double C[6], float hi[6], flo
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Float-float arithmetic: a value is h + l with |l| <= ulp(h) / 2, every step in
+// fp32 (Knuth's TwoSum, fma residual products, Dekker's normalisation), about
+// 44 bits of precision.  The compensated-fp32 factor below runs the FP64 path's
+// operation sequence in it, so Gaussians between the factor threshold and the
+// FP64 threshold get an accurate conic factor without FP64 arithmetic.
+// ---------------------------------------------------------------------------
+struct FF {
+    float h, l;
+};
+
+__device__ __forceinline__ FF ff_quick(float a, float b)  // |a| >= |b| or a == 0
+{
+    const float s = a + b;
+    return FF{ s, b - (s - a) };
+}
+__device__ __forceinline__ FF ff_two_sum(float a, float b)
+{
+    const float s = a + b;
+    const float bb = s - a;
+    return FF{ s, (a - (s - bb)) + (b - bb) };
+}
+__device__ __forceinline__ FF ff_two_prod(float a, float b)
+{
+    const float p = a * b;
+    return FF{ p, __builtin_fmaf(a, b, -p) };
+}
+__device__ __forceinline__ FF ff_neg(FF x) { return FF{ -x.h, -x.l }; }
+__device__ __forceinline__ FF ff_add(FF x, FF y)
+{
+    FF s = ff_two_sum(x.h, y.h);
+    const FF t = ff_two_sum(x.l, y.l);
+    s.l += t.h;
+    s = ff_quick(s.h, s.l);
+    s.l += t.l;
+    return ff_quick(s.h, s.l);
+}
+__device__ __forceinline__ FF ff_sub(FF x, FF y) { return ff_add(x, ff_neg(y)); }
+__device__ __forceinline__ FF ff_mul(FF x, FF y)
+{
+    FF p = ff_two_prod(x.h, y.h);
+    p.l = __builtin_fmaf(x.h, y.l, __builtin_fmaf(x.l, y.h, p.l));
+    return ff_quick(p.h, p.l);
+}
+__device__ __forceinline__ FF ff_mul_f(FF x, float y)
+{
+    FF p = ff_two_prod(x.h, y);
+    p.l = __builtin_fmaf(x.l, y, p.l);
+    return ff_quick(p.h, p.l);
+}
+// x / y: three quotient digits, each from the remainder's leading float.
+__device__ __forceinline__ FF ff_div(FF x, FF y)
+{
+    const float q1 = x.h / y.h;
+    FF r = ff_sub(x, ff_mul_f(y, q1));
+    const float q2 = r.h / y.h;
+    r = ff_sub(r, ff_mul_f(y, q2));
+    const float q3 = r.h / y.h;
+    return ff_add(ff_quick(q1, q2), FF{ q3, 0.0f });
+}
+// sqrt(x) for x > 0: the float root and one Newton correction from the remainder.
+__device__ __forceinline__ FF ff_sqrt(FF x)
+{
+    const float s = __builtin_sqrtf(x.h);
+    const FF r = ff_sub(x, ff_two_prod(s, s));
+    return ff_quick(s, r.h / (s + s));
+}
+__device__ __forceinline__ bool ff_less(FF a, FF b) { return a.h < b.h || (a.h == b.h && a.l < b.l); }
+
+// 1e-5 as a float-float constant (hi = the float nearest 1e-5, lo = the rest).
+constexpr float kEpsHi = (float)1e-5;
+constexpr float kEpsLo = (float)(1e-5 - (double)(float)1e-5);
+
+// The upper Cholesky factor of the conic (Sigma + eps I)^-1 of scales sc (times
+// k) and quaternion q, computed in float-float arithmetic along the FP64 path's
+// operation sequence (rotation, M = S R^T, Sigma = M^T M, eps = 1e-5 max diag,
+// adjugate, determinant, quotients, Cholesky); entries u00 u01 u02 u11 u12 u22
+// as hi + lo floats.  False when the conic is not positive definite in this
+// arithmetic or a factor entry is not finite (the Gaussian keeps its fp32 conic).
+__device__ __forceinline__ bool conic_factor_ff(const float sc[3], const float q[4], float k, float hi[6],
+                                                float lo[6])
+{
+    const float r = q[0], x = q[1], y = q[2], z = q[3];
+    const FF one{ 1.0f, 0.0f };
+    auto two = [](FF v) { return FF{ 2.0f * v.h, 2.0f * v.l }; };
+    FF R[3][3];
+    R[0][0] = ff_sub(one, two(ff_add(ff_two_prod(y, y), ff_two_prod(z, z))));
+    R[0][1] = two(ff_sub(ff_two_prod(x, y), ff_two_prod(r, z)));
+    R[0][2] = two(ff_add(ff_two_prod(x, z), ff_two_prod(r, y)));
+    R[1][0] = two(ff_add(ff_two_prod(x, y), ff_two_prod(r, z)));
+    R[1][1] = ff_sub(one, two(ff_add(ff_two_prod(x, x), ff_two_prod(z, z))));
+    R[1][2] = two(ff_sub(ff_two_prod(y, z), ff_two_prod(r, x)));
+    R[2][0] = two(ff_sub(ff_two_prod(x, z), ff_two_prod(r, y)));
+    R[2][1] = two(ff_add(ff_two_prod(y, z), ff_two_prod(r, x)));
+    R[2][2] = ff_sub(one, two(ff_add(ff_two_prod(x, x), ff_two_prod(y, y))));
+    FF M[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const FF sa = ff_two_prod(k, sc[a]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) M[a][c] = ff_mul(sa, R[c][a]);
+    }
+    auto sig = [&](int i, int j) {
+        return ff_add(ff_add(ff_mul(M[0][i], M[0][j]), ff_mul(M[1][i], M[1][j])), ff_mul(M[2][i], M[2][j]));
+    };
+    const FF S00 = sig(0, 0), S01 = sig(0, 1), S02 = sig(0, 2), S11 = sig(1, 1), S12 = sig(1, 2), S22 = sig(2, 2);
+    FF mx = S00;   // diagonal entries are sums of squares: no absolute value needed
+    if (ff_less(mx, S11)) mx = S11;
+    if (ff_less(mx, S22)) mx = S22;
+    const FF eps = ff_mul(mx, FF{ kEpsHi, kEpsLo });
+    const FF a = ff_add(S00, eps), b = S01, c = S02, d = ff_add(S11, eps), e = S12, f = ff_add(S22, eps);
+    const FF A00 = ff_sub(ff_mul(d, f), ff_mul(e, e));
+    const FF A01 = ff_sub(ff_mul(c, e), ff_mul(b, f));
+    const FF A02 = ff_sub(ff_mul(b, e), ff_mul(c, d));
+    const FF A11 = ff_sub(ff_mul(a, f), ff_mul(c, c));
+    const FF A12 = ff_sub(ff_mul(b, c), ff_mul(a, e));
+    const FF A22 = ff_sub(ff_mul(a, d), ff_mul(b, b));
+    const FF det = ff_add(ff_add(ff_mul(a, A00), ff_mul(b, A01)), ff_mul(c, A02));
+    if (!(det.h > 0.0f)) return false;
+    const FF C0 = ff_div(A00, det), C1 = ff_div(A01, det), C2 = ff_div(A02, det);
+    const FF C3 = ff_div(A11, det), C4 = ff_div(A12, det), C5 = ff_div(A22, det);
+    if (!(C0.h > 0.0f)) return false;
+    FF u[6];
+    u[0] = ff_sqrt(C0);
+    u[1] = ff_div(C1, u[0]);
+    u[2] = ff_div(C2, u[0]);
+    const FF a1 = ff_sub(C3, ff_mul(u[1], u[1]));
+    if (!(a1.h > 0.0f)) return false;
+    u[3] = ff_sqrt(a1);
+    u[4] = ff_div(ff_sub(C4, ff_mul(u[1], u[2])), u[3]);
+    const FF a2 = ff_sub(ff_sub(C5, ff_mul(u[2], u[2])), ff_mul(u[4], u[4]));
+    if (!(a2.h > 0.0f)) return false;
+    u[5] = ff_sqrt(a2);
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        if (!(fabsf(u[t].h) <= 3.4e38f && fabsf(u[t].l) <= 3.4e38f)) return false;
+        hi[t] = u[t].h;
+        lo[t] = u[t].l;
+    }
+    return true;
+}
+
 // CR/forward.cu:10-114 with the canonical operation order.  conic = packed
 // inverse of Sigma + eps I (xx xy xz yy yz zz); box = lo.xyz hi.xyz.
 // fp64_ratio > 0: Gaussians whose scale ratio exceeds it (cond(Sigma) above its
 // square) take the FP64 covariance inverse; with fac_hi/fac_lo given, such a
 // Gaussian also gets the Cholesky factor of that FP64 conic and returns true
 // (the precise path: its pairs are evaluated from the factor, precise_power).
+// factor_ratio > 0 (with fac_hi/fac_lo): a Gaussian whose scale ratio exceeds it
+// but not fp64_ratio keeps its fp32 conic and gets the factor in float-float
+// arithmetic (conic_factor_ff), and returns true when that succeeds: the precise
+// path without FP64.
 __device__ __forceinline__ bool preprocess_one(const float mu[3], const float sc[3], const float q[4],
                                                float w, float k, float fp64_ratio, float conic[6],
                                                float box[6], float* fac_hi = nullptr,
-                                               float* fac_lo = nullptr)
+                                               float* fac_lo = nullptr, float factor_ratio = 0.f)
 {
     float R[3][3];
     rotation(q[0], q[1], q[2], q[3], R);
@@ -175,6 +322,8 @@ __device__ __forceinline__ bool preprocess_one(const float mu[3], const float sc
     bool ill = fp64_ratio > 0.f && ill_conditioned(sc[0], sc[1], sc[2], k, fp64_ratio);
     bool precise = false;
     if (!ill) {
+        if (fac_hi && factor_ratio > 0.f && ill_conditioned(sc[0], sc[1], sc[2], k, factor_ratio))
+            precise = conic_factor_ff(sc, q, k, fac_hi, fac_lo);
         float M[3][3];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
@@ -251,14 +400,14 @@ __device__ __forceinline__ bool preprocess_one(const float mu[3], const float sc
 // (r x y z).  The forward it differentiates: M = diag(k s) R^T, Sigma = M^T M,
 // conic = (Sigma + eps I)^-1 with eps = 1e-5 max_i |Sigma_ii| (CR/forward.cu:26-75;
 // the eps term's own derivative is kept on the largest diagonal entry).
-// fp64_ratio > 0: a Gaussian on the precise path (the preprocess's test: scale
-// ratio above fp64_ratio and a positive definite factor of the same FP64 conic)
-// has g6 with respect to the factor-frame outer products (sum of dquad y y^T,
-// y = U d, conic = U^T U), and dL/dSigma = -C G C is formed as -U^T G_y U (exact
-// algebra with G = U^-1 G_y U^-T; no cancellation against the conic's large entries).
+// factor_frame: the Gaussian is on the precise path (its record carries the
+// precise bit), so g6 is with respect to the factor-frame outer products (sum of
+// dquad y y^T, y = U d, conic = U^T U), and dL/dSigma = -C G C is formed as
+// -U^T G_y U with U the Cholesky factor of the FP64 conic (exact algebra with
+// G = U^-1 G_y U^-T; no cancellation against the conic's large entries).
 __device__ __forceinline__ void covariance_chain(const double* __restrict__ g6, const float* __restrict__ sc,
                                                  const float* __restrict__ q, float kmod, double ds[3],
-                                                 double dq[4], float fp64_ratio = 0.f)
+                                                 double dq[4], bool factor_frame = false)
 {
     const float q4[4] = { q[0], q[1], q[2], q[3] };
     double R[3][3];
@@ -290,7 +439,7 @@ __device__ __forceinline__ void covariance_chain(const double* __restrict__ g6, 
     double CG[3][3], K[3][3];
     const double c6[6] = { C[0][0], C[0][1], C[0][2], C[1][1], C[1][2], C[2][2] };
     double u6[6];
-    if (fp64_ratio > 0.f && ill_conditioned(sc[0], sc[1], sc[2], kmod, fp64_ratio) && conic_factor_d(c6, u6)) {
+    if (factor_frame && conic_factor_d(c6, u6)) {
         const double U[3][3] = { { u6[0], u6[1], u6[2] }, { 0.0, u6[3], u6[4] }, { 0.0, 0.0, u6[5] } };
         for (int r = 0; r < 3; ++r)   // CG = U^T G_y
             for (int c = 0; c < 3; ++c) CG[r][c] = U[0][r] * G[0][c] + U[1][r] * G[1][c] + U[2][r] * G[2][c];

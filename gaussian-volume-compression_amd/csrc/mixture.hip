@@ -144,21 +144,30 @@ __global__ void morton63_kernel(const float* __restrict__ xyz, int n, int clamp,
     if (i < n) codes[i] = morton3d_unit(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], clamp != 0);
 }
 
+// factors (optional, [P, 12]): the precise path's conic factor hi[6] lo[6], NaN
+// rows for Gaussians off that path.
 __global__ void preprocess_kernel(int P, const float* __restrict__ means, const float* __restrict__ scales,
                                   const float* __restrict__ rots, const float* __restrict__ weights, float k,
-                                  float fp64_ratio, float* __restrict__ conics, float* __restrict__ boxes)
+                                  float fp64_ratio, float factor_ratio, float* __restrict__ conics,
+                                  float* __restrict__ boxes, float* __restrict__ factors)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
     float mu[3] = { means[3 * i], means[3 * i + 1], means[3 * i + 2] };
     float sc[3] = { scales[3 * i], scales[3 * i + 1], scales[3 * i + 2] };
     float q[4] = { rots[4 * i], rots[4 * i + 1], rots[4 * i + 2], rots[4 * i + 3] };
-    float cn[6], bx[6];
-    preprocess_one(mu, sc, q, weights[i], k, fp64_ratio, cn, bx);
+    float cn[6], bx[6], fh[6], fl[6];
+    const bool precise = preprocess_one(mu, sc, q, weights[i], k, fp64_ratio, cn, bx, factors ? fh : nullptr, fl,
+                                        factor_ratio);
     if (conics)
         for (int c = 0; c < 6; ++c) conics[6 * i + c] = cn[c];
     if (boxes)
         for (int c = 0; c < 6; ++c) boxes[6 * i + c] = bx[c];
+    if (factors)
+        for (int c = 0; c < 6; ++c) {
+            factors[12 * (size_t)i + c] = precise ? fh[c] : __builtin_nanf("");
+            factors[12 * (size_t)i + 6 + c] = precise ? fl[c] : __builtin_nanf("");
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -268,7 +277,9 @@ __global__ __launch_bounds__(256) void inv_perm_kernel(int P, const uint32_t* __
 
 // Canonical preprocess, in the caller's order (coalesced parameter reads and
 // conic writes), written as the record of BVH index inv_perm[j].  With fac_lo
-// (FP64 mode), a Gaussian on the precise path carries the hi parts of its conic's
+// (FP64 mode), a Gaussian on the precise path (scale ratio above factor_ratio:
+// the factor of its FP64 conic above fp64_ratio, the float-float factor below)
+// carries the hi parts of its conic's
 // Cholesky factor in the conic slots and kPreciseBit in its id; the lo parts go
 // to fac_lo[2j], [2j + 1], j = the caller's index.  Only those entries are
 // written: a pair loop reads an entry only for an id with kPreciseBit, which
@@ -278,8 +289,8 @@ __global__ __launch_bounds__(256) void record_kernel(int P, const uint32_t* __re
                                                      const float* __restrict__ scales, const float* __restrict__ rots,
                                                      const float* __restrict__ values,
                                                      const float* __restrict__ weights, float k, float fp64_ratio,
-                                                     GRecord* __restrict__ rec, float4* __restrict__ fac_lo,
-                                                     float* __restrict__ conics_out)
+                                                     float factor_ratio, GRecord* __restrict__ rec,
+                                                     float4* __restrict__ fac_lo, float* __restrict__ conics_out)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= P) return;
@@ -289,7 +300,8 @@ __global__ __launch_bounds__(256) void record_kernel(int P, const uint32_t* __re
     float q[4] = { rots[4 * j], rots[4 * j + 1], rots[4 * j + 2], rots[4 * j + 3] };
     const float w = weights[j];
     float cn[6], bx[6], fh[6], fl[6];
-    const bool precise = preprocess_one(mu, sc, q, w, k, fp64_ratio, cn, bx, fac_lo ? fh : nullptr, fl);
+    const bool precise = preprocess_one(mu, sc, q, w, k, fp64_ratio, cn, bx, fac_lo ? fh : nullptr, fl,
+                                        factor_ratio);
     const float* cr = precise ? fh : cn;
     GRecord r;
     r.mx = mu[0]; r.my = mu[1]; r.mz = mu[2]; r.w = w;
@@ -1938,7 +1950,8 @@ __global__ __launch_bounds__(256) void reduce_long_kernel(const uint32_t* __rest
 // records) gives zero scale and rotation gradients without the chain.
 // fp64_ratio > 0 (FP64 mode): Gaussians on the precise path have their conic
 // gradient summed in the factor frame (covariance_chain repeats the test).
-__global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __restrict__ inv_perm, float fp64_ratio,
+__global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __restrict__ inv_perm,
+                                                    const GRecord* __restrict__ rec,
                                                     const double* __restrict__ sums, const float* __restrict__ scales,
                                                     const float* __restrict__ rots, float kmod,
                                                     float* __restrict__ dmeans, float* __restrict__ dscales,
@@ -1947,7 +1960,8 @@ __global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __res
 {
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= P) return;
-    const double2* s2 = reinterpret_cast<const double2*>(sums + (size_t)kSumStride * inv_perm[id]);
+    const uint32_t ri = inv_perm[id];
+    const double2* s2 = reinterpret_cast<const double2*>(sums + (size_t)kSumStride * ri);
     double m[12];
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
@@ -1962,7 +1976,11 @@ __global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __res
     dmeans[3 * id + 2] = (float)m[4];
     double ds[3] = { 0.0, 0.0, 0.0 }, dq[4] = { 0.0, 0.0, 0.0, 0.0 };
     const bool any = m[5] != 0.0 || m[6] != 0.0 || m[7] != 0.0 || m[8] != 0.0 || m[9] != 0.0 || m[10] != 0.0;
-    if (any) covariance_chain(m + 5, scales + 3 * id, rots + 4 * id, kmod, ds, dq, fp64_ratio);
+    if (any) {
+        // the precise path's conic gradient is in the factor frame (record bit set by the preprocess)
+        const bool factor_frame = ((uint32_t)rec[ri].id & kPreciseBit) != 0u;
+        covariance_chain(m + 5, scales + 3 * id, rots + 4 * id, kmod, ds, dq, factor_frame);
+    }
     for (int c = 0; c < 3; ++c) dscales[3 * id + c] = (float)ds[c];
     for (int c = 0; c < 4; ++c) drots[4 * id + c] = (float)dq[c];
 }
@@ -2207,11 +2225,19 @@ int check_params(const gvc_params* prm)
         return set_error(GVC_ERR_INVALID_ARGUMENT, "GVC_FLAG_LANE_NEVER and GVC_FLAG_LANE_ALWAYS are exclusive");
     if (!(prm->fp64_scale_ratio == 0.f || (prm->fp64_scale_ratio >= 1.f && prm->fp64_scale_ratio <= 1e18f)))
         return set_error(GVC_ERR_INVALID_ARGUMENT, "fp64_scale_ratio must be 0 (default) or in [1, 1e18]");
+    if (!(prm->factor_scale_ratio == 0.f || (prm->factor_scale_ratio >= 1.f && prm->factor_scale_ratio <= 1e18f)))
+        return set_error(GVC_ERR_INVALID_ARGUMENT, "factor_scale_ratio must be 0 (default) or in [1, 1e18]");
     return GVC_OK;
 }
 
 // The scale ratio above which a Gaussian takes the FP64 path (0 = the default).
 float fp64_ratio_of(const gvc_params* prm) { return prm->fp64_scale_ratio > 0.f ? prm->fp64_scale_ratio : kCondRatio; }
+// The scale ratio above which a Gaussian below the FP64 threshold takes the
+// float-float factor (0 = the default).
+float factor_ratio_of(const gvc_params* prm)
+{
+    return prm->factor_scale_ratio > 0.f ? prm->factor_scale_ratio : kFactorRatio;
+}
 
 // The table a call uses: the batch's, or for one volume the buffer's own.
 const void* table_of(const Batch& b, const int* own) { return b.V == 1 && !b.table ? own : b.table; }
@@ -2312,6 +2338,7 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
         return GVC_OK;
     }
     const float fp64_ratio = (prm->flags & GVC_FLAG_FP64_COND) ? fp64_ratio_of(prm) : 0.f;
+    const float factor_ratio = (prm->flags & GVC_FLAG_FP64_COND) ? factor_ratio_of(prm) : 0.f;
     if (stages & GVC_STAGE_BUILD) {
         // 1. Morton order of the Gaussians (within each volume)
         gaussian_codes_kernel<<<blocks_for(P, 256), 256, 0, st>>>(means, P, l.codes);
@@ -2341,7 +2368,7 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
         // 2. records
         inv_perm_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, perm, l.inv_perm);
         record_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, means, scales, rots, values, weights, k,
-                                                          fp64_ratio, l.rec, l.fac_lo, conics);
+                                                          fp64_ratio, factor_ratio, l.rec, l.fac_lo, conics);
         GVC_LAUNCH_CHECK("record");
         extent_kernel<<<bt.V, 256, 0, st>>>(l.rec, bv, l.ext_sum);
         GVC_LAUNCH_CHECK("extent");
@@ -2542,8 +2569,7 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
     GVC_LAUNCH_CHECK("reduce_sums");
     reduce_long_kernel<<<1024, 256, 0, st>>>(bl.begin, bl.end, svals, bl.parts, bl.long_list, bl.sums);
     GVC_LAUNCH_CHECK("reduce_long");
-    const float fp64_ratio = (prm->flags & GVC_FLAG_FP64_COND) ? fp64_ratio_of(prm) : 0.f;
-    chain_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, fp64_ratio, bl.sums, scales, rots, k, dmeans,
+    chain_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, l.rec, bl.sums, scales, rots, k, dmeans,
                                                      dscales, drots, dvalues, dweights);
     GVC_LAUNCH_CHECK("chain");
     return GVC_OK;
@@ -2559,8 +2585,22 @@ int preprocess_launch(int P, const float* means, const float* scales, const floa
 {
     if (P <= 0) return GVC_OK;
     preprocess_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, means, scales, rots, weights, k,
-                                                          (flags & GVC_FLAG_FP64_COND) ? kCondRatio : 0.f, conics,
-                                                          boxes);
+                                                          (flags & GVC_FLAG_FP64_COND) ? kCondRatio : 0.f, 0.f,
+                                                          conics, boxes, nullptr);
+    GVC_LAUNCH_CHECK("preprocess");
+    return GVC_OK;
+}
+
+int preprocess_factors_launch(int P, const float* means, const float* scales, const float* rots, const float* weights,
+                              float k, const gvc_params* prm, float* conics, float* boxes, float* factors,
+                              hipStream_t st)
+{
+    GVC_TRY(check_params(prm));
+    if (P <= 0) return GVC_OK;
+    const bool on = (prm->flags & GVC_FLAG_FP64_COND) != 0;
+    preprocess_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, means, scales, rots, weights, k,
+                                                          on ? fp64_ratio_of(prm) : 0.f,
+                                                          on ? factor_ratio_of(prm) : 0.f, conics, boxes, factors);
     GVC_LAUNCH_CHECK("preprocess");
     return GVC_OK;
 }

@@ -42,7 +42,8 @@ class gvc_batch(ctypes.Structure):
 class gvc_params(ctypes.Structure):
     _fields_ = [("leaf_size", ctypes.c_int), ("block_threads", ctypes.c_int),
                 ("flags", ctypes.c_uint), ("stages", ctypes.c_uint), ("chunk_slots", ctypes.c_uint),
-                ("run_points", ctypes.c_uint), ("fp64_scale_ratio", ctypes.c_float)]
+                ("run_points", ctypes.c_uint), ("fp64_scale_ratio", ctypes.c_float),
+                ("factor_scale_ratio", ctypes.c_float)]
 
 
 GVC_ADAM_MAX_TENSORS = 8
@@ -87,6 +88,7 @@ _SIGS = {
     "gvc_status_string": (ctypes.c_char_p, [_I]),
     "gvc_last_error": (ctypes.c_char_p, []),
     "gvc_preprocess": (_I, [_I, _P, _P, _P, _P, _F, _U, _P, _P, _P]),
+    "gvc_preprocess_factors": (_I, [_I, _P, _P, _P, _P, _F, _PP, _P, _P, _P, _P]),
     "gvc_morton3d_unit": (_I, [_I, _P, _I, _P, _P]),
     "gvc_sort_pairs_u64_bytes": (_SZ, [_I]),
     "gvc_sort_pairs_u64": (_I, [_I, _P, _P, _P, _P, _I, _P, _SZ, _P]),

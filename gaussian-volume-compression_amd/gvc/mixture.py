@@ -22,7 +22,9 @@ class MixtureParams:
     """Launch parameters (the autotuner's search space) and precision flags.
     fp64_cond: Gaussians whose scale ratio exceeds fp64_scale_ratio (0 = 1000,
     i.e. cond(Sigma) > 1e6) take the FP64 covariance inverse, and their pairs are
-    evaluated from the Cholesky factor of that conic in compensated fp32.
+    evaluated from the Cholesky factor of that conic in compensated fp32; those
+    above factor_scale_ratio (0 = 10) but below the FP64 threshold get the same
+    factor computed in float-float arithmetic (no FP64).
     candidate_log: the forward keeps the candidate chunks it visited and the
     backward replays them; False re-walks the tree in the backward (no
     candidate storage).  chunk_slots: candidate-log slots per 64-point group;
@@ -39,6 +41,7 @@ class MixtureParams:
     run_points: int = 64
     lane_mode: str = "auto"
     fp64_scale_ratio: float = 0.0
+    factor_scale_ratio: float = 0.0
 
     def c(self, groups=0):
         flags = ((GVC_FLAG_KAHAN if self.kahan else 0) | (GVC_FLAG_FP64_COND if self.fp64_cond else 0) |
@@ -46,7 +49,7 @@ class MixtureParams:
                  {"auto": 0, "never": _lib.GVC_FLAG_LANE_NEVER, "always": _lib.GVC_FLAG_LANE_ALWAYS}[self.lane_mode])
         return gvc_params(int(self.leaf_size), int(self.block_threads), flags, 0,
                           int(self.chunk_slots or chunk_slots_for(groups)), int(self.run_points),
-                          float(self.fp64_scale_ratio))
+                          float(self.fp64_scale_ratio), float(self.factor_scale_ratio))
 
 
 DEFAULT_CHUNK_SLOTS = 16
@@ -343,6 +346,24 @@ def preprocess(means3D, scales, rotations, weights, scale_modifier=1.0, fp64_con
     call("gvc_preprocess", P, _ptr(means3D), _ptr(scales), _ptr(rotations), _ptr(weights), float(scale_modifier),
          GVC_FLAG_FP64_COND if fp64_cond else 0, _ptr(conic), _ptr(box), _stream(dev))
     return conic, box
+
+
+def preprocess_factors(means3D, scales, rotations, weights, params, scale_modifier=1.0):
+    """Canonical conic [P,6], box [P,6] and the precise path's factor [P,12]
+    (hi[6] lo[6], NaN rows off the path) under ``params`` (fp64_cond and ratios)."""
+    means3D = _as_f32(means3D, "means3D", 3)
+    scales = _as_f32(scales, "scales", 3)
+    rotations = _as_f32(rotations, "rotations", 4)
+    weights = _as_f32(weights, "weights", 1)
+    P = int(means3D.shape[0])
+    dev = means3D.device
+    conic = torch.empty((P, 6), dtype=torch.float32, device=dev)
+    box = torch.empty((P, 6), dtype=torch.float32, device=dev)
+    fac = torch.empty((P, 12), dtype=torch.float32, device=dev)
+    cp = params.c()
+    call("gvc_preprocess_factors", P, _ptr(means3D), _ptr(scales), _ptr(rotations), _ptr(weights),
+         float(scale_modifier), ctypes.byref(cp), _ptr(conic), _ptr(box), _ptr(fac), _stream(dev))
+    return conic, box, fac
 
 
 def morton3d_unit(xyz, clamp=False):

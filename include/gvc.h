@@ -42,7 +42,12 @@ enum gvc_status {
                                   (gvc_params.fp64_scale_ratio squared); in the
                                   mixture calls those Gaussians' pairs are then
                                   evaluated from the Cholesky factor of that FP64
-                                  conic in compensated fp32 (the precise path)   */
+                                  conic in compensated fp32 (the precise path).
+                                  Gaussians above gvc_params.factor_scale_ratio
+                                  (cond > 100 by default) but below the FP64
+                                  threshold take the precise path too, with the
+                                  factor computed in float-float arithmetic (no
+                                  FP64)                                          */
 #define GVC_FLAG_CANDIDATE_LOG 4u  /* forward keeps a candidate log that the backward
                                      * replays; clear: the backward walks the tree again */
 /* How a 64-point group is walked.  Default: the group shares one walk of its
@@ -69,6 +74,9 @@ typedef struct gvc_params {
     unsigned run_points;  /* most points walked together: 16, 32, 64 (0 = 64)     */
     float fp64_scale_ratio; /* GVC_FLAG_FP64_COND threshold on s_max / s_min:
                                0 = 1000 (cond(Sigma) > 1e6), else in [1, 1e18]   */
+    float factor_scale_ratio; /* GVC_FLAG_FP64_COND: s_max / s_min above which a
+                                 Gaussian below the FP64 threshold takes the
+                                 float-float factor: 0 = 10, else in [1, 1e18]   */
 } gvc_params;
 
 /* Error flags reported by gvc_forward_stats (bitwise OR). */
@@ -87,6 +95,15 @@ const char *gvc_last_error(void);
 int gvc_preprocess(int P, const float *means3D, const float *scales, const float *rotations,
                    const float *weights, float scale_modifier, unsigned flags, float *conics,
                    float *boxes, void *stream);
+
+/* gvc_preprocess with the precise path's factors: params->flags (GVC_FLAG_FP64_COND)
+ * and its two ratios select the paths as in the mixture calls; factors [P,12] (may
+ * be NULL) = u00 u01 u02 u11 u12 u22 hi parts then lo parts of the conic's Cholesky
+ * factor for Gaussians on the precise path, NaN elsewhere.  For inspection and
+ * tests of the precise path (CR/forward.cu:10-114 has no counterpart of it). */
+int gvc_preprocess_factors(int P, const float *means3D, const float *scales, const float *rotations,
+                           const float *weights, float scale_modifier, const gvc_params *params, float *conics,
+                           float *boxes, float *factors, void *stream);
 
 /* 63-bit Morton codes of [n,3] points, the reference's morton3d_unit,
  * R/gaussian_renderer/__init__.py:45-58 (clamp = 0) and the clamped variant of

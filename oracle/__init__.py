@@ -53,7 +53,9 @@ def lib():
         L.orc_box_radius.restype = F
         L.orc_box_radius.argtypes = [F]
         L.orc_preprocess.restype = None
-        L.orc_preprocess.argtypes = [I, P, P, P, P, F, I, F, P, P, P, P]
+        L.orc_preprocess.argtypes = [I, P, P, P, P, F, I, F, F, P, P, P, P]
+        L.orc_conic_factor_ff.restype = I
+        L.orc_conic_factor_ff.argtypes = [P, P, F, P, P]
         L.orc_forward.restype = None
         L.orc_forward.argtypes = [I, I, P, P, P, P, P, P, P, P, P, P, P]
         L.orc_forward_binned.restype = I
@@ -107,11 +109,12 @@ def box_radius(w):
 
 
 def preprocess(means, scales, rots, weights, scale_modifier=1.0, fp64_cond=False, fp64_ratio=0.0,
-               factor=False):
+               factor=False, factor_ratio=0.0):
     """Canonical conic (P,6) and box (P,6: lo.xyz hi.xyz) -- CR/forward.cu:10-114 --
     and the FP64-path mask (scale ratio above fp64_ratio, 0 = 1000).  factor=True
     also returns the precise-path factors (P,12: hi[6] lo[6] of the Cholesky factor
-    of the FP64 conic, NaN rows off the path), for forward/backward/pair_mask."""
+    of the FP64 conic, or below the FP64 threshold and above factor_ratio (0 = 10)
+    the float-float factor; NaN rows off the path), for forward/backward/pair_mask."""
     means, scales, rots, weights = _f32(means), _f32(scales), _f32(rots), _f32(weights).reshape(-1)
     P = means.shape[0]
     conic = np.zeros((P, 6), np.float32)
@@ -119,7 +122,8 @@ def preprocess(means, scales, rots, weights, scale_modifier=1.0, fp64_cond=False
     ill = np.zeros(P, np.int32)
     fac = np.zeros((P, 12), np.float32) if factor else None
     lib().orc_preprocess(P, _p(means), _p(scales), _p(rots), _p(weights), float(scale_modifier),
-                         int(bool(fp64_cond)), float(fp64_ratio), _p(conic), _p(box), _p(ill), _p(fac))
+                         int(bool(fp64_cond)), float(fp64_ratio), float(factor_ratio), _p(conic), _p(box), _p(ill),
+                         _p(fac))
     if factor:
         return conic, box, ill.astype(bool), fac
     return conic, box, ill.astype(bool)
@@ -134,6 +138,13 @@ def conic_factor(conic6_d):
     """(ok, hi[6], lo[6]) Cholesky factor of one FP64 conic."""
     hi = np.zeros(6, np.float32); lo = np.zeros(6, np.float32)
     ok = lib().orc_conic_factor(_p(_f64(conic6_d)), _p(hi), _p(lo))
+    return bool(ok), hi, lo
+
+
+def conic_factor_ff(scales3, q4, scale_modifier=1.0):
+    """(ok, hi[6], lo[6]) float-float Cholesky factor of one Gaussian's conic."""
+    hi = np.zeros(6, np.float32); lo = np.zeros(6, np.float32)
+    ok = lib().orc_conic_factor_ff(_p(_f32(scales3)), _p(_f32(q4)), float(scale_modifier), _p(hi), _p(lo))
     return bool(ok), hi, lo
 
 

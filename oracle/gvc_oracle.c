@@ -57,6 +57,7 @@
 
 #define ORC_WEIGHT_CUTOFF 1e-2 /* CR/config.h:9 */
 #define ORC_COND_RATIO 1000.0f /* cond(Sigma) = (s_max/s_min)^2 > 1e6 */
+#define ORC_FACTOR_RATIO 10.0f /* float-float factor below the FP64 threshold: cond > 100 */
 
 /* ------------------------------------------------------------------------- */
 /* canonical log: x = 2^k * f, f in [sqrt(1/2), sqrt(2));  log f = 2 atanh(s),  */
@@ -180,6 +181,145 @@ int orc_conic_factor(const double C[6], float hi[6], float lo[6])
     return 1;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Float-float arithmetic (value = h + l, fp32 operations only): TwoSum, fma     */
+/* residual products, Dekker's normalisation; three-digit quotient, one Newton   */
+/* step for the root.  orc_conic_factor_ff runs the FP64 path's formulas in it   */
+/* (the same operation sequence as the HIP kernels' compensated-fp32 factor).    */
+/* ------------------------------------------------------------------------- */
+typedef struct { float h, l; } orc_ff;
+
+static inline orc_ff ff_quick(float a, float b)
+{
+    orc_ff r;
+    r.h = a + b;
+    r.l = b - (r.h - a);
+    return r;
+}
+static inline orc_ff ff_two_sum(float a, float b)
+{
+    orc_ff r;
+    r.h = a + b;
+    float bb = r.h - a;
+    r.l = (a - (r.h - bb)) + (b - bb);
+    return r;
+}
+static inline orc_ff ff_two_prod(float a, float b)
+{
+    orc_ff r;
+    r.h = a * b;
+    r.l = fmaf(a, b, -r.h);
+    return r;
+}
+static inline orc_ff ff_neg(orc_ff x) { orc_ff r = { -x.h, -x.l }; return r; }
+static inline orc_ff ff_add(orc_ff x, orc_ff y)
+{
+    orc_ff s = ff_two_sum(x.h, y.h);
+    orc_ff t = ff_two_sum(x.l, y.l);
+    s.l += t.h;
+    s = ff_quick(s.h, s.l);
+    s.l += t.l;
+    return ff_quick(s.h, s.l);
+}
+static inline orc_ff ff_sub(orc_ff x, orc_ff y) { return ff_add(x, ff_neg(y)); }
+static inline orc_ff ff_mul(orc_ff x, orc_ff y)
+{
+    orc_ff p = ff_two_prod(x.h, y.h);
+    p.l = fmaf(x.h, y.l, fmaf(x.l, y.h, p.l));
+    return ff_quick(p.h, p.l);
+}
+static inline orc_ff ff_mul_f(orc_ff x, float y)
+{
+    orc_ff p = ff_two_prod(x.h, y);
+    p.l = fmaf(x.l, y, p.l);
+    return ff_quick(p.h, p.l);
+}
+static inline orc_ff ff_div(orc_ff x, orc_ff y)
+{
+    float q1 = x.h / y.h;
+    orc_ff r = ff_sub(x, ff_mul_f(y, q1));
+    float q2 = r.h / y.h;
+    r = ff_sub(r, ff_mul_f(y, q2));
+    float q3 = r.h / y.h;
+    orc_ff q3f = { q3, 0.0f };
+    return ff_add(ff_quick(q1, q2), q3f);
+}
+static inline orc_ff ff_sqrt(orc_ff x)
+{
+    float s = sqrtf(x.h);
+    orc_ff r = ff_sub(x, ff_two_prod(s, s));
+    return ff_quick(s, r.h / (s + s));
+}
+static inline int ff_less(orc_ff a, orc_ff b) { return a.h < b.h || (a.h == b.h && a.l < b.l); }
+static inline orc_ff ff_twice(orc_ff v) { orc_ff r = { 2.0f * v.h, 2.0f * v.l }; return r; }
+
+/* Upper Cholesky factor of (Sigma + eps I)^-1 in float-float arithmetic, along
+ * the FP64 path: rotation of q, M = S R^T (S = k s), Sigma = M^T M, eps = 1e-5 max
+ * diag, adjugate, determinant, quotients, Cholesky.  hi/lo = the entries' float
+ * parts.  Returns 0 when not positive definite or not finite.  Pinned in
+ * tests/test_oracle.py: agrees with the FP64 factor (orc_conic_factor of
+ * orc_conic_fp64) to ~1e-12 relative and U^T U reproduces the FP64 conic. */
+int orc_conic_factor_ff(const float sc[3], const float q[4], float k, float hi[6], float lo[6])
+{
+    const float r = q[0], x = q[1], y = q[2], z = q[3];
+    const orc_ff one = { 1.0f, 0.0f };
+    orc_ff R[3][3];
+    R[0][0] = ff_sub(one, ff_twice(ff_add(ff_two_prod(y, y), ff_two_prod(z, z))));
+    R[0][1] = ff_twice(ff_sub(ff_two_prod(x, y), ff_two_prod(r, z)));
+    R[0][2] = ff_twice(ff_add(ff_two_prod(x, z), ff_two_prod(r, y)));
+    R[1][0] = ff_twice(ff_add(ff_two_prod(x, y), ff_two_prod(r, z)));
+    R[1][1] = ff_sub(one, ff_twice(ff_add(ff_two_prod(x, x), ff_two_prod(z, z))));
+    R[1][2] = ff_twice(ff_sub(ff_two_prod(y, z), ff_two_prod(r, x)));
+    R[2][0] = ff_twice(ff_sub(ff_two_prod(x, z), ff_two_prod(r, y)));
+    R[2][1] = ff_twice(ff_add(ff_two_prod(y, z), ff_two_prod(r, x)));
+    R[2][2] = ff_sub(one, ff_twice(ff_add(ff_two_prod(x, x), ff_two_prod(y, y))));
+    orc_ff M[3][3];
+    for (int a = 0; a < 3; ++a) {
+        orc_ff sa = ff_two_prod(k, sc[a]);
+        for (int c = 0; c < 3; ++c) M[a][c] = ff_mul(sa, R[c][a]);
+    }
+    orc_ff S[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = i; j < 3; ++j)
+            S[i][j] = ff_add(ff_add(ff_mul(M[0][i], M[0][j]), ff_mul(M[1][i], M[1][j])), ff_mul(M[2][i], M[2][j]));
+    orc_ff mx = S[0][0];
+    if (ff_less(mx, S[1][1])) mx = S[1][1];
+    if (ff_less(mx, S[2][2])) mx = S[2][2];
+    const float eps_hi = (float)1e-5;
+    const orc_ff eps_c = { eps_hi, (float)(1e-5 - (double)eps_hi) };
+    orc_ff eps = ff_mul(mx, eps_c);
+    orc_ff a = ff_add(S[0][0], eps), b = S[0][1], c = S[0][2], d = ff_add(S[1][1], eps), e = S[1][2],
+           f = ff_add(S[2][2], eps);
+    orc_ff A00 = ff_sub(ff_mul(d, f), ff_mul(e, e));
+    orc_ff A01 = ff_sub(ff_mul(c, e), ff_mul(b, f));
+    orc_ff A02 = ff_sub(ff_mul(b, e), ff_mul(c, d));
+    orc_ff A11 = ff_sub(ff_mul(a, f), ff_mul(c, c));
+    orc_ff A12 = ff_sub(ff_mul(b, c), ff_mul(a, e));
+    orc_ff A22 = ff_sub(ff_mul(a, d), ff_mul(b, b));
+    orc_ff det = ff_add(ff_add(ff_mul(a, A00), ff_mul(b, A01)), ff_mul(c, A02));
+    if (!(det.h > 0.0f)) return 0;
+    orc_ff C0 = ff_div(A00, det), C1 = ff_div(A01, det), C2 = ff_div(A02, det);
+    orc_ff C3 = ff_div(A11, det), C4 = ff_div(A12, det), C5 = ff_div(A22, det);
+    if (!(C0.h > 0.0f)) return 0;
+    orc_ff u[6];
+    u[0] = ff_sqrt(C0);
+    u[1] = ff_div(C1, u[0]);
+    u[2] = ff_div(C2, u[0]);
+    orc_ff a1 = ff_sub(C3, ff_mul(u[1], u[1]));
+    if (!(a1.h > 0.0f)) return 0;
+    u[3] = ff_sqrt(a1);
+    u[4] = ff_div(ff_sub(C4, ff_mul(u[1], u[2])), u[3]);
+    orc_ff a2 = ff_sub(ff_sub(C5, ff_mul(u[2], u[2])), ff_mul(u[4], u[4]));
+    if (!(a2.h > 0.0f)) return 0;
+    u[5] = ff_sqrt(a2);
+    for (int t = 0; t < 6; ++t) {
+        if (!(fabsf(u[t].h) <= 3.4e38f && fabsf(u[t].l) <= 3.4e38f)) return 0;
+        hi[t] = u[t].h;
+        lo[t] = u[t].l;
+    }
+    return 1;
+}
+
 /* Knuth's TwoSum: s + e == a + b exactly. */
 static inline void orc_two_sum(float a, float b, float *s, float *e)
 {
@@ -235,13 +375,16 @@ float orc_precise_power(const float *fac, const float *mu, const float *x)
  * cond(Sigma) > 1e6) get their covariance, epsilon, determinant and adjugate in
  * double (plain ops), conic = (float)(adj/det).  factor_out (P x 12, may be NULL):
  * for those, the Cholesky factor of the double conic as hi[6] lo[6]
- * (orc_conic_factor) -- the precise path; NaN in every other row.
+ * (orc_conic_factor) -- the precise path; NaN in every other row.  Gaussians
+ * whose scale ratio exceeds factor_ratio (0 = 10) but not the FP64 threshold keep
+ * the fp32 conic and get the float-float factor (orc_conic_factor_ff) instead.
  */
 void orc_preprocess(int P, const float *means, const float *scales, const float *rots,
                     const float *weights, float scale_modifier, int fp64_cond, float fp64_ratio,
-                    float *conic_out, float *box_out, int32_t *ill_out, float *factor_out)
+                    float factor_ratio, float *conic_out, float *box_out, int32_t *ill_out, float *factor_out)
 {
     const float ratio = fp64_ratio > 0.f ? fp64_ratio : ORC_COND_RATIO;
+    const float fratio = factor_ratio > 0.f ? factor_ratio : ORC_FACTOR_RATIO;
     for (int i = 0; i < P; ++i) {
         const float *q = rots + 4 * i;
         float s[3] = { scale_modifier * scales[3 * i + 0], scale_modifier * scales[3 * i + 1],
@@ -279,6 +422,14 @@ void orc_preprocess(int P, const float *means, const float *scales, const float 
             cn[3] = A11 * det_inv;
             cn[4] = A12 * det_inv;
             cn[5] = A22 * det_inv;
+            if (fp64_cond && factor_out && orc_ill_conditioned(scales + 3 * i, scale_modifier, fratio)) {
+                float hi[6], lo[6];
+                if (orc_conic_factor_ff(scales + 3 * i, q, scale_modifier, hi, lo))
+                    for (int k = 0; k < 6; ++k) {
+                        factor_out[12 * i + k] = hi[k];
+                        factor_out[12 * i + 6 + k] = lo[k];
+                    }
+            }
         } else {
             double Rd[3][3];
             orc_rotation_d(q, Rd);

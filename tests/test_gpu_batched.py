@@ -90,7 +90,8 @@ def check_against_oracle(g, params, outs, gV, gW):
     if g["means"].shape[0] == 0:
         return
     conic, box, _, fac = oracle.preprocess(g["means"], g["scales"], g["rotations"], g["weights"], 1.0,
-                                           params.fp64_cond, params.fp64_scale_ratio, factor=True)
+                                           params.fp64_cond, params.fp64_scale_ratio, factor=True,
+                                           factor_ratio=params.factor_scale_ratio)
     fac = fac if params.fp64_cond else None
     cd = oracle.conic_fp64(g["scales"], g["rotations"])
     Vr, Wr, cnt = oracle.forward(g["means"], g["values"], g["weights"], conic, box, g["samples"], cd, factor=fac)
@@ -133,11 +134,13 @@ def test_batch_equals_single_calls(log):
     check_against_singles(vols, mx.MixtureParams(candidate_log=log))
 
 
-def test_batch_precise_path_equals_single_calls():
+@pytest.mark.parametrize("fp64_ratio", [10.0, 0.0])
+def test_batch_precise_path_equals_single_calls(fp64_ratio):
     """FP64 mode with Gaussians on the precise path in every volume (a tenth made
-    anisotropic, one scale divided by 3500 or 35, randomly rotated; threshold at
-    scale ratio 10): the batched launch reads each volume's factors at its own
-    Gaussians' entries, bit-identical to one call per volume."""
+    anisotropic, one scale divided by 3500 or 35, randomly rotated; FP64
+    threshold at scale ratio 10, or the default 1000 with the float-float factor
+    for the ratio-35 ones): the batched launch reads each volume's factors at its
+    own Gaussians' entries, bit-identical to one call per volume."""
     vols = [make_volume(*spec) for spec in VOLUMES]
     for k, g in enumerate(vols):
         rng = np.random.default_rng(100 + k)
@@ -148,7 +151,7 @@ def test_batch_precise_path_equals_single_calls():
         g["scales"][pick, ax] = (g["scales"][pick, ax] / div).astype(f32)
         q = rng.normal(size=(pick.size, 4))
         g["rotations"][pick] = (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(f32)
-    check_against_singles(vols, mx.MixtureParams(fp64_cond=True, fp64_scale_ratio=10.0))
+    check_against_singles(vols, mx.MixtureParams(fp64_cond=True, fp64_scale_ratio=fp64_ratio))
 
 
 @pytest.mark.parametrize("leaf,block", [(2, 512), (8, 1024)])

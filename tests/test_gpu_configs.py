@@ -301,12 +301,11 @@ def central_differences(host, samples, out, conic, box, chk, n_params=256, rel_h
         assert e_gpu <= 1e-4, (kinds[w][0], e_gpu)
 
 
-@pytest.mark.xfail(reason="the ratio-35 Gaussians below the FP64 threshold are evaluated from the fp32 conic; "
-                          "its cancellation exceeds the 1e-5 field bar (3.5e-5 measured)", strict=True)
 def test_precision_workload_4m():
     """4M Gaussians on the 512^3 precision field, 2% with one log-scale at ln 1e-4
     (scale ratio ~35) and 0.5% at ln 1e-6 (ratio > 1000, cond > 1e6), rotated;
-    2^23 uniform points, FP64-cond at the default threshold.  Oracle on a
+    2^23 uniform points, FP64-cond at the default thresholds (FP64 factor above
+    scale ratio 1000, float-float factor above 10).  Oracle on a
     65,536-point subset plus 4,096 points inside the ill-conditioned Gaussians'
     boxes; gradients of 256 Gaussians each from the ratio > 1000, ratio-35 and
     remaining sets."""

@@ -70,6 +70,35 @@ def test_preprocess_bit_exact(fp64_cond):
     assert np.array_equal(b2.cpu().numpy(), b2r, equal_nan=True)
 
 
+@pytest.mark.parametrize("fp64_ratio,factor_ratio", [(0.0, 0.0), (10.0, 0.0), (0.0, 1000.0), (200.0, 3.0)])
+def test_precise_factors_bit_exact(fp64_ratio, factor_ratio):
+    """The precise path's conic factors (FP64 above fp64_ratio, float-float
+    between factor_ratio and fp64_ratio) bit-identical to the oracle's, NaN rows
+    and edge cases included."""
+    means, scales, rots, values, w = edge_gaussians(7)
+    rng = np.random.default_rng(8)
+    pick = rng.permutation(np.arange(30, means.shape[0]))[:1200]
+    ax = rng.integers(0, 3, pick.size)
+    div = np.asarray([4.0, 12.0, 35.0, 300.0, 999.0, 3500.0])[np.arange(pick.size) % 6]
+    scales[pick, ax] = (scales[pick, ax] / div).astype(f32)
+    prm = mx.MixtureParams(fp64_cond=True, fp64_scale_ratio=fp64_ratio, factor_scale_ratio=factor_ratio)
+    c, b, fac = mx.preprocess_factors(T(means), T(scales), T(rots), T(w), prm)
+    c_ref, b_ref, ill, fac_ref = oracle.preprocess(means, scales, rots, w, 1.0, True, fp64_ratio, factor=True,
+                                                   factor_ratio=factor_ratio)
+    assert np.array_equal(c.cpu().numpy(), c_ref, equal_nan=True)
+    assert np.array_equal(b.cpu().numpy(), b_ref, equal_nan=True)
+    got = fac.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), fac_ref.view(np.uint32)) or np.array_equal(got, fac_ref,
+                                                                                            equal_nan=True)
+    on = np.isfinite(fac_ref[:, 0])
+    band = on & ~ill
+    print(f"factor rows: FP64 {int((on & ill).sum())}, float-float {int(band.sum())}")
+    if (factor_ratio or 10.0) < (fp64_ratio or 1000.0):
+        assert band.sum() > 100
+    else:
+        assert band.sum() == 0
+
+
 def test_morton_bit_exact():
     rng = np.random.default_rng(2)
     pts = np.concatenate([rng.uniform(-0.1, 1.1, (100000, 3)),
@@ -217,25 +246,25 @@ def ill_conditioned_set(seed=31, frac=0.15, divisors=(35.0, 3500.0)):
     return dict(g, rotations=rots), scales, samples
 
 
-@pytest.mark.parametrize("ratio,leaf,log,lane", [(0.0, 4, True, "auto"), (10.0, 4, True, "auto"),
-                                                 (10.0, 1, False, "auto"), (10.0, 16, True, "always"),
-                                                 (10.0, 2, True, "never")])
+@pytest.mark.parametrize("ratio,leaf,log,lane", [(0.0, 4, True, "auto"), (0.0, 1, False, "always"),
+                                                 (10.0, 4, True, "auto"), (10.0, 1, False, "auto"),
+                                                 (10.0, 16, True, "always"), (10.0, 2, True, "never")])
 def test_precise_path_parity(ratio, leaf, log, lane):
     """FP64 mode: Gaussians above the scale ratio take the FP64 inverse and their
-    pairs are evaluated from the Cholesky factor (compensated fp32).  Bars: conics
+    pairs are evaluated from the Cholesky factor (compensated fp32); at the
+    default threshold (ratio 1000) the ratio-35 Gaussians below it take the same
+    path with the factor computed in float-float arithmetic.  Bars: conics
     bit-exact, pair count equal to the oracle's (which evaluates the same factor
-    form), field within 1e-5 and gradients within 1e-5 of the FP64 oracle.  At the
-    default threshold (ratio 1000) every anisotropic Gaussian is above it; a
-    Gaussian below it keeps the fp32 conic form, whose error at ratio 35 exceeds
-    the field bar."""
-    g, scales, samples = ill_conditioned_set(divisors=(3500.0,) if ratio == 0.0 else (35.0, 3500.0))
+    form), field within 1e-5 and gradients within 1e-5 of the FP64 oracle."""
+    g, scales, samples = ill_conditioned_set(divisors=(35.0, 3500.0))
     params = mx.MixtureParams(leaf_size=leaf, candidate_log=log, lane_mode=lane, fp64_cond=True,
                               fp64_scale_ratio=ratio)
     ss, cells, cw, conics, st = run_gpu(g["means"], scales, g["rotations"], g["values"], g["weights"], samples,
                                         params)
     conic_ref, box_ref, ill, fac = oracle.preprocess(g["means"], scales, g["rotations"], g["weights"], 1.0, True,
                                                      ratio, factor=True)
-    assert ill.sum() == int(0.15 * 2048)
+    assert np.isfinite(fac[:, 0]).sum() == int(0.15 * 2048)
+    assert ill.sum() == (int(0.15 * 2048) if ratio == 10.0 else int(0.15 * 2048) // 2)
     assert np.array_equal(conics.cpu().numpy().reshape(-1, 6), conic_ref)
     cd = oracle.conic_fp64(scales, g["rotations"])
     Vr, Wr, cnt = oracle.forward(g["means"], g["values"], g["weights"], conic_ref, box_ref, samples, cd,

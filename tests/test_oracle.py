@@ -221,6 +221,72 @@ def test_precise_power_against_exact_quadratic_form(ratio):
     assert worst_conic > 10.0
 
 
+@pytest.mark.parametrize("ratio", [12.0, 35.0, 300.0, 999.0])
+def test_float_float_factor(ratio):
+    # below the FP64 threshold the precise path's factor is computed in
+    # float-float arithmetic: it agrees with the FP64 factor of the FP64 conic
+    # (relative, normwise) and U^T U reproduces that conic, the rows are flagged
+    # as off the FP64 path, and the conic output stays the fp32 one
+    mu, s, q, w = _ill_set(200, ratio, 27)
+    conic, box, ill, fac = oracle.preprocess(mu, s, q, w, fp64_cond=True, factor=True)
+    conic32, box32, _ = oracle.preprocess(mu, s, q, w)
+    r = s.max(axis=1) / s.min(axis=1)
+    band = (r > 10.0) & (r <= 1000.0)
+    assert band.sum() >= 60
+    assert np.array_equal(ill, r > 1000.0)
+    assert np.isfinite(fac[band]).all() and np.isnan(fac[~band & ~ill]).all()
+    assert np.array_equal(conic[~ill], conic32[~ill]) and np.array_equal(box, box32)
+    cd = oracle.conic_fp64(s, q)
+    tol = 1e-12 * max(ratio, 10.0) ** 1.5
+    for i in np.flatnonzero(band):
+        ok, h2, l2 = oracle.conic_factor(cd[i])
+        assert ok
+        u = fac[i, :6].astype(np.float64) + fac[i, 6:]
+        u2 = h2.astype(np.float64) + l2
+        assert np.abs(u - u2).max() <= tol * np.abs(u2).max(), (i, np.abs(u - u2).max() / np.abs(u2).max())
+        U = np.zeros((3, 3))
+        U[np.triu_indices(3)] = u
+        C = _sym(cd[i])
+        assert np.abs(U.T @ U - C).max() <= 2 * tol * np.abs(C).max()
+        assert np.all(np.abs(fac[i, 6:]) <= np.spacing(np.abs(fac[i, :6])))
+    # above the factor threshold only; raising it to the FP64 threshold empties the band
+    _, _, _, fac_off = oracle.preprocess(mu, s, q, w, fp64_cond=True, factor=True, factor_ratio=1000.0)
+    assert np.isnan(fac_off[~ill]).all()
+    # the precise power from this factor meets the exact quadratic form as the FP64 factor's does
+    rng = np.random.default_rng(28)
+    worst = 0.0
+    for i in np.flatnonzero(band)[::10]:
+        C = _sym(cd[i])
+        K = np.linalg.cholesky(np.linalg.inv(C))
+        y = rng.normal(size=(100, 3))
+        y *= (rng.uniform(0, 28, 100) ** 0.5 / np.linalg.norm(y, axis=1))[:, None]
+        X = (mu[i].astype(np.float64) + y @ K.T).astype(f32)
+        d = X.astype(np.float64) - mu[i].astype(np.float64)
+        exact = -0.5 * np.einsum('ni,ij,nj->n', d, C, d)
+        for x, e in zip(X, exact):
+            tol_p = 4 * float(np.spacing(f32(max(abs(e), 1.0))))
+            worst = max(worst, abs(oracle.precise_power(fac[i], mu[i], x) - e) / tol_p)
+    assert worst <= 1.0
+
+
+def test_float_float_factor_special_cases():
+    # identity rotation: the factor is diag(1 / sqrt(s^2 + eps)) to float-float accuracy
+    s = np.array([0.02, 0.001, 0.004], f32)
+    ok, hi, lo = oracle.conic_factor_ff(s, np.array([1, 0, 0, 0], f32))
+    assert ok
+    s2 = s.astype(np.float64) ** 2
+    eps = s2.max() * 1e-5
+    want = np.zeros(6)
+    want[[0, 3, 5]] = 1.0 / np.sqrt(s2 + eps)
+    got = hi.astype(np.float64) + lo
+    assert np.abs(got - want).max() <= 1e-12 * want.max()
+    # zero scales: not positive definite, no factor
+    assert not oracle.conic_factor_ff(np.zeros(3, f32), np.array([1, 0, 0, 0], f32))[0]
+    # the zero quaternion gives the identity rotation of the reference's formula
+    ok0, hi0, lo0 = oracle.conic_factor_ff(s, np.zeros(4, f32))
+    assert ok0 and np.array_equal(hi0, hi) and np.array_equal(lo0, lo)
+
+
 def test_precise_path_forward_and_mask():
     # the forward with the factor matches FP64 sums at the same pairs, and the
     # pair set is exactly the points whose precise power lies in the window
