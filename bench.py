@@ -26,10 +26,15 @@ samples are drawn into a static buffer before each replay).
 value = Gaussian-point evaluations per second, forward + backward
 (2 x accepted pairs per step, summed over the timed steps / wall time).
 The roofline object is the dominant kernel (the larger of the forward and
-backward walks): achieved = 44 B x accepted pairs / its mean duration, measured
-with HIP events on the launch stream inside the timed steps.  cpu_baseline is
-the plain fp32 reference formula (brute force, box predicate) from the oracle,
-timed on the host cores on a bounded subset of the same step's points.
+backward walks), which is bound by VALU issue: achieved = the pair math's
+lane-ops x accepted pairs / its mean duration, measured with HIP events on the
+launch stream inside the timed steps, against the non-packed fp32 VALU peak;
+the 44-B-per-pair streaming model and the committed rocprofv3 counters (HBM
+bytes, VALU issue) are reported beside it.  cpu_baseline is the plain fp32
+reference formula (brute force, box predicate) from the oracle, timed on the
+host cores on a bounded subset of the same step's points (its rate is
+extrapolated to the step), plus one full step of the 2,048-Gaussian CPU
+configuration.
 """
 import argparse
 import json
@@ -45,6 +50,15 @@ for _p in (PKG, ROOT):
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PAIR = 44  # mean 12 + conic 24 + weight 4 + value 4
+# VALU issue peak with packed fp32 off: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+# (one wave64 instruction per SIMD every 4 cycles), in 1e12 lane-ops/s.
+VALU_PEAK_TLANE = 256 * 4 * 16 * 2.4e9 / 1e12
+# VALU lane-ops of one accepted pair in the walks' pair math: forward d (3),
+# conic.d (9), quad (3), power (1), window (2), exp (2), w e and w e v (2), two
+# Kahan sums (8); backward the same quadratic form and exp (20) plus the 11
+# partial sums (20).
+VALU_OPS_PER_PAIR = {"fwd_walk": 30, "bwd_walk": 40}
+PMC_FILE = os.path.join(ROOT, "profiles", "pmc_counters.json")
 ADAM_BYTES_PER_GAUSSIAN = 12 * 7 * 4  # 12 floats: p, g, m, v read; p, m, v written
 
 
@@ -88,6 +102,10 @@ def parse():
                     help="choose leaf size x run points x block shape on the workload before timing")
     ap.add_argument("--recompute", action="store_true",
                     help="backward walks the tree again instead of replaying the forward's candidate log")
+    ap.add_argument("--no-batch32", action="store_true",
+                    help="mesh: skip the 32-volume batch legs that the default line reports beside the headline")
+    ap.add_argument("--batch32-steps", type=int, default=10,
+                    help="mesh: timed fused steps of the 32-volume batch leg (the plain sequence takes 2)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (x steps): the calibration launch for the "
                          "FETCH_SIZE / WRITE_SIZE counters")
@@ -176,8 +194,7 @@ def main():
                                    base=params, use_cache=False)
         params = best
         mx.set_default_params(params)
-        tune = {"chosen": {"leaf_size": best.leaf_size, "run_points": best.run_points,
-                           "block_threads": best.block_threads},
+        tune = {"chosen": best.describe(),
                 "ms_fwd_bwd": {k: (round(v, 3) if v is not None else None) for k, v in table.items()}}
         log(f"autotune: {tune['chosen']}")
 
@@ -291,16 +308,7 @@ def main():
         raise RuntimeError(f"mixture error flags during the timed steps: {errs}")
     pairs_per_step = total_pairs / args.steps
     value = 2.0 * total_pairs / wall
-    dom = max(("fwd_walk", "bwd_walk"), key=lambda k: stage_avg.get(k, 0.0))
-    dom_s = stage_avg[dom] / 1e3
-    achieved = BYTES_PER_PAIR * pairs_per_step / dom_s / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-    if args.config == "mesh" and os.path.exists(tf):
-        try:
-            traffic = json.load(open(tf)).get(dom)
-        except Exception:
-            traffic = None
+    roofline = walk_roofline(stage_avg, pairs_per_step, args.config == "mesh")
 
     out = {
         "metric": "Gaussian-point evals/s fwd+bwd @1M Gaussians",
@@ -332,9 +340,7 @@ def main():
                       "ms": stage_avg.get("adam"),
                       "GB/s": (ADAM_BYTES_PER_GAUSSIAN * P / (stage_avg["adam"] * 1e6)
                                if stage_avg.get("adam") else None)},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_launch": BYTES_PER_PAIR * pairs_per_step, "mean_ms": stage_avg[dom]},
+        "roofline": roofline,
     }
 
     if not args.no_error or not args.no_cpu_baseline:
@@ -380,11 +386,91 @@ def main():
                 if dt >= 0.5 * args.cpu_seconds or n >= S:
                     break
                 n = int(min(S, n * min(8.0, max(1.5, args.cpu_seconds / max(dt, 1e-3)))))
+            c0 = cpu_config0_step()
             out["cpu_baseline"] = {"value": 2.0 * pr / dt, "unit": "evals/s", "cores": oracle.num_threads(),
-                                   "kind": "port",
+                                   "cpu_model": cpu_model(), "kind": "port", "extrapolated": n < S,
                                    "sample": f"{n} of the step's {S} points against all {P} Gaussians, plain fp32 "
-                                             f"brute force fwd+bwd with the box predicate ({dt:.1f} s)"}
+                                             f"brute force fwd+bwd with the box predicate ({dt:.1f} s); the rate "
+                                             f"is extrapolated to the whole step",
+                                   "step_s_extrapolated": dt * S / n,
+                                   "config0_full_step": c0}
+    if args.config == "mesh" and not args.no_batch32:
+        # the 32-volume batch (fused batched step vs the plain per-volume
+        # sequence, both from the same training state) beside the headline
+        del model, rast, last
+        mx.set_default_params(mx.MixtureParams())
+        torch.cuda.empty_cache()
+        b = batch32_legs(args, args.batch32_steps, 2, 3)
+        out["batch32"] = {k: b[k] for k in ("ms_per_step", "pairs_per_step", "records_per_step", "evals_per_s",
+                                            "speedup_vs_plain", "stage_ms", "same_start_state")}
+        out["batch32"].update(workload=b["config"]["workload"], plain_ms_per_step=b["plain_sequence"]["ms_per_step"],
+                              plain_pairs_per_step=b["plain_sequence"]["pairs_per_step"], steps=b["steps"],
+                              plain_steps=b["plain_sequence"]["steps"])
     print(json.dumps(out))
+
+
+def walk_roofline(stage_avg, pairs_per_step, counters_apply):
+    """Roofline of the dominant kernel (the larger of the two walks).  The walks
+    are bound by VALU issue (their counters show the issue slots nearly full and
+    HBM at a tenth of its peak): achieved = the pair math's algorithmic lane-ops
+    (VALU_OPS_PER_PAIR x accepted pairs) / the kernel's mean duration (HIP events
+    on the launch stream), against the non-packed fp32 VALU peak.  Beside it: the
+    streaming model (44 B per pair against HBM peak: above 1 when records are
+    reused from LDS and L2), and, for the mesh workload, the committed rocprofv3
+    counters of that kernel (HBM bytes per launch from FETCH_SIZE / WRITE_SIZE,
+    VALU issue and lane activity), labelled with the file and the commit they were
+    measured at."""
+    dom = max(("fwd_walk", "bwd_walk"), key=lambda k: stage_avg.get(k, 0.0))
+    dom_s = stage_avg[dom] / 1e3
+    ops = VALU_OPS_PER_PAIR[dom] * pairs_per_step
+    achieved = ops / dom_s / 1e12
+    stream = BYTES_PER_PAIR * pairs_per_step / dom_s / 1e9
+    pmc = None
+    if counters_apply and os.path.exists(PMC_FILE):
+        try:
+            allc = json.load(open(PMC_FILE))
+            k = allc["kernels"].get(dom)
+            if k:
+                pmc = dict(k, source=os.path.relpath(PMC_FILE, ROOT), measured_at=allc.get("measured_at"),
+                           hbm_GBps=k["hbm_bytes"] / dom_s / 1e9, hbm_frac=k["hbm_bytes"] / dom_s / 1e9 / HBM_PEAK_GBS)
+        except (OSError, ValueError, KeyError):
+            pmc = None
+    return {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": VALU_PEAK_TLANE, "unit": "Tlane-op/s",
+            "frac": achieved / VALU_PEAK_TLANE, "traffic": pmc["hbm_bytes"] if pmc else None,
+            "ops_per_launch": ops, "ops_per_pair": VALU_OPS_PER_PAIR[dom], "mean_ms": stage_avg[dom],
+            "streaming_model": {"achieved": stream, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": stream / HBM_PEAK_GBS, "bytes_per_launch": BYTES_PER_PAIR * pairs_per_step},
+            "counters": pmc}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_config0_step():
+    """The CPU baseline at full size on the CPU configuration (2,048 Gaussians,
+    the 32^3 grid): one plain fp32 forward + backward, seconds."""
+    import numpy as np
+
+    import oracle
+    from gvc import workloads
+    g = workloads.config0(0)
+    scales = np.exp(g["log_scales"]).astype(np.float32)
+    conic, box, _ = oracle.preprocess(g["means"], scales, g["rotations"], g["weights"])
+    S = g["samples"].shape[0]
+    rng = np.random.default_rng(3)
+    t0 = time.perf_counter()
+    _, _, _, pairs = oracle.cpu_plain_fwdbwd(g["means"], g["values"], g["weights"], conic, box, g["samples"],
+                                             rng.normal(size=S), rng.normal(size=S))
+    dt = time.perf_counter() - t0
+    return {"step_s": dt, "pairs": pairs, "evals_per_s": 2.0 * pairs / dt, "gaussians": int(g["means"].shape[0]),
+            "points": int(S)}
 
 
 def precision_study(args, W, dev, n_points=65536, n_stress=16384):
@@ -710,7 +796,13 @@ def main_batch(args):
     reduction); the plain sequence runs the same steps with the naive per-volume
     kernels (one wavefront per Gaussian over a 256-point-leaf sample BVH, float
     atomics, separate normalise, no compensation), volume after volume.  Both
-    use the same activations, loss and Adam; the report gives both steps/s."""
+    use the same activations, loss and Adam, and both legs are timed from the
+    same training state; the report gives both steps/s."""
+    out = batch32_legs(args, args.steps, args.plain_steps, args.warmup)
+    print(json.dumps(out))
+
+
+def batch32_legs(args, steps, plain_steps, warmup):
     import numpy as np
     import torch
 
@@ -726,14 +818,14 @@ def main_batch(args):
     W = workloads.batch_workload(dev, seed=args.seed)
     V = len(W["gaussians"])
     P, S = W["P"], W["S"]
-    log(f"setup {time.perf_counter() - t_setup:.1f}s volumes={V} P={P} S={S}")
-    params = mx.default_params()
+    log(f"batch32 setup {time.perf_counter() - t_setup:.1f}s volumes={V} P={P} S={S}")
+    params = mx.MixtureParams()
     if args.leaf or args.block or args.recompute:
         params = mx.MixtureParams(leaf_size=args.leaf or params.leaf_size,
                                   block_threads=args.block or params.block_threads,
                                   candidate_log=not args.recompute)
     tune = None
-    if args.autotune:
+    if args.autotune and args.config == "batch32":
         from gvc.autotune import autotune_batched
         with torch.no_grad():
             wts, vls = W["weights"].reshape(-1, 1).contiguous(), W["values"].reshape(-1, 1).contiguous()
@@ -741,22 +833,18 @@ def main_batch(args):
                                            W["rotations"], vls, wts, W["point_batches"][0], base=params,
                                            use_cache=False)
         params = best
-        tune = {"chosen": {"leaf_size": best.leaf_size, "run_points": best.run_points,
-                           "block_threads": best.block_threads},
-                "ms_fwd_bwd": {k: (round(v, 3) if v is not None else None) for k, v in table.items()}}
+        tune = {"chosen": best.describe(), "ms_fwd_bwd": {k: (round(v, 3) if v is not None else None)
+                                                          for k, v in table.items()}}
         log(f"autotune: {tune['chosen']}")
     batch = bx.Batch(W["gaussians"], W["samples"], leaf_size=params.leaf_size, device=dev)
     g_off = batch.P_off
     nb = len(W["point_batches"])
 
-    def make_params():
-        return make_model(W, args.max_scale, dev), None
-
     audit = []
     acc = torch.zeros(3, dtype=torch.int64, device=dev)
     plain_pairs = torch.zeros(1, dtype=torch.int64, device=dev)
 
-    def fused_step(k, model, _):
+    def fused_step(k, model):
         pts, gt = W["point_batches"][k % nb], W["ground_truth"][k % nb]
         audit.clear()
         ss = bx.BatchedSampleSet(batch, pts)
@@ -768,7 +856,7 @@ def main_batch(args):
         model.zero_grad()
         audit[-1].accumulate(acc)
 
-    def plain_step(k, model, _):
+    def plain_step(k, model):
         pts, gt = W["point_batches"][k % nb], W["ground_truth"][k % nb]
         trees = [plain.PlainSampleTree(pts[batch.sample_slice(v)]) for v in range(V)]
         scales, rots, w, v = model.activated()
@@ -778,14 +866,15 @@ def main_batch(args):
         model.optimizer_step()
         model.zero_grad()
 
-    # fused: warm-up sizes the record capacity, then stage profile, then timed
-    fp, fopt = make_params()
+    # fused: warm-up sizes the record capacity, then a stage profile; the
+    # training state after it is where both legs' timed steps start
+    fp = make_model(W, args.max_scale, dev)
     mx.set_audit(audit)
     mx.set_record_capacity(None)
     max_rec = 0
     lane_groups = 0
-    for k in range(args.warmup):
-        fused_step(k, fp, fopt)
+    for k in range(warmup):
+        fused_step(k, fp)
         torch.cuda.synchronize()
         max_rec = max(max_rec, audit[-1].stats()[0])
         lane_groups = audit[-1].lane_groups
@@ -793,50 +882,52 @@ def main_batch(args):
     timer = []
     mx.set_stage_timer(timer)
     for k in range(2):
-        fused_step(k, fp, fopt)
+        fused_step(k, fp)
     torch.cuda.synchronize()
     mx.set_stage_timer(None)
     stage = {}
     for name, e0, e1 in timer:
         stage.setdefault(name, []).append(e0.elapsed_time(e1))
     stage = {k: float(np.mean(v)) for k, v in stage.items()}
+    pp = make_model(W, args.max_scale, dev)
+    pp.load_state(fp)                      # the plain leg starts from the same state
     acc.zero_()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        fused_step(k, fp, fopt)
+    for k in range(steps):
+        fused_step(k, fp)
     torch.cuda.synchronize()
-    fused_s = (time.perf_counter() - t0) / args.steps
+    fused_s = (time.perf_counter() - t0) / steps
     mx.set_audit(None)
     pairs, records, errs = (int(x) for x in acc.cpu().tolist())
     if errs:
         raise RuntimeError(f"mixture error flags during the timed steps: {errs}")
-    fused_pairs = pairs / args.steps
+    fused_pairs = pairs / steps
     log(f"fused {1e3 * fused_s:.2f} ms/step, {fused_pairs:.4g} pairs/step")
-    del fp, fopt
+    del fp
     mx.set_record_capacity(None)
     torch.cuda.empty_cache()
 
-    pp, popt = make_params()
-    plain_step(0, pp, popt)
-    torch.cuda.synchronize()
     plain_pairs.zero_()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.plain_steps):
-        plain_step(k, pp, popt)
+    for k in range(plain_steps):
+        plain_step(k, pp)
         torch.cuda.synchronize()
         log(f"plain step {k} done")
-    plain_s = (time.perf_counter() - t0) / args.plain_steps
-    plain_pairs_step = int(plain_pairs.item()) / args.plain_steps
+    plain_s = (time.perf_counter() - t0) / plain_steps
+    plain_pairs_step = int(plain_pairs.item()) / plain_steps
     log(f"plain {1e3 * plain_s:.2f} ms/step, {plain_pairs_step:.4g} pairs/step")
+    del pp
+    torch.cuda.empty_cache()
 
-    out = {
+    return {
         "metric": "steps/s, 32-volume batched fit fwd+bwd",
         "value": 1.0 / fused_s,
         "unit": "steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": 1e3 * fused_s,
         "higher_is_better": True,
         "scaling": "weak",
@@ -850,19 +941,19 @@ def main_batch(args):
                    "block_threads": params.block_threads, "run_points": params.run_points, "kahan": params.kahan,
                    "candidate_log": params.candidate_log, "parallelism": "single GPU"},
         "pairs_per_step": fused_pairs,
-        "records_per_step": records / args.steps,
+        "records_per_step": records / steps,
         "groups": int(batch.c.groups),
         "groups_walked_per_lane": lane_groups,
         "evals_per_s": 2.0 * fused_pairs / fused_s,
         "stage_ms": stage,
         "autotune": tune,
-        "plain_sequence": {"steps_per_s": 1.0 / plain_s, "ms_per_step": 1e3 * plain_s, "steps": args.plain_steps,
+        "plain_sequence": {"steps_per_s": 1.0 / plain_s, "ms_per_step": 1e3 * plain_s, "steps": plain_steps,
                            "pairs_per_step": plain_pairs_step,
                            "kernels": "per volume: preprocess, Morton sort, 256-point-leaf sample tree, one "
                                       "wavefront per Gaussian with float atomics, normalise; backward walks again"},
+        "same_start_state": True,
         "speedup_vs_plain": plain_s / fused_s,
     }
-    print(json.dumps(out))
 
 
 if __name__ == "__main__":

@@ -632,7 +632,7 @@ __device__ __forceinline__ float box_cost(const GroupBox& b, const float r[3])
     return (b.hx - b.lx + r[0]) * (b.hy - b.ly + r[1]) * (b.hz - b.lz + r[2]);
 }
 
-// Runs: a group's valid lanes [0, nv) are cut into at most 8 runs of consecutive
+// Runs: a group's valid lanes [0, nv) are cut into runs of consecutive
 // lanes, each walked with its own box.  Consecutive Morton-sorted points can
 // straddle a coarse Morton boundary; their common box is then much larger than
 // the point clouds on either side.  A run [a, b) is cut where its first and last
@@ -640,7 +640,8 @@ __device__ __forceinline__ float box_cost(const GroupBox& b, const float r[3])
 // suffix) when the two halves' cost is below 3/4 of the whole's.  Returns the
 // run starts as a lane bit mask.  Any partition gives the same sums; this one
 // depends only on the group's points, the mean Gaussian extent and run_points
-// (runs never exceed run_points lanes: 16, 32 or 64, a tuning parameter).
+// (runs never exceed run_points lanes: 4, 8, 16, 32 or 64, a tuning parameter;
+// at 64 a group has at most 8 runs).
 __device__ __forceinline__ uint64_t split_runs(float px, float py, float pz, int nv, const float r[3], int run_points)
 {
     const int lane = threadIdx.x & 63;
@@ -2219,8 +2220,9 @@ int check_params(const gvc_params* prm)
         return set_error(GVC_ERR_INVALID_ARGUMENT, "leaf_size must be in [1, 64]");
     if (prm->block_threads < 64 || prm->block_threads > 1024 || (prm->block_threads % 64) != 0)
         return set_error(GVC_ERR_INVALID_ARGUMENT, "block_threads must be a multiple of 64 in [64, 1024]");
-    if (prm->run_points != 0 && prm->run_points != 16 && prm->run_points != 32 && prm->run_points != 64)
-        return set_error(GVC_ERR_INVALID_ARGUMENT, "run_points must be 0, 16, 32 or 64");
+    if (prm->run_points != 0 && prm->run_points != 4 && prm->run_points != 8 && prm->run_points != 16 &&
+        prm->run_points != 32 && prm->run_points != 64)
+        return set_error(GVC_ERR_INVALID_ARGUMENT, "run_points must be 0, 4, 8, 16, 32 or 64");
     if ((prm->flags & GVC_FLAG_LANE_NEVER) && (prm->flags & GVC_FLAG_LANE_ALWAYS))
         return set_error(GVC_ERR_INVALID_ARGUMENT, "GVC_FLAG_LANE_NEVER and GVC_FLAG_LANE_ALWAYS are exclusive");
     if (!(prm->fp64_scale_ratio == 0.f || (prm->fp64_scale_ratio >= 1.f && prm->fp64_scale_ratio <= 1e18f)))

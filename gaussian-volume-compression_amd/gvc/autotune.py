@@ -1,8 +1,14 @@
 """Autotuner for the mixture kernels.
 
-Sweeps BVH leaf size x points walked together (run_points: the wavefront's
-point grouping) x block shape on the caller's own data, times a forward +
-backward per configuration with HIP events, and keeps the fastest.  Results
+Sweeps BVH leaf size x points per walk x block shape on the caller's own data,
+times a forward + backward per configuration with HIP events, and keeps the
+fastest.  Points per walk is how a wavefront's 64 points (one per lane) share
+the tree walk: 1 = every lane walks for its own point (lane mode), k = runs of
+up to k consecutive points share one walk of their box (run_points).  The
+reference's launch shape has no counterpart (one warp per Gaussian over a
+sample BVH); BASELINE's axes are leaf size {2, 4, 8, 16}, points per wavefront
+{1, 4, 16} and block size {256, 512, 1024}; the 64-point shared walk is added
+because it is the measured default.  Results
 are cached per (log2 P, log2 S) bucket in memory and, if ``GVC_TUNE_CACHE``
 names a file, in that JSON file.  Every configuration accepts exactly the
 same (Gaussian, point) pairs; the order in which a point's compensated sum
@@ -19,7 +25,7 @@ from . import mixture as mx
 from ._lib import GvcError
 
 LEAF_SIZES = (2, 4, 8, 16)
-RUN_POINTS = (16, 32, 64)
+POINTS_PER_WALK = (1, 4, 16, 64)
 BLOCKS = (256, 512, 1024)
 
 _cache = {}
@@ -65,17 +71,25 @@ def _time(run, reps):
     return min(a.elapsed_time(b) for a, b in ev)
 
 
-def _sweep(make_run, base, leaf_sizes, run_points, blocks, reps):
+def params_for(base, leaf_size, points_per_walk, block_threads):
+    """MixtureParams of one configuration of the sweep (base's precision and
+    candidate-log choices kept)."""
+    lane = "always" if points_per_walk == 1 else base.lane_mode
+    rp = 64 if points_per_walk == 1 else int(points_per_walk)
+    return mx.MixtureParams(leaf_size=leaf_size, block_threads=block_threads, run_points=rp, kahan=base.kahan,
+                            fp64_cond=base.fp64_cond, candidate_log=base.candidate_log, lane_mode=lane,
+                            fp64_scale_ratio=base.fp64_scale_ratio, factor_scale_ratio=base.factor_scale_ratio)
+
+
+def _sweep(make_run, base, leaf_sizes, points_per_walk, blocks, reps):
     """Times make_run(params)() for every configuration; returns (best, its ms, table)."""
     table = {}
     best, best_ms = None, float("inf")
     for L in leaf_sizes:
-        for rp in run_points:
+        for ppw in points_per_walk:
             for bt in blocks:
-                prm = mx.MixtureParams(leaf_size=L, block_threads=bt, run_points=rp, kahan=base.kahan,
-                                       fp64_cond=base.fp64_cond, candidate_log=base.candidate_log,
-                                       lane_mode=base.lane_mode)
-                label = f"leaf{L}/run{rp}/block{bt}"
+                prm = params_for(base, L, ppw, bt)
+                label = f"leaf{L}/points{ppw}/block{bt}"
                 try:
                     ms = _time(make_run(prm), reps)
                 except GvcError:
@@ -95,10 +109,8 @@ def _cached(key, base, use_cache):
     if key in _cache:
         return _cache[key]
     disk = _load_file().get(key)
-    if disk:
-        best = mx.MixtureParams(leaf_size=disk["leaf_size"], block_threads=disk["block_threads"],
-                                run_points=disk["run_points"], kahan=base.kahan, fp64_cond=base.fp64_cond,
-                                candidate_log=base.candidate_log, lane_mode=base.lane_mode)
+    if disk and "points_per_walk" in disk:
+        best = params_for(base, disk["leaf_size"], disk["points_per_walk"], disk["block_threads"])
         _cache[key] = best
         return best
     return None
@@ -106,12 +118,11 @@ def _cached(key, base, use_cache):
 
 def _remember(key, best, best_ms):
     _cache[key] = best
-    _save_file({key: {"leaf_size": best.leaf_size, "block_threads": best.block_threads,
-                      "run_points": best.run_points, "ms": best_ms}})
+    _save_file({key: dict(best.describe(), ms=best_ms)})
 
 
 def autotune(means, scales, rots, values, weights, sample_set, base=None, leaf_sizes=LEAF_SIZES,
-             run_points=RUN_POINTS, blocks=BLOCKS, reps=2, use_cache=True):
+             points_per_walk=POINTS_PER_WALK, blocks=BLOCKS, reps=2, use_cache=True):
     """Returns (best MixtureParams, {config label: ms}).  ``base`` supplies the
     precision and candidate-log choices, which are not tuned."""
     base = base or mx.default_params()
@@ -132,13 +143,13 @@ def autotune(means, scales, rots, values, weights, sample_set, base=None, leaf_s
             mx.mixture_backward(st, means, scales, rots, values, weights, 1.0, sample_set, cells, cw, gV, gW)
         return run
 
-    best, best_ms, table = _sweep(make_run, base, leaf_sizes, run_points, blocks, reps)
+    best, best_ms, table = _sweep(make_run, base, leaf_sizes, points_per_walk, blocks, reps)
     _remember(key, best, best_ms)
     return best, table
 
 
 def autotune_batched(gaussians_per_volume, samples_per_volume, means, scales, rots, values, weights, samples,
-                     base=None, leaf_sizes=LEAF_SIZES, run_points=RUN_POINTS, blocks=BLOCKS, reps=2,
+                     base=None, leaf_sizes=LEAF_SIZES, points_per_walk=POINTS_PER_WALK, blocks=BLOCKS, reps=2,
                      use_cache=True):
     """The same sweep for a batch of volumes (inputs concatenated volume-major);
     one Batch and sample set per leaf size.  Returns (best, table)."""
@@ -167,6 +178,6 @@ def autotune_batched(gaussians_per_volume, samples_per_volume, means, scales, ro
             bx.mixture_backward_batched(st, means, scales, rots, values, weights, 1.0, ss, cells, cw, gV, gW)
         return run
 
-    best, best_ms, table = _sweep(make_run, base, leaf_sizes, run_points, blocks, reps)
+    best, best_ms, table = _sweep(make_run, base, leaf_sizes, points_per_walk, blocks, reps)
     _remember(key, best, best_ms)
     return best, table

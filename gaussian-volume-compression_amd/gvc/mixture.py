@@ -43,6 +43,15 @@ class MixtureParams:
     fp64_scale_ratio: float = 0.0
     factor_scale_ratio: float = 0.0
 
+    def describe(self):
+        """The launch shape the autotuner chooses: BVH leaf size, points walked
+        together per wavefront walk (1 = every lane walks the tree for its own
+        point, lane_mode "always"; else up to run_points consecutive points share
+        one walk) and threads per block."""
+        return {"leaf_size": int(self.leaf_size),
+                "points_per_walk": 1 if self.lane_mode == "always" else int(self.run_points),
+                "block_threads": int(self.block_threads)}
+
     def c(self, groups=0):
         flags = ((GVC_FLAG_KAHAN if self.kahan else 0) | (GVC_FLAG_FP64_COND if self.fp64_cond else 0) |
                  (GVC_FLAG_CANDIDATE_LOG if self.candidate_log else 0) |

@@ -188,6 +188,22 @@ class GaussianModel:
         call("gvc_lr_advance", _ptr(self._clock), ctypes.byref(self._schedule), 0, ctypes.c_int64(it),
              _stream(self.device))
 
+    def load_state(self, other):
+        """Copy another model's live rows, Adam moments and clock into this one
+        (same capacity, both after training_setup): both then continue from the
+        same training state."""
+        if other.capacity != self.capacity or self._m is None or other._m is None:
+            raise ValueError("load_state needs two set-up models of the same capacity")
+        with torch.no_grad():
+            for g in GROUPS:
+                self._buf[g].copy_(other._buf[g])
+                self._m[g].copy_(other._m[g])
+                self._v[g].copy_(other._v[g])
+            self._clock.copy_(other._clock)
+        if self.count != other.count:
+            self.count = other.count
+            self._views()
+
     def clock(self):
         """(iteration, adam steps, learning rates) read from the device clock."""
         c = gvc_train_clock.from_buffer_copy(self._clock.cpu().numpy().tobytes())

@@ -71,7 +71,7 @@ typedef struct gvc_params {
     unsigned flags;       /* GVC_FLAG_*                                           */
     unsigned stages;      /* GVC_STAGE_* mask, 0 = all                            */
     unsigned chunk_slots; /* candidate-log slots per 64-point group, 0 = 16       */
-    unsigned run_points;  /* most points walked together: 16, 32, 64 (0 = 64)     */
+    unsigned run_points;  /* most points walked together: 4, 8, 16, 32, 64 (0 = 64) */
     float fp64_scale_ratio; /* GVC_FLAG_FP64_COND threshold on s_max / s_min:
                                0 = 1000 (cond(Sigma) > 1e6), else in [1, 1e18]   */
     float factor_scale_ratio; /* GVC_FLAG_FP64_COND: s_max / s_min above which a

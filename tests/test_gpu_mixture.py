@@ -426,7 +426,7 @@ def test_candidate_log_overflow_recovers():
         mx._chunk_slots.update(saved)
 
 
-@pytest.mark.parametrize("leaf,block,run", [(4, 512, 32), (8, 1024, 16), (2, 256, 16)])
+@pytest.mark.parametrize("leaf,block,run", [(4, 512, 32), (8, 1024, 16), (2, 256, 16), (4, 256, 4), (16, 512, 8)])
 def test_launch_shapes_parity(leaf, block, run):
     """Block shape, run length and leaf size change the schedule, never the
     predicate: exact pair counts and oracle-level fields and gradients."""
@@ -438,17 +438,34 @@ def test_launch_shapes_parity(leaf, block, run):
     conic_ref, box_ref, _ = oracle.preprocess(g["means"], scales, g["rotations"], g["weights"])
     Vr, Wr, cnt = oracle.forward(g["means"], g["values"], g["weights"], conic_ref, box_ref, g["samples"])
     nrec, npairs, err = st.stats()
-    assert err == 0 and npairs == int(cnt.sum())
+    # short runs log more chunks per group than the default candidate log holds:
+    # the eager backward redoes such a forward with a larger log
+    assert (err & ~mx._lib.GVC_ERRFLAG_CHUNK_OVERFLOW) == 0 and npairs == int(cnt.sum())
     assert field_err(cells.cpu().numpy(), cw.cpu().numpy(), Vr, Wr) <= 1e-5
     _grad_check(g, scales, g["samples"][::4].copy(), params)
 
 
 def test_autotune_small_sweep():
-    from gvc.autotune import autotune
+    """The autotuner times every configuration of its sweep (leaf size x points
+    per walk x block shape), returns the fastest of its own table, and every
+    configuration accepts exactly the oracle's pairs."""
+    from gvc.autotune import autotune, params_for
     g = workloads.config0(6)
     scales = np.exp(g["log_scales"]).astype(f32)
     ss = mx.SampleSet(T(g["samples"]))
-    best, table = autotune(T(g["means"]), T(scales), T(g["rotations"]), T(g["values"]), T(g["weights"]), ss,
-                           leaf_sizes=(2, 4), run_points=(32, 64), blocks=(256,), reps=1, use_cache=False)
-    assert best.leaf_size in (2, 4) and best.run_points in (32, 64)
-    assert len(table) == 4 and all(v is not None and v > 0 for v in table.values())
+    args = (T(g["means"]), T(scales), T(g["rotations"]), T(g["values"]), T(g["weights"]))
+    best, table = autotune(*args, ss, leaf_sizes=(2, 4), points_per_walk=(1, 16, 64), blocks=(256, 512), reps=1,
+                           use_cache=False)
+    assert len(table) == 12 and all(v is not None and v > 0 for v in table.values())
+    label = min(table, key=table.get)
+    d = best.describe()
+    assert label == f"leaf{d['leaf_size']}/points{d['points_per_walk']}/block{d['block_threads']}"
+    conic_ref, box_ref, _ = oracle.preprocess(g["means"], scales, g["rotations"], g["weights"])
+    _, _, cnt = oracle.forward(g["means"], g["values"], g["weights"], conic_ref, box_ref, g["samples"])
+    for L in (2, 4):
+        for ppw in (1, 4, 16, 64):
+            prm = params_for(mx.MixtureParams(), L, ppw, 256)
+            _, _, _, st = mx.mixture_forward(*args, 1.0, ss, prm, want_conics=False)
+            nrec, npairs, err = st.stats()
+            assert npairs == int(cnt.sum()), (L, ppw)
+            assert err & ~mx._lib.GVC_ERRFLAG_CHUNK_OVERFLOW == 0

@@ -14,6 +14,7 @@ import pytest
 
 import oracle
 from oracle import autograd_ref
+from gvc import workloads
 
 f32 = np.float32
 
@@ -538,3 +539,31 @@ def test_binned_forward_equals_brute_force():
     for x, y in zip(a, b):
         assert np.array_equal(x.view(np.uint64) if x.dtype == np.float64 else x,
                               y.view(np.uint64) if y.dtype == np.float64 else y)
+
+
+def test_cpu_plain_baseline_matches_oracle():
+    # The CPU baseline bench.py times (plain fp32 brute force, no compensation)
+    # computes the same thing as the oracle: the same pair set, the field within
+    # 1e-4 relative of the FP64 forward, and its per-Gaussian partial gradients
+    # (dv, dw, dmu, packed dconic) within 1e-3 normwise of the FP64 backward's.
+    g = workloads.config0(0)
+    scales = np.exp(g["log_scales"]).astype(f32)
+    samples = g["samples"][::8].copy()
+    conic, box, _ = oracle.preprocess(g["means"], scales, g["rotations"], g["weights"])
+    rng = np.random.default_rng(29)
+    gV = rng.normal(size=samples.shape[0]).astype(f32)
+    gW = rng.normal(size=samples.shape[0]).astype(f32)
+    V, W, g11, pairs = oracle.cpu_plain_fwdbwd(g["means"], g["values"], g["weights"], conic, box, samples, gV, gW)
+    Vr, Wr, cnt = oracle.forward(g["means"], g["values"], g["weights"], conic, box, samples)
+    assert pairs == int(cnt.sum()) > 0
+    ok = Wr > 0
+    assert np.array_equal(W > 0, ok)
+    assert np.abs(V[ok] - Vr[ok]).max() <= 1e-4 * max(np.abs(Vr[ok]).max(), 1e-3)
+    assert (np.abs(W[ok] - Wr[ok]) / Wr[ok]).max() <= 1e-4
+    ref = oracle.backward(g["means"], scales, g["rotations"], g["values"], g["weights"], conic, box, samples, Vr,
+                          Wr, gV.astype(np.float64), gW.astype(np.float64))
+    for cols, name in ((slice(0, 1), "values"), (slice(1, 2), "weights"), (slice(2, 5), "means"),
+                       (slice(5, 11), "conic")):
+        a = g11[:, cols].astype(np.float64).reshape(-1)
+        b = np.asarray(ref[name], np.float64).reshape(-1)
+        assert np.abs(a - b).max() <= 1e-3 * np.abs(b).max(), name
