@@ -582,6 +582,74 @@ __device__ __forceinline__ float record_power(const float4& A, const float4& B, 
     return pair_power(B.x, B.y, B.z, B.w, C.x, C.y, dx, dy, dz, r0, r1, r2);
 }
 
+// ---------------------------------------------------------------------------
+// Cross-lane exchanges on the VALU (no LDS round trip), for wave64 code whose 64
+// lanes are all active: lane_xor<J>(v) is the value of the lane whose index
+// differs in bit J.  J = 1, 2, 4, 8 use DPP within rows of 16 lanes (quad
+// permutations; the half-row mirror followed by a quad reversal for 4; a row
+// rotation by 8); J = 16, 32 use gfx950's v_permlane16_swap / v_permlane32_swap.
+// ---------------------------------------------------------------------------
+template <int CTRL, int ROWS = 0xF, bool ZERO = false>
+__device__ __forceinline__ int dpp_mov(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, ZERO);
+}
+
+template <int J>
+__device__ __forceinline__ int lane_xor(int v)
+{
+    if constexpr (J == 1) {
+        return dpp_mov<0xB1>(v);                 // quad_perm [1, 0, 3, 2]
+    } else if constexpr (J == 2) {
+        return dpp_mov<0x4E>(v);                 // quad_perm [2, 3, 0, 1]
+    } else if constexpr (J == 4) {
+        return dpp_mov<0x1B>(dpp_mov<0x141>(v)); // half-row mirror (i ^ 7), then quad_perm [3, 2, 1, 0] (i ^ 3)
+    } else if constexpr (J == 8) {
+        return dpp_mov<0x128>(v);                // row_ror:8
+    } else if constexpr (J == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (threadIdx.x & 16u) ? (int)r[0] : (int)r[1];
+    } else {
+        static_assert(J == 32, "lane_xor: J must be a power of two up to 32");
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return (threadIdx.x & 32u) ? (int)r[0] : (int)r[1];
+    }
+}
+template <int J>
+__device__ __forceinline__ float lane_xor(float v)
+{
+    return __builtin_bit_cast(float, lane_xor<J>(__builtin_bit_cast(int, v)));
+}
+template <int J>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v)
+{
+    return (uint32_t)lane_xor<J>((int)v);
+}
+
+// The value of lane (lane - D) of the same wave, 0 for lanes below D (D = 1, 2,
+// 4, 8 shift within rows of 16; callers combine rows with row_carry below).
+template <int D>
+__device__ __forceinline__ uint32_t row_shr(uint32_t v)
+{
+    return (uint32_t)dpp_mov<0x110 + D, 0xF, true>((int)v);
+}
+
+// Inclusive scan over the wave's 64 lanes in lane order with an associative,
+// commutative OP whose identity is 0 (add, or): Hillis-Steele within each row of
+// 16 lanes (DPP row shifts), then the last lane of row 0 broadcast into row 1
+// and of row 2 into row 3, then the last lane of row 1 into rows 2 and 3.
+template <class OP>
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v, OP op)
+{
+    v = op(v, row_shr<1>(v));
+    v = op(v, row_shr<2>(v));
+    v = op(v, row_shr<4>(v));
+    v = op(v, row_shr<8>(v));
+    v = op(v, (uint32_t)dpp_mov<0x142, 0xA>((int)v));  // row_bcast:15 into rows 1, 3
+    v = op(v, (uint32_t)dpp_mov<0x143, 0xC>((int)v));  // row_bcast:31 into rows 2, 3
+    return v;
+}
+
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  For a
 // grid over spatially sorted data, map block b to an index such that every XCD
 // works through one contiguous 1/8 of the data, so neighbours share its L2.

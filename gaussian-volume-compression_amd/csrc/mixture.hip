@@ -52,23 +52,33 @@ __device__ __forceinline__ float uniformf(float v)
 }
 __device__ __forceinline__ int uniformi(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Wave-wide min / max / sum, every lane gets the result (all lanes active).
 __device__ __forceinline__ float wave_minf(float v)
 {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
-    return v;
+    v = fminf(v, lane_xor<1>(v));
+    v = fminf(v, lane_xor<2>(v));
+    v = fminf(v, lane_xor<4>(v));
+    v = fminf(v, lane_xor<8>(v));
+    v = fminf(v, lane_xor<16>(v));
+    return fminf(v, lane_xor<32>(v));
 }
 __device__ __forceinline__ float wave_maxf(float v)
 {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
-    return v;
+    v = fmaxf(v, lane_xor<1>(v));
+    v = fmaxf(v, lane_xor<2>(v));
+    v = fmaxf(v, lane_xor<4>(v));
+    v = fmaxf(v, lane_xor<8>(v));
+    v = fmaxf(v, lane_xor<16>(v));
+    return fmaxf(v, lane_xor<32>(v));
 }
 __device__ __forceinline__ uint32_t wave_sumu(uint32_t v)
 {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
+    v += lane_xor<1>(v);
+    v += lane_xor<2>(v);
+    v += lane_xor<4>(v);
+    v += lane_xor<8>(v);
+    v += lane_xor<16>(v);
+    return v + lane_xor<32>(v);
 }
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v)
@@ -716,7 +726,7 @@ __device__ __forceinline__ int next_group(uint32_t* __restrict__ queue, int NG, 
         const uint32_t nh = (uint32_t)uniformi((int)sc.heavy[0]);
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(sc.cursor, 1u);
-        t = (uint32_t)uniformi(__shfl((int)t, 0, 64));
+        t = (uint32_t)__builtin_amdgcn_readlane((int)t, 0);
         if (t < nh) return uniformi((int)sc.heavy[1 + t]);
         sc.heavy_done = true;
     }
@@ -725,8 +735,7 @@ __device__ __forceinline__ int next_group(uint32_t* __restrict__ queue, int NG, 
         const uint32_t e = (uint32_t)(((uint64_t)NG * (uint32_t)(x + 1)) / kQueues);
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(&queue[x * kLine], 1u);
-        t = (uint32_t)__shfl((int)t, 0, 64);
-        t = (uint32_t)uniformi((int)t);
+        t = (uint32_t)__builtin_amdgcn_readlane((int)t, 0);
         if (t < e - b) {
             const int g = (int)(b + t);
             if (uniformi((int)sc.flag[g]) != 0) continue;
@@ -973,14 +982,23 @@ __global__ __launch_bounds__(256) void group_cost_kernel(const float4* __restric
     const float inf = __builtin_inff();
     float lx = valid ? p.x : inf, ly = valid ? p.y : inf, lz = valid ? p.z : inf;
     float hx = valid ? p.x : -inf, hy = valid ? p.y : -inf, hz = valid ? p.z : -inf;
-#pragma unroll
-    for (int d = 1; d < 16; d <<= 1) {
-        lx = fminf(lx, __shfl_xor(lx, d, 64)); ly = fminf(ly, __shfl_xor(ly, d, 64)); lz = fminf(lz, __shfl_xor(lz, d, 64));
-        hx = fmaxf(hx, __shfl_xor(hx, d, 64)); hy = fmaxf(hy, __shfl_xor(hy, d, 64)); hz = fmaxf(hz, __shfl_xor(hz, d, 64));
-    }
+    auto seg_min = [](float v) {   // min over the lane's 16-lane segment
+        v = fminf(v, lane_xor<1>(v));
+        v = fminf(v, lane_xor<2>(v));
+        v = fminf(v, lane_xor<4>(v));
+        return fminf(v, lane_xor<8>(v));
+    };
+    auto seg_max = [](float v) {
+        v = fmaxf(v, lane_xor<1>(v));
+        v = fmaxf(v, lane_xor<2>(v));
+        v = fmaxf(v, lane_xor<4>(v));
+        return fmaxf(v, lane_xor<8>(v));
+    };
+    lx = seg_min(lx); ly = seg_min(ly); lz = seg_min(lz);
+    hx = seg_max(hx); hy = seg_max(hy); hz = seg_max(hz);
     float c = ((lane & 15) == 0 && hx >= lx) ? (hx - lx + reach[0]) * (hy - ly + reach[1]) * (hz - lz + reach[2]) : 0.f;
-#pragma unroll
-    for (int d = 16; d < 64; d <<= 1) c += __shfl_xor(c, d, 64);
+    c += lane_xor<16>(c);
+    c += lane_xor<32>(c);
     const bool is_heavy = gaussian_density(tv) * c > kHeavyCandidates;
     if (lane == 0) {
         flag[g] = is_heavy ? 1 : 0;
@@ -1072,7 +1090,7 @@ __device__ __forceinline__ uint32_t log_take(LogCursor& c, uint32_t* __restrict_
     if (c.next == c.end) {
         uint32_t b = 0;
         if ((threadIdx.x & 63) == 0) b = atomicAdd(counter, kSlotBlock);
-        b = (uint32_t)uniformi(__shfl((int)b, 0, 64));
+        b = (uint32_t)__builtin_amdgcn_readlane((int)b, 0);
         c.next = b;
         c.end = b + kSlotBlock;
     }
@@ -1088,12 +1106,15 @@ __device__ __forceinline__ uint64_t wave_transpose64(uint64_t a)
     const int lane = threadIdx.x & 63;
     uint32_t lo = (uint32_t)a, hi = (uint32_t)(a >> 32);
     {
-        const uint32_t plo = (uint32_t)__shfl_xor((int)lo, 32, 64), phi = (uint32_t)__shfl_xor((int)hi, 32, 64);
-        if (lane & 32) lo = phi;
-        else hi = plo;
+        // lanes < 32 send their high word and keep the partner's low word
+        const uint32_t send = (lane & 32) ? lo : hi;
+        const uint32_t got = lane_xor<32>(send);
+        if (lane & 32) lo = got;
+        else hi = got;
     }
-    auto stage = [&](const int j, const uint32_t m) {
-        const uint32_t plo = (uint32_t)__shfl_xor((int)lo, j, 64), phi = (uint32_t)__shfl_xor((int)hi, j, 64);
+    auto stage = [&](auto jc, const uint32_t m) {
+        constexpr int j = decltype(jc)::value;
+        const uint32_t plo = lane_xor<j>(lo), phi = lane_xor<j>(hi);
         if (lane & j) {
             lo = (lo & ~m) | ((plo & ~m) >> j);
             hi = (hi & ~m) | ((phi & ~m) >> j);
@@ -1102,11 +1123,11 @@ __device__ __forceinline__ uint64_t wave_transpose64(uint64_t a)
             hi = (hi & m) | ((phi & m) << j);
         }
     };
-    stage(16, 0x0000ffffu);
-    stage(8, 0x00ff00ffu);
-    stage(4, 0x0f0f0f0fu);
-    stage(2, 0x33333333u);
-    stage(1, 0x55555555u);
+    stage(std::integral_constant<int, 16>{}, 0x0000ffffu);
+    stage(std::integral_constant<int, 8>{}, 0x00ff00ffu);
+    stage(std::integral_constant<int, 4>{}, 0x0f0f0f0fu);
+    stage(std::integral_constant<int, 2>{}, 0x33333333u);
+    stage(std::integral_constant<int, 1>{}, 0x55555555u);
     return ((uint64_t)hi << 32) | lo;
 }
 
@@ -1123,11 +1144,27 @@ struct GroupAxes {
     uint64_t pref[3][65];
 };
 
-__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int d)
+// Bitonic sort of (v, id) over the wave's 64 lanes, ascending by (v, id): the
+// compare-exchange passes of merge size K, partner distance J = K/2 .. 1.
+template <int K, int J>
+__device__ __forceinline__ void bitonic_pass(float& v, int& id, int lane)
 {
-    const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, d, 64);
-    const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), d, 64);
-    return ((uint64_t)hi << 32) | lo;
+    const float ov = lane_xor<J>(v);
+    const int oid = lane_xor<J>(id);
+    const bool asc = (lane & K) == 0;
+    const bool low = (lane & J) == 0;
+    const bool other_less = (ov < v) || (ov == v && oid < id);
+    if (low == asc ? other_less : !other_less) {
+        v = ov;
+        id = oid;
+    }
+    if constexpr (J > 1) bitonic_pass<K, J / 2>(v, id, lane);
+}
+template <int K>
+__device__ __forceinline__ void bitonic_sort64(float& v, int& id, int lane)
+{
+    bitonic_pass<K, K / 2>(v, id, lane);
+    if constexpr (K < 64) bitonic_sort64<2 * K>(v, id, lane);
 }
 
 // Returns the mask of lanes whose point has no NaN coordinate among `valid` lanes.
@@ -1136,34 +1173,18 @@ __device__ __forceinline__ uint64_t build_axes(float px, float py, float pz, boo
     const int lane = threadIdx.x & 63;
     const bool finite = valid && px == px && py == py && pz == pz;
     const float inf = __builtin_inff();
+    auto bit_or = [](uint32_t x, uint32_t y) { return x | y; };
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const float c = a == 0 ? px : (a == 1 ? py : pz);
         float v = finite ? c : inf;
         int id = lane;
-#pragma unroll
-        for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                const float ov = __shfl_xor(v, j, 64);
-                const int oid = __shfl_xor(id, j, 64);
-                const bool asc = (lane & k) == 0;
-                const bool low = (lane & j) == 0;
-                const bool other_less = (ov < v) || (ov == v && oid < id);
-                if (low == asc ? other_less : !other_less) {
-                    v = ov;
-                    id = oid;
-                }
-            }
-        }
+        bitonic_sort64<2>(v, id, lane);
         ax.vals[a][lane] = v;
-        uint64_t b = 1ull << id;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint64_t o = shfl_up_u64(b, d);
-            if (lane >= d) b |= o;
-        }
-        ax.pref[a][lane + 1] = b;
+        // prefix masks: lane r holds the lanes of the r + 1 smallest values
+        const uint32_t lo = wave_inclusive_scan(id < 32 ? 1u << id : 0u, bit_or);
+        const uint32_t hi = wave_inclusive_scan(id >= 32 ? 1u << (id - 32) : 0u, bit_or);
+        ax.pref[a][lane + 1] = ((uint64_t)hi << 32) | lo;
         if (lane == 0) ax.pref[a][0] = 0ull;
     }
     return __ballot(finite);
@@ -1205,13 +1226,6 @@ struct FwdLds {
     float4 chunk[64][3];   // mean|w, conic c0..c3, c4 c5 v id
     GroupAxes axes;
 };
-
-__device__ __forceinline__ uint32_t wave_or32(uint32_t v)
-{
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v |= (uint32_t)__shfl_xor((int)v, d, 64);
-    return v;
-}
 
 template <bool KAHAN, bool PRECISE>
 __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict__ rec,
@@ -1511,13 +1525,7 @@ struct BwdLds<false> : BwdLdsBase {
 // Inclusive prefix sum over the wavefront.
 __device__ __forceinline__ uint32_t wave_inscan_u32(uint32_t v)
 {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t o = (uint32_t)__shfl_up((int)v, d, 64);
-        if (lane >= d) v += o;
-    }
-    return v;
+    return wave_inclusive_scan(v, [](uint32_t x, uint32_t y) { return x + y; });
 }
 
 // m with its k lowest set bits cleared (k < popcount(m)).
