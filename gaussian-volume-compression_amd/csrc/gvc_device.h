@@ -475,18 +475,37 @@ __device__ __forceinline__ void covariance_chain(const double* __restrict__ g6, 
 }
 
 // Canonical quadratic form and window (CR/forward.cu:160-166 in a fixed order).
-// Returns the fp32 power; the pair is accepted iff in_window(power).
-__device__ __forceinline__ float pair_power(float c0, float c1, float c2, float c3, float c4, float c5,
-                                            float dx, float dy, float dz, float& r0, float& r1, float& r2)
+// pair_quad returns the fp32 quadratic form; the power is -0.5f * quad and the
+// pair is accepted iff in_window(power).
+__device__ __forceinline__ float pair_quad(float c0, float c1, float c2, float c3, float c4, float c5,
+                                           float dx, float dy, float dz, float& r0, float& r1, float& r2)
 {
     r0 = __builtin_fmaf(c2, dz, __builtin_fmaf(c1, dy, c0 * dx));
     r1 = __builtin_fmaf(c4, dz, __builtin_fmaf(c3, dy, c1 * dx));
     r2 = __builtin_fmaf(c5, dz, __builtin_fmaf(c4, dy, c2 * dx));
-    float quad = __builtin_fmaf(dz, r2, __builtin_fmaf(dy, r1, dx * r0));
-    return -0.5f * quad;
+    return __builtin_fmaf(dz, r2, __builtin_fmaf(dy, r1, dx * r0));
+}
+
+__device__ __forceinline__ float pair_power(float c0, float c1, float c2, float c3, float c4, float c5,
+                                            float dx, float dy, float dz, float& r0, float& r1, float& r2)
+{
+    return -0.5f * pair_quad(c0, c1, c2, c3, c4, c5, dx, dy, dz, r0, r1, r2);
 }
 
 __device__ __forceinline__ bool in_window(float power) { return !(power < -14.0f || power > 0.0f); }
+
+// The same window decided on quad: in_window(-0.5f * q) == quad_in_window(q) for
+// every float q.  -0.5f * q is exact except for an odd multiple of the smallest
+// subnormal, which rounds half to even; so power < -14 iff q > 28, and power > 0
+// iff q < -2^-149 (q = -2^-149 gives power +0); NaN passes both.
+__device__ __forceinline__ bool quad_in_window(float q) { return !(q > 28.0f || q < -0x1p-149f); }
+
+// e^power = 2^(power log2 e) with power = -0.5f * q: q * (-0.5 log2 e) rounds the
+// same product once (the halving is exact), so the weight equals the one formed
+// from the power, bit for bit (subnormal powers give 1 either way).
+constexpr float kLog2eF = 1.4426950408889634f;
+constexpr float kNegHalfLog2e = -0.5f * kLog2eF;
+__device__ __forceinline__ float quad_exp(float q) { return __builtin_amdgcn_exp2f(q * kNegHalfLog2e); }
 
 // Knuth's TwoSum: s + e == a + b exactly (round to nearest, no contraction).
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
@@ -538,6 +557,14 @@ __device__ __forceinline__ float precise_power(float px, float py, float pz, flo
     const float quad = __builtin_fmaf(y2, y2, __builtin_fmaf(y1, y1, y0 * y0));
     return -0.5f * quad;
 }
+// precise_power's quadratic form (-0.5f * precise_quad == precise_power).
+__device__ __forceinline__ float precise_quad(float px, float py, float pz, float mx, float my, float mz, float h0,
+                                              float h1, float h2, float h3, float h4, float h5, const float4& l0123,
+                                              const float4& l45, float& y0, float& y1, float& y2)
+{
+    (void)precise_power(px, py, pz, mx, my, mz, h0, h1, h2, h3, h4, h5, l0123, l45, y0, y1, y2);
+    return __builtin_fmaf(y2, y2, __builtin_fmaf(y1, y1, y0 * y0));
+}
 
 // conic d = U^T y on the precise path (the mean's gradient term of pair_power's r).
 __device__ __forceinline__ void precise_conic_d(float h0, float h1, float h2, float h3, float h4, float h5,
@@ -580,6 +607,33 @@ __device__ __forceinline__ float record_power(const float4& A, const float4& B, 
         }
     }
     return pair_power(B.x, B.y, B.z, B.w, C.x, C.y, dx, dy, dz, r0, r1, r2);
+}
+
+// record_power's quadratic form: -0.5f * record_quad == record_power.
+template <bool PRECISE>
+__device__ __forceinline__ float record_quad(const float4& A, const float4& B, const float4& C,
+                                             const float4* __restrict__ fac_lo, float px, float py, float pz,
+                                             float& dx, float& dy, float& dz, float& r0, float& r1, float& r2)
+{
+    dx = px - A.x;
+    dy = py - A.y;
+    dz = pz - A.z;
+    if constexpr (PRECISE) {
+        const int id = __builtin_bit_cast(int, C.w);
+        if (id < 0) {
+            const size_t j = (size_t)(uint32_t)(id & 0x7fffffff);
+            const float4 l0 = fac_lo[2 * j], l1 = fac_lo[2 * j + 1];
+            float y0, y1, y2;
+            const float q = precise_quad(px, py, pz, A.x, A.y, A.z, B.x, B.y, B.z, B.w, C.x, C.y, l0, l1, y0, y1,
+                                         y2);
+            precise_conic_d(B.x, B.y, B.z, B.w, C.x, C.y, y0, y1, y2, r0, r1, r2);
+            dx = y0;
+            dy = y1;
+            dz = y2;
+            return q;
+        }
+    }
+    return pair_quad(B.x, B.y, B.z, B.w, C.x, C.y, dx, dy, dz, r0, r1, r2);
 }
 
 // ---------------------------------------------------------------------------

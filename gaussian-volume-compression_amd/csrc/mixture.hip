@@ -1006,9 +1006,9 @@ __global__ __launch_bounds__(256) void group_cost_kernel(const float4* __restric
     }
 }
 
-// visit(hit, gi, A = mean|w, B = conic c0..c3, C = c4 c5 v id, power, r0, r1, r2,
-// dx, dy, dz): hit = Gaussian gi's box holds the point and the power is inside
-// the window (the predicate of the shared walk).  Returns false if a lane's
+// visit(hit, gi, A = mean|w, B = conic c0..c3, C = c4 c5 v id, quad, r0, r1, r2,
+// dx, dy, dz): hit = Gaussian gi's box holds the point and the power -0.5 quad is
+// inside the window (the predicate of the shared walk).  Returns false if a lane's
 // stack overflowed (that lane's walk is then incomplete).
 template <bool PRECISE, class Visit>
 __device__ __forceinline__ bool lane_walk(const GRecord* __restrict__ rec, const float4* __restrict__ fac_lo,
@@ -1025,7 +1025,7 @@ __device__ __forceinline__ bool lane_walk(const GRecord* __restrict__ rec, const
         bool hit = false;
         int gi = -1;
         float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A, C = A;
-        float power = 0.f, r0 = 0.f, r1 = 0.f, r2 = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
+        float quad = 0.f, r0 = 0.f, r1 = 0.f, r2 = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
         if (g < gend) {
             gi = g++;
             const float4* q = reinterpret_cast<const float4*>(&rec[gi]);
@@ -1034,8 +1034,8 @@ __device__ __forceinline__ bool lane_walk(const GRecord* __restrict__ rec, const
                 A = q[0];
                 B = q[1];
                 C = q[2];
-                power = record_power<PRECISE>(A, B, C, fac_lo, px, py, pz, dx, dy, dz, r0, r1, r2);
-                hit = in_window(power);
+                quad = record_quad<PRECISE>(A, B, C, fac_lo, px, py, pz, dx, dy, dz, r0, r1, r2);
+                hit = quad_in_window(quad);
             }
         } else if (sp > 0) {
             const int n = stk[--sp];
@@ -1057,7 +1057,7 @@ __device__ __forceinline__ bool lane_walk(const GRecord* __restrict__ rec, const
                 }
             }
         }
-        visit(hit, gi, A, B, C, power, r0, r1, r2, dx, dy, dz);
+        visit(hit, gi, A, B, C, quad, r0, r1, r2, dx, dy, dz);
     }
     return ok;
 }
@@ -1270,7 +1270,7 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
         lds_fence();
 
         float W = 0.f, cW = 0.f, U = 0.f, cU = 0.f;
-        uint32_t npairs = 0, nrec = 0;
+        uint32_t npairs = 0, tp = 0, nrec = 0;   // accepted pairs (per lane, wave-uniform), records
         uint64_t runmask = 0;
         uint32_t head = kLogEnd, prev = kLogEnd;
         bool logged = log_on != 0;
@@ -1318,9 +1318,9 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
             // which leaves the compensated sum W - cW unchanged.
             auto pair = [&](const float4& A, const float4& B, const float4& C, bool live) {
                 float dx, dy, dz, r0, r1, r2;
-                const float power = record_power<PRECISE>(A, B, C, fac_lo, px, py, pz, dx, dy, dz, r0, r1, r2);
-                const bool in = live && in_window(power);
-                const float e = __builtin_amdgcn_exp2f(power * kLog2e);
+                const float quad = record_quad<PRECISE>(A, B, C, fac_lo, px, py, pz, dx, dy, dz, r0, r1, r2);
+                const bool in = live && quad_in_window(quad);
+                const float e = quad_exp(quad);
                 const float wgt = in ? A.w * e : 0.0f;
                 const float wv = in ? wgt * C.z : 0.0f;
                 if (KAHAN) {
@@ -1336,7 +1336,7 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
                     W += wgt;
                     U += wv;
                 }
-                npairs += in ? 1u : 0u;
+                npairs += in ? 1u : 0u;   // per lane: the candidate loop diverges
             };
             while (m != 0ull) {
                 const uint32_t j0 = (uint32_t)__builtin_ctzll(m);
@@ -1355,10 +1355,11 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
         if (lane_mode) {
             const bool active = orig >= 0 && px == px && py == py && pz == pz;
             ok = lane_walk<PRECISE>(rec, fac_lo, nodes, tv, L, px, py, pz, active,
-                           [&](bool hit, int, const float4& A, const float4&, const float4& C, float power, float,
+                           [&](bool hit, int, const float4& A, const float4&, const float4& C, float quad, float,
                                float, float, float, float, float) {
+                               tp += (uint32_t)__popcll(__ballot(hit));
                                if (!hit) return;
-                               const float e = __builtin_amdgcn_exp2f(power * kLog2e);
+                               const float e = quad_exp(quad);
                                const float wgt = A.w * e;
                                const float wv = wgt * C.z;
                                if (KAHAN) {
@@ -1374,9 +1375,8 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
                                    W += wgt;
                                    U += wv;
                                }
-                               ++npairs;
                            }) && ok;
-            nrec = wave_sumu(npairs);   // the backward writes one record per pair
+            nrec = tp;   // the backward writes one record per pair
             head = kLogLaneMode;
             if (lane == 0) atomicAdd(log_counter + (kQLane - kQLog), 1u);   // lane-walked groups
         } else {
@@ -1402,7 +1402,7 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
             out_cells[orig] = V;
             out_weights[orig] = Wo;
         }
-        const uint32_t tp = wave_sumu(npairs);
+        if (!lane_mode) tp = wave_sumu(npairs);
         if (lane == 0) {
             group_records[g] = nrec;
             group_pairs[g] = tp;
@@ -1479,8 +1479,8 @@ __global__ __launch_bounds__(256) void pair_count_kernel(const GRecord* __restri
                 m &= m - 1;
                 const float4 A = cand[wid][j][0], B = cand[wid][j][1], C = cand[wid][j][2];
                 float dx, dy, dz, r0, r1, r2;
-                const float power = record_power<PRECISE>(A, B, C, fac_lo, pme.x, pme.y, pme.z, dx, dy, dz, r0, r1, r2);
-                tally(in_window(power), C);
+                const float quad = record_quad<PRECISE>(A, B, C, fac_lo, pme.x, pme.y, pme.z, dx, dy, dz, r0, r1, r2);
+                tally(quad_in_window(quad), C);
             }
             lds_fence();
             slot = nxt;
@@ -1658,10 +1658,10 @@ __global__ __launch_bounds__(1024) void backward_kernel(
                         const float4 a = s.ptsA[p];
                         const float2 b = s.ptsB[p];
                         float dx, dy, dz, r0_, r1_, r2_;
-                        const float power = record_power<PRECISE>(ca, cb, cc, fac_lo, a.x, a.y, a.z, dx, dy, dz,
-                                                                  r0_, r1_, r2_);
-                        if (in_window(power)) {
-                            const float e = __builtin_amdgcn_exp2f(power * kLog2e);
+                        const float quad = record_quad<PRECISE>(ca, cb, cc, fac_lo, a.x, a.y, a.z, dx, dy, dz,
+                                                                r0_, r1_, r2_);
+                        if (quad_in_window(quad)) {
+                            const float e = quad_exp(quad);
                             const float wgt = cw * e;
                             sv = __builtin_fmaf(b.x, wgt, sv);
                             const float tt = __builtin_fmaf(b.x, cv - a.w, b.y);
@@ -1721,13 +1721,13 @@ __global__ __launch_bounds__(1024) void backward_kernel(
             // one record per accepted pair, in step order then lane order
             const bool active = orig >= 0 && pme.x == pme.x && pme.y == pme.y && pme.z == pme.z;
             ok = lane_walk<PRECISE>(rec, fac_lo, nodes, tv, L, pme.x, pme.y, pme.z, active,
-                           [&](bool hit, int gi, const float4& A, const float4&, const float4& C, float power,
+                           [&](bool hit, int gi, const float4& A, const float4&, const float4& C, float quad,
                                float r0_, float r1_, float r2_, float dx, float dy, float dz) {
                                const uint64_t bal = __ballot(hit);
                                if (hit) {
                                    const uint64_t pos = (uint64_t)base_out + (uint32_t)__popcll(bal & lt);
                                    if (pos < limit && pos < capacity) {
-                                       const float e = __builtin_amdgcn_exp2f(power * kLog2e);
+                                       const float e = quad_exp(quad);
                                        const float wgt = A.w * e;
                                        const float tt = __builtin_fmaf(gVi, C.z - V, gWe);
                                        const float st = wgt * tt;
