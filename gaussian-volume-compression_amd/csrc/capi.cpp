@@ -95,9 +95,11 @@ int loss_backward(int S, const float* cells, const float* weights, const float* 
 size_t tet_locator_bytes(int n_points, int C, unsigned long long cap);
 int tet_locator_build(int n_points, const float* pts, int C, const int* tets, unsigned long long cap, void* loc,
                       size_t bytes, unsigned long long* needed, hipStream_t st);
+int tet_locator_bind_values(int n_points, const float* pts, int C, void* loc, size_t bytes, unsigned long long cap,
+                            const double* values, hipStream_t st);
 int tet_locate_interp(int n_points, const float* pts, int C, const void* loc, size_t bytes, unsigned long long cap,
-                      const double* values, int S, const float* samples, const void* sample_set, int* cell_ids,
-                      double* out_values, float* bary, hipStream_t st);
+                      const double* values, bool bound, int S, const float* samples, const void* sample_set,
+                      int* cell_ids, double* out_values, float* bary, hipStream_t st);
 int grid_sample(const long long dims[3], const float origin[3], const float spacing[3], int dtype, const void* values,
                 int S, const float* samples, void* out, hipStream_t st);
 int cell_interp(int S, const float* samples, const int* cell_ids, const uint8_t* types, const int* offsets,
@@ -306,8 +308,8 @@ int gvc_tet_locate_interp(int n_points, const float* points, int n_tets, const v
     if (S == 0) return GVC_OK;
     GVC_REQUIRE(points && locator && samples && cell_ids, "NULL pointer");
     GVC_REQUIRE(!values_out || point_values, "values_out needs point_values");
-    return tet_locate_interp(n_points, points, n_tets, locator, locator_bytes, entry_capacity, point_values, S,
-                             samples, nullptr, cell_ids, values_out, bary_out, (hipStream_t)stream);
+    return tet_locate_interp(n_points, points, n_tets, locator, locator_bytes, entry_capacity, point_values, false,
+                             S, samples, nullptr, cell_ids, values_out, bary_out, (hipStream_t)stream);
 }
 
 int gvc_tet_locate_interp_set(int n_points, const float* points, int n_tets, const void* locator,
@@ -319,8 +321,29 @@ int gvc_tet_locate_interp_set(int n_points, const float* points, int n_tets, con
     if (S == 0) return GVC_OK;
     GVC_REQUIRE(points && locator && sample_set && cell_ids, "NULL pointer");
     GVC_REQUIRE(!values_out || point_values, "values_out needs point_values");
-    return tet_locate_interp(n_points, points, n_tets, locator, locator_bytes, entry_capacity, point_values, S,
-                             nullptr, sample_set, cell_ids, values_out, bary_out, (hipStream_t)stream);
+    return tet_locate_interp(n_points, points, n_tets, locator, locator_bytes, entry_capacity, point_values, false,
+                             S, nullptr, sample_set, cell_ids, values_out, bary_out, (hipStream_t)stream);
+}
+
+int gvc_tet_locator_bind_values(int n_points, const float* points, int n_tets, void* locator, size_t locator_bytes,
+                                uint64_t entry_capacity, const double* point_values, void* stream)
+{
+    GVC_REQUIRE(n_points > 0 && n_tets > 0, "empty mesh");
+    GVC_REQUIRE(points && locator && point_values, "NULL pointer");
+    return tet_locator_bind_values(n_points, points, n_tets, locator, locator_bytes, entry_capacity, point_values,
+                                   (hipStream_t)stream);
+}
+
+int gvc_tet_locate_bound(int n_points, const float* points, int n_tets, const void* locator, size_t locator_bytes,
+                         uint64_t entry_capacity, int S, const float* samples, const void* sample_set,
+                         int32_t* cell_ids, double* values_out, float* bary_out, void* stream)
+{
+    GVC_REQUIRE(S >= 0, "S < 0");
+    if (S == 0) return GVC_OK;
+    GVC_REQUIRE(points && locator && cell_ids && (samples || sample_set), "NULL pointer");
+    return tet_locate_interp(n_points, points, n_tets, locator, locator_bytes, entry_capacity, nullptr, true, S,
+                             sample_set ? nullptr : samples, sample_set, cell_ids, values_out, bary_out,
+                             (hipStream_t)stream);
 }
 
 int gvc_cell_interp(int S, const float* samples, const int32_t* cell_ids, const uint8_t* cell_types,

@@ -13,20 +13,25 @@
 //   evaluation grid clips samples onto the boundary planes -- and a point on a
 //   face, edge or vertex shared by several tets takes the lowest of them.
 //
-// Locator: a uniform grid over the mesh's bounding box (about one cell per eight
-// tets).  Each cell lists every tet whose bounding box overlaps it as an 8-byte
-// entry: the tet id and that bounding box quantised conservatively to 1/32 of the
-// cell per axis (5 bits per bound), so most candidates are rejected from the
-// entry alone.  The survivors are tested with a per-tet barycentric matrix (float,
-// precomputed in double): b = M (p - v3), evaluated in float; a decision is taken
-// only when every |b_i| exceeds a margin e_i = 2e-6 sum_j |M_ij d_j| + 2e-9
-// |M|_1 |d|_1 with d = p - v3 (e_3 also 6e-7 (1 + sum |b_i|)): the first term
-// covers the float rounding of each M_ij and of the evaluation (<= 5 x 2^-24, a
-// 6x margin), the second the error of the double inverse, |dM_ij| <= 6 u
-// cond_F(T) |M|_F with u = 2^-53 and cond_F(T) <= 1e6 (a 300x margin).  Otherwise -- and always for
-// tets with cond_F(T) > 1e6 -- the exact predicate decides.  A cell's entries are
-// sorted by tet index, so the scan stops at the first tet that contains the point
-// (lowest index wins) or at the first undecided one (the exact pass redoes it).
+// Locator: a uniform grid over the mesh's bounding box (about one cell per tet),
+// each cell split 3 x 3 x 3 into sub-cells.  A cell lists every tet that
+// may meet it as an 8-byte entry: the tet id and a 27-bit mask of the sub-cells
+// the tet may meet -- a sub-cell (expanded by the rounding of the point-to-cell
+// mapping) is dropped when it misses the tet's bounding box or lies strictly
+// outside one of the tet's face planes (double, with a rounding allowance).  So
+// most candidates are rejected from the entry alone, and a cell that the tet's
+// box reaches but the tet does not gets no entry.  The survivors are tested with
+// a per-tet barycentric matrix M (computed in double, stored as float):
+// b = M (p - v3), evaluated in double; a decision is taken only when every |b_i|
+// exceeds a margin e_i = 1e-6 sum_j |M_ij d_j| + 2e-9 |M|_1 |d|_1 with
+// d = p - v3 (and e_3 = e_0 + e_1 + e_2 + 1e-15 for b_3 = 1 - b_0 - b_1 - b_2):
+// the first term covers the float rounding of each M_ij (2^-24, a 16x margin),
+// the second the error of the double inverse, |dM_ij| <= 6 u cond_F(T) |M|_F
+// with u = 2^-53 and cond_F(T) <= 1e6 (a 300x margin).  Otherwise -- and
+// always for tets with cond_F(T) > 1e6 -- the exact predicate decides.  A cell's
+// entries are sorted by tet index, so the scan stops at the first tet that
+// contains the point (lowest index wins) or at the first undecided one (the
+// exact pass redoes it).
 #include <math.h>
 #include <string.h>
 
@@ -58,28 +63,14 @@ namespace {
 
 inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-// Per-tet barycentric matrix, computed in double from the float vertices.
-__global__ void tet_records_kernel(const float* __restrict__ pts, const int* __restrict__ tets, int C,
-                                   TetRec* __restrict__ recs, float* __restrict__ boxes)
+// Inverse of [v0-v3 | v1-v3 | v2-v3] in double from the float vertices (rows give
+// b_0..b_2 = inv (p - v3)); ok = finite, non-zero determinant and
+// cond_F = |T|_F |inv|_F <= 1e6.
+__device__ void tet_inverse(const float* __restrict__ pts, const int v[4], double inv[9], bool& ok)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= C) return;
-    TetRec r;
-    for (int k = 0; k < 4; ++k) r.v[k] = tets[4 * t + k];
     double V[4][3];
     for (int k = 0; k < 4; ++k)
-        for (int a = 0; a < 3; ++a) V[k][a] = pts[3 * r.v[k] + a];
-    for (int a = 0; a < 3; ++a) {
-        r.v3[a] = (float)V[3][a];
-        float lo = pts[3 * r.v[0] + a], hi = lo;
-        for (int k = 1; k < 4; ++k) {
-            float x = pts[3 * r.v[k] + a];
-            lo = fminf(lo, x);
-            hi = fmaxf(hi, x);
-        }
-        boxes[6 * t + a] = lo;
-        boxes[6 * t + 3 + a] = hi;
-    }
+        for (int a = 0; a < 3; ++a) V[k][a] = pts[3 * v[k] + a];
     // T columns: v0-v3, v1-v3, v2-v3
     double T[3][3];
     for (int a = 0; a < 3; ++a)
@@ -94,15 +85,63 @@ __global__ void tet_records_kernel(const float* __restrict__ pts, const int* __r
     const double A21 = T[0][1] * T[2][0] - T[0][0] * T[2][1];
     const double A22 = T[0][0] * T[1][1] - T[0][1] * T[1][0];
     const double det = T[0][0] * A00 + T[0][1] * A10 + T[0][2] * A20;
-    const double inv[9] = { A00 / det, A01 / det, A02 / det, A10 / det, A11 / det,
-                            A12 / det, A20 / det, A21 / det, A22 / det };
+    inv[0] = A00 / det; inv[1] = A01 / det; inv[2] = A02 / det;
+    inv[3] = A10 / det; inv[4] = A11 / det; inv[5] = A12 / det;
+    inv[6] = A20 / det; inv[7] = A21 / det; inv[8] = A22 / det;
     double nT = 0.0, nI = 0.0;
     for (int a = 0; a < 3; ++a)
         for (int c = 0; c < 3; ++c) nT += T[a][c] * T[a][c];
     for (int k = 0; k < 9; ++k) nI += inv[k] * inv[k];
-    const bool ok = det != 0.0 && isfinite(det) && sqrt(nT * nI) <= 1e6;
+    ok = det != 0.0 && isfinite(det) && sqrt(nT * nI) <= 1e6;
+}
+
+// Per-tet barycentric matrix (float copy of tet_inverse) and bounding box.
+__global__ void tet_records_kernel(const float* __restrict__ pts, const int* __restrict__ tets, int C,
+                                   TetRec* __restrict__ recs, float* __restrict__ boxes)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= C) return;
+    TetRec r;
+    for (int k = 0; k < 4; ++k) r.v[k] = tets[4 * t + k];
+    for (int a = 0; a < 3; ++a) {
+        r.v3[a] = pts[3 * r.v[3] + a];
+        float lo = pts[3 * r.v[0] + a], hi = lo;
+        for (int k = 1; k < 4; ++k) {
+            float x = pts[3 * r.v[k] + a];
+            lo = fminf(lo, x);
+            hi = fmaxf(hi, x);
+        }
+        boxes[6 * t + a] = lo;
+        boxes[6 * t + 3 + a] = hi;
+    }
+    double inv[9];
+    bool ok;
+    tet_inverse(pts, r.v, inv, ok);
     for (int k = 0; k < 9; ++k) r.m[k] = ok ? (float)inv[k] : __builtin_nanf("");
     recs[t] = r;
+}
+
+// Bound values (gvc_tet_locator_bind_values): per tet the affine interpolant
+// value(p) = w3 + g . (p - v3), g_j = sum_k inv_kj (w_k - w3), in double.  Tets
+// whose matrix is not kept (cond_F > 1e6, degenerate) store the four vertex
+// values instead and are interpolated from orientation volumes.
+__global__ void tet_coef_kernel(const float* __restrict__ pts, const TetRec* __restrict__ recs,
+                                const double* __restrict__ values, int C, double4* __restrict__ coef)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= C) return;
+    const TetRec r = recs[t];
+    const double w0 = values[r.v[0]], w1 = values[r.v[1]], w2 = values[r.v[2]], w3 = values[r.v[3]];
+    if (r.m[0] != r.m[0]) {
+        coef[t] = make_double4(w0, w1, w2, w3);
+        return;
+    }
+    double inv[9];
+    bool ok;
+    tet_inverse(pts, r.v, inv, ok);
+    const double d0 = w0 - w3, d1 = w1 - w3, d2 = w2 - w3;
+    coef[t] = make_double4(w3, inv[0] * d0 + inv[3] * d1 + inv[6] * d2, inv[1] * d0 + inv[4] * d1 + inv[7] * d2,
+                           inv[2] * d0 + inv[5] * d1 + inv[8] * d2);
 }
 
 __device__ __forceinline__ int cell_coord(float x, float lo, float cell, int dim)
@@ -112,14 +151,120 @@ __device__ __forceinline__ int cell_coord(float x, float lo, float cell, int dim
     return c < 0 ? 0 : (c >= dim ? dim - 1 : c);
 }
 
-// 5-bit position of x inside the cell [clo, clo + cell): floor(32 u), clamped.
-// Monotone in x for a fixed cell, so quantised bounds stay conservative.
-__device__ __forceinline__ uint32_t quant5(float x, float clo, float cell)
+// Sub-cells: each grid cell is split 3 x 3 x 3; an entry carries a 27-bit mask
+// of the sub-cells its tet may meet (bit s0 + 3 s1 + 9 s2).  A point's sub-cell
+// comes from its grid coordinate u = (x - lo) / cell (float, as for the cell):
+// s = floor(3 (u - c)) clamped to [0, 2].  Float rounding moves u by at most
+// 2^-23 dims + 2^-24 (the subtraction, the division, the scaling by 3), so every
+// sub-cell is tested on the build side expanded by kSubMargin (dims + 2) cells
+// (an 8x margin).
+constexpr int kSub = 3;
+constexpr double kSubMargin = 0x1p-20;
+
+__device__ __forceinline__ int sub_index(float u, int c)
 {
-    const float u = (x - clo) / cell * 32.0f;
-    if (!(u > 0.0f)) return 0u;
-    if (u >= 31.0f) return 31u;
-    return (uint32_t)u;
+    const int s = (int)floorf((u - (float)c) * (float)kSub);
+    return s < 0 ? 0 : (s >= kSub ? kSub - 1 : s);
+}
+
+// Inward face planes of a tet from its float vertices, in double: every point x
+// of the closed tet has n_f . x + o_f >= 0 for f = 0..3 (face f is opposite
+// vertex f).  A face is used to prune only if the side of the opposite vertex is
+// certain: |n_f . (v_f - a_f)| above a bound 1e3 x larger than the rounding of
+// the cross product and dot product (~ 8 u sum |n_i| |d_i| with u = 2^-53);
+// otherwise (nearly degenerate tets) the tet is pruned by its bounding box only.
+// tol_f covers the rounding of n_f and of the evaluation at any point within
+// `reach` of the face's first vertex (again with a margin of ~1e3).
+struct FacePlanes {
+    double n[4][3];
+    double o[4];
+    double tol[4];
+    bool use;
+};
+
+__device__ void face_planes(const float* __restrict__ pts, const int v[4], double reach, FacePlanes& F)
+{
+    double V[4][3];
+    for (int k = 0; k < 4; ++k)
+        for (int a = 0; a < 3; ++a) V[k][a] = (double)pts[3 * v[k] + a];
+    F.use = true;
+    for (int f = 0; f < 4; ++f) {
+        const int i0 = f == 0 ? 1 : 0, i1 = f <= 1 ? 2 : 1, i2 = f <= 2 ? 3 : 2;
+        double u[3], w[3], d[3];
+        for (int a = 0; a < 3; ++a) {
+            u[a] = V[i1][a] - V[i0][a];
+            w[a] = V[i2][a] - V[i0][a];
+            d[a] = V[f][a] - V[i0][a];
+        }
+        double n[3] = { u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0] };
+        const double m[3] = { fabs(u[1] * w[2]) + fabs(u[2] * w[1]), fabs(u[2] * w[0]) + fabs(u[0] * w[2]),
+                              fabs(u[0] * w[1]) + fabs(u[1] * w[0]) };
+        const double side = n[0] * d[0] + n[1] * d[1] + n[2] * d[2];
+        const double bound = 1e-12 * (m[0] * fabs(d[0]) + m[1] * fabs(d[1]) + m[2] * fabs(d[2]));
+        if (!(fabs(side) > bound)) F.use = false;
+        const double sg = side < 0.0 ? -1.0 : 1.0;
+        for (int a = 0; a < 3; ++a) F.n[f][a] = sg * n[a];
+        F.o[f] = -(F.n[f][0] * V[i0][0] + F.n[f][1] * V[i0][1] + F.n[f][2] * V[i0][2]);
+        F.tol[f] = 1e-12 * (m[0] + m[1] + m[2]) * (reach + fabs(V[i0][0]) + fabs(V[i0][1]) + fabs(V[i0][2]));
+    }
+}
+
+// Sub-cell mask of a tet in grid cell ci: bit set unless the expanded sub-cell
+// misses the tet's bounding box or lies strictly outside one of its face planes
+// (a conservative test: a set bit may still miss the tet).
+__device__ uint32_t cell_mask(const TetGridHeader* __restrict__ h, const int ci[3], const float* __restrict__ b,
+                              const FacePlanes& F)
+{
+    uint32_t axis_ok[3];
+    double ctr[3][kSub], half[3];
+    for (int a = 0; a < 3; ++a) {
+        // margin: the float rounding of the point's grid coordinate, plus the
+        // double rounding of the sub-cell bounds themselves
+        const double cs = (double)h->cell[a], lo = (double)h->lo[a];
+        const double mg = cs * kSubMargin * (double)(h->dims[a] + 2) +
+                          1e-15 * (fabs(lo) + fabs((double)h->hi[a]) + cs * h->dims[a]);
+        half[a] = cs * (0.5 / kSub) + mg;
+        axis_ok[a] = 0u;
+        for (int s = 0; s < kSub; ++s) {
+            const double xlo = lo + cs * (ci[a] + (double)s / kSub) - mg;
+            const double xhi = lo + cs * (ci[a] + (double)(s + 1) / kSub) + mg;
+            ctr[a][s] = lo + cs * (ci[a] + (s + 0.5) / kSub);
+            if (xhi >= (double)b[a] && xlo <= (double)b[3 + a]) axis_ok[a] |= 1u << s;
+        }
+    }
+    uint32_t mask = 0u;
+    for (int s2 = 0; s2 < kSub; ++s2) {
+        if (!(axis_ok[2] >> s2 & 1u)) continue;
+        for (int s1 = 0; s1 < kSub; ++s1) {
+            if (!(axis_ok[1] >> s1 & 1u)) continue;
+            for (int s0 = 0; s0 < kSub; ++s0) {
+                if (!(axis_ok[0] >> s0 & 1u)) continue;
+                bool in = true;
+                if (F.use) {
+                    for (int f = 0; f < 4 && in; ++f) {
+                        const double top = F.n[f][0] * ctr[0][s0] + F.n[f][1] * ctr[1][s1] + F.n[f][2] * ctr[2][s2] +
+                                           F.o[f] + fabs(F.n[f][0]) * half[0] + fabs(F.n[f][1]) * half[1] +
+                                           fabs(F.n[f][2]) * half[2];
+                        in = top >= -F.tol[f];
+                    }
+                }
+                if (in) mask |= 1u << (s0 + kSub * (s1 + kSub * s2));
+            }
+        }
+    }
+    return mask;
+}
+
+// Face planes of tet t, with the reach of every point tested against them: the
+// tet's bounding-box diagonal plus two cells per axis.
+__device__ __forceinline__ void tet_planes_for(const float* __restrict__ pts, const int* __restrict__ tets, int t,
+                                               const float* __restrict__ b, const TetGridHeader* __restrict__ h,
+                                               FacePlanes& F)
+{
+    const int v[4] = { tets[4 * t], tets[4 * t + 1], tets[4 * t + 2], tets[4 * t + 3] };
+    double reach = 0.0;
+    for (int a = 0; a < 3; ++a) reach += ((double)b[3 + a] - b[a]) + 2.0 * (double)h->cell[a];
+    face_planes(pts, v, reach, F);
 }
 
 // 64-bit total of the per-cell entry counts (the scan that turns them into
@@ -140,34 +285,16 @@ __global__ __launch_bounds__(256) void count_total_kernel(const uint32_t* __rest
     if (threadIdx.x == 0 && part[0]) atomicAdd(total, part[0]);
 }
 
-__global__ void tet_count_kernel(const float* __restrict__ boxes, int C, const TetGridHeader* __restrict__ h,
+__global__ void tet_count_kernel(const float* __restrict__ pts, const int* __restrict__ tets,
+                                 const float* __restrict__ boxes, int C, const TetGridHeader* __restrict__ h,
                                  uint32_t* __restrict__ counts)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= C) return;
     const float* b = boxes + 6 * t;
     if (!(b[0] == b[0] && b[1] == b[1] && b[2] == b[2])) return;
-    int c0[3], c1[3];
-    for (int a = 0; a < 3; ++a) {
-        c0[a] = cell_coord(b[a], h->lo[a], h->cell[a], h->dims[a]);
-        c1[a] = cell_coord(b[3 + a], h->lo[a], h->cell[a], h->dims[a]);
-    }
-    for (int k = c0[2]; k <= c1[2]; ++k)
-        for (int j = c0[1]; j <= c1[1]; ++j)
-            for (int i = c0[0]; i <= c1[0]; ++i)
-                atomicAdd(&counts[(size_t)i + (size_t)h->dims[0] * ((size_t)j + (size_t)h->dims[1] * k)], 1u);
-}
-
-// entry = (tet id, quantised box): bits 0-4 lo.x, 5-9 lo.y, 10-14 lo.z, 15-19 hi.x,
-// 20-24 hi.y, 25-29 hi.z, each as quant5 relative to the entry's cell.
-__global__ void tet_fill_kernel(const float* __restrict__ boxes, int C, const TetGridHeader* __restrict__ h,
-                                const uint32_t* __restrict__ starts, uint32_t* __restrict__ cursor,
-                                uint2* __restrict__ entries, unsigned long long capacity)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= C) return;
-    const float* b = boxes + 6 * t;
-    if (!(b[0] == b[0] && b[1] == b[1] && b[2] == b[2])) return;
+    FacePlanes F;
+    tet_planes_for(pts, tets, t, b, h, F);
     int c0[3], c1[3];
     for (int a = 0; a < 3; ++a) {
         c0[a] = cell_coord(b[a], h->lo[a], h->cell[a], h->dims[a]);
@@ -177,12 +304,35 @@ __global__ void tet_fill_kernel(const float* __restrict__ boxes, int C, const Te
         for (int j = c0[1]; j <= c1[1]; ++j)
             for (int i = c0[0]; i <= c1[0]; ++i) {
                 const int ci[3] = { i, j, k };
-                uint32_t q = 0;
-                for (int a = 0; a < 3; ++a) {
-                    const float clo = h->lo[a] + (float)ci[a] * h->cell[a];
-                    q |= quant5(b[a], clo, h->cell[a]) << (5 * a);
-                    q |= quant5(b[3 + a], clo, h->cell[a]) << (15 + 5 * a);
-                }
+                if (cell_mask(h, ci, b, F) == 0u) continue;
+                atomicAdd(&counts[(size_t)i + (size_t)h->dims[0] * ((size_t)j + (size_t)h->dims[1] * k)], 1u);
+            }
+}
+
+// entry = (tet id, sub-cell mask of the tet in the entry's cell); cells whose
+// mask is empty get no entry (the same test as tet_count_kernel).
+__global__ void tet_fill_kernel(const float* __restrict__ pts, const int* __restrict__ tets,
+                                const float* __restrict__ boxes, int C, const TetGridHeader* __restrict__ h,
+                                const uint32_t* __restrict__ starts, uint32_t* __restrict__ cursor,
+                                uint2* __restrict__ entries, unsigned long long capacity)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= C) return;
+    const float* b = boxes + 6 * t;
+    if (!(b[0] == b[0] && b[1] == b[1] && b[2] == b[2])) return;
+    FacePlanes F;
+    tet_planes_for(pts, tets, t, b, h, F);
+    int c0[3], c1[3];
+    for (int a = 0; a < 3; ++a) {
+        c0[a] = cell_coord(b[a], h->lo[a], h->cell[a], h->dims[a]);
+        c1[a] = cell_coord(b[3 + a], h->lo[a], h->cell[a], h->dims[a]);
+    }
+    for (int k = c0[2]; k <= c1[2]; ++k)
+        for (int j = c0[1]; j <= c1[1]; ++j)
+            for (int i = c0[0]; i <= c1[0]; ++i) {
+                const int ci[3] = { i, j, k };
+                const uint32_t q = cell_mask(h, ci, b, F);
+                if (q == 0u) continue;
                 const size_t g = (size_t)i + (size_t)h->dims[0] * ((size_t)j + (size_t)h->dims[1] * k);
                 const unsigned long long pos = (unsigned long long)starts[g] + atomicAdd(&cursor[g], 1u);
                 if (pos < capacity) entries[pos] = make_uint2((uint32_t)t, q);
@@ -299,14 +449,6 @@ __device__ __forceinline__ int mark_undecided(int i) { return -2 - i; }
 constexpr int kLocateThreads = 256;
 constexpr int kCandBatch = 8;  // candidates a lane collects before testing them
 
-// Per-axis 5-bit fields of an entry: lo <= qp <= hi (padding: lo.x = 31 > hi.x =
-// 0, never passes).
-__device__ __forceinline__ bool entry_box_in(uint32_t q, const uint32_t qp[3])
-{
-    return ((q & 31u) <= qp[0]) && (((q >> 5) & 31u) <= qp[1]) && (((q >> 10) & 31u) <= qp[2]) &&
-           (((q >> 15) & 31u) >= qp[0]) && (((q >> 20) & 31u) >= qp[1]) && (((q >> 25) & 31u) >= qp[2]);
-}
-
 // EXACT = false (filter pass): threads follow the sample set's Morton order when
 // one is given (sorted != nullptr; neighbouring threads share grid cells and
 // tets; results go to the original index carried in .w), else the caller's
@@ -314,17 +456,19 @@ __device__ __forceinline__ bool entry_box_in(uint32_t q, const uint32_t qp[3])
 // is marked (mark_undecided, no result written).  EXACT = true: threads follow
 // the caller's order (coalesced reads of the marks); only marked samples are
 // redone, undecided candidates by the exact predicate.
-// A lane first collects the entries of its grid cell that pass the quantised box
-// test (up to kCandBatch, in tet order, in LDS), then tests those with the
-// filter, the next candidate's record in flight, so a wave runs the filter once
-// per candidate of its busiest lane rather than once per entry of its longest
-// cell.
+// A lane first collects the entries of its grid cell whose sub-cell mask holds
+// the point's sub-cell (up to kCandBatch, in tet order, in LDS), then tests those
+// with the filter, the next candidate's record in flight, so a wave runs the
+// filter once per candidate of its busiest lane rather than once per entry of
+// its longest cell.  Values: coef != nullptr (bound values) evaluates the
+// tet's affine interpolant (one 32-byte read); otherwise barycentric
+// coordinates from orientation volumes weight values[] at the four vertices.
 template <bool EXACT>
 __global__ __launch_bounds__(kLocateThreads) void tet_locate_kernel(
     const float* __restrict__ pts, const TetRec* __restrict__ recs, const TetGridHeader* __restrict__ h,
-    const uint32_t* __restrict__ starts, const uint2* __restrict__ entries, const double* __restrict__ values, int S,
-    const float* __restrict__ samples, const float4* __restrict__ sorted, int* __restrict__ cell_ids,
-    double* __restrict__ out_values, float* __restrict__ bary_out)
+    const uint32_t* __restrict__ starts, const uint2* __restrict__ entries, const double* __restrict__ values,
+    const double4* __restrict__ coef, int S, const float* __restrict__ samples, const float4* __restrict__ sorted,
+    int* __restrict__ cell_ids, double* __restrict__ out_values, float* __restrict__ bary_out)
 {
     __shared__ int cand[kCandBatch][kLocateThreads];
     const int tid = threadIdx.x;
@@ -348,17 +492,18 @@ __global__ __launch_bounds__(kLocateThreads) void tet_locate_kernel(
         p[0] = samples[3 * i]; p[1] = samples[3 * i + 1]; p[2] = samples[3 * i + 2];
     }
     int best = -1;
+    TetRec br;   // best's record
     bool undecided = false;
     bool inside_grid = true;
-    int c[3];
-    uint32_t qp[3];
-    for (int a = 0; a < 3; ++a) {
+    int c[3], sb = 0;
+    for (int a = 0, w = 1; a < 3; ++a, w *= kSub) {
         if (!(p[a] >= h->lo[a] && p[a] <= h->hi[a])) inside_grid = false;
         const float t = (p[a] - h->lo[a]) / h->cell[a];
         int ci = inside_grid ? (int)floorf(t) : 0;
         c[a] = ci < 0 ? 0 : (ci >= h->dims[a] ? h->dims[a] - 1 : ci);
-        qp[a] = quant5(p[a], h->lo[a] + (float)c[a] * h->cell[a], h->cell[a]);
+        sb += w * sub_index(t, c[a]);
     }
+    const uint32_t sbit = 1u << sb;
     if (inside_grid) {
         // entries of a cell are sorted by tet index: the first tet that contains
         // p is the answer (lowest index wins), unless an earlier one is undecided
@@ -371,10 +516,10 @@ __global__ __launch_bounds__(kLocateThreads) void tet_locate_kernel(
             while (k < e && n <= kCandBatch - 4) {
                 uint2 en[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) en[u] = (k + u < e) ? entries[k + u] : make_uint2(0u, 31u);
+                for (int u = 0; u < 4; ++u) en[u] = (k + u < e) ? entries[k + u] : make_uint2(0u, 0u);
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
-                    if (entry_box_in(en[u].y, qp)) cand[n++][tid] = (int)en[u].x;
+                    if (en[u].y & sbit) cand[n++][tid] = (int)en[u].x;
                 k += 4;
             }
             TetRec rn;
@@ -386,6 +531,7 @@ __global__ __launch_bounds__(kLocateThreads) void tet_locate_kernel(
                 const int f = tet_filter(r, p);
                 if (f == 1) {
                     best = t;
+                    br = r;
                     done = true;
                     break;
                 }
@@ -393,6 +539,7 @@ __global__ __launch_bounds__(kLocateThreads) void tet_locate_kernel(
                     if (EXACT) {
                         if (tet_contains_exact(pts, r, p)) {
                             best = t;
+                            br = r;
                             done = true;
                             break;
                         }
@@ -410,16 +557,26 @@ __global__ __launch_bounds__(kLocateThreads) void tet_locate_kernel(
         return;
     }
     cell_ids[s] = best;
+    double v = -1.0;
     double bb[4] = { 0, 0, 0, 0 };
-    if (best >= 0 && (out_values || bary_out)) tet_bary(pts, recs[best], p, bb);
-    if (out_values) {
-        double v = -1.0;
-        if (best >= 0 && values) {
-            const TetRec r = recs[best];
-            v = bb[0] * values[r.v[0]] + bb[1] * values[r.v[1]] + bb[2] * values[r.v[2]] + bb[3] * values[r.v[3]];
+    if (best >= 0) {
+        // the matrix is kept (finite m) for every tet the filter accepts
+        const bool affine = coef && br.m[0] == br.m[0];
+        double4 cf = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (coef && out_values) cf = coef[best];
+        if (bary_out || (out_values && !affine && (coef || values))) tet_bary(pts, br, p, bb);
+        if (out_values) {
+            if (affine)
+                v = cf.x + cf.y * ((double)p[0] - br.v3[0]) + cf.z * ((double)p[1] - br.v3[1]) +
+                    cf.w * ((double)p[2] - br.v3[2]);
+            else if (coef)
+                v = bb[0] * cf.x + bb[1] * cf.y + bb[2] * cf.z + bb[3] * cf.w;
+            else if (values)
+                v = bb[0] * values[br.v[0]] + bb[1] * values[br.v[1]] + bb[2] * values[br.v[2]] +
+                    bb[3] * values[br.v[3]];
         }
-        out_values[s] = v;
     }
+    if (out_values) out_values[s] = v;
     if (bary_out)
         for (int k = 0; k < 4; ++k) bary_out[4 * s + k] = (float)bb[k];
 }
@@ -432,6 +589,7 @@ struct TetLayout {
     TetRec* recs;
     uint2* entries;
     float* boxes;
+    double4* coef;   // bound values' per-tet interpolants (NaN until bound)
     uint32_t *starts, *cursor, *scan_scratch;
     TetLayout(Arena& a, int C, size_t n_grid_max, unsigned long long cap)
     {
@@ -443,14 +601,16 @@ struct TetLayout {
         cursor = a.take<uint32_t>(n_grid_max);
         scan_scratch = a.take<uint32_t>(scan_scratch_elems(n_grid_max) + 1);
         boxes = a.take<float>((size_t)(C > 0 ? C : 1) * 6);
+        coef = a.take<double4>((size_t)(C > 0 ? C : 1));
     }
 };
 
-// Grid resolution: about one grid cell per eight tets (at most 2^26), cells as
-// cubic as the bounding box allows; the cell count never exceeds max_grid_cells.
+// Grid resolution: about one grid cell per kTetsPerCell tets (at most 2^26), cells
+// as cubic as the bounding box allows; the cell count never exceeds max_grid_cells.
+constexpr double kTetsPerCell = 1.0;
 double grid_target(int C)
 {
-    double target = C > 8 ? C / 8.0 : 1.0;
+    double target = C > kTetsPerCell ? C / kTetsPerCell : 1.0;
     return target > 67108864.0 ? 67108864.0 : target;
 }
 size_t max_grid_cells(int C) { return (size_t)(2.0 * grid_target(C)) + 8; }
@@ -642,8 +802,9 @@ int tet_locator_build(int n_points, const float* pts, int C, const int* tets, un
     GVC_HIP(hipMemcpyAsync(l.header, &H, sizeof(H), hipMemcpyHostToDevice, st));
     tet_records_kernel<<<blocks_for(C, 256), 256, 0, st>>>(pts, tets, C, l.recs, l.boxes);
     GVC_LAUNCH_CHECK("tet_records");
+    GVC_TRY(fill_u32(reinterpret_cast<uint32_t*>(l.coef), 0xffffffffu, (size_t)C * 8, st));   // NaN: unbound
     GVC_TRY(fill_u32(l.starts, 0u, n_grid + 1, st));
-    tet_count_kernel<<<blocks_for(C, 256), 256, 0, st>>>(l.boxes, C, l.header, l.starts);
+    tet_count_kernel<<<blocks_for(C, 256), 256, 0, st>>>(pts, tets, l.boxes, C, l.header, l.starts);
     GVC_LAUNCH_CHECK("tet_count");
     // the bounding-box words are free again: 64-bit entry total
     unsigned long long* total64 = reinterpret_cast<unsigned long long*>(l.bbox);
@@ -660,7 +821,8 @@ int tet_locator_build(int n_points, const float* pts, int C, const int* tets, un
     if (needed) *needed = total;
     if (total > cap) return set_error(GVC_ERR_CAPACITY, "tet locator entry capacity too small");
     GVC_TRY(fill_u32(l.cursor, 0u, n_grid, st));
-    tet_fill_kernel<<<blocks_for(C, 256), 256, 0, st>>>(l.boxes, C, l.header, l.starts, l.cursor, l.entries, cap);
+    tet_fill_kernel<<<blocks_for(C, 256), 256, 0, st>>>(pts, tets, l.boxes, C, l.header, l.starts, l.cursor,
+                                                        l.entries, cap);
     GVC_LAUNCH_CHECK("tet_fill");
     tet_sort_cells_kernel<<<blocks_for(n_grid, 4), 256, 0, st>>>(l.starts, n_grid, l.entries);
     GVC_LAUNCH_CHECK("tet_sort_cells");
@@ -680,9 +842,22 @@ int cell_interp(int S, const float* samples, const int* cell_ids, const uint8_t*
     return GVC_OK;
 }
 
+int tet_locator_bind_values(int n_points, const float* pts, int C, void* loc, size_t bytes, unsigned long long cap,
+                            const double* values, hipStream_t st)
+{
+    (void)n_points;
+    if (C <= 0) return GVC_OK;
+    Arena a(loc, bytes);
+    TetLayout l(a, C, max_grid_cells(C), cap);
+    if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "tet locator buffer too small");
+    tet_coef_kernel<<<blocks_for(C, 256), 256, 0, st>>>(pts, l.recs, values, C, l.coef);
+    GVC_LAUNCH_CHECK("tet_coef");
+    return GVC_OK;
+}
+
 int tet_locate_interp(int n_points, const float* pts, int C, const void* loc, size_t bytes, unsigned long long cap,
-                      const double* values, int S, const float* samples, const void* sample_set, int* cell_ids,
-                      double* out_values, float* bary, hipStream_t st)
+                      const double* values, bool bound, int S, const float* samples, const void* sample_set,
+                      int* cell_ids, double* out_values, float* bary, hipStream_t st)
 {
     (void)n_points;
     if (S <= 0) return GVC_OK;
@@ -691,12 +866,14 @@ int tet_locate_interp(int n_points, const float* pts, int C, const void* loc, si
     if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "tet locator buffer too small");
     const float4* sorted = sample_set ? sample_set_points(sample_set, S) : nullptr;
     if ((size_t)S > (size_t)INT32_MAX - 2) return set_error(GVC_ERR_INVALID_ARGUMENT, "too many samples");
+    const double4* coef = bound ? l.coef : nullptr;
+    if (bound) values = nullptr;
     const unsigned nb = blocks_for((size_t)S, kLocateThreads);
-    tet_locate_kernel<false><<<nb, kLocateThreads, 0, st>>>(pts, l.recs, l.header, l.starts, l.entries, values, S,
-                                                            samples, sorted, cell_ids, out_values, bary);
+    tet_locate_kernel<false><<<nb, kLocateThreads, 0, st>>>(pts, l.recs, l.header, l.starts, l.entries, values, coef,
+                                                            S, samples, sorted, cell_ids, out_values, bary);
     GVC_LAUNCH_CHECK("tet_locate");
-    tet_locate_kernel<true><<<nb, kLocateThreads, 0, st>>>(pts, l.recs, l.header, l.starts, l.entries, values, S,
-                                                           samples, sorted, cell_ids, out_values, bary);
+    tet_locate_kernel<true><<<nb, kLocateThreads, 0, st>>>(pts, l.recs, l.header, l.starts, l.entries, values, coef,
+                                                           S, samples, sorted, cell_ids, out_values, bary);
     GVC_LAUNCH_CHECK("tet_locate_exact");
     return GVC_OK;
 }

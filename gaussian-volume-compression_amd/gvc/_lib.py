@@ -108,6 +108,8 @@ _SIGS = {
     "gvc_tet_locator_build": (_I, [_I, _P, _I, _P, _U64, _P, _SZ, ctypes.POINTER(_U64), _P]),
     "gvc_tet_locate_interp": (_I, [_I, _P, _I, _P, _SZ, _U64, _P, _I, _P, _P, _P, _P, _P]),
     "gvc_tet_locate_interp_set": (_I, [_I, _P, _I, _P, _SZ, _U64, _P, _P, _I, _P, _P, _P, _P]),
+    "gvc_tet_locator_bind_values": (_I, [_I, _P, _I, _P, _SZ, _U64, _P, _P]),
+    "gvc_tet_locate_bound": (_I, [_I, _P, _I, _P, _SZ, _U64, _I, _P, _P, _P, _P, _P, _P]),
     "gvc_grid_sample": (_I, [_P, _P, _P, _I, _P, _I, _P, _P, _P]),
     "gvc_knn3_bytes": (_SZ, [_I]),
     "gvc_knn3": (_I, [_I, _P, _P, _P, _SZ, _P]),

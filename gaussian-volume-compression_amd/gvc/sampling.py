@@ -16,9 +16,12 @@ class TetMeshSampler:
     points [N,3] float32, tets [C,4] int (vertex indices), values [N] (optional,
     interpolated as float64).  ``sample(samples)`` returns (values [S] float64 with
     -1 outside, cell_ids [S] int32 with -1 outside) and never leaves the device.
+    With ``bind=True`` (default) the values are bound into the locator as per-tet
+    interpolants (gvc_tet_locator_bind_values); call ``bind_values`` again after
+    changing them.
     """
 
-    def __init__(self, points, tets, values=None, device="cuda"):
+    def __init__(self, points, tets, values=None, device="cuda", bind=True):
         dev = torch.device(device) if not isinstance(points, torch.Tensor) else points.device
         if dev.type != "cuda":
             raise ValueError("TetMeshSampler needs a GPU device; there is no CPU path")
@@ -54,8 +57,22 @@ class TetMeshSampler:
                 check(st, "gvc_tet_locator_build")
             self.locator, self.locator_bytes, self.capacity = buf, nbytes, cap
             self.entries = int(need.value)
+            self.bound = False
+            if bind and self.values is not None:
+                self.bind_values(self.values)
             return
         raise GvcError("tet locator build did not converge")
+
+    def bind_values(self, values):
+        """Stores per-tet interpolants of ``values`` [N] (float64) in the locator;
+        later samples use them (stream-ordered, no host sync)."""
+        v = torch.as_tensor(values, dtype=torch.float64).to(self.device).contiguous().reshape(-1)
+        if v.numel() != self.N:
+            raise ValueError("values must have one entry per point")
+        self.values = v
+        call("gvc_tet_locator_bind_values", self.N, _ptr(self.points), self.C, _ptr(self.locator),
+             self.locator_bytes, self.capacity, _ptr(v), _stream(self.device))
+        self.bound = True
 
     def sample(self, samples, with_values=True, with_bary=False, sample_set=None):
         """Locate ``samples`` [S,3]; with ``sample_set`` (a gvc.mixture.SampleSet of
@@ -65,7 +82,19 @@ class TetMeshSampler:
         vals = torch.empty(S, dtype=torch.float64, device=self.device) if (with_values and self.values is not None) \
             else None
         bary = torch.empty((S, 4), dtype=torch.float32, device=self.device) if with_bary else None
-        if sample_set is not None:
+        if self.bound and vals is not None:
+            s = None
+            if sample_set is not None:
+                if sample_set.device != self.device:
+                    raise ValueError("sample set on another device")
+            else:
+                s = torch.as_tensor(samples, dtype=torch.float32).to(self.device).contiguous()
+                if s.dim() != 2 or s.shape[1] != 3:
+                    raise ValueError("samples must have shape (S, 3)")
+            call("gvc_tet_locate_bound", self.N, _ptr(self.points), self.C, _ptr(self.locator), self.locator_bytes,
+                 self.capacity, S, _ptr(s), _ptr(sample_set.buffer if sample_set is not None else None), _ptr(ids),
+                 _ptr(vals), _ptr(bary), _stream(self.device))
+        elif sample_set is not None:
             if sample_set.device != self.device:
                 raise ValueError("sample set on another device")
             call("gvc_tet_locate_interp_set", self.N, _ptr(self.points), self.C, _ptr(self.locator),

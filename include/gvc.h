@@ -246,6 +246,27 @@ int gvc_tet_locate_interp_set(int n_points, const float *points, int n_tets, con
                               const void *sample_set, int S, int32_t *cell_ids, double *values_out,
                               float *bary_out, void *stream);
 
+/* Bound values: gvc_tet_locator_bind_values stores, per tet, the affine
+ * interpolant of point_values [n_points] f64 (value(p) = w3 + g . (p - v3) in
+ * double; for tets whose barycentric matrix is ill-conditioned, cond_F > 1e6,
+ * the four vertex values) in the locator; stream-ordered, copies nothing back.
+ * gvc_tet_locate_bound then locates like gvc_tet_locate_interp (samples [S,3] in
+ * caller order) or gvc_tet_locate_interp_set (sample_set != NULL, samples
+ * ignored) and evaluates the bound interpolant: one 32-byte read per located
+ * sample instead of four vertices and four values.  Values agree with
+ * gvc_tet_locate_interp's to the rounding of double (1e-9 relative covers tets
+ * up to cond_F = 1e6).  Before the first bind, values_out is NaN where a tet
+ * holds the sample.  Rebind after the values change.  No counterpart in the
+ * reference, whose sampler takes the values with every call
+ * (GS/sample_mesh.cu:380-390); for callers that sample one field many times. */
+int gvc_tet_locator_bind_values(int n_points, const float *points, int n_tets, void *locator,
+                                size_t locator_bytes, uint64_t entry_capacity, const double *point_values,
+                                void *stream);
+int gvc_tet_locate_bound(int n_points, const float *points, int n_tets, const void *locator,
+                         size_t locator_bytes, uint64_t entry_capacity, int S, const float *samples,
+                         const void *sample_set, int32_t *cell_ids, double *values_out, float *bary_out,
+                         void *stream);
+
 /* Mixed meshes (the rest of sample_meshu's cell table, GS/sample_mesh.cu:300-336,
  * interpolated like Viskores Probe): for each sample whose cell (cell_ids[i], e.g.
  * the parent of the tetrahedron gvc_tet_locate_interp found in the cells'

@@ -37,6 +37,63 @@ def lattice_samples(n, rng, count):
     return np.concatenate([rand, np.array(special)]).astype(f32)
 
 
+def test_tet_locate_subcell_boundaries():
+    """Points on and one ulp either side of the locator's sub-cell boundaries.
+    An 8^3 Kuhn mesh on [0, 1]^3 (3,072 tets) gets a 15^3 grid (about one cell
+    per tet) split 3 x 3 x 3, so the sub-cell planes sit at multiples of 1/45:
+    ids must stay the exact predicate's wherever rounding puts the point's
+    sub-cell."""
+    rng = np.random.default_rng(36)
+    pts = workloads.jittered_lattice(rng, 8, 0.3)
+    tets = workloads.kuhn_tets(8)
+    vals = rng.normal(size=pts.shape[0])
+    g = np.arange(46) / 45.0
+    X = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(f32)
+    X = X[rng.choice(X.shape[0], 1500, replace=False)]
+    up = np.nextafter(X, np.float32(np.inf))
+    down = np.nextafter(X, np.float32(-np.inf))
+    mixed = np.where(rng.uniform(size=X.shape) < 0.5, up, down)
+    samples = np.concatenate([X, up, down, mixed]).astype(f32)
+    sampler = TetMeshSampler(torch.tensor(pts, device=DEV), tets, vals)
+    gv, gid = sampler.sample(torch.tensor(samples, device=DEV))
+    ids, out = mesh_ref.locate(pts, tets, samples, vals)
+    assert np.array_equal(gid.cpu().numpy(), ids)
+    ok = ids >= 0
+    assert ok.mean() > 0.9
+    np.testing.assert_allclose(gv.cpu().numpy()[ok], out[ok], rtol=1e-9, atol=1e-12)
+
+
+def test_bound_values_match_vertex_interpolation():
+    """Bound per-tet interpolants (gvc_tet_locate_bound) against the per-call
+    vertex values path (gvc_tet_locate_interp[_set]) and the oracle: same ids and
+    barycentrics, values within 1e-12 relative; rebinding new values follows
+    them (an affine map of the field maps the samples' values)."""
+    from gvc.mixture import SampleSet
+    rng = np.random.default_rng(12)
+    pts = workloads.jittered_lattice(rng, 10, 0.3)
+    tets = workloads.kuhn_tets(10)
+    vals = rng.normal(size=pts.shape[0])
+    samples = lattice_samples(10, rng, 4000)
+    st = torch.tensor(samples, device=DEV)
+    bound = TetMeshSampler(torch.tensor(pts, device=DEV), tets, vals)
+    plain = TetMeshSampler(torch.tensor(pts, device=DEV), tets, vals, bind=False)
+    assert bound.bound and not plain.bound
+    ids, out = mesh_ref.locate(pts, tets, samples, vals)
+    ok = ids >= 0
+    for ss in (None, SampleSet(st)):
+        bv, bid, bb = bound.sample(st, with_bary=True, sample_set=ss)
+        pv, pid, pb = plain.sample(st, with_bary=True, sample_set=ss)
+        assert np.array_equal(bid.cpu().numpy(), ids) and np.array_equal(pid.cpu().numpy(), ids)
+        assert torch.equal(bb, pb)
+        b, p = bv.cpu().numpy(), pv.cpu().numpy()
+        np.testing.assert_allclose(b[ok], p[ok], rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(b[ok], out[ok], rtol=1e-9, atol=1e-12)
+        assert np.all(b[~ok] == -1.0)
+    bound.bind_values(torch.tensor(2.0 * vals + 1.0, device=DEV))
+    b2, _ = bound.sample(st)
+    np.testing.assert_allclose(b2.cpu().numpy()[ok], 2.0 * out[ok] + 1.0, rtol=1e-9, atol=1e-12)
+
+
 @pytest.mark.parametrize("n,jitter", [(4, 0.0), (6, 0.3)])
 def test_tet_locate_bit_exact(n, jitter):
     rng = np.random.default_rng(n)
