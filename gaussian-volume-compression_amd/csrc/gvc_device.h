@@ -713,11 +713,47 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+// Six ordered comparisons (false on NaN), each a vector compare into a scalar
+// lane mask, and-ed on the scalar unit; the lane's own bit of the result is the
+// answer.  Same value as the && chain for every active lane, without the
+// compiler's bit-packing of the chain on the vector unit.
+__device__ __forceinline__ bool all6(bool c0, bool c1, bool c2, bool c3, bool c4, bool c5)
+{
+    const uint64_t m = __builtin_amdgcn_ballot_w64(c0) & __builtin_amdgcn_ballot_w64(c1) &
+                       __builtin_amdgcn_ballot_w64(c2) & __builtin_amdgcn_ballot_w64(c3) &
+                       __builtin_amdgcn_ballot_w64(c4) & __builtin_amdgcn_ballot_w64(c5);
+    return __builtin_amdgcn_inverse_ballot_w64(m);
+}
+
 __device__ __forceinline__ bool boxes_overlap(float alx, float aly, float alz, float ahx, float ahy,
                                               float ahz, float blx, float bly, float blz, float bhx,
                                               float bhy, float bhz)
 {
-    return alx <= bhx && blx <= ahx && aly <= bhy && bly <= ahy && alz <= bhz && blz <= ahz;
+    return all6(alx <= bhx, blx <= ahx, aly <= bhy, bly <= ahy, alz <= bhz, blz <= ahz);
+}
+
+// Wave mask of the active lanes whose box [al, ah] overlaps [bl, bh] (the same
+// comparisons as boxes_overlap), for callers that go on with lane masks.
+__device__ __forceinline__ uint64_t boxes_overlap_mask(float alx, float aly, float alz, float ahx, float ahy,
+                                                       float ahz, float blx, float bly, float blz, float bhx,
+                                                       float bhy, float bhz)
+{
+    return __builtin_amdgcn_ballot_w64(alx <= bhx) & __builtin_amdgcn_ballot_w64(blx <= ahx) &
+           __builtin_amdgcn_ballot_w64(aly <= bhy) & __builtin_amdgcn_ballot_w64(bly <= ahy) &
+           __builtin_amdgcn_ballot_w64(alz <= bhz) & __builtin_amdgcn_ballot_w64(blz <= ahz);
+}
+
+// Set bits of m in the lanes below this one (mbcnt: two vector instructions).
+__device__ __forceinline__ uint32_t lane_rank(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// lo <= p <= hi on every axis (closed box), in the same form.
+__device__ __forceinline__ bool point_in_box(float px, float py, float pz, float lx, float ly, float lz, float hx,
+                                             float hy, float hz)
+{
+    return all6(lx <= px, px <= hx, ly <= py, py <= hy, lz <= pz, pz <= hz);
 }
 
 // Morton bit spread, R/gaussian_renderer/__init__.py:49-56.

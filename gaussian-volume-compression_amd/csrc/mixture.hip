@@ -794,7 +794,6 @@ __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__
     const float* __restrict__ root_box = tv.root_box;
     const int NL = tv.NL;
     const int lane = threadIdx.x & 63;
-    const uint64_t lt = lanemask_lt();
     uint32_t chead = 0, ctail = 0;
     bool ok = true;
     int nl = 0, eb = 0;             // leaves s.leaves[0..nl) under expansion; next Gaussian offset
@@ -826,19 +825,21 @@ __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__
         if (!exp_done) {
             // box-test the next 64 Gaussians of the pending leaves
             const int t = eb + lane;
-            bool hit = false;
-            int g = 0;
+            int g = tv.g_hi;
             if (t < nl * L) {
                 const int q = L > 1 ? (int)__umulhi((uint32_t)t, l_recip) : t;
                 g = tv.g_lo + (s.leaves[q] - tv.leaf_base) * L + (t - q * L);
-                if (g < tv.g_hi) {
-                    const float4 b0 = *reinterpret_cast<const float4*>(&rec[g].lox);
-                    const float2 b1 = *reinterpret_cast<const float2*>(&rec[g].hiy);
-                    hit = box_overlaps(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, gb);
-                }
             }
-            const uint64_t bal = __ballot(hit);
-            if (hit) s.ring[(ctail + (uint32_t)__popcll(bal & lt)) & (kRing - 1)] = g;
+            const bool live = g < tv.g_hi;
+            float4 b0 = make_float4(0.f, 0.f, 0.f, 0.f);
+            float2 b1 = make_float2(0.f, 0.f);
+            if (live) {
+                b0 = *reinterpret_cast<const float4*>(&rec[g].lox);
+                b1 = *reinterpret_cast<const float2*>(&rec[g].hiy);
+            }
+            const uint64_t bal = __ballot(live) & boxes_overlap_mask(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, gb.lx,
+                                                                      gb.ly, gb.lz, gb.hx, gb.hy, gb.hz);
+            if (__builtin_amdgcn_inverse_ballot_w64(bal)) s.ring[(ctail + lane_rank(bal)) & (kRing - 1)] = g;
             ctail += (uint32_t)__popcll(bal);
             eb += 64;
             lds_fence();
@@ -891,26 +892,29 @@ __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__
         int node = -1;
         if (lane < k) node = s.wstack[base + lane];
         sp = base;
-        bool ovl = false, ovr = false;
-        int cl = 0, cr = 0;
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, c = a, d = a;
         if (lane < k) {
             const float4* q = reinterpret_cast<const float4*>(&nodes[node]);
-            const float4 a = q[0], b = q[1], c = q[2], d = q[3];
-            // a = llo.xyz lhi.x | b = lhi.yz rlo.xy | c = rlo.z rhi.xyz | d = left right . .
-            ovl = box_overlaps(a.x, a.y, a.z, a.w, b.x, b.y, gb);
-            ovr = box_overlaps(b.z, b.w, c.x, c.y, c.z, c.w, gb);
-            cl = __builtin_bit_cast(int, d.x);
-            cr = __builtin_bit_cast(int, d.y);
+            a = q[0];
+            b = q[1];
+            c = q[2];
+            d = q[3];
         }
-        const bool pl = ovl && cl >= 0, pr = ovr && cr >= 0;
-        const bool ll = ovl && cl < 0, lr = ovr && cr < 0;
-        const uint64_t bpl = __ballot(pl), bpr = __ballot(pr), bll = __ballot(ll), blr = __ballot(lr);
+        // a = llo.xyz lhi.x | b = lhi.yz rlo.xy | c = rlo.z rhi.xyz | d = left right . .
+        const int cl = __builtin_bit_cast(int, d.x), cr = __builtin_bit_cast(int, d.y);
+        const uint64_t act = __ballot(lane < k);
+        const uint64_t ovl = act & boxes_overlap_mask(a.x, a.y, a.z, a.w, b.x, b.y, gb.lx, gb.ly, gb.lz, gb.hx,
+                                                      gb.hy, gb.hz);
+        const uint64_t ovr = act & boxes_overlap_mask(b.z, b.w, c.x, c.y, c.z, c.w, gb.lx, gb.ly, gb.lz, gb.hx,
+                                                      gb.hy, gb.hz);
+        const uint64_t inl = __ballot(cl >= 0), inr = __ballot(cr >= 0);   // internal children
+        const uint64_t bpl = ovl & inl, bpr = ovr & inr, bll = ovl & ~inl, blr = ovr & ~inr;
         const int npl = __popcll(bpl), npr = __popcll(bpr), nll = __popcll(bll);
         lds_fence();
-        if (pl) s.wstack[sp + __popcll(bpl & lt)] = cl;
-        if (pr) s.wstack[sp + npl + __popcll(bpr & lt)] = cr;
-        if (ll) s.leaves[__popcll(bll & lt)] = ~cl;
-        if (lr) s.leaves[nll + __popcll(blr & lt)] = ~cr;
+        if (__builtin_amdgcn_inverse_ballot_w64(bpl)) s.wstack[sp + lane_rank(bpl)] = cl;
+        if (__builtin_amdgcn_inverse_ballot_w64(bpr)) s.wstack[sp + npl + lane_rank(bpr)] = cr;
+        if (__builtin_amdgcn_inverse_ballot_w64(bll)) s.leaves[lane_rank(bll)] = ~cl;
+        if (__builtin_amdgcn_inverse_ballot_w64(blr)) s.leaves[nll + lane_rank(blr)] = ~cr;
         sp += npl + npr;
         nl = nll + __popcll(blr);
         lds_fence();
@@ -1030,7 +1034,7 @@ __device__ __forceinline__ bool lane_walk(const GRecord* __restrict__ rec, const
             gi = g++;
             const float4* q = reinterpret_cast<const float4*>(&rec[gi]);
             const float4 d = q[3], e = q[4];   // lo.xyz hi.x | hi.yz
-            if (d.x <= px && px <= d.w && d.y <= py && py <= e.x && d.z <= pz && pz <= e.y) {
+            if (point_in_box(px, py, pz, d.x, d.y, d.z, d.w, e.x, e.y)) {
                 A = q[0];
                 B = q[1];
                 C = q[2];
@@ -1046,8 +1050,8 @@ __device__ __forceinline__ bool lane_walk(const GRecord* __restrict__ rec, const
                 const float4* q = reinterpret_cast<const float4*>(&nodes[n]);
                 const float4 a = q[0], b = q[1], c = q[2], d = q[3];
                 // a = llo.xyz lhi.x | b = lhi.yz rlo.xy | c = rlo.z rhi.xyz | d = left right . .
-                const bool ovl = a.x <= px && px <= a.w && a.y <= py && py <= b.x && a.z <= pz && pz <= b.y;
-                const bool ovr = b.z <= px && px <= c.y && b.w <= py && py <= c.z && c.x <= pz && pz <= c.w;
+                const bool ovl = point_in_box(px, py, pz, a.x, a.y, a.z, a.w, b.x, b.y);
+                const bool ovr = point_in_box(px, py, pz, b.z, b.w, c.x, c.y, c.z, c.w);
                 if (sp + 2 > kLaneStack) {
                     ok = false;
                     sp = 0;
