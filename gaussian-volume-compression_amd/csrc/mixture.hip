@@ -803,7 +803,8 @@ __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__
     const uint32_t l_recip = L > 1 ? (uint32_t)((0x100000000ull + (uint64_t)L - 1u) / (uint64_t)L) : 0u;
     if (NL <= 0) return true;
     if (NL == 1) {
-        if (box_overlaps(root_box[0], root_box[1], root_box[2], root_box[3], root_box[4], root_box[5], gb)) {
+        if (boxes_overlap_mask(root_box[0], root_box[1], root_box[2], root_box[3], root_box[4], root_box[5], gb.lx,
+                               gb.ly, gb.lz, gb.hx, gb.hy, gb.hz) != 0ull) {
             if (lane == 0) s.leaves[0] = tv.leaf_base;
             nl = 1;
         }
@@ -813,6 +814,15 @@ __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__
     }
     lds_fence();
     while (true) {
+        // the walk state is wave-uniform: say so, so that it lives in scalar
+        // registers and the loop branches on the scalar unit
+        chead = (uint32_t)uniformi((int)chead);
+        ctail = (uint32_t)uniformi((int)ctail);
+        nl = uniformi(nl);
+        eb = uniformi(eb);
+        sp = uniformi(sp);
+        dnode = uniformi(dnode);
+        ssp = uniformi(ssp);
         const uint32_t pending = ctail - chead;
         const bool exp_done = eb >= nl * L;
         const bool trav_done = sp == 0 && dnode < 0;
@@ -851,9 +861,12 @@ __device__ __forceinline__ bool enumerate_candidates(const GRecord* __restrict__
         if (dnode >= 0) {
             // one node of the depth-first walk
             const BNode nd = nodes[dnode];
-            const bool ovl = box_overlaps(nd.llo[0], nd.llo[1], nd.llo[2], nd.lhi[0], nd.lhi[1], nd.lhi[2], gb);
-            const bool ovr = box_overlaps(nd.rlo[0], nd.rlo[1], nd.rlo[2], nd.rhi[0], nd.rhi[1], nd.rhi[2], gb);
-            const int cl = nd.left, cr = nd.right;
+            // every lane tests the same node: the masks are all lanes or none
+            const bool ovl = boxes_overlap_mask(nd.llo[0], nd.llo[1], nd.llo[2], nd.lhi[0], nd.lhi[1], nd.lhi[2],
+                                                gb.lx, gb.ly, gb.lz, gb.hx, gb.hy, gb.hz) != 0ull;
+            const bool ovr = boxes_overlap_mask(nd.rlo[0], nd.rlo[1], nd.rlo[2], nd.rhi[0], nd.rhi[1], nd.rhi[2],
+                                                gb.lx, gb.ly, gb.lz, gb.hx, gb.hy, gb.hz) != 0ull;
+            const int cl = uniformi(nd.left), cr = uniformi(nd.right);
             if (ovl && cl < 0) { if (lane == 0) s.leaves[nl] = ~cl; ++nl; }
             if (ovr && cr < 0) { if (lane == 0) s.leaves[nl] = ~cr; ++nl; }
             const bool dl = ovl && cl >= 0, dr = ovr && cr >= 0;
@@ -1148,20 +1161,31 @@ struct GroupAxes {
     uint64_t pref[3][65];
 };
 
-// Bitonic sort of (v, id) over the wave's 64 lanes, ascending by (v, id): the
-// compare-exchange passes of merge size K, partner distance J = K/2 .. 1.
+// Bitonic sort of (v, id) over the wave's 64 lanes, ascending by v: the
+// compare-exchange passes of merge size K, partner distance J = K/2 .. 1.  The
+// lane that keeps the smaller of its pair takes the partner's entry if that is
+// strictly smaller, the other lane if it is strictly larger, so equal values
+// stay put on both sides (the pair stays a permutation).  Their order among
+// themselves is immaterial: box_points only reads prefix masks at ranks
+// #(v < x) and #(v <= x), which end between distinct values.
+constexpr uint64_t keep_min_lanes(int K, int J)
+{
+    uint64_t m = 0;
+    for (int l = 0; l < 64; ++l)
+        if (((l & K) == 0) == ((l & J) == 0)) m |= 1ull << l;
+    return m;
+}
+
 template <int K, int J>
 __device__ __forceinline__ void bitonic_pass(float& v, int& id, int lane)
 {
     const float ov = lane_xor<J>(v);
     const int oid = lane_xor<J>(id);
-    const bool asc = (lane & K) == 0;
-    const bool low = (lane & J) == 0;
-    const bool other_less = (ov < v) || (ov == v && oid < id);
-    if (low == asc ? other_less : !other_less) {
-        v = ov;
-        id = oid;
-    }
+    constexpr uint64_t keep_min = keep_min_lanes(K, J);
+    const uint64_t less = __builtin_amdgcn_ballot_w64(ov < v), more = __builtin_amdgcn_ballot_w64(ov > v);
+    const bool take = __builtin_amdgcn_inverse_ballot_w64((keep_min & less) | (~keep_min & more));
+    v = take ? ov : v;
+    id = take ? oid : id;
     if constexpr (J > 1) bitonic_pass<K, J / 2>(v, id, lane);
 }
 template <int K>
