@@ -188,6 +188,7 @@ __device__ void face_planes(const float* __restrict__ pts, const int v[4], doubl
     for (int k = 0; k < 4; ++k)
         for (int a = 0; a < 3; ++a) V[k][a] = (double)pts[3 * v[k] + a];
     F.use = true;
+#pragma unroll
     for (int f = 0; f < 4; ++f) {
         const int i0 = f == 0 ? 1 : 0, i1 = f <= 1 ? 2 : 1, i2 = f <= 2 ? 3 : 2;
         double u[3], w[3], d[3];
@@ -212,11 +213,14 @@ __device__ void face_planes(const float* __restrict__ pts, const int v[4], doubl
 // Sub-cell mask of a tet in grid cell ci: bit set unless the expanded sub-cell
 // misses the tet's bounding box or lies strictly outside one of its face planes
 // (a conservative test: a set bit may still miss the tet).
-__device__ uint32_t cell_mask(const TetGridHeader* __restrict__ h, const int ci[3], const float* __restrict__ b,
-                              const FacePlanes& F)
+__device__ __forceinline__ uint32_t cell_mask(const TetGridHeader* __restrict__ h, const int ci[3],
+                                              const float* __restrict__ b, const FacePlanes& F)
 {
-    uint32_t axis_ok[3];
+    // per axis: sub-cell centres, half-width (with the margin) and the
+    // sub-cells that meet the tet's bounding box
     double ctr[3][kSub], half[3];
+    uint32_t axis_ok[3];
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
         // margin: the float rounding of the point's grid coordinate, plus the
         // double rounding of the sub-cell bounds themselves
@@ -225,6 +229,7 @@ __device__ uint32_t cell_mask(const TetGridHeader* __restrict__ h, const int ci[
                           1e-15 * (fabs(lo) + fabs((double)h->hi[a]) + cs * h->dims[a]);
         half[a] = cs * (0.5 / kSub) + mg;
         axis_ok[a] = 0u;
+#pragma unroll
         for (int s = 0; s < kSub; ++s) {
             const double xlo = lo + cs * (ci[a] + (double)s / kSub) - mg;
             const double xhi = lo + cs * (ci[a] + (double)(s + 1) / kSub) + mg;
@@ -233,24 +238,35 @@ __device__ uint32_t cell_mask(const TetGridHeader* __restrict__ h, const int ci[
         }
     }
     uint32_t mask = 0u;
-    for (int s2 = 0; s2 < kSub; ++s2) {
-        if (!(axis_ok[2] >> s2 & 1u)) continue;
-        for (int s1 = 0; s1 < kSub; ++s1) {
-            if (!(axis_ok[1] >> s1 & 1u)) continue;
-            for (int s0 = 0; s0 < kSub; ++s0) {
-                if (!(axis_ok[0] >> s0 & 1u)) continue;
-                bool in = true;
-                if (F.use) {
-                    for (int f = 0; f < 4 && in; ++f) {
-                        const double top = F.n[f][0] * ctr[0][s0] + F.n[f][1] * ctr[1][s1] + F.n[f][2] * ctr[2][s2] +
-                                           F.o[f] + fabs(F.n[f][0]) * half[0] + fabs(F.n[f][1]) * half[1] +
-                                           fabs(F.n[f][2]) * half[2];
-                        in = top >= -F.tol[f];
-                    }
-                }
-                if (in) mask |= 1u << (s0 + kSub * (s1 + kSub * s2));
-            }
+#pragma unroll
+    for (int s2 = 0; s2 < kSub; ++s2)
+#pragma unroll
+        for (int s1 = 0; s1 < kSub; ++s1)
+#pragma unroll
+            for (int s0 = 0; s0 < kSub; ++s0)
+                if ((axis_ok[0] >> s0) & (axis_ok[1] >> s1) & (axis_ok[2] >> s2) & 1u)
+                    mask |= 1u << (s0 + kSub * (s1 + kSub * s2));
+    if (!F.use || mask == 0u) return mask;
+    // face by face: the sub-cells whose highest point over the face's plane is
+    // not below -tol (n . ctr + |n| . half + o, separable per axis)
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        double tx[kSub], ty[kSub], tz[kSub];
+#pragma unroll
+        for (int s = 0; s < kSub; ++s) {
+            tx[s] = F.n[f][0] * ctr[0][s] + fabs(F.n[f][0]) * half[0];
+            ty[s] = F.n[f][1] * ctr[1][s] + fabs(F.n[f][1]) * half[1];
+            tz[s] = F.n[f][2] * ctr[2][s] + fabs(F.n[f][2]) * half[2] + F.o[f];
         }
+        uint32_t keep = 0u;
+#pragma unroll
+        for (int s2 = 0; s2 < kSub; ++s2)
+#pragma unroll
+            for (int s1 = 0; s1 < kSub; ++s1)
+#pragma unroll
+                for (int s0 = 0; s0 < kSub; ++s0)
+                    if ((tx[s0] + ty[s1]) + tz[s2] >= -F.tol[f]) keep |= 1u << (s0 + kSub * (s1 + kSub * s2));
+        mask &= keep;
     }
     return mask;
 }
@@ -285,7 +301,12 @@ __global__ __launch_bounds__(256) void count_total_kernel(const uint32_t* __rest
     if (threadIdx.x == 0 && part[0]) atomicAdd(total, part[0]);
 }
 
-__global__ void tet_count_kernel(const float* __restrict__ pts, const int* __restrict__ tets,
+// The build kernels keep a tet's four face planes and the sub-cell geometry in
+// registers (about 240 VGPRs, no scratch): launched with kBuildThreads-thread
+// blocks.
+constexpr int kBuildThreads = 64;
+
+__global__ __launch_bounds__(kBuildThreads) void tet_count_kernel(const float* __restrict__ pts, const int* __restrict__ tets,
                                  const float* __restrict__ boxes, int C, const TetGridHeader* __restrict__ h,
                                  uint32_t* __restrict__ counts)
 {
@@ -311,7 +332,7 @@ __global__ void tet_count_kernel(const float* __restrict__ pts, const int* __res
 
 // entry = (tet id, sub-cell mask of the tet in the entry's cell); cells whose
 // mask is empty get no entry (the same test as tet_count_kernel).
-__global__ void tet_fill_kernel(const float* __restrict__ pts, const int* __restrict__ tets,
+__global__ __launch_bounds__(kBuildThreads) void tet_fill_kernel(const float* __restrict__ pts, const int* __restrict__ tets,
                                 const float* __restrict__ boxes, int C, const TetGridHeader* __restrict__ h,
                                 const uint32_t* __restrict__ starts, uint32_t* __restrict__ cursor,
                                 uint2* __restrict__ entries, unsigned long long capacity)
@@ -804,7 +825,8 @@ int tet_locator_build(int n_points, const float* pts, int C, const int* tets, un
     GVC_LAUNCH_CHECK("tet_records");
     GVC_TRY(fill_u32(reinterpret_cast<uint32_t*>(l.coef), 0xffffffffu, (size_t)C * 8, st));   // NaN: unbound
     GVC_TRY(fill_u32(l.starts, 0u, n_grid + 1, st));
-    tet_count_kernel<<<blocks_for(C, 256), 256, 0, st>>>(pts, tets, l.boxes, C, l.header, l.starts);
+    tet_count_kernel<<<blocks_for(C, kBuildThreads), kBuildThreads, 0, st>>>(pts, tets, l.boxes, C, l.header,
+                                                                              l.starts);
     GVC_LAUNCH_CHECK("tet_count");
     // the bounding-box words are free again: 64-bit entry total
     unsigned long long* total64 = reinterpret_cast<unsigned long long*>(l.bbox);
@@ -821,8 +843,8 @@ int tet_locator_build(int n_points, const float* pts, int C, const int* tets, un
     if (needed) *needed = total;
     if (total > cap) return set_error(GVC_ERR_CAPACITY, "tet locator entry capacity too small");
     GVC_TRY(fill_u32(l.cursor, 0u, n_grid, st));
-    tet_fill_kernel<<<blocks_for(C, 256), 256, 0, st>>>(pts, tets, l.boxes, C, l.header, l.starts, l.cursor,
-                                                        l.entries, cap);
+    tet_fill_kernel<<<blocks_for(C, kBuildThreads), kBuildThreads, 0, st>>>(pts, tets, l.boxes, C, l.header,
+                                                                             l.starts, l.cursor, l.entries, cap);
     GVC_LAUNCH_CHECK("tet_fill");
     tet_sort_cells_kernel<<<blocks_for(n_grid, 4), 256, 0, st>>>(l.starts, n_grid, l.entries);
     GVC_LAUNCH_CHECK("tet_sort_cells");
