@@ -1342,15 +1342,31 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
             // each point lane walks its candidates in ascending order, two per pass
             // (both loads issued before either pair is evaluated)
             const float4* __restrict__ cb = &s.chunk[0][0];
-            // Branch-free: a rejected pair (or the missing second one) adds zero,
-            // which leaves the compensated sum W - cW unchanged.
-            auto pair = [&](const float4& A, const float4& B, const float4& C, bool live) {
+            // Branch-free: a rejected pair (or the missing second one) adds zero.
+            auto pair = [&](const float4& A, const float4& B, const float4& C, bool live, float& wgt, float& wv) {
                 float dx, dy, dz, r0, r1, r2;
                 const float quad = record_quad<PRECISE>(A, B, C, fac_lo, px, py, pz, dx, dy, dz, r0, r1, r2);
                 const bool in = live && quad_in_window(quad);
                 const float e = quad_exp(quad);
-                const float wgt = in ? A.w * e : 0.0f;
-                const float wv = in ? wgt * C.z : 0.0f;
+                wgt = in ? A.w * e : 0.0f;
+                wv = in ? wgt * C.z : 0.0f;
+                npairs += in ? 1u : 0u;   // per lane: the candidate loop diverges
+            };
+            while (m != 0ull) {
+                const uint32_t j0 = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                const bool two = m != 0ull;
+                const uint32_t j1 = two ? (uint32_t)__builtin_ctzll(m) : j0;
+                m &= m - 1;  // stays 0 when there was no second bit
+                const float4 A0 = cb[3 * j0], B0 = cb[3 * j0 + 1], C0 = cb[3 * j0 + 2];
+                const float4 A1 = cb[3 * j1], B1 = cb[3 * j1 + 1], C1 = cb[3 * j1 + 2];
+                float w0, v0, w1, v1;
+                pair(A0, B0, C0, true, w0, v0);
+                pair(A1, B1, C1, two, w1, v1);
+                // the two terms are added first (one rounding of at most u of
+                // their sum), then compensated into the running sums: the total
+                // error stays within a few u of the sum
+                const float wgt = w0 + w1, wv = v0 + v1;
                 if (KAHAN) {
                     float y = wgt - cW;
                     float t = W + y;
@@ -1364,18 +1380,6 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
                     W += wgt;
                     U += wv;
                 }
-                npairs += in ? 1u : 0u;   // per lane: the candidate loop diverges
-            };
-            while (m != 0ull) {
-                const uint32_t j0 = (uint32_t)__builtin_ctzll(m);
-                m &= m - 1;
-                const bool two = m != 0ull;
-                const uint32_t j1 = two ? (uint32_t)__builtin_ctzll(m) : j0;
-                m &= m - 1;  // stays 0 when there was no second bit
-                const float4 A0 = cb[3 * j0], B0 = cb[3 * j0 + 1], C0 = cb[3 * j0 + 2];
-                const float4 A1 = cb[3 * j1], B1 = cb[3 * j1 + 1], C1 = cb[3 * j1 + 2];
-                pair(A0, B0, C0, true);
-                pair(A1, B1, C1, two);
             }
             lds_fence();
         };
