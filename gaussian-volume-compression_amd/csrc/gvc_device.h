@@ -10,6 +10,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace gvc {
 
@@ -643,10 +644,16 @@ __device__ __forceinline__ float record_quad(const float4& A, const float4& B, c
 // permutations; the half-row mirror followed by a quad reversal for 4; a row
 // rotation by 8); J = 16, 32 use gfx950's v_permlane16_swap / v_permlane32_swap.
 // ---------------------------------------------------------------------------
+// dpp_mov: lanes outside ROWS, and lanes without a source when ZERO, get 0.  A
+// full-row move that cannot lack a source (a permutation) leaves the old value
+// undefined, so no register has to be cleared first.
 template <int CTRL, int ROWS = 0xF, bool ZERO = false>
 __device__ __forceinline__ int dpp_mov(int v)
 {
-    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, ZERO);
+    if constexpr (ROWS == 0xF && !ZERO)
+        return __builtin_amdgcn_mov_dpp(v, CTRL, ROWS, 0xF, false);
+    else
+        return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, ZERO);
 }
 
 template <int J>
@@ -702,6 +709,42 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v, OP op)
     v = op(v, (uint32_t)dpp_mov<0x142, 0xA>((int)v));  // row_bcast:15 into rows 1, 3
     v = op(v, (uint32_t)dpp_mov<0x143, 0xC>((int)v));  // row_bcast:31 into rows 2, 3
     return v;
+}
+
+// Order-preserving integer key of a float (an involution): non-negative floats
+// keep their bits, negative ones flip all but the sign bit, so signed integer
+// order is float order (-0 just below +0).  NaN has no meaningful key: callers
+// exclude it.
+__device__ __forceinline__ int fkey(float x)
+{
+    const int b = __builtin_bit_cast(int, x);
+    return b ^ ((b >> 31) & 0x7fffffff);
+}
+__device__ __forceinline__ float fkey_inv(int k) { return __builtin_bit_cast(float, k ^ ((k >> 31) & 0x7fffffff)); }
+constexpr int kKeyPosInf = 0x7f800000;        // fkey(+inf)
+constexpr int kKeyNegInf = (int)0x807fffffu;  // fkey(-inf)
+
+// Min / max of integer keys over the wave (all lanes active), returned
+// wave-uniform: an inclusive scan within each row of 16 lanes (DPP row shifts; a
+// lane without a source keeps its value), then row_bcast:15 into rows 1 and 3 and
+// row_bcast:31 into rows 2 and 3, so lane 63 holds the whole wave's result.  The
+// old value of each DPP move is the operation's identity, so the move folds into
+// the min / max: one VALU instruction per step.
+template <bool MAX>
+__device__ __forceinline__ int wave_extreme_key(int k)
+{
+    constexpr int id = MAX ? (int)0x80000000u : 0x7fffffff;
+    auto step = [&](auto ctrl, auto rows) {
+        const int o = __builtin_amdgcn_update_dpp(id, k, decltype(ctrl)::value, decltype(rows)::value, 0xF, false);
+        k = MAX ? (o > k ? o : k) : (o < k ? o : k);
+    };
+    step(std::integral_constant<int, 0x111>{}, std::integral_constant<int, 0xF>{});  // row_shr:1
+    step(std::integral_constant<int, 0x112>{}, std::integral_constant<int, 0xF>{});  // row_shr:2
+    step(std::integral_constant<int, 0x114>{}, std::integral_constant<int, 0xF>{});  // row_shr:4
+    step(std::integral_constant<int, 0x118>{}, std::integral_constant<int, 0xF>{});  // row_shr:8
+    step(std::integral_constant<int, 0x142>{}, std::integral_constant<int, 0xA>{});  // row_bcast:15
+    step(std::integral_constant<int, 0x143>{}, std::integral_constant<int, 0xC>{});  // row_bcast:31
+    return __builtin_amdgcn_readlane(k, 63);
 }
 
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  For a

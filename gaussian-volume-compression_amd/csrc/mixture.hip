@@ -52,25 +52,7 @@ __device__ __forceinline__ float uniformf(float v)
 }
 __device__ __forceinline__ int uniformi(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Wave-wide min / max / sum, every lane gets the result (all lanes active).
-__device__ __forceinline__ float wave_minf(float v)
-{
-    v = fminf(v, lane_xor<1>(v));
-    v = fminf(v, lane_xor<2>(v));
-    v = fminf(v, lane_xor<4>(v));
-    v = fminf(v, lane_xor<8>(v));
-    v = fminf(v, lane_xor<16>(v));
-    return fminf(v, lane_xor<32>(v));
-}
-__device__ __forceinline__ float wave_maxf(float v)
-{
-    v = fmaxf(v, lane_xor<1>(v));
-    v = fmaxf(v, lane_xor<2>(v));
-    v = fmaxf(v, lane_xor<4>(v));
-    v = fmaxf(v, lane_xor<8>(v));
-    v = fmaxf(v, lane_xor<16>(v));
-    return fmaxf(v, lane_xor<32>(v));
-}
+// Wave-wide sum, every lane gets the result (all lanes active).
 __device__ __forceinline__ uint32_t wave_sumu(uint32_t v)
 {
     v += lane_xor<1>(v);
@@ -584,14 +566,6 @@ struct GroupBox {
     float lx, ly, lz, hx, hy, hz;
 };
 
-__device__ __forceinline__ GroupBox group_box(float px, float py, float pz)
-{
-    GroupBox b;
-    b.lx = uniformf(wave_minf(px)); b.ly = uniformf(wave_minf(py)); b.lz = uniformf(wave_minf(pz));
-    b.hx = uniformf(wave_maxf(px)); b.hy = uniformf(wave_maxf(py)); b.hz = uniformf(wave_maxf(pz));
-    return b;
-}
-
 __device__ __forceinline__ bool box_overlaps(float lx, float ly, float lz, float hx, float hy, float hz,
                                              const GroupBox& b)
 {
@@ -621,17 +595,16 @@ __device__ __forceinline__ void lds_fence()
     __builtin_amdgcn_wave_barrier();
 }
 
-// Box of the points of the lanes with `in` set (wave-uniform result).
+// Box of the points of the lanes with `in` set (wave-uniform result; NaN
+// coordinates are skipped, as fminf / fmaxf skip them; no such point gives the
+// empty box +inf .. -inf).
 __device__ __forceinline__ GroupBox masked_box(float px, float py, float pz, bool in)
 {
-    const float inf = __builtin_inff();
+    auto lo = [&](float v) { return fkey_inv(wave_extreme_key<false>(in && v == v ? fkey(v) : kKeyPosInf)); };
+    auto hi = [&](float v) { return fkey_inv(wave_extreme_key<true>(in && v == v ? fkey(v) : kKeyNegInf)); };
     GroupBox b;
-    b.lx = uniformf(wave_minf(in ? px : inf));
-    b.ly = uniformf(wave_minf(in ? py : inf));
-    b.lz = uniformf(wave_minf(in ? pz : inf));
-    b.hx = uniformf(wave_maxf(in ? px : -inf));
-    b.hy = uniformf(wave_maxf(in ? py : -inf));
-    b.hz = uniformf(wave_maxf(in ? pz : -inf));
+    b.lx = lo(px); b.ly = lo(py); b.lz = lo(pz);
+    b.hx = hi(px); b.hy = hi(py); b.hz = hi(pz);
     return b;
 }
 
@@ -996,28 +969,32 @@ __global__ __launch_bounds__(256) void group_cost_kernel(const float4* __restric
     mean_extent(ext_sum + 4 * vol, reach);
     const float4 p = pts[(size_t)g * 64 + lane];
     const bool valid = __builtin_bit_cast(int, p.w) >= 0 && p.x == p.x && p.y == p.y && p.z == p.z;
-    const float inf = __builtin_inff();
-    float lx = valid ? p.x : inf, ly = valid ? p.y : inf, lz = valid ? p.z : inf;
-    float hx = valid ? p.x : -inf, hy = valid ? p.y : -inf, hz = valid ? p.z : -inf;
-    auto seg_min = [](float v) {   // min over the lane's 16-lane segment
-        v = fminf(v, lane_xor<1>(v));
-        v = fminf(v, lane_xor<2>(v));
-        v = fminf(v, lane_xor<4>(v));
-        return fminf(v, lane_xor<8>(v));
+    // min / max over each 16-lane segment: an inclusive scan of the coordinates'
+    // integer keys within the row (DPP row shifts, one instruction per step;
+    // lanes without a source keep their value), complete at the row's last lane
+    auto seg = [&](float v, auto mx) {
+        constexpr bool MAX = decltype(mx)::value;
+        constexpr int id = MAX ? (int)0x80000000u : 0x7fffffff;
+        int k = valid ? fkey(v) : (MAX ? kKeyNegInf : kKeyPosInf);
+        auto step = [&](auto ctrl) {
+            const int o = __builtin_amdgcn_update_dpp(id, k, decltype(ctrl)::value, 0xF, 0xF, false);
+            k = MAX ? (o > k ? o : k) : (o < k ? o : k);
+        };
+        step(std::integral_constant<int, 0x111>{});
+        step(std::integral_constant<int, 0x112>{});
+        step(std::integral_constant<int, 0x114>{});
+        step(std::integral_constant<int, 0x118>{});
+        return fkey_inv(k);
     };
-    auto seg_max = [](float v) {
-        v = fmaxf(v, lane_xor<1>(v));
-        v = fmaxf(v, lane_xor<2>(v));
-        v = fmaxf(v, lane_xor<4>(v));
-        return fmaxf(v, lane_xor<8>(v));
-    };
-    lx = seg_min(lx); ly = seg_min(ly); lz = seg_min(lz);
-    hx = seg_max(hx); hy = seg_max(hy); hz = seg_max(hz);
-    float c = ((lane & 15) == 0 && hx >= lx) ? (hx - lx + reach[0]) * (hy - ly + reach[1]) * (hz - lz + reach[2]) : 0.f;
+    const std::false_type kmin;
+    const std::true_type kmax;
+    const float lx = seg(p.x, kmin), ly = seg(p.y, kmin), lz = seg(p.z, kmin);
+    const float hx = seg(p.x, kmax), hy = seg(p.y, kmax), hz = seg(p.z, kmax);
+    float c = ((lane & 15) == 15 && hx >= lx) ? (hx - lx + reach[0]) * (hy - ly + reach[1]) * (hz - lz + reach[2]) : 0.f;
     c += lane_xor<16>(c);
     c += lane_xor<32>(c);
     const bool is_heavy = gaussian_density(tv) * c > kHeavyCandidates;
-    if (lane == 0) {
+    if (lane == 15) {   // the segment sums end in lanes 15, 31, 47, 63
         flag[g] = is_heavy ? 1 : 0;
         if (is_heavy) heavy[1 + atomicAdd(heavy, 1u)] = (uint32_t)g;
     }
@@ -1219,18 +1196,23 @@ __device__ __forceinline__ uint64_t build_axes(float px, float py, float pz, boo
 }
 
 // #(vals < x) (strict) or #(vals <= x) over 64 ascending values in LDS.
+// The search moves an LDS pointer (one add and one select per step; the probe's
+// displacement is an immediate of the LDS read).  After the six halving steps at
+// most 63 values have passed, so the last probe is always inside the array.
+typedef __attribute__((address_space(3))) const float lds_cfloat;
 template <bool STRICT>
 __device__ __forceinline__ int rank64(const float* __restrict__ vals, float x)
 {
-    int c = 0;
+    lds_cfloat* const v0 = (lds_cfloat*)vals;
+    lds_cfloat* q = v0;   // q - v0 = values known to pass
 #pragma unroll
     for (int st = 32; st >= 1; st >>= 1) {
-        const float v = vals[c + st - 1];
-        if (STRICT ? (v < x) : (v <= x)) c += st;
+        const float v = q[st - 1];
+        if (STRICT ? (v < x) : (v <= x)) q += st;
     }
-    const float v = vals[c < 63 ? c : 63];
-    if (c < 64 && (STRICT ? (v < x) : (v <= x))) ++c;
-    return c;
+    const float v = q[0];
+    if (STRICT ? (v < x) : (v <= x)) ++q;
+    return (int)(q - v0);
 }
 
 // Points of the group inside the box (lox..hiz), before run and finite masks.
@@ -1847,16 +1829,18 @@ __global__ __launch_bounds__(1024) void backward_kernel(
     if (!ok && lane == 0) atomicOr(err, (uint32_t)GVC_ERRFLAG_STACK_OVERFLOW);
 }
 
+// Segment bounds of the sorted records: seg[2 g] = first record of Gaussian g,
+// seg[2 g + 1] = one past its last (both zero for a Gaussian without records).
 __global__ void segment_kernel(const uint32_t* __restrict__ keys, uint64_t n_max, const uint32_t* __restrict__ n_dev,
-                               uint32_t P, uint32_t* __restrict__ begin, uint32_t* __restrict__ end)
+                               uint32_t P, uint32_t* __restrict__ seg)
 {
     const uint64_t n = (uint64_t)*n_dev < n_max ? (uint64_t)*n_dev : n_max;
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t key = keys[k];
     if (key >= P) return;
-    if (k == 0 || keys[k - 1] != key) begin[key] = (uint32_t)k;
-    if (k + 1 == n || keys[k + 1] != key) end[key] = (uint32_t)(k + 1);
+    if (k == 0 || keys[k - 1] != key) seg[2 * (size_t)key] = (uint32_t)k;
+    if (k + 1 == n || keys[k + 1] != key) seg[2 * (size_t)key + 1] = (uint32_t)(k + 1);
 }
 
 // Gradient sums of one Gaussian, in BVH order: 11 doubles and a pad (96 bytes).
@@ -1916,10 +1900,10 @@ __device__ __forceinline__ void store_sums(double* __restrict__ sums, int gi, co
 // lanes (team_sums); a wave covers 64 consecutive Gaussians in TEAM passes (lane
 // L first loads Gaussian L's segment bounds; a pass whose teams have no records
 // is skipped), hands lane L the sums of Gaussian L and writes them to sums[] in
-// BVH order.  Longer segments are listed for reduce_long_kernel.
+// BVH order.  Longer segments are listed for reduce_long_kernel.  Gaussians
+// without records get no sums (the chain reads their empty segment instead).
 template <int TEAM>
-__global__ __launch_bounds__(256) void reduce_sums_kernel(int P, const uint32_t* __restrict__ begin,
-                                                          const uint32_t* __restrict__ end,
+__global__ __launch_bounds__(256) void reduce_sums_kernel(int P, const uint2* __restrict__ seg,
                                                           const uint32_t* __restrict__ sorted_vals,
                                                           const PartialRecord* __restrict__ parts,
                                                           double* __restrict__ sums, uint32_t* __restrict__ long_list)
@@ -1932,8 +1916,9 @@ __global__ __launch_bounds__(256) void reduce_sums_kernel(int P, const uint32_t*
     const int team = lane / TEAM, j = lane % TEAM;
     const int gi = g0 + lane;
     const bool valid = gi < P;
-    uint32_t my_b = valid ? begin[gi] : 0u;
-    uint32_t my_e = valid ? end[gi] : 0u;
+    const uint2 be = valid ? seg[gi] : make_uint2(0u, 0u);
+    uint32_t my_b = be.x, my_e = be.y;
+    const bool empty = my_e == my_b;
     const bool is_long = my_e - my_b > kLongSegment;
     const uint64_t longs = __ballot(is_long);
     if (longs != 0ull) {
@@ -1960,14 +1945,13 @@ __global__ __launch_bounds__(256) void reduce_sums_kernel(int P, const uint32_t*
             if (lane / kPer == p) mine[c] = x;
         }
     }
-    if (!valid || is_long) return;
+    if (!valid || is_long || empty) return;
     store_sums(sums, gi, mine);
 }
 
 // One wave per listed Gaussian (long segments): team_sums with 64 lanes.  A
 // fixed grid takes the list with a grid-stride loop over its device-side count.
-__global__ __launch_bounds__(256) void reduce_long_kernel(const uint32_t* __restrict__ begin,
-                                                          const uint32_t* __restrict__ end,
+__global__ __launch_bounds__(256) void reduce_long_kernel(const uint2* __restrict__ seg,
                                                           const uint32_t* __restrict__ sorted_vals,
                                                           const PartialRecord* __restrict__ parts,
                                                           const uint32_t* __restrict__ long_list,
@@ -1979,7 +1963,8 @@ __global__ __launch_bounds__(256) void reduce_long_kernel(const uint32_t* __rest
     for (uint32_t w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); w < n; w += waves) {
         const int gi = (int)long_list[1 + w];
         double acc[11];
-        team_sums<64>(begin[gi], end[gi], lane, sorted_vals, parts, acc);
+        const uint2 be = seg[gi];
+        team_sums<64>(be.x, be.y, lane, sorted_vals, parts, acc);
         if (lane == 0) store_sums(sums, gi, acc);
     }
 }
@@ -1988,11 +1973,13 @@ __global__ __launch_bounds__(256) void reduce_long_kernel(const uint32_t* __rest
 // writes; one 96-byte read of the BVH-order sums): gradients of value, weight
 // and mean, and the covariance chain (conic gradient -> Sigma -> M -> scales,
 // rotations) in double from the float inputs.  A zero conic gradient (no
-// records) gives zero scale and rotation gradients without the chain.
+// records) gives zero scale and rotation gradients without the chain; a
+// Gaussian without records (empty segment) has all-zero gradients and its sums
+// are not read.
 // fp64_ratio > 0 (FP64 mode): Gaussians on the precise path have their conic
 // gradient summed in the factor frame (covariance_chain repeats the test).
 __global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __restrict__ inv_perm,
-                                                    const GRecord* __restrict__ rec,
+                                                    const GRecord* __restrict__ rec, const uint2* __restrict__ seg,
                                                     const double* __restrict__ sums, const float* __restrict__ scales,
                                                     const float* __restrict__ rots, float kmod,
                                                     float* __restrict__ dmeans, float* __restrict__ dscales,
@@ -2002,13 +1989,19 @@ __global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __res
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= P) return;
     const uint32_t ri = inv_perm[id];
-    const double2* s2 = reinterpret_cast<const double2*>(sums + (size_t)kSumStride * ri);
+    const uint2 be = seg[ri];
     double m[12];
+    if (be.y > be.x) {
+        const double2* s2 = reinterpret_cast<const double2*>(sums + (size_t)kSumStride * ri);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
-        const double2 v = s2[c];
-        m[2 * c] = v.x;
-        m[2 * c + 1] = v.y;
+        for (int c = 0; c < 6; ++c) {
+            const double2 v = s2[c];
+            m[2 * c] = v.x;
+            m[2 * c + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 12; ++c) m[c] = 0.0;
     }
     dvalues[id] = (float)m[0];
     dweights[id] = (float)m[1];
@@ -2234,8 +2227,9 @@ struct ForwardLayout {
 
 struct BackwardLayout {
     PartialRecord* parts;
-    uint32_t *keys, *vals, *keys_alt, *vals_alt, *scratch, *begin, *end;
-    double* sums;  // kSumStride per Gaussian, BVH order
+    uint32_t *keys, *vals, *keys_alt, *vals_alt, *scratch;
+    uint2* seg;    // per Gaussian (BVH order): its records [x, y) in the sorted order
+    double* sums;  // kSumStride per Gaussian, BVH order (written only for Gaussians with records)
     uint32_t* long_list;  // [0] = count, [1..] = Gaussians with long segments
     BackwardLayout(Arena& a, int P, uint64_t cap)
     {
@@ -2246,8 +2240,7 @@ struct BackwardLayout {
         keys_alt = a.take<uint32_t>(c);
         vals_alt = a.take<uint32_t>(c);
         scratch = a.take<uint32_t>(radix_scratch_elems(c));
-        begin = a.take<uint32_t>((size_t)(P > 0 ? P : 1));
-        end = a.take<uint32_t>((size_t)(P > 0 ? P : 1));
+        seg = a.take<uint2>((size_t)(P > 0 ? P : 1));
         sums = a.take<double>((size_t)kSumStride * (size_t)(P > 0 ? P : 1));
         long_list = a.take<uint32_t>((size_t)P + 1);
     }
@@ -2589,8 +2582,7 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
         GVC_LAUNCH_CHECK("backward");
     }
     if (!(stages & GVC_STAGE_REDUCE)) return GVC_OK;
-    GVC_TRY(fill_u32(bl.begin, 0u, (size_t)P, st));
-    GVC_TRY(fill_u32(bl.end, 0u, (size_t)P, st));
+    GVC_TRY(fill_u32(reinterpret_cast<uint32_t*>(bl.seg), 0u, 2 * (size_t)P, st));
     const uint32_t* skeys = bl.keys;
     const uint32_t* svals = bl.vals;
     if (NG > 0 && capacity > 0) {
@@ -2602,16 +2594,16 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
         skeys = alt ? bl.keys_alt : bl.keys;
         svals = alt ? bl.vals_alt : bl.vals;
         segment_kernel<<<blocks_for(capacity, 256), 256, 0, st>>>(skeys, capacity, l.group_offsets + NG, (uint32_t)P,
-                                                                  bl.begin, bl.end);
+                                                                  reinterpret_cast<uint32_t*>(bl.seg));
         GVC_LAUNCH_CHECK("segment");
     }
     GVC_TRY(fill_u32(bl.long_list, 0u, 1, st));
-    reduce_sums_kernel<kReduceTeam><<<blocks_for(P, 256), 256, 0, st>>>(P, bl.begin, bl.end, svals, bl.parts,
+    reduce_sums_kernel<kReduceTeam><<<blocks_for(P, 256), 256, 0, st>>>(P, bl.seg, svals, bl.parts,
                                                                         bl.sums, bl.long_list);
     GVC_LAUNCH_CHECK("reduce_sums");
-    reduce_long_kernel<<<1024, 256, 0, st>>>(bl.begin, bl.end, svals, bl.parts, bl.long_list, bl.sums);
+    reduce_long_kernel<<<1024, 256, 0, st>>>(bl.seg, svals, bl.parts, bl.long_list, bl.sums);
     GVC_LAUNCH_CHECK("reduce_long");
-    chain_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, l.rec, bl.sums, scales, rots, k, dmeans,
+    chain_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, l.rec, bl.seg, bl.sums, scales, rots, k, dmeans,
                                                      dscales, drots, dvalues, dweights);
     GVC_LAUNCH_CHECK("chain");
     return GVC_OK;
