@@ -104,6 +104,9 @@ def parse():
                     help="backward walks the tree again instead of replaying the forward's candidate log")
     ap.add_argument("--no-batch32", action="store_true",
                     help="mesh: skip the 32-volume batch legs that the default line reports beside the headline")
+    ap.add_argument("--no-batch-autotune", action="store_true",
+                    help="mesh: run the 32-volume batch leg at the default launch parameters instead of the "
+                         "autotuner's choice (leaf size x points per walk x block size, swept on the batch)")
     ap.add_argument("--batch32-steps", type=int, default=10,
                     help="mesh: timed fused steps of the 32-volume batch leg (the plain sequence takes 2)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -402,8 +405,10 @@ def main():
         torch.cuda.empty_cache()
         b = batch32_legs(args, args.batch32_steps, 2, 3)
         out["batch32"] = {k: b[k] for k in ("ms_per_step", "pairs_per_step", "records_per_step", "evals_per_s",
-                                            "speedup_vs_plain", "stage_ms", "same_start_state")}
-        out["batch32"].update(workload=b["config"]["workload"], plain_ms_per_step=b["plain_sequence"]["ms_per_step"],
+                                            "speedup_vs_plain", "stage_ms", "same_start_state", "autotune")}
+        out["batch32"].update(workload=b["config"]["workload"], leaf_size=b["config"]["leaf_size"],
+                              block_threads=b["config"]["block_threads"], run_points=b["config"]["run_points"],
+                              plain_ms_per_step=b["plain_sequence"]["ms_per_step"],
                               plain_pairs_per_step=b["plain_sequence"]["pairs_per_step"], steps=b["steps"],
                               plain_steps=b["plain_sequence"]["steps"])
     print(json.dumps(out))
@@ -825,7 +830,7 @@ def batch32_legs(args, steps, plain_steps, warmup):
                                   block_threads=args.block or params.block_threads,
                                   candidate_log=not args.recompute)
     tune = None
-    if args.autotune and args.config == "batch32":
+    if args.autotune if args.config == "batch32" else not args.no_batch_autotune:
         from gvc.autotune import autotune_batched
         with torch.no_grad():
             wts, vls = W["weights"].reshape(-1, 1).contiguous(), W["values"].reshape(-1, 1).contiguous()
