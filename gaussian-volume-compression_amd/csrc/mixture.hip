@@ -405,10 +405,12 @@ __global__ void karras_kernel(const uint64_t* __restrict__ all_keys, BatchView b
 }
 
 // Node boxes.  A node of the radix tree covers the leaf range [first, last]; its
-// box is the union of the leaves' boxes, read from a sparse table
-// T_k[i] = box of leaves [i, i + 2^k): any range is the union of two (possibly
-// overlapping) table entries.  Unions use fminf/fmaxf, which skip NaN boxes; a
-// leaf whose Gaussians all have NaN boxes keeps a NaN box.
+// box is the union of the leaves' boxes, read from a segment tree over the
+// leaves: M = the power of two at or above NL, seg[M + j] = box of leaf j (NaN
+// boxes for j >= NL), seg[i] = seg[2i] u seg[2i + 1]; a range is the union of the
+// O(log) aligned nodes of its canonical decomposition.  Unions use fminf/fmaxf,
+// which skip NaN boxes (a leaf whose Gaussians all have NaN boxes keeps a NaN
+// box); min and max are exact, so any grouping of the unions gives the same box.
 struct Box6 {
     float lo[3], hi[3];
 };
@@ -423,58 +425,88 @@ __device__ __forceinline__ Box6 box_union(const Box6& a, const Box6& b)
     return r;
 }
 
-__global__ void leaf_box_kernel(const GRecord* __restrict__ rec, BatchView bv, int NL, int L, Box6* __restrict__ t0)
+// The NaN box is the identity of box_union (fminf / fmaxf skip NaN): it pads the
+// segment tree and starts the range unions, so a range whose leaves all have NaN
+// boxes keeps a NaN box.
+__device__ __forceinline__ Box6 nan_box()
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= NL) return;
     Box6 b;
-    for (int c = 0; c < 3; ++c) {
-        b.lo[c] = __builtin_inff();
-        b.hi[c] = -__builtin_inff();
-    }
-    bool any = false;
-    int g1;
-    const int g0 = leaf_first(bv, L, j, g1);
-    for (int g = g0; g < g1; ++g) {
-        const GRecord& r = rec[g];
-        b.lo[0] = fminf(b.lo[0], r.lox); b.lo[1] = fminf(b.lo[1], r.loy); b.lo[2] = fminf(b.lo[2], r.loz);
-        b.hi[0] = fmaxf(b.hi[0], r.hix); b.hi[1] = fmaxf(b.hi[1], r.hiy); b.hi[2] = fmaxf(b.hi[2], r.hiz);
-        any |= (r.lox == r.lox);
-    }
-    if (!any)
-        for (int c = 0; c < 3; ++c) b.lo[c] = b.hi[c] = __builtin_nanf("");
-    t0[j] = b;
+    for (int c = 0; c < 3; ++c) b.lo[c] = b.hi[c] = __builtin_nanf("");
+    return b;
 }
 
-// T_{k+1}[i] = T_k[i] u T_k[i + 2^k] for i + 2^{k+1} <= NL.
-// Up to three levels of the sparse table per launch: from level k (ranges of
-// h = 2^k leaves), level k+1+m at index i is the union of level k at i, i + h, ...,
-// i + (2^(m+1) - 1) h; each level is written where its range fits in [0, NL).
-__global__ void sparse_level_kernel(const Box6* __restrict__ tk, Box6* __restrict__ out, int NL, int half, int nlev)
+// Segment-tree levels built per block: a block of kSegBlock threads reduces
+// kSegBlock consecutive nodes of one level through kSegLevels levels in LDS.
+constexpr int kSegBlock = 256;
+constexpr int kSegLevels = 8;   // log2(kSegBlock)
+
+// One block: the nodes [base, base + kSegBlock) of a level (indices into seg[],
+// level nodes base .. base + count), their parents, grandparents, ... up to
+// kSegLevels levels (or the root) are formed in LDS and written.  With leaves
+// (rec != nullptr) the level's boxes are first formed from the leaves' records.
+__global__ __launch_bounds__(kSegBlock) void seg_level_kernel(const GRecord* __restrict__ rec, BatchView bv, int NL,
+                                                               int L, Box6* __restrict__ seg, uint32_t base,
+                                                               uint32_t count, int levels)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i + 2 * half > NL) return;
-    Box6 u = box_union(tk[i], tk[i + half]);
-    out[i] = u;
-    for (int m = 1; m < nlev; ++m) {
-        const int span = half << m;  // level k+m box covers [i, i + span)
-        if (i + 2 * span > NL) return;
-        // level k+m at i + span: the union of level k at i + span, ..., i + 2 span - h
-        Box6 v = tk[i + span];
-        for (int t = span + half; t < 2 * span; t += half) v = box_union(v, tk[i + t]);
-        u = box_union(u, v);
-        out[(size_t)m * NL + i] = u;
+    __shared__ Box6 sb[kSegBlock];
+    const int t = threadIdx.x;
+    const uint32_t i = base + blockIdx.x * kSegBlock + t;   // node index at this level
+    Box6 b = nan_box();
+    if (i < base + count) {
+        if (rec) {
+            const uint32_t j = i - base;   // leaf j (base = M)
+            if ((int)j < NL) {
+                bool any = false;
+                int g1;
+                const int g0 = leaf_first(bv, L, (int)j, g1);
+                for (int g = g0; g < g1; ++g) {
+                    const GRecord& r = rec[g];
+                    b.lo[0] = fminf(b.lo[0], r.lox); b.lo[1] = fminf(b.lo[1], r.loy); b.lo[2] = fminf(b.lo[2], r.loz);
+                    b.hi[0] = fmaxf(b.hi[0], r.hix); b.hi[1] = fmaxf(b.hi[1], r.hiy); b.hi[2] = fmaxf(b.hi[2], r.hiz);
+                    any |= (r.lox == r.lox);
+                }
+                if (!any)
+                    for (int c = 0; c < 3; ++c) b.lo[c] = b.hi[c] = __builtin_nanf("");
+            }
+            seg[i] = b;
+        } else {
+            b = seg[i];
+        }
+    }
+    sb[t] = b;
+    uint32_t lvl_base = base;
+    for (int k = 1; k <= levels; ++k) {
+        __syncthreads();
+        const int n = kSegBlock >> k;   // nodes of this block at level k
+        Box6 u;
+        if (t < n) u = box_union(sb[2 * t], sb[2 * t + 1]);
+        __syncthreads();
+        lvl_base >>= 1;
+        if (t < n) {
+            sb[t] = u;
+            // the level holds count >> k nodes (fewer than a block's share when
+            // the whole level fits in one block)
+            const uint32_t q = blockIdx.x * (uint32_t)n + (uint32_t)t;
+            if (q < (count >> k)) seg[lvl_base + q] = u;
+        }
     }
 }
 
-__device__ __forceinline__ Box6 range_box(const Box6* __restrict__ table, int NL, int a, int b)
+// Union of leaves [a, b] (inclusive) by the canonical decomposition.
+__device__ __forceinline__ Box6 range_box(const Box6* __restrict__ seg, uint32_t M, int a, int b)
 {
-    const int k = 31 - __builtin_clz((unsigned)(b - a + 1));
-    const Box6* tk = table + (size_t)k * NL;
-    return box_union(tk[a], tk[b - (1 << k) + 1]);
+    uint32_t l = M + (uint32_t)a, r = M + (uint32_t)b + 1u;
+    Box6 x = nan_box(), y = nan_box();   // left part, right part
+    while (l < r) {
+        if (l & 1u) x = box_union(x, seg[l++]);
+        if (r & 1u) y = box_union(seg[--r], y);
+        l >>= 1;
+        r >>= 1;
+    }
+    return box_union(x, y);
 }
 
-__global__ void node_box_kernel(const Box6* __restrict__ table, BatchView bv, int NL, int NI,
+__global__ void node_box_kernel(const Box6* __restrict__ seg, uint32_t M, BatchView bv, int NI,
                                 BNode* __restrict__ nodes, float* __restrict__ root_box)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -484,8 +516,8 @@ __global__ void node_box_kernel(const Box6* __restrict__ table, BatchView bv, in
     const int lo = n.pad0, hi = n.pad1, left = n.left;
     // the left child covers [lo, gamma]; an internal left child has local index gamma
     const int gamma = left < 0 ? ~left : bv.L_off[v] + (left - bv.N_off[v]);
-    const Box6 l = range_box(table, NL, lo, gamma);
-    const Box6 r = range_box(table, NL, gamma + 1, hi);
+    const Box6 l = range_box(seg, M, lo, gamma);
+    const Box6 r = range_box(seg, M, gamma + 1, hi);
     for (int c = 0; c < 3; ++c) {
         n.llo[c] = l.lo[c];
         n.lhi[c] = l.hi[c];
@@ -503,15 +535,15 @@ __global__ void node_box_kernel(const Box6* __restrict__ table, BatchView bv, in
 
 // Root boxes of volumes with a single leaf (no internal node); empty volumes get
 // an empty box.
-__global__ void leaf_root_kernel(const Box6* __restrict__ t0, BatchView bv, float* __restrict__ root_box)
+__global__ void leaf_root_kernel(const Box6* __restrict__ leaves, BatchView bv, float* __restrict__ root_box)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= bv.V) return;
     const int nl = bv.L_off[v + 1] - bv.L_off[v];
     if (nl > 1) return;
     for (int c = 0; c < 3; ++c) {
-        root_box[6 * v + c] = nl == 1 ? t0[bv.L_off[v]].lo[c] : 1.0f;
-        root_box[6 * v + 3 + c] = nl == 1 ? t0[bv.L_off[v]].hi[c] : 0.0f;
+        root_box[6 * v + c] = nl == 1 ? leaves[bv.L_off[v]].lo[c] : 1.0f;
+        root_box[6 * v + 3 + c] = nl == 1 ? leaves[bv.L_off[v]].hi[c] : 0.0f;
     }
 }
 
@@ -2076,12 +2108,12 @@ inline unsigned reduce_blocks(int NG)
 
 inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-// levels of the leaf-range sparse table: 1 + floor(log2(NL))
-inline int sparse_levels(int NL)
+// Leaves of the segment tree over the leaf boxes: the power of two at or above NL.
+inline uint32_t seg_leaves(int NL)
 {
-    int k = 1;
-    while (NL > 0 && (2ll << (k - 1)) <= NL) ++k;
-    return k;
+    uint32_t m = 1;
+    while ((int64_t)m < (int64_t)NL) m <<= 1;
+    return m;
 }
 
 // Grid of a persistent walk: as many blocks as fit on the device at once (a
@@ -2175,7 +2207,7 @@ struct ForwardLayout {
     uint32_t *idx, *idx_alt, *vkeys, *pos, *sort_scratch;
     uint32_t* inv_perm;  // caller's Gaussian index -> record (BVH) index
     BNode* nodes;
-    Box6* sparse;  // levels x NL leaf-range boxes
+    Box6* seg;  // segment tree over the leaf boxes, 2 M entries (leaf j at M + j)
     float* root_box;  // 6 per volume
     uint32_t *group_records, *group_offsets, *group_pairs, *scan_scratch, *err;
     uint32_t* queues;  // walk counters, kQWords words (kQ* offsets)
@@ -2204,7 +2236,7 @@ struct ForwardLayout {
         sort_scratch = a.take<uint32_t>(radix_scratch_elems((size_t)P));
         leaf_codes = a.take<uint64_t>((size_t)NL);
         nodes = a.take<BNode>((size_t)(b.NI > 0 ? b.NI : 1));
-        sparse = a.take<Box6>((size_t)sparse_levels(NL) * (size_t)(NL > 0 ? NL : 1));
+        seg = a.take<Box6>(2 * (size_t)seg_leaves(NL));
         root_box = a.take<float>(6 * (size_t)b.V + 2);
         group_records = a.take<uint32_t>((size_t)NG + 1);
         group_offsets = a.take<uint32_t>((size_t)NG + 1);
@@ -2414,23 +2446,30 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
             karras_kernel<<<blocks_for(bt.NI, 256), 256, 0, st>>>(l.leaf_codes, bv, bt.NI, l.nodes);
             GVC_LAUNCH_CHECK("karras");
         }
-        // 4. boxes: leaf boxes, sparse table of leaf ranges, node boxes
-        leaf_box_kernel<<<blocks_for(NL, 256), 256, 0, st>>>(l.rec, bv, NL, L, l.sparse);
-        GVC_LAUNCH_CHECK("leaf_box");
-        if (bt.NI > 0) {
-            const int levels = sparse_levels(NL);
-            for (int kk = 0; kk + 1 < levels; kk += 3) {
-                const int half = 1 << kk;
-                const int n_out = NL - 2 * half + 1;
-                const int nlev = (levels - 1 - kk) < 3 ? (levels - 1 - kk) : 3;
-                sparse_level_kernel<<<blocks_for((size_t)n_out, 256), 256, 0, st>>>(
-                    l.sparse + (size_t)kk * NL, l.sparse + (size_t)(kk + 1) * NL, NL, half, nlev);
-                GVC_LAUNCH_CHECK("sparse_level");
+        // 4. boxes: segment tree over the leaf boxes (kSegLevels levels per
+        // launch), then both child boxes of every node by range queries
+        const uint32_t M = seg_leaves(NL);
+        if (NL > 0) {
+            uint32_t lvl = M;   // first node of the level being reduced
+            int total = 0;
+            while ((1u << total) < M) ++total;   // levels above the leaves
+            bool first = true;
+            while (true) {
+                const int lv = total < kSegLevels ? total : kSegLevels;
+                seg_level_kernel<<<blocks_for(lvl, kSegBlock), kSegBlock, 0, st>>>(first ? l.rec : nullptr, bv, NL, L,
+                                                                                  l.seg, lvl, lvl, lv);
+                GVC_LAUNCH_CHECK("seg_level");
+                first = false;
+                total -= lv;
+                lvl >>= lv;
+                if (total <= 0) break;
             }
-            node_box_kernel<<<blocks_for(bt.NI, 256), 256, 0, st>>>(l.sparse, bv, NL, bt.NI, l.nodes, l.root_box);
+        }
+        if (bt.NI > 0) {
+            node_box_kernel<<<blocks_for(bt.NI, 256), 256, 0, st>>>(l.seg, M, bv, bt.NI, l.nodes, l.root_box);
             GVC_LAUNCH_CHECK("node_box");
         }
-        leaf_root_kernel<<<blocks_for(bt.V, 64), 64, 0, st>>>(l.sparse, bv, l.root_box);
+        leaf_root_kernel<<<blocks_for(bt.V, 64), 64, 0, st>>>(l.seg + M, bv, l.root_box);
         GVC_LAUNCH_CHECK("root_box");
         // 4b. heavy groups, walked first
         GVC_TRY(fill_u32(l.heavy, 0u, 1, st));
