@@ -64,13 +64,16 @@ const float4* sample_set_points(const void* sample_set, int S);
 
 // ---- radix_sort.hip --------------------------------------------------------
 // Stable LSD radix sort of (key, value) pairs on the low key_bits bits.
-// Scratch elements (uint32) needed for n items.  n_dev (optional, device memory):
-// sort only the first min(n, *n_dev) items.  vals_identity: vals is not read;
-// item i's value is i.
-size_t radix_scratch_elems(size_t n);
+// Scratch elements (uint32) needed for n items in `segments` segments.  n_dev
+// (optional, device memory): sort only the first min(n, *n_dev) items.
+// vals_identity: vals is not read; item i's value is i.  seg_off (optional,
+// device memory, segments + 1 item offsets): the items form contiguous segments,
+// each sorted on its own (the order of the segments is kept).
+size_t radix_scratch_elems(size_t n, int segments = 1);
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, size_t n,
                    int key_bits, uint32_t* scratch, bool* result_in_alt, hipStream_t st,
-                   const uint32_t* n_dev = nullptr, bool vals_identity = false);
+                   const uint32_t* n_dev = nullptr, bool vals_identity = false, const int* seg_off = nullptr,
+                   int segments = 1);
 int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, size_t n,
                    int key_bits, uint32_t* scratch, bool* result_in_alt, hipStream_t st);
 

@@ -217,22 +217,6 @@ __global__ void sample_gather_kernel(const float* __restrict__ s, BatchView b, i
 }
 
 // Volume of each sorted item (the second, stable sort key of a batch).
-__global__ void volume_keys_kernel(const uint32_t* __restrict__ idx, int n, const int* __restrict__ off, int V,
-                                   uint32_t* __restrict__ vkeys, uint32_t* __restrict__ pos)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    vkeys[i] = (uint32_t)volume_of(off, V, (int)idx[i]);
-    pos[i] = (uint32_t)i;
-}
-
-__global__ void gather_u32_kernel(const uint32_t* __restrict__ src, const uint32_t* __restrict__ pos, int n,
-                                  uint32_t* __restrict__ dst)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[pos[i]];
-}
-
 __global__ void sample_header_kernel(SampleSetHeader* h, int S, int NG)
 {
     h->S = S;
@@ -2182,8 +2166,9 @@ struct SampleSetLayout {
     SampleSetHeader* header;
     int* table;  // the set's own one-volume table
     float4* pts;
-    uint32_t *keys, *idx, *keys_alt, *idx_alt, *vkeys, *pos, *scratch;
-    SampleSetLayout(Arena& a, int S, int NG)
+    uint32_t *keys, *idx, *keys_alt, *idx_alt, *scratch;
+    // V: the volumes of a batch (the sort scratch, last, is sized for them)
+    SampleSetLayout(Arena& a, int S, int NG, int V = 1)
     {
         header = a.take<SampleSetHeader>(1);
         table = a.take<int>(batch_table_ints(1));
@@ -2192,9 +2177,7 @@ struct SampleSetLayout {
         idx = a.take<uint32_t>((size_t)S);
         keys_alt = a.take<uint32_t>((size_t)S);
         idx_alt = a.take<uint32_t>((size_t)S);
-        vkeys = a.take<uint32_t>((size_t)S);
-        pos = a.take<uint32_t>((size_t)S);
-        scratch = a.take<uint32_t>(radix_scratch_elems((size_t)S));
+        scratch = a.take<uint32_t>(radix_scratch_elems((size_t)S, V));
     }
 };
 
@@ -2204,7 +2187,7 @@ struct ForwardLayout {
     float4* fac_lo;  // FP64 mode: lo parts of the precise-path factors, 2 per Gaussian (caller order)
     uint32_t *codes, *codes_alt;
     uint64_t* leaf_codes;
-    uint32_t *idx, *idx_alt, *vkeys, *pos, *sort_scratch;
+    uint32_t *idx, *idx_alt, *sort_scratch;
     uint32_t* inv_perm;  // caller's Gaussian index -> record (BVH) index
     BNode* nodes;
     Box6* seg;  // segment tree over the leaf boxes, 2 M entries (leaf j at M + j)
@@ -2230,10 +2213,8 @@ struct ForwardLayout {
         codes_alt = a.take<uint32_t>((size_t)P);
         idx = a.take<uint32_t>((size_t)P);
         idx_alt = a.take<uint32_t>((size_t)P);
-        vkeys = a.take<uint32_t>((size_t)P);
-        pos = a.take<uint32_t>((size_t)P);
         inv_perm = a.take<uint32_t>((size_t)P);
-        sort_scratch = a.take<uint32_t>(radix_scratch_elems((size_t)P));
+        sort_scratch = a.take<uint32_t>(radix_scratch_elems((size_t)P, b.V));
         leaf_codes = a.take<uint64_t>((size_t)NL);
         nodes = a.take<BNode>((size_t)(b.NI > 0 ? b.NI : 1));
         seg = a.take<Box6>(2 * (size_t)seg_leaves(NL));
@@ -2309,10 +2290,10 @@ float factor_ratio_of(const gvc_params* prm)
 // The table a call uses: the batch's, or for one volume the buffer's own.
 const void* table_of(const Batch& b, const int* own) { return b.V == 1 && !b.table ? own : b.table; }
 
-size_t sample_set_bytes_impl(int S, int NG)
+size_t sample_set_bytes_impl(int S, int NG, int V = 1)
 {
     Arena a(nullptr, 0);
-    SampleSetLayout l(a, S, NG);
+    SampleSetLayout l(a, S, NG, V);
     return a.off + 256;
 }
 
@@ -2320,7 +2301,7 @@ int sample_set_build_impl(const Batch& bt, const float* samples, void* set, size
 {
     const int S = bt.S, NG = bt.NG;
     Arena a(set, bytes);
-    SampleSetLayout l(a, S, NG);
+    SampleSetLayout l(a, S, NG, bt.V);
     if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "sample set buffer too small");
     if (!bt.table) {
         single_table_kernel<<<1, 64, 0, st>>>(l.table, 0, S, 1);
@@ -2330,21 +2311,12 @@ int sample_set_build_impl(const Batch& bt, const float* samples, void* set, size
     if (S > 0) {
         sample_codes_kernel<<<blocks_for(S, 256), 256, 0, st>>>(samples, S, l.keys, l.idx);
         GVC_LAUNCH_CHECK("sample_codes");
+        // Morton order within each volume: a batch's samples are volume-major, and
+        // each volume's range is sorted on its own (segmented sort)
         bool alt = false;
-        GVC_TRY(radix_sort_u32(l.keys, l.idx, l.keys_alt, l.idx_alt, (size_t)S, 30, l.scratch, &alt, st));
+        GVC_TRY(radix_sort_u32(l.keys, l.idx, l.keys_alt, l.idx_alt, (size_t)S, 30, l.scratch, &alt, st, nullptr,
+                               false, bt.V > 1 ? bv.S_off : nullptr, bt.V));
         const uint32_t* perm = alt ? l.idx_alt : l.idx;
-        if (bt.V > 1) {
-            // stable second key: the volume, so each volume's samples are contiguous
-            uint32_t* spare = alt ? l.idx : l.idx_alt;
-            volume_keys_kernel<<<blocks_for(S, 256), 256, 0, st>>>(perm, S, bv.S_off, bt.V, l.vkeys, l.pos);
-            GVC_LAUNCH_CHECK("volume_keys");
-            bool alt2 = false;
-            GVC_TRY(radix_sort_u32(l.vkeys, l.pos, l.keys, l.keys_alt, (size_t)S, key_bits_for((uint64_t)bt.V),
-                                   l.scratch, &alt2, st));
-            gather_u32_kernel<<<blocks_for(S, 256), 256, 0, st>>>(perm, alt2 ? l.keys_alt : l.pos, S, spare);
-            GVC_LAUNCH_CHECK("gather");
-            perm = spare;
-        }
         sample_gather_kernel<<<blocks_for((size_t)NG * 64, 256), 256, 0, st>>>(samples, bv, NG * 64, perm, l.pts);
         GVC_LAUNCH_CHECK("sample_gather");
     }
@@ -2410,28 +2382,13 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
         // 1. Morton order of the Gaussians (within each volume)
         gaussian_codes_kernel<<<blocks_for(P, 256), 256, 0, st>>>(means, P, l.codes);
         GVC_LAUNCH_CHECK("gaussian_codes");
+        // (a batch's Gaussians are volume-major: each volume's range is sorted on
+        // its own)
         bool alt = false;
         GVC_TRY(radix_sort_u32(l.codes, l.idx, l.codes_alt, l.idx_alt, (size_t)P, kGaussianCodeBits, l.sort_scratch,
-                               &alt, st, nullptr, true));
+                               &alt, st, nullptr, true, bt.V > 1 ? bv.P_off : nullptr, bt.V));
         const uint32_t* scodes = alt ? l.codes_alt : l.codes;
         const uint32_t* perm = alt ? l.idx_alt : l.idx;
-        if (bt.V > 1) {
-            // stable second key: the volume (codes2 and perm2 are free here)
-            uint32_t* codes2 = alt ? l.codes : l.codes_alt;
-            uint32_t* perm2 = alt ? l.idx : l.idx_alt;
-            volume_keys_kernel<<<blocks_for(P, 256), 256, 0, st>>>(perm, P, bv.P_off, bt.V, l.vkeys, l.pos);
-            GVC_LAUNCH_CHECK("volume_keys");
-            bool alt2 = false;
-            uint32_t* spare = codes2;
-            GVC_TRY(radix_sort_u32(l.vkeys, l.pos, perm2, spare, (size_t)P, key_bits_for((uint64_t)bt.V),
-                                   l.sort_scratch, &alt2, st));
-            if (alt2) GVC_HIP(hipMemcpyAsync(l.pos, spare, (size_t)P * 4, hipMemcpyDeviceToDevice, st));
-            gather_u32_kernel<<<blocks_for(P, 256), 256, 0, st>>>(perm, l.pos, P, perm2);
-            gather_u32_kernel<<<blocks_for(P, 256), 256, 0, st>>>(scodes, l.pos, P, codes2);
-            GVC_LAUNCH_CHECK("gather");
-            scodes = codes2;
-            perm = perm2;
-        }
         // 2. records
         inv_perm_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, perm, l.inv_perm);
         record_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, means, scales, rots, values, weights, k,
@@ -2825,7 +2782,10 @@ static Batch from_c(const gvc_batch* b)
     return r;
 }
 
-size_t sample_set_bytes_batched(const gvc_batch* b) { return sample_set_bytes_impl(b->S_total, b->groups); }
+size_t sample_set_bytes_batched(const gvc_batch* b)
+{
+    return sample_set_bytes_impl(b->S_total, b->groups, b->volumes);
+}
 
 int sample_set_build_batched(const gvc_batch* b, const float* samples, void* set, size_t bytes, hipStream_t st)
 {

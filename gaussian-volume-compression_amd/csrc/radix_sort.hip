@@ -12,7 +12,15 @@
 // device-side count (n_dev) sorts only the first min(n, *n_dev) items: the grid is
 // sized for n, tiles past the count stay idle and contribute empty histograms.
 // Optional identity values: the first pass takes item i's value to be i (the
-// caller's value array is then only written).  Replaces the sorts of
+// caller's value array is then only written).
+// Segmented mode (seg_off: nseg + 1 item offsets on the device): the items are
+// already grouped into contiguous segments (the volumes of a batch) and every
+// segment is sorted on its own, in place of a further pass on a segment key.
+// Tiles never span segments (a segment of n_s items takes ceil(n_s / 8192)
+// tiles); the histogram of each pass is laid out segment-major -- column
+// (d, t) = (digits) * tau_s + d * T_s + (t - tau_s) for tile t of segment s, whose
+// T_s tiles start at tile tau_s -- so one exclusive scan gives every tile its
+// output offsets, segments included.  Replaces the sorts of
 // R/gaussian_renderer/__init__.py:77 and R/train.py:311 (torch.argsort) and the
 // CUB usage of CR/rasterizer_impl.cu; nothing is allocated inside.
 #include "gvc_internal.h"
@@ -35,29 +43,91 @@ __device__ __forceinline__ size_t count_of(size_t n, const uint32_t* n_dev)
     return m < n ? m : n;
 }
 
+// A tile: items [begin, end), histogram column of digit d = col + d * stride.
+struct TileSpan {
+    size_t begin, end;
+    size_t col, stride;
+};
+
+// tiles (segmented mode): per tile {begin, end, tau_s, T_s}; empty tiles past the
+// last segment's have begin == end and tau_s = the tile, T_s = 1.
+__device__ __forceinline__ TileSpan tile_span(const uint4* __restrict__ tiles, size_t n, uint32_t n_tiles,
+                                              uint32_t digits)
+{
+    TileSpan sp;
+    if (tiles) {
+        const uint4 q = tiles[blockIdx.x];
+        sp.begin = q.x;
+        sp.end = q.y;
+        sp.col = (size_t)digits * q.z + (blockIdx.x - q.z);
+        sp.stride = q.w;
+    } else {
+        sp.begin = (size_t)blockIdx.x * kTile;
+        sp.end = sp.begin + kTile < n ? sp.begin + kTile : n;
+        sp.col = blockIdx.x;
+        sp.stride = n_tiles;
+    }
+    return sp;
+}
+
+// First tile of every segment (segmented mode): tau[s] = sum of ceil(n_s' / kTile)
+// over the segments before s, tau[nseg] = all tiles.  One thread (a batch has few
+// segments).
+__global__ void seg_tiles_kernel(const int* __restrict__ seg_off, int nseg, uint32_t* __restrict__ tau)
+{
+    uint32_t t = 0;
+    for (int s = 0; s < nseg; ++s) {
+        tau[s] = t;
+        const uint32_t ns = (uint32_t)(seg_off[s + 1] - seg_off[s]);
+        t += (ns + kTile - 1) / kTile;
+    }
+    tau[nseg] = t;
+}
+
+__global__ void tile_table_kernel(const int* __restrict__ seg_off, int nseg, const uint32_t* __restrict__ tau,
+                                  uint32_t max_tiles, uint4* __restrict__ tiles)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= max_tiles) return;
+    if (t >= tau[nseg]) {
+        tiles[t] = make_uint4(0u, 0u, t, 1u);
+        return;
+    }
+    int lo = 0, hi = nseg - 1;   // the segment s with tau[s] <= t < tau[s + 1]
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (tau[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const uint32_t b = (uint32_t)seg_off[lo] + (t - tau[lo]) * (uint32_t)kTile;
+    const uint32_t e = b + (uint32_t)kTile < (uint32_t)seg_off[lo + 1] ? b + (uint32_t)kTile : (uint32_t)seg_off[lo + 1];
+    tiles[t] = make_uint4(b, e, tau[lo], tau[lo + 1] - tau[lo]);
+}
+
 template <class K>
 __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* __restrict__ keys, size_t n_max,
                                                               const uint32_t* __restrict__ n_dev, int shift,
                                                               uint32_t mask, uint32_t* __restrict__ hist,
-                                                              uint32_t n_tiles)
+                                                              uint32_t n_tiles, const uint4* __restrict__ tiles)
 {
     const size_t n = count_of(n_max, n_dev);
+    const TileSpan sp = tile_span(tiles, n, n_tiles, mask + 1u);
     __shared__ uint32_t h[kWaves][kMaxBins];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     for (int i = t; i < kWaves * kMaxBins; i += kThreads) (&h[0][0])[i] = 0u;
     __syncthreads();
-    const size_t base = (size_t)blockIdx.x * kTile + (size_t)w * kWaveItems + lane;
+    const size_t base = sp.begin + (size_t)w * kWaveItems + lane;
 #pragma unroll 4
     for (int r = 0; r < kItems; ++r) {
         const size_t i = base + (size_t)r * 64;
-        if (i < n) atomicAdd(&h[w][(uint32_t)(keys[i] >> shift) & mask], 1u);
+        if (i < sp.end) atomicAdd(&h[w][(uint32_t)(keys[i] >> shift) & mask], 1u);
     }
     __syncthreads();
     for (uint32_t d = t; d <= mask; d += kThreads) {
         uint32_t s = 0;
 #pragma unroll
         for (int v = 0; v < kWaves; ++v) s += h[v][d];
-        hist[(size_t)d * n_tiles + blockIdx.x] = s;
+        hist[(size_t)d * sp.stride + sp.col] = s;
     }
 }
 
@@ -72,10 +142,10 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __rest
                                                                  uint32_t* __restrict__ vals_out, size_t n_max,
                                                                  const uint32_t* __restrict__ n_dev, int shift,
                                                                  uint32_t mask, const uint32_t* __restrict__ offsets,
-                                                                 uint32_t n_tiles)
+                                                                 uint32_t n_tiles, const uint4* __restrict__ tiles)
 {
-    const size_t n = count_of(n_max, n_dev);
-    const size_t base = (size_t)blockIdx.x * kTile;
+    const TileSpan sp = tile_span(tiles, count_of(n_max, n_dev), n_tiles, mask + 1u);
+    const size_t base = sp.begin, n = sp.end;   // this tile: items [base, n)
     if (base >= n) return;
     extern __shared__ __align__(16) unsigned char dyn[];
     K* skeys = reinterpret_cast<K*>(dyn);
@@ -148,7 +218,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __rest
             for (int v = 0; v < w; ++v) b += wsum[v];
             const uint32_t s0 = b + inc - tot;
             start[d] = s0;
-            gofs[d] = (d <= mask) ? offsets[(size_t)d * n_tiles + blockIdx.x] - s0 : 0u;
+            gofs[d] = (d <= mask) ? offsets[(size_t)d * sp.stride + sp.col] - s0 : 0u;
         }
     }
     __syncthreads();
@@ -184,9 +254,16 @@ int allow_scatter_lds()
     return GVC_OK;
 }
 
+// Tiles of a sort of n items in nseg segments (nseg <= 1: not segmented).
+inline size_t max_tiles(size_t n, int nseg)
+{
+    return (n + kTile - 1) / kTile + (nseg > 1 ? (size_t)nseg : 0);
+}
+
 template <class K>
 int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, size_t n, const uint32_t* n_dev,
-                    int key_bits, uint32_t* scratch, bool* result_in_alt, hipStream_t st, bool vals_identity)
+                    int key_bits, uint32_t* scratch, bool* result_in_alt, hipStream_t st, bool vals_identity,
+                    const int* seg_off = nullptr, int nseg = 1)
 {
     *result_in_alt = false;
     if (n <= 1 || key_bits <= 0) {
@@ -199,9 +276,20 @@ int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, si
     if (n > 0xffffffffull) return set_error(GVC_ERR_INVALID_ARGUMENT, "radix sort: more than 2^32 items");
     if (key_bits > (int)(8 * sizeof(K))) key_bits = (int)(8 * sizeof(K));
     GVC_TRY(allow_scatter_lds<K>());
-    const uint32_t n_tiles = (uint32_t)((n + kTile - 1) / kTile);
+    const bool segmented = seg_off && nseg > 1;
+    if (segmented && n_dev) return set_error(GVC_ERR_INVALID_ARGUMENT, "radix sort: segments with a device count");
+    const uint32_t n_tiles = (uint32_t)max_tiles(n, segmented ? nseg : 1);
     uint32_t* hist = scratch;
     uint32_t* scan_scratch = scratch + (size_t)kMaxBins * n_tiles;
+    uint4* tiles = nullptr;
+    if (segmented) {
+        uint32_t* tau = scan_scratch + ((scan_scratch_elems((size_t)kMaxBins * n_tiles) + 3) & ~(size_t)3);
+        tiles = reinterpret_cast<uint4*>(tau + (((size_t)nseg + 1 + 3) & ~(size_t)3));
+        seg_tiles_kernel<<<1, 1, 0, st>>>(seg_off, nseg, tau);
+        GVC_LAUNCH_CHECK("seg_tiles");
+        tile_table_kernel<<<(n_tiles + 255) / 256, 256, 0, st>>>(seg_off, nseg, tau, n_tiles, tiles);
+        GVC_LAUNCH_CHECK("tile_table");
+    }
     const int passes = (key_bits + 7) / 8;
     const int width = (key_bits + passes - 1) / passes;
     K* src_k = keys;
@@ -213,11 +301,11 @@ int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, si
     for (int shift = 0; shift < key_bits; shift += width) {
         const int bits = (key_bits - shift) < width ? (key_bits - shift) : width;
         const uint32_t mask = (1u << bits) - 1u;
-        radix_hist_kernel<K><<<n_tiles, kThreads, 0, st>>>(src_k, n, n_dev, shift, mask, hist, n_tiles);
+        radix_hist_kernel<K><<<n_tiles, kThreads, 0, st>>>(src_k, n, n_dev, shift, mask, hist, n_tiles, tiles);
         GVC_LAUNCH_CHECK("radix_hist");
         GVC_TRY(exclusive_scan_u32(hist, hist, (size_t)(mask + 1) * n_tiles, false, scan_scratch, st));
         radix_scatter_kernel<K><<<n_tiles, kThreads, scatter_lds<K>(), st>>>(src_k, src_v, dst_k, dst_v, n, n_dev,
-                                                                            shift, mask, hist, n_tiles);
+                                                                            shift, mask, hist, n_tiles, tiles);
         GVC_LAUNCH_CHECK("radix_scatter");
         K* tk = src_k; src_k = dst_k; dst_k = tk;
         uint32_t* nv = other_v;
@@ -232,17 +320,23 @@ int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, si
 
 }  // namespace
 
-size_t radix_scratch_elems(size_t n)
+size_t radix_scratch_elems(size_t n, int segments)
 {
-    const size_t n_tiles = (n + kTile - 1) / kTile;
-    return (size_t)kMaxBins * n_tiles + scan_scratch_elems((size_t)kMaxBins * n_tiles) + 1;
+    const size_t n_tiles = max_tiles(n, segments);
+    const size_t scan = scan_scratch_elems((size_t)kMaxBins * n_tiles);
+    size_t e = (size_t)kMaxBins * n_tiles + scan + 1;
+    if (segments > 1)   // 16-byte aligned tau and tile table after the scan scratch
+        e = (size_t)kMaxBins * n_tiles + ((scan + 3) & ~(size_t)3) + (((size_t)segments + 1 + 3) & ~(size_t)3) +
+            4 * n_tiles + 4;
+    return e;
 }
 
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, size_t n, int key_bits,
-                   uint32_t* scratch, bool* result_in_alt, hipStream_t st, const uint32_t* n_dev, bool vals_identity)
+                   uint32_t* scratch, bool* result_in_alt, hipStream_t st, const uint32_t* n_dev, bool vals_identity,
+                   const int* seg_off, int segments)
 {
     return radix_sort_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, n_dev, key_bits, scratch, result_in_alt,
-                                     st, vals_identity);
+                                     st, vals_identity, seg_off, segments);
 }
 
 int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, size_t n, int key_bits,
