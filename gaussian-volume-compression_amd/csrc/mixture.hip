@@ -1,15 +1,15 @@
 // mixture.hip -- the differentiable Gaussian-mixture evaluator on gfx950.
 //
 // Pipeline (one stream, no allocation, no host synchronisation):
-//   sample set   : 30-bit Morton keys of the samples -> stable radix sort -> float4
+//   sample set   : Morton keys of the samples (top 24 of 30 bits) -> stable radix sort -> float4
 //                  points (x, y, z, original index) in Morton order; every 64
 //                  consecutive points form one group = one wavefront.
 //   forward      : top 32 bits of the means' 63-bit Morton codes (the reference's
 //                  morton3d_unit) -> stable radix sort -> canonical preprocess in
 //                  the caller's order written as 80-byte records at their BVH
 //                  positions -> Karras radix tree over leaves of
-//                  `leaf_size` consecutive records -> node boxes from a sparse
-//                  table of leaf-range boxes -> persistent wavefronts take groups
+//                  `leaf_size` consecutive records -> node boxes from a segment
+//                  tree over the leaf boxes -> persistent wavefronts take groups
 //                  from per-XCD queues; a group is cut into runs of consecutive
 //                  lanes where it straddles a coarse Morton boundary, and each run
 //                  walks the tree with its box (wide wave-uniform traversal,
@@ -175,6 +175,13 @@ constexpr int kSampleMagic = 0x47564353;
 
 // 30-bit Morton key of a sample: coordinates clamped to the unit cube (NaN -> 0),
 // 10 bits per axis.
+// Bits of sample_code dropped from the sample set's sort key: the points are
+// ordered on the top 24 bits, which leaves a 256^3 lattice of cells -- finer than
+// the groups of 64 points of any set this build sorts -- in three 8-bit radix
+// passes instead of four.
+constexpr int kSampleKeyDrop = 6;
+constexpr int kSampleKeyBits = 30 - kSampleKeyDrop;
+
 __device__ __forceinline__ uint32_t sample_code(float x, float y, float z)
 {
     const float v[3] = { x, y, z };
@@ -193,7 +200,9 @@ __global__ void sample_codes_kernel(const float* __restrict__ s, int S, uint32_t
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= S) return;
-    keys[i] = sample_code(s[3 * i], s[3 * i + 1], s[3 * i + 2]);
+    // the sort key: the top 24 bits (8 per axis, 256^3 cells; ties keep the
+    // caller's order)
+    keys[i] = sample_code(s[3 * i], s[3 * i + 1], s[3 * i + 2]) >> kSampleKeyDrop;
     idx[i] = (uint32_t)i;
 }
 
@@ -1916,8 +1925,8 @@ __device__ __forceinline__ void store_sums(double* __restrict__ sums, int gi, co
 // lanes (team_sums); a wave covers 64 consecutive Gaussians in TEAM passes (lane
 // L first loads Gaussian L's segment bounds; a pass whose teams have no records
 // is skipped), hands lane L the sums of Gaussian L and writes them to sums[] in
-// BVH order.  Longer segments are listed for reduce_long_kernel.  Gaussians
-// without records get no sums (the chain reads their empty segment instead).
+// BVH order (zeros for a Gaussian without records).  Longer segments are listed
+// for reduce_long_kernel.
 template <int TEAM>
 __global__ __launch_bounds__(256) void reduce_sums_kernel(int P, const uint2* __restrict__ seg,
                                                           const uint32_t* __restrict__ sorted_vals,
@@ -1934,7 +1943,6 @@ __global__ __launch_bounds__(256) void reduce_sums_kernel(int P, const uint2* __
     const bool valid = gi < P;
     const uint2 be = valid ? seg[gi] : make_uint2(0u, 0u);
     uint32_t my_b = be.x, my_e = be.y;
-    const bool empty = my_e == my_b;
     const bool is_long = my_e - my_b > kLongSegment;
     const uint64_t longs = __ballot(is_long);
     if (longs != 0ull) {
@@ -1961,7 +1969,7 @@ __global__ __launch_bounds__(256) void reduce_sums_kernel(int P, const uint2* __
             if (lane / kPer == p) mine[c] = x;
         }
     }
-    if (!valid || is_long || empty) return;
+    if (!valid || is_long) return;
     store_sums(sums, gi, mine);
 }
 
@@ -1989,13 +1997,11 @@ __global__ __launch_bounds__(256) void reduce_long_kernel(const uint2* __restric
 // writes; one 96-byte read of the BVH-order sums): gradients of value, weight
 // and mean, and the covariance chain (conic gradient -> Sigma -> M -> scales,
 // rotations) in double from the float inputs.  A zero conic gradient (no
-// records) gives zero scale and rotation gradients without the chain; a
-// Gaussian without records (empty segment) has all-zero gradients and its sums
-// are not read.
+// records) gives zero scale and rotation gradients without the chain.
 // fp64_ratio > 0 (FP64 mode): Gaussians on the precise path have their conic
 // gradient summed in the factor frame (covariance_chain repeats the test).
 __global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __restrict__ inv_perm,
-                                                    const GRecord* __restrict__ rec, const uint2* __restrict__ seg,
+                                                    const GRecord* __restrict__ rec,
                                                     const double* __restrict__ sums, const float* __restrict__ scales,
                                                     const float* __restrict__ rots, float kmod,
                                                     float* __restrict__ dmeans, float* __restrict__ dscales,
@@ -2005,19 +2011,13 @@ __global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __res
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= P) return;
     const uint32_t ri = inv_perm[id];
-    const uint2 be = seg[ri];
+    const double2* s2 = reinterpret_cast<const double2*>(sums + (size_t)kSumStride * ri);
     double m[12];
-    if (be.y > be.x) {
-        const double2* s2 = reinterpret_cast<const double2*>(sums + (size_t)kSumStride * ri);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            const double2 v = s2[c];
-            m[2 * c] = v.x;
-            m[2 * c + 1] = v.y;
-        }
-    } else {
-#pragma unroll
-        for (int c = 0; c < 12; ++c) m[c] = 0.0;
+    for (int c = 0; c < 6; ++c) {
+        const double2 v = s2[c];
+        m[2 * c] = v.x;
+        m[2 * c + 1] = v.y;
     }
     dvalues[id] = (float)m[0];
     dweights[id] = (float)m[1];
@@ -2242,7 +2242,7 @@ struct BackwardLayout {
     PartialRecord* parts;
     uint32_t *keys, *vals, *keys_alt, *vals_alt, *scratch;
     uint2* seg;    // per Gaussian (BVH order): its records [x, y) in the sorted order
-    double* sums;  // kSumStride per Gaussian, BVH order (written only for Gaussians with records)
+    double* sums;  // kSumStride per Gaussian, BVH order
     uint32_t* long_list;  // [0] = count, [1..] = Gaussians with long segments
     BackwardLayout(Arena& a, int P, uint64_t cap)
     {
@@ -2314,8 +2314,8 @@ int sample_set_build_impl(const Batch& bt, const float* samples, void* set, size
         // Morton order within each volume: a batch's samples are volume-major, and
         // each volume's range is sorted on its own (segmented sort)
         bool alt = false;
-        GVC_TRY(radix_sort_u32(l.keys, l.idx, l.keys_alt, l.idx_alt, (size_t)S, 30, l.scratch, &alt, st, nullptr,
-                               false, bt.V > 1 ? bv.S_off : nullptr, bt.V));
+        GVC_TRY(radix_sort_u32(l.keys, l.idx, l.keys_alt, l.idx_alt, (size_t)S, kSampleKeyBits, l.scratch, &alt, st,
+                               nullptr, false, bt.V > 1 ? bv.S_off : nullptr, bt.V));
         const uint32_t* perm = alt ? l.idx_alt : l.idx;
         sample_gather_kernel<<<blocks_for((size_t)NG * 64, 256), 256, 0, st>>>(samples, bv, NG * 64, perm, l.pts);
         GVC_LAUNCH_CHECK("sample_gather");
@@ -2599,7 +2599,7 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
     GVC_LAUNCH_CHECK("reduce_sums");
     reduce_long_kernel<<<1024, 256, 0, st>>>(bl.seg, svals, bl.parts, bl.long_list, bl.sums);
     GVC_LAUNCH_CHECK("reduce_long");
-    chain_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, l.rec, bl.seg, bl.sums, scales, rots, k, dmeans,
+    chain_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, l.rec, bl.sums, scales, rots, k, dmeans,
                                                      dscales, drots, dvalues, dweights);
     GVC_LAUNCH_CHECK("chain");
     return GVC_OK;
