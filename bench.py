@@ -95,7 +95,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
-    ap.add_argument("--graph", action="store_true", help="capture the training step in a hipGraph")
+    ap.add_argument("--graph", action="store_true", help="capture the training step in a hipGraph (default for "
+                                                         "--config grid, whose eager step is bound by launches)")
+    ap.add_argument("--eager", action="store_true", help="grid: run the step eagerly instead of from a graph")
     ap.add_argument("--max-scale", type=float, default=0.02,
                     help="soft scale cap of the reference model (R/scene/gaussian_model.py:64); 0 disables it")
     ap.add_argument("--autotune", action="store_true",
@@ -275,7 +277,8 @@ def main():
     stage_avg = {k: float(np.mean(v)) for k, v in stage_ms.items()}
 
     graph = None
-    if args.graph:
+    if args.graph or (args.config == "grid" and not args.eager):
+        args.graph = True
         static = torch.empty((S, 3), device=dev)
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
