@@ -4,7 +4,7 @@
 //   sample set   : Morton keys of the samples (top 24 of 30 bits) -> stable radix sort -> float4
 //                  points (x, y, z, original index) in Morton order; every 64
 //                  consecutive points form one group = one wavefront.
-//   forward      : top 32 bits of the means' 63-bit Morton codes (the reference's
+//   forward      : top 24 bits of the means' 63-bit Morton codes (the reference's
 //                  morton3d_unit) -> stable radix sort -> canonical preprocess in
 //                  the caller's order written as 80-byte records at their BVH
 //                  positions -> Karras radix tree over leaves of
@@ -237,12 +237,12 @@ __global__ void sample_header_kernel(SampleSetHeader* h, int S, int NG)
 // ---------------------------------------------------------------------------
 // Gaussian BVH
 // ---------------------------------------------------------------------------
-// The tree orders Gaussians by the top 32 bits of their 63-bit Morton codes
-// (about 11 bits per axis, a lattice of ~2^32 cells, far finer than the Gaussians
-// of a volume): four 8-bit radix passes on 32-bit keys instead of eight on 64-bit
-// ones; equal keys keep the caller's order (stable sort) and the Karras build
-// breaks ties by index.
-constexpr int kGaussianCodeBits = 32;
+// The tree orders Gaussians by the top 24 bits of their 63-bit Morton codes
+// (8 bits per axis, a 256^3 lattice, finer than the Gaussians of a volume): three
+// 8-bit radix passes on 32-bit words instead of eight on 64-bit codes; equal keys
+// keep the caller's order (stable sort) and the Karras build breaks ties by
+// index.
+constexpr int kGaussianCodeBits = 24;
 
 __global__ void gaussian_codes_kernel(const float* __restrict__ means, int P, uint32_t* __restrict__ codes)
 {

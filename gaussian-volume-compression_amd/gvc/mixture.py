@@ -192,7 +192,7 @@ class ForwardState:
     def tree(self):
         """The Gaussian BVH of this forward (gvc_forward_tree): dict of device
         tensors order [P] int32 (caller index per BVH position), keys [P] int64
-        (the 32-bit Morton keys), nodes [NL-1, 4] int32 (left, right, first leaf,
+        (the 24-bit Morton keys), nodes [NL-1, 4] int32 (left, right, first leaf,
         last leaf; a child < 0 is leaf ~c), boxes [NL-1, 12] float32 (left child's
         lo.xyz hi.xyz, right child's lo.xyz hi.xyz).  Stream-ordered."""
         dev = self.buffer.device

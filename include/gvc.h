@@ -172,7 +172,7 @@ int gvc_forward_accumulate(const void *state, int P, int S, const gvc_params *pa
  *   order [P] u32: the caller's index of the Gaussian at each BVH position (stable
  *         sort of the keys, equal keys in caller order); leaf j holds positions
  *         [j L, min((j + 1) L, P));
- *   keys  [P] u32: the sorted keys, the top 32 bits of morton3d_unit of the means;
+ *   keys  [P] u32: the sorted keys, the top 24 bits of morton3d_unit of the means;
  *   nodes [NL - 1][4] i32 (NL = ceil(P / L) leaves; none when NL < 2): left child,
  *         right child (>= 0 internal node, < 0 leaf ~j), first and last leaf under
  *         the node; node 0 is the root.  Ties between equal leaf keys are broken by

@@ -4,7 +4,7 @@ the oracle: bit-exact Morton keys and order, topology and node boxes.
 The reference builds its Gaussian tree with cuBQL's radixBuilder
 (DGR/ext.cu:39-42, CR/rasterizer_impl.cu:78-83); the canonical tree here is the
 Karras radix tree over leaves of L consecutive Gaussians in Morton order.  Bars:
-  * order and keys: the stable sort of the top 32 bits of oracle.morton3d_unit
+  * order and keys: the stable sort of the top 24 bits of oracle.morton3d_unit
     of the means (equal keys keep the caller's order), bit for bit;
   * links and leaf ranges: oracle.karras_build over the leaves' first keys
     (pinned in tests/test_oracle.py against a recursive top-down split);
@@ -87,7 +87,7 @@ def test_tree_bit_exact(P, leaf, dup):
     nodes = tree["nodes"].cpu().numpy()
     boxes = tree["boxes"].cpu().numpy()
 
-    codes = oracle.morton3d_unit(means) >> np.uint64(31)
+    codes = oracle.morton3d_unit(means) >> np.uint64(39)
     ref_order = np.argsort(codes, kind="stable")
     assert np.array_equal(order, ref_order)
     assert np.array_equal(keys, codes[ref_order])
