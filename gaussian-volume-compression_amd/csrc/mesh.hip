@@ -20,8 +20,12 @@
 // mapping) is dropped when it misses the tet's bounding box or lies strictly
 // outside one of the tet's face planes (double, with a rounding allowance).  So
 // most candidates are rejected from the entry alone, and a cell that the tet's
-// box reaches but the tet does not gets no entry.  The survivors are tested with
-// a per-tet barycentric matrix M (computed in double, stored as float):
+// box reaches but the tet does not gets no entry.  Each cell's list starts on a
+// 128-byte line and is padded with empty entries (mask 0) to whole lines of 16
+// entries, so a sample reads its cell's list one whole line per load.  The
+// survivors are tested with a per-tet barycentric matrix M (computed in double,
+// stored as float) -- a 48-byte filter record {M, v3}; the vertex ids sit in a
+// separate array, read only by the exact predicate and vertex interpolation:
 // b = M (p - v3), evaluated in double; a decision is taken only when every |b_i|
 // exceeds a margin e_i = 1e-6 sum_j |M_ij d_j| + 2e-9 |M|_1 |d|_1 with
 // d = p - v3 (and e_3 = e_0 + e_1 + e_2 + 1e-15 for b_3 = 1 - b_0 - b_1 - b_2):
@@ -42,11 +46,15 @@
 namespace gvc {
 
 struct alignas(16) TetRec {
-    int v[4];
     float m[9];   // rows of inverse([v0-v3 | v1-v3 | v2-v3]); m[0] = NaN: exact test only
     float v3[3];
 };
-static_assert(sizeof(TetRec) == 64, "TetRec layout");
+static_assert(sizeof(TetRec) == 48, "TetRec layout");
+
+// Entries per 128-byte line of a cell's list; an empty entry has tet id
+// kNoTet and mask 0 (never a candidate).
+constexpr int kLineEntries = 16;
+constexpr uint32_t kNoTet = 0xffffffffu;
 
 struct alignas(16) TetGridHeader {
     float lo[3];
@@ -97,17 +105,17 @@ __device__ void tet_inverse(const float* __restrict__ pts, const int v[4], doubl
 
 // Per-tet barycentric matrix (float copy of tet_inverse) and bounding box.
 __global__ void tet_records_kernel(const float* __restrict__ pts, const int* __restrict__ tets, int C,
-                                   TetRec* __restrict__ recs, float* __restrict__ boxes)
+                                   TetRec* __restrict__ recs, int4* __restrict__ verts, float* __restrict__ boxes)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= C) return;
     TetRec r;
-    for (int k = 0; k < 4; ++k) r.v[k] = tets[4 * t + k];
+    const int v[4] = { tets[4 * t], tets[4 * t + 1], tets[4 * t + 2], tets[4 * t + 3] };
     for (int a = 0; a < 3; ++a) {
-        r.v3[a] = pts[3 * r.v[3] + a];
-        float lo = pts[3 * r.v[0] + a], hi = lo;
+        r.v3[a] = pts[3 * v[3] + a];
+        float lo = pts[3 * v[0] + a], hi = lo;
         for (int k = 1; k < 4; ++k) {
-            float x = pts[3 * r.v[k] + a];
+            float x = pts[3 * v[k] + a];
             lo = fminf(lo, x);
             hi = fmaxf(hi, x);
         }
@@ -116,9 +124,10 @@ __global__ void tet_records_kernel(const float* __restrict__ pts, const int* __r
     }
     double inv[9];
     bool ok;
-    tet_inverse(pts, r.v, inv, ok);
+    tet_inverse(pts, v, inv, ok);
     for (int k = 0; k < 9; ++k) r.m[k] = ok ? (float)inv[k] : __builtin_nanf("");
     recs[t] = r;
+    verts[t] = make_int4(v[0], v[1], v[2], v[3]);
 }
 
 // Bound values (gvc_tet_locator_bind_values): per tet the affine interpolant
@@ -126,19 +135,22 @@ __global__ void tet_records_kernel(const float* __restrict__ pts, const int* __r
 // whose matrix is not kept (cond_F > 1e6, degenerate) store the four vertex
 // values instead and are interpolated from orientation volumes.
 __global__ void tet_coef_kernel(const float* __restrict__ pts, const TetRec* __restrict__ recs,
-                                const double* __restrict__ values, int C, double4* __restrict__ coef)
+                                const int4* __restrict__ verts, const double* __restrict__ values, int C,
+                                double4* __restrict__ coef)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= C) return;
-    const TetRec r = recs[t];
-    const double w0 = values[r.v[0]], w1 = values[r.v[1]], w2 = values[r.v[2]], w3 = values[r.v[3]];
-    if (r.m[0] != r.m[0]) {
+    const float m0 = recs[t].m[0];
+    const int4 vi = verts[t];
+    const int v[4] = { vi.x, vi.y, vi.z, vi.w };
+    const double w0 = values[v[0]], w1 = values[v[1]], w2 = values[v[2]], w3 = values[v[3]];
+    if (m0 != m0) {
         coef[t] = make_double4(w0, w1, w2, w3);
         return;
     }
     double inv[9];
     bool ok;
-    tet_inverse(pts, r.v, inv, ok);
+    tet_inverse(pts, v, inv, ok);
     const double d0 = w0 - w3, d1 = w1 - w3, d2 = w2 - w3;
     coef[t] = make_double4(w3, inv[0] * d0 + inv[3] * d1 + inv[6] * d2, inv[1] * d0 + inv[4] * d1 + inv[7] * d2,
                            inv[2] * d0 + inv[5] * d1 + inv[8] * d2);
@@ -330,8 +342,23 @@ __global__ __launch_bounds__(kBuildThreads) void tet_count_kernel(const float* _
             }
 }
 
+// Each cell's entry count rounded up to whole lines of kLineEntries.
+__global__ __launch_bounds__(256) void pad_counts_kernel(uint32_t* __restrict__ counts, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) counts[i] = (counts[i] + (kLineEntries - 1)) & ~(uint32_t)(kLineEntries - 1);
+}
+
+// Every entry slot empty: tet id kNoTet, mask 0.
+__global__ __launch_bounds__(256) void empty_entries_kernel(uint2* __restrict__ entries, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        entries[i] = make_uint2(kNoTet, 0u);
+}
+
 // entry = (tet id, sub-cell mask of the tet in the entry's cell); cells whose
-// mask is empty get no entry (the same test as tet_count_kernel).
+// mask is empty get no entry (the same test as tet_count_kernel).  A cell's
+// entries fill the front of its padded list (cursor = the cell's entry count).
 __global__ __launch_bounds__(kBuildThreads) void tet_fill_kernel(const float* __restrict__ pts, const int* __restrict__ tets,
                                 const float* __restrict__ boxes, int C, const TetGridHeader* __restrict__ h,
                                 const uint32_t* __restrict__ starts, uint32_t* __restrict__ cursor,
@@ -360,22 +387,23 @@ __global__ __launch_bounds__(kBuildThreads) void tet_fill_kernel(const float* __
             }
 }
 
-// Sorts each cell's entries by tet index, so the locate kernel can stop at the
-// first containing tet.  One wavefront per cell: up to kSortCap entries are staged
+// Sorts each cell's entries (the first counts[g] of its list; the empty padding
+// stays behind them) by tet index, so the locate kernel can stop at the first
+// containing tet.  One wavefront per cell: up to kSortCap entries are staged
 // in LDS and each lane places its entries at their rank (count of smaller tet
 // indices; a tet appears at most once per cell); longer cells fall back to an
 // insertion sort by one lane.
 constexpr int kSortCap = 512;
 
-__global__ __launch_bounds__(256) void tet_sort_cells_kernel(const uint32_t* __restrict__ starts, size_t n_grid,
+__global__ __launch_bounds__(256) void tet_sort_cells_kernel(const uint32_t* __restrict__ starts,
+                                                             const uint32_t* __restrict__ counts, size_t n_grid,
                                                              uint2* __restrict__ entries)
 {
     __shared__ uint2 buf[4][kSortCap];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const size_t g = (size_t)blockIdx.x * 4 + wid;
     if (g >= n_grid) return;
-    const uint32_t b = starts[g], e = starts[g + 1];
-    const uint32_t n = e - b;
+    const uint32_t b = starts[g], n = counts[g], e = b + n;
     if (n <= 1) return;
     if (n <= (uint32_t)kSortCap) {
         for (uint32_t i = lane; i < n; i += 64) buf[wid][i] = entries[b + i];
@@ -401,10 +429,10 @@ __global__ __launch_bounds__(256) void tet_sort_cells_kernel(const uint32_t* __r
     }
 }
 
-// Exact containment of p in the closed tet r.
-__device__ __noinline__ bool tet_contains_exact(const float* __restrict__ pts, const TetRec& r, const float* p)
+// Exact containment of p in the closed tet with vertex ids v.
+__device__ __noinline__ bool tet_contains_exact(const float* __restrict__ pts, const int4 v, const float* p)
 {
-    const float* V[4] = { pts + 3 * r.v[0], pts + 3 * r.v[1], pts + 3 * r.v[2], pts + 3 * r.v[3] };
+    const float* V[4] = { pts + 3 * v.x, pts + 3 * v.y, pts + 3 * v.z, pts + 3 * v.w };
     const int sT = orient3d_sign(V[0], V[1], V[2], V[3], nullptr);
     if (sT == 0) return false;
     for (int i = 0; i < 4; ++i) {
@@ -428,10 +456,11 @@ __device__ __forceinline__ double orient3d_value(const float* a, const float* b,
     return adz * (bdx * cdy - bdy * cdx) + bdz * (cdx * ady - cdy * adx) + cdz * (adx * bdy - ady * bdx);
 }
 
-// Barycentric coordinates of p in r from double orientation volumes.
-__device__ void tet_bary(const float* __restrict__ pts, const TetRec& r, const float* p, double b[4])
+// Barycentric coordinates of p in the tet with vertex ids v from double
+// orientation volumes.
+__device__ void tet_bary(const float* __restrict__ pts, const int4 v, const float* p, double b[4])
 {
-    const float* V[4] = { pts + 3 * r.v[0], pts + 3 * r.v[1], pts + 3 * r.v[2], pts + 3 * r.v[3] };
+    const float* V[4] = { pts + 3 * v.x, pts + 3 * v.y, pts + 3 * v.z, pts + 3 * v.w };
     const double vol = orient3d_value(V[0], V[1], V[2], V[3]);
     b[0] = -orient3d_value(V[1], V[2], V[3], p) / vol;
     b[1] = orient3d_value(V[0], V[2], V[3], p) / vol;
@@ -468,7 +497,6 @@ __device__ __forceinline__ int tet_filter(const TetRec& r, const float* p)
 __device__ __forceinline__ int mark_undecided(int i) { return -2 - i; }
 
 constexpr int kLocateThreads = 256;
-constexpr int kCandBatch = 8;  // candidates a lane collects before testing them
 
 // EXACT = false (filter pass): threads follow the sample set's Morton order when
 // one is given (sorted != nullptr; neighbouring threads share grid cells and
@@ -477,21 +505,23 @@ constexpr int kCandBatch = 8;  // candidates a lane collects before testing them
 // is marked (mark_undecided, no result written).  EXACT = true: threads follow
 // the caller's order (coalesced reads of the marks); only marked samples are
 // redone, undecided candidates by the exact predicate.
-// A lane first collects the entries of its grid cell whose sub-cell mask holds
-// the point's sub-cell (up to kCandBatch, in tet order, in LDS), then tests those
-// with the filter, the next candidate's record in flight, so a wave runs the
-// filter once per candidate of its busiest lane rather than once per entry of
-// its longest cell.  Values: coef != nullptr (bound values) evaluates the
-// tet's affine interpolant (one 32-byte read); otherwise barycentric
-// coordinates from orientation volumes weight values[] at the four vertices.
+// A lane reads its cell's list one 128-byte line (16 entries) per load, keeps
+// the entries whose sub-cell mask holds the point's sub-cell (in tet order, in
+// LDS) and tests those with the filter, the next candidate's record in flight,
+// stopping at the first decided containment; so a wave runs the filter once per
+// candidate of its busiest lane.  Values: coef != nullptr (bound values)
+// evaluates the tet's affine interpolant (one 32-byte read); otherwise
+// barycentric coordinates from orientation volumes weight values[] at the four
+// vertices.
 template <bool EXACT>
 __global__ __launch_bounds__(kLocateThreads) void tet_locate_kernel(
-    const float* __restrict__ pts, const TetRec* __restrict__ recs, const TetGridHeader* __restrict__ h,
-    const uint32_t* __restrict__ starts, const uint2* __restrict__ entries, const double* __restrict__ values,
-    const double4* __restrict__ coef, int S, const float* __restrict__ samples, const float4* __restrict__ sorted,
-    int* __restrict__ cell_ids, double* __restrict__ out_values, float* __restrict__ bary_out)
+    const float* __restrict__ pts, const TetRec* __restrict__ recs, const int4* __restrict__ verts,
+    const TetGridHeader* __restrict__ h, const uint32_t* __restrict__ starts, const uint2* __restrict__ entries,
+    const double* __restrict__ values, const double4* __restrict__ coef, int S, const float* __restrict__ samples,
+    const float4* __restrict__ sorted, int* __restrict__ cell_ids, double* __restrict__ out_values,
+    float* __restrict__ bary_out)
 {
-    __shared__ int cand[kCandBatch][kLocateThreads];
+    __shared__ int cand[kLineEntries][kLocateThreads];
     const int tid = threadIdx.x;
     const int i0 = (int)xcd_block(blockIdx.x, gridDim.x) * kLocateThreads + tid;
     if (i0 >= S) return;
@@ -533,15 +563,16 @@ __global__ __launch_bounds__(kLocateThreads) void tet_locate_kernel(
         const uint32_t e = starts[g + 1];
         bool done = false;
         while (!done && k < e) {
+            const uint4* __restrict__ line = reinterpret_cast<const uint4*>(entries + k);
+            uint4 en[kLineEntries / 2];
+#pragma unroll
+            for (int u = 0; u < kLineEntries / 2; ++u) en[u] = line[u];
+            k += kLineEntries;
             int n = 0;
-            while (k < e && n <= kCandBatch - 4) {
-                uint2 en[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) en[u] = (k + u < e) ? entries[k + u] : make_uint2(0u, 0u);
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    if (en[u].y & sbit) cand[n++][tid] = (int)en[u].x;
-                k += 4;
+            for (int u = 0; u < kLineEntries / 2; ++u) {
+                if (en[u].y & sbit) cand[n++][tid] = (int)en[u].x;
+                if (en[u].w & sbit) cand[n++][tid] = (int)en[u].z;
             }
             TetRec rn;
             if (n > 0) rn = recs[cand[0][tid]];
@@ -558,7 +589,7 @@ __global__ __launch_bounds__(kLocateThreads) void tet_locate_kernel(
                 }
                 if (f == -1) {
                     if (EXACT) {
-                        if (tet_contains_exact(pts, r, p)) {
+                        if (tet_contains_exact(pts, verts[t], p)) {
                             best = t;
                             br = r;
                             done = true;
@@ -585,7 +616,11 @@ __global__ __launch_bounds__(kLocateThreads) void tet_locate_kernel(
         const bool affine = coef && br.m[0] == br.m[0];
         double4 cf = make_double4(0.0, 0.0, 0.0, 0.0);
         if (coef && out_values) cf = coef[best];
-        if (bary_out || (out_values && !affine && (coef || values))) tet_bary(pts, br, p, bb);
+        int4 bv = make_int4(0, 0, 0, 0);
+        if (bary_out || (out_values && !affine && (coef || values))) {
+            bv = verts[best];
+            tet_bary(pts, bv, p, bb);
+        }
         if (out_values) {
             if (affine)
                 v = cf.x + cf.y * ((double)p[0] - br.v3[0]) + cf.z * ((double)p[1] - br.v3[1]) +
@@ -593,8 +628,7 @@ __global__ __launch_bounds__(kLocateThreads) void tet_locate_kernel(
             else if (coef)
                 v = bb[0] * cf.x + bb[1] * cf.y + bb[2] * cf.z + bb[3] * cf.w;
             else if (values)
-                v = bb[0] * values[br.v[0]] + bb[1] * values[br.v[1]] + bb[2] * values[br.v[2]] +
-                    bb[3] * values[br.v[3]];
+                v = bb[0] * values[bv.x] + bb[1] * values[bv.y] + bb[2] * values[bv.z] + bb[3] * values[bv.w];
         }
     }
     if (out_values) out_values[s] = v;
@@ -608,6 +642,7 @@ struct TetLayout {
     TetGridHeader* header;
     unsigned* bbox;
     TetRec* recs;
+    int4* verts;     // vertex ids per tet
     uint2* entries;
     float* boxes;
     double4* coef;   // bound values' per-tet interpolants (NaN until bound)
@@ -617,6 +652,7 @@ struct TetLayout {
         header = a.take<TetGridHeader>(1);
         bbox = a.take<unsigned>(8);
         recs = a.take<TetRec>((size_t)(C > 0 ? C : 1));
+        verts = a.take<int4>((size_t)(C > 0 ? C : 1));
         entries = a.take<uint2>((size_t)(cap > 0 ? cap : 1));
         starts = a.take<uint32_t>(n_grid_max + 1);
         cursor = a.take<uint32_t>(n_grid_max);
@@ -821,14 +857,16 @@ int tet_locator_build(int n_points, const float* pts, int C, const int* tets, un
     H.capacity = cap;
     const size_t n_grid = (size_t)H.dims[0] * H.dims[1] * H.dims[2];
     GVC_HIP(hipMemcpyAsync(l.header, &H, sizeof(H), hipMemcpyHostToDevice, st));
-    tet_records_kernel<<<blocks_for(C, 256), 256, 0, st>>>(pts, tets, C, l.recs, l.boxes);
+    tet_records_kernel<<<blocks_for(C, 256), 256, 0, st>>>(pts, tets, C, l.recs, l.verts, l.boxes);
     GVC_LAUNCH_CHECK("tet_records");
     GVC_TRY(fill_u32(reinterpret_cast<uint32_t*>(l.coef), 0xffffffffu, (size_t)C * 8, st));   // NaN: unbound
     GVC_TRY(fill_u32(l.starts, 0u, n_grid + 1, st));
     tet_count_kernel<<<blocks_for(C, kBuildThreads), kBuildThreads, 0, st>>>(pts, tets, l.boxes, C, l.header,
                                                                               l.starts);
     GVC_LAUNCH_CHECK("tet_count");
-    // the bounding-box words are free again: 64-bit entry total
+    pad_counts_kernel<<<blocks_for(n_grid, 256), 256, 0, st>>>(l.starts, n_grid);
+    GVC_LAUNCH_CHECK("pad_counts");
+    // the bounding-box words are free again: 64-bit total of the padded lists
     unsigned long long* total64 = reinterpret_cast<unsigned long long*>(l.bbox);
     GVC_TRY(fill_u32(l.bbox, 0u, 2, st));
     count_total_kernel<<<256, 256, 0, st>>>(l.starts, n_grid, total64);
@@ -842,11 +880,16 @@ int tet_locator_build(int n_points, const float* pts, int C, const int* tets, un
     const uint32_t total = (uint32_t)total_all;
     if (needed) *needed = total;
     if (total > cap) return set_error(GVC_ERR_CAPACITY, "tet locator entry capacity too small");
+    if (total > 0) {
+        const unsigned nb_empty = blocks_for(total, 256);
+        empty_entries_kernel<<<nb_empty < 8192u ? nb_empty : 8192u, 256, 0, st>>>(l.entries, total);
+        GVC_LAUNCH_CHECK("empty_entries");
+    }
     GVC_TRY(fill_u32(l.cursor, 0u, n_grid, st));
     tet_fill_kernel<<<blocks_for(C, kBuildThreads), kBuildThreads, 0, st>>>(pts, tets, l.boxes, C, l.header,
                                                                              l.starts, l.cursor, l.entries, cap);
     GVC_LAUNCH_CHECK("tet_fill");
-    tet_sort_cells_kernel<<<blocks_for(n_grid, 4), 256, 0, st>>>(l.starts, n_grid, l.entries);
+    tet_sort_cells_kernel<<<blocks_for(n_grid, 4), 256, 0, st>>>(l.starts, l.cursor, n_grid, l.entries);
     GVC_LAUNCH_CHECK("tet_sort_cells");
     H.n_entries = total;
     GVC_HIP(hipMemcpyAsync(l.header, &H, sizeof(H), hipMemcpyHostToDevice, st));
@@ -872,7 +915,7 @@ int tet_locator_bind_values(int n_points, const float* pts, int C, void* loc, si
     Arena a(loc, bytes);
     TetLayout l(a, C, max_grid_cells(C), cap);
     if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "tet locator buffer too small");
-    tet_coef_kernel<<<blocks_for(C, 256), 256, 0, st>>>(pts, l.recs, values, C, l.coef);
+    tet_coef_kernel<<<blocks_for(C, 256), 256, 0, st>>>(pts, l.recs, l.verts, values, C, l.coef);
     GVC_LAUNCH_CHECK("tet_coef");
     return GVC_OK;
 }
@@ -891,10 +934,10 @@ int tet_locate_interp(int n_points, const float* pts, int C, const void* loc, si
     const double4* coef = bound ? l.coef : nullptr;
     if (bound) values = nullptr;
     const unsigned nb = blocks_for((size_t)S, kLocateThreads);
-    tet_locate_kernel<false><<<nb, kLocateThreads, 0, st>>>(pts, l.recs, l.header, l.starts, l.entries, values, coef,
+    tet_locate_kernel<false><<<nb, kLocateThreads, 0, st>>>(pts, l.recs, l.verts, l.header, l.starts, l.entries, values, coef,
                                                             S, samples, sorted, cell_ids, out_values, bary);
     GVC_LAUNCH_CHECK("tet_locate");
-    tet_locate_kernel<true><<<nb, kLocateThreads, 0, st>>>(pts, l.recs, l.header, l.starts, l.entries, values, coef,
+    tet_locate_kernel<true><<<nb, kLocateThreads, 0, st>>>(pts, l.recs, l.verts, l.header, l.starts, l.entries, values, coef,
                                                            S, samples, sorted, cell_ids, out_values, bary);
     GVC_LAUNCH_CHECK("tet_locate_exact");
     return GVC_OK;

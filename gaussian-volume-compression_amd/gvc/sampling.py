@@ -42,7 +42,7 @@ class TetMeshSampler:
             if self.values.numel() != N:
                 raise ValueError("values must have one entry per point")
         self.N, self.C = N, int(self.tets.shape[0])
-        cap = max(16 * self.C, 1024)
+        cap = max(24 * self.C, 1024)
         for _ in range(2):
             nbytes = lib().gvc_tet_locator_bytes(self.N, self.C, cap)
             buf = torch.empty(nbytes, dtype=torch.uint8, device=dev)

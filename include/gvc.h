@@ -223,7 +223,8 @@ int gvc_mixture_backward(int P, const float *means3D, const float *scales, const
  * int32 containing-cell id (-1 outside) besides the value (-1.0 outside, :382).
  * points [n_points,3] f32, tets [n_tets,4] i32 (device).  The locator buffer
  * (gvc_tet_locator_bytes) holds a uniform grid of candidate lists with room for
- * entry_capacity (tet, grid cell) entries.
+ * entry_capacity (tet, grid cell) entry slots; each cell's list is padded to a
+ * multiple of 16 slots (one 128-byte line each).
  * gvc_tet_locator_build is BLOCKING (two small D2H reads); GVC_ERR_CAPACITY with
  * *entries_needed set means: rebuild with a capacity >= *entries_needed. */
 size_t gvc_tet_locator_bytes(int n_points, int n_tets, uint64_t entry_capacity);
