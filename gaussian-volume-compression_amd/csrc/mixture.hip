@@ -109,9 +109,9 @@ __device__ __forceinline__ int volume_of(const int* __restrict__ off, int V, int
 }
 
 // one volume of P Gaussians and S samples with leaves of L Gaussians
-__global__ void single_table_kernel(int* __restrict__ t, int P, int S, int L)
+// The one-volume table of P Gaussians, S samples, leaf size L (one thread).
+__device__ __forceinline__ void write_single_table(int* __restrict__ t, int P, int S, int L)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int NL = P > 0 ? (P + L - 1) / L : 0;
     const int v[5][2] = { { 0, P }, { 0, S }, { 0, (S + 63) / 64 }, { 0, NL }, { 0, NL > 1 ? NL - 1 : 0 } };
     t[0] = 1; t[1] = t[2] = t[3] = 0;
@@ -119,6 +119,11 @@ __global__ void single_table_kernel(int* __restrict__ t, int P, int S, int L)
         t[4 + 2 * k] = v[k][0];
         t[4 + 2 * k + 1] = v[k][1];
     }
+}
+
+__global__ void single_table_kernel(int* __restrict__ t, int P, int S, int L)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) write_single_table(t, P, S, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -173,6 +178,14 @@ struct SampleSetHeader {
 };
 constexpr int kSampleMagic = 0x47564353;
 
+__device__ __forceinline__ void write_sample_header(SampleSetHeader* h, int S, int NG)
+{
+    h->S = S;
+    h->NG = NG;
+    h->magic = kSampleMagic;
+    h->pad = 0;
+}
+
 // 30-bit Morton key of a sample: coordinates clamped to the unit cube (NaN -> 0),
 // 10 bits per axis.
 // Bits of sample_code dropped from the sample set's sort key: the points are
@@ -195,10 +208,13 @@ __device__ __forceinline__ uint32_t sample_code(float x, float y, float z)
     return c;
 }
 
+// With table != nullptr (a single-volume set), thread 0 also writes the set's
+// one-volume table.
 __global__ void sample_codes_kernel(const float* __restrict__ s, int S, uint32_t* __restrict__ keys,
-                                    uint32_t* __restrict__ idx)
+                                    uint32_t* __restrict__ idx, int* __restrict__ table)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (table && i == 0) write_single_table(table, 0, S, 1);
     if (i >= S) return;
     // the sort key: the top 24 bits (8 per axis, 256^3 cells; ties keep the
     // caller's order)
@@ -208,11 +224,14 @@ __global__ void sample_codes_kernel(const float* __restrict__ s, int S, uint32_t
 
 // Sorted points (volume-major, Morton order within a volume), each volume
 // padded with NaN points (original index -1) to whole 64-point groups, so every
-// group can be read with wave-uniform loads and belongs to one volume.
+// group can be read with wave-uniform loads and belongs to one volume.  Thread 0
+// also writes the set's header.
 __global__ void sample_gather_kernel(const float* __restrict__ s, BatchView b, int S64,
-                                     const uint32_t* __restrict__ perm, float4* __restrict__ pts)
+                                     const uint32_t* __restrict__ perm, float4* __restrict__ pts,
+                                     SampleSetHeader* __restrict__ header, int S)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) write_sample_header(header, S, S64 / 64);
     if (i >= S64) return;
     const int v = b.V == 1 ? 0 : volume_of(b.G_off, b.V, i >> 6);
     const int r = i - 64 * b.G_off[v];
@@ -225,13 +244,10 @@ __global__ void sample_gather_kernel(const float* __restrict__ s, BatchView b, i
     pts[i] = make_float4(s[3 * j], s[3 * j + 1], s[3 * j + 2], __builtin_bit_cast(float, (int)j));
 }
 
-// Volume of each sorted item (the second, stable sort key of a batch).
+// The header of a set without samples (sample_gather_kernel writes it otherwise).
 __global__ void sample_header_kernel(SampleSetHeader* h, int S, int NG)
 {
-    h->S = S;
-    h->NG = NG;
-    h->magic = kSampleMagic;
-    h->pad = 0;
+    write_sample_header(h, S, NG);
 }
 
 // ---------------------------------------------------------------------------
@@ -244,9 +260,16 @@ __global__ void sample_header_kernel(SampleSetHeader* h, int S, int NG)
 // index.
 constexpr int kGaussianCodeBits = 24;
 
-__global__ void gaussian_codes_kernel(const float* __restrict__ means, int P, uint32_t* __restrict__ codes)
+// Thread 0 also clears the state's four error words and, for a state without a
+// caller's table, writes its one-volume table (S samples, leaf size L).
+__global__ void gaussian_codes_kernel(const float* __restrict__ means, int P, uint32_t* __restrict__ codes,
+                                      uint32_t* __restrict__ err, int* __restrict__ table, int S, int L)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        for (int k = 0; k < 4; ++k) err[k] = 0u;
+        if (table) write_single_table(table, P, S, L);
+    }
     if (i >= P) return;
     codes[i] = (uint32_t)(morton3d_unit(means[3 * i], means[3 * i + 1], means[3 * i + 2], false) >>
                           (63 - kGaussianCodeBits));
@@ -309,11 +332,10 @@ __global__ __launch_bounds__(256) void record_kernel(int P, const uint32_t* __re
 // the mean box extent per axis over up to 1024 records taken at a fixed stride,
 // summed in units of 2^-20 (integers: the same total in any order).  ext[4v..]
 // = (sum x, sum y, sum z, count).
-__global__ __launch_bounds__(256) void extent_kernel(const GRecord* __restrict__ rec, BatchView bv,
-                                                     unsigned long long* __restrict__ ext)
+__device__ __forceinline__ void volume_extent(const GRecord* __restrict__ rec, const BatchView& bv, int v,
+                                              unsigned long long* __restrict__ ext)
 {
     __shared__ unsigned long long red[3][256];
-    const int v = blockIdx.x;
     const long long g0 = bv.P_off[v], n = bv.P_off[v + 1] - g0;
     const long long m = n < 1024 ? n : 1024;
     unsigned long long sum[3] = { 0, 0, 0 };
@@ -345,10 +367,19 @@ __device__ __forceinline__ int leaf_first(const BatchView& b, int L, int j, int&
     return g0;
 }
 
-__global__ void leaf_codes_kernel(const uint32_t* __restrict__ sorted_codes, BatchView b, int NL, int L,
-                                  uint64_t* __restrict__ leaf_codes)
+// Blocks [0, V): the extent sums of volume blockIdx.x (volume_extent); the
+// others: the sorted code of every leaf's first Gaussian.  256 threads.
+__global__ __launch_bounds__(256) void extent_leaf_codes_kernel(const GRecord* __restrict__ rec,
+                                                                const uint32_t* __restrict__ sorted_codes,
+                                                                BatchView b, int NL, int L,
+                                                                unsigned long long* __restrict__ ext,
+                                                                uint64_t* __restrict__ leaf_codes)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int)blockIdx.x < b.V) {
+        volume_extent(rec, b, (int)blockIdx.x, ext);
+        return;
+    }
+    const int j = ((int)blockIdx.x - b.V) * 256 + (int)threadIdx.x;
     if (j >= NL) return;
     int end;
     leaf_codes[j] = sorted_codes[leaf_first(b, L, j, end)];
@@ -528,9 +559,12 @@ __global__ void node_box_kernel(const Box6* __restrict__ seg, uint32_t M, BatchV
 
 // Root boxes of volumes with a single leaf (no internal node); empty volumes get
 // an empty box.
-__global__ void leaf_root_kernel(const Box6* __restrict__ leaves, BatchView bv, float* __restrict__ root_box)
+// Thread 0 also clears the heavy-group count (*heavy) for group_cost_kernel.
+__global__ void leaf_root_kernel(const Box6* __restrict__ leaves, BatchView bv, float* __restrict__ root_box,
+                                 uint32_t* __restrict__ heavy)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v == 0) *heavy = 0u;
     if (v >= bv.V) return;
     const int nl = bv.L_off[v + 1] - bv.L_off[v];
     if (nl > 1) return;
@@ -980,11 +1014,18 @@ __device__ __forceinline__ bool lane_mode_group(const TreeView& tv, float px, fl
 // are flagged and listed in heavy[1 .. heavy[0]] (heavy[0] zero at launch).
 constexpr float kHeavyCandidates = 4096.0f;
 
+// Block 0 also clears the forward walk's queue words (its group queues and the
+// words from kQLog on), which nothing reads before the walk.
 __global__ __launch_bounds__(256) void group_cost_kernel(const float4* __restrict__ pts, int NG, BatchView bv,
                                                          const float* __restrict__ root_box,
                                                          const unsigned long long* __restrict__ ext_sum,
-                                                         uint32_t* __restrict__ heavy, uint8_t* __restrict__ flag)
+                                                         uint32_t* __restrict__ heavy, uint8_t* __restrict__ flag,
+                                                         uint32_t* __restrict__ queues)
 {
+    if (blockIdx.x == 0) {
+        for (int k = (int)threadIdx.x; k < kQueues * kLine; k += 256) queues[kQFwd + k] = 0u;
+        for (int k = (int)threadIdx.x; k < kQWords - kQLog; k += 256) queues[kQLog + k] = 0u;
+    }
     const int g = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     if (g >= NG) return;
     const int lane = threadIdx.x & 63;
@@ -2303,13 +2344,15 @@ int sample_set_build_impl(const Batch& bt, const float* samples, void* set, size
     Arena a(set, bytes);
     SampleSetLayout l(a, S, NG, bt.V);
     if (!a.fits()) return set_error(GVC_ERR_WORKSPACE_TOO_SMALL, "sample set buffer too small");
-    if (!bt.table) {
+    if (!bt.table && S == 0) {
         single_table_kernel<<<1, 64, 0, st>>>(l.table, 0, S, 1);
         GVC_LAUNCH_CHECK("sample_table");
     }
     const BatchView bv = batch_view(table_of(bt, l.table), bt.V);
     if (S > 0) {
-        sample_codes_kernel<<<blocks_for(S, 256), 256, 0, st>>>(samples, S, l.keys, l.idx);
+        // the codes kernel also writes the one-volume table of a set without one
+        sample_codes_kernel<<<blocks_for(S, 256), 256, 0, st>>>(samples, S, l.keys, l.idx,
+                                                               bt.table ? nullptr : l.table);
         GVC_LAUNCH_CHECK("sample_codes");
         // Morton order within each volume: a batch's samples are volume-major, and
         // each volume's range is sorted on its own (segmented sort)
@@ -2317,11 +2360,13 @@ int sample_set_build_impl(const Batch& bt, const float* samples, void* set, size
         GVC_TRY(radix_sort_u32(l.keys, l.idx, l.keys_alt, l.idx_alt, (size_t)S, kSampleKeyBits, l.scratch, &alt, st,
                                nullptr, false, bt.V > 1 ? bv.S_off : nullptr, bt.V));
         const uint32_t* perm = alt ? l.idx_alt : l.idx;
-        sample_gather_kernel<<<blocks_for((size_t)NG * 64, 256), 256, 0, st>>>(samples, bv, NG * 64, perm, l.pts);
+        sample_gather_kernel<<<blocks_for((size_t)NG * 64, 256), 256, 0, st>>>(samples, bv, NG * 64, perm, l.pts,
+                                                                               l.header, S);
         GVC_LAUNCH_CHECK("sample_gather");
+    } else {
+        sample_header_kernel<<<1, 1, 0, st>>>(l.header, S, NG);
+        GVC_LAUNCH_CHECK("sample_header");
     }
-    sample_header_kernel<<<1, 1, 0, st>>>(l.header, S, NG);
-    GVC_LAUNCH_CHECK("sample_header");
     return GVC_OK;
 }
 
@@ -2357,10 +2402,12 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
     const unsigned stages = prm->stages ? (unsigned)prm->stages : 7u;
 
     if (stages & GVC_STAGE_BUILD) {
-        GVC_TRY(fill_u32(l.err, 0u, 4, st));
-        if (!bt.table) {
-            single_table_kernel<<<1, 64, 0, st>>>(l.table, P, S, L);
-            GVC_LAUNCH_CHECK("table");
+        if (P == 0) {   // else gaussian_codes_kernel does both
+            GVC_TRY(fill_u32(l.err, 0u, 4, st));
+            if (!bt.table) {
+                single_table_kernel<<<1, 64, 0, st>>>(l.table, P, S, L);
+                GVC_LAUNCH_CHECK("table");
+            }
         }
     }
     const BatchView bv = batch_view(table_of(bt, l.table), bt.V);
@@ -2380,7 +2427,8 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
     const float factor_ratio = (prm->flags & GVC_FLAG_FP64_COND) ? factor_ratio_of(prm) : 0.f;
     if (stages & GVC_STAGE_BUILD) {
         // 1. Morton order of the Gaussians (within each volume)
-        gaussian_codes_kernel<<<blocks_for(P, 256), 256, 0, st>>>(means, P, l.codes);
+        gaussian_codes_kernel<<<blocks_for(P, 256), 256, 0, st>>>(means, P, l.codes, l.err,
+                                                                  bt.table ? nullptr : l.table, S, L);
         GVC_LAUNCH_CHECK("gaussian_codes");
         // (a batch's Gaussians are volume-major: each volume's range is sorted on
         // its own)
@@ -2394,11 +2442,10 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
         record_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, means, scales, rots, values, weights, k,
                                                           fp64_ratio, factor_ratio, l.rec, l.fac_lo, conics);
         GVC_LAUNCH_CHECK("record");
-        extent_kernel<<<bt.V, 256, 0, st>>>(l.rec, bv, l.ext_sum);
-        GVC_LAUNCH_CHECK("extent");
-        // 3. trees, one per volume
-        leaf_codes_kernel<<<blocks_for(NL, 256), 256, 0, st>>>(scodes, bv, NL, L, l.leaf_codes);
-        GVC_LAUNCH_CHECK("leaf_codes");
+        // 3. trees, one per volume (with the volumes' extent sums, one launch)
+        extent_leaf_codes_kernel<<<(unsigned)bt.V + blocks_for(NL, 256), 256, 0, st>>>(l.rec, scodes, bv, NL, L,
+                                                                                      l.ext_sum, l.leaf_codes);
+        GVC_LAUNCH_CHECK("extent_leaf_codes");
         if (bt.NI > 0) {
             karras_kernel<<<blocks_for(bt.NI, 256), 256, 0, st>>>(l.leaf_codes, bv, bt.NI, l.nodes);
             GVC_LAUNCH_CHECK("karras");
@@ -2426,21 +2473,23 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
             node_box_kernel<<<blocks_for(bt.NI, 256), 256, 0, st>>>(l.seg, M, bv, bt.NI, l.nodes, l.root_box);
             GVC_LAUNCH_CHECK("node_box");
         }
-        leaf_root_kernel<<<blocks_for(bt.V, 64), 64, 0, st>>>(l.seg + M, bv, l.root_box);
+        leaf_root_kernel<<<blocks_for(bt.V, 64), 64, 0, st>>>(l.seg + M, bv, l.root_box, l.heavy);
         GVC_LAUNCH_CHECK("root_box");
-        // 4b. heavy groups, walked first
-        GVC_TRY(fill_u32(l.heavy, 0u, 1, st));
+        // 4b. heavy groups, walked first (the kernel also clears the walk's queue
+        // words)
         if (NG > 0) {
             group_cost_kernel<<<blocks_for((size_t)NG, 4), 256, 0, st>>>(pts, NG, bv, l.root_box, l.ext_sum, l.heavy,
-                                                                        l.group_flag);
+                                                                        l.group_flag, l.queues);
             GVC_LAUNCH_CHECK("group_cost");
         }
     }
     // 5. walk
     if ((stages & GVC_STAGE_WALK) && NG > 0) {
         const int T = prm->block_threads;
-        GVC_TRY(fill_u32(l.queues + kQFwd, 0u, kQueues * kLine, st));
-        GVC_TRY(fill_u32(l.queues + kQLog, 0u, kQWords - kQLog, st));
+        if (!(stages & GVC_STAGE_BUILD)) {   // else group_cost_kernel cleared them
+            GVC_TRY(fill_u32(l.queues + kQFwd, 0u, kQueues * kLine, st));
+            GVC_TRY(fill_u32(l.queues + kQLog, 0u, kQWords - kQLog, st));
+        }
         const int log_on = (prm->flags & GVC_FLAG_CANDIDATE_LOG) ? 1 : 0;
         const int rp = run_points_of(prm);
         const int lp = lane_policy_of(prm);

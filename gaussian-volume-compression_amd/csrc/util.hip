@@ -69,15 +69,17 @@ __global__ __launch_bounds__(kScanThreads) void scan_tile_kernel(const uint32_t*
     }
 }
 
-__global__ __launch_bounds__(kScanThreads) void scan_add_kernel(uint32_t* out, size_t n, const uint32_t* block_offsets)
+// Adds each tile's offset; with total_out, block 0 also copies the scanned total
+// (block_offsets[gridDim.x]) there.
+__global__ __launch_bounds__(kScanThreads) void scan_add_kernel(uint32_t* out, size_t n, const uint32_t* block_offsets,
+                                                                uint32_t* total_out)
 {
     const size_t base = (size_t)blockIdx.x * kScanTile;
     const uint32_t add = block_offsets[blockIdx.x];
     for (int k = threadIdx.x; k < kScanTile; k += kScanThreads)
         if (base + k < n) out[base + k] += add;
+    if (total_out && blockIdx.x == 0 && threadIdx.x == 0) *total_out = block_offsets[gridDim.x];
 }
-
-__global__ void copy_one_kernel(uint32_t* dst, const uint32_t* src) { *dst = *src; }
 
 __device__ __forceinline__ unsigned ord_key(float f)
 {
@@ -170,12 +172,8 @@ int exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, bool with_to
     scan_tile_kernel<<<dim3((unsigned)nb), kScanThreads, 0, st>>>(in, out, n, sums, nullptr);
     GVC_LAUNCH_CHECK("scan_tile");
     GVC_TRY(exclusive_scan_u32(sums, sums, nb, true, scratch + nb + 1, st));
-    scan_add_kernel<<<dim3((unsigned)nb), kScanThreads, 0, st>>>(out, n, sums);
+    scan_add_kernel<<<dim3((unsigned)nb), kScanThreads, 0, st>>>(out, n, sums, with_total ? out + n : nullptr);
     GVC_LAUNCH_CHECK("scan_add");
-    if (with_total) {
-        copy_one_kernel<<<1, 1, 0, st>>>(out + n, sums + nb);
-        GVC_LAUNCH_CHECK("scan_total");
-    }
     return GVC_OK;
 }
 
