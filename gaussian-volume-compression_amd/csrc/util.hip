@@ -81,6 +81,34 @@ __global__ __launch_bounds__(kScanThreads) void scan_add_kernel(uint32_t* out, s
     if (total_out && blockIdx.x == 0 && threadIdx.x == 0) *total_out = block_offsets[gridDim.x];
 }
 
+// Second step of a two-level scan when the tile sums are few (at most
+// kPrefixSums): each block sums the tile sums before its own tile (integer sums:
+// the same in any order), adds that to its tile, and the last block also writes
+// the total.  Replaces the scan of the tile sums and its launch.
+constexpr size_t kPrefixSums = 4096;
+
+__global__ __launch_bounds__(kScanThreads) void scan_add_prefix_kernel(uint32_t* out, size_t n,
+                                                                       const uint32_t* block_sums,
+                                                                       uint32_t* total_out)
+{
+    __shared__ uint32_t red[kScanThreads / 64];
+    const uint32_t b = blockIdx.x;
+    uint32_t part = 0;
+    for (uint32_t k = threadIdx.x; k < b; k += kScanThreads) part += block_sums[k];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) add += red[w];
+    const size_t base = (size_t)b * kScanTile;
+    if (add)
+        for (int k = threadIdx.x; k < kScanTile; k += kScanThreads)
+            if (base + k < n) out[base + k] += add;
+    if (total_out && b == gridDim.x - 1 && threadIdx.x == 0) *total_out = add + block_sums[b];
+}
+
 __device__ __forceinline__ unsigned ord_key(float f)
 {
     unsigned u = __float_as_uint(f);
@@ -171,6 +199,11 @@ int exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, bool with_to
     uint32_t* sums = scratch;  // nb + 1 entries
     scan_tile_kernel<<<dim3((unsigned)nb), kScanThreads, 0, st>>>(in, out, n, sums, nullptr);
     GVC_LAUNCH_CHECK("scan_tile");
+    if (nb <= kPrefixSums) {
+        scan_add_prefix_kernel<<<dim3((unsigned)nb), kScanThreads, 0, st>>>(out, n, sums, with_total ? out + n : nullptr);
+        GVC_LAUNCH_CHECK("scan_add_prefix");
+        return GVC_OK;
+    }
     GVC_TRY(exclusive_scan_u32(sums, sums, nb, true, scratch + nb + 1, st));
     scan_add_kernel<<<dim3((unsigned)nb), kScanThreads, 0, st>>>(out, n, sums, with_total ? out + n : nullptr);
     GVC_LAUNCH_CHECK("scan_add");
