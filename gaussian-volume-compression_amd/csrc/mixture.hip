@@ -1131,8 +1131,9 @@ __device__ __forceinline__ bool lane_walk(const GRecord* __restrict__ rec, const
 // compensation.
 // ---------------------------------------------------------------------------
 // Candidate log: the forward appends every chunk it evaluates to a slot of 832
-// bytes {next slot, n, pad | Gaussian index per lane | point-major box-hit mask
-// per point lane}; a group's slots form a linked list headed by group_head[g].
+// bytes {next slot, n, pad | Gaussian index per lane | candidate-major box-hit
+// mask per candidate lane (the points inside its box: the backward's layout)};
+// a group's slots form a linked list headed by group_head[g].
 // Waves reserve slots 8 at a time from one counter.  The backward replays the
 // log instead of walking the tree and box-testing again.
 constexpr uint32_t kSlotBytes = 832;
@@ -1373,7 +1374,7 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
                 if (slot < log_cap) {
                     unsigned char* sb = log + (size_t)slot * kSlotBytes;
                     reinterpret_cast<int*>(sb + 64)[lane] = gi;
-                    reinterpret_cast<uint2*>(sb + 320)[lane] = make_uint2((uint32_t)m, (uint32_t)(m >> 32));
+                    reinterpret_cast<uint2*>(sb + 320)[lane] = make_uint2((uint32_t)cm, (uint32_t)(cm >> 32));
                     if (lane == 0) {
                         reinterpret_cast<uint32_t*>(sb)[0] = kLogEnd;
                         reinterpret_cast<uint32_t*>(sb)[1] = (uint32_t)n;
@@ -1499,8 +1500,8 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
 // Debug counts (gvc_forward_pair_counts): accepted pairs per sample and per
 // Gaussian, the reference's count_intersections (CR/forward.cu:180, :257)
 // printed by its debug mode (CR/rasterizer_impl.cu:175-216).  One wave per group
-// replays the group's candidate log -- lane p reads its point's row of box hits
-// and evaluates the predicate for each hit candidate -- or, for a lane-walked
+// replays the group's candidate log -- a bit transpose gives lane p its point's
+// row of box hits, and it evaluates the predicate for each hit candidate -- or, for a lane-walked
 // group, walks the tree per lane again.  Integer atomics only (the totals do not
 // depend on their order).  Not on the hot path: the walks count nothing per
 // Gaussian.
@@ -1552,7 +1553,8 @@ __global__ __launch_bounds__(256) void pair_count_kernel(const GRecord* __restri
                 cand[wid][lane][2] = q[2];
             }
             const uint2 row = reinterpret_cast<const uint2*>(sb + 320)[lane];
-            uint64_t m = ((uint64_t)row.y << 32) | row.x;
+            // the log keeps candidate-major rows; lane p: the candidates holding point p
+            uint64_t m = wave_transpose64(((uint64_t)row.y << 32) | row.x);
             lds_fence();
             while (m != 0ull) {
                 const int j = __builtin_ctzll(m);
@@ -1854,7 +1856,8 @@ __global__ __launch_bounds__(1024) void backward_kernel(
                 s.cand[lane][0] = q[0];
                 s.cand[lane][1] = q[1];
                 s.cand[lane][2] = q[2];
-                const uint64_t tm = wave_transpose64(((uint64_t)mr.y << 32) | mr.x);
+                // the logged row of lane j: the points inside candidate j's box
+                const uint64_t tm = ((uint64_t)mr.y << 32) | mr.x;
                 lds_fence();
                 process(n, gi, active ? tm : 0ull);
                 if (nxt == kLogEnd) break;
