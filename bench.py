@@ -109,6 +109,11 @@ def parse():
     ap.add_argument("--no-batch-autotune", action="store_true",
                     help="mesh: run the 32-volume batch leg at the default launch parameters instead of the "
                          "autotuner's choice (leaf size x points per walk x block size, swept on the batch)")
+    ap.add_argument("--no-side-configs", action="store_true",
+                    help="mesh: skip the structured-grid (200k Gaussians, 2^20 points) and precision (4M "
+                         "Gaussians, 2^23 points, FP64 path) legs the default line reports beside the headline")
+    ap.add_argument("--no-study", action="store_true",
+                    help="precision: skip the five-mode precision study (the step and its error check still run)")
     ap.add_argument("--batch32-steps", type=int, default=10,
                     help="mesh: timed fused steps of the 32-volume batch leg (the plain sequence takes 2)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -185,7 +190,7 @@ def main():
                                   fp64_scale_ratio=args.fp64_ratio if args.config == "precision" else 0.0)
         mx.set_default_params(params)
 
-    study = precision_study(args, W, dev) if args.config == "precision" else None
+    study = precision_study(args, W, dev) if args.config == "precision" and not args.no_study else None
 
     tune = None
     if args.autotune:
@@ -414,7 +419,38 @@ def main():
                               plain_ms_per_step=b["plain_sequence"]["ms_per_step"],
                               plain_pairs_per_step=b["plain_sequence"]["pairs_per_step"], steps=b["steps"],
                               plain_steps=b["plain_sequence"]["steps"])
+    if args.config == "mesh" and not args.no_side_configs:
+        # the structured-grid fit and the precision workload, each in a child
+        # process of its own, reported beside the headline
+        out["grid"] = side_config(["--config", "grid", "--steps", "20", "--warmup", "5", "--no-cpu-baseline"])
+        out["precision"] = side_config(["--config", "precision", "--graph", "--steps", "10", "--warmup", "3",
+                                        "--no-cpu-baseline", "--no-study"])
     print(json.dumps(out))
+
+
+def side_config(extra):
+    """Runs this script on another workload in a child process (its progress
+    lines go to this process's stderr) and returns the main figures of its line,
+    or the error it ended with."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__)] + extra
+    log("side leg: " + " ".join(extra))
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, text=True, timeout=600)
+    except subprocess.TimeoutExpired:
+        return {"error": "timed out", "args": extra}
+    lines = [l for l in r.stdout.strip().splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"exit status {r.returncode}", "args": extra}
+    d = json.loads(lines[-1])
+    keep = ("ms_per_step", "value", "unit", "pairs_per_step", "records_per_step", "stage_ms", "roofline",
+            "rel_err_vs_fp64")
+    res = {k: d[k] for k in keep if k in d}
+    res["workload"] = d["config"]["workload"]
+    res["graph"] = d["config"].get("graph")
+    res["fp64_cond"] = d["config"].get("fp64_cond")
+    res["args"] = " ".join(extra)
+    return res
 
 
 def walk_roofline(stage_avg, pairs_per_step, counters_apply):
