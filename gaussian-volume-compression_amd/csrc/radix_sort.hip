@@ -117,11 +117,16 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* __restric
     for (int i = t; i < kWaves * kMaxBins; i += kThreads) (&h[0][0])[i] = 0u;
     __syncthreads();
     const size_t base = sp.begin + (size_t)w * kWaveItems + lane;
-#pragma unroll 4
+    // all of a thread's keys in flight at once, then the counts
+    K kv[kItems];
+#pragma unroll
     for (int r = 0; r < kItems; ++r) {
         const size_t i = base + (size_t)r * 64;
-        if (i < sp.end) atomicAdd(&h[w][(uint32_t)(keys[i] >> shift) & mask], 1u);
+        kv[r] = i < sp.end ? keys[i] : K(0);
     }
+#pragma unroll
+    for (int r = 0; r < kItems; ++r)
+        if (base + (size_t)r * 64 < sp.end) atomicAdd(&h[w][(uint32_t)(kv[r] >> shift) & mask], 1u);
     __syncthreads();
     for (uint32_t d = t; d <= mask; d += kThreads) {
         uint32_t s = 0;
