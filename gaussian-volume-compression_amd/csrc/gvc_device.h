@@ -305,17 +305,17 @@ __device__ __forceinline__ bool conic_factor_ff(const float sc[3], const float q
 // CR/forward.cu:10-114 with the canonical operation order.  conic = packed
 // inverse of Sigma + eps I (xx xy xz yy yz zz); box = lo.xyz hi.xyz.
 // fp64_ratio > 0: Gaussians whose scale ratio exceeds it (cond(Sigma) above its
-// square) take the FP64 covariance inverse; with fac_hi/fac_lo given, such a
-// Gaussian also gets the Cholesky factor of that FP64 conic and returns true
-// (the precise path: its pairs are evaluated from the factor, precise_power).
-// factor_ratio > 0 (with fac_hi/fac_lo): a Gaussian whose scale ratio exceeds it
+// square) take the FP64 covariance inverse; with want_factor, such a Gaussian
+// also gets the Cholesky factor of that FP64 conic in fac_hi/fac_lo and returns
+// true (the precise path: its pairs are evaluated from the factor, precise_power).
+// factor_ratio > 0 (with want_factor): a Gaussian whose scale ratio exceeds it
 // but not fp64_ratio keeps its fp32 conic and gets the factor in float-float
 // arithmetic (conic_factor_ff), and returns true when that succeeds: the precise
 // path without FP64.
 __device__ __forceinline__ bool preprocess_one(const float mu[3], const float sc[3], const float q[4],
                                                float w, float k, float fp64_ratio, float conic[6],
-                                               float box[6], float* fac_hi = nullptr,
-                                               float* fac_lo = nullptr, float factor_ratio = 0.f)
+                                               float box[6], float fac_hi[6], float fac_lo[6],
+                                               float factor_ratio, bool want_factor)
 {
     float R[3][3];
     rotation(q[0], q[1], q[2], q[3], R);
@@ -323,7 +323,7 @@ __device__ __forceinline__ bool preprocess_one(const float mu[3], const float sc
     bool ill = fp64_ratio > 0.f && ill_conditioned(sc[0], sc[1], sc[2], k, fp64_ratio);
     bool precise = false;
     if (!ill) {
-        if (fac_hi && factor_ratio > 0.f && ill_conditioned(sc[0], sc[1], sc[2], k, factor_ratio))
+        if (want_factor && factor_ratio > 0.f && ill_conditioned(sc[0], sc[1], sc[2], k, factor_ratio))
             precise = conic_factor_ff(sc, q, k, fac_hi, fac_lo);
         float M[3][3];
 #pragma unroll
@@ -378,7 +378,7 @@ __device__ __forceinline__ bool preprocess_one(const float mu[3], const float sc
         const double cd[6] = { A00 / det, A01 / det, A02 / det, A11 / det, A12 / det, A22 / det };
 #pragma unroll
         for (int t = 0; t < 6; ++t) conic[t] = (float)cd[t];
-        if (fac_hi) precise = conic_factor(cd, fac_hi, fac_lo);
+        if (want_factor) precise = conic_factor(cd, fac_hi, fac_lo);
     }
     float m = box_radius(w);
     float sS[3] = { s[0] * m, s[1] * m, s[2] * m };

@@ -154,8 +154,8 @@ __global__ void preprocess_kernel(int P, const float* __restrict__ means, const 
     float sc[3] = { scales[3 * i], scales[3 * i + 1], scales[3 * i + 2] };
     float q[4] = { rots[4 * i], rots[4 * i + 1], rots[4 * i + 2], rots[4 * i + 3] };
     float cn[6], bx[6], fh[6], fl[6];
-    const bool precise = preprocess_one(mu, sc, q, weights[i], k, fp64_ratio, cn, bx, factors ? fh : nullptr, fl,
-                                        factor_ratio);
+    const bool precise = preprocess_one(mu, sc, q, weights[i], k, fp64_ratio, cn, bx, fh, fl, factor_ratio,
+                                        factors != nullptr);
     if (conics)
         for (int c = 0; c < 6; ++c) conics[6 * i + c] = cn[c];
     if (boxes)
@@ -308,12 +308,13 @@ __global__ __launch_bounds__(256) void record_kernel(int P, const uint32_t* __re
     float q[4] = { rots[4 * j], rots[4 * j + 1], rots[4 * j + 2], rots[4 * j + 3] };
     const float w = weights[j];
     float cn[6], bx[6], fh[6], fl[6];
-    const bool precise = preprocess_one(mu, sc, q, w, k, fp64_ratio, cn, bx, fac_lo ? fh : nullptr, fl,
-                                        factor_ratio);
-    const float* cr = precise ? fh : cn;
+    const bool precise = preprocess_one(mu, sc, q, w, k, fp64_ratio, cn, bx, fh, fl, factor_ratio,
+                                        fac_lo != nullptr);
+    // element-wise selects (a pointer to either array would put both in scratch)
     GRecord r;
     r.mx = mu[0]; r.my = mu[1]; r.mz = mu[2]; r.w = w;
-    r.c0 = cr[0]; r.c1 = cr[1]; r.c2 = cr[2]; r.c3 = cr[3]; r.c4 = cr[4]; r.c5 = cr[5];
+    r.c0 = precise ? fh[0] : cn[0]; r.c1 = precise ? fh[1] : cn[1]; r.c2 = precise ? fh[2] : cn[2];
+    r.c3 = precise ? fh[3] : cn[3]; r.c4 = precise ? fh[4] : cn[4]; r.c5 = precise ? fh[5] : cn[5];
     r.v = values[j];
     r.id = (int)((uint32_t)j | (precise ? kPreciseBit : 0u));
     r.lox = bx[0]; r.loy = bx[1]; r.loz = bx[2];
