@@ -2046,7 +2046,7 @@ __global__ __launch_bounds__(256) void reduce_long_kernel(const uint2* __restric
 // fp64_ratio > 0 (FP64 mode): Gaussians on the precise path have their conic
 // gradient summed in the factor frame (covariance_chain repeats the test).
 __global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __restrict__ inv_perm,
-                                                    const GRecord* __restrict__ rec,
+                                                    const GRecord* __restrict__ rec, int precise_mode,
                                                     const double* __restrict__ sums, const float* __restrict__ scales,
                                                     const float* __restrict__ rots, float kmod,
                                                     float* __restrict__ dmeans, float* __restrict__ dscales,
@@ -2072,8 +2072,9 @@ __global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __res
     double ds[3] = { 0.0, 0.0, 0.0 }, dq[4] = { 0.0, 0.0, 0.0, 0.0 };
     const bool any = m[5] != 0.0 || m[6] != 0.0 || m[7] != 0.0 || m[8] != 0.0 || m[9] != 0.0 || m[10] != 0.0;
     if (any) {
-        // the precise path's conic gradient is in the factor frame (record bit set by the preprocess)
-        const bool factor_frame = ((uint32_t)rec[ri].id & kPreciseBit) != 0u;
+        // the precise path's conic gradient is in the factor frame (record bit set
+        // by the preprocess, in FP64 mode only: the record is not read otherwise)
+        const bool factor_frame = precise_mode && ((uint32_t)rec[ri].id & kPreciseBit) != 0u;
         covariance_chain(m + 5, scales + 3 * id, rots + 4 * id, kmod, ds, dq, factor_frame);
     }
     for (int c = 0; c < 3; ++c) dscales[3 * id + c] = (float)ds[c];
@@ -2652,8 +2653,8 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
     GVC_LAUNCH_CHECK("reduce_sums");
     reduce_long_kernel<<<1024, 256, 0, st>>>(bl.seg, svals, bl.parts, bl.long_list, bl.sums);
     GVC_LAUNCH_CHECK("reduce_long");
-    chain_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, l.rec, bl.sums, scales, rots, k, dmeans,
-                                                     dscales, drots, dvalues, dweights);
+    chain_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, l.rec, l.fac_lo != nullptr ? 1 : 0, bl.sums,
+                                                     scales, rots, k, dmeans, dscales, drots, dvalues, dweights);
     GVC_LAUNCH_CHECK("chain");
     return GVC_OK;
 }
