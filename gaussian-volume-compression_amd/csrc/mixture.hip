@@ -1686,11 +1686,12 @@ __global__ __launch_bounds__(1024) void backward_kernel(
                    : lane_mode_group(tv, pme.x, pme.y, pme.z, nv, starts, reach, lane_policy);
         float V = 0.f, gVi = 0.f, gWe = 0.f;
         if (orig >= 0) {
-            const float Wp = out_weights[orig];
+            // the four gathers in flight together (one round trip), then the test
+            const float Wp = out_weights[orig], Vp = out_cells[orig], gV = gcells[orig], gW = gweights[orig];
             if ((double)Wp > kWeightCutoff) {
-                V = out_cells[orig];
-                gVi = gcells[orig] / Wp;
-                gWe = gweights[orig];
+                V = Vp;
+                gVi = gV / Wp;
+                gWe = gW;
             }
         }
         lds_fence();
