@@ -1374,8 +1374,10 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
                 const uint32_t slot = log_take(cur, log_counter);
                 if (slot < log_cap) {
                     unsigned char* sb = log + (size_t)slot * kSlotBytes;
-                    reinterpret_cast<int*>(sb + 64)[lane] = gi;
-                    reinterpret_cast<uint2*>(sb + 320)[lane] = make_uint2((uint32_t)cm, (uint32_t)(cm >> 32));
+                    // streaming stores: the backward reads the log back long after
+                    __builtin_nontemporal_store(gi, reinterpret_cast<int*>(sb + 64) + lane);
+                    __builtin_nontemporal_store((unsigned long long)cm,
+                                                reinterpret_cast<unsigned long long*>(sb + 320) + lane);
                     if (lane == 0) {
                         reinterpret_cast<uint32_t*>(sb)[0] = kLogEnd;
                         reinterpret_cast<uint32_t*>(sb)[1] = (uint32_t)n;
@@ -1589,6 +1591,20 @@ struct alignas(64) PartialRecord {
 };
 static_assert(sizeof(PartialRecord) == 64, "PartialRecord layout");
 
+// A record and its sort key written with streaming stores: they are read back
+// only after the walk (the sort and the sums), from HBM, and would otherwise
+// take L2 lines from the records and nodes the walk reads again.
+__device__ __forceinline__ void put_record(PartialRecord* __restrict__ out, uint32_t* __restrict__ keys, uint64_t pos,
+                                           const PartialRecord& o)
+{
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f* src = reinterpret_cast<const v4f*>(&o);
+    v4f* dst = reinterpret_cast<v4f*>(out + pos);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) __builtin_nontemporal_store(src[k], dst + k);
+    __builtin_nontemporal_store((uint32_t)o.gid, keys + pos);
+}
+
 struct BwdLdsBase {
     float4 ptsA[64];      // x, y, z, V
     float2 ptsB[64];      // gV / W, gW   (zero where W <= cutoff)
@@ -1793,8 +1809,7 @@ __global__ __launch_bounds__(1024) void backward_kernel(
                     o.dmx = dmx; o.dmy = dmy; o.dmz = dmz;
                     o.dc[0] = dc0; o.dc[1] = dc1; o.dc[2] = dc2; o.dc[3] = dc3; o.dc[4] = dc4; o.dc[5] = dc5;
                     o.pad[0] = o.pad[1] = o.pad[2] = o.pad[3] = 0.f;
-                    out_rec[pos] = o;
-                    keys[pos] = (uint32_t)gi;
+                    put_record(out_rec, keys, pos, o);
                 } else {
                     atomicOr(err, (uint32_t)GVC_ERRFLAG_RECORD_OVERFLOW);
                 }
@@ -1826,8 +1841,7 @@ __global__ __launch_bounds__(1024) void backward_kernel(
                                        o.dc[0] = h0 * dx; o.dc[1] = h0x2 * dy; o.dc[2] = h0x2 * dz;
                                        o.dc[3] = h1 * dy; o.dc[4] = h1x2 * dz; o.dc[5] = h2 * dz;
                                        o.pad[0] = o.pad[1] = o.pad[2] = o.pad[3] = 0.f;
-                                       out_rec[pos] = o;
-                                       keys[pos] = (uint32_t)gi;
+                                       put_record(out_rec, keys, pos, o);
                                    } else {
                                        atomicOr(err, (uint32_t)GVC_ERRFLAG_RECORD_OVERFLOW);
                                    }
