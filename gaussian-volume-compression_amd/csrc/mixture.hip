@@ -1483,8 +1483,9 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
                 V = U / W;
                 Wo = W;
             }
-            out_cells[orig] = V;
-            out_weights[orig] = Wo;
+            // streaming stores: the loss reads these after the walk
+            __builtin_nontemporal_store(V, out_cells + orig);
+            __builtin_nontemporal_store(Wo, out_weights + orig);
         }
         if (!lane_mode) tp = wave_sumu(npairs);
         if (lane == 0) {
