@@ -314,8 +314,8 @@ __device__ __forceinline__ bool conic_factor_ff(const float sc[3], const float q
 // path without FP64.
 __device__ __forceinline__ bool preprocess_one(const float mu[3], const float sc[3], const float q[4],
                                                float w, float k, float fp64_ratio, float conic[6],
-                                               float box[6], float fac_hi[6], float fac_lo[6],
-                                               float factor_ratio, bool want_factor)
+                                               float box[6], float* fac_hi = nullptr, float* fac_lo = nullptr,
+                                               float factor_ratio = 0.f, bool want_factor = false)
 {
     float R[3][3];
     rotation(q[0], q[1], q[2], q[3], R);
