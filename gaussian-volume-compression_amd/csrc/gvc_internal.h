@@ -68,12 +68,14 @@ const float4* sample_set_points(const void* sample_set, int S);
 // (optional, device memory): sort only the first min(n, *n_dev) items.
 // vals_identity: vals is not read; item i's value is i.  seg_off (optional,
 // device memory, segments + 1 item offsets): the items form contiguous segments,
-// each sorted on its own (the order of the segments is kept).
+// each sorted on its own (the order of the segments is kept).  inv_out
+// (optional, n entries; values a permutation of 0..n-1): inv_out[value] = its
+// sorted position, written by the last pass.
 size_t radix_scratch_elems(size_t n, int segments = 1);
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, size_t n,
                    int key_bits, uint32_t* scratch, bool* result_in_alt, hipStream_t st,
                    const uint32_t* n_dev = nullptr, bool vals_identity = false, const int* seg_off = nullptr,
-                   int segments = 1);
+                   int segments = 1, uint32_t* inv_out = nullptr);
 int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, size_t n,
                    int key_bits, uint32_t* scratch, bool* result_in_alt, hipStream_t st);
 

@@ -275,14 +275,6 @@ __global__ void gaussian_codes_kernel(const float* __restrict__ means, int P, ui
                           (63 - kGaussianCodeBits));
 }
 
-// inv_perm[perm[i]] = i: the record (BVH) index of each caller's Gaussian.
-__global__ __launch_bounds__(256) void inv_perm_kernel(int P, const uint32_t* __restrict__ perm,
-                                                       uint32_t* __restrict__ inv_perm)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < P) inv_perm[perm[i]] = (uint32_t)i;
-}
-
 // Canonical preprocess, in the caller's order (coalesced parameter reads and
 // conic writes), written as the record of BVH index inv_perm[j].  With fac_lo
 // (FP64 mode), a Gaussian on the precise path (scale ratio above factor_ratio:
@@ -2100,6 +2092,15 @@ __global__ __launch_bounds__(256) void chain_kernel(int P, const uint32_t* __res
 // Team size of the gradient reduction (lanes per Gaussian).
 constexpr int kReduceTeam = 4;
 
+// The reduction's segment bounds (2 words per Gaussian, zero: no records) and
+// long-segment count, cleared in one launch.
+__global__ void clear_reduce_kernel(uint32_t* __restrict__ seg, size_t n, uint32_t* __restrict__ long_list)
+{
+    const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i0 == 0) long_list[0] = 0u;
+    for (size_t i = i0; i < n; i += (size_t)gridDim.x * blockDim.x) seg[i] = 0u;
+}
+
 __global__ void zero_floats_kernel(float* p, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2454,12 +2455,11 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
         // (a batch's Gaussians are volume-major: each volume's range is sorted on
         // its own)
         bool alt = false;
+        // (the last pass also writes inv_perm: each caller index's BVH position)
         GVC_TRY(radix_sort_u32(l.codes, l.idx, l.codes_alt, l.idx_alt, (size_t)P, kGaussianCodeBits, l.sort_scratch,
-                               &alt, st, nullptr, true, bt.V > 1 ? bv.P_off : nullptr, bt.V));
+                               &alt, st, nullptr, true, bt.V > 1 ? bv.P_off : nullptr, bt.V, l.inv_perm));
         const uint32_t* scodes = alt ? l.codes_alt : l.codes;
-        const uint32_t* perm = alt ? l.idx_alt : l.idx;
         // 2. records
-        inv_perm_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, perm, l.inv_perm);
         record_kernel<<<blocks_for(P, 256), 256, 0, st>>>(P, l.inv_perm, means, scales, rots, values, weights, k,
                                                           fp64_ratio, factor_ratio, l.rec, l.fac_lo, conics);
         GVC_LAUNCH_CHECK("record");
@@ -2648,7 +2648,12 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
         GVC_LAUNCH_CHECK("backward");
     }
     if (!(stages & GVC_STAGE_REDUCE)) return GVC_OK;
-    GVC_TRY(fill_u32(reinterpret_cast<uint32_t*>(bl.seg), 0u, 2 * (size_t)P, st));
+    {
+        const unsigned nb = blocks_for(2 * (size_t)P, 256);
+        clear_reduce_kernel<<<nb < 4096u ? (nb > 0u ? nb : 1u) : 4096u, 256, 0, st>>>(
+            reinterpret_cast<uint32_t*>(bl.seg), 2 * (size_t)P, bl.long_list);
+        GVC_LAUNCH_CHECK("clear_reduce");
+    }
     const uint32_t* skeys = bl.keys;
     const uint32_t* svals = bl.vals;
     if (NG > 0 && capacity > 0) {
@@ -2663,7 +2668,6 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
                                                                   reinterpret_cast<uint32_t*>(bl.seg));
         GVC_LAUNCH_CHECK("segment");
     }
-    GVC_TRY(fill_u32(bl.long_list, 0u, 1, st));
     reduce_sums_kernel<kReduceTeam><<<blocks_for(P, 256), 256, 0, st>>>(P, bl.seg, svals, bl.parts,
                                                                         bl.sums, bl.long_list);
     GVC_LAUNCH_CHECK("reduce_sums");

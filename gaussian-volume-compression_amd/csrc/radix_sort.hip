@@ -147,7 +147,8 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __rest
                                                                  uint32_t* __restrict__ vals_out, size_t n_max,
                                                                  const uint32_t* __restrict__ n_dev, int shift,
                                                                  uint32_t mask, const uint32_t* __restrict__ offsets,
-                                                                 uint32_t n_tiles, const uint4* __restrict__ tiles)
+                                                                 uint32_t n_tiles, const uint4* __restrict__ tiles,
+                                                                 uint32_t* __restrict__ inv_out)
 {
     const TileSpan sp = tile_span(tiles, count_of(n_max, n_dev), n_tiles, mask + 1u);
     const size_t base = sp.begin, n = sp.end;   // this tile: items [base, n)
@@ -244,6 +245,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __rest
         const size_t g = (size_t)(gofs[d] + k);
         keys_out[g] = key;
         vals_out[g] = svals[k];
+        if (inv_out) inv_out[svals[k]] = (uint32_t)g;   // last pass: the inverse permutation
     }
 }
 
@@ -268,14 +270,15 @@ inline size_t max_tiles(size_t n, int nseg)
 template <class K>
 int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, size_t n, const uint32_t* n_dev,
                     int key_bits, uint32_t* scratch, bool* result_in_alt, hipStream_t st, bool vals_identity,
-                    const int* seg_off = nullptr, int nseg = 1)
+                    const int* seg_off = nullptr, int nseg = 1, uint32_t* inv_out = nullptr)
 {
     *result_in_alt = false;
     if (n <= 1 || key_bits <= 0) {
         // nothing to reorder; identity values still have to be written
-        if (vals_identity && n > 1)
+        if ((vals_identity || inv_out) && n > 1)
             return set_error(GVC_ERR_INVALID_ARGUMENT, "radix sort: identity values need a key to sort on");
         if (vals_identity && n == 1) GVC_TRY(fill_u32(vals, 0u, 1, st));
+        if (inv_out && n == 1) GVC_TRY(fill_u32(inv_out, 0u, 1, st));
         return GVC_OK;
     }
     if (n > 0xffffffffull) return set_error(GVC_ERR_INVALID_ARGUMENT, "radix sort: more than 2^32 items");
@@ -309,8 +312,10 @@ int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, si
         radix_hist_kernel<K><<<n_tiles, kThreads, 0, st>>>(src_k, n, n_dev, shift, mask, hist, n_tiles, tiles);
         GVC_LAUNCH_CHECK("radix_hist");
         GVC_TRY(exclusive_scan_u32(hist, hist, (size_t)(mask + 1) * n_tiles, false, scan_scratch, st));
+        const bool last = shift + width >= key_bits;
         radix_scatter_kernel<K><<<n_tiles, kThreads, scatter_lds<K>(), st>>>(src_k, src_v, dst_k, dst_v, n, n_dev,
-                                                                            shift, mask, hist, n_tiles, tiles);
+                                                                            shift, mask, hist, n_tiles, tiles,
+                                                                            last ? inv_out : nullptr);
         GVC_LAUNCH_CHECK("radix_scatter");
         K* tk = src_k; src_k = dst_k; dst_k = tk;
         uint32_t* nv = other_v;
@@ -338,10 +343,10 @@ size_t radix_scratch_elems(size_t n, int segments)
 
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, size_t n, int key_bits,
                    uint32_t* scratch, bool* result_in_alt, hipStream_t st, const uint32_t* n_dev, bool vals_identity,
-                   const int* seg_off, int segments)
+                   const int* seg_off, int segments, uint32_t* inv_out)
 {
     return radix_sort_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, n_dev, key_bits, scratch, result_in_alt,
-                                     st, vals_identity, seg_off, segments);
+                                     st, vals_identity, seg_off, segments, inv_out);
 }
 
 int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, size_t n, int key_bits,
