@@ -1304,6 +1304,7 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
                                                       const float* __restrict__ root_box, BatchView bv, int P, int L,
                                                       const float4* __restrict__ pts, int NG,
                                                       float* __restrict__ out_cells, float* __restrict__ out_weights,
+                                                      float2* __restrict__ pvw,
                                                       uint32_t* __restrict__ group_records,
                                                       uint32_t* __restrict__ group_pairs, uint32_t* __restrict__ err,
                                                       uint32_t* __restrict__ queue,
@@ -1466,18 +1467,22 @@ __global__ __launch_bounds__(1024) void forward_kernel(const GRecord* __restrict
                 ok = enumerate_candidates(rec, nodes, tv, L, gb, s.walk, chunk) && ok;
             }
         }
-        if (orig >= 0) {
-            float V, Wo;
-            if ((double)W <= kWeightCutoff) {
-                V = -1.0f;
-                Wo = 0.0f;
-            } else {
-                V = U / W;
-                Wo = W;
+        {
+            float V = 0.0f, Wo = 0.0f;
+            if (orig >= 0) {
+                if ((double)W <= kWeightCutoff) {
+                    V = -1.0f;
+                } else {
+                    V = U / W;
+                    Wo = W;
+                }
+                // streaming stores: the loss reads these after the walk
+                __builtin_nontemporal_store(V, out_cells + orig);
+                __builtin_nontemporal_store(Wo, out_weights + orig);
             }
-            // streaming stores: the loss reads these after the walk
-            __builtin_nontemporal_store(V, out_cells + orig);
-            __builtin_nontemporal_store(Wo, out_weights + orig);
+            // and in the sample set's order for the backward (coalesced)
+            typedef float v2f __attribute__((ext_vector_type(2)));
+            __builtin_nontemporal_store(v2f{ V, Wo }, reinterpret_cast<v2f*>(pvw) + (size_t)g * 64 + lane);
         }
         if (!lane_mode) tp = wave_sumu(npairs);
         if (lane == 0) {
@@ -1651,8 +1656,8 @@ __device__ __forceinline__ uint64_t drop_low_bits(uint64_t m, uint32_t k)
 template <bool REPLAY, bool PRECISE>
 __global__ __launch_bounds__(1024) void backward_kernel(
     const GRecord* __restrict__ rec, const float4* __restrict__ fac_lo, const BNode* __restrict__ nodes, const float* __restrict__ root_box, BatchView bv,
-    int P, int L, const float4* __restrict__ pts, int NG, const float* __restrict__ out_cells,
-    const float* __restrict__ out_weights, const float* __restrict__ gcells, const float* __restrict__ gweights,
+    int P, int L, const float4* __restrict__ pts, int NG, const float2* __restrict__ pvw,
+    const float* __restrict__ gcells, const float* __restrict__ gweights,
     const uint32_t* __restrict__ group_offsets, uint64_t capacity, PartialRecord* __restrict__ out_rec,
     uint32_t* __restrict__ keys, uint32_t* __restrict__ err, uint32_t* __restrict__ queue,
     const unsigned long long* __restrict__ ext_sum, const unsigned char* __restrict__ log,
@@ -1696,7 +1701,10 @@ __global__ __launch_bounds__(1024) void backward_kernel(
         float V = 0.f, gVi = 0.f, gWe = 0.f;
         if (orig >= 0) {
             // the four gathers in flight together (one round trip), then the test
-            const float Wp = out_weights[orig], Vp = out_cells[orig], gV = gcells[orig], gW = gweights[orig];
+            // the forward's value and weight from the state (coalesced), the
+            // upstream gradients gathered, all in flight together
+            const float2 vw = pvw[(size_t)g * 64 + lane];
+            const float Wp = vw.y, Vp = vw.x, gV = gcells[orig], gW = gweights[orig];
             if ((double)Wp > kWeightCutoff) {
                 V = Vp;
                 gVi = gV / Wp;
@@ -2261,6 +2269,7 @@ struct ForwardLayout {
     uint32_t* group_head;
     uint32_t* heavy;        // [0] = count, [1..] = heavy groups (walked first)
     uint8_t* group_flag;    // 1 = heavy
+    float2* pvw;            // per sorted point (group g, lane j at 64 g + j): the forward's (value, weight)
     unsigned char* log;
     uint32_t log_cap;
     int NL, NG;
@@ -2293,6 +2302,7 @@ struct ForwardLayout {
         group_head = a.take<uint32_t>((size_t)NG + 1);
         heavy = a.take<uint32_t>((size_t)NG + 1);
         group_flag = reinterpret_cast<uint8_t*>(a.take<uint32_t>(((size_t)NG + 3) / 4 + 1));
+        pvw = a.take<float2>((size_t)NG * 64 + 1);
         const uint64_t cap = (flags & GVC_FLAG_CANDIDATE_LOG)
                                  ? (uint64_t)(chunk_slots ? chunk_slots : 16u) * (uint64_t)(NG > 0 ? NG : 1)
                                  : 1ull;
@@ -2521,7 +2531,8 @@ int mixture_forward_impl(const Batch& bt, const float* means, const float* scale
             GVC_TRY(allow_lds(kf, lds));
             const unsigned nb = walk_blocks(kf, T, lds, NG);
             forward_kernel<K, PR><<<nb, T, lds, st>>>(l.rec, l.fac_lo, l.nodes, l.root_box, bv, P, L, pts, NG,
-                                                      out_cells, out_weights, l.group_records, l.group_pairs, l.err,
+                                                      out_cells, out_weights, l.pvw, l.group_records, l.group_pairs,
+                                                      l.err,
                                                       l.queues + kQFwd, l.ext_sum, log_on, l.log, l.log_cap,
                                                       l.queues + kQLog, l.group_head, rp, lp, l.heavy,
                                                       l.queues + kQHeavyF, l.group_flag);
@@ -2634,7 +2645,7 @@ int mixture_backward_impl(const Batch& bt, const float* scales, const float* rot
             GVC_TRY(allow_lds(kf, lds));
             const unsigned nb = walk_blocks(kf, T, lds, NG);
             backward_kernel<RE, PR><<<nb, T, lds, st>>>(l.rec, l.fac_lo, l.nodes, l.root_box, bv, P, L, pts, NG,
-                                                        out_cells, out_weights, gcells, gweights, l.group_offsets,
+                                                        l.pvw, gcells, gweights, l.group_offsets,
                                                         capacity, bl.parts, bl.keys, l.err, l.queues + kQBwd,
                                                         l.ext_sum, l.log, l.group_head, rp, lp, l.heavy,
                                                         l.queues + kQHeavyB, l.group_flag);

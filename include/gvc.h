@@ -203,7 +203,10 @@ int gvc_forward_pair_counts(const void *state, int P, int S, const gvc_params *p
  * partials are sorted by Gaussian (stable) and each Gaussian's are summed in FP64
  * in an order fixed by its own sorted partials; no float atomics.
  * record_capacity >= n_records of the forward (gvc_forward_stats) or the call
- * flags GVC_ERRFLAG_RECORD_OVERFLOW and the gradients are incomplete. */
+ * flags GVC_ERRFLAG_RECORD_OVERFLOW and the gradients are incomplete.
+ * out_cells/out_weights are the outputs of the forward that filled `state`: the
+ * walk reads the same values from the copy that forward kept in the state, in
+ * the sample set's order. */
 size_t gvc_backward_bytes(int P, uint64_t record_capacity);
 int gvc_mixture_backward(int P, const float *means3D, const float *scales, const float *rotations,
                          const float *values, const float *weights, float scale_modifier,
