@@ -190,6 +190,15 @@ def test_empty_and_tiny():
     cr, br, _ = oracle.preprocess(m, s, q, w)
     Vr, Wr, _ = oracle.forward(m, v, w, cr, br, samples)
     assert field_err(cells.cpu().numpy(), cw.cpu().numpy(), Vr, Wr) <= 1e-5
+    # the adjoint of the one-Gaussian problem (a one-element Gaussian sort and
+    # its inverse permutation; 100 samples: a partly filled second group)
+    gV = rng.normal(size=samples.shape[0]).astype(f32)
+    gW = rng.normal(size=samples.shape[0]).astype(f32)
+    grads = mx.mixture_backward(st, T(m), T(s), T(q), T(v), T(w), 1.0, ss, cells, cw, T(gV), T(gW))
+    cd = oracle.conic_fp64(s, q)
+    Vd, Wd, _ = oracle.forward(m, v, w, cr, br, samples, cd)
+    ref = oracle.backward(m, s, q, v, w, cr, br, samples, Vd, Wd, gV.astype(np.float64), gW.astype(np.float64))
+    assert_grads(grads, ref, GRAD_BAR)
     ss0 = mx.SampleSet(T(np.zeros((0, 3), f32)))
     c0, w0, _, _ = mx.mixture_forward(T(m), T(s), T(q), T(v), T(w), 1.0, ss0)
     assert c0.numel() == 0
