@@ -129,8 +129,8 @@ def log(msg):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "bench.py measures one GPU; --gpus above 1 is not supported in this build "
-                                   "(the sharding design is described in DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "bench.py measures one GPU; --gpus above 1 is not supported: "
+                                   "the path runs on one GPU and is not sharded across several", "n_gpus": args.gpus}))
         sys.exit(2)
     import numpy as np
     import torch

@@ -6,7 +6,7 @@ fastest.  Points per walk is how a wavefront's 64 points (one per lane) share
 the tree walk: 1 = every lane walks for its own point (lane mode), k = runs of
 up to k consecutive points share one walk of their box (run_points).  The
 reference's launch shape has no counterpart (one warp per Gaussian over a
-sample BVH); BASELINE's axes are leaf size {2, 4, 8, 16}, points per wavefront
+sample BVH); the sweep's axes are leaf size {2, 4, 8, 16}, points per wavefront
 {1, 4, 16} and block size {256, 512, 1024}; the 64-point shared walk is added
 because it is the measured default.  Results
 are cached per (log2 P, log2 S) bucket in memory and, if ``GVC_TUNE_CACHE``

@@ -911,7 +911,7 @@ void orc_karras_build(int n, const uint64_t *keys, int32_t *left, int32_t *right
 
 /* ------------------------------------------------------------------------- */
 /* Plain fp32 CPU baseline: the reference formula on CPU, brute force with the  */
-/* box predicate, no compensation (BASELINE "reference formula on CPU").        */
+/* box predicate, no compensation: the reference formula run on the CPU.       */
 /* Evaluates points [0, S) of `samples` against all P Gaussians; forward then   */
 /* backward with upstream dL/dV = gV, dL/dW = gW.  Per-Gaussian gradients are   */
 /* accumulated per thread and summed (fp32 packed-conic gradient, no chain).    */
