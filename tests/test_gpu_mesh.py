@@ -180,6 +180,19 @@ def test_distcuda2_bit_exact():
         assert np.array_equal(got, ref)
 
 
+
+def test_distcuda2_few_and_coincident_points():
+    """Fewer than four points (a missing neighbour counts +inf, as an unfilled
+    3-best list) and coincident points (zero extent on some or every axis)."""
+    from simple_knn._C import distCUDA2
+    rng = np.random.default_rng(10)
+    cases = [rng.uniform(0, 1, (k, 3)) for k in (1, 2, 3, 4)]
+    cases += [np.full((5, 3), 0.25), np.concatenate([np.full((4, 3), 0.5), [[0.5, 0.5, 0.75]]])]
+    for P in cases:
+        P = np.asarray(P, f32)
+        got = distCUDA2(torch.tensor(P, device=DEV)).cpu().numpy()
+        assert np.array_equal(got, mesh_ref.knn3(P)), P.shape
+
 def test_mixed_cells_against_oracle():
     """Hexahedra, wedges, pyramids and tetrahedra (planar faces): the GPU picks the
     oracle's cell wherever the oracle's choice is clear of a face, interpolates
